@@ -1,0 +1,167 @@
+"""Benchmark: the fused compressed-chunk -> filter -> join-probe -> int64-sum pipeline on one GPU.
+
+Workload (one "step" = one query): 600,000,000 fact rows (date_key int32 over 2,556 days,
+discount uint8 U[0,10], quantity uint8 U[1,50], price_cents int32 U[90000,10500000]) stored as
+LZ4 pages in 262,144-row chunks, resident in HBM. Each step builds the date hash table
+(year == 1995) on the device and runs the fused kernel for
+discount in [4,6] and quantity >= 30, SUM(price*discount) as int64.
+
+Prints one JSON line (see README of the driver contract): rows/s over the whole job, the
+roofline of the fused kernel (algorithmic bytes / kernel time, against 8 TB/s), and the CPU
+path (oracle decode + plan on the host cores) timed on the same data.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "fact rows/s and % HBM roofline (fused pipeline); queries/s + p99 latency, 1 GPU"
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--rows", type=int, default=600_000_000)
+    ap.add_argument("--page-rows", type=int, default=4096)
+    ap.add_argument("--chunk-rows", type=int, default=262144)
+    ap.add_argument("--min-match", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=20240601)
+    ap.add_argument("--cpu-runs", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "q1_fused_600M_traffic.json"))
+    return ap.parse_args()
+
+
+def cpu_baseline(args, hosts, key, year, gpu_sum):
+    """The CPU path (CPU chunk decode + Q1 plan with thread-local partials) on the host cores."""
+    import ctypes as C
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    lib = oracle.load()
+    threads = len(os.sched_getaffinity(0))
+    refs = (oracle.ColRef * 4)()
+    for i, h in enumerate(hosts):
+        d = h.desc()
+        refs[i] = oracle.ColRef(d.comp, d.page_off, d.page_csize, d.page_codec)
+    d0 = hosts[0].desc()
+    preds, keep = oracle.make_preds([(year, 1995, 1995)])
+    c = oracle.Q1Consts(4, 6, 30, 50, -2**63, 2**63 - 1)
+    key = np.ascontiguousarray(key)
+    times, res = [], None
+    lib.oracle_q1_compressed(key.ctypes.data, key.size, C.cast(preds, C.c_void_p), 1, C.cast(refs, C.c_void_p),
+                             d0.n_rows, d0.page_rows, d0.chunk_rows, 0, d0.n_chunks, C.byref(c), threads,
+                             C.byref(C.c_uint64()))  # warm-up
+    for _ in range(args.cpu_runs):
+        m = C.c_uint64()
+        t0 = time.perf_counter()
+        s = lib.oracle_q1_compressed(key.ctypes.data, key.size, C.cast(preds, C.c_void_p), 1,
+                                     C.cast(refs, C.c_void_p), d0.n_rows, d0.page_rows, d0.chunk_rows, 0,
+                                     d0.n_chunks, C.byref(c), threads, C.byref(m))
+        times.append(time.perf_counter() - t0)
+        res = s
+    med = statistics.median(times)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next(l.split(":", 1)[1].strip() for l in f if l.startswith("model name"))
+    except Exception:
+        pass
+    log(f"cpu baseline: {threads} threads, median {med*1e3:.1f} ms, sum {res} (gpu {gpu_sum})")
+    return {"value": d0.n_rows / med, "unit": "rows/s", "cores": threads, "kind": "port",
+            "sample": f"full workload ({d0.n_rows} rows, same compressed pages), median of "
+                      f"{args.cpu_runs} runs, CPU: {model}",
+            "bit_exact_vs_gpu": bool(res == gpu_sum)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the sharded form is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import golap_public_amd as g
+
+    t_setup = time.perf_counter()
+    key, year, _ = g.gen_date_dim(2556)
+    log(f"generating {args.rows} rows")
+    cols = g.gen_q1_fact(args.seed, args.rows, key)
+    log("encoding")
+    hosts = [g.HostColumn.encode(c, codec=g.CODEC_LZ4, page_rows=args.page_rows, chunk_rows=args.chunk_rows,
+                                 lz4_min_match=args.min_match) for c in cols]
+    del cols
+    raw_bytes = args.rows * 10
+    comp_payload = sum(h.desc().payload_bytes for h in hosts)
+    log(f"compressed payload {comp_payload/1e9:.3f} GB for {raw_bytes/1e9:.3f} GB raw")
+    ctx = g.Context(0)
+    cus, name = ctx.info()
+    dcols = [ctx.upload(h) for h in hosts]
+    dk, dy = ctx.to_device(key), ctx.to_device(year)
+    m = ctx.hashmap(key.size, dk, g.I32)
+    log(f"setup {time.perf_counter()-t_setup:.1f} s on {name} ({cus} CUs)")
+
+    def step():
+        ctx.hashmap_build(m, key.size, [(dy, g.I16, 1995, 1995)])
+        return ctx.q1_run(*dcols, m, (4, 6), (30, 50))
+
+    for _ in range(args.warmup):
+        r = step()
+    ctx.sync()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = step()
+        kms.append(r.kernel_ms)
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = args.rows * args.steps / elapsed
+    kernel_ms = statistics.mean(kms)
+    algo_bytes = r.comp_bytes + key.size * (4 + 2) + 24
+    achieved = algo_bytes / (kernel_ms * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(args.traffic_file):
+        try:
+            with open(args.traffic_file) as f:
+                traffic = json.load(f).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+    log(f"sum {r.sum} matches {r.matches}; {ms_per_step:.3f} ms/step, kernel {kernel_ms:.3f} ms, "
+        f"{achieved:.0f} GB/s")
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "int64",
+        "data": "synthetic: SplitMix64-seeded columns with the workload's distributions, LZ4 pages",
+        "config": {"workload": "q1_fused_600M_lz4", "rows": args.rows, "chunk_rows": args.chunk_rows,
+                   "page_rows": args.page_rows, "codec": "lz4", "lz4_min_match": args.min_match,
+                   "comp_bytes": r.comp_bytes, "raw_bytes": raw_bytes,
+                   "plan": "year=1995, discount 4..6, quantity>=30, SUM(price*discount)",
+                   "result_sum": r.sum, "result_matches": r.matches, "kernel_ms": kernel_ms,
+                   "device": name},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(args, hosts, key, year, r.sum)
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

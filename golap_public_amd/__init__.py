@@ -1,0 +1,368 @@
+"""ctypes binding of the C ABI in include/golap_hip.h (libgolap_hip.so, built for gfx950).
+
+This module is plumbing for tests and the benchmark driver: every call goes straight to the
+native library, which runs the HIP kernels. There is no Python or CPU fallback for any compute
+entry point; if the library is missing, load() raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgolap_hip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "golap_hip.h")
+
+OK, EINVAL, ENOMEM, EHIP, ECORRUPT, ECAPACITY, EBUDGET, ENODEV = 0, -1, -2, -3, -4, -5, -6, -7
+I8, U8, I16, U16, I32, U32, I64 = 1, 2, 3, 4, 5, 6, 7
+CODEC_RAW, CODEC_LZ4, CODEC_DBP, CODEC_RLE, CODEC_AUTO = 0, 1, 2, 3, 255
+PAGE_LITERAL_ONLY = 0x80
+
+NP_TYPE = {I8: np.int8, U8: np.uint8, I16: np.int16, U16: np.uint16, I32: np.int32,
+           U32: np.uint32, I64: np.int64}
+TYPE_OF = {np.dtype(v): k for k, v in NP_TYPE.items()}
+
+vp = C.c_void_p
+u64 = C.c_uint64
+u32 = C.c_uint32
+i64 = C.c_int64
+
+
+class EncodeOpts(C.Structure):
+    _fields_ = [("codec", u32), ("page_rows", u32), ("chunk_rows", u32),
+                ("lz4_min_match", u32), ("threads", u32)]
+
+
+class ColumnDesc(C.Structure):
+    _fields_ = [("n_rows", u64), ("n_pages", u64), ("n_chunks", u64), ("comp_bytes", u64),
+                ("payload_bytes", u64), ("uncomp_bytes", u64), ("type", u32), ("elem", u32),
+                ("page_rows", u32), ("chunk_rows", u32), ("comp", vp), ("page_off", vp),
+                ("page_csize", vp), ("page_codec", vp), ("chunk_min", vp), ("chunk_max", vp)]
+
+
+class Pred(C.Structure):
+    _fields_ = [("dev_col", vp), ("type", u32), ("lo", i64), ("hi", i64)]
+
+
+class Q1Plan(C.Structure):
+    _fields_ = [("date", vp), ("disc", vp), ("qty", vp), ("price", vp), ("date_map", vp),
+                ("disc_lo", i64), ("disc_hi", i64), ("qty_lo", i64), ("qty_hi", i64),
+                ("price_lo", i64), ("price_hi", i64), ("chunk_begin", u64), ("chunk_end", u64),
+                ("dev_chunk_mask", vp)]
+
+
+class Q1Result(C.Structure):
+    _fields_ = [("sum", i64), ("matches", u64), ("chunks_run", u64), ("chunks_pruned", u64),
+                ("comp_bytes", u64), ("kernel_ms", C.c_double)]
+
+
+_PROTOS = {
+    "golap_api_version": (C.c_int, []),
+    "golap_status_string": (C.c_char_p, [C.c_int]),
+    "golap_ctx_create": (C.c_int, [C.c_int, C.POINTER(vp)]),
+    "golap_ctx_destroy": (C.c_int, [vp]),
+    "golap_last_error": (C.c_char_p, [vp]),
+    "golap_sync": (C.c_int, [vp]),
+    "golap_device_info": (C.c_int, [vp, C.POINTER(C.c_int), C.c_char_p, C.c_int]),
+    "golap_dev_alloc": (C.c_int, [vp, u64, C.POINTER(vp)]),
+    "golap_dev_free": (C.c_int, [vp, vp]),
+    "golap_h2d": (C.c_int, [vp, vp, vp, u64]),
+    "golap_d2h": (C.c_int, [vp, vp, vp, u64]),
+    "golap_memset": (C.c_int, [vp, vp, C.c_int, u64]),
+    "golap_device_allocated": (u64, [vp]),
+    "golap_encode_opts_default": (None, [C.POINTER(EncodeOpts)]),
+    "golap_encode": (C.c_int, [vp, u64, u32, C.POINTER(EncodeOpts), C.POINTER(vp)]),
+    "golap_host_column_assemble": (C.c_int, [u32, u64, u32, u32, C.POINTER(vp),
+                                             C.POINTER(u32), C.POINTER(C.c_uint8), C.POINTER(vp)]),
+    "golap_host_column_free": (C.c_int, [vp]),
+    "golap_host_column_info": (C.c_int, [vp, C.POINTER(ColumnDesc)]),
+    "golap_column_upload": (C.c_int, [vp, vp, C.POINTER(vp)]),
+    "golap_column_free": (C.c_int, [vp, vp]),
+    "golap_column_info": (C.c_int, [vp, C.POINTER(ColumnDesc)]),
+    "golap_prepare_compressed_device": (C.c_int, [vp, vp, u64, u32, C.POINTER(EncodeOpts),
+                                                  C.POINTER(vp), C.POINTER(C.c_double)]),
+    "golap_decompress_chunks": (C.c_int, [vp, vp, u64, u64, vp, C.POINTER(C.c_double)]),
+    "golap_hashmap_create": (C.c_int, [vp, u64, vp, u32, C.POINTER(vp)]),
+    "golap_hashmap_build": (C.c_int, [vp, vp, u64, C.POINTER(Pred), u32, vp]),
+    "golap_hashmap_probe": (C.c_int, [vp, vp, vp, u32, u64, vp]),
+    "golap_hash_join_count": (C.c_int, [vp, vp, vp, u32, u64, C.POINTER(u64)]),
+    "golap_hashmap_filled": (C.c_int, [vp, vp, C.POINTER(u64)]),
+    "golap_hashmap_free": (C.c_int, [vp, vp]),
+    "golap_minmax_check": (C.c_int, [vp, vp, i64, i64, vp]),
+    "golap_combine_and": (C.c_int, [vp, vp, vp, u64]),
+    "golap_q1_run": (C.c_int, [vp, C.POINTER(Q1Plan), C.POINTER(Q1Result)]),
+    "golap_gen_date_dim": (C.c_int, [u32, vp, vp, vp]),
+    "golap_gen_q1_fact": (C.c_int, [u64, u64, u64, vp, u32, vp, vp, vp, vp, u32]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libgolap_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with python -m golap_public_amd.build")
+        lib = C.CDLL(LIB_PATH)
+        for name, (res, args) in _PROTOS.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+class GolapError(RuntimeError):
+    def __init__(self, status, msg=""):
+        super().__init__(f"status {status} ({load().golap_status_string(status).decode()}) {msg}")
+        self.status = status
+
+
+def _ptr(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def encode_opts(codec=CODEC_LZ4, page_rows=4096, chunk_rows=262144, lz4_min_match=4, threads=0):
+    o = EncodeOpts()
+    load().golap_encode_opts_default(C.byref(o))
+    o.codec, o.page_rows, o.chunk_rows, o.lz4_min_match, o.threads = (
+        codec, page_rows, chunk_rows, lz4_min_match, threads)
+    return o
+
+
+class HostColumn:
+    """An encoded column in host memory (CompInfo + BlockInfo + MinMaxMeta)."""
+
+    def __init__(self, handle):
+        self.h = C.c_void_p(handle)
+
+    @classmethod
+    def encode(cls, values, **kw):
+        values = np.ascontiguousarray(values)
+        t = TYPE_OF[values.dtype]
+        out = vp()
+        st = load().golap_encode(_ptr(values), values.size, t, C.byref(encode_opts(**kw)), C.byref(out))
+        if st != OK:
+            raise GolapError(st, "encode")
+        return cls(out.value)
+
+    @classmethod
+    def assemble(cls, type_, n_rows, page_rows, chunk_rows, pages, codecs):
+        n = len(pages)
+        bufs = [np.frombuffer(p, dtype=np.uint8) if len(p) else np.zeros(1, np.uint8) for p in pages]
+        ptrs = (vp * max(n, 1))(*[b.ctypes.data for b in bufs])
+        sizes = (u32 * max(n, 1))(*[len(p) for p in pages])
+        cods = (C.c_uint8 * max(n, 1))(*codecs)
+        out = vp()
+        st = load().golap_host_column_assemble(type_, n_rows, page_rows, chunk_rows, ptrs, sizes, cods,
+                                               C.byref(out))
+        if st != OK:
+            raise GolapError(st, "assemble")
+        return cls(out.value)
+
+    def desc(self):
+        d = ColumnDesc()
+        st = load().golap_host_column_info(self.h, C.byref(d))
+        if st != OK:
+            raise GolapError(st)
+        return d
+
+    def arrays(self):
+        """Copies of the catalog and the column buffer as numpy arrays."""
+        d = self.desc()
+
+        def arr(ptr, n, dt):
+            if n == 0:
+                return np.zeros(0, dt)
+            return np.ctypeslib.as_array(C.cast(ptr, C.POINTER(np.ctypeslib.as_ctypes_type(dt))),
+                                         shape=(n,)).copy()
+        return dict(comp=arr(d.comp, d.comp_bytes, np.uint8),
+                    page_off=arr(d.page_off, d.n_pages, np.uint64),
+                    page_csize=arr(d.page_csize, d.n_pages, np.uint32),
+                    page_codec=arr(d.page_codec, d.n_pages, np.uint8),
+                    chunk_min=arr(d.chunk_min, d.n_chunks, np.int64),
+                    chunk_max=arr(d.chunk_max, d.n_chunks, np.int64),
+                    desc=d)
+
+    def free(self):
+        if self.h:
+            load().golap_host_column_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Context:
+    """One device; owns every device object created through it."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        h = vp()
+        st = self.lib.golap_ctx_create(device, C.byref(h))
+        if st != OK:
+            raise GolapError(st, "ctx_create")
+        self.h = h
+
+    def check(self, st, what=""):
+        if st != OK:
+            raise GolapError(st, f"{what}: {self.lib.golap_last_error(self.h).decode()}")
+
+    def close(self):
+        if self.h:
+            self.lib.golap_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def info(self):
+        cus = C.c_int()
+        name = C.create_string_buffer(128)
+        self.check(self.lib.golap_device_info(self.h, C.byref(cus), name, 128))
+        return cus.value, name.value.decode()
+
+    # memory
+    def alloc(self, nbytes):
+        p = vp()
+        self.check(self.lib.golap_dev_alloc(self.h, nbytes, C.byref(p)), "alloc")
+        return p.value
+
+    def free(self, p):
+        self.check(self.lib.golap_dev_free(self.h, p), "free")
+
+    def to_device(self, arr):
+        arr = np.ascontiguousarray(arr)
+        p = self.alloc(arr.nbytes)
+        self.check(self.lib.golap_h2d(self.h, p, _ptr(arr), arr.nbytes), "h2d")
+        return p
+
+    def to_host(self, p, n, dtype):
+        out = np.empty(n, dtype=dtype)
+        self.check(self.lib.golap_d2h(self.h, _ptr(out), p, out.nbytes), "d2h")
+        return out
+
+    def memset(self, p, value, nbytes):
+        self.check(self.lib.golap_memset(self.h, p, value, nbytes), "memset")
+
+    def sync(self):
+        self.check(self.lib.golap_sync(self.h), "sync")
+
+    # columns
+    def upload(self, hcol):
+        out = vp()
+        self.check(self.lib.golap_column_upload(self.h, hcol.h, C.byref(out)), "upload")
+        return out.value
+
+    def prepare_compressed_device(self, values, **kw):
+        values = np.ascontiguousarray(values)
+        out = vp()
+        ms = C.c_double()
+        self.check(self.lib.golap_prepare_compressed_device(
+            self.h, _ptr(values), values.size, TYPE_OF[values.dtype], C.byref(encode_opts(**kw)),
+            C.byref(out), C.byref(ms)), "prepare_compressed_device")
+        return out.value, ms.value
+
+    def column_desc(self, col):
+        d = ColumnDesc()
+        self.check(self.lib.golap_column_info(col, C.byref(d)))
+        return d
+
+    def free_column(self, col):
+        self.check(self.lib.golap_column_free(self.h, col), "column_free")
+
+    def decompress(self, col, chunk_begin=0, chunk_end=None):
+        """Decode chunks into a fresh device buffer and return it as a host numpy array."""
+        d = self.column_desc(col)
+        if chunk_end is None:
+            chunk_end = d.n_chunks
+        nbytes = (chunk_end - chunk_begin) * d.chunk_rows * d.elem
+        buf = self.alloc(max(nbytes, 16))
+        try:
+            ms = C.c_double()
+            self.check(self.lib.golap_decompress_chunks(self.h, col, chunk_begin, chunk_end, buf,
+                                                        C.byref(ms)), "decompress")
+            out = self.to_host(buf, nbytes, np.uint8)
+        finally:
+            self.free(buf)
+        return out, ms.value
+
+    # hash join
+    def hashmap(self, ht_size, dev_keys, key_type):
+        out = vp()
+        self.check(self.lib.golap_hashmap_create(self.h, ht_size, dev_keys, key_type, C.byref(out)),
+                   "hashmap_create")
+        return out.value
+
+    def hashmap_build(self, m, n_rows, preds=(), dev_payload=None):
+        arr = (Pred * max(len(preds), 1))(*[Pred(*p) for p in preds])
+        st = self.lib.golap_hashmap_build(self.h, m, n_rows, arr, len(preds), dev_payload)
+        self.check(st, "hashmap_build")
+
+    def hashmap_filled(self, m):
+        v = u64()
+        self.check(self.lib.golap_hashmap_filled(self.h, m, C.byref(v)))
+        return v.value
+
+    def hashmap_probe(self, m, dev_keys, key_type, n, dev_out):
+        self.check(self.lib.golap_hashmap_probe(self.h, m, dev_keys, key_type, n, dev_out), "probe")
+
+    def hash_join_count(self, m, dev_keys, key_type, n):
+        v = u64()
+        self.check(self.lib.golap_hash_join_count(self.h, m, dev_keys, key_type, n, C.byref(v)), "count")
+        return v.value
+
+    def free_hashmap(self, m):
+        self.check(self.lib.golap_hashmap_free(self.h, m), "hashmap_free")
+
+    # zone maps
+    def minmax_check(self, col, lo, hi, dev_flags):
+        self.check(self.lib.golap_minmax_check(self.h, col, lo, hi, dev_flags), "minmax_check")
+
+    def combine_and(self, dev_a, dev_b, n):
+        self.check(self.lib.golap_combine_and(self.h, dev_a, dev_b, n), "combine_and")
+
+    # plans
+    def q1_run(self, date, disc, qty, price, date_map, disc_rng, qty_rng, price_rng=(-2**31, 2**31 - 1),
+               chunk_begin=0, chunk_end=None, dev_chunk_mask=None):
+        if chunk_end is None:
+            chunk_end = self.column_desc(date).n_chunks
+        p = Q1Plan(date, disc, qty, price, date_map, disc_rng[0], disc_rng[1], qty_rng[0],
+                   qty_rng[1], price_rng[0], price_rng[1], chunk_begin, chunk_end, dev_chunk_mask)
+        r = Q1Result()
+        self.check(self.lib.golap_q1_run(self.h, C.byref(p), C.byref(r)), "q1_run")
+        return r
+
+
+def gen_date_dim(n_days):
+    key = np.empty(n_days, np.int32)
+    year = np.empty(n_days, np.int16)
+    month = np.empty(n_days, np.uint8)
+    load().golap_gen_date_dim(n_days, _ptr(key), _ptr(year), _ptr(month))
+    return key, year, month
+
+
+def gen_q1_fact(seed, n, date_keys, row0=0, threads=0):
+    """Seeded Q1 fact columns (date_key, discount, quantity, price) for rows [row0, row0+n)."""
+    date_keys = np.ascontiguousarray(date_keys, dtype=np.int32)
+    cols = (np.empty(n, np.int32), np.empty(n, np.uint8), np.empty(n, np.uint8), np.empty(n, np.int32))
+    st = load().golap_gen_q1_fact(seed, row0, n, _ptr(date_keys), date_keys.size,
+                                  *[_ptr(c) for c in cols], threads)
+    if st != OK:
+        raise GolapError(st, "gen_q1_fact")
+    return cols
+
+
+def header_symbols():
+    """Function names declared in include/golap_hip.h."""
+    import re
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\s*\*?\s*(golap_[a-z_0-9]+)\s*\(",
+                                 text, re.M)))

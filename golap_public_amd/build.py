@@ -1,0 +1,56 @@
+"""Builds the gfx950 library golap_public_amd/libgolap_hip.so in-tree with hipcc.
+
+Usage: python -m golap_public_amd.build  (or build() from Python).
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libgolap_hip.so")
+
+HIP_SOURCES = ["kernels.hip", "capi.cpp", "encode.cpp", "synth.cpp"]
+
+# -packed-fp32-ops: no v_pk_*_f32 generation (nothing on this path is floating point, the flag
+# keeps it that way if code is added).
+HIPCC_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-Wall",
+    "-Wno-unused-result",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def hipcc():
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def build(verbose=False):
+    """Compile every source for gfx950 and link the shared library (always recompiles)."""
+    out_tmp = LIB + ".tmp"
+    cmd = [hipcc()] + HIPCC_FLAGS + ["-shared", "-o", out_tmp, "-x", "hip"]
+    cmd += [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    # the host compilation pass does not know the gfx950 feature flag and says so; harmless
+    lines = [l for l in proc.stdout.splitlines() if "packed-fp32-ops" not in l]
+    if lines:
+        print("\n".join(lines), flush=True)
+    if proc.returncode != 0:
+        raise subprocess.CalledProcessError(proc.returncode, cmd)
+    os.replace(out_tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose="-v" in sys.argv)
+    print(LIB)

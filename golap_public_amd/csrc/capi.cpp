@@ -1,0 +1,676 @@
+// C ABI implementation (include/golap_hip.h). Host orchestration of the reference's plan
+// functions (usecases/ssb/queries/01.hpp:53-449) with HIP streams and events.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <limits>
+#include <memory>
+
+#include "common.hpp"
+#include "format.hpp"
+
+using namespace golap_impl;
+
+namespace {
+
+using Lock = std::lock_guard<std::recursive_mutex>;
+
+struct ColumnExtra {
+    std::vector<uint32_t> nondirect_prefix;  // pages that cannot be read in place, prefix count
+    std::vector<uint64_t> chunk_payload;     // payload bytes per chunk, prefix
+};
+std::mutex g_extra_mu;
+std::unordered_map<const golap_column *, ColumnExtra> g_extra;
+
+bool page_direct_host(uint8_t codec) {
+    return (codec & golap_fmt::PAGE_CODEC_MASK) == golap_fmt::RAW ||
+           (codec & golap_fmt::PAGE_LITERAL_ONLY);
+}
+
+int dev_alloc(golap_ctx *ctx, uint64_t bytes, void **p) {
+    *p = nullptr;
+    if (!bytes) bytes = 16;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return fail(ctx, GOLAP_ENOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    }
+    ctx->allocated += bytes;
+    ctx->allocs[*p] = bytes;
+    return GOLAP_OK;
+}
+
+void dev_release(golap_ctx *ctx, void *p) {
+    if (!p) return;
+    auto it = ctx->allocs.find(p);
+    if (it != ctx->allocs.end()) {
+        ctx->allocated -= it->second;
+        ctx->allocs.erase(it);
+    }
+    (void)hipFree(p);
+}
+
+int set_device(golap_ctx *ctx) {
+    GOLAP_HIP_CHECK(ctx, hipSetDevice(ctx->device));
+    return GOLAP_OK;
+}
+
+// Reads and clears the device error word; maps it to a status.
+int check_device_errors(golap_ctx *ctx, const char *what) {
+    uint32_t flags = 0;
+    GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(&flags, ctx->d_scratch, 4, hipMemcpyDeviceToHost, ctx->stream));
+    GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (!flags) return GOLAP_OK;
+    GOLAP_HIP_CHECK(ctx, hipMemsetAsync(ctx->d_scratch, 0, 4, ctx->stream));
+    if (flags & DEVERR_CORRUPT) return fail(ctx, GOLAP_ECORRUPT, std::string(what) + ": corrupt page");
+    if (flags & DEVERR_CAPACITY) return fail(ctx, GOLAP_ECAPACITY, std::string(what) + ": table full");
+    return fail(ctx, GOLAP_EINVAL, std::string(what) + ": key outside (INT32_MIN, INT32_MAX]");
+}
+
+bool valid_type(uint32_t t) { return t >= GOLAP_I8 && t <= GOLAP_I64; }
+
+}  // namespace
+
+extern "C" {
+
+int golap_api_version(void) { return GOLAP_API_VERSION; }
+
+const char *golap_status_string(int s) {
+    switch (s) {
+    case GOLAP_OK: return "ok";
+    case GOLAP_EINVAL: return "invalid argument";
+    case GOLAP_ENOMEM: return "out of memory";
+    case GOLAP_EHIP: return "HIP runtime error";
+    case GOLAP_ECORRUPT: return "corrupt compressed page";
+    case GOLAP_ECAPACITY: return "capacity exceeded";
+    case GOLAP_EBUDGET: return "HBM budget exceeded";
+    case GOLAP_ENODEV: return "no HIP device";
+    default: return "unknown status";
+    }
+}
+
+int golap_ctx_create(int device, golap_ctx **out) {
+    if (!out) return GOLAP_EINVAL;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        (void)hipGetLastError();
+        return GOLAP_ENODEV;
+    }
+    if (device < 0 || device >= n) return GOLAP_EINVAL;
+    std::unique_ptr<golap_ctx> c(new golap_ctx());
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess) return GOLAP_EHIP;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return GOLAP_EHIP;
+    c->cus = prop.multiProcessorCount;
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+        c->lds_max = lds;
+    strncpy(c->name, prop.name, sizeof(c->name) - 1);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return GOLAP_EHIP;
+    if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) return GOLAP_EHIP;
+    void *s = nullptr;
+    if (dev_alloc(c.get(), golap_ctx::SCRATCH_BYTES, &s) != GOLAP_OK) return GOLAP_ENOMEM;
+    c->d_scratch = static_cast<uint8_t *>(s);
+    if (hipMemset(c->d_scratch, 0, golap_ctx::SCRATCH_BYTES) != hipSuccess) return GOLAP_EHIP;
+    *out = c.release();
+    return GOLAP_OK;
+}
+
+int golap_ctx_destroy(golap_ctx *ctx) {
+    if (!ctx) return GOLAP_EINVAL;
+    {
+        Lock l(ctx->mu);
+        (void)hipSetDevice(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        for (auto &kv : ctx->allocs) (void)hipFree(kv.first);
+        ctx->allocs.clear();
+        (void)hipEventDestroy(ctx->ev0);
+        (void)hipEventDestroy(ctx->ev1);
+        (void)hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+    return GOLAP_OK;
+}
+
+const char *golap_last_error(golap_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int golap_sync(golap_ctx *ctx) {
+    if (!ctx) return GOLAP_EINVAL;
+    Lock l(ctx->mu);
+    if (int r = set_device(ctx)) return r;
+    GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return GOLAP_OK;
+}
+
+int golap_device_info(golap_ctx *ctx, int *cus, char *name, int name_len) {
+    if (!ctx) return GOLAP_EINVAL;
+    if (cus) *cus = ctx->cus;
+    if (name && name_len > 0) {
+        strncpy(name, ctx->name, (size_t)name_len - 1);
+        name[name_len - 1] = 0;
+    }
+    return GOLAP_OK;
+}
+
+// ------------------------------------------------------------------------------ memory
+int golap_dev_alloc(golap_ctx *ctx, uint64_t bytes, void **dev_ptr) {
+    if (!ctx || !dev_ptr) return GOLAP_EINVAL;
+    Lock l(ctx->mu);
+    if (int r = set_device(ctx)) return r;
+    return dev_alloc(ctx, bytes, dev_ptr);
+}
+int golap_dev_free(golap_ctx *ctx, void *dev_ptr) {
+    if (!ctx) return GOLAP_EINVAL;
+    Lock l(ctx->mu);
+    if (!dev_ptr) return GOLAP_OK;
+    if (!ctx->allocs.count(dev_ptr)) return fail(ctx, GOLAP_EINVAL, "dev_free: unknown pointer");
+    (void)hipStreamSynchronize(ctx->stream);
+    dev_release(ctx, dev_ptr);
+    return GOLAP_OK;
+}
+int golap_h2d(golap_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return GOLAP_EINVAL;
+    Lock l(ctx->mu);
+    if (!bytes) return GOLAP_OK;
+    if (int r = set_device(ctx)) return r;
+    GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return GOLAP_OK;
+}
+int golap_d2h(golap_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return GOLAP_EINVAL;
+    Lock l(ctx->mu);
+    if (!bytes) return GOLAP_OK;
+    if (int r = set_device(ctx)) return r;
+    GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return GOLAP_OK;
+}
+int golap_memset(golap_ctx *ctx, void *dst, int value, uint64_t bytes) {
+    if (!ctx || (bytes && !dst)) return GOLAP_EINVAL;
+    Lock l(ctx->mu);
+    if (!bytes) return GOLAP_OK;
+    if (int r = set_device(ctx)) return r;
+    GOLAP_HIP_CHECK(ctx, hipMemsetAsync(dst, value, bytes, ctx->stream));
+    return GOLAP_OK;
+}
+uint64_t golap_device_allocated(golap_ctx *ctx) { return ctx ? ctx->allocated : 0; }
+
+// ------------------------------------------------------------------------------ write path
+void golap_encode_opts_default(golap_encode_opts *o) {
+    if (!o) return;
+    o->codec = GOLAP_CODEC_LZ4;
+    o->page_rows = 4096;
+    o->chunk_rows = 262144;
+    o->lz4_min_match = 4;
+    o->threads = 0;
+}
+
+int golap_encode(const void *host_values, uint64_t n_rows, uint32_t type,
+                 const golap_encode_opts *opts, golap_host_column **out) {
+    if (!out || !valid_type(type)) return GOLAP_EINVAL;
+    *out = nullptr;
+    golap_encode_opts o;
+    golap_encode_opts_default(&o);
+    if (opts) o = *opts;
+    std::unique_ptr<golap_host_column> c(new golap_host_column());
+    std::string err;
+    int r = golap_host::encode_column(host_values, n_rows, type, o, *c, err);
+    if (r != GOLAP_OK) return r;
+    *out = c.release();
+    return GOLAP_OK;
+}
+
+int golap_host_column_assemble(uint32_t type, uint64_t n_rows, uint32_t page_rows,
+                               uint32_t chunk_rows, const uint8_t *const *pages,
+                               const uint32_t *csize, const uint8_t *codec,
+                               golap_host_column **out) {
+    if (!out || !valid_type(type) || !page_rows || page_rows % 64 || !chunk_rows ||
+        chunk_rows % page_rows)
+        return GOLAP_EINVAL;
+    *out = nullptr;
+    std::unique_ptr<golap_host_column> c(new golap_host_column());
+    c->type = type;
+    c->elem = golap_host::type_size(type);
+    c->page_rows = page_rows;
+    c->chunk_rows = chunk_rows;
+    c->n_rows = n_rows;
+    c->n_pages = (n_rows + page_rows - 1) / page_rows;
+    c->n_chunks = (n_rows + chunk_rows - 1) / chunk_rows;
+    if (c->n_pages && (!pages || !csize || !codec)) return GOLAP_EINVAL;
+    c->page_off.resize(c->n_pages);
+    c->page_csize.assign(csize, csize + c->n_pages);
+    c->page_codec.resize(c->n_pages);
+    std::vector<const uint8_t *> ptrs(c->n_pages);
+    for (uint64_t p = 0; p < c->n_pages; ++p) {
+        uint8_t cd = codec[p] & golap_fmt::PAGE_CODEC_MASK;
+        if (cd > golap_fmt::RLE) return GOLAP_EINVAL;
+        if (csize[p] && !pages[p]) return GOLAP_EINVAL;
+        uint64_t rows = std::min<uint64_t>(page_rows, n_rows - p * page_rows);
+        if (cd == golap_fmt::LZ4 && golap_host::lz4_is_literal_only(pages[p], csize[p], rows * c->elem))
+            cd |= golap_fmt::PAGE_LITERAL_ONLY;
+        c->page_codec[p] = cd;
+        ptrs[p] = pages[p];
+    }
+    golap_host::layout_pages(*c, ptrs);
+    // no statistics are known: every chunk may qualify
+    c->chunk_min.assign(c->n_chunks, std::numeric_limits<int64_t>::min());
+    c->chunk_max.assign(c->n_chunks, std::numeric_limits<int64_t>::max());
+    *out = c.release();
+    return GOLAP_OK;
+}
+
+int golap_host_column_free(golap_host_column *col) {
+    delete col;
+    return GOLAP_OK;
+}
+
+int golap_host_column_info(const golap_host_column *c, golap_column_desc *o) {
+    if (!c || !o) return GOLAP_EINVAL;
+    o->n_rows = c->n_rows;
+    o->n_pages = c->n_pages;
+    o->n_chunks = c->n_chunks;
+    o->comp_bytes = c->comp.size();
+    o->payload_bytes = c->payload_bytes;
+    o->uncomp_bytes = c->n_rows * c->elem;
+    o->type = c->type;
+    o->elem = c->elem;
+    o->page_rows = c->page_rows;
+    o->chunk_rows = c->chunk_rows;
+    o->comp = c->comp.data();
+    o->page_off = c->page_off.data();
+    o->page_csize = c->page_csize.data();
+    o->page_codec = c->page_codec.data();
+    o->chunk_min = c->chunk_min.data();
+    o->chunk_max = c->chunk_max.data();
+    return GOLAP_OK;
+}
+
+int golap_column_upload(golap_ctx *ctx, const golap_host_column *h, golap_column **out) {
+    if (!ctx || !h || !out) return GOLAP_EINVAL;
+    *out = nullptr;
+    Lock l(ctx->mu);
+    if (int r = set_device(ctx)) return r;
+    std::unique_ptr<golap_column> c(new golap_column());
+    c->ctx = ctx;
+    c->type = h->type;
+    c->elem = h->elem;
+    c->page_rows = h->page_rows;
+    c->chunk_rows = h->chunk_rows;
+    c->n_rows = h->n_rows;
+    c->n_pages = h->n_pages;
+    c->n_chunks = h->n_chunks;
+    c->comp_bytes = h->comp.size();
+    c->payload_bytes = h->payload_bytes;
+    c->page_off = h->page_off;
+    c->page_csize = h->page_csize;
+    c->page_codec = h->page_codec;
+    c->chunk_min = h->chunk_min;
+    c->chunk_max = h->chunk_max;
+    void *p[6];
+    const uint64_t sz[6] = {h->comp.size(), h->n_pages * 8, h->n_pages * 4, h->n_pages,
+                            h->n_chunks * 8, h->n_chunks * 8};
+    const void *src[6] = {h->comp.data(), h->page_off.data(), h->page_csize.data(),
+                          h->page_codec.data(), h->chunk_min.data(), h->chunk_max.data()};
+    for (int i = 0; i < 6; ++i) {
+        if (int r = dev_alloc(ctx, sz[i], &p[i])) {
+            for (int j = 0; j < i; ++j) dev_release(ctx, p[j]);
+            return r;
+        }
+        if (sz[i]) {
+            hipError_t e = hipMemcpyAsync(p[i], src[i], sz[i], hipMemcpyHostToDevice, ctx->stream);
+            if (e != hipSuccess) {
+                for (int j = 0; j <= i; ++j) dev_release(ctx, p[j]);
+                return fail(ctx, GOLAP_EHIP, std::string("upload: ") + hipGetErrorString(e));
+            }
+        }
+    }
+    GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    c->d_comp = static_cast<uint8_t *>(p[0]);
+    c->d_page_off = static_cast<uint64_t *>(p[1]);
+    c->d_page_csize = static_cast<uint32_t *>(p[2]);
+    c->d_page_codec = static_cast<uint8_t *>(p[3]);
+    c->d_chunk_min = static_cast<int64_t *>(p[4]);
+    c->d_chunk_max = static_cast<int64_t *>(p[5]);
+    ColumnExtra ex;
+    ex.nondirect_prefix.resize(c->n_pages + 1, 0);
+    for (uint64_t i = 0; i < c->n_pages; ++i)
+        ex.nondirect_prefix[i + 1] = ex.nondirect_prefix[i] + (page_direct_host(c->page_codec[i]) ? 0 : 1);
+    ex.chunk_payload.resize(c->n_chunks + 1, 0);
+    const uint64_t ppc = c->chunk_rows / c->page_rows;
+    for (uint64_t ch = 0; ch < c->n_chunks; ++ch) {
+        uint64_t s = 0;
+        for (uint64_t i = ch * ppc; i < std::min<uint64_t>(c->n_pages, (ch + 1) * ppc); ++i) s += c->page_csize[i];
+        ex.chunk_payload[ch + 1] = ex.chunk_payload[ch] + s;
+    }
+    {
+        std::lock_guard<std::mutex> g(g_extra_mu);
+        g_extra[c.get()] = std::move(ex);
+    }
+    *out = c.release();
+    return GOLAP_OK;
+}
+
+int golap_column_free(golap_ctx *ctx, golap_column *c) {
+    if (!ctx || !c) return GOLAP_EINVAL;
+    Lock l(ctx->mu);
+    (void)hipStreamSynchronize(ctx->stream);
+    dev_release(ctx, c->d_comp);
+    dev_release(ctx, c->d_page_off);
+    dev_release(ctx, c->d_page_csize);
+    dev_release(ctx, c->d_page_codec);
+    dev_release(ctx, c->d_chunk_min);
+    dev_release(ctx, c->d_chunk_max);
+    {
+        std::lock_guard<std::mutex> g(g_extra_mu);
+        g_extra.erase(c);
+    }
+    delete c;
+    return GOLAP_OK;
+}
+
+int golap_column_info(const golap_column *c, golap_column_desc *o) {
+    if (!c || !o) return GOLAP_EINVAL;
+    o->n_rows = c->n_rows;
+    o->n_pages = c->n_pages;
+    o->n_chunks = c->n_chunks;
+    o->comp_bytes = c->comp_bytes;
+    o->payload_bytes = c->payload_bytes;
+    o->uncomp_bytes = c->n_rows * c->elem;
+    o->type = c->type;
+    o->elem = c->elem;
+    o->page_rows = c->page_rows;
+    o->chunk_rows = c->chunk_rows;
+    o->comp = c->d_comp;
+    o->page_off = c->d_page_off;
+    o->page_csize = c->d_page_csize;
+    o->page_codec = c->d_page_codec;
+    o->chunk_min = c->d_chunk_min;
+    o->chunk_max = c->d_chunk_max;
+    return GOLAP_OK;
+}
+
+int golap_prepare_compressed_device(golap_ctx *ctx, const void *host_values, uint64_t n_rows,
+                                    uint32_t type, const golap_encode_opts *opts,
+                                    golap_column **out, double *comp_ms) {
+    if (!ctx || !out) return GOLAP_EINVAL;
+    auto t0 = std::chrono::steady_clock::now();
+    golap_host_column *h = nullptr;
+    golap_encode_opts o;
+    golap_encode_opts_default(&o);
+    if (opts) o = *opts;
+    std::unique_ptr<golap_host_column> hc(new golap_host_column());
+    std::string err;
+    int r = golap_host::encode_column(host_values, n_rows, type, o, *hc, err);
+    if (r != GOLAP_OK) return fail(ctx, r, err);
+    h = hc.get();
+    auto t1 = std::chrono::steady_clock::now();
+    if (comp_ms) *comp_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    return golap_column_upload(ctx, h, out);
+}
+
+// ------------------------------------------------------------------------------ decompression
+int golap_decompress_chunks(golap_ctx *ctx, const golap_column *c, uint64_t cb, uint64_t ce,
+                            void *dev_out, double *kernel_ms) {
+    if (!ctx || !c || !dev_out || cb > ce || ce > c->n_chunks) return GOLAP_EINVAL;
+    Lock l(ctx->mu);
+    if (int r = set_device(ctx)) return r;
+    if (kernel_ms) *kernel_ms = 0;
+    if (cb == ce) return GOLAP_OK;
+    const uint64_t ppc = c->chunk_rows / c->page_rows;
+    const uint64_t row0 = cb * c->chunk_rows, row_end = ce * c->chunk_rows;
+    if (row_end > c->n_rows) {  // zero-padded tail of the last chunk
+        uint64_t pad0 = (c->n_rows - row0) * c->elem;
+        GOLAP_HIP_CHECK(ctx, hipMemsetAsync(static_cast<uint8_t *>(dev_out) + pad0, 0,
+                                            (row_end - c->n_rows) * c->elem, ctx->stream));
+    }
+    GOLAP_HIP_CHECK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    GOLAP_HIP_CHECK(ctx, launch_decode_pages(ctx, c, cb * ppc, std::min<uint64_t>(c->n_pages, ce * ppc),
+                                             static_cast<uint8_t *>(dev_out), row0));
+    GOLAP_HIP_CHECK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    if (int r = check_device_errors(ctx, "decompress")) return r;
+    if (kernel_ms) {
+        float ms = 0;
+        GOLAP_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+        *kernel_ms = ms;
+    }
+    return GOLAP_OK;
+}
+
+// ------------------------------------------------------------------------------ hash join
+static uint32_t log2_ceil(uint64_t v) {
+    uint32_t b = 0;
+    while ((1ull << b) < v) ++b;
+    return b;
+}
+
+int golap_hashmap_create(golap_ctx *ctx, uint64_t ht_size, const void *dev_keys, uint32_t key_type,
+                         golap_hashmap **out) {
+    if (!ctx || !out || !dev_keys || !valid_type(key_type) || ht_size > (1ull << 31))
+        return GOLAP_EINVAL;
+    *out = nullptr;
+    Lock l(ctx->mu);
+    if (int r = set_device(ctx)) return r;
+    std::unique_ptr<golap_hashmap> m(new golap_hashmap());
+    m->ctx = ctx;
+    m->ht_size = ht_size;
+    m->d_keys = dev_keys;
+    m->key_type = key_type;
+    m->log2cap = std::max<uint32_t>(6, log2_ceil(std::max<uint64_t>(1, ht_size) * 2));  // load <= 1/2
+    void *p = nullptr;
+    if (int r = dev_alloc(ctx, 8ull << m->log2cap, &p)) return r;
+    m->d_slots = static_cast<uint64_t *>(p);
+    GOLAP_HIP_CHECK(ctx, launch_hash_clear(ctx, m->d_slots, 1ull << m->log2cap));
+    *out = m.release();
+    return GOLAP_OK;
+}
+
+int golap_hashmap_build(golap_ctx *ctx, golap_hashmap *m, uint64_t n_rows, const golap_pred *preds,
+                        uint32_t n_preds, const int32_t *dev_payload) {
+    if (!ctx || !m || m->ctx != ctx || n_preds > MAX_PREDS || (n_preds && !preds)) return GOLAP_EINVAL;
+    Lock l(ctx->mu);
+    if (int r = set_device(ctx)) return r;
+    PredView pv[MAX_PREDS];
+    for (uint32_t i = 0; i < n_preds; ++i) {
+        if (!preds[i].dev_col || !valid_type(preds[i].type)) return fail(ctx, GOLAP_EINVAL, "build: bad predicate");
+        pv[i] = PredView{preds[i].dev_col, preds[i].type, preds[i].lo, preds[i].hi};
+    }
+    // a rebuild starts from an empty table
+    GOLAP_HIP_CHECK(ctx, launch_hash_clear(ctx, m->d_slots, 1ull << m->log2cap));
+    uint64_t *d_filled = reinterpret_cast<uint64_t *>(ctx->d_scratch + 64);
+    GOLAP_HIP_CHECK(ctx, hipMemsetAsync(d_filled, 0, 8, ctx->stream));
+    GOLAP_HIP_CHECK(ctx, launch_hash_build(ctx, m, n_rows, pv, n_preds, dev_payload, d_filled));
+    GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(&m->filled, d_filled, 8, hipMemcpyDeviceToHost, ctx->stream));
+    if (int r = check_device_errors(ctx, "hashmap_build")) return r;
+    if (m->filled > m->ht_size)
+        return fail(ctx, GOLAP_ECAPACITY, "hashmap_build: more qualifying rows than ht_size");
+    // compact copy for LDS pinning (load <= 1/2), at most 4096 slots = 32 KiB
+    dev_release(ctx, m->d_small);
+    m->d_small = nullptr;
+    m->small_log2 = 0;
+    const uint32_t sl = std::max<uint32_t>(6, log2_ceil(std::max<uint64_t>(1, m->filled) * 2));
+    if (sl <= 12 && sl < m->log2cap) {
+        void *p = nullptr;
+        if (int r = dev_alloc(ctx, 8ull << sl, &p)) return r;
+        m->d_small = static_cast<uint64_t *>(p);
+        m->small_log2 = sl;
+        GOLAP_HIP_CHECK(ctx, launch_hash_clear(ctx, m->d_small, 1ull << sl));
+        GOLAP_HIP_CHECK(ctx, launch_hash_compact(ctx, m));
+        if (int r = check_device_errors(ctx, "hashmap_compact")) return r;
+    }
+    m->built = true;
+    return GOLAP_OK;
+}
+
+int golap_hashmap_probe(golap_ctx *ctx, const golap_hashmap *m, const void *keys, uint32_t key_type,
+                        uint64_t n, uint64_t *dev_out_rid) {
+    if (!ctx || !m || m->ctx != ctx || (n && (!keys || !dev_out_rid)) || !valid_type(key_type))
+        return GOLAP_EINVAL;
+    Lock l(ctx->mu);
+    if (int r = set_device(ctx)) return r;
+    GOLAP_HIP_CHECK(ctx, launch_hash_probe(ctx, m, keys, key_type, n, dev_out_rid, nullptr));
+    GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return GOLAP_OK;
+}
+
+int golap_hash_join_count(golap_ctx *ctx, const golap_hashmap *m, const void *keys, uint32_t key_type,
+                          uint64_t n, uint64_t *hits) {
+    if (!ctx || !m || m->ctx != ctx || !hits || (n && !keys) || !valid_type(key_type)) return GOLAP_EINVAL;
+    Lock l(ctx->mu);
+    if (int r = set_device(ctx)) return r;
+    uint64_t *d_hits = reinterpret_cast<uint64_t *>(ctx->d_scratch + 64);
+    GOLAP_HIP_CHECK(ctx, hipMemsetAsync(d_hits, 0, 8, ctx->stream));
+    GOLAP_HIP_CHECK(ctx, launch_hash_probe(ctx, m, keys, key_type, n, nullptr, d_hits));
+    GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(hits, d_hits, 8, hipMemcpyDeviceToHost, ctx->stream));
+    GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return GOLAP_OK;
+}
+
+int golap_hashmap_filled(golap_ctx *ctx, const golap_hashmap *m, uint64_t *filled) {
+    if (!ctx || !m || !filled) return GOLAP_EINVAL;
+    *filled = m->filled;
+    return GOLAP_OK;
+}
+
+int golap_hashmap_free(golap_ctx *ctx, golap_hashmap *m) {
+    if (!ctx || !m || m->ctx != ctx) return GOLAP_EINVAL;
+    Lock l(ctx->mu);
+    (void)hipStreamSynchronize(ctx->stream);
+    dev_release(ctx, m->d_slots);
+    dev_release(ctx, m->d_small);
+    delete m;
+    return GOLAP_OK;
+}
+
+// ------------------------------------------------------------------------------ zone maps
+int golap_minmax_check(golap_ctx *ctx, const golap_column *c, int64_t lo, int64_t hi, uint16_t *flags) {
+    if (!ctx || !c || !flags) return GOLAP_EINVAL;
+    Lock l(ctx->mu);
+    if (int r = set_device(ctx)) return r;
+    GOLAP_HIP_CHECK(ctx, launch_minmax_check(ctx, c, lo, hi, flags));
+    return GOLAP_OK;
+}
+
+int golap_combine_and(golap_ctx *ctx, uint16_t *a, const uint16_t *b, uint64_t n) {
+    if (!ctx || (n && (!a || !b))) return GOLAP_EINVAL;
+    Lock l(ctx->mu);
+    if (int r = set_device(ctx)) return r;
+    GOLAP_HIP_CHECK(ctx, launch_combine_and(ctx, a, b, n));
+    return GOLAP_OK;
+}
+
+// ------------------------------------------------------------------------------ Q1 plan
+static int32_t clamp32(int64_t v, int64_t lo, int64_t hi) { return (int32_t)std::min(std::max(v, lo), hi); }
+
+int golap_q1_run(golap_ctx *ctx, const golap_q1_plan *plan, golap_q1_result *res) {
+    if (!ctx || !plan || !res) return GOLAP_EINVAL;
+    const golap_column *cols[4] = {plan->date, plan->disc, plan->qty, plan->price};
+    for (auto *c : cols)
+        if (!c || c->ctx != ctx) return fail(ctx, GOLAP_EINVAL, "q1: missing column");
+    const golap_hashmap *m = plan->date_map;
+    if (!m || m->ctx != ctx || !m->built) return fail(ctx, GOLAP_EINVAL, "q1: date map not built");
+    const golap_column *d = plan->date;
+    for (auto *c : cols)
+        if (c->n_rows != d->n_rows || c->page_rows != d->page_rows || c->chunk_rows != d->chunk_rows)
+            return fail(ctx, GOLAP_EINVAL, "q1: columns are not row-aligned");
+    auto is32 = [](const golap_column *c) { return c->type == GOLAP_I32 || c->type == GOLAP_U32; };
+    if (!is32(plan->date) || !is32(plan->price) || plan->disc->type != GOLAP_U8 || plan->qty->type != GOLAP_U8)
+        return fail(ctx, GOLAP_EINVAL, "q1: expected date/price 32-bit, discount/quantity uint8");
+    for (const golap_column *c : {plan->date, plan->price})
+        if (c->type == GOLAP_U32)
+            for (int64_t mx : c->chunk_max)
+                if (mx > INT32_MAX) return fail(ctx, GOLAP_EINVAL, "q1: uint32 values above INT32_MAX");
+    if (plan->chunk_begin > plan->chunk_end || plan->chunk_end > d->n_chunks)
+        return fail(ctx, GOLAP_EINVAL, "q1: bad chunk range");
+    Lock l(ctx->mu);
+    if (int r = set_device(ctx)) return r;
+    memset(res, 0, sizeof(*res));
+
+    const uint64_t ppc = d->chunk_rows / d->page_rows;
+    Q1Launch a{};
+    a.date = view(plan->date);
+    a.disc = view(plan->disc);
+    a.qty = view(plan->qty);
+    a.price = view(plan->price);
+    a.page_rows = d->page_rows;
+    a.n_rows = d->n_rows;
+    a.page_begin = plan->chunk_begin * ppc;
+    a.page_end = std::min<uint64_t>(d->n_pages, plan->chunk_end * ppc);
+    a.chunk_mask = plan->dev_chunk_mask;
+    a.pages_per_chunk = (uint32_t)ppc;
+    a.disc_lo = clamp32(plan->disc_lo, -1, 256);
+    a.disc_hi = clamp32(plan->disc_hi, -1, 256);
+    a.qty_lo = clamp32(plan->qty_lo, -1, 256);
+    a.qty_hi = clamp32(plan->qty_hi, -1, 256);
+    a.price_lo = clamp32(plan->price_lo, INT32_MIN, INT32_MAX);
+    a.price_hi = clamp32(plan->price_hi, INT32_MIN, INT32_MAX);
+    if (m->d_small) {
+        a.slots = m->d_small;
+        a.log2cap = m->small_log2;
+        a.in_lds = true;
+    } else {
+        a.slots = m->d_slots;
+        a.log2cap = m->log2cap;
+        a.in_lds = false;
+    }
+    a.err = reinterpret_cast<uint32_t *>(ctx->d_scratch);
+    a.sum = reinterpret_cast<int64_t *>(ctx->d_scratch + 8);
+    a.count = reinterpret_cast<uint64_t *>(ctx->d_scratch + 16);
+    GOLAP_HIP_CHECK(ctx, hipMemsetAsync(ctx->d_scratch + 8, 0, 16, ctx->stream));
+
+    bool any_general = false;
+    {
+        std::lock_guard<std::mutex> g(g_extra_mu);
+        for (auto *c : cols) {
+            auto &ex = g_extra[c];
+            if (ex.nondirect_prefix[a.page_end] != ex.nondirect_prefix[a.page_begin]) any_general = true;
+        }
+    }
+    GOLAP_HIP_CHECK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    if (a.page_end > a.page_begin) {
+        GOLAP_HIP_CHECK(ctx, launch_q1(ctx, a, true));
+        if (any_general) GOLAP_HIP_CHECK(ctx, launch_q1(ctx, a, false));
+    }
+    GOLAP_HIP_CHECK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    uint64_t host[3] = {0, 0, 0};
+    GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(host, ctx->d_scratch, 24, hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<uint16_t> mask;
+    if (plan->dev_chunk_mask && plan->chunk_end > plan->chunk_begin) {
+        mask.resize(plan->chunk_end);
+        GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(mask.data() + plan->chunk_begin,
+                                            plan->dev_chunk_mask + plan->chunk_begin,
+                                            (plan->chunk_end - plan->chunk_begin) * 2,
+                                            hipMemcpyDeviceToHost, ctx->stream));
+    }
+    GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    float ms = 0;
+    GOLAP_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    if (host[0]) {
+        GOLAP_HIP_CHECK(ctx, hipMemsetAsync(ctx->d_scratch, 0, 4, ctx->stream));
+        GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        return fail(ctx, GOLAP_ECORRUPT, "q1: corrupt page");
+    }
+    res->sum = (int64_t)host[1];
+    res->matches = host[2];
+    res->kernel_ms = ms;
+    {
+        std::lock_guard<std::mutex> g(g_extra_mu);
+        for (uint64_t ch = plan->chunk_begin; ch < plan->chunk_end; ++ch) {
+            if (!mask.empty() && !mask[ch]) {
+                res->chunks_pruned++;
+                continue;
+            }
+            res->chunks_run++;
+            for (auto *c : cols) {
+                auto &ex = g_extra[c];
+                res->comp_bytes += ex.chunk_payload[ch + 1] - ex.chunk_payload[ch];
+            }
+        }
+    }
+    return GOLAP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,131 @@
+// Host-side objects behind the C ABI handles.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/golap_hip.h"
+#include "host_column.hpp"
+
+// Device-side error flags raised by kernels (bit-or into ctx scratch word 0).
+enum : uint32_t { DEVERR_CORRUPT = 1u, DEVERR_CAPACITY = 2u, DEVERR_KEYRANGE = 4u };
+
+struct golap_ctx {
+    int device = 0;
+    int cus = 0;
+    int lds_max = 65536;  // dynamic LDS bytes a workgroup may request
+    char name[128] = {0};
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::recursive_mutex mu;
+    std::string err;
+    // Device scratch: word 0 error flags; [8,16) int64 sum; [16,24) u64 count; rest free.
+    uint8_t *d_scratch = nullptr;
+    static constexpr uint64_t SCRATCH_BYTES = 4096;
+    uint64_t allocated = 0;
+    std::unordered_map<void *, uint64_t> allocs;
+};
+
+struct golap_column {
+    golap_ctx *ctx = nullptr;
+    uint32_t type = 0, elem = 0, page_rows = 0, chunk_rows = 0;
+    uint64_t n_rows = 0, n_pages = 0, n_chunks = 0, comp_bytes = 0, payload_bytes = 0;
+    // host copies of the catalog, used to plan launches without device round trips
+    std::vector<uint64_t> page_off;
+    std::vector<uint32_t> page_csize;
+    std::vector<uint8_t> page_codec;
+    std::vector<int64_t> chunk_min, chunk_max;
+    // device copies
+    uint8_t *d_comp = nullptr;
+    uint64_t *d_page_off = nullptr;
+    uint32_t *d_page_csize = nullptr;
+    uint8_t *d_page_codec = nullptr;
+    int64_t *d_chunk_min = nullptr, *d_chunk_max = nullptr;
+};
+
+// Open addressing over 8-byte slots: slot = (uint64(payload) << 32) | (uint32(key) ^ 0x80000000).
+// The all-zero slot is empty, so INT32_MIN cannot be a build key.
+struct golap_hashmap {
+    golap_ctx *ctx = nullptr;
+    uint64_t ht_size = 0;
+    const void *d_keys = nullptr;
+    uint32_t key_type = 0;
+    uint32_t log2cap = 0;          // main table: 2^log2cap slots
+    uint64_t *d_slots = nullptr;
+    uint64_t filled = 0;           // rows inserted by the last build
+    // compact copy sized to `filled` (2^small_log2 slots), pinned in LDS by the fused plans
+    uint32_t small_log2 = 0;
+    uint64_t *d_small = nullptr;
+    bool built = false;
+};
+
+namespace golap_impl {
+
+inline int fail(golap_ctx *ctx, int code, const std::string &msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define GOLAP_HIP_CHECK(ctx, expr)                                                          \
+    do {                                                                                    \
+        hipError_t e__ = (expr);                                                            \
+        if (e__ != hipSuccess)                                                              \
+            return golap_impl::fail((ctx), GOLAP_EHIP,                                      \
+                                    std::string(#expr ": ") + hipGetErrorString(e__));      \
+    } while (0)
+
+// Device views passed to kernels by value.
+struct ColView {
+    const uint8_t *comp;
+    const uint64_t *page_off;
+    const uint32_t *page_csize;
+    const uint8_t *page_codec;
+    uint32_t elem;
+};
+inline ColView view(const golap_column *c) {
+    return ColView{c->d_comp, c->d_page_off, c->d_page_csize, c->d_page_codec, c->elem};
+}
+
+struct PredView {
+    const void *col;
+    uint32_t type;
+    int64_t lo, hi;
+};
+constexpr int MAX_PREDS = 4;
+
+// kernel launchers (kernels.hip)
+hipError_t launch_decode_pages(golap_ctx *ctx, const golap_column *col, uint64_t page_begin,
+                               uint64_t page_end, uint8_t *dev_out, uint64_t out_row0);
+hipError_t launch_hash_clear(golap_ctx *ctx, uint64_t *slots, uint64_t n);
+hipError_t launch_hash_build(golap_ctx *ctx, golap_hashmap *m, uint64_t n_rows,
+                             const PredView *preds, uint32_t n_preds, const int32_t *payload,
+                             uint64_t *d_filled);
+hipError_t launch_hash_compact(golap_ctx *ctx, golap_hashmap *m);
+hipError_t launch_hash_probe(golap_ctx *ctx, const golap_hashmap *m, const void *keys,
+                             uint32_t key_type, uint64_t n, uint64_t *out_rid, uint64_t *hits);
+hipError_t launch_minmax_check(golap_ctx *ctx, const golap_column *col, int64_t lo, int64_t hi,
+                               uint16_t *flags);
+hipError_t launch_combine_and(golap_ctx *ctx, uint16_t *a, const uint16_t *b, uint64_t n);
+
+struct Q1Launch {
+    ColView date, disc, qty, price;
+    uint32_t page_rows;
+    uint64_t n_rows;
+    uint64_t page_begin, page_end;  // row tiles (page indices) to process
+    const uint16_t *chunk_mask;     // chunk flags, 0 = pruned; may be null
+    uint32_t pages_per_chunk;
+    int32_t disc_lo, disc_hi, qty_lo, qty_hi, price_lo, price_hi;  // clamped to the types
+    const uint64_t *slots;          // date hash table
+    uint32_t log2cap;
+    bool in_lds;                    // copy the table into LDS per workgroup
+    int64_t *sum;
+    uint64_t *count;
+    uint32_t *err;
+};
+hipError_t launch_q1(golap_ctx *ctx, const Q1Launch &a, bool direct);
+
+}  // namespace golap_impl

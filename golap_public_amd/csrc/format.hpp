@@ -1,0 +1,62 @@
+// Paged compressed column format shared by the host encoder and the gfx950 decoders.
+//
+// A column of n_rows values of `elem` bytes is cut into chunks of chunk_rows rows (the
+// reference's BlockInfo unit, core/comp.cuh:23-32, and its zone-map unit, core/metadata.cuh:
+// 110-160). Each chunk is cut into pages of page_rows rows; page p covers rows
+// [p*page_rows, min(n_rows, (p+1)*page_rows)) and is coded independently. Pages play the role of
+// nvcomp's 64 KiB sub-blocks (nvchunk, core/comp.cuh:40), but are sized in rows, so page p of
+// every column of a table covers the same rows and one workgroup can consume a row tile of all
+// columns without a second pass.
+//
+// Page codecs:
+//   RAW  (0)  the values themselves.
+//   LZ4  (1)  one LZ4 block (lz4_Block_format.md): token | lit-len bytes | literals | offset |
+//             match-len bytes; the last sequence has literals only.
+//   DBP  (2)  frame-of-reference delta bit-packing (sorted or slowly varying integers).
+//   RLE  (3)  run-length encoding (value, exclusive end row) pairs.
+// Page flag bit 7 (PAGE_LITERAL_ONLY) marks an LZ4 page made of a single literal run; such a
+// page's payload starts lz4_literal_header_bytes(n_bytes) bytes after the page start and the
+// encoder places it 16-byte aligned in the column buffer, so a scan reads it in place.
+#pragma once
+#include <cstdint>
+
+namespace golap_fmt {
+
+enum Codec : uint8_t { RAW = 0, LZ4 = 1, DBP = 2, RLE = 3 };
+constexpr uint8_t PAGE_CODEC_MASK = 0x0f;
+constexpr uint8_t PAGE_LITERAL_ONLY = 0x80;
+
+constexpr uint32_t LZ4_MINMATCH = 4;
+constexpr uint32_t LZ4_LASTLITERALS = 5;  // the last 5 bytes of a block are literals
+constexpr uint32_t LZ4_MFLIMIT = 12;      // the last match starts >= 12 bytes before the end
+constexpr uint32_t LZ4_MAX_OFFSET = 65535;
+
+// Bytes of token + literal-length extension for a literal-only block of n bytes.
+__host__ __device__ inline uint32_t lz4_literal_header_bytes(uint32_t n) {
+    return n < 15 ? 1u : 2u + (n - 15) / 255u;
+}
+
+// DBP page header (32 bytes, little endian) followed by (n-1) packed deltas.
+struct DbpHeader {
+    int64_t base;       // v[0]
+    int64_t min_delta;  // min over i>=1 of v[i]-v[i-1]
+    uint32_t n;         // rows in the page
+    uint8_t bw;         // bits per packed delta, 0..32
+    uint8_t elem;       // bytes per value
+    uint8_t pad[6];
+};
+static_assert(sizeof(DbpHeader) == 32, "DbpHeader layout");
+
+// RLE page header (16 bytes) followed by values[nruns] (elem bytes each, padded to 4) and
+// uint32 ends[nruns].
+struct RleHeader {
+    uint32_t nruns;
+    uint32_t n;
+    uint8_t elem;
+    uint8_t pad[7];
+};
+static_assert(sizeof(RleHeader) == 16, "RleHeader layout");
+
+constexpr uint32_t PAGE_ALIGN = 16;
+
+}  // namespace golap_fmt

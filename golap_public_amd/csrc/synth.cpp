@@ -1,0 +1,85 @@
+// Seeded synthetic columns for the benchmark workloads (host, multithreaded).
+//
+// Counter-based: value(seed, column, row) = lo + (((splitmix64(seed*PHI ^ col<<48 ^ row) >> 32)
+// * span) >> 32), the multiply-shift range reduction of util::uniform_int (core/util.hpp:177-181)
+// applied to a SplitMix64 counter hash, so any row range can be generated independently.
+#include <algorithm>
+#include <cstdint>
+#include <thread>
+#include <vector>
+
+#include "../../include/golap_hip.h"
+
+namespace {
+
+inline uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+inline uint32_t uniform(uint64_t seed, uint32_t col, uint64_t row, uint32_t lo, uint32_t hi) {
+    const uint64_t r = splitmix64((seed * 0x9E3779B97F4A7C15ull) ^ ((uint64_t)col << 48) ^ row);
+    const uint64_t span = (uint64_t)hi - lo + 1;
+    return lo + (uint32_t)(((r >> 32) * span) >> 32);
+}
+
+template <typename F>
+void parallel_rows(uint64_t n, uint32_t threads, F f) {
+    if (!threads) threads = std::max(1u, std::thread::hardware_concurrency());
+    threads = (uint32_t)std::min<uint64_t>(threads, std::max<uint64_t>(1, n / 65536));
+    std::vector<std::thread> ts;
+    for (uint32_t t = 0; t < threads; ++t)
+        ts.emplace_back([&, t] { f(n * t / threads, n * (t + 1) / threads); });
+    for (auto &th : ts) th.join();
+}
+
+// Civil date from days since 1970-01-01 (H. Hinnant, "chrono-Compatible Low-Level Date
+// Algorithms", civil_from_days).
+void civil_from_days(int64_t z, int &y, unsigned &m, unsigned &d) {
+    z += 719468;
+    const int64_t era = (z >= 0 ? z : z - 146096) / 146097;
+    const unsigned doe = (unsigned)(z - era * 146097);
+    const unsigned yoe = (doe - doe / 1460 + doe / 36524 - doe / 146096) / 365;
+    const int64_t yy = (int64_t)yoe + era * 400;
+    const unsigned doy = doe - (365 * yoe + yoe / 4 - yoe / 100);
+    const unsigned mp = (5 * doy + 2) / 153;
+    d = doy - (153 * mp + 2) / 5 + 1;
+    m = mp < 10 ? mp + 3 : mp - 9;
+    y = (int)(yy + (m <= 2));
+}
+
+}  // namespace
+
+extern "C" {
+
+int golap_gen_date_dim(uint32_t n_days, int32_t *key, int16_t *year, uint8_t *month) {
+    for (uint32_t i = 0; i < n_days; ++i) {
+        int y;
+        unsigned m, d;
+        civil_from_days(8035 + (int64_t)i, y, m, d);  // day 8035 = 1992-01-01
+        if (key) key[i] = y * 10000 + (int)m * 100 + (int)d;
+        if (year) year[i] = (int16_t)y;
+        if (month) month[i] = (uint8_t)m;
+    }
+    return GOLAP_OK;
+}
+
+int golap_gen_q1_fact(uint64_t seed, uint64_t row0, uint64_t n, const int32_t *date_keys,
+                      uint32_t n_dates, int32_t *date_key, uint8_t *discount, uint8_t *quantity,
+                      int32_t *price, uint32_t threads) {
+    if ((date_key && (!date_keys || !n_dates))) return GOLAP_EINVAL;
+    parallel_rows(n, threads, [&](uint64_t b, uint64_t e) {
+        for (uint64_t i = b; i < e; ++i) {
+            const uint64_t r = row0 + i;
+            if (date_key) date_key[i] = date_keys[uniform(seed, 0, r, 0, n_dates - 1)];
+            if (discount) discount[i] = (uint8_t)uniform(seed, 1, r, 0, 10);
+            if (quantity) quantity[i] = (uint8_t)uniform(seed, 2, r, 1, 50);
+            if (price) price[i] = (int32_t)uniform(seed, 3, r, 90000, 10500000);
+        }
+    });
+    return GOLAP_OK;
+}
+
+}  // extern "C"

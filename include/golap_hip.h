@@ -1,0 +1,217 @@
+/*
+ * golap_hip.h - C ABI of the MI355X-native in-data-path query pipeline.
+ *
+ * The reference (nboeschen/golap_public) exposes this path only as header-only CUDA/C++
+ * templates (core/apps.cuh, core/comp.cuh, core/dev_structs.cuh, core/join.cuh, core/group.cuh,
+ * core/metadata.cuh, usecases/ssb/queries/). This header is the binary boundary beneath a C++
+ * facade (include/golap/golap.hpp) that keeps the reference's names. Each entry point names the
+ * reference interface it replaces.
+ *
+ * Conventions: every function returns an int status (GOLAP_OK or a negative GOLAP_E* code; the
+ * reference aborts via checkCudaErrors instead). Handles are opaque, owned by the caller and
+ * freed explicitly. Pointers named dev_* are device (HBM) addresses obtained from
+ * golap_dev_alloc; all others are host addresses. A context is bound to one device and may be
+ * used from several threads; calls on one context are serialised internally.
+ */
+#ifndef GOLAP_HIP_H
+#define GOLAP_HIP_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GOLAP_API_VERSION 1
+
+enum golap_status {
+    GOLAP_OK = 0,
+    GOLAP_EINVAL = -1,    /* bad argument or handle */
+    GOLAP_ENOMEM = -2,    /* host or device allocation failed */
+    GOLAP_EHIP = -3,      /* HIP runtime error (message in golap_last_error) */
+    GOLAP_ECORRUPT = -4,  /* a compressed page failed its bounds checks while decoding */
+    GOLAP_ECAPACITY = -5, /* a fixed-capacity structure (hash table, group table) overflowed */
+    GOLAP_EBUDGET = -6,   /* admission refused: request exceeds the HBM budget */
+    GOLAP_ENODEV = -7     /* no HIP device */
+};
+
+/* Value types of columns and predicates. */
+enum golap_type {
+    GOLAP_I8 = 1, GOLAP_U8 = 2, GOLAP_I16 = 3, GOLAP_U16 = 4,
+    GOLAP_I32 = 5, GOLAP_U32 = 6, GOLAP_I64 = 7
+};
+
+/* Page codecs (CompInfo::comp_algo, core/comp.cuh:35-78). */
+enum golap_codec {
+    GOLAP_CODEC_RAW = 0, GOLAP_CODEC_LZ4 = 1, GOLAP_CODEC_DBP = 2, GOLAP_CODEC_RLE = 3,
+    GOLAP_CODEC_AUTO = 255 /* smallest of LZ4/DBP/RLE per column */
+};
+
+typedef struct golap_ctx golap_ctx;
+typedef struct golap_host_column golap_host_column;
+typedef struct golap_column golap_column;
+typedef struct golap_hashmap golap_hashmap;
+
+/* ------------------------------------------------------------------ library / context */
+int golap_api_version(void);
+const char *golap_status_string(int status);
+
+/* Replaces cudaSetDevice + the implicit per-process CUDA context (01.hpp:61). */
+int golap_ctx_create(int device, golap_ctx **out);
+int golap_ctx_destroy(golap_ctx *ctx);
+/* Replaces util::Log::error_fmt (core/util.hpp:67-144): the last error of this context. */
+const char *golap_last_error(golap_ctx *ctx);
+int golap_sync(golap_ctx *ctx);
+/* Number of compute units and the device name (for reports). */
+int golap_device_info(golap_ctx *ctx, int *cus, char *name, int name_len);
+
+/* ------------------------------------------------------------------ device memory
+ * DeviceMem / MirrorMem RAII (core/mem.hpp:136-371) and the global DEVICE_ALLOCATED counter
+ * (core/mem.hpp:14-15). */
+int golap_dev_alloc(golap_ctx *ctx, uint64_t bytes, void **dev_ptr);
+int golap_dev_free(golap_ctx *ctx, void *dev_ptr);
+int golap_h2d(golap_ctx *ctx, void *dev_dst, const void *src, uint64_t bytes);
+int golap_d2h(golap_ctx *ctx, void *dst, const void *dev_src, uint64_t bytes);
+int golap_memset(golap_ctx *ctx, void *dev_dst, int value, uint64_t bytes);
+uint64_t golap_device_allocated(golap_ctx *ctx);
+
+/* ------------------------------------------------------------------ write path
+ * prepare_compressed (core/apps.cuh:148-255): the column is cut into row-aligned chunks and
+ * pages, each page coded independently, the catalog (BlockInfo, core/comp.cuh:23-32) and the
+ * per-chunk min/max zone map (MinMaxMeta::init_chunk, core/metadata.cuh:157-160) recorded. */
+typedef struct {
+    uint32_t codec;          /* golap_codec */
+    uint32_t page_rows;      /* rows per independently coded page */
+    uint32_t chunk_rows;     /* rows per chunk; a multiple of page_rows */
+    uint32_t lz4_min_match;  /* shortest match the LZ4 encoder emits (>= 4) */
+    uint32_t threads;        /* host encoder threads, 0 = all hardware threads */
+} golap_encode_opts;
+
+/* Default options: LZ4, 4096-row pages, 262,144-row chunks (1 MiB of an int32 column, the
+ * reference's tuples_per_chunk = chunk_bytes / max sizeof, usecases/ssb/queries/01.hpp:62-63). */
+void golap_encode_opts_default(golap_encode_opts *opts);
+
+int golap_encode(const void *host_values, uint64_t n_rows, uint32_t type,
+                 const golap_encode_opts *opts, golap_host_column **out);
+/* Assemble a column from pages coded elsewhere (e.g. by the reference's CPU codec liblz4);
+ * pages[i] holds csize[i] bytes coded with codec[i]. No statistics are known, so every chunk's
+ * zone map is [INT64_MIN, INT64_MAX] and no chunk is ever pruned. */
+int golap_host_column_assemble(uint32_t type, uint64_t n_rows, uint32_t page_rows,
+                               uint32_t chunk_rows, const uint8_t *const *pages,
+                               const uint32_t *csize, const uint8_t *codec,
+                               golap_host_column **out);
+int golap_host_column_free(golap_host_column *col);
+
+/* Catalog of an encoded column. For a host column the arrays are host memory; for a device
+ * column (golap_column_info) they are device memory. */
+typedef struct {
+    uint64_t n_rows, n_pages, n_chunks;
+    uint64_t comp_bytes;    /* bytes of the column buffer (pages + alignment gaps) */
+    uint64_t payload_bytes; /* sum of page sizes */
+    uint64_t uncomp_bytes;  /* n_rows * elem */
+    uint32_t type, elem, page_rows, chunk_rows;
+    const uint8_t *comp;
+    const uint64_t *page_off;
+    const uint32_t *page_csize;
+    const uint8_t *page_codec; /* codec | PAGE_LITERAL_ONLY (0x80) */
+    const int64_t *chunk_min;
+    const int64_t *chunk_max;
+} golap_column_desc;
+int golap_host_column_info(const golap_host_column *col, golap_column_desc *out);
+
+/* Upload an encoded column to HBM (prepare_compressed_device's device-resident result,
+ * core/apps.cuh:257-275; the configs keep compressed chunks resident instead of on SSD). */
+int golap_column_upload(golap_ctx *ctx, const golap_host_column *col, golap_column **out);
+int golap_column_free(golap_ctx *ctx, golap_column *col);
+int golap_column_info(const golap_column *col, golap_column_desc *out);
+/* Encode + upload; *comp_ms receives the host encode time like prepare_compressed_device's
+ * return value. */
+int golap_prepare_compressed_device(golap_ctx *ctx, const void *host_values, uint64_t n_rows,
+                                    uint32_t type, const golap_encode_opts *opts,
+                                    golap_column **out, double *comp_ms);
+
+/* ------------------------------------------------------------------ decompression
+ * ChunkDecompressor::step (core/comp.cuh:171-195), nvcomp decompress (core/comp.cuh:186):
+ * decode chunks [chunk_begin, chunk_end) into dev_out, which holds
+ * (chunk_end-chunk_begin)*chunk_rows*elem bytes; rows past n_rows are zero, like the
+ * reference's zero-padded last chunk (core/apps.cuh:185-187). GOLAP_ECORRUPT if any page fails
+ * its bounds checks. *kernel_ms (may be NULL) gets the device time. */
+int golap_decompress_chunks(golap_ctx *ctx, const golap_column *col, uint64_t chunk_begin,
+                            uint64_t chunk_end, void *dev_out, double *kernel_ms);
+
+/* ------------------------------------------------------------------ predicates
+ * PredInfo / ConstPred (core/dev_structs.cuh:21-33) generalised to an inclusive range:
+ * a row passes iff lo <= col[row] <= hi (ConstPred<C> is lo == hi == C). */
+typedef struct {
+    const void *dev_col;
+    uint32_t type;
+    int64_t lo, hi;
+} golap_pred;
+
+/* ------------------------------------------------------------------ hash join
+ * HashMap (core/dev_structs.cuh:36-132). The device table is open addressing over 8-byte
+ * slots {int32 key, int32 payload}; probe semantics match HashMap::probe for unique build keys:
+ * the rid of the build row with that key, or UINT64_MAX. ht_size is the reference's
+ * constructor argument (number of build rows the map must hold). */
+int golap_hashmap_create(golap_ctx *ctx, uint64_t ht_size, const void *dev_keys,
+                         uint32_t key_type, golap_hashmap **out);
+/* hash_map_build_pred_other (core/join.cuh:107-135) over rows [0, n_rows) of the key column:
+ * a row is inserted iff every predicate passes. dev_payload (int32 per build row, or NULL)
+ * is stored beside the key and returned by the fused plans' probes; NULL stores the rid.
+ * GOLAP_ECAPACITY if more than ht_size rows qualify. */
+int golap_hashmap_build(golap_ctx *ctx, golap_hashmap *map, uint64_t n_rows,
+                        const golap_pred *preds, uint32_t n_preds, const int32_t *dev_payload);
+/* HashMap::probe for n keys (hash_join_rids, core/join.cuh:81-97, without compaction):
+ * dev_out_rid[i] = rid or UINT64_MAX. */
+int golap_hashmap_probe(golap_ctx *ctx, const golap_hashmap *map, const void *dev_probe_keys,
+                        uint32_t key_type, uint64_t n, uint64_t *dev_out_rid);
+/* hash_join_count (core/join.cuh:45-56). */
+int golap_hash_join_count(golap_ctx *ctx, const golap_hashmap *map, const void *dev_probe_keys,
+                          uint32_t key_type, uint64_t n, uint64_t *hits);
+/* HashMap::filled (core/dev_structs.cuh:52,98). */
+int golap_hashmap_filled(golap_ctx *ctx, const golap_hashmap *map, uint64_t *filled);
+int golap_hashmap_free(golap_ctx *ctx, golap_hashmap *map);
+
+/* ------------------------------------------------------------------ zone maps
+ * check_mmmeta (core/metadata.cuh:539-547): dev_flags[c] = (lo <= max[c] && hi >= min[c]).
+ * combine_and (core/metadata.cuh:632-636): dev_a[c] = dev_a[c] && dev_b[c]. */
+int golap_minmax_check(golap_ctx *ctx, const golap_column *col, int64_t lo, int64_t hi,
+                       uint16_t *dev_flags);
+int golap_combine_and(golap_ctx *ctx, uint16_t *dev_a, const uint16_t *dev_b, uint64_t n);
+
+/* ------------------------------------------------------------------ Q1 plan
+ * query1 (usecases/ssb/queries/01.hpp:53-449) with pipeline_q1_lineorder
+ * (usecases/ssb/queries/01.cuh:26-54) fused with decompression: over chunks
+ * [chunk_begin, chunk_end) of four row-aligned columns, rows with disc, qty and price in their
+ * ranges whose date key probes into date_map contribute price*disc to an int64 sum. If
+ * dev_chunk_mask is set, chunks whose flag is 0 are skipped (skip_step, 01.hpp:368-378). */
+typedef struct {
+    const golap_column *date, *disc, *qty, *price;
+    const golap_hashmap *date_map;
+    int64_t disc_lo, disc_hi, qty_lo, qty_hi, price_lo, price_hi;
+    uint64_t chunk_begin, chunk_end;
+    const uint16_t *dev_chunk_mask;
+} golap_q1_plan;
+typedef struct {
+    int64_t sum;
+    uint64_t matches;
+    uint64_t chunks_run, chunks_pruned;
+    uint64_t comp_bytes;   /* compressed bytes of the non-pruned pages read */
+    double kernel_ms;      /* device time of the fused kernels */
+} golap_q1_result;
+int golap_q1_run(golap_ctx *ctx, const golap_q1_plan *plan, golap_q1_result *res);
+
+/* ------------------------------------------------------------------ synthetic data
+ * Seeded column generators of the benchmark workloads (the reference seeds its generator with
+ * util::uniform_int, core/util.hpp:177-181; these use a SplitMix64 counter hash so row ranges
+ * are generated independently). Date dimension: n_days consecutive days from 1992-01-01, key
+ * yyyymmdd. Q1 fact rows [row0, row0+n): date_key uniform over date_keys, discount U[0,10],
+ * quantity U[1,50], price U[90000,10500000]; any output may be NULL. threads 0 = all. */
+int golap_gen_date_dim(uint32_t n_days, int32_t *key, int16_t *year, uint8_t *month);
+int golap_gen_q1_fact(uint64_t seed, uint64_t row0, uint64_t n, const int32_t *date_keys,
+                      uint32_t n_dates, int32_t *date_key, uint8_t *discount, uint8_t *quantity,
+                      int32_t *price, uint32_t threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,145 @@
+/*
+ * CPU page decoders (TEST INFRASTRUCTURE: the oracle, never the product).
+ *
+ * The reference's CPU codec calls LZ4_decompress_safe_partial(in, out, comp, chunk, chunk)
+ * (core/comp_cpu.hpp:22-32) per chunk; the LZ4 library is an un-vendored dependency
+ * (lz4 @ce8ee024, scripts/init.sh:32). oracle_lz4_decode restates the published LZ4 block
+ * format (lz4/doc/lz4_Block_format.md): sequences of token | literal-length bytes | literals |
+ * 2-byte little-endian offset | match-length bytes, the last sequence carrying literals only.
+ * Every read and write is bounds checked, as in the library's "safe" decoder.
+ */
+#include "oracle.h"
+#include <string.h>
+
+int64_t oracle_lz4_decode(const uint8_t *src, uint64_t csize, uint8_t *dst, uint64_t cap) {
+    uint64_t ip = 0, op = 0;
+    if (csize == 0) return cap == 0 ? 0 : -1;
+    for (;;) {
+        if (ip >= csize) return -1;
+        uint32_t token = src[ip++];
+        uint64_t lit = token >> 4;
+        if (lit == 15) {
+            uint32_t b;
+            do {
+                if (ip >= csize) return -1;
+                b = src[ip++];
+                lit += b;
+            } while (b == 255);
+        }
+        if (ip + lit > csize || op + lit > cap) return -1;
+        memcpy(dst + op, src + ip, lit);
+        ip += lit;
+        op += lit;
+        if (ip == csize) break; /* last sequence: literals only */
+        if (ip + 2 > csize) return -1;
+        uint64_t off = (uint64_t)src[ip] | ((uint64_t)src[ip + 1] << 8);
+        ip += 2;
+        if (off == 0 || off > op) return -1;
+        uint64_t ml = token & 15;
+        if (ml == 15) {
+            uint32_t b;
+            do {
+                if (ip >= csize) return -1;
+                b = src[ip++];
+                ml += b;
+            } while (b == 255);
+        }
+        ml += 4; /* MINMATCH */
+        if (op + ml > cap) return -1;
+        for (uint64_t k = 0; k < ml; ++k) dst[op + k] = dst[op - off + k]; /* overlap-safe */
+        op += ml;
+    }
+    return (int64_t)op;
+}
+
+static inline uint64_t rd_le(const uint8_t *p, uint32_t n) {
+    uint64_t v = 0;
+    for (uint32_t i = 0; i < n; ++i) v |= (uint64_t)p[i] << (8 * i);
+    return v;
+}
+static inline void wr_le(uint8_t *p, uint64_t v, uint32_t n) {
+    for (uint32_t i = 0; i < n; ++i) p[i] = (uint8_t)(v >> (8 * i));
+}
+
+/* Delta-bitpack page: 32-byte header {i64 base, i64 min_delta, u32 n, u8 bw, u8 elem, 6 pad},
+ * then (n-1) packed values of bw bits (little-endian bit stream in u32 words);
+ * v[0] = base, v[i] = v[i-1] + min_delta + packed[i-1] (mod 2^64), low elem bytes stored. */
+int64_t oracle_dbp_decode(const uint8_t *src, uint64_t csize, uint8_t *dst, uint64_t cap) {
+    if (csize < 32) return -1;
+    uint64_t base = rd_le(src, 8), mind = rd_le(src + 8, 8);
+    uint32_t n = (uint32_t)rd_le(src + 16, 4), bw = src[20], elem = src[21];
+    if (bw > 32 || (elem != 1 && elem != 2 && elem != 4 && elem != 8)) return -1;
+    if ((uint64_t)n * elem > cap) return -1;
+    uint64_t words = n ? (((uint64_t)(n - 1) * bw + 31) / 32) : 0;
+    if (32 + words * 4 > csize) return -1;
+    const uint8_t *pk = src + 32;
+    uint64_t v = base;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (i > 0) {
+            uint64_t bit = (uint64_t)(i - 1) * bw, p = 0;
+            if (bw) {
+                uint64_t w = bit / 32;
+                uint64_t lo = rd_le(pk + 4 * w, 4);
+                uint64_t hi = (w + 1 < words) ? rd_le(pk + 4 * (w + 1), 4) : 0;
+                uint64_t both = lo | (hi << 32);
+                p = (both >> (bit % 32)) & ((bw == 64) ? ~0ull : ((1ull << bw) - 1));
+            }
+            v = v + mind + p;
+        }
+        wr_le(dst + (uint64_t)i * elem, v, elem);
+    }
+    return n;
+}
+
+/* RLE page: 16-byte header {u32 nruns, u32 n, u8 elem, 7 pad}, values[nruns] of elem bytes
+ * padded to 4, then u32 ends[nruns] (exclusive end row of each run, ends[nruns-1] == n). */
+int64_t oracle_rle_decode(const uint8_t *src, uint64_t csize, uint8_t *dst, uint64_t cap) {
+    if (csize < 16) return -1;
+    uint32_t nruns = (uint32_t)rd_le(src, 4), n = (uint32_t)rd_le(src + 4, 4), elem = src[8];
+    if (elem != 1 && elem != 2 && elem != 4 && elem != 8) return -1;
+    uint64_t vbytes = ((uint64_t)nruns * elem + 3) & ~3ull;
+    if (16 + vbytes + 4ull * nruns > csize || (uint64_t)n * elem > cap) return -1;
+    const uint8_t *vals = src + 16, *ends = src + 16 + vbytes;
+    uint32_t row = 0;
+    for (uint32_t k = 0; k < nruns; ++k) {
+        uint32_t e = (uint32_t)rd_le(ends + 4 * k, 4);
+        if (e < row || e > n) return -1;
+        for (; row < e; ++row) memcpy(dst + (uint64_t)row * elem, vals + (uint64_t)k * elem, elem);
+    }
+    if (row != n) return -1;
+    return n;
+}
+
+/* codec: low 4 bits are the codec id; bit 7 (literal-only LZ4) is a layout hint only. */
+int64_t oracle_page_decode(uint32_t codec, const uint8_t *src, uint64_t csize, uint8_t *dst,
+                           uint64_t cap) {
+    switch (codec & 0x0f) {
+    case 0:
+        if (csize > cap) return -1;
+        memcpy(dst, src, csize);
+        return (int64_t)csize;
+    case 1: return oracle_lz4_decode(src, csize, dst, cap);
+    case 2: return oracle_dbp_decode(src, csize, dst, cap);
+    case 3: return oracle_rle_decode(src, csize, dst, cap);
+    default: return -1;
+    }
+}
+
+int oracle_decode_column(const uint8_t *comp, const uint64_t *page_off, const uint32_t *page_csize,
+                         const uint8_t *page_codec, uint64_t n_rows, uint32_t elem,
+                         uint32_t page_rows, uint32_t chunk_rows, uint8_t *dst) {
+    if (!page_rows || !chunk_rows || chunk_rows % page_rows) return -1;
+    uint64_t n_pages = (n_rows + page_rows - 1) / page_rows;
+    uint64_t n_chunks = (n_rows + chunk_rows - 1) / chunk_rows;
+    memset(dst, 0, n_chunks * (uint64_t)chunk_rows * elem);
+    for (uint64_t p = 0; p < n_pages; ++p) {
+        uint64_t r0 = p * page_rows;
+        uint64_t rows = (n_rows - r0 < page_rows) ? n_rows - r0 : page_rows;
+        int64_t got = oracle_page_decode(page_codec[p], comp + page_off[p], page_csize[p],
+                                         dst + r0 * elem, rows * elem);
+        uint32_t cd = page_codec[p] & 0x0f;
+        uint64_t want = cd == 0 || cd == 1 ? rows * elem : rows;
+        if (got < 0 || (uint64_t)got != want) return -1;
+    }
+    return 0;
+}

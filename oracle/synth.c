@@ -1,0 +1,63 @@
+/*
+ * Seeded synthetic columns (TEST INFRASTRUCTURE: the oracle, never the product).
+ *
+ * Counter-based so that any row range can be produced independently and in parallel:
+ * r = splitmix64(seed*PHI ^ (col << 48) ^ row), value = lo + (((r >> 32) * span) >> 32), the
+ * same multiply-shift range reduction as util::uniform_int (core/util.hpp:177-181) applied to a
+ * counter hash instead of an LCG stream.
+ */
+#include "oracle.h"
+
+/* SplitMix64 finaliser (Steele, Lea, Flood, "Fast splittable pseudorandom number generators",
+ * OOPSLA 2014). */
+uint64_t oracle_splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+uint32_t oracle_uniform(uint64_t seed, uint32_t col, uint64_t row, uint32_t lo, uint32_t hi) {
+    uint64_t r = oracle_splitmix64((seed * 0x9E3779B97F4A7C15ull) ^ ((uint64_t)col << 48) ^ row);
+    uint64_t span = (uint64_t)hi - lo + 1;
+    return lo + (uint32_t)(((r >> 32) * span) >> 32);
+}
+
+void oracle_gen_q1_fact(uint64_t seed, uint64_t row0, uint64_t n, const int32_t *date_keys,
+                        uint32_t n_dates, int32_t *date_key, uint8_t *discount,
+                        uint8_t *quantity, int32_t *price) {
+    for (uint64_t i = 0; i < n; ++i) {
+        uint64_t r = row0 + i;
+        if (date_key) date_key[i] = date_keys[oracle_uniform(seed, 0, r, 0, n_dates - 1)];
+        if (discount) discount[i] = (uint8_t)oracle_uniform(seed, 1, r, 0, 10);
+        if (quantity) quantity[i] = (uint8_t)oracle_uniform(seed, 2, r, 1, 50);
+        if (price) price[i] = (int32_t)oracle_uniform(seed, 3, r, 90000, 10500000);
+    }
+}
+
+/* Civil date from a day count since 1970-01-01 (H. Hinnant, "chrono-Compatible Low-Level Date
+ * Algorithms", civil_from_days). */
+static void civil_from_days(int64_t z, int *y, unsigned *m, unsigned *d) {
+    z += 719468;
+    int64_t era = (z >= 0 ? z : z - 146096) / 146097;
+    unsigned doe = (unsigned)(z - era * 146097);
+    unsigned yoe = (doe - doe / 1460 + doe / 36524 - doe / 146096) / 365;
+    int64_t yy = (int64_t)yoe + era * 400;
+    unsigned doy = doe - (365 * yoe + yoe / 4 - yoe / 100);
+    unsigned mp = (5 * doy + 2) / 153;
+    *d = doy - (153 * mp + 2) / 5 + 1;
+    *m = mp < 10 ? mp + 3 : mp - 9;
+    *y = (int)(yy + (*m <= 2));
+}
+
+void oracle_gen_date_dim(uint32_t n_days, int32_t *key, int16_t *year, uint8_t *month) {
+    const int64_t day0 = 8035; /* 1992-01-01 */
+    for (uint32_t i = 0; i < n_days; ++i) {
+        int y;
+        unsigned m, d;
+        civil_from_days(day0 + i, &y, &m, &d);
+        if (key) key[i] = y * 10000 + (int)m * 100 + (int)d;
+        if (year) year[i] = (int16_t)y;
+        if (month) month[i] = (uint8_t)m;
+    }
+}
