@@ -15,7 +15,7 @@ def load_liblz4_fixture():
     z = np.load(os.path.join(HERE, "golden", "liblz4_pages.npz"))
     sizes = z["sizes"]
     blob = z["blob"]
-    offs = np.concatenate([[0], np.cumsum(sizes)])
+    offs = np.concatenate([[0], np.cumsum(sizes.astype(np.int64))]).astype(np.int64)
     pages = [blob[offs[i]:offs[i + 1]].tobytes() for i in range(sizes.size)]
     return z["data"], int(z["page_rows"]), pages
 
