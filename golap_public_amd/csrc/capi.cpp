@@ -441,6 +441,9 @@ int golap_decompress_chunks(golap_ctx *ctx, const golap_column *c, uint64_t cb, 
 }
 
 // ------------------------------------------------------------------------------ hash join
+// Largest key range given a membership bitmap (2^18 bits = 32 KiB, pinned in LDS by the scans).
+constexpr uint64_t HASHMAP_DENSE_MAX_KEYS = 1ull << 18;
+
 static uint32_t log2_ceil(uint64_t v) {
     uint32_t b = 0;
     while ((1ull << b) < v) ++b;
@@ -480,13 +483,35 @@ int golap_hashmap_build(golap_ctx *ctx, golap_hashmap *m, uint64_t n_rows, const
     }
     // a rebuild starts from an empty table
     GOLAP_HIP_CHECK(ctx, launch_hash_clear(ctx, m->d_slots, 1ull << m->log2cap));
+    // scratch [64,72): inserted count, [72,80): min and max inserted key
     uint64_t *d_filled = reinterpret_cast<uint64_t *>(ctx->d_scratch + 64);
+    int32_t *d_kminmax = reinterpret_cast<int32_t *>(ctx->d_scratch + 72);
+    static const int32_t init_minmax[2] = {INT32_MAX, INT32_MIN};
     GOLAP_HIP_CHECK(ctx, hipMemsetAsync(d_filled, 0, 8, ctx->stream));
-    GOLAP_HIP_CHECK(ctx, launch_hash_build(ctx, m, n_rows, pv, n_preds, dev_payload, d_filled));
-    GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(&m->filled, d_filled, 8, hipMemcpyDeviceToHost, ctx->stream));
+    GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(d_kminmax, init_minmax, 8, hipMemcpyHostToDevice, ctx->stream));
+    GOLAP_HIP_CHECK(ctx, launch_hash_build(ctx, m, n_rows, pv, n_preds, dev_payload, d_filled, d_kminmax));
+    uint64_t back[2] = {0, 0};
+    GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(back, d_filled, 16, hipMemcpyDeviceToHost, ctx->stream));
     if (int r = check_device_errors(ctx, "hashmap_build")) return r;
+    m->filled = back[0];
+    memcpy(&m->kmin, &back[1], 4);
+    memcpy(&m->kmax, reinterpret_cast<uint8_t *>(&back[1]) + 4, 4);
     if (m->filled > m->ht_size)
         return fail(ctx, GOLAP_ECAPACITY, "hashmap_build: more qualifying rows than ht_size");
+    // dense form: membership bits over the inserted key range when it is small
+    dev_release(ctx, m->d_bitmap);
+    m->d_bitmap = nullptr;
+    m->krange = 0;
+    if (m->filled && (int64_t)m->kmax - m->kmin + 1 <= (int64_t)HASHMAP_DENSE_MAX_KEYS) {
+        m->krange = (uint32_t)((int64_t)m->kmax - m->kmin + 1);
+        void *p = nullptr;
+        const uint64_t words = (m->krange + 31) / 32;
+        if (int r = dev_alloc(ctx, words * 4, &p)) return r;
+        m->d_bitmap = static_cast<uint32_t *>(p);
+        GOLAP_HIP_CHECK(ctx, hipMemsetAsync(m->d_bitmap, 0, words * 4, ctx->stream));
+        GOLAP_HIP_CHECK(ctx, launch_hash_bitmap(ctx, m));
+        if (int r = check_device_errors(ctx, "hashmap_bitmap")) return r;
+    }
     // compact copy for LDS pinning (load <= 1/2), at most 4096 slots = 32 KiB
     dev_release(ctx, m->d_small);
     m->d_small = nullptr;
@@ -541,6 +566,7 @@ int golap_hashmap_free(golap_ctx *ctx, golap_hashmap *m) {
     (void)hipStreamSynchronize(ctx->stream);
     dev_release(ctx, m->d_slots);
     dev_release(ctx, m->d_small);
+    dev_release(ctx, m->d_bitmap);
     delete m;
     return GOLAP_OK;
 }
@@ -563,7 +589,16 @@ int golap_combine_and(golap_ctx *ctx, uint16_t *a, const uint16_t *b, uint64_t n
 }
 
 // ------------------------------------------------------------------------------ Q1 plan
-static int32_t clamp32(int64_t v, int64_t lo, int64_t hi) { return (int32_t)std::min(std::max(v, lo), hi); }
+// Inclusive range [lo, hi] intersected with the value range of a column type, as lo + span in
+// unsigned arithmetic; false if nothing can pass.
+static bool range_span(int64_t lo, int64_t hi, int64_t tmin, int64_t tmax, uint32_t &ulo, uint32_t &span) {
+    lo = std::max(lo, tmin);
+    hi = std::min(hi, tmax);
+    if (lo > hi) return false;
+    ulo = (uint32_t)lo;
+    span = (uint32_t)(hi - lo);
+    return true;
+}
 
 int golap_q1_run(golap_ctx *ctx, const golap_q1_plan *plan, golap_q1_result *res) {
     if (!ctx || !plan || !res) return GOLAP_EINVAL;
@@ -601,20 +636,24 @@ int golap_q1_run(golap_ctx *ctx, const golap_q1_plan *plan, golap_q1_result *res
     a.page_end = std::min<uint64_t>(d->n_pages, plan->chunk_end * ppc);
     a.chunk_mask = plan->dev_chunk_mask;
     a.pages_per_chunk = (uint32_t)ppc;
-    a.disc_lo = clamp32(plan->disc_lo, -1, 256);
-    a.disc_hi = clamp32(plan->disc_hi, -1, 256);
-    a.qty_lo = clamp32(plan->qty_lo, -1, 256);
-    a.qty_hi = clamp32(plan->qty_hi, -1, 256);
-    a.price_lo = clamp32(plan->price_lo, INT32_MIN, INT32_MAX);
-    a.price_hi = clamp32(plan->price_hi, INT32_MIN, INT32_MAX);
-    if (m->d_small) {
+    const int64_t pmin = plan->price->type == GOLAP_U32 ? 0 : INT32_MIN;
+    bool any = range_span(plan->disc_lo, plan->disc_hi, 0, 255, a.disc_lo, a.disc_span);
+    any = range_span(plan->qty_lo, plan->qty_hi, 0, 255, a.qty_lo, a.qty_span) && any;
+    any = range_span(plan->price_lo, plan->price_hi, pmin, INT32_MAX, a.price_lo, a.price_span) && any;
+    any = any && m->filled > 0;
+    if (m->d_bitmap && (m->krange + 31) / 32 * 4 <= 32768) {
+        a.probe = 2;
+        a.bitmap = m->d_bitmap;
+        a.kmin = m->kmin;
+        a.krange = m->krange;
+    } else if (m->d_small) {
+        a.probe = 1;
         a.slots = m->d_small;
         a.log2cap = m->small_log2;
-        a.in_lds = true;
     } else {
+        a.probe = 0;
         a.slots = m->d_slots;
         a.log2cap = m->log2cap;
-        a.in_lds = false;
     }
     a.err = reinterpret_cast<uint32_t *>(ctx->d_scratch);
     a.sum = reinterpret_cast<int64_t *>(ctx->d_scratch + 8);
@@ -630,7 +669,7 @@ int golap_q1_run(golap_ctx *ctx, const golap_q1_plan *plan, golap_q1_result *res
         }
     }
     GOLAP_HIP_CHECK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    if (a.page_end > a.page_begin) {
+    if (any && a.page_end > a.page_begin) {  // an empty predicate range or map matches nothing
         GOLAP_HIP_CHECK(ctx, launch_q1(ctx, a, true));
         if (any_general) GOLAP_HIP_CHECK(ctx, launch_q1(ctx, a, false));
     }

@@ -60,6 +60,11 @@ struct golap_hashmap {
     // compact copy sized to `filled` (2^small_log2 slots), pinned in LDS by the fused plans
     uint32_t small_log2 = 0;
     uint64_t *d_small = nullptr;
+    // dense form when the inserted keys span a small range: membership bits over
+    // [kmin, kmin + krange)
+    int32_t kmin = 0, kmax = -1;
+    uint32_t krange = 0;
+    uint32_t *d_bitmap = nullptr;
     bool built = false;
 };
 
@@ -103,7 +108,8 @@ hipError_t launch_decode_pages(golap_ctx *ctx, const golap_column *col, uint64_t
 hipError_t launch_hash_clear(golap_ctx *ctx, uint64_t *slots, uint64_t n);
 hipError_t launch_hash_build(golap_ctx *ctx, golap_hashmap *m, uint64_t n_rows,
                              const PredView *preds, uint32_t n_preds, const int32_t *payload,
-                             uint64_t *d_filled);
+                             uint64_t *d_filled, int32_t *d_kminmax);
+hipError_t launch_hash_bitmap(golap_ctx *ctx, golap_hashmap *m);
 hipError_t launch_hash_compact(golap_ctx *ctx, golap_hashmap *m);
 hipError_t launch_hash_probe(golap_ctx *ctx, const golap_hashmap *m, const void *keys,
                              uint32_t key_type, uint64_t n, uint64_t *out_rid, uint64_t *hits);
@@ -118,10 +124,13 @@ struct Q1Launch {
     uint64_t page_begin, page_end;  // row tiles (page indices) to process
     const uint16_t *chunk_mask;     // chunk flags, 0 = pruned; may be null
     uint32_t pages_per_chunk;
-    int32_t disc_lo, disc_hi, qty_lo, qty_hi, price_lo, price_hi;  // clamped to the types
-    const uint64_t *slots;          // date hash table
+    uint32_t disc_lo, disc_span, qty_lo, qty_span, price_lo, price_span;  // lo <= v <= lo+span
+    int probe;                      // 0 hash in HBM, 1 hash in LDS, 2 bitmap in LDS
+    const uint64_t *slots;
     uint32_t log2cap;
-    bool in_lds;                    // copy the table into LDS per workgroup
+    const uint32_t *bitmap;
+    int32_t kmin;
+    uint32_t krange;
     int64_t *sum;
     uint64_t *count;
     uint32_t *err;

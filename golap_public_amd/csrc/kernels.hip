@@ -207,14 +207,29 @@ struct BuildArgs {
     uint64_t *slots;
     uint32_t log2cap;
     uint64_t *filled;
+    int32_t *kminmax;  // [0] min, [1] max of the inserted keys
     uint32_t *err;
 };
+
+template <typename T>
+__device__ __forceinline__ T wave_min(T v) {
+#pragma unroll
+    for (int d = WAVE / 2; d > 0; d >>= 1) v = min(v, (T)__shfl_xor(v, d, WAVE));
+    return v;
+}
+template <typename T>
+__device__ __forceinline__ T wave_max(T v) {
+#pragma unroll
+    for (int d = WAVE / 2; d > 0; d >>= 1) v = max(v, (T)__shfl_xor(v, d, WAVE));
+    return v;
+}
 
 // hash_map_build_pred_other (core/join.cuh:107-135): one thread per build row; the row is
 // inserted iff every predicate passes. Insert is one 64-bit CAS on an empty slot (the key is
 // the occupancy marker), so no thread ever waits on another.
 __global__ __launch_bounds__(BLOCK) void k_hash_build(BuildArgs a) {
     uint32_t inserted = 0;
+    int32_t kmin = INT32_MAX, kmax = INT32_MIN;
     for (uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; r < a.n;
          r += (uint64_t)gridDim.x * blockDim.x) {
         bool ok = true;
@@ -229,16 +244,54 @@ __global__ __launch_bounds__(BLOCK) void k_hash_build(BuildArgs a) {
             continue;
         }
         const uint32_t pay = a.payload ? (uint32_t)a.payload[r] : (uint32_t)r;
-        if (ht_insert(a.slots, a.log2cap, (int32_t)key, pay)) ++inserted;
-        else raise_err(a.err, DEVERR_CAPACITY);
+        if (ht_insert(a.slots, a.log2cap, (int32_t)key, pay)) {
+            ++inserted;
+            kmin = min(kmin, (int32_t)key);
+            kmax = max(kmax, (int32_t)key);
+        } else {
+            raise_err(a.err, DEVERR_CAPACITY);
+        }
     }
     const uint32_t tot = wave_sum(inserted);
-    if (__lane_id() == 0 && tot) atomicAdd(reinterpret_cast<unsigned long long *>(a.filled), (unsigned long long)tot);
+    kmin = wave_min(kmin);
+    kmax = wave_max(kmax);
+    if (__lane_id() == 0 && tot) {
+        atomicAdd(reinterpret_cast<unsigned long long *>(a.filled), (unsigned long long)tot);
+        atomicMin(&a.kminmax[0], kmin);
+        atomicMax(&a.kminmax[1], kmax);
+    }
+}
+
+// Exact membership bitmap over [kmin, kmin + krange) of the keys in a built table: the dense
+// form of the join that the fused scans pin in LDS when the key range is small.
+__global__ void k_bitmap_from_table(const uint64_t *slots, uint64_t n, uint32_t *bitmap,
+                                    int32_t kmin, uint32_t krange, uint32_t *err) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t s = slots[i];
+        if (s == 0) continue;
+        const int32_t key = (int32_t)((uint32_t)s ^ 0x80000000u);
+        const uint32_t off = (uint32_t)key - (uint32_t)kmin;
+        if (off >= krange) {
+            raise_err(err, DEVERR_CAPACITY);
+            continue;
+        }
+        atomicOr(&bitmap[off >> 5], 1u << (off & 31));
+    }
+}
+
+hipError_t launch_hash_bitmap(golap_ctx *ctx, golap_hashmap *m) {
+    const uint64_t n = 1ull << m->log2cap;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((n + BLOCK - 1) / BLOCK, 4096);
+    hipLaunchKernelGGL(k_bitmap_from_table, dim3(std::max(grid, 1u)), dim3(BLOCK), 0, ctx->stream,
+                       m->d_slots, n, m->d_bitmap, m->kmin, m->krange,
+                       reinterpret_cast<uint32_t *>(ctx->d_scratch));
+    return hipGetLastError();
 }
 
 hipError_t launch_hash_build(golap_ctx *ctx, golap_hashmap *m, uint64_t n_rows,
                              const PredView *preds, uint32_t n_preds, const int32_t *payload,
-                             uint64_t *d_filled) {
+                             uint64_t *d_filled, int32_t *d_kminmax) {
     BuildArgs a{};
     a.keys = m->d_keys;
     a.key_type = m->key_type;
@@ -249,6 +302,7 @@ hipError_t launch_hash_build(golap_ctx *ctx, golap_hashmap *m, uint64_t n_rows,
     a.slots = m->d_slots;
     a.log2cap = m->log2cap;
     a.filled = d_filled;
+    a.kminmax = d_kminmax;
     a.err = reinterpret_cast<uint32_t *>(ctx->d_scratch);
     const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((n_rows + BLOCK - 1) / BLOCK, 8192));
     hipLaunchKernelGGL(k_hash_build, dim3(grid), dim3(BLOCK), 0, ctx->stream, a);
@@ -356,117 +410,90 @@ struct Q1Args {
     uint64_t page_begin, page_end;
     const uint16_t *chunk_mask;  // indexed by page / pages_per_chunk, may be null
     uint32_t pages_per_chunk;
-    int32_t disc_lo, disc_hi, qty_lo, qty_hi, price_lo, price_hi;
-    const uint64_t *slots;
+    // predicates as lo <= v <= lo + span (unsigned arithmetic)
+    uint32_t disc_lo, disc_span, qty_lo, qty_span, price_lo, price_span;
+    const uint64_t *slots;   // PROBE_HASH_*: open-addressing table, 2^log2cap slots
     uint32_t log2cap;
+    const uint32_t *bitmap;  // PROBE_BITMAP: membership bits of keys in [kmin, kmin+krange)
+    int32_t kmin;
+    uint32_t krange;
     int64_t *sum;
     uint64_t *count;
     uint32_t *err;
 };
 
-template <bool LDS_TABLE>
-__device__ __forceinline__ void q1_row(const Q1Args &a, const uint64_t *tab, int32_t dk, uint32_t d,
+// How the date probe is answered: the open-addressing table in HBM/L2, the same table pinned
+// in LDS, or (dense key range) an exact membership bitmap pinned in LDS.
+enum : int { PROBE_HASH_GLOBAL = 0, PROBE_HASH_LDS = 1, PROBE_BITMAP = 2 };
+
+template <int PROBE>
+__device__ __forceinline__ void q1_row(const Q1Args &a, const void *tab, int32_t dk, uint32_t d,
                                        uint32_t q, int32_t pr, int64_t &sum, uint32_t &cnt) {
-    if ((int32_t)d >= a.disc_lo && (int32_t)d <= a.disc_hi && (int32_t)q >= a.qty_lo &&
-        (int32_t)q <= a.qty_hi && pr >= a.price_lo && pr <= a.price_hi) {
+    const bool pass = (d - a.disc_lo <= a.disc_span) & (q - a.qty_lo <= a.qty_span) &
+                      ((uint32_t)pr - a.price_lo <= a.price_span);
+    if (PROBE == PROBE_BITMAP) {
+        // branch-free: one LDS word per row
+        const uint32_t off = (uint32_t)dk - (uint32_t)a.kmin;
+        const bool in = off < a.krange;
+        const uint32_t w = static_cast<const uint32_t *>(tab)[in ? (off >> 5) : 0u];
+        const bool hit = pass & in & (((w >> (off & 31)) & 1u) != 0);
+        sum += (int64_t)pr * (int64_t)(hit ? d : 0u);
+        cnt += hit ? 1u : 0u;
+    } else if (pass) {
         uint32_t pay;
-        if (ht_probe(tab, a.log2cap, dk, pay)) {
+        if (ht_probe(static_cast<const uint64_t *>(tab), a.log2cap, dk, pay)) {
             sum += (int64_t)pr * (int64_t)d;
             cnt += 1;
         }
     }
 }
 
-template <bool DIRECT, bool LDS_TABLE>
-__global__ __launch_bounds__(BLOCK) void k_q1(Q1Args a) {
-    extern __shared__ __align__(16) uint8_t lds[];
-    __shared__ uint32_t tile_bad;
-    const uint64_t *tab = a.slots;
-    uint32_t tab_bytes = 0;
-    if (LDS_TABLE) {
-        const uint32_t n = 1u << a.log2cap;
-        uint64_t *t = reinterpret_cast<uint64_t *>(lds);
-        for (uint32_t i = threadIdx.x; i < n; i += BLOCK) t[i] = a.slots[i];
-        tab = t;
-        tab_bytes = n * 8;
-        __syncthreads();
-    }
-    const uint32_t P = a.page_rows;
-    uint8_t *w_date = lds + tab_bytes;
-    uint8_t *w_price = w_date + (size_t)P * 4;
-    uint8_t *w_disc = w_price + (size_t)P * 4;
-    uint8_t *w_qty = w_disc + P;
-    const uint32_t wave = threadIdx.x / WAVE;
+template <int PROBE>
+__device__ __forceinline__ void q1_group(const Q1Args &a, const void *tab, int4 dv, int4 pv,
+                                         uint32_t sv, uint32_t qv, int64_t &sum, uint32_t &cnt) {
+    q1_row<PROBE>(a, tab, dv.x, sv & 0xff, qv & 0xff, pv.x, sum, cnt);
+    q1_row<PROBE>(a, tab, dv.y, (sv >> 8) & 0xff, (qv >> 8) & 0xff, pv.y, sum, cnt);
+    q1_row<PROBE>(a, tab, dv.z, (sv >> 16) & 0xff, (qv >> 16) & 0xff, pv.z, sum, cnt);
+    q1_row<PROBE>(a, tab, dv.w, sv >> 24, qv >> 24, pv.w, sum, cnt);
+}
 
-    int64_t sum = 0;
-    uint32_t cnt = 0;
-    for (uint64_t p = a.page_begin + blockIdx.x; p < a.page_end; p += gridDim.x) {
-        if (a.chunk_mask && a.chunk_mask[p / a.pages_per_chunk] == 0) continue;
-        const uint8_t cd = a.date.page_codec[p], cs = a.disc.page_codec[p],
-                      cq = a.qty.page_codec[p], cp = a.price.page_codec[p];
-        const bool direct = page_direct(cd) && page_direct(cs) && page_direct(cq) && page_direct(cp);
-        if (direct != DIRECT) continue;
-        const uint64_t r0 = p * P;
-        const uint32_t rows = (uint32_t)min<uint64_t>(P, a.n_rows - r0);
-        const uint8_t *dp, *sp, *qp, *pp;
-        bool bad = false;
-        if (DIRECT) {
-            const uint32_t hd = direct_header(cd, rows * 4), hs = direct_header(cs, rows),
-                           hq = direct_header(cq, rows), hp = direct_header(cp, rows * 4);
-            dp = a.date.comp + a.date.page_off[p] + hd;
-            sp = a.disc.comp + a.disc.page_off[p] + hs;
-            qp = a.qty.comp + a.qty.page_off[p] + hq;
-            pp = a.price.comp + a.price.page_off[p] + hp;
-            // the catalog must say exactly "header + values", at 16-byte aligned payloads
-            if (a.date.page_csize[p] != hd + rows * 4 || a.disc.page_csize[p] != hs + rows ||
-                a.qty.page_csize[p] != hq + rows || a.price.page_csize[p] != hp + rows * 4 ||
-                ((reinterpret_cast<uintptr_t>(dp) | reinterpret_cast<uintptr_t>(sp) |
-                  reinterpret_cast<uintptr_t>(qp) | reinterpret_cast<uintptr_t>(pp)) & 15)) {
-                if (threadIdx.x == 0) raise_err(a.err, DEVERR_CORRUPT);
-                continue;  // uniform: every thread evaluates the same test
-            }
+// Rows [0, rows) of one tile whose four columns are readable at dp/sp/qp/pp (in HBM or LDS).
+template <int PROBE>
+__device__ __forceinline__ void q1_rows_generic(const Q1Args &a, const void *tab,
+                                                const uint8_t *dp, const uint8_t *sp,
+                                                const uint8_t *qp, const uint8_t *pp, uint32_t rows,
+                                                int64_t &sum, uint32_t &cnt) {
+    for (uint32_t g = threadIdx.x; g * 4 < rows; g += BLOCK) {
+        const uint32_t rr = g * 4;
+        if (rr + 4 <= rows) {
+            q1_group<PROBE>(a, tab, *reinterpret_cast<const int4 *>(dp + 16 * (size_t)g),
+                     *reinterpret_cast<const int4 *>(pp + 16 * (size_t)g),
+                     *reinterpret_cast<const uint32_t *>(sp + 4 * (size_t)g),
+                     *reinterpret_cast<const uint32_t *>(qp + 4 * (size_t)g), sum, cnt);
         } else {
-            if (threadIdx.x == 0) tile_bad = 0;
-            __syncthreads();
-            const ColView *cv = wave == 0 ? &a.date : wave == 1 ? &a.disc : wave == 2 ? &a.qty : &a.price;
-            uint8_t *win = wave == 0 ? w_date : wave == 1 ? w_disc : wave == 2 ? w_qty : w_price;
-            const uint32_t nb = rows * cv->elem;
-            // in-place pages decode as RAW/LZ4 too, so this single call covers every codec
-            const int64_t got = page_decode_wave(cv->page_codec[p], cv->comp + cv->page_off[p],
-                                                 cv->page_csize[p], win, nb, cv->elem);
-            if (got != (int64_t)nb && __lane_id() == 0) {
-                tile_bad = 1;
-                raise_err(a.err, DEVERR_CORRUPT);
-            }
-            __syncthreads();
-            bad = tile_bad != 0;
-            dp = w_date;
-            sp = w_disc;
-            qp = w_qty;
-            pp = w_price;
-        }
-        for (uint32_t g = threadIdx.x; !bad && g * 4 < rows; g += BLOCK) {
-            const uint32_t rr = g * 4;
-            if (rr + 4 <= rows) {
-                const int4 dv = *reinterpret_cast<const int4 *>(dp + 16 * (size_t)g);
-                const int4 pv = *reinterpret_cast<const int4 *>(pp + 16 * (size_t)g);
-                const uint32_t sv = *reinterpret_cast<const uint32_t *>(sp + 4 * (size_t)g);
-                const uint32_t qv = *reinterpret_cast<const uint32_t *>(qp + 4 * (size_t)g);
-                q1_row<LDS_TABLE>(a, tab, dv.x, sv & 0xff, qv & 0xff, pv.x, sum, cnt);
-                q1_row<LDS_TABLE>(a, tab, dv.y, (sv >> 8) & 0xff, (qv >> 8) & 0xff, pv.y, sum, cnt);
-                q1_row<LDS_TABLE>(a, tab, dv.z, (sv >> 16) & 0xff, (qv >> 16) & 0xff, pv.z, sum, cnt);
-                q1_row<LDS_TABLE>(a, tab, dv.w, sv >> 24, qv >> 24, pv.w, sum, cnt);
-            } else {
-                for (uint32_t r = rr; r < rows; ++r) {
-                    int32_t dk, pr;
-                    __builtin_memcpy(&dk, dp + 4 * (size_t)r, 4);
-                    __builtin_memcpy(&pr, pp + 4 * (size_t)r, 4);
-                    q1_row<LDS_TABLE>(a, tab, dk, sp[r], qp[r], pr, sum, cnt);
-                }
+            for (uint32_t r = rr; r < rows; ++r) {
+                int32_t dk, pr;
+                __builtin_memcpy(&dk, dp + 4 * (size_t)r, 4);
+                __builtin_memcpy(&pr, pp + 4 * (size_t)r, 4);
+                q1_row<PROBE>(a, tab, dk, sp[r], qp[r], pr, sum, cnt);
             }
         }
-        if (!DIRECT) __syncthreads();  // windows and tile_bad are reused by the next tile
     }
+}
+
+// Copies the probe structure into LDS (PROBE_HASH_LDS, PROBE_BITMAP); returns what to probe.
+template <int PROBE>
+__device__ __forceinline__ const void *q1_table(const Q1Args &a, uint8_t *lds) {
+    if (PROBE == PROBE_HASH_GLOBAL) return a.slots;
+    uint32_t *t = reinterpret_cast<uint32_t *>(lds);
+    const uint32_t *src = PROBE == PROBE_BITMAP ? a.bitmap : reinterpret_cast<const uint32_t *>(a.slots);
+    const uint32_t words = PROBE == PROBE_BITMAP ? (a.krange + 31) / 32 : (2u << a.log2cap);
+    for (uint32_t i = threadIdx.x; i < words; i += BLOCK) t[i] = src[i];
+    __syncthreads();
+    return t;
+}
+
+__device__ __forceinline__ void q1_flush(const Q1Args &a, int64_t sum, uint32_t cnt) {
     const int64_t ws = wave_sum(sum);
     const uint32_t wc = wave_sum(cnt);
     if (__lane_id() == 0 && wc) {
@@ -475,18 +502,151 @@ __global__ __launch_bounds__(BLOCK) void k_q1(Q1Args a) {
     }
 }
 
-template <bool DIRECT, bool LDS_TABLE>
-static hipError_t launch_q1_t(golap_ctx *ctx, const Q1Args &a, size_t lds) {
-    hipError_t e = prepare_lds(ctx, k_q1<DIRECT, LDS_TABLE>, lds);
+// Per-tile catalog entries of the four columns. Every load is issued unconditionally and the
+// decision is taken on the loaded values, so the twelve loads share one memory round trip.
+struct Q1Tile {
+    const uint8_t *dp, *sp, *qp, *pp;
+    uint32_t rows;
+    uint32_t state;  // 0 skip (pruned or not in place), 1 run, 2 corrupt catalog
+};
+
+__device__ __forceinline__ Q1Tile q1_tile(const Q1Args &a, uint64_t p) {
+    const uint32_t P = a.page_rows;
+    const uint8_t cd = a.date.page_codec[p], cs = a.disc.page_codec[p], cq = a.qty.page_codec[p],
+                  cp = a.price.page_codec[p];
+    const uint32_t zd = a.date.page_csize[p], zs = a.disc.page_csize[p], zq = a.qty.page_csize[p],
+                   zp = a.price.page_csize[p];
+    const uint64_t od = a.date.page_off[p], os = a.disc.page_off[p], oq = a.qty.page_off[p],
+                   op = a.price.page_off[p];
+    const uint32_t keep = a.chunk_mask ? (uint32_t)a.chunk_mask[p / a.pages_per_chunk] : 1u;
+    Q1Tile t;
+    t.rows = (uint32_t)min<uint64_t>(P, a.n_rows - p * P);
+    const uint32_t hd = direct_header(cd, t.rows * 4), hs = direct_header(cs, t.rows),
+                   hq = direct_header(cq, t.rows), hp = direct_header(cp, t.rows * 4);
+    t.dp = a.date.comp + od + hd;
+    t.sp = a.disc.comp + os + hs;
+    t.qp = a.qty.comp + oq + hq;
+    t.pp = a.price.comp + op + hp;
+    const uint32_t direct = (uint32_t)page_direct(cd) & (uint32_t)page_direct(cs) &
+                            (uint32_t)page_direct(cq) & (uint32_t)page_direct(cp);
+    const uint32_t sizes_ok = (uint32_t)(zd == hd + t.rows * 4) & (uint32_t)(zs == hs + t.rows) &
+                              (uint32_t)(zq == hq + t.rows) & (uint32_t)(zp == hp + t.rows * 4);
+    const uint32_t aligned = ((reinterpret_cast<uintptr_t>(t.dp) | reinterpret_cast<uintptr_t>(t.sp) |
+                               reinterpret_cast<uintptr_t>(t.qp) | reinterpret_cast<uintptr_t>(t.pp)) & 15) == 0;
+    t.state = (keep == 0 || !direct) ? 0u : ((sizes_ok & aligned) ? 1u : 2u);
+    return t;
+}
+
+// Tiles whose four pages are readable in place. G = page_rows / (4 * BLOCK) row groups of
+// four per thread: a full tile issues all 4*G loads of a thread before any row is evaluated,
+// and the catalog entries of the workgroup's next tile are loaded before this tile's data.
+template <int PROBE, int G>
+__global__ __launch_bounds__(BLOCK) void k_q1_direct(Q1Args a) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    const void *tab = q1_table<PROBE>(a, lds);
+    int64_t sum = 0;
+    uint32_t cnt = 0;
+    uint64_t p = a.page_begin + blockIdx.x;
+    Q1Tile cur{};
+    if (p < a.page_end) cur = q1_tile(a, p);
+    while (p < a.page_end) {
+        const uint64_t pn = p + gridDim.x;
+        Q1Tile nxt{};
+        if (pn < a.page_end) nxt = q1_tile(a, pn);
+        if (cur.state == 2) {
+            if (threadIdx.x == 0) raise_err(a.err, DEVERR_CORRUPT);
+        } else if (cur.state == 1) {
+            if (G > 0 && cur.rows == (uint32_t)(G * 4 * BLOCK)) {
+                int4 dv[G > 0 ? G : 1], pv[G > 0 ? G : 1];
+                uint32_t sv[G > 0 ? G : 1], qv[G > 0 ? G : 1];
+#pragma unroll
+                for (int k = 0; k < G; ++k) {
+                    const size_t g = threadIdx.x + (size_t)k * BLOCK;
+                    dv[k] = *reinterpret_cast<const int4 *>(cur.dp + 16 * g);
+                    pv[k] = *reinterpret_cast<const int4 *>(cur.pp + 16 * g);
+                    sv[k] = *reinterpret_cast<const uint32_t *>(cur.sp + 4 * g);
+                    qv[k] = *reinterpret_cast<const uint32_t *>(cur.qp + 4 * g);
+                }
+#pragma unroll
+                for (int k = 0; k < G; ++k) q1_group<PROBE>(a, tab, dv[k], pv[k], sv[k], qv[k], sum, cnt);
+            } else {
+                q1_rows_generic<PROBE>(a, tab, cur.dp, cur.sp, cur.qp, cur.pp, cur.rows, sum, cnt);
+            }
+        }
+        cur = nxt;
+        p = pn;
+    }
+    q1_flush(a, sum, cnt);
+}
+
+// Tiles with at least one coded page: wave w decodes column w's page into an LDS window, then
+// the workgroup evaluates the tile from LDS.
+template <int PROBE>
+__global__ __launch_bounds__(BLOCK) void k_q1_general(Q1Args a) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    __shared__ uint32_t tile_bad;
+    const void *tab = q1_table<PROBE>(a, lds);
+    const uint32_t tab_bytes = PROBE == PROBE_HASH_GLOBAL ? 0u
+                               : PROBE == PROBE_BITMAP ? ((a.krange + 31) / 32 * 4 + 15) & ~15u
+                                                       : (8u << a.log2cap);
+    const uint32_t P = a.page_rows;
+    uint8_t *w_date = lds + tab_bytes;
+    uint8_t *w_price = w_date + (size_t)P * 4;
+    uint8_t *w_disc = w_price + (size_t)P * 4;
+    uint8_t *w_qty = w_disc + P;
+    const uint32_t wave = threadIdx.x / WAVE;
+    int64_t sum = 0;
+    uint32_t cnt = 0;
+    for (uint64_t p = a.page_begin + blockIdx.x; p < a.page_end; p += gridDim.x) {
+        if (a.chunk_mask && a.chunk_mask[p / a.pages_per_chunk] == 0) continue;
+        const bool direct = page_direct(a.date.page_codec[p]) && page_direct(a.disc.page_codec[p]) &&
+                            page_direct(a.qty.page_codec[p]) && page_direct(a.price.page_codec[p]);
+        if (direct) continue;  // k_q1_direct's tile
+        const uint32_t rows = (uint32_t)min<uint64_t>(P, a.n_rows - p * P);
+        if (threadIdx.x == 0) tile_bad = 0;
+        __syncthreads();
+        const ColView *cv = wave == 0 ? &a.date : wave == 1 ? &a.disc : wave == 2 ? &a.qty : &a.price;
+        uint8_t *win = wave == 0 ? w_date : wave == 1 ? w_disc : wave == 2 ? w_qty : w_price;
+        const uint32_t nb = rows * cv->elem;
+        // in-place pages decode as RAW/LZ4 too, so this single call covers every codec
+        const int64_t got = page_decode_wave(cv->page_codec[p], cv->comp + cv->page_off[p],
+                                             cv->page_csize[p], win, nb, cv->elem);
+        if (got != (int64_t)nb && __lane_id() == 0) {
+            tile_bad = 1;
+            raise_err(a.err, DEVERR_CORRUPT);
+        }
+        __syncthreads();
+        if (!tile_bad) q1_rows_generic<PROBE>(a, tab, w_date, w_disc, w_qty, w_price, rows, sum, cnt);
+        __syncthreads();  // windows and tile_bad are reused by the next tile
+    }
+    q1_flush(a, sum, cnt);
+}
+
+template <typename K>
+static hipError_t launch_persistent(golap_ctx *ctx, K kernel, const Q1Args &a, size_t lds) {
+    hipError_t e = prepare_lds(ctx, kernel, lds);
     if (e != hipSuccess) return e;
     int occ = 0;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_q1<DIRECT, LDS_TABLE>, BLOCK, lds);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, BLOCK, lds);
     if (e != hipSuccess) return e;
     occ = std::max(occ, 1);
     const uint64_t tiles = a.page_end - a.page_begin;
     const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(tiles, (uint64_t)ctx->cus * occ));
-    hipLaunchKernelGGL((k_q1<DIRECT, LDS_TABLE>), dim3(grid), dim3(BLOCK), lds, ctx->stream, a);
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(BLOCK), lds, ctx->stream, a);
     return hipGetLastError();
+}
+
+template <int PROBE>
+static hipError_t launch_q1_mode(golap_ctx *ctx, const Q1Args &a, size_t tab, bool direct) {
+    if (!direct)
+        return launch_persistent(ctx, k_q1_general<PROBE>, a, tab + (size_t)a.page_rows * 10);
+    switch (a.page_rows) {
+    case 1024: return launch_persistent(ctx, k_q1_direct<PROBE, 1>, a, tab);
+    case 2048: return launch_persistent(ctx, k_q1_direct<PROBE, 2>, a, tab);
+    case 4096: return launch_persistent(ctx, k_q1_direct<PROBE, 4>, a, tab);
+    case 8192: return launch_persistent(ctx, k_q1_direct<PROBE, 8>, a, tab);
+    default: return launch_persistent(ctx, k_q1_direct<PROBE, 0>, a, tab);
+    }
 }
 
 hipError_t launch_q1(golap_ctx *ctx, const Q1Launch &l, bool direct) {
@@ -502,21 +662,27 @@ hipError_t launch_q1(golap_ctx *ctx, const Q1Launch &l, bool direct) {
     a.chunk_mask = l.chunk_mask;
     a.pages_per_chunk = l.pages_per_chunk;
     a.disc_lo = l.disc_lo;
-    a.disc_hi = l.disc_hi;
+    a.disc_span = l.disc_span;
     a.qty_lo = l.qty_lo;
-    a.qty_hi = l.qty_hi;
+    a.qty_span = l.qty_span;
     a.price_lo = l.price_lo;
-    a.price_hi = l.price_hi;
+    a.price_span = l.price_span;
     a.slots = l.slots;
     a.log2cap = l.log2cap;
+    a.bitmap = l.bitmap;
+    a.kmin = l.kmin;
+    a.krange = l.krange;
     a.sum = l.sum;
     a.count = l.count;
     a.err = l.err;
-    const size_t tab = l.in_lds ? (8ull << l.log2cap) : 0;
-    const size_t win = direct ? 0 : (size_t)l.page_rows * 10;
-    const size_t lds = tab + win;
-    if (direct) return l.in_lds ? launch_q1_t<true, true>(ctx, a, lds) : launch_q1_t<true, false>(ctx, a, lds);
-    return l.in_lds ? launch_q1_t<false, true>(ctx, a, lds) : launch_q1_t<false, false>(ctx, a, lds);
+    switch (l.probe) {
+    case PROBE_BITMAP:
+        return launch_q1_mode<PROBE_BITMAP>(ctx, a, (((size_t)l.krange + 31) / 32 * 4 + 15) & ~(size_t)15, direct);
+    case PROBE_HASH_LDS:
+        return launch_q1_mode<PROBE_HASH_LDS>(ctx, a, 8ull << l.log2cap, direct);
+    default:
+        return launch_q1_mode<PROBE_HASH_GLOBAL>(ctx, a, 0, direct);
+    }
 }
 
 }  // namespace golap_impl
