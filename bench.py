@@ -42,6 +42,8 @@ def parse():
     ap.add_argument("--seed", type=int, default=20240601)
     ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="after the timed steps, run 3 device copies of this many bytes (counter scale)")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "q1_fused_600M_traffic.json"))
     return ap.parse_args()
 
@@ -157,6 +159,14 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
     }
+    if args.calibrate_copy:
+        a_, b_ = ctx.alloc(args.calibrate_copy), ctx.alloc(args.calibrate_copy)
+        ctx.memset(a_, 1, args.calibrate_copy)
+        for _ in range(3):
+            ctx.copy(b_, a_, args.calibrate_copy)
+        ctx.free(a_)
+        ctx.free(b_)
+        log(f"calibration: 3 copies of {args.calibrate_copy} bytes")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args, hosts, key, year, r.sum)
     print(json.dumps(out), flush=True)

@@ -69,6 +69,7 @@ _PROTOS = {
     "golap_h2d": (C.c_int, [vp, vp, vp, u64]),
     "golap_d2h": (C.c_int, [vp, vp, vp, u64]),
     "golap_memset": (C.c_int, [vp, vp, C.c_int, u64]),
+    "golap_d2d": (C.c_int, [vp, vp, vp, u64]),
     "golap_device_allocated": (u64, [vp]),
     "golap_encode_opts_default": (None, [C.POINTER(EncodeOpts)]),
     "golap_encode": (C.c_int, [vp, u64, u32, C.POINTER(EncodeOpts), C.POINTER(vp)]),
@@ -251,6 +252,9 @@ class Context:
 
     def memset(self, p, value, nbytes):
         self.check(self.lib.golap_memset(self.h, p, value, nbytes), "memset")
+
+    def copy(self, dst, src, nbytes):
+        self.check(self.lib.golap_d2d(self.h, dst, src, nbytes), "d2d")
 
     def sync(self):
         self.check(self.lib.golap_sync(self.h), "sync")

@@ -197,6 +197,15 @@ int golap_memset(golap_ctx *ctx, void *dst, int value, uint64_t bytes) {
     GOLAP_HIP_CHECK(ctx, hipMemsetAsync(dst, value, bytes, ctx->stream));
     return GOLAP_OK;
 }
+int golap_d2d(golap_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
+    if (!ctx || (bytes && (!dst || !src))) return GOLAP_EINVAL;
+    Lock l(ctx->mu);
+    if (!bytes) return GOLAP_OK;
+    if (int r = set_device(ctx)) return r;
+    GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return GOLAP_OK;
+}
 uint64_t golap_device_allocated(golap_ctx *ctx) { return ctx ? ctx->allocated : 0; }
 
 // ------------------------------------------------------------------------------ write path
@@ -498,33 +507,40 @@ int golap_hashmap_build(golap_ctx *ctx, golap_hashmap *m, uint64_t n_rows, const
     memcpy(&m->kmax, reinterpret_cast<uint8_t *>(&back[1]) + 4, 4);
     if (m->filled > m->ht_size)
         return fail(ctx, GOLAP_ECAPACITY, "hashmap_build: more qualifying rows than ht_size");
-    // dense form: membership bits over the inserted key range when it is small
-    dev_release(ctx, m->d_bitmap);
-    m->d_bitmap = nullptr;
+    // dense form: membership bits over the inserted key range when it is small. Buffers are
+    // kept across rebuilds and grown only when a larger one is needed.
     m->krange = 0;
+    m->use_bitmap = false;
     if (m->filled && (int64_t)m->kmax - m->kmin + 1 <= (int64_t)HASHMAP_DENSE_MAX_KEYS) {
         m->krange = (uint32_t)((int64_t)m->kmax - m->kmin + 1);
-        void *p = nullptr;
         const uint64_t words = (m->krange + 31) / 32;
-        if (int r = dev_alloc(ctx, words * 4, &p)) return r;
-        m->d_bitmap = static_cast<uint32_t *>(p);
+        if (words > m->bitmap_words) {
+            dev_release(ctx, m->d_bitmap);
+            m->d_bitmap = nullptr;
+            m->bitmap_words = 0;
+            void *p = nullptr;
+            if (int r = dev_alloc(ctx, words * 4, &p)) return r;
+            m->d_bitmap = static_cast<uint32_t *>(p);
+            m->bitmap_words = words;
+        }
         GOLAP_HIP_CHECK(ctx, hipMemsetAsync(m->d_bitmap, 0, words * 4, ctx->stream));
+        // no sync: the range is exact by construction; a raised flag surfaces at the next check
         GOLAP_HIP_CHECK(ctx, launch_hash_bitmap(ctx, m));
-        if (int r = check_device_errors(ctx, "hashmap_bitmap")) return r;
+        m->use_bitmap = true;
     }
     // compact copy for LDS pinning (load <= 1/2), at most 4096 slots = 32 KiB
-    dev_release(ctx, m->d_small);
-    m->d_small = nullptr;
     m->small_log2 = 0;
     const uint32_t sl = std::max<uint32_t>(6, log2_ceil(std::max<uint64_t>(1, m->filled) * 2));
-    if (sl <= 12 && sl < m->log2cap) {
-        void *p = nullptr;
-        if (int r = dev_alloc(ctx, 8ull << sl, &p)) return r;
-        m->d_small = static_cast<uint64_t *>(p);
+    if (!m->use_bitmap && sl <= 12 && sl < m->log2cap) {
+        if (!m->d_small) {
+            void *p = nullptr;
+            if (int r = dev_alloc(ctx, 8ull << 12, &p)) return r;
+            m->d_small = static_cast<uint64_t *>(p);
+        }
         m->small_log2 = sl;
         GOLAP_HIP_CHECK(ctx, launch_hash_clear(ctx, m->d_small, 1ull << sl));
+        // no sync: sized for twice the inserted keys, it cannot fill up
         GOLAP_HIP_CHECK(ctx, launch_hash_compact(ctx, m));
-        if (int r = check_device_errors(ctx, "hashmap_compact")) return r;
     }
     m->built = true;
     return GOLAP_OK;
@@ -641,12 +657,12 @@ int golap_q1_run(golap_ctx *ctx, const golap_q1_plan *plan, golap_q1_result *res
     any = range_span(plan->qty_lo, plan->qty_hi, 0, 255, a.qty_lo, a.qty_span) && any;
     any = range_span(plan->price_lo, plan->price_hi, pmin, INT32_MAX, a.price_lo, a.price_span) && any;
     any = any && m->filled > 0;
-    if (m->d_bitmap && (m->krange + 31) / 32 * 4 <= 32768) {
+    if (m->use_bitmap && (m->krange + 31) / 32 * 4 <= 32768) {
         a.probe = 2;
         a.bitmap = m->d_bitmap;
         a.kmin = m->kmin;
         a.krange = m->krange;
-    } else if (m->d_small) {
+    } else if (m->small_log2) {
         a.probe = 1;
         a.slots = m->d_small;
         a.log2cap = m->small_log2;

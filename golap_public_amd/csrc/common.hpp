@@ -57,14 +57,17 @@ struct golap_hashmap {
     uint32_t log2cap = 0;          // main table: 2^log2cap slots
     uint64_t *d_slots = nullptr;
     uint64_t filled = 0;           // rows inserted by the last build
-    // compact copy sized to `filled` (2^small_log2 slots), pinned in LDS by the fused plans
+    // compact copy sized to `filled` (2^small_log2 slots, 0 = none), pinned in LDS by the fused
+    // plans; its buffer holds 4096 slots
     uint32_t small_log2 = 0;
     uint64_t *d_small = nullptr;
     // dense form when the inserted keys span a small range: membership bits over
     // [kmin, kmin + krange)
     int32_t kmin = 0, kmax = -1;
     uint32_t krange = 0;
+    bool use_bitmap = false;
     uint32_t *d_bitmap = nullptr;
+    uint64_t bitmap_words = 0;  // allocated size of d_bitmap
     bool built = false;
 };
 

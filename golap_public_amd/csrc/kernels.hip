@@ -502,6 +502,21 @@ __device__ __forceinline__ void q1_flush(const Q1Args &a, int64_t sum, uint32_t 
     }
 }
 
+// XCD-aware tile order. Workgroups are dispatched round-robin over the XCDs (workgroup i runs
+// on XCD i mod XCDS), and each XCD has its own L2. Giving each XCD one contiguous range of pages
+// makes neighbouring pages, which share the cache lines at their boundaries, meet in the same L2.
+constexpr uint32_t XCDS = 8;
+struct XcdRange {
+    uint64_t first, end, stride;
+};
+__device__ __forceinline__ XcdRange xcd_range(uint64_t begin, uint64_t end) {
+    if (gridDim.x % XCDS != 0) return XcdRange{begin + blockIdx.x, end, gridDim.x};
+    const uint64_t n = end - begin;
+    const uint32_t x = blockIdx.x % XCDS;
+    return XcdRange{begin + n * x / XCDS + blockIdx.x / XCDS, begin + n * (x + 1) / XCDS,
+                    gridDim.x / XCDS};
+}
+
 // Per-tile catalog entries of the four columns. Every load is issued unconditionally and the
 // decision is taken on the loaded values, so the twelve loads share one memory round trip.
 struct Q1Tile {
@@ -546,13 +561,14 @@ __global__ __launch_bounds__(BLOCK) void k_q1_direct(Q1Args a) {
     const void *tab = q1_table<PROBE>(a, lds);
     int64_t sum = 0;
     uint32_t cnt = 0;
-    uint64_t p = a.page_begin + blockIdx.x;
+    const XcdRange xr = xcd_range(a.page_begin, a.page_end);
+    uint64_t p = xr.first;
     Q1Tile cur{};
-    if (p < a.page_end) cur = q1_tile(a, p);
-    while (p < a.page_end) {
-        const uint64_t pn = p + gridDim.x;
+    if (p < xr.end) cur = q1_tile(a, p);
+    while (p < xr.end) {
+        const uint64_t pn = p + xr.stride;
         Q1Tile nxt{};
-        if (pn < a.page_end) nxt = q1_tile(a, pn);
+        if (pn < xr.end) nxt = q1_tile(a, pn);
         if (cur.state == 2) {
             if (threadIdx.x == 0) raise_err(a.err, DEVERR_CORRUPT);
         } else if (cur.state == 1) {
@@ -631,8 +647,9 @@ static hipError_t launch_persistent(golap_ctx *ctx, K kernel, const Q1Args &a, s
     if (e != hipSuccess) return e;
     occ = std::max(occ, 1);
     const uint64_t tiles = a.page_end - a.page_begin;
-    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(tiles, (uint64_t)ctx->cus * occ));
-    hipLaunchKernelGGL(kernel, dim3(grid), dim3(BLOCK), lds, ctx->stream, a);
+    uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>(tiles, (uint64_t)ctx->cus * occ));
+    if (grid >= XCDS) grid -= grid % XCDS;  // whole rounds over the XCDs (see xcd_range)
+    hipLaunchKernelGGL(kernel, dim3((uint32_t)grid), dim3(BLOCK), lds, ctx->stream, a);
     return hipGetLastError();
 }
 

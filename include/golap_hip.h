@@ -72,6 +72,8 @@ int golap_dev_free(golap_ctx *ctx, void *dev_ptr);
 int golap_h2d(golap_ctx *ctx, void *dev_dst, const void *src, uint64_t bytes);
 int golap_d2h(golap_ctx *ctx, void *dst, const void *dev_src, uint64_t bytes);
 int golap_memset(golap_ctx *ctx, void *dev_dst, int value, uint64_t bytes);
+/* DataCopy (core/access.hpp:16-67): device-to-device copy on the context's stream. */
+int golap_d2d(golap_ctx *ctx, void *dev_dst, const void *dev_src, uint64_t bytes);
 uint64_t golap_device_allocated(golap_ctx *ctx);
 
 /* ------------------------------------------------------------------ write path
