@@ -51,6 +51,28 @@ class Q1Plan(C.Structure):
                 ("dev_chunk_mask", vp)]
 
 
+class GroupKey(C.Structure):
+    _fields_ = [("dev_col", vp), ("type", u32), ("lo", C.c_int32), ("card", u32)]
+
+
+class StarDim(C.Structure):
+    _fields_ = [("fk", vp), ("map", vp), ("group_lo", C.c_int32), ("group_card", u32)]
+
+
+class StarPlan(C.Structure):
+    _fields_ = [("dims", StarDim * 4), ("n_dims", u32), ("value", vp), ("chunk_begin", u64),
+                ("chunk_end", u64), ("dev_chunk_mask", vp), ("agg_mode", u32), ("lds_group_slots", u32)]
+
+
+class StarResult(C.Structure):
+    _fields_ = [("n_groups", u64), ("rows_matched", u64), ("total_sum", i64), ("chunks_run", u64),
+                ("chunks_pruned", u64), ("comp_bytes", u64), ("kernel_ms", C.c_double),
+                ("driver_dim", u32), ("lds_group_slots", u32)]
+
+
+AGG_AUTO, AGG_LDS, AGG_GLOBAL = 0, 1, 2
+
+
 class Q1Result(C.Structure):
     _fields_ = [("sum", i64), ("matches", u64), ("chunks_run", u64), ("chunks_pruned", u64),
                 ("comp_bytes", u64), ("kernel_ms", C.c_double)]
@@ -92,6 +114,11 @@ _PROTOS = {
     "golap_minmax_check": (C.c_int, [vp, vp, i64, i64, vp]),
     "golap_combine_and": (C.c_int, [vp, vp, vp, u64]),
     "golap_q1_run": (C.c_int, [vp, C.POINTER(Q1Plan), C.POINTER(Q1Result)]),
+    "golap_group_by": (C.c_int, [vp, C.POINTER(GroupKey), u32, vp, u32, u64, u32, C.POINTER(u64), vp, vp, vp,
+                                 u64]),
+    "golap_star_run": (C.c_int, [vp, C.POINTER(StarPlan), C.POINTER(StarResult), vp, vp, vp, u64]),
+    "golap_gen_star_fact": (C.c_int, [u64, u64, u64, u64, vp, u32, u32, u32, u32, vp, vp, vp, vp, vp, u32]),
+    "golap_gen_uniform_i32": (C.c_int, [u64, u32, u64, C.c_int32, C.c_int32, vp, u32]),
     "golap_gen_date_dim": (C.c_int, [u32, vp, vp, vp]),
     "golap_gen_q1_fact": (C.c_int, [u64, u64, u64, vp, u32, vp, vp, vp, vp, u32]),
 }
@@ -333,6 +360,40 @@ class Context:
     def combine_and(self, dev_a, dev_b, n):
         self.check(self.lib.golap_combine_and(self.h, dev_a, dev_b, n), "combine_and")
 
+    # group-by
+    def group_by(self, keys, dev_val, val_type, n_rows, agg_mode=AGG_AUTO, max_groups=1 << 20):
+        """keys: list of (dev_col, type, lo, card). Returns (keys[n, k], sums, counts)."""
+        arr = (GroupKey * len(keys))(*[GroupKey(*k) for k in keys])
+        ng = u64()
+        kout = np.zeros(max_groups * len(keys), np.int32)
+        sums = np.zeros(max_groups, np.int64)
+        cnts = np.zeros(max_groups, np.uint64)
+        self.check(self.lib.golap_group_by(self.h, arr, len(keys), dev_val, val_type, n_rows, agg_mode,
+                                           C.byref(ng), _ptr(kout), _ptr(sums), _ptr(cnts), max_groups), "group_by")
+        n = ng.value
+        return kout[:n * len(keys)].reshape(n, len(keys)), sums[:n], cnts[:n]
+
+    def star_run(self, dims, value, chunk_begin=0, chunk_end=None, dev_chunk_mask=None, agg_mode=AGG_AUTO,
+                 lds_group_slots=0, max_groups=1 << 20):
+        """dims: list of (fk column, map, group_lo, group_card). Returns (result, keys, sums, counts)."""
+        if chunk_end is None:
+            chunk_end = self.column_desc(value).n_chunks
+        p = StarPlan()
+        for i, d in enumerate(dims):
+            p.dims[i] = StarDim(*d)
+        p.n_dims = len(dims)
+        p.value, p.chunk_begin, p.chunk_end = value, chunk_begin, chunk_end
+        p.dev_chunk_mask, p.agg_mode, p.lds_group_slots = dev_chunk_mask, agg_mode, lds_group_slots
+        na = sum(1 for d in dims if d[3])
+        kout = np.zeros(max_groups * max(na, 1), np.int32)
+        sums = np.zeros(max_groups, np.int64)
+        cnts = np.zeros(max_groups, np.uint64)
+        r = StarResult()
+        self.check(self.lib.golap_star_run(self.h, C.byref(p), C.byref(r), _ptr(kout), _ptr(sums), _ptr(cnts),
+                                           max_groups), "star_run")
+        n = r.n_groups
+        return r, kout[:n * na].reshape(n, na), sums[:n], cnts[:n]
+
     # plans
     def q1_run(self, date, disc, qty, price, date_map, disc_rng, qty_rng, price_rng=(-2**31, 2**31 - 1),
                chunk_begin=0, chunk_end=None, dev_chunk_mask=None):
@@ -362,6 +423,25 @@ def gen_q1_fact(seed, n, date_keys, row0=0, threads=0):
     if st != OK:
         raise GolapError(st, "gen_q1_fact")
     return cols
+
+
+def gen_star_fact(seed, n, n_total, date_keys, n_cust, n_supp, n_part, row0=0, threads=0):
+    """Star fact columns (date_key sorted, cust, supp, part keys, revenue) for rows [row0, row0+n)."""
+    date_keys = np.ascontiguousarray(date_keys, dtype=np.int32)
+    cols = tuple(np.empty(n, np.int32) for _ in range(5))
+    st = load().golap_gen_star_fact(seed, row0, n, n_total, _ptr(date_keys), date_keys.size, n_cust, n_supp,
+                                    n_part, *[_ptr(c) for c in cols], threads)
+    if st != OK:
+        raise GolapError(st, "gen_star_fact")
+    return cols
+
+
+def gen_uniform_i32(seed, col, n, lo, hi, threads=0):
+    out = np.empty(n, np.int32)
+    st = load().golap_gen_uniform_i32(seed, col, n, lo, hi, _ptr(out), threads)
+    if st != OK:
+        raise GolapError(st, "gen_uniform_i32")
+    return out
 
 
 def header_symbols():

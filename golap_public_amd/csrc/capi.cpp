@@ -10,15 +10,10 @@
 #include "format.hpp"
 
 using namespace golap_impl;
-
-namespace {
-
 using Lock = std::lock_guard<std::recursive_mutex>;
 
-struct ColumnExtra {
-    std::vector<uint32_t> nondirect_prefix;  // pages that cannot be read in place, prefix count
-    std::vector<uint64_t> chunk_payload;     // payload bytes per chunk, prefix
-};
+namespace golap_impl {
+
 std::mutex g_extra_mu;
 std::unordered_map<const golap_column *, ColumnExtra> g_extra;
 
@@ -61,16 +56,21 @@ int check_device_errors(golap_ctx *ctx, const char *what) {
     uint32_t flags = 0;
     GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(&flags, ctx->d_scratch, 4, hipMemcpyDeviceToHost, ctx->stream));
     GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return error_flags_status(ctx, flags, what);
+}
+
+int error_flags_status(golap_ctx *ctx, uint32_t flags, const char *what) {
     if (!flags) return GOLAP_OK;
     GOLAP_HIP_CHECK(ctx, hipMemsetAsync(ctx->d_scratch, 0, 4, ctx->stream));
+    GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     if (flags & DEVERR_CORRUPT) return fail(ctx, GOLAP_ECORRUPT, std::string(what) + ": corrupt page");
     if (flags & DEVERR_CAPACITY) return fail(ctx, GOLAP_ECAPACITY, std::string(what) + ": table full");
-    return fail(ctx, GOLAP_EINVAL, std::string(what) + ": key outside (INT32_MIN, INT32_MAX]");
+    return fail(ctx, GOLAP_EINVAL, std::string(what) + ": value outside its declared range");
 }
 
 bool valid_type(uint32_t t) { return t >= GOLAP_I8 && t <= GOLAP_I64; }
 
-}  // namespace
+}  // namespace golap_impl
 
 extern "C" {
 
@@ -528,20 +528,25 @@ int golap_hashmap_build(golap_ctx *ctx, golap_hashmap *m, uint64_t n_rows, const
         GOLAP_HIP_CHECK(ctx, launch_hash_bitmap(ctx, m));
         m->use_bitmap = true;
     }
-    // compact copy for LDS pinning (load <= 1/2), at most 4096 slots = 32 KiB
+    // compact copy (load <= 1/2): the table the scans probe
     m->small_log2 = 0;
     const uint32_t sl = std::max<uint32_t>(6, log2_ceil(std::max<uint64_t>(1, m->filled) * 2));
-    if (!m->use_bitmap && sl <= 12 && sl < m->log2cap) {
-        if (!m->d_small) {
+    if (sl < m->log2cap) {
+        if (sl > m->small_cap_log2) {
+            dev_release(ctx, m->d_small);
+            m->d_small = nullptr;
+            m->small_cap_log2 = 0;
             void *p = nullptr;
-            if (int r = dev_alloc(ctx, 8ull << 12, &p)) return r;
+            if (int r = dev_alloc(ctx, 8ull << sl, &p)) return r;
             m->d_small = static_cast<uint64_t *>(p);
+            m->small_cap_log2 = sl;
         }
         m->small_log2 = sl;
         GOLAP_HIP_CHECK(ctx, launch_hash_clear(ctx, m->d_small, 1ull << sl));
         // no sync: sized for twice the inserted keys, it cannot fill up
         GOLAP_HIP_CHECK(ctx, launch_hash_compact(ctx, m));
     }
+    m->build_id++;
     m->built = true;
     return GOLAP_OK;
 }
@@ -583,6 +588,7 @@ int golap_hashmap_free(golap_ctx *ctx, golap_hashmap *m) {
     dev_release(ctx, m->d_slots);
     dev_release(ctx, m->d_small);
     dev_release(ctx, m->d_bitmap);
+    dev_release(ctx, m->d_hfilter);
     delete m;
     return GOLAP_OK;
 }
@@ -662,7 +668,7 @@ int golap_q1_run(golap_ctx *ctx, const golap_q1_plan *plan, golap_q1_result *res
         a.bitmap = m->d_bitmap;
         a.kmin = m->kmin;
         a.krange = m->krange;
-    } else if (m->small_log2) {
+    } else if (m->small_log2 && m->small_log2 <= 12) {
         a.probe = 1;
         a.slots = m->d_small;
         a.log2cap = m->small_log2;
@@ -703,11 +709,7 @@ int golap_q1_run(golap_ctx *ctx, const golap_q1_plan *plan, golap_q1_result *res
     GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     float ms = 0;
     GOLAP_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-    if (host[0]) {
-        GOLAP_HIP_CHECK(ctx, hipMemsetAsync(ctx->d_scratch, 0, 4, ctx->stream));
-        GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-        return fail(ctx, GOLAP_ECORRUPT, "q1: corrupt page");
-    }
+    if (host[0]) return error_flags_status(ctx, (uint32_t)host[0], "q1");
     res->sum = (int64_t)host[1];
     res->matches = host[2];
     res->kernel_ms = ms;

@@ -28,6 +28,9 @@ struct golap_ctx {
     static constexpr uint64_t SCRATCH_BYTES = 4096;
     uint64_t allocated = 0;
     std::unordered_map<void *, uint64_t> allocs;
+    // dense group-by result buffers (sum int64, count uint64), grown on demand
+    void *d_groups = nullptr;
+    uint64_t groups_cap = 0;
 };
 
 struct golap_column {
@@ -57,10 +60,16 @@ struct golap_hashmap {
     uint32_t log2cap = 0;          // main table: 2^log2cap slots
     uint64_t *d_slots = nullptr;
     uint64_t filled = 0;           // rows inserted by the last build
-    // compact copy sized to `filled` (2^small_log2 slots, 0 = none), pinned in LDS by the fused
-    // plans; its buffer holds 4096 slots
+    // compact copy sized to twice `filled` (2^small_log2 slots, 0 = none): probed by the star
+    // plans, pinned in LDS by the Q1 plan when it has at most 4096 slots
     uint32_t small_log2 = 0;
+    uint32_t small_cap_log2 = 0;  // allocated size of d_small
     uint64_t *d_small = nullptr;
+    // one-bit-per-key hashed filter (2^HASH_FILTER_LOG2 bits), built on demand when the key
+    // range is too wide for the dense bitmap; a cache, valid for the build it was made from
+    mutable uint32_t *d_hfilter = nullptr;
+    mutable uint64_t hfilter_build = 0;
+    uint64_t build_id = 0;
     // dense form when the inserted keys span a small range: membership bits over
     // [kmin, kmin + krange)
     int32_t kmin = 0, kmax = -1;
@@ -86,6 +95,21 @@ inline int fail(golap_ctx *ctx, int code, const std::string &msg) {
                                     std::string(#expr ": ") + hipGetErrorString(e__));      \
     } while (0)
 
+// Runtime helpers shared by the C ABI translation units (capi.cpp).
+struct ColumnExtra {
+    std::vector<uint32_t> nondirect_prefix;  // pages that cannot be read in place, prefix count
+    std::vector<uint64_t> chunk_payload;     // payload bytes per chunk, prefix
+};
+extern std::mutex g_extra_mu;
+extern std::unordered_map<const golap_column *, ColumnExtra> g_extra;
+bool page_direct_host(uint8_t codec);
+int dev_alloc(golap_ctx *ctx, uint64_t bytes, void **p);
+void dev_release(golap_ctx *ctx, void *p);
+int set_device(golap_ctx *ctx);
+int check_device_errors(golap_ctx *ctx, const char *what);
+int error_flags_status(golap_ctx *ctx, uint32_t flags, const char *what);
+bool valid_type(uint32_t t);
+
 // Device views passed to kernels by value.
 struct ColView {
     const uint8_t *comp;
@@ -97,6 +121,64 @@ struct ColView {
 inline ColView view(const golap_column *c) {
     return ColView{c->d_comp, c->d_page_off, c->d_page_csize, c->d_page_codec, c->elem};
 }
+
+constexpr uint32_t STAR_MAX_DIMS = 4;
+constexpr uint32_t HASH_FILTER_LOG2 = 18;  // 32 KiB of LDS
+
+// One dimension of a star plan as the kernel sees it: exact table and group-key contribution.
+struct StarDimProbe {
+    const uint64_t *slots;
+    uint32_t log2cap;
+    int32_t group_lo;
+    uint32_t group_card;    // 0: the dimension only filters
+    uint64_t group_stride;
+};
+
+struct StarLaunch {
+    ColView fk[STAR_MAX_DIMS];
+    int fk_window[STAR_MAX_DIMS];
+    ColView val;
+    int val_window;
+    uint32_t val_signed;
+    StarDimProbe dim[STAR_MAX_DIMS];
+    uint32_t n_dims, driver;
+    const uint32_t *filter;
+    uint32_t filter_words;
+    int32_t fkmin;
+    uint32_t fkrange, fhash_log2;
+    uint32_t page_rows;
+    uint64_t n_rows, page_begin, page_end;
+    const uint16_t *chunk_mask;
+    uint32_t pages_per_chunk;
+    uint32_t n_windows;
+    uint64_t n_groups;
+    uint32_t lds_gslots;
+    int64_t *g_sum;
+    uint64_t *g_cnt;
+    uint64_t *rows_matched;
+    uint32_t *err;
+};
+hipError_t launch_star(golap_ctx *ctx, const StarLaunch &l);
+size_t star_lds_bytes(uint32_t filter_words, uint32_t page_rows, uint32_t n_windows, uint32_t lds_gslots);
+hipError_t launch_hash_filter(golap_ctx *ctx, const uint64_t *slots, uint32_t log2cap, uint32_t *filter,
+                              uint32_t log2bits);
+
+struct GroupByLaunch {
+    const void *key[4];
+    uint32_t key_type[4];
+    int32_t key_lo[4];
+    uint32_t key_card[4];
+    uint64_t key_stride[4];
+    uint32_t n_keys;
+    const void *val;
+    uint32_t val_type;
+    uint64_t n;
+    uint32_t lds_gslots;
+    int64_t *g_sum;
+    uint64_t *g_cnt;
+    uint32_t *err;
+};
+hipError_t launch_group_by(golap_ctx *ctx, const GroupByLaunch &l);
 
 struct PredView {
     const void *col;

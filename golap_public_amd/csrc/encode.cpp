@@ -205,8 +205,9 @@ void code_page(const uint8_t *src, uint64_t rows, uint32_t type, uint32_t elem, 
     if (codec == LZ4 || codec == GOLAP_CODEC_AUTO) {
         cand.resize(lz4_bound(nbytes));
         uint64_t sz = lz4_encode_block(src, nbytes, cand.data(), min_match, ht, base);
+        // a block with a match can have exactly the literal-only size: parse, do not compare sizes
         uint8_t c = LZ4;
-        if (sz == lz4_literal_header_bytes((uint32_t)nbytes) + nbytes) c |= PAGE_LITERAL_ONLY;
+        if (lz4_is_literal_only(cand.data(), sz, nbytes)) c |= PAGE_LITERAL_ONLY;
         consider(sz, c);
     }
     if (codec == DBP || codec == GOLAP_CODEC_AUTO) {
@@ -220,8 +221,9 @@ void code_page(const uint8_t *src, uint64_t rows, uint32_t type, uint32_t elem, 
     if (best == UINT64_MAX) {  // DBP requested but the page does not fit 32-bit deltas
         cand.resize(lz4_bound(nbytes));
         uint64_t sz = lz4_encode_block(src, nbytes, cand.data(), min_match, ht, base);
+        // a block with a match can have exactly the literal-only size: parse, do not compare sizes
         uint8_t c = LZ4;
-        if (sz == lz4_literal_header_bytes((uint32_t)nbytes) + nbytes) c |= PAGE_LITERAL_ONLY;
+        if (lz4_is_literal_only(cand.data(), sz, nbytes)) c |= PAGE_LITERAL_ONLY;
         consider(sz, c);
     }
     emit(keep.data(), best, best_codec);

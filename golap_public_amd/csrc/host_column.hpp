@@ -49,6 +49,8 @@ namespace golap_host {
 void layout_pages(golap_host_column &col, const std::vector<const uint8_t *> &pages);
 // Fills page_off / payload_bytes from page_csize / page_codec; returns the buffer size.
 uint64_t compute_page_offsets(golap_host_column &col);
-// Is this LZ4 block exactly one literal run of n bytes?
+// Is this LZ4 block exactly one literal run of n bytes (a single sequence whose token and
+// length bytes announce n literals)? Size alone does not tell: a block with one short match
+// can have the same size.
 bool lz4_is_literal_only(const uint8_t *blk, uint64_t csize, uint64_t n);
 }  // namespace golap_host

@@ -83,3 +83,41 @@ int golap_gen_q1_fact(uint64_t seed, uint64_t row0, uint64_t n, const int32_t *d
 }
 
 }  // extern "C"
+
+extern "C" {
+
+// Star schema fact rows [row0, row0+n) of n_total: date_key sorted (row r falls on day
+// floor(r * n_dates / n_total)), cust/supp/part keys uniform over the dimension sizes, revenue
+// U[100, 5000000]. Any output may be NULL.
+int golap_gen_star_fact(uint64_t seed, uint64_t row0, uint64_t n, uint64_t n_total,
+                        const int32_t *date_keys, uint32_t n_dates, uint32_t n_cust,
+                        uint32_t n_supp, uint32_t n_part, int32_t *date_key, int32_t *cust_key,
+                        int32_t *supp_key, int32_t *part_key, int32_t *revenue, uint32_t threads) {
+    if ((date_key && (!date_keys || !n_dates || !n_total)) || (cust_key && !n_cust) ||
+        (supp_key && !n_supp) || (part_key && !n_part))
+        return GOLAP_EINVAL;
+    parallel_rows(n, threads, [&](uint64_t b, uint64_t e) {
+        for (uint64_t i = b; i < e; ++i) {
+            const uint64_t r = row0 + i;
+            if (date_key) date_key[i] = date_keys[(unsigned __int128)r * n_dates / n_total];
+            if (cust_key) cust_key[i] = (int32_t)uniform(seed, 10, r, 0, n_cust - 1);
+            if (supp_key) supp_key[i] = (int32_t)uniform(seed, 11, r, 0, n_supp - 1);
+            if (part_key) part_key[i] = (int32_t)uniform(seed, 12, r, 0, n_part - 1);
+            if (revenue) revenue[i] = (int32_t)uniform(seed, 13, r, 100, 5000000);
+        }
+    });
+    return GOLAP_OK;
+}
+
+// A dimension attribute uniform over [lo, hi] for rows [0, n) (column id `col`).
+int golap_gen_uniform_i32(uint64_t seed, uint32_t col, uint64_t n, int32_t lo, int32_t hi,
+                          int32_t *out, uint32_t threads) {
+    if (!out || hi < lo) return GOLAP_EINVAL;
+    parallel_rows(n, threads, [&](uint64_t b, uint64_t e) {
+        for (uint64_t i = b; i < e; ++i)
+            out[i] = lo + (int32_t)uniform(seed, col, i, 0, (uint32_t)((int64_t)hi - lo));
+    });
+    return GOLAP_OK;
+}
+
+}  // extern "C"

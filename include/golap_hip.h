@@ -202,6 +202,68 @@ typedef struct {
 } golap_q1_result;
 int golap_q1_run(golap_ctx *ctx, const golap_q1_plan *plan, golap_q1_result *res);
 
+/* ------------------------------------------------------------------ group-by
+ * Results of the grouping entry points are dense over a declared group domain: each grouping
+ * attribute a has a range [lo, lo + card); the group index is the mixed-radix number of the
+ * attributes in the order given (first = most significant), so groups come out sorted by their
+ * key tuple. Groups with at least one row are returned: keys [n_groups][n_attrs] int32, SUM as
+ * int64 and COUNT. The product of the cards must stay below 2^31.
+ * Aggregation: GOLAP_AGG_LDS keeps workgroup-private partial tables in LDS (hashed, probing a
+ * few slots) and spills rows that find no slot to global int64 atomics; GOLAP_AGG_GLOBAL adds
+ * every row with global atomics; GOLAP_AGG_AUTO picks. */
+enum golap_agg_mode { GOLAP_AGG_AUTO = 0, GOLAP_AGG_LDS = 1, GOLAP_AGG_GLOBAL = 2 };
+
+typedef struct {
+    const void *dev_col;
+    uint32_t type;
+    int32_t lo;
+    uint32_t card;
+} golap_group_key;
+
+/* group_by_agg_2col / group_by_agg_4col (core/group.cuh:14-36) with HashAggregate::add
+ * (core/dev_structs.cuh:209-244) and an atomic SUM functor: n_keys (1..4) key columns, value
+ * column (any integer type), n_rows. GOLAP_ECAPACITY if more than max_groups groups exist
+ * (*n_groups is still set); GOLAP_EINVAL if a key lies outside its declared range. */
+int golap_group_by(golap_ctx *ctx, const golap_group_key *keys, uint32_t n_keys, const void *dev_val,
+                   uint32_t val_type, uint64_t n_rows, uint32_t agg_mode, uint64_t *n_groups,
+                   int32_t *group_keys, int64_t *sums, uint64_t *counts, uint64_t max_groups);
+
+/* ------------------------------------------------------------------ star join + group-by
+ * query2 / query3 (usecases/ssb/queries/02.hpp:54-307, 03.hpp:79-677) with pipeline_q2 /
+ * pipeline_q3 (02.cuh:102-193, 03.cuh:118-170) fused with decompression: over chunks
+ * [chunk_begin, chunk_end) of row-aligned fact columns, a row qualifies if every dimension's
+ * foreign key probes into its built map; the group key is made of the payload attributes of the
+ * dimensions with group_card > 0 (their maps must be built with that attribute as payload), in
+ * dimension order; SUM(value) as int64 and COUNT per group. */
+#define GOLAP_STAR_MAX_DIMS 4
+typedef struct {
+    const golap_column *fk;
+    const golap_hashmap *map;
+    int32_t group_lo;
+    uint32_t group_card;  /* 0: this dimension only filters */
+} golap_star_dim;
+typedef struct {
+    golap_star_dim dims[GOLAP_STAR_MAX_DIMS];
+    uint32_t n_dims;
+    const golap_column *value;
+    uint64_t chunk_begin, chunk_end;
+    const uint16_t *dev_chunk_mask;
+    uint32_t agg_mode;         /* golap_agg_mode */
+    uint32_t lds_group_slots;  /* LDS partial-table slots (power of two); 0 = library choice */
+} golap_star_plan;
+typedef struct {
+    uint64_t n_groups;
+    uint64_t rows_matched;
+    int64_t total_sum;
+    uint64_t chunks_run, chunks_pruned;
+    uint64_t comp_bytes;      /* compressed bytes of the non-pruned pages of the fact columns */
+    double kernel_ms;
+    uint32_t driver_dim;      /* dimension whose filter is pinned in LDS */
+    uint32_t lds_group_slots; /* LDS partial-table slots used (0 = global atomics) */
+} golap_star_result;
+int golap_star_run(golap_ctx *ctx, const golap_star_plan *plan, golap_star_result *res,
+                   int32_t *group_keys, int64_t *sums, uint64_t *counts, uint64_t max_groups);
+
 /* ------------------------------------------------------------------ synthetic data
  * Seeded column generators of the benchmark workloads (the reference seeds its generator with
  * util::uniform_int, core/util.hpp:177-181; these use a SplitMix64 counter hash so row ranges
@@ -212,6 +274,16 @@ int golap_gen_date_dim(uint32_t n_days, int32_t *key, int16_t *year, uint8_t *mo
 int golap_gen_q1_fact(uint64_t seed, uint64_t row0, uint64_t n, const int32_t *date_keys,
                       uint32_t n_dates, int32_t *date_key, uint8_t *discount, uint8_t *quantity,
                       int32_t *price, uint32_t threads);
+/* Star-schema fact rows [row0, row0+n) of n_total: date_key sorted (row r on day
+ * floor(r*n_dates/n_total)), customer/supplier/part keys uniform over [0, n_*), revenue
+ * U[100, 5000000]. */
+int golap_gen_star_fact(uint64_t seed, uint64_t row0, uint64_t n, uint64_t n_total,
+                        const int32_t *date_keys, uint32_t n_dates, uint32_t n_cust,
+                        uint32_t n_supp, uint32_t n_part, int32_t *date_key, int32_t *cust_key,
+                        int32_t *supp_key, int32_t *part_key, int32_t *revenue, uint32_t threads);
+/* out[i] uniform over [lo, hi], rows [0, n), generator column id col. */
+int golap_gen_uniform_i32(uint64_t seed, uint32_t col, uint64_t n, int32_t lo, int32_t hi,
+                          int32_t *out, uint32_t threads);
 
 #ifdef __cplusplus
 }

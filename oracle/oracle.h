@@ -61,6 +61,15 @@ void oracle_gen_q1_fact(uint64_t seed, uint64_t row0, uint64_t n, const int32_t 
                         uint8_t *quantity, int32_t *price);
 /* Date dimension of n_days consecutive days starting 1992-01-01: key yyyymmdd, year, month. */
 void oracle_gen_date_dim(uint32_t n_days, int32_t *key, int16_t *year, uint8_t *month);
+/* Star-schema fact: date_key sorted (day floor(r*n_dates/n_total)), cust/supp/part keys uniform
+ * over the dimension sizes (columns 10, 11, 12), revenue U[100,5000000] (column 13). */
+void oracle_gen_star_fact(uint64_t seed, uint64_t row0, uint64_t n, uint64_t n_total,
+                          const int32_t *date_keys, uint32_t n_dates, uint32_t n_cust,
+                          uint32_t n_supp, uint32_t n_part, int32_t *date_key, int32_t *cust_key,
+                          int32_t *supp_key, int32_t *part_key, int32_t *revenue);
+/* out[i] uniform over [lo, hi] (column col), rows [0, n). */
+void oracle_gen_uniform_i32(uint64_t seed, uint32_t col, uint64_t n, int32_t lo, int32_t hi,
+                            int32_t *out);
 
 /* ------------------------------------------------------------------ codecs (CPU decode)
  * LZ4 block decode with the semantics the reference's CPU codec relies on:

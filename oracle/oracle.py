@@ -60,6 +60,8 @@ def load():
             "oracle_uniform": (u32, [u64, u32, u64, u32, u32]),
             "oracle_gen_q1_fact": (None, [u64, u64, u64, vp, u32, vp, vp, vp, vp]),
             "oracle_gen_date_dim": (None, [u32, vp, vp, vp]),
+            "oracle_gen_star_fact": (None, [u64, u64, u64, u64, vp, u32, u32, u32, u32, vp, vp, vp, vp, vp]),
+            "oracle_gen_uniform_i32": (None, [u64, u32, u64, C.c_int32, C.c_int32, vp]),
             "oracle_lz4_decode": (i64, [vp, u64, vp, u64]),
             "oracle_dbp_decode": (i64, [vp, u64, vp, u64]),
             "oracle_rle_decode": (i64, [vp, u64, vp, u64]),
@@ -138,6 +140,20 @@ def gen_q1_fact(seed, n, date_keys, row0=0):
     cols = (np.empty(n, np.int32), np.empty(n, np.uint8), np.empty(n, np.uint8), np.empty(n, np.int32))
     load().oracle_gen_q1_fact(seed, row0, n, _p(date_keys), date_keys.size, *[_p(c) for c in cols])
     return cols
+
+
+def gen_star_fact(seed, n, n_total, date_keys, n_cust, n_supp, n_part, row0=0):
+    date_keys = np.ascontiguousarray(date_keys, dtype=np.int32)
+    cols = tuple(np.empty(n, np.int32) for _ in range(5))
+    load().oracle_gen_star_fact(seed, row0, n, n_total, _p(date_keys), date_keys.size, n_cust, n_supp, n_part,
+                                *[_p(c) for c in cols])
+    return cols
+
+
+def gen_uniform_i32(seed, col, n, lo, hi):
+    out = np.empty(n, np.int32)
+    load().oracle_gen_uniform_i32(seed, col, n, lo, hi, _p(out))
+    return out
 
 
 def lz4_decode(block, n_out):

@@ -61,3 +61,25 @@ void oracle_gen_date_dim(uint32_t n_days, int32_t *key, int16_t *year, uint8_t *
         if (month) month[i] = (uint8_t)m;
     }
 }
+
+/* Star-schema fact columns, the same formulas as the product generator (restated here so the
+ * checker does not depend on the product). */
+void oracle_gen_star_fact(uint64_t seed, uint64_t row0, uint64_t n, uint64_t n_total,
+                          const int32_t *date_keys, uint32_t n_dates, uint32_t n_cust,
+                          uint32_t n_supp, uint32_t n_part, int32_t *date_key, int32_t *cust_key,
+                          int32_t *supp_key, int32_t *part_key, int32_t *revenue) {
+    for (uint64_t i = 0; i < n; ++i) {
+        uint64_t r = row0 + i;
+        if (date_key) date_key[i] = date_keys[(unsigned __int128)r * n_dates / n_total];
+        if (cust_key) cust_key[i] = (int32_t)oracle_uniform(seed, 10, r, 0, n_cust - 1);
+        if (supp_key) supp_key[i] = (int32_t)oracle_uniform(seed, 11, r, 0, n_supp - 1);
+        if (part_key) part_key[i] = (int32_t)oracle_uniform(seed, 12, r, 0, n_part - 1);
+        if (revenue) revenue[i] = (int32_t)oracle_uniform(seed, 13, r, 100, 5000000);
+    }
+}
+
+void oracle_gen_uniform_i32(uint64_t seed, uint32_t col, uint64_t n, int32_t lo, int32_t hi,
+                            int32_t *out) {
+    for (uint64_t i = 0; i < n; ++i)
+        out[i] = lo + (int32_t)oracle_uniform(seed, col, i, 0, (uint32_t)((int64_t)hi - lo));
+}

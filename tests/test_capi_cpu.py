@@ -47,3 +47,14 @@ def test_product_generator_matches_oracle(orc, golap):
         assert np.array_equal(x, y)
     assert a[1].min() == 0 and a[1].max() == 10 and a[2].min() == 1 and a[2].max() == 50
     assert a[3].min() >= 90000 and a[3].max() <= 10500000
+
+
+def test_star_generators_match_oracle(orc, golap):
+    import numpy as np
+    key, _, _ = golap.gen_date_dim(2556)
+    a = golap.gen_star_fact(5, 100_000, 1_000_000, key, 2000, 200, 1400, row0=333_333, threads=3)
+    b = orc.gen_star_fact(5, 100_000, 1_000_000, key, 2000, 200, 1400, row0=333_333)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    assert np.all(np.diff(a[0]) >= 0)  # date_key sorted
+    assert np.array_equal(golap.gen_uniform_i32(9, 21, 50_000, -3, 24), orc.gen_uniform_i32(9, 21, 50_000, -3, 24))

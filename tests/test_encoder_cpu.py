@@ -137,3 +137,26 @@ def test_encode_rejects_bad_options(golap):
         golap.HostColumn.encode(np.zeros(100, np.int32), page_rows=100)  # not a multiple of 64
     with pytest.raises(golap.GolapError):
         golap.HostColumn.encode(np.zeros(100, np.int32), page_rows=1024, chunk_rows=1536)
+
+
+@pytest.mark.parametrize("min_match", [4, 5, 8])
+def test_literal_only_flag_means_no_match(orc, golap, min_match):
+    """A page is flagged literal-only (read in place by the scans) only if the block is one
+    literal run: values < 2^23 give blocks with a single short match whose size equals the
+    literal-only size, which must not be flagged."""
+    r = rng(17)
+    v = r.integers(100, 5_000_000, 2_000_000).astype(np.int32)
+    a = golap.HostColumn.encode(v, codec=1, page_rows=4096, chunk_rows=65536, lz4_min_match=min_match).arrays()
+    d = a["desc"]
+    raw = v.view(np.uint8)
+    flagged = 0
+    for p in range(d.n_pages):
+        rows = min(4096, d.n_rows - p * 4096)
+        nb = rows * 4
+        off, cs = int(a["page_off"][p]), int(a["page_csize"][p])
+        if a["page_codec"][p] & PAGE_LITERAL_ONLY:
+            flagged += 1
+            hdr = 1 if nb < 15 else 2 + (nb - 15) // 255
+            assert np.array_equal(a["comp"][off + hdr:off + hdr + nb], raw[p * 16384:p * 16384 + nb]), p
+        assert orc.lz4_decode(a["comp"][off:off + cs].tobytes(), nb) == raw[p * 16384:p * 16384 + nb].tobytes()
+    assert flagged > 0

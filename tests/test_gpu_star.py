@@ -1,0 +1,166 @@
+"""Fused star join + group-by (query2/query3 with pipeline_q2/pipeline_q3,
+usecases/ssb/queries/02.hpp, 02.cuh:102-193, 03.cuh:118-170) against the oracle's restatement of
+the CPU plan (02inmem.hpp:58-206) and the reference's logged SSB Q2.x results."""
+import numpy as np
+import pytest
+
+from gpu_util import say
+
+pytestmark = pytest.mark.gpu
+
+
+def star_data(golap, n, seed=17, n_cust=20000, n_supp=2000, n_part=14000):
+    key, year, _ = golap.gen_date_dim(2556)
+    fact = golap.gen_star_fact(seed, n, n, key, n_cust, n_supp, n_part)
+    nation = golap.gen_uniform_i32(seed, 20, n_cust, 0, 24)
+    region = golap.gen_uniform_i32(seed, 21, n_supp, 0, 4)
+    brand = golap.gen_uniform_i32(seed, 22, n_part, 1, 1000)
+    return dict(d_key=key, d_year=year.astype(np.int32), c_nation=nation, s_region=region, p_brand=brand,
+                c_key=np.arange(n_cust, dtype=np.int32), s_key=np.arange(n_supp, dtype=np.int32),
+                p_key=np.arange(n_part, dtype=np.int32), date=fact[0], cust=fact[1], supp=fact[2], part=fact[3],
+                rev=fact[4])
+
+
+def build_map(ctx, golap, key, n, preds, payload):
+    dk = ctx.to_device(key)
+    bufs = [dk]
+    dp = []
+    for col, lo, hi in preds:
+        d = ctx.to_device(col)
+        bufs.append(d)
+        dp.append((d, golap.TYPE_OF[col.dtype], lo, hi))
+    dpay = None
+    if payload is not None:
+        dpay = ctx.to_device(np.ascontiguousarray(payload, np.int32))
+        bufs.append(dpay)
+    m = ctx.hashmap(n, dk, golap.I32)
+    ctx.hashmap_build(m, n, dp, dpay)
+    return m, bufs
+
+
+QUERIES = [
+    # (supplier region, customer nation, brand range) -> group (year, brand)
+    (1, 7, (42, 42)),
+    (3, 11, (100, 199)),
+    (0, 0, (1, 1000)),
+]
+
+
+@pytest.mark.parametrize("q", range(len(QUERIES)))
+@pytest.mark.parametrize("codec,mm,date_codec", [(1, 16, 255), (1, 4, 1), (0, 4, 0)])
+@pytest.mark.parametrize("mode", [0, 2])
+def test_star_matches_oracle(orc, golap, ctx, q, codec, mm, date_codec, mode):
+    n = 2_000_000
+    D = star_data(golap, n)
+    reg, nat, (blo, bhi) = QUERIES[q]
+    say("star", q, "codec", codec, mm, "date codec", date_codec, "mode", mode)
+    enc = dict(page_rows=4096, chunk_rows=65536, lz4_min_match=mm)
+    cols = {}
+    for name in ("date", "cust", "supp", "part", "rev"):
+        cols[name] = ctx.upload(golap.HostColumn.encode(D[name], codec=date_codec if name == "date" else codec, **enc))
+    maps, bufs = [], []
+    for key, n_dim, preds, pay in (
+            (D["d_key"], 2556, [], D["d_year"]),
+            (D["c_key"], D["c_key"].size, [(D["c_nation"], nat, nat)], None),
+            (D["s_key"], D["s_key"].size, [(D["s_region"], reg, reg)], None),
+            (D["p_key"], D["p_key"].size, [(D["p_brand"], blo, bhi)], D["p_brand"])):
+        m, b = build_map(ctx, golap, key, n_dim, preds, pay)
+        maps.append(m)
+        bufs += b
+    try:
+        dims = [(cols["date"], maps[0], 1992, 7), (cols["cust"], maps[1], 0, 0), (cols["supp"], maps[2], 0, 0),
+                (cols["part"], maps[3], 1, 1000)]
+        r, k, s, c = ctx.star_run(dims, cols["rev"], agg_mode=mode)
+        wk, ws, wc = orc.star_groupby(
+            [(D["d_key"], [], D["d_year"]), (D["c_key"], [(D["c_nation"], nat, nat)], None),
+             (D["s_key"], [(D["s_region"], reg, reg)], None), (D["p_key"], [(D["p_brand"], blo, bhi)], D["p_brand"])],
+            [D["date"], D["cust"], D["supp"], D["part"]], D["rev"], threads=4)
+        assert np.array_equal(k, wk) and np.array_equal(s, ws) and np.array_equal(c, wc)
+        assert r.rows_matched == int(wc.sum()) and r.total_sum == int(ws.sum())
+        sel = [1.0, float(np.mean(D["c_nation"] == nat)), float(np.mean(D["s_region"] == reg)),
+               float(np.mean((D["p_brand"] >= blo) & (D["p_brand"] <= bhi)))]
+        assert r.driver_dim == int(np.argmin(sel))  # the most selective dimension drives
+    finally:
+        for c_ in cols.values():
+            ctx.free_column(c_)
+        for m in maps:
+            ctx.free_hashmap(m)
+        for b in bufs:
+            ctx.free(b)
+
+
+def test_star_with_zone_map_pruning(orc, golap, ctx):
+    """Date-sorted fact: pruning chunks by a date range gives the result of the date predicate."""
+    n = 3_000_000
+    D = star_data(golap, n, seed=3)
+    enc = dict(page_rows=4096, chunk_rows=65536, lz4_min_match=16)
+    cols = {nm: ctx.upload(golap.HostColumn.encode(D[nm], codec=255 if nm == "date" else 1, **enc))
+            for nm in ("date", "cust", "supp", "part", "rev")}
+    # the date map keeps 1995..1996 only; pruning by the same range must not change the result
+    maps, bufs = [], []
+    for key, n_dim, preds, pay in ((D["d_key"], 2556, [(D["d_year"], 1995, 1996)], D["d_year"]),
+                                   (D["s_key"], D["s_key"].size, [(D["s_region"], 2, 2)], None)):
+        m, b = build_map(ctx, golap, key, n_dim, preds, pay)
+        maps.append(m)
+        bufs += b
+    nch = ctx.column_desc(cols["date"]).n_chunks
+    flags = ctx.alloc(nch * 2)
+    try:
+        ctx.minmax_check(cols["date"], 19950101, 19961231, flags)
+        dims = [(cols["date"], maps[0], 1992, 7), (cols["supp"], maps[1], 0, 0)]
+        r1, k1, s1, c1 = ctx.star_run(dims, cols["rev"], dev_chunk_mask=flags)
+        r2, k2, s2, c2 = ctx.star_run(dims, cols["rev"])
+        assert np.array_equal(k1, k2) and np.array_equal(s1, s2) and np.array_equal(c1, c2)
+        assert r1.chunks_pruned > nch // 2
+        wk, ws, wc = orc.star_groupby([(D["d_key"], [(D["d_year"], 1995, 1996)], D["d_year"]),
+                                       (D["s_key"], [(D["s_region"], 2, 2)], None)],
+                                      [D["date"], D["supp"]], D["rev"], threads=4)
+        assert np.array_equal(k1, wk) and np.array_equal(s1, ws) and np.array_equal(c1, wc)
+    finally:
+        for c_ in cols.values():
+            ctx.free_column(c_)
+        for m in maps:
+            ctx.free_hashmap(m)
+        for b in bufs + [flags]:
+            ctx.free(b)
+
+
+SF10_Q2 = [(1737351738436, 280), (347570089279, 56), (43762644335, 7)]  # query_sf10_export.csv
+
+
+@pytest.fixture(scope="module")
+def ssb10(orc):
+    return orc.ssb_generate(10)
+
+
+@pytest.mark.parametrize("q", [0, 1, 2])
+def test_ssb_sf10_q2_reproduces_reference_log(golap, ctx, ssb10, q):
+    """Q2.1: p_category = MFGR#12, s_region = AMERICA; Q2.2: p_brand1 in MFGR#2221..MFGR#2228,
+    s_region = ASIA; Q2.3: p_brand1 = MFGR#2239, s_region = EUROPE; group by (d_year, p_brand1)."""
+    lo, d, p, s = ssb10
+    say("ssb sf10 q2." + str(q + 1), "rows", lo["orderdate"].size)
+    n_parts, n_supp = p["category"].size, s["region"].size
+    part_pred = [(p["category"], 12, 12)] if q == 0 else [(p["brand1"], 2221, 2228)] if q == 1 else \
+        [(p["brand1"], 2239, 2239)]
+    region = [1, 2, 3][q]
+    enc = dict(codec=1, page_rows=4096, chunk_rows=262144, lz4_min_match=4)
+    cols = {k: ctx.upload(golap.HostColumn.encode(lo[k], **enc)) for k in ("orderdate", "partkey", "suppkey", "revenue")}
+    maps, bufs = [], []
+    for key, n_dim, preds, pay in ((d["key"], 2555, [], d["year"]),
+                                   (np.arange(n_parts, dtype=np.int32), n_parts, part_pred, p["brand1"]),
+                                   (np.arange(n_supp, dtype=np.int32), n_supp, [(s["region"], region, region)], None)):
+        m, b = build_map(ctx, golap, key, n_dim, preds, pay)
+        maps.append(m)
+        bufs += b
+    try:
+        dims = [(cols["orderdate"], maps[0], 1992, 7), (cols["partkey"], maps[1], 1101, 4500),
+                (cols["suppkey"], maps[2], 0, 0)]
+        r, k, sums, cnt = ctx.star_run(dims, cols["revenue"])
+        assert (r.total_sum, r.n_groups) == SF10_Q2[q]
+    finally:
+        for c_ in cols.values():
+            ctx.free_column(c_)
+        for m in maps:
+            ctx.free_hashmap(m)
+        for b in bufs:
+            ctx.free(b)
