@@ -33,6 +33,8 @@ def log(*a):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", choices=["q1", "star"], default="q1",
+                    help="q1: fused filter/probe/sum (the headline line); star: star join + group-by")
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--rows", type=int, default=600_000_000)
@@ -90,12 +92,96 @@ def cpu_baseline(args, hosts, key, year, gpu_sum):
             "bit_exact_vs_gpu": bool(res == gpu_sum)}
 
 
+def run_star(args):
+    """Star join + group-by on 600M fact rows: date_key sorted (bit-packed delta), customer,
+    supplier, part keys uniform (LZ4), revenue (LZ4); dimensions of 2,556 / 2,000,000 /
+    200,000 / 1,400,000 rows; supplier region = 1 (1/5), customer nation = 7 (1/25), part
+    brand = 42 (1/1000); group by (year, brand), SUM(revenue) int64 and COUNT."""
+    import golap_public_amd as g
+    n_cust, n_supp, n_part = 2_000_000, 200_000, 1_400_000
+    key, year, _ = g.gen_date_dim(2556)
+    log(f"generating {args.rows} star rows")
+    fact = g.gen_star_fact(args.seed, args.rows, args.rows, key, n_cust, n_supp, n_part)
+    nation = g.gen_uniform_i32(args.seed, 20, n_cust, 0, 24)
+    region = g.gen_uniform_i32(args.seed, 21, n_supp, 0, 4)
+    brand = g.gen_uniform_i32(args.seed, 22, n_part, 1, 1000)
+    log("encoding")
+    enc = dict(page_rows=args.page_rows, chunk_rows=args.chunk_rows, lz4_min_match=args.min_match)
+    hosts = [g.HostColumn.encode(fact[0], codec=g.CODEC_DBP, **enc)] + \
+            [g.HostColumn.encode(c, codec=g.CODEC_LZ4, **enc) for c in fact[1:]]
+    del fact
+    ctx = g.Context(0)
+    cols = [ctx.upload(h) for h in hosts]
+    comp = sum(h.desc().payload_bytes for h in hosts)
+    log(f"compressed payload {comp/1e9:.3f} GB for {args.rows*20/1e9:.3f} GB raw")
+    dev = lambda a: ctx.to_device(np.ascontiguousarray(a))
+    d_key, d_year = dev(key), dev(year.astype(np.int32))
+    c_key, c_nat = dev(np.arange(n_cust, dtype=np.int32)), dev(nation)
+    s_key, s_reg = dev(np.arange(n_supp, dtype=np.int32)), dev(region)
+    p_key, p_brand = dev(np.arange(n_part, dtype=np.int32)), dev(brand)
+    maps = [ctx.hashmap(2556, d_key, g.I32), ctx.hashmap(n_cust, c_key, g.I32),
+            ctx.hashmap(n_supp, s_key, g.I32), ctx.hashmap(n_part, p_key, g.I32)]
+
+    def step():
+        ctx.hashmap_build(maps[0], 2556, [], d_year)
+        ctx.hashmap_build(maps[1], n_cust, [(c_nat, g.I32, 7, 7)])
+        ctx.hashmap_build(maps[2], n_supp, [(s_reg, g.I32, 1, 1)])
+        ctx.hashmap_build(maps[3], n_part, [(p_brand, g.I32, 42, 42)], p_brand)
+        dims = [(cols[0], maps[0], 1992, 7), (cols[1], maps[1], 0, 0), (cols[2], maps[2], 0, 0),
+                (cols[4 - 1], maps[3], 1, 1000)]
+        return ctx.star_run(dims, cols[4])
+
+    for _ in range(args.warmup):
+        r = step()
+    ctx.sync()
+    kms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = step()
+        kms.append(r[0].kernel_ms)
+    ctx.sync()
+    elapsed = time.perf_counter() - t0
+    res, keys, sums, counts = r
+    kernel_ms = statistics.mean(kms)
+    log(f"groups {res.n_groups} rows {res.rows_matched} sum {res.total_sum}; "
+        f"{elapsed*1e3/args.steps:.3f} ms/step, kernel {kernel_ms:.3f} ms")
+    out = {"metric": "fact rows/s (star join + group-by, 1 GPU)", "value": args.rows * args.steps / elapsed,
+           "unit": "rows/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "int64", "data": "synthetic: SplitMix64-seeded star schema",
+           "config": {"workload": "star_groupby_600M", "rows": args.rows, "page_rows": args.page_rows,
+                      "chunk_rows": args.chunk_rows, "codecs": "date_key DBP, keys/revenue LZ4",
+                      "comp_bytes": res.comp_bytes, "kernel_ms": kernel_ms, "groups": res.n_groups,
+                      "rows_matched": res.rows_matched, "result_sum": res.total_sum,
+                      "driver_dim": res.driver_dim, "lds_group_slots": res.lds_group_slots,
+                      "comp_bytes_per_s_GBps": res.comp_bytes / (kernel_ms * 1e-3) / 1e9}}
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        threads = len(os.sched_getaffinity(0))
+        sample = min(args.rows, 60_000_000)
+        f2 = oracle.gen_star_fact(args.seed, sample, args.rows, key, n_cust, n_supp, n_part)
+        t0 = time.perf_counter()
+        w = oracle.star_groupby([(key, [], year.astype(np.int32)),
+                                 (np.arange(n_cust, dtype=np.int32), [(nation, 7, 7)], None),
+                                 (np.arange(n_supp, dtype=np.int32), [(region, 1, 1)], None),
+                                 (np.arange(n_part, dtype=np.int32), [(brand, 42, 42)], brand)],
+                                list(f2[:4]), f2[4], threads=threads)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": sample / dt, "unit": "rows/s", "cores": threads, "kind": "port",
+                               "sample": f"first {sample} rows (decoded columns), star plan of 02inmem.hpp"}
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the sharded form is described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "star":
+        return run_star(args)
     import golap_public_amd as g
 
     t_setup = time.perf_counter()

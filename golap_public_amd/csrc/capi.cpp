@@ -261,6 +261,8 @@ int golap_host_column_assemble(uint32_t type, uint64_t n_rows, uint32_t page_row
         uint64_t rows = std::min<uint64_t>(page_rows, n_rows - p * page_rows);
         if (cd == golap_fmt::LZ4 && golap_host::lz4_is_literal_only(pages[p], csize[p], rows * c->elem))
             cd |= golap_fmt::PAGE_LITERAL_ONLY;
+        if (cd == golap_fmt::RLE || (cd == golap_fmt::DBP && csize[p] >= 32 && pages[p][20] == 0))
+            cd |= golap_fmt::PAGE_POINT;
         c->page_codec[p] = cd;
         ptrs[p] = pages[p];
     }
@@ -346,8 +348,13 @@ int golap_column_upload(golap_ctx *ctx, const golap_host_column *h, golap_column
     c->d_chunk_max = static_cast<int64_t *>(p[5]);
     ColumnExtra ex;
     ex.nondirect_prefix.resize(c->n_pages + 1, 0);
-    for (uint64_t i = 0; i < c->n_pages; ++i)
-        ex.nondirect_prefix[i + 1] = ex.nondirect_prefix[i] + (page_direct_host(c->page_codec[i]) ? 0 : 1);
+    ex.window_prefix.resize(c->n_pages + 1, 0);
+    for (uint64_t i = 0; i < c->n_pages; ++i) {
+        const bool direct = page_direct_host(c->page_codec[i]);
+        ex.nondirect_prefix[i + 1] = ex.nondirect_prefix[i] + (direct ? 0 : 1);
+        const bool point = (c->page_codec[i] & golap_fmt::PAGE_POINT) != 0;
+        ex.window_prefix[i + 1] = ex.window_prefix[i] + (direct || point ? 0 : 1);
+    }
     ex.chunk_payload.resize(c->n_chunks + 1, 0);
     const uint64_t ppc = c->chunk_rows / c->page_rows;
     for (uint64_t ch = 0; ch < c->n_chunks; ++ch) {

@@ -151,7 +151,8 @@ int golap_star_run(golap_ctx *ctx, const golap_star_plan *plan, golap_star_resul
             for (int64_t mx : sd.fk->chunk_max)
                 if (mx > INT32_MAX) return fail(ctx, GOLAP_EINVAL, "star: uint32 keys above INT32_MAX");
     }
-    if (v->page_rows > 8192) return fail(ctx, GOLAP_EINVAL, "star: page_rows above 8192");
+    if (v->page_rows != 1024 && v->page_rows != 2048 && v->page_rows != 4096 && v->page_rows != 8192)
+        return fail(ctx, GOLAP_EINVAL, "star: page_rows must be 1024, 2048, 4096 or 8192");
     if (plan->chunk_begin > plan->chunk_end || plan->chunk_end > v->n_chunks)
         return fail(ctx, GOLAP_EINVAL, "star: bad chunk range");
     Lock l(ctx->mu);
@@ -200,7 +201,7 @@ int golap_star_run(golap_ctx *ctx, const golap_star_plan *plan, golap_star_resul
     auto window_of = [&](const golap_column *c) {
         std::lock_guard<std::mutex> g(g_extra_mu);
         auto &ex = g_extra[c];
-        return ex.nondirect_prefix[L.page_end] != ex.nondirect_prefix[L.page_begin] ? (int)windows++ : -1;
+        return ex.window_prefix[L.page_end] != ex.window_prefix[L.page_begin] ? (int)windows++ : -1;
     };
     size_t a_idx = 0;
     for (uint32_t d = 0; d < nd; ++d) {
@@ -221,14 +222,14 @@ int golap_star_run(golap_ctx *ctx, const golap_star_plan *plan, golap_star_resul
 
     // driver filter: dense exact bitmap, or the hashed one-bit filter of the map's keys
     const golap_hashmap *dm = plan->dims[L.driver].map;
-    if (dm->use_bitmap && (dm->krange + 31) / 32 <= (1u << (HASH_FILTER_LOG2 - 5))) {
+    if (dm->use_bitmap && (dm->krange + 31) / 32 <= (1u << HASH_FILTER_WORDS_LOG2)) {
         L.filter = dm->d_bitmap;
         L.filter_words = (dm->krange + 31) / 32;
         L.fkmin = dm->kmin;
         L.fkrange = dm->krange;
         L.fhash_log2 = 0;
     } else {
-        const uint32_t words = 1u << (HASH_FILTER_LOG2 - 5);
+        const uint32_t words = 1u << HASH_FILTER_WORDS_LOG2;
         if (!dm->d_hfilter) {
             void *p = nullptr;
             if (int r = dev_alloc(ctx, words * 4, &p)) return r;
@@ -239,15 +240,24 @@ int golap_star_run(golap_ctx *ctx, const golap_star_plan *plan, golap_star_resul
             GOLAP_HIP_CHECK(ctx, hipMemsetAsync(dm->d_hfilter, 0, words * 4, ctx->stream));
             GOLAP_HIP_CHECK(ctx, launch_hash_filter(ctx, dm->small_log2 ? dm->d_small : dm->d_slots,
                                                     dm->small_log2 ? dm->small_log2 : dm->log2cap,
-                                                    dm->d_hfilter, HASH_FILTER_LOG2));
+                                                    dm->d_hfilter, HASH_FILTER_WORDS_LOG2));
             dm->hfilter_build = dm->build_id;
         }
         L.filter = dm->d_hfilter;
         L.filter_words = words;
-        L.fhash_log2 = HASH_FILTER_LOG2;
+        L.fhash_log2 = HASH_FILTER_WORDS_LOG2;
     }
     L.n_groups = domain;
-    L.lds_gslots = choose_lds_slots(plan->agg_mode, plan->lds_group_slots, domain);
+    // AUTO: LDS partial tables pay off when many rows land in each group; estimate the
+    // qualifying rows from the maps' fill ratios (uniform foreign keys)
+    double expect = (double)(L.page_end - L.page_begin) * L.page_rows;
+    for (uint32_t d = 0; d < nd; ++d) {
+        const golap_hashmap *m = plan->dims[d].map;
+        expect *= m->ht_size ? (double)m->filled / (double)m->ht_size : 1.0;
+    }
+    uint32_t agg = plan->agg_mode;
+    if (agg == GOLAP_AGG_AUTO) agg = expect >= 16.0 * (double)domain && expect >= 65536.0 ? GOLAP_AGG_LDS : GOLAP_AGG_GLOBAL;
+    L.lds_gslots = choose_lds_slots(agg, plan->lds_group_slots, domain);
     while (L.lds_gslots && star_lds_bytes(L.filter_words, L.page_rows, L.n_windows, L.lds_gslots) >
                                (size_t)ctx->lds_max)
         L.lds_gslots >>= 1;

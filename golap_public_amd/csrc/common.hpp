@@ -65,7 +65,7 @@ struct golap_hashmap {
     uint32_t small_log2 = 0;
     uint32_t small_cap_log2 = 0;  // allocated size of d_small
     uint64_t *d_small = nullptr;
-    // one-bit-per-key hashed filter (2^HASH_FILTER_LOG2 bits), built on demand when the key
+    // blocked Bloom filter (2^HASH_FILTER_WORDS_LOG2 words), built on demand when the key
     // range is too wide for the dense bitmap; a cache, valid for the build it was made from
     mutable uint32_t *d_hfilter = nullptr;
     mutable uint64_t hfilter_build = 0;
@@ -98,6 +98,7 @@ inline int fail(golap_ctx *ctx, int code, const std::string &msg) {
 // Runtime helpers shared by the C ABI translation units (capi.cpp).
 struct ColumnExtra {
     std::vector<uint32_t> nondirect_prefix;  // pages that cannot be read in place, prefix count
+    std::vector<uint32_t> window_prefix;     // pages neither in place nor point-readable
     std::vector<uint64_t> chunk_payload;     // payload bytes per chunk, prefix
 };
 extern std::mutex g_extra_mu;
@@ -123,7 +124,7 @@ inline ColView view(const golap_column *c) {
 }
 
 constexpr uint32_t STAR_MAX_DIMS = 4;
-constexpr uint32_t HASH_FILTER_LOG2 = 18;  // 32 KiB of LDS
+constexpr uint32_t HASH_FILTER_WORDS_LOG2 = 12;  // blocked Bloom filter: 4096 words = 16 KiB of LDS
 
 // One dimension of a star plan as the kernel sees it: exact table and group-key contribution.
 struct StarDimProbe {
@@ -161,7 +162,7 @@ struct StarLaunch {
 hipError_t launch_star(golap_ctx *ctx, const StarLaunch &l);
 size_t star_lds_bytes(uint32_t filter_words, uint32_t page_rows, uint32_t n_windows, uint32_t lds_gslots);
 hipError_t launch_hash_filter(golap_ctx *ctx, const uint64_t *slots, uint32_t log2cap, uint32_t *filter,
-                              uint32_t log2bits);
+                              uint32_t words_log2);
 
 struct GroupByLaunch {
     const void *key[4];

@@ -212,11 +212,13 @@ void code_page(const uint8_t *src, uint64_t rows, uint32_t type, uint32_t elem, 
     }
     if (codec == DBP || codec == GOLAP_CODEC_AUTO) {
         cand.resize(dbp_bound(rows));
-        consider(dbp_encode_page(src, rows, type, cand.data()), DBP);
+        const uint64_t sz = dbp_encode_page(src, rows, type, cand.data());
+        // bw == 0: every row is base + r * min_delta
+        consider(sz, sz && cand[20] == 0 ? (uint8_t)(DBP | PAGE_POINT) : (uint8_t)DBP);
     }
     if (codec == RLE || codec == GOLAP_CODEC_AUTO) {
         cand.resize(rle_bound(rows, elem));
-        consider(rle_encode_page(src, rows, elem, cand.data()), RLE);
+        consider(rle_encode_page(src, rows, elem, cand.data()), (uint8_t)(RLE | PAGE_POINT));
     }
     if (best == UINT64_MAX) {  // DBP requested but the page does not fit 32-bit deltas
         cand.resize(lz4_bound(nbytes));

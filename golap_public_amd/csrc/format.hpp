@@ -14,8 +14,8 @@
 //             match-len bytes; the last sequence has literals only.
 //   DBP  (2)  frame-of-reference delta bit-packing (sorted or slowly varying integers).
 //   RLE  (3)  run-length encoding (value, exclusive end row) pairs.
-// Page flag bit 7 (PAGE_LITERAL_ONLY) marks an LZ4 page made of a single literal run; such a
-// page's payload starts lz4_literal_header_bytes(n_bytes) bytes after the page start and the
+// Flag bits above the codec: PAGE_POINT (0x40, see below) and PAGE_LITERAL_ONLY (0x80), which
+// marks an LZ4 page made of a single literal run; such a page's payload starts lz4_literal_header_bytes(n_bytes) bytes after the page start and the
 // encoder places it 16-byte aligned in the column buffer, so a scan reads it in place.
 #pragma once
 #include <cstdint>
@@ -25,6 +25,9 @@ namespace golap_fmt {
 enum Codec : uint8_t { RAW = 0, LZ4 = 1, DBP = 2, RLE = 3 };
 constexpr uint8_t PAGE_CODEC_MASK = 0x0f;
 constexpr uint8_t PAGE_LITERAL_ONLY = 0x80;
+// Bit 6 (PAGE_POINT): any row of the page can be read without decoding the page: every RLE
+// page (binary search over the run ends) and DBP pages with bw == 0 (v[r] = base + r*min_delta).
+constexpr uint8_t PAGE_POINT = 0x40;
 
 constexpr uint32_t LZ4_MINMATCH = 4;
 constexpr uint32_t LZ4_LASTLITERALS = 5;  // the last 5 bytes of a block are literals

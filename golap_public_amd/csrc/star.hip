@@ -42,27 +42,24 @@ __device__ __forceinline__ uint32_t direct_hdr(uint8_t codec, uint32_t nbytes) {
     return (codec & PAGE_LITERAL_ONLY) ? golap_fmt::lz4_literal_header_bytes(nbytes) : 0u;
 }
 __device__ __forceinline__ uint32_t hash32(uint32_t x) { return x * 0x9E3779B1u; }
-
-__device__ __forceinline__ bool probe_slots(const uint64_t *slots, uint32_t log2cap, int32_t key,
-                                            uint32_t &payload) {
-    const uint32_t mask = (1u << log2cap) - 1, sk = (uint32_t)key ^ 0x80000000u;
-    uint32_t h = hash32((uint32_t)key) >> (32 - log2cap);
-    for (uint32_t i = 0; i <= mask; ++i) {
-        const uint64_t s = slots[h];
-        if (s == 0) return false;
-        if ((uint32_t)s == sk) {
-            payload = (uint32_t)(s >> 32);
-            return true;
-        }
-        h = (h + 1) & mask;
-    }
-    return false;
+// MurmurHash3 finaliser: all output bits depend on all key bits (the filter uses low and high).
+__device__ __forceinline__ uint32_t fmix32(uint32_t h) {
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    return h;
 }
+// Blocked Bloom filter of 2^wlog2 words: a key sets two bits of one word (one LDS read to test).
+__device__ __forceinline__ uint32_t bloom_word(uint32_t h, uint32_t wlog2) { return h >> (32 - wlog2); }
+__device__ __forceinline__ uint32_t bloom_bits(uint32_t h) { return (1u << (h & 31)) | (1u << ((h >> 5) & 31)); }
+
 }  // namespace
 
 struct StarCol {
     ColView c;
-    int window;  // LDS window index when some page of the column in range is coded, else -1
+    int window;  // LDS window index when some page of the column in range needs decoding, else -1
 };
 
 struct StarArgs {
@@ -73,7 +70,7 @@ struct StarArgs {
     uint32_t n_dims;
     uint32_t driver;
     // driver filter (LDS): dense bits over [fkmin, fkmin + fkrange) when fhash_log2 == 0,
-    // else bit hash32(key) >> (32 - fhash_log2)
+    // else a blocked Bloom filter of 2^fhash_log2 words (bloom_word / bloom_bits of fmix32(key))
     const uint32_t *filter;
     uint32_t filter_words;
     int32_t fkmin;
@@ -102,23 +99,83 @@ struct StarLds {
     unsigned long long *gsum;
 };
 
-__device__ __forceinline__ bool star_filter(const StarArgs &a, const uint32_t *f, int32_t key) {
-    uint32_t bit;
-    bool in = true;
-    if (a.fhash_log2 == 0) {
-        bit = (uint32_t)key - (uint32_t)a.fkmin;
-        in = bit < a.fkrange;
-        bit = in ? bit : 0u;
+// How one 32-bit column is read at a tile row.
+enum : uint32_t { ACC_DIRECT = 0, ACC_WINDOW = 1, ACC_AFFINE = 2, ACC_RLE = 3, ACC_BAD = 4 };
+struct ColAccess {
+    const uint8_t *base;  // DIRECT: payload in HBM; WINDOW: LDS window; RLE: page start
+    int64_t a, b;         // AFFINE: v[r] = a + r * b
+    uint32_t mode;
+    uint32_t nruns;       // RLE
+};
+
+// Sets up the access to column sc for page p (rows rows); wwin: its LDS window or null.
+__device__ inline ColAccess col_access(const StarCol &sc, uint64_t p, uint32_t rows, uint8_t *wwin) {
+    ColAccess x{};
+    const uint8_t codec = sc.c.page_codec[p];
+    const uint8_t *src = sc.c.comp + sc.c.page_off[p];
+    const uint32_t csize = sc.c.page_csize[p];
+    if (is_direct(codec)) {
+        const uint32_t hdr = direct_hdr(codec, rows * 4);
+        x.base = src + hdr;
+        x.mode = (csize == hdr + rows * 4 && !(reinterpret_cast<uintptr_t>(x.base) & 15)) ? ACC_DIRECT : ACC_BAD;
+    } else if ((codec & PAGE_CODEC_MASK) == golap_fmt::DBP && (codec & golap_fmt::PAGE_POINT)) {
+        if (csize < 32 || src[20] != 0 || src[21] != 4 || ld_u32_unaligned(src + 16) != rows) {
+            x.mode = ACC_BAD;
+        } else {
+            x.a = (int64_t)ld_u64_unaligned(src);
+            x.b = (int64_t)ld_u64_unaligned(src + 8);
+            x.mode = ACC_AFFINE;
+        }
+    } else if ((codec & PAGE_CODEC_MASK) == golap_fmt::RLE) {
+        const uint32_t nruns = ld_u32_unaligned(src), n = ld_u32_unaligned(src + 4);
+        const bool ok = csize >= 16 && src[8] == 4 && n == rows && nruns >= 1 &&
+                        16ull + 4ull * nruns + 4ull * nruns <= csize;
+        x.base = src;
+        x.nruns = nruns;
+        x.mode = ok ? ACC_RLE : ACC_BAD;
     } else {
-        bit = hash32((uint32_t)key) >> (32 - a.fhash_log2);
+        x.base = wwin;
+        x.mode = wwin ? ACC_WINDOW : ACC_BAD;
     }
-    return in && ((f[bit >> 5] >> (bit & 31)) & 1u);
+    return x;
+}
+
+__device__ __forceinline__ int32_t col_value(const ColAccess &x, uint32_t r) {
+    int32_t v;
+    switch (x.mode) {
+    case ACC_AFFINE: return (int32_t)(uint32_t)(uint64_t)(x.a + (int64_t)r * x.b);
+    case ACC_RLE: {
+        // first run whose exclusive end is above r
+        const uint8_t *ends = x.base + 16 + 4ull * x.nruns;
+        uint32_t lo = 0, hi = x.nruns - 1;
+        while (lo < hi) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (ld_u32_unaligned(ends + 4ull * mid) > r) hi = mid;
+            else lo = mid + 1;
+        }
+        return (int32_t)ld_u32_unaligned(x.base + 16 + 4ull * lo);
+    }
+    default:
+        __builtin_memcpy(&v, x.base + 4 * (size_t)r, 4);
+        return v;
+    }
+}
+
+__device__ __forceinline__ bool star_filter(const StarArgs &a, const uint32_t *f, int32_t key) {
+    if (a.fhash_log2 == 0) {
+        uint32_t bit = (uint32_t)key - (uint32_t)a.fkmin;
+        const bool in = bit < a.fkrange;
+        bit = in ? bit : 0u;
+        return in && ((f[bit >> 5] >> (bit & 31)) & 1u);
+    }
+    const uint32_t h = fmix32((uint32_t)key), m = bloom_bits(h);
+    return (f[bloom_word(h, a.fhash_log2)] & m) == m;
 }
 
 __device__ __forceinline__ void group_add(const StarArgs &a, const StarLds &L, uint32_t gidx, int64_t v) {
     if (a.lds_gslots) {
         const uint32_t mask = a.lds_gslots - 1;
-        uint32_t h = hash32(gidx) >> 7 & mask;
+        uint32_t h = (hash32(gidx) >> 7) & mask;
         for (int i = 0; i < GROUP_PROBES; ++i) {
             uint32_t k = L.gkey[h];
             if (k == GROUP_EMPTY) {
@@ -138,17 +195,13 @@ __device__ __forceinline__ void group_add(const StarArgs &a, const StarLds &L, u
     atomicAdd(reinterpret_cast<unsigned long long *>(&a.g_cnt[gidx]), 1ull);
 }
 
-// Value of 32-bit column `sc` at tile row r: in-place page (HBM) or decoded LDS window.
-__device__ __forceinline__ int32_t col_at(const StarCol &sc, const uint8_t *base, uint32_t r) {
-    int32_t v;
-    __builtin_memcpy(&v, base + 4 * (size_t)r, 4);
-    return v;
-}
-
+// PER = page_rows / BLOCK rows per thread (4, 8, 16 or 32), taken as groups of four.
+template <int PER>
 __global__ __launch_bounds__(BLOCK) void k_star(StarArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     __shared__ uint32_t cand_n, tile_bad;
-    __shared__ const uint8_t *colbase[STAR_MAX_DIMS + 1];
+    __shared__ ColAccess acc[STAR_MAX_DIMS + 1];
+    constexpr int G = PER / 4;
     const uint32_t P = a.page_rows;
     StarLds L;
     size_t off = 0;
@@ -185,45 +238,35 @@ __global__ __launch_bounds__(BLOCK) void k_star(StarArgs a) {
         end = a.page_begin + n * (x + 1) / XCDS;
         stride = gridDim.x / XCDS;
     }
+    int4 kv[G];  // driver keys of the current tile
     for (uint64_t p = first; p < end; p += stride) {
-        if (a.chunk_mask && a.chunk_mask[p / a.pages_per_chunk] == 0) continue;
+        if (a.chunk_mask && a.chunk_mask[p / a.pages_per_chunk] == 0) {
+                continue;
+        }
         const uint32_t rows = (uint32_t)min<uint64_t>(P, a.n_rows - p * P);
         if (threadIdx.x == 0) {
             cand_n = 0;
             tile_bad = 0;
         }
-        // catalog of every column: in-place base, or decode into its window (driver first)
         if (threadIdx.x < ncols) {
             const StarCol &sc = threadIdx.x < a.n_dims ? a.fk[threadIdx.x] : a.val;
-            const uint8_t codec = sc.c.page_codec[p];
-            const uint32_t hdr = direct_hdr(codec, rows * 4);
-            const uint8_t *b = sc.c.comp + sc.c.page_off[p] + hdr;
-            if (is_direct(codec)) {
-                if (sc.c.page_csize[p] != hdr + rows * 4 || (reinterpret_cast<uintptr_t>(b) & 3)) {
-                    tile_bad = 1;
-                    raise_err(a.err, DEVERR_CORRUPT);
-                }
-                colbase[threadIdx.x] = b;
-            } else if (sc.window < 0) {
-                tile_bad = 1;  // the host said this column has no coded page here
-                raise_err(a.err, DEVERR_CORRUPT);
-                colbase[threadIdx.x] = nullptr;
-            } else {
-                colbase[threadIdx.x] = nullptr;  // decoded below
-            }
+            acc[threadIdx.x] = col_access(sc, p, rows, sc.window >= 0 ? L.win + (size_t)sc.window * P * 4 : nullptr);
         }
         __syncthreads();
-        if (tile_bad) {
-            __syncthreads();
+        bool bad = false;
+        for (uint32_t c = 0; c < ncols; ++c) bad |= acc[c].mode == ACC_BAD;
+        if (bad) {  // uniform: every thread read the same entries
+            if (threadIdx.x == 0) raise_err(a.err, DEVERR_CORRUPT);
+                __syncthreads();
             continue;
         }
-        // driver column decoded first when coded
         const StarCol &dsc = a.fk[a.driver];
-        if (colbase[a.driver] == nullptr) {
-            uint8_t *w = L.win + (size_t)dsc.window * P * 4;
+        const ColAccess dacc = acc[a.driver];
+        if (dacc.mode == ACC_WINDOW) {
             if (wave == 0) {
                 const int64_t got = page_decode_wave(dsc.c.page_codec[p], dsc.c.comp + dsc.c.page_off[p],
-                                                     dsc.c.page_csize[p], w, rows * 4, 4);
+                                                     dsc.c.page_csize[p], const_cast<uint8_t *>(dacc.base),
+                                                     rows * 4, 4);
                 if (got != (int64_t)rows * 4 && lane == 0) {
                     tile_bad = 1;
                     raise_err(a.err, DEVERR_CORRUPT);
@@ -231,18 +274,31 @@ __global__ __launch_bounds__(BLOCK) void k_star(StarArgs a) {
             }
             __syncthreads();
         }
-        const uint8_t *dbase = colbase[a.driver] ? colbase[a.driver] : L.win + (size_t)dsc.window * P * 4;
-        // filter + compaction: each thread tests its rows, a wave prefix sum gives its slots
-        uint32_t mine = 0;
-        uint32_t passbits = 0;  // up to 32 rows per thread (P <= 8192)
-        const uint32_t per = (P + BLOCK - 1) / BLOCK;
-        for (uint32_t k = 0; k < per && !tile_bad; ++k) {
-            const uint32_t r = threadIdx.x + k * BLOCK;
-            if (r < rows && star_filter(a, L.filter, col_at(dsc, dbase, r))) {
-                passbits |= 1u << k;
-                ++mine;
+        // filter + compaction: thread t tests rows 4(t + k*BLOCK) + j; a wave prefix sum over the
+        // per-thread counts gives every thread its slots in the candidate list
+        uint32_t passbits = 0;
+        const bool vec = rows == (uint32_t)(PER * BLOCK) && dacc.mode <= ACC_WINDOW;
+        if (!tile_bad) {
+            if (vec) {
+#pragma unroll
+                for (int k = 0; k < G; ++k)
+                    kv[k] = *reinterpret_cast<const int4 *>(dacc.base + 16 * ((size_t)threadIdx.x + (size_t)k * BLOCK));
+#pragma unroll
+                for (int k = 0; k < G; ++k) {
+                    passbits |= (uint32_t)star_filter(a, L.filter, kv[k].x) << (4 * k);
+                    passbits |= (uint32_t)star_filter(a, L.filter, kv[k].y) << (4 * k + 1);
+                    passbits |= (uint32_t)star_filter(a, L.filter, kv[k].z) << (4 * k + 2);
+                    passbits |= (uint32_t)star_filter(a, L.filter, kv[k].w) << (4 * k + 3);
+                }
+            } else {
+                for (int k = 0; k < G; ++k)
+                    for (int j = 0; j < 4; ++j) {
+                        const uint32_t r = 4 * (threadIdx.x + k * BLOCK) + j;
+                        if (r < rows && star_filter(a, L.filter, col_value(dacc, r))) passbits |= 1u << (4 * k + j);
+                    }
             }
         }
+        const uint32_t mine = __popc(passbits);
         uint32_t incl = mine;
 #pragma unroll
         for (int d = 1; d < WAVE; d <<= 1) {
@@ -254,21 +310,23 @@ __global__ __launch_bounds__(BLOCK) void k_star(StarArgs a) {
         if (lane == WAVE - 1 && wtotal) wbase = atomicAdd(&cand_n, wtotal);
         wbase = __shfl(wbase, WAVE - 1, WAVE);
         uint32_t slot = wbase + incl - mine;
-        for (uint32_t k = 0; passbits; ++k, passbits >>= 1)
-            if (passbits & 1u) L.cand[slot++] = (uint16_t)(threadIdx.x + k * BLOCK);
+        for (uint32_t bits = passbits; bits; bits &= bits - 1) {
+            const uint32_t b = __ffs(bits) - 1;
+            L.cand[slot++] = (uint16_t)(4 * (threadIdx.x + (b >> 2) * BLOCK) + (b & 3));
+        }
         __syncthreads();
         const uint32_t nc = cand_n;
         if (nc == 0 || tile_bad) {
             __syncthreads();
             continue;
         }
-        // other coded columns: wave w decodes columns w, w+4, ...
+        // other columns that need a decode: wave w decodes columns w, w+4, ...
         for (uint32_t c = wave; c < ncols; c += BLOCK / WAVE) {
-            if (c == a.driver || colbase[c] != nullptr) continue;
+            if (c == a.driver || acc[c].mode != ACC_WINDOW) continue;
             const StarCol &sc = c < a.n_dims ? a.fk[c] : a.val;
-            uint8_t *w = L.win + (size_t)sc.window * P * 4;
             const int64_t got = page_decode_wave(sc.c.page_codec[p], sc.c.comp + sc.c.page_off[p],
-                                                 sc.c.page_csize[p], w, rows * 4, 4);
+                                                 sc.c.page_csize[p], const_cast<uint8_t *>(acc[c].base),
+                                                 rows * 4, 4);
             if (got != (int64_t)rows * 4 && lane == 0) {
                 tile_bad = 1;
                 raise_err(a.err, DEVERR_CORRUPT);
@@ -278,14 +336,44 @@ __global__ __launch_bounds__(BLOCK) void k_star(StarArgs a) {
         if (!tile_bad) {
             for (uint32_t i = threadIdx.x; i < nc; i += BLOCK) {
                 const uint32_t r = L.cand[i];
+                // exact probe of the driver first (its small table stays in L2): filter false
+                // positives stop here, before any other column is read
+                const StarDimProbe &dd = a.dim[a.driver];
+                {
+                    const int32_t dk = col_value(acc[a.driver], r);
+                    const uint32_t mask = (1u << dd.log2cap) - 1, sk = (uint32_t)dk ^ 0x80000000u;
+                    uint32_t hh = hash32((uint32_t)dk) >> (32 - dd.log2cap);
+                    uint64_t s = dd.slots[hh];
+                    for (uint32_t t = 0; s != 0 && (uint32_t)s != sk && t < mask; ++t) {
+                        hh = (hh + 1) & mask;
+                        s = dd.slots[hh];
+                    }
+                    if (s == 0 || (uint32_t)s != sk) continue;
+                }
+                // then every column value of the row and the first slot of every table: two
+                // memory round trips for the row instead of two per dimension
+                int32_t key[STAR_MAX_DIMS];
+                for (uint32_t d = 0; d < a.n_dims; ++d) key[d] = col_value(acc[d], r);
+                const int32_t raw = col_value(acc[a.n_dims], r);
+                uint32_t h[STAR_MAX_DIMS];
+                uint64_t slot[STAR_MAX_DIMS];
+                for (uint32_t d = 0; d < a.n_dims; ++d) {
+                    h[d] = hash32((uint32_t)key[d]) >> (32 - a.dim[d].log2cap);
+                    slot[d] = a.dim[d].slots[h[d]];
+                }
                 uint64_t gidx = 0;
                 bool ok = true;
                 for (uint32_t d = 0; d < a.n_dims && ok; ++d) {
-                    const uint8_t *b = colbase[d] ? colbase[d] : L.win + (size_t)a.fk[d].window * P * 4;
-                    uint32_t pay = 0;
-                    ok = probe_slots(a.dim[d].slots, a.dim[d].log2cap, col_at(a.fk[d], b, r), pay);
+                    const uint32_t mask = (1u << a.dim[d].log2cap) - 1, sk = (uint32_t)key[d] ^ 0x80000000u;
+                    uint64_t s = slot[d];
+                    uint32_t hh = h[d];
+                    for (uint32_t t = 0; s != 0 && (uint32_t)s != sk && t < mask; ++t) {
+                        hh = (hh + 1) & mask;
+                        s = a.dim[d].slots[hh];
+                    }
+                    ok = s != 0 && (uint32_t)s == sk;
                     if (ok && a.dim[d].group_card) {
-                        const uint32_t gv = pay - (uint32_t)a.dim[d].group_lo;
+                        const uint32_t gv = (uint32_t)(s >> 32) - (uint32_t)a.dim[d].group_lo;
                         if (gv >= a.dim[d].group_card) {
                             raise_err(a.err, DEVERR_KEYRANGE);
                             ok = false;
@@ -294,14 +382,12 @@ __global__ __launch_bounds__(BLOCK) void k_star(StarArgs a) {
                     }
                 }
                 if (!ok) continue;
-                const uint8_t *vb = colbase[a.n_dims] ? colbase[a.n_dims] : L.win + (size_t)a.val.window * P * 4;
-                const int32_t raw = col_at(a.val, vb, r);
                 const int64_t v = a.val_signed ? (int64_t)raw : (int64_t)(uint32_t)raw;
                 group_add(a, L, (uint32_t)gidx, v);
                 ++matched;
             }
         }
-        __syncthreads();  // candidate list, windows and colbase are reused by the next tile
+        __syncthreads();  // candidate list, windows and access entries are reused by the next tile
     }
     // merge the LDS-private table into the global dense result
     __syncthreads();
@@ -320,6 +406,24 @@ __global__ __launch_bounds__(BLOCK) void k_star(StarArgs a) {
 size_t star_lds_bytes(uint32_t filter_words, uint32_t page_rows, uint32_t n_windows, uint32_t lds_gslots) {
     return (((size_t)filter_words * 4 + 15) & ~(size_t)15) + (((size_t)page_rows * 2 + 15) & ~(size_t)15) +
            (size_t)n_windows * page_rows * 4 + (size_t)lds_gslots * 16;
+}
+
+template <typename K>
+static hipError_t launch_star_t(golap_ctx *ctx, K kernel, const StarArgs &a, size_t lds, const StarLaunch &l) {
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    int occ = 0;
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, BLOCK, lds);
+    if (e != hipSuccess) return e;
+    occ = std::max(occ, 1);
+    const uint64_t tiles = l.page_end - l.page_begin;
+    uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>(tiles, (uint64_t)ctx->cus * occ));
+    if (grid >= XCDS) grid -= grid % XCDS;
+    hipLaunchKernelGGL(kernel, dim3((uint32_t)grid), dim3(BLOCK), lds, ctx->stream, a);
+    return hipGetLastError();
 }
 
 hipError_t launch_star(golap_ctx *ctx, const StarLaunch &l) {
@@ -352,20 +456,13 @@ hipError_t launch_star(golap_ctx *ctx, const StarLaunch &l) {
     a.err = l.err;
     const size_t lds = star_lds_bytes(l.filter_words, l.page_rows, l.n_windows, l.lds_gslots);
     if (lds > (size_t)ctx->lds_max) return hipErrorInvalidConfiguration;
-    if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_star),
-                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
+    switch (l.page_rows) {
+    case 1024: return launch_star_t(ctx, k_star<4>, a, lds, l);
+    case 2048: return launch_star_t(ctx, k_star<8>, a, lds, l);
+    case 4096: return launch_star_t(ctx, k_star<16>, a, lds, l);
+    case 8192: return launch_star_t(ctx, k_star<32>, a, lds, l);
+    default: return hipErrorInvalidValue;
     }
-    int occ = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_star, BLOCK, lds);
-    if (e != hipSuccess) return e;
-    occ = std::max(occ, 1);
-    const uint64_t tiles = l.page_end - l.page_begin;
-    uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>(tiles, (uint64_t)ctx->cus * occ));
-    if (grid >= XCDS) grid -= grid % XCDS;
-    hipLaunchKernelGGL(k_star, dim3((uint32_t)grid), dim3(BLOCK), lds, ctx->stream, a);
-    return hipGetLastError();
 }
 
 // group_by_agg_2col / _4col (core/group.cuh:14-36) over plain device columns: one thread per
@@ -465,25 +562,24 @@ hipError_t launch_group_by(golap_ctx *ctx, const GroupByLaunch &l) {
     return hipGetLastError();
 }
 
-// Hashed one-bit-per-key filter over the keys of a table (the driver filter when the key range
-// is too wide for the dense bitmap).
-__global__ void k_hash_filter(const uint64_t *slots, uint64_t n, uint32_t *filter, uint32_t log2bits) {
+// Blocked Bloom filter (2^wlog2 words) over the keys of a table: the driver filter when the key
+// range is too wide for the dense bitmap.
+__global__ void k_hash_filter(const uint64_t *slots, uint64_t n, uint32_t *filter, uint32_t wlog2) {
     for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
          i += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t s = slots[i];
         if (s == 0) continue;
-        const uint32_t key = (uint32_t)s ^ 0x80000000u;
-        const uint32_t bit = hash32(key) >> (32 - log2bits);
-        atomicOr(&filter[bit >> 5], 1u << (bit & 31));
+        const uint32_t h = fmix32((uint32_t)s ^ 0x80000000u);
+        atomicOr(&filter[bloom_word(h, wlog2)], bloom_bits(h));
     }
 }
 
 hipError_t launch_hash_filter(golap_ctx *ctx, const uint64_t *slots, uint32_t log2cap, uint32_t *filter,
-                              uint32_t log2bits) {
+                              uint32_t words_log2) {
     const uint64_t n = 1ull << log2cap;
     const uint32_t grid = (uint32_t)std::min<uint64_t>((n + BLOCK - 1) / BLOCK, 4096);
     hipLaunchKernelGGL(k_hash_filter, dim3(std::max(grid, 1u)), dim3(BLOCK), 0, ctx->stream, slots, n, filter,
-                       log2bits);
+                       words_log2);
     return hipGetLastError();
 }
 
