@@ -51,6 +51,20 @@ def build(verbose=False):
     return LIB
 
 
+def build_cpp_tests(verbose=False):
+    """The C++ facade test program (host code only: the facade calls the C ABI)."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cpp", "facade_q1.cpp")
+    exe = os.path.join(root, "tests", "cpp", "facade_q1")
+    cmd = ["g++", "-std=c++17", "-O2", "-o", exe, src, "-I" + os.path.join(root, "include"),
+           "-I/opt/rocm/include", "-L" + HERE, "-lgolap_hip", "-L" + os.path.join(root, "oracle"), "-loracle",
+           "-Wl,-rpath," + HERE, "-Wl,-rpath," + os.path.join(root, "oracle"), "-pthread"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return exe
+
+
 if __name__ == "__main__":
     build(verbose="-v" in sys.argv)
     print(LIB)

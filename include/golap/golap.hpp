@@ -1,0 +1,462 @@
+// golap.hpp - C++ facade over the C ABI (golap_hip.h) that keeps the reference's names for the
+// compressed-chunk -> filter -> join-probe -> aggregate path of nboeschen/golap_public, so plan
+// code written against core/apps.cuh, core/comp.cuh, core/dev_structs.cuh, core/join.cuh,
+// core/metadata.cuh and core/core.hpp reads the same. Header-only; link libgolap_hip.so.
+//
+// What differs from the reference, by design:
+//  * errors throw golap::Error (the reference aborts in checkCudaErrors);
+//  * compressed chunks stay resident in HBM (no SSD loaders, no cuFile);
+//  * the fused plans (query1, star) take whole chunk ranges in one launch instead of one
+//    kernel per chunk driven from host worker threads; the per-chunk Op protocol remains for
+//    callers that step chunks themselves (ChunkDecompressor, Collector, TableLoader).
+#pragma once
+#include <algorithm>
+#include <cstdint>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+#include "../golap_hip.h"
+
+namespace golap {
+
+struct Error : std::runtime_error {
+    int status;
+    Error(int s, const std::string &what) : std::runtime_error(what), status(s) {}
+};
+
+namespace detail {
+inline void check(int status, golap_ctx *ctx, const char *what) {
+    if (status != GOLAP_OK)
+        throw Error(status, std::string(what) + ": " + golap_status_string(status) + " (" +
+                                (ctx ? golap_last_error(ctx) : "") + ")");
+}
+template <typename T>
+constexpr uint32_t type_of() {
+    if (std::is_same<T, int8_t>::value) return GOLAP_I8;
+    if (std::is_same<T, uint8_t>::value) return GOLAP_U8;
+    if (std::is_same<T, int16_t>::value) return GOLAP_I16;
+    if (std::is_same<T, uint16_t>::value) return GOLAP_U16;
+    if (std::is_same<T, int32_t>::value) return GOLAP_I32;
+    if (std::is_same<T, uint32_t>::value) return GOLAP_U32;
+    return GOLAP_I64;
+}
+}  // namespace detail
+
+// The device context (cudaSetDevice + stream set-up of the reference, 01.hpp:61).
+class Device {
+public:
+    explicit Device(int device = 0) { detail::check(golap_ctx_create(device, &ctx_), nullptr, "ctx_create"); }
+    ~Device() {
+        if (ctx_) golap_ctx_destroy(ctx_);
+    }
+    Device(const Device &) = delete;
+    Device &operator=(const Device &) = delete;
+    golap_ctx *get() const { return ctx_; }
+    void sync() { detail::check(golap_sync(ctx_), ctx_, "sync"); }
+    // golap::DEVICE_ALLOCATED (core/mem.hpp:14-15), per context here
+    uint64_t allocated() const { return golap_device_allocated(ctx_); }
+
+private:
+    golap_ctx *ctx_ = nullptr;
+};
+
+template <typename T>
+struct Tag {};
+
+// DeviceMem (core/mem.hpp:136-219): owning device buffer.
+class DeviceMem {
+public:
+    DeviceMem() = default;
+    template <typename T>
+    DeviceMem(Device &dev, Tag<T>, uint64_t num) : dev_(&dev), bytes_(num * sizeof(T)) {
+        detail::check(golap_dev_alloc(dev.get(), bytes_, &ptr_), dev.get(), "dev_alloc");
+    }
+    DeviceMem(DeviceMem &&o) noexcept { *this = std::move(o); }
+    DeviceMem &operator=(DeviceMem &&o) noexcept {
+        std::swap(dev_, o.dev_);
+        std::swap(ptr_, o.ptr_);
+        std::swap(bytes_, o.bytes_);
+        return *this;
+    }
+    ~DeviceMem() {
+        if (ptr_ && dev_) golap_dev_free(dev_->get(), ptr_);
+    }
+    template <typename T>
+    T *ptr() const { return static_cast<T *>(ptr_); }
+    uint64_t size_bytes() const { return bytes_; }
+    template <typename T>
+    void copy_from(const T *host, uint64_t num) {
+        detail::check(golap_h2d(dev_->get(), ptr_, host, num * sizeof(T)), dev_->get(), "h2d");
+    }
+    template <typename T>
+    void copy_to(T *host, uint64_t num) const {
+        detail::check(golap_d2h(dev_->get(), host, ptr_, num * sizeof(T)), dev_->get(), "d2h");
+    }
+
+private:
+    Device *dev_ = nullptr;
+    void *ptr_ = nullptr;
+    uint64_t bytes_ = 0;
+};
+
+// BlockInfo (core/comp.cuh:23-32): one chunk of a compressed column.
+struct BlockInfo {
+    uint64_t offset;   // byte offset of the chunk's first page in the column buffer
+    uint64_t size;     // bytes of the chunk's pages including alignment gaps
+    uint64_t payload;  // bytes of the chunk's pages
+    uint64_t tuples;   // rows in the chunk
+};
+
+// MinMaxMeta<T> (core/metadata.cuh:110-221): per-chunk zone map, host copy.
+template <typename T>
+struct MinMaxMeta {
+    std::vector<int64_t> mins, maxs;
+    // check_pred_host (metadata.cuh:195-200): chunk qualifies iff lo <= max && hi >= min
+    bool check_pred_host(uint64_t chunk, T lo, T hi) const {
+        return (int64_t)lo <= maxs[chunk] && (int64_t)hi >= mins[chunk];
+    }
+};
+
+// CompInfo (core/comp.cuh:35-78): catalog of one HBM-resident compressed column.
+struct CompInfo {
+    uint64_t chunk_bytes = 0;   // uncompressed bytes per chunk
+    uint64_t uncomp_bytes = 0;
+    std::string comp_algo = "LZ4";
+    uint64_t page_rows = 4096;  // rows per independently coded page (nvcomp's nvchunk role)
+    uint32_t lz4_min_match = 4;
+    uint32_t value_size = 0;    // bytes per value, set by prepare_compressed_device
+    std::vector<BlockInfo> blocks;
+    std::shared_ptr<golap_column> col;  // device column
+
+    uint64_t get_comp_bytes() const {
+        uint64_t s = 0;
+        for (auto &b : blocks) s += b.payload;
+        return s;
+    }
+    uint64_t max_uncomp_chunk() const { return chunk_bytes; }
+};
+
+inline uint32_t codec_of(const std::string &algo) {
+    if (algo == "LZ4") return GOLAP_CODEC_LZ4;
+    if (algo == "DELTA" || algo == "Cascaded") return GOLAP_CODEC_DBP;
+    if (algo == "RLE") return GOLAP_CODEC_RLE;
+    if (algo == "UNCOMPRESSED") return GOLAP_CODEC_RAW;
+    if (algo == "AUTO") return GOLAP_CODEC_AUTO;
+    throw Error(GOLAP_EINVAL, "unknown comp_algo " + algo);
+}
+
+// prepare_compressed_device (core/apps.cuh:257-275): encode `col` into row-aligned chunks of
+// ci.chunk_bytes, upload to HBM, fill ci.blocks and the zone map. Returns the encode time (ms).
+template <typename T>
+double prepare_compressed_device(Device &dev, const T *col, uint64_t num_tuples, CompInfo &ci,
+                                 MinMaxMeta<T> *minmax = nullptr) {
+    golap_encode_opts o;
+    golap_encode_opts_default(&o);
+    o.codec = codec_of(ci.comp_algo);
+    o.page_rows = (uint32_t)ci.page_rows;
+    o.chunk_rows = (uint32_t)(ci.chunk_bytes / sizeof(T));
+    o.lz4_min_match = ci.lz4_min_match;
+    golap_column *c = nullptr;
+    double ms = 0;
+    detail::check(golap_prepare_compressed_device(dev.get(), col, num_tuples, detail::type_of<T>(), &o, &c, &ms),
+                  dev.get(), "prepare_compressed_device");
+    golap_ctx *ctx = dev.get();
+    ci.col = std::shared_ptr<golap_column>(c, [ctx](golap_column *p) { golap_column_free(ctx, p); });
+    ci.uncomp_bytes = num_tuples * sizeof(T);
+    ci.value_size = sizeof(T);
+    golap_column_desc d;
+    golap_column_info(c, &d);
+    // catalog and zone maps back to the host
+    std::vector<uint64_t> off(d.n_pages);
+    std::vector<uint32_t> csz(d.n_pages);
+    golap_d2h(ctx, off.data(), d.page_off, d.n_pages * 8);
+    golap_d2h(ctx, csz.data(), d.page_csize, d.n_pages * 4);
+    const uint64_t ppc = d.chunk_rows / d.page_rows;
+    ci.blocks.clear();
+    for (uint64_t ch = 0; ch < d.n_chunks; ++ch) {
+        const uint64_t p0 = ch * ppc, p1 = std::min<uint64_t>(d.n_pages, p0 + ppc);
+        BlockInfo b{off[p0], off[p1 - 1] + csz[p1 - 1] - off[p0], 0,
+                    std::min<uint64_t>(d.chunk_rows, num_tuples - ch * d.chunk_rows)};
+        for (uint64_t p = p0; p < p1; ++p) b.payload += csz[p];
+        ci.blocks.push_back(b);
+    }
+    if (minmax) {
+        minmax->mins.resize(d.n_chunks);
+        minmax->maxs.resize(d.n_chunks);
+        golap_d2h(ctx, minmax->mins.data(), d.chunk_min, d.n_chunks * 8);
+        golap_d2h(ctx, minmax->maxs.data(), d.chunk_max, d.n_chunks * 8);
+    }
+    return ms;
+}
+
+// ---------------------------------------------------------------- Op protocol (core/core.hpp:94-124)
+class Op {
+public:
+    virtual ~Op() = default;
+    // produce the next chunk; false when exhausted (core.hpp:109)
+    virtual bool step() = 0;
+    // advance without work (pruned chunk, 01.hpp:368-378)
+    virtual bool skip_step() = 0;
+    virtual void finish_step() {}
+    uint64_t last_produced = 0;  // bytes produced by the last step
+};
+
+// ChunkDecompressor (core/comp.cuh:151-216): decodes the next chunk of its block list into its
+// own device buffer (decomp_buf).
+class ChunkDecompressor : public Op {
+public:
+    ChunkDecompressor(Device &dev, const CompInfo &ci, std::vector<uint64_t> block_idxs)
+        : dev_(dev), ci_(ci), idxs_(std::move(block_idxs)),
+          decomp_buf(dev, Tag<uint8_t>{}, ci.chunk_bytes) {}
+    bool step() override {
+        if (cur_ >= idxs_.size()) return false;
+        const uint64_t b = idxs_[cur_++];
+        detail::check(golap_decompress_chunks(dev_.get(), ci_.col.get(), b, b + 1, decomp_buf.ptr<void>(), nullptr),
+                      dev_.get(), "decompress");
+        last_produced = ci_.blocks[b].tuples * ci_.value_size;
+        return true;
+    }
+    bool skip_step() override {
+        if (cur_ >= idxs_.size()) return false;
+        ++cur_;
+        last_produced = 0;
+        return true;
+    }
+    uint64_t current_block() const { return idxs_[cur_ - 1]; }
+    Device &dev_;
+    const CompInfo &ci_;
+    std::vector<uint64_t> idxs_;
+    uint64_t cur_ = 0;
+    DeviceMem decomp_buf;
+};
+
+// Collector (core/core.hpp:252-307): steps every child column chain for the same chunk.
+class Collector : public Op {
+public:
+    void add(Op *child) { children.push_back(child); }
+    bool step() override {
+        bool any = false;
+        for (Op *c : children) any = c->step() || any;
+        return any;
+    }
+    bool skip_step() override {
+        bool any = false;
+        for (Op *c : children) any = c->skip_step() || any;
+        return any;
+    }
+    std::vector<Op *> children;
+};
+
+// DecompressEnv (core/apps.cuh:341-396) for one column of one worker's block slice.
+struct DecompressEnv {
+    DecompressEnv(Device &dev, const std::vector<uint64_t> &all_block_idxs, uint64_t start, uint64_t past_end,
+                  const CompInfo &ci)
+        : myblock_idxs(all_block_idxs.begin() + start, all_block_idxs.begin() + past_end),
+          myblocks(past_end - start), root(dev, ci, myblock_idxs) {}
+    std::vector<uint64_t> myblock_idxs;
+    uint64_t myblocks;
+    ChunkDecompressor root;
+    DeviceMem &decomp_buf() { return root.decomp_buf; }
+};
+
+// TableLoader<BLOCKENV> (core/apps.cuh:611-630): one env per column, stepped together.
+template <typename BLOCKENV>
+struct TableLoader {
+    explicit TableLoader(Device &dev, uint32_t reserve = 4) : dev_(dev) { blockenvs.reserve(reserve); }
+    void add(const std::string &name, const std::vector<uint64_t> &all_block_idxs, uint64_t start,
+             uint64_t past_end, const CompInfo &ci) {
+        names.push_back(name);
+        blockenvs.emplace_back(new BLOCKENV(dev_, all_block_idxs, start, past_end, ci));
+        rootop.add(&blockenvs.back()->root);
+    }
+    Device &dev_;
+    std::vector<std::string> names;
+    std::vector<std::unique_ptr<BLOCKENV>> blockenvs;
+    Collector rootop;
+};
+
+// ---------------------------------------------------------------- joins (dev_structs.cuh, join.cuh)
+// PredInfo<T, ConstPred> (dev_structs.cuh:21-33) as an inclusive range over a device column.
+template <typename T>
+struct PredInfo {
+    const T *col;
+    int64_t lo, hi;
+};
+template <typename T>
+PredInfo<T> ConstPred(const T *col, T c) { return PredInfo<T>{col, (int64_t)c, (int64_t)c}; }
+
+// HashMap(ht_size, table) (dev_structs.cuh:47-113) over a device key column.
+class HashMap {
+public:
+    template <typename KEY>
+    HashMap(Device &dev, uint64_t ht_size, const KEY *dev_table) : dev_(dev), ht_size(ht_size) {
+        detail::check(golap_hashmap_create(dev.get(), ht_size, dev_table, detail::type_of<KEY>(), &m_), dev.get(),
+                      "hashmap_create");
+    }
+    ~HashMap() {
+        if (m_) golap_hashmap_free(dev_.get(), m_);
+    }
+    HashMap(const HashMap &) = delete;
+    HashMap &operator=(const HashMap &) = delete;
+    uint64_t filled() const {
+        uint64_t f = 0;
+        golap_hashmap_filled(dev_.get(), m_, &f);
+        return f;
+    }
+    // probe n keys: rid (or payload) per key, UINT64_MAX when absent (HashMap::probe)
+    template <typename KEY>
+    void probe(const KEY *dev_keys, uint64_t n, uint64_t *dev_out) {
+        detail::check(golap_hashmap_probe(dev_.get(), m_, dev_keys, detail::type_of<KEY>(), n, dev_out), dev_.get(),
+                      "probe");
+    }
+    golap_hashmap *get() const { return m_; }
+    Device &dev_;
+    uint64_t ht_size;
+
+private:
+    golap_hashmap *m_ = nullptr;
+};
+
+namespace detail {
+inline void collect_preds(std::vector<golap_pred> &) {}
+template <typename T, typename... REST>
+void collect_preds(std::vector<golap_pred> &v, const PredInfo<T> &p, const REST &...rest) {
+    v.push_back(golap_pred{p.col, type_of<T>(), p.lo, p.hi});
+    collect_preds(v, rest...);
+}
+}  // namespace detail
+
+// hash_map_build_pred_other (core/join.cuh:131-135): insert every row passing all predicates.
+template <typename... PREDS>
+void hash_map_build_pred_other(HashMap &m, uint64_t num_tuples, const PREDS &...preds) {
+    std::vector<golap_pred> v;
+    detail::collect_preds(v, preds...);
+    detail::check(golap_hashmap_build(m.dev_.get(), m.get(), num_tuples, v.data(), (uint32_t)v.size(), nullptr),
+                  m.dev_.get(), "hash_map_build_pred_other");
+}
+// Same, storing a per-row int32 payload (a group attribute) beside each key.
+template <typename... PREDS>
+void hash_map_build_pred_payload(HashMap &m, uint64_t num_tuples, const int32_t *dev_payload, const PREDS &...preds) {
+    std::vector<golap_pred> v;
+    detail::collect_preds(v, preds...);
+    detail::check(golap_hashmap_build(m.dev_.get(), m.get(), num_tuples, v.data(), (uint32_t)v.size(), dev_payload),
+                  m.dev_.get(), "hash_map_build");
+}
+// hash_join_count (core/join.cuh:45-56)
+template <typename KEY>
+uint64_t hash_join_count(HashMap &m, const KEY *dev_probe, uint64_t num) {
+    uint64_t hits = 0;
+    detail::check(golap_hash_join_count(m.dev_.get(), m.get(), dev_probe, detail::type_of<KEY>(), num, &hits),
+                  m.dev_.get(), "hash_join_count");
+    return hits;
+}
+
+// ---------------------------------------------------------------- pruning (metadata.cuh)
+// MetaChecker (core/metadata.cuh:638-698): AND of zone-map predicates into one u16 chunk mask.
+class MetaChecker {
+public:
+    MetaChecker(Device &dev, uint64_t chunk_num)
+        : dev_(dev), chunk_num(chunk_num), combined(dev, Tag<uint16_t>{}, chunk_num),
+          tmp(dev, Tag<uint16_t>{}, chunk_num) {
+        detail::check(golap_memset(dev.get(), combined.ptr<void>(), 0, chunk_num * 2), dev.get(), "memset");
+    }
+    // check_mmmeta + combine_and (metadata.cuh:539-547, 632-636)
+    template <typename T>
+    void add_minmax(const CompInfo &ci, T lo, T hi) {
+        uint16_t *dst = first_ ? combined.ptr<uint16_t>() : tmp.ptr<uint16_t>();
+        detail::check(golap_minmax_check(dev_.get(), ci.col.get(), (int64_t)lo, (int64_t)hi, dst), dev_.get(),
+                      "minmax_check");
+        if (!first_)
+            detail::check(golap_combine_and(dev_.get(), combined.ptr<uint16_t>(), tmp.ptr<uint16_t>(), chunk_num),
+                          dev_.get(), "combine_and");
+        first_ = false;
+    }
+    const uint16_t *mask() const { return first_ ? nullptr : combined.ptr<uint16_t>(); }
+    std::vector<uint16_t> to_host() const {
+        std::vector<uint16_t> h(chunk_num, 1);
+        if (!first_) combined.copy_to(h.data(), chunk_num);
+        return h;
+    }
+    Device &dev_;
+    uint64_t chunk_num;
+
+private:
+    DeviceMem combined, tmp;
+    bool first_ = true;
+};
+
+// ---------------------------------------------------------------- plans
+// query1 (usecases/ssb/queries/01.hpp:53-449) fused: SUM(price*discount) over the chunk range.
+struct Q1Result {
+    int64_t sum;
+    uint64_t matches, chunks_run, chunks_pruned, comp_bytes;
+    double kernel_ms;
+};
+inline Q1Result query1(Device &dev, const CompInfo &orderdate, const CompInfo &discount, const CompInfo &quantity,
+                       const CompInfo &extendedprice, const HashMap &date_map, int64_t disc_lo, int64_t disc_hi,
+                       int64_t qty_lo, int64_t qty_hi, const MetaChecker *checker = nullptr,
+                       int64_t price_lo = INT32_MIN, int64_t price_hi = INT32_MAX) {
+    golap_q1_plan p{};
+    p.date = orderdate.col.get();
+    p.disc = discount.col.get();
+    p.qty = quantity.col.get();
+    p.price = extendedprice.col.get();
+    p.date_map = date_map.get();
+    p.disc_lo = disc_lo;
+    p.disc_hi = disc_hi;
+    p.qty_lo = qty_lo;
+    p.qty_hi = qty_hi;
+    p.price_lo = price_lo;
+    p.price_hi = price_hi;
+    p.chunk_begin = 0;
+    p.chunk_end = orderdate.blocks.size();
+    p.dev_chunk_mask = checker ? checker->mask() : nullptr;
+    golap_q1_result r{};
+    detail::check(golap_q1_run(dev.get(), &p, &r), dev.get(), "query1");
+    return Q1Result{r.sum, r.matches, r.chunks_run, r.chunks_pruned, r.comp_bytes, r.kernel_ms};
+}
+
+// HashAggregate result of a grouping plan (dev_structs.cuh:157-253), canonical order.
+struct GroupResult {
+    std::vector<std::vector<int32_t>> keys;
+    std::vector<int64_t> sums;
+    std::vector<uint64_t> counts;
+};
+
+// query2 / query3 (02.hpp, 03.hpp) fused star join + group-by.
+struct StarDim {
+    const CompInfo *fk;
+    const HashMap *map;
+    int32_t group_lo = 0;
+    uint32_t group_card = 0;
+};
+inline GroupResult star_query(Device &dev, const std::vector<StarDim> &dims, const CompInfo &value,
+                              const MetaChecker *checker = nullptr, uint64_t max_groups = 1 << 20) {
+    golap_star_plan p{};
+    p.n_dims = (uint32_t)dims.size();
+    uint32_t na = 0;
+    for (size_t i = 0; i < dims.size(); ++i) {
+        p.dims[i] = golap_star_dim{dims[i].fk->col.get(), dims[i].map->get(), dims[i].group_lo, dims[i].group_card};
+        na += dims[i].group_card ? 1 : 0;
+    }
+    p.value = value.col.get();
+    p.chunk_end = value.blocks.size();
+    p.dev_chunk_mask = checker ? checker->mask() : nullptr;
+    golap_star_result r{};
+    std::vector<int32_t> k(max_groups * std::max(1u, na));
+    GroupResult out;
+    out.sums.resize(max_groups);
+    out.counts.resize(max_groups);
+    detail::check(golap_star_run(dev.get(), &p, &r, k.data(), out.sums.data(), out.counts.data(), max_groups),
+                  dev.get(), "star_query");
+    out.sums.resize(r.n_groups);
+    out.counts.resize(r.n_groups);
+    for (uint64_t g = 0; g < r.n_groups; ++g) out.keys.emplace_back(k.begin() + g * na, k.begin() + (g + 1) * na);
+    return out;
+}
+
+}  // namespace golap

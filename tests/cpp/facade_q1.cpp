@@ -1,0 +1,113 @@
+// The C++ facade used the way the reference's plan code uses its operator API
+// (usecases/ssb/queries/01.hpp:53-449): prepare_compressed_device per column with MinMaxMeta,
+// HashMap + hash_map_build_pred_other on the date dimension, MetaChecker pruning, the fused
+// query1, and a TableLoader<DecompressEnv> stepping every chunk (the select path,
+// core/hl/select.cuh:11-83). Results are checked against the oracle (oracle/liboracle.so).
+// Prints "OK" and exits 0 on success.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <numeric>
+#include <vector>
+
+#include "../../include/golap/golap.hpp"
+#include "../../oracle/oracle.h"
+
+#define REQUIRE(c)                                                        \
+    do {                                                                  \
+        if (!(c)) {                                                       \
+            std::printf("FAIL %s:%d: %s\n", __FILE__, __LINE__, #c);      \
+            return 1;                                                     \
+        }                                                                 \
+    } while (0)
+
+int main() {
+    const uint64_t n = 3'000'000;
+    std::vector<int32_t> d_key(2556), date(n), price(n);
+    std::vector<int16_t> d_year(2556);
+    std::vector<uint8_t> d_month(2556), disc(n), qty(n);
+    golap_gen_date_dim(2556, d_key.data(), d_year.data(), d_month.data());
+    golap_gen_q1_fact(7, 0, n, d_key.data(), 2556, date.data(), disc.data(), qty.data(), price.data(), 0);
+    // sort by date so zone maps prune (SortHelper's dimension sort, usecases/ssb/helper.hpp)
+    std::vector<uint32_t> ord(n);
+    std::iota(ord.begin(), ord.end(), 0);
+    std::stable_sort(ord.begin(), ord.end(), [&](uint32_t a, uint32_t b) { return date[a] < date[b]; });
+    auto permute = [&](auto &v) {
+        auto c = v;
+        for (uint64_t i = 0; i < n; ++i) v[i] = c[ord[i]];
+    };
+    permute(date);
+    permute(disc);
+    permute(qty);
+    permute(price);
+
+    golap::Device dev(0);
+    const uint64_t tuples_per_chunk = (1 << 20) / sizeof(int32_t);  // 01.hpp:62-63
+    golap::CompInfo ci_date, ci_disc, ci_qty, ci_price;
+    for (auto *ci : {&ci_date, &ci_price}) ci->chunk_bytes = tuples_per_chunk * 4;
+    for (auto *ci : {&ci_disc, &ci_qty}) ci->chunk_bytes = tuples_per_chunk * 1;
+    for (auto *ci : {&ci_date, &ci_disc, &ci_qty, &ci_price}) ci->lz4_min_match = 16;
+    golap::MinMaxMeta<int32_t> mm_date;
+    golap::MinMaxMeta<uint8_t> mm_qty;
+    golap::prepare_compressed_device(dev, date.data(), n, ci_date, &mm_date);
+    golap::prepare_compressed_device(dev, disc.data(), n, ci_disc);
+    golap::prepare_compressed_device(dev, qty.data(), n, ci_qty, &mm_qty);
+    golap::prepare_compressed_device(dev, price.data(), n, ci_price);
+    const uint64_t chunk_num = ci_date.blocks.size();
+    REQUIRE(chunk_num == (n + tuples_per_chunk - 1) / tuples_per_chunk);  // apps_tests.cu:12-65
+    REQUIRE(ci_disc.blocks.size() == chunk_num);
+
+    golap::DeviceMem dk(dev, golap::Tag<int32_t>{}, 2556), dy(dev, golap::Tag<int16_t>{}, 2556);
+    dk.copy_from(d_key.data(), 2556);
+    dy.copy_from(d_year.data(), 2556);
+    golap::HashMap date_map(dev, 2556, dk.ptr<int32_t>());
+    golap::hash_map_build_pred_other(date_map, 2556, golap::ConstPred<int16_t>(dy.ptr<int16_t>(), 1995));
+    REQUIRE(date_map.filled() == 365);
+
+    golap::MetaChecker checker(dev, chunk_num);
+    checker.add_minmax<int32_t>(ci_date, 19950101, 19951231);
+    checker.add_minmax<uint8_t>(ci_qty, 30, 50);
+    auto mask = checker.to_host();
+    uint64_t keep = 0;
+    for (uint64_t c = 0; c < chunk_num; ++c) {
+        const bool want = mm_date.check_pred_host(c, 19950101, 19951231) && mm_qty.check_pred_host(c, 30, 50);
+        REQUIRE((mask[c] != 0) == want);
+        keep += want;
+    }
+    REQUIRE(keep > 0 && keep < chunk_num);
+
+    auto full = golap::query1(dev, ci_date, ci_disc, ci_qty, ci_price, date_map, 4, 6, 30, 50);
+    auto pruned = golap::query1(dev, ci_date, ci_disc, ci_qty, ci_price, date_map, 4, 6, 30, 50, &checker);
+    oracle_pred pred{d_year.data(), 3, 1995, 1995};
+    oracle_q1_consts qc{4, 6, 30, 50, INT32_MIN, INT32_MAX};
+    uint64_t want_matches = 0;
+    const int64_t want = oracle_q1_plan(d_key.data(), 2556, &pred, 1, date.data(), disc.data(), qty.data(),
+                                        price.data(), n, &qc, 4, &want_matches);
+    REQUIRE(full.sum == want && full.matches == want_matches);
+    REQUIRE(pruned.sum == want && pruned.matches == want_matches);
+    REQUIRE(pruned.chunks_pruned == chunk_num - keep);
+
+    // select path: step every chunk of every column through a TableLoader
+    std::vector<uint64_t> all(chunk_num);
+    std::iota(all.begin(), all.end(), 0);
+    golap::TableLoader<golap::DecompressEnv> loader(dev);
+    loader.add("lo_orderdate", all, 0, chunk_num, ci_date);
+    loader.add("lo_discount", all, 0, chunk_num, ci_disc);
+    std::vector<int32_t> chunk_d(tuples_per_chunk);
+    std::vector<uint8_t> chunk_s(tuples_per_chunk);
+    uint64_t c = 0;
+    while (loader.rootop.step()) {
+        const uint64_t rows = ci_date.blocks[c].tuples;
+        loader.blockenvs[0]->decomp_buf().copy_to(chunk_d.data(), tuples_per_chunk);
+        loader.blockenvs[1]->decomp_buf().copy_to(chunk_s.data(), tuples_per_chunk);
+        REQUIRE(!std::memcmp(chunk_d.data(), date.data() + c * tuples_per_chunk, rows * 4));
+        REQUIRE(!std::memcmp(chunk_s.data(), disc.data() + c * tuples_per_chunk, rows));
+        REQUIRE(loader.blockenvs[0]->root.last_produced == rows * 4);
+        ++c;
+    }
+    REQUIRE(c == chunk_num);
+    std::printf("OK sum=%lld matches=%llu chunks=%llu pruned=%llu\n", (long long)full.sum,
+                (unsigned long long)full.matches, (unsigned long long)chunk_num,
+                (unsigned long long)pruned.chunks_pruned);
+    return 0;
+}
