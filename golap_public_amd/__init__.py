@@ -73,6 +73,22 @@ class StarResult(C.Structure):
 AGG_AUTO, AGG_LDS, AGG_GLOBAL = 0, 1, 2
 
 
+class BatchQuery(C.Structure):
+    _fields_ = [("plan", StarPlan), ("zone_col", vp), ("zone_lo", i64), ("zone_hi", i64)]
+
+
+class BatchConfig(C.Structure):
+    _fields_ = [("pages_per_item", u32), ("lds_group_slots", u32), ("lds_cutover", u32), ("filter_lds", u32),
+                ("compact_tables", u32), ("max_blocks_per_cu", u32)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+class BatchRunInfo(C.Structure):
+    _fields_ = [("ms", C.c_double), ("items", u64), ("pages", u64), ("comp_bytes", u64)]
+
+
 class Q1Result(C.Structure):
     _fields_ = [("sum", i64), ("matches", u64), ("chunks_run", u64), ("chunks_pruned", u64),
                 ("comp_bytes", u64), ("kernel_ms", C.c_double)]
@@ -117,6 +133,13 @@ _PROTOS = {
     "golap_group_by": (C.c_int, [vp, C.POINTER(GroupKey), u32, vp, u32, u64, u32, C.POINTER(u64), vp, vp, vp,
                                  u64]),
     "golap_star_run": (C.c_int, [vp, C.POINTER(StarPlan), C.POINTER(StarResult), vp, vp, vp, u64]),
+    "golap_star_batch_create": (C.c_int, [vp, C.POINTER(BatchQuery), u32, C.POINTER(vp)]),
+    "golap_star_batch_config_get": (C.c_int, [vp, C.POINTER(BatchConfig)]),
+    "golap_star_batch_config_set": (C.c_int, [vp, C.POINTER(BatchConfig)]),
+    "golap_star_batch_run": (C.c_int, [vp, u32, C.POINTER(BatchRunInfo)]),
+    "golap_star_batch_tune": (C.c_int, [vp, u32, C.POINTER(BatchConfig), C.POINTER(C.c_double)]),
+    "golap_star_batch_result": (C.c_int, [vp, u32, C.POINTER(StarResult), vp, vp, vp, u64]),
+    "golap_star_batch_free": (C.c_int, [vp]),
     "golap_gen_star_fact": (C.c_int, [u64, u64, u64, u64, vp, u32, u32, u32, u32, vp, vp, vp, vp, vp, u32]),
     "golap_gen_uniform_i32": (C.c_int, [u64, u32, u64, C.c_int32, C.c_int32, vp, u32]),
     "golap_gen_date_dim": (C.c_int, [u32, vp, vp, vp]),
@@ -373,9 +396,8 @@ class Context:
         n = ng.value
         return kout[:n * len(keys)].reshape(n, len(keys)), sums[:n], cnts[:n]
 
-    def star_run(self, dims, value, chunk_begin=0, chunk_end=None, dev_chunk_mask=None, agg_mode=AGG_AUTO,
-                 lds_group_slots=0, max_groups=1 << 20):
-        """dims: list of (fk column, map, group_lo, group_card). Returns (result, keys, sums, counts)."""
+    def star_plan(self, dims, value, chunk_begin=0, chunk_end=None, dev_chunk_mask=None, agg_mode=AGG_AUTO,
+                  lds_group_slots=0):
         if chunk_end is None:
             chunk_end = self.column_desc(value).n_chunks
         p = StarPlan()
@@ -384,6 +406,12 @@ class Context:
         p.n_dims = len(dims)
         p.value, p.chunk_begin, p.chunk_end = value, chunk_begin, chunk_end
         p.dev_chunk_mask, p.agg_mode, p.lds_group_slots = dev_chunk_mask, agg_mode, lds_group_slots
+        return p
+
+    def star_run(self, dims, value, chunk_begin=0, chunk_end=None, dev_chunk_mask=None, agg_mode=AGG_AUTO,
+                 lds_group_slots=0, max_groups=1 << 20):
+        """dims: list of (fk column, map, group_lo, group_card). Returns (result, keys, sums, counts)."""
+        p = self.star_plan(dims, value, chunk_begin, chunk_end, dev_chunk_mask, agg_mode, lds_group_slots)
         na = sum(1 for d in dims if d[3])
         kout = np.zeros(max_groups * max(na, 1), np.int32)
         sums = np.zeros(max_groups, np.int64)
@@ -393,6 +421,10 @@ class Context:
                                            max_groups), "star_run")
         n = r.n_groups
         return r, kout[:n * na].reshape(n, na), sums[:n], cnts[:n]
+
+    def star_batch(self, queries):
+        """queries: list of dicts with star_plan() arguments plus optional zone=(column, lo, hi)."""
+        return StarBatch(self, queries)
 
     # plans
     def q1_run(self, date, disc, qty, price, date_map, disc_rng, qty_rng, price_rng=(-2**31, 2**31 - 1),
@@ -404,6 +436,69 @@ class Context:
         r = Q1Result()
         self.check(self.lib.golap_q1_run(self.h, C.byref(p), C.byref(r)), "q1_run")
         return r
+
+
+class StarBatch:
+    """Many star plans over one fact table in one persistent launch (golap_star_batch_*)."""
+
+    def __init__(self, ctx, queries):
+        self.ctx, self.h = ctx, None
+        arr = (BatchQuery * len(queries))()
+        self.n_attrs = []
+        for i, q in enumerate(queries):
+            q = dict(q)
+            zone = q.pop("zone", None)
+            arr[i].plan = ctx.star_plan(**q)
+            if zone is not None:
+                arr[i].zone_col, arr[i].zone_lo, arr[i].zone_hi = zone
+            self.n_attrs.append(sum(1 for d in q["dims"] if d[3]))
+        h = vp()
+        ctx.check(ctx.lib.golap_star_batch_create(ctx.h, arr, len(queries), C.byref(h)), "star_batch_create")
+        self.h = h.value
+        self.n = len(queries)
+
+    def config(self):
+        c = BatchConfig()
+        self.ctx.check(self.ctx.lib.golap_star_batch_config_get(self.h, C.byref(c)), "batch_config_get")
+        return c
+
+    def set_config(self, **kw):
+        c = self.config()
+        for k, v in kw.items():
+            setattr(c, k, v)
+        self.ctx.check(self.ctx.lib.golap_star_batch_config_set(self.h, C.byref(c)), "batch_config_set")
+
+    def run(self, use_graph=True):
+        info = BatchRunInfo()
+        self.ctx.check(self.ctx.lib.golap_star_batch_run(self.h, int(use_graph), C.byref(info)), "batch_run")
+        return info
+
+    def tune(self, reps=3):
+        c, ms = BatchConfig(), C.c_double()
+        self.ctx.check(self.ctx.lib.golap_star_batch_tune(self.h, reps, C.byref(c), C.byref(ms)), "batch_tune")
+        return c, ms.value
+
+    def result(self, q, max_groups=1 << 20):
+        na = self.n_attrs[q]
+        kout = np.zeros(max_groups * max(na, 1), np.int32)
+        sums = np.zeros(max_groups, np.int64)
+        cnts = np.zeros(max_groups, np.uint64)
+        r = StarResult()
+        self.ctx.check(self.ctx.lib.golap_star_batch_result(self.h, q, C.byref(r), _ptr(kout), _ptr(sums),
+                                                            _ptr(cnts), max_groups), "batch_result")
+        n = r.n_groups
+        return r, kout[:n * na].reshape(n, na), sums[:n], cnts[:n]
+
+    def free(self):
+        if self.h:
+            self.ctx.check(self.ctx.lib.golap_star_batch_free(self.h), "batch_free")
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.free()
 
 
 def gen_date_dim(n_days):

@@ -1,21 +1,23 @@
-// C ABI: group-by and the fused star join + group-by plan (include/golap_hip.h). Host side of
-// query2 / query3 (usecases/ssb/queries/02.hpp:54-307, 03.hpp:79-677): choose the driving
-// dimension and its LDS filter, the LDS windows for coded columns and the aggregation table,
-// launch one fused kernel, and return the canonical (sorted) group list.
+// C ABI: group-by, the fused star join + group-by plan and its batched form (include/golap_hip.h).
+// Host side of query2 / query3 (usecases/ssb/queries/02.hpp:54-307, 03.hpp:79-677): choose the
+// driving dimension and its LDS filter, the LDS windows for coded columns and the aggregation
+// table, launch one persistent fused kernel, and return the canonical (sorted) group list.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
 #include "format.hpp"
+#include "star_host.hpp"
 
 using namespace golap_impl;
+using namespace golap_impl::starhost;
 using Lock = std::lock_guard<std::recursive_mutex>;
 
-namespace {
-
-constexpr uint64_t MAX_GROUPS = 1ull << 31;
-constexpr uint32_t LDS_GROUP_SLOTS_MAX = 4096;  // 64 KiB of LDS (key, count, int64 sum)
+namespace golap_impl {
+namespace starhost {
 
 uint32_t pow2_at_least(uint64_t v) {
     uint32_t p = 1;
@@ -47,21 +49,23 @@ uint32_t choose_lds_slots(uint32_t agg_mode, uint32_t requested, uint64_t n_grou
 }
 
 // Copies the dense result back and emits the groups with rows, in index (= key tuple) order.
+// *rows (optional) = the sum of the counts = the rows that reached the aggregation.
 int emit_groups(golap_ctx *ctx, uint64_t n, const int64_t *d_sum, const uint64_t *d_cnt,
                 const std::vector<int32_t> &lo, const std::vector<uint32_t> &card,
                 const std::vector<uint64_t> &stride, uint64_t *n_groups, int64_t *total,
-                int32_t *keys, int64_t *sums, uint64_t *counts, uint64_t max_groups) {
+                int32_t *keys, int64_t *sums, uint64_t *counts, uint64_t max_groups, uint64_t *rows) {
     std::vector<int64_t> hs(n);
     std::vector<uint64_t> hc(n);
     GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(hs.data(), d_sum, n * 8, hipMemcpyDeviceToHost, ctx->stream));
     GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(hc.data(), d_cnt, n * 8, hipMemcpyDeviceToHost, ctx->stream));
     GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    uint64_t g = 0;
+    uint64_t g = 0, nr = 0;
     int64_t t = 0;
     const size_t na = card.size();
     for (uint64_t i = 0; i < n; ++i) {
         if (!hc[i]) continue;
         t += hs[i];
+        nr += hc[i];
         if (g < max_groups) {
             for (size_t a = 0; a < na && keys; ++a)
                 keys[g * na + a] = lo[a] + (int32_t)((i / stride[a]) % card[a]);
@@ -72,13 +76,186 @@ int emit_groups(golap_ctx *ctx, uint64_t n, const int64_t *d_sum, const uint64_t
     }
     *n_groups = g;
     if (total) *total = t;
+    if (rows) *rows = nr;
     if (g > max_groups) return fail(ctx, GOLAP_ECAPACITY, "more groups than max_groups");
     return GOLAP_OK;
 }
 
 bool is_int_type(uint32_t t) { return t >= GOLAP_I8 && t <= GOLAP_I64; }
 
-}  // namespace
+// Smallest b with 2^b >= n.
+uint32_t bits_for(uint64_t n) {
+    uint32_t b = 0;
+    while (b < 63 && (1ull << b) < n) ++b;
+    return b;
+}
+
+
+
+// Blocked Bloom filter of a map's keys, cached until the map is rebuilt.
+int ensure_hfilter(golap_ctx *ctx, const golap_hashmap *dm) {
+    const uint32_t words = 1u << HASH_FILTER_WORDS_LOG2;
+    if (!dm->d_hfilter) {
+        void *p = nullptr;
+        if (int r = dev_alloc(ctx, words * 4, &p)) return r;
+        dm->d_hfilter = static_cast<uint32_t *>(p);
+        dm->hfilter_build = 0;
+    }
+    if (dm->hfilter_build != dm->build_id) {
+        GOLAP_HIP_CHECK(ctx, hipMemsetAsync(dm->d_hfilter, 0, words * 4, ctx->stream));
+        GOLAP_HIP_CHECK(ctx, launch_hash_filter(ctx, dm->small_log2 ? dm->d_small : dm->d_slots,
+                                                dm->small_log2 ? dm->small_log2 : dm->log2cap, dm->d_hfilter,
+                                                HASH_FILTER_WORDS_LOG2));
+        dm->hfilter_build = dm->build_id;
+    }
+    return GOLAP_OK;
+}
+
+// Validates a plan and fills its device descriptor (all but use_lds, item_begin and the result
+// pointers): fact column indexes, driver, probe tables, driver filter, slice and group strides.
+int prepare_query(golap_ctx *ctx, const golap_star_plan *plan, bool compact, ColumnSet &cs, StarQueryDev &qd,
+                  QueryHost &qh) {
+    if (!plan || plan->n_dims < 1 || plan->n_dims > STAR_MAX_DIMS || !plan->value || plan->agg_mode > GOLAP_AGG_GLOBAL)
+        return fail(ctx, GOLAP_EINVAL, "star: bad plan");
+    const uint32_t nd = plan->n_dims;
+    const golap_column *v = plan->value;
+    auto is32 = [](const golap_column *c) { return c->type == GOLAP_I32 || c->type == GOLAP_U32; };
+    if (v->ctx != ctx || !is32(v)) return fail(ctx, GOLAP_EINVAL, "star: value column must be 32-bit");
+    for (uint32_t d = 0; d < nd; ++d) {
+        const golap_star_dim &sd = plan->dims[d];
+        if (!sd.fk || !sd.map || sd.fk->ctx != ctx || sd.map->ctx != ctx || !sd.map->built)
+            return fail(ctx, GOLAP_EINVAL, "star: missing column or unbuilt map");
+        if (!is32(sd.fk)) return fail(ctx, GOLAP_EINVAL, "star: foreign keys must be 32-bit");
+        if (sd.fk->n_rows != v->n_rows || sd.fk->page_rows != v->page_rows || sd.fk->chunk_rows != v->chunk_rows)
+            return fail(ctx, GOLAP_EINVAL, "star: columns are not row-aligned");
+        if (sd.fk->type == GOLAP_U32)
+            for (int64_t mx : sd.fk->chunk_max)
+                if (mx > INT32_MAX) return fail(ctx, GOLAP_EINVAL, "star: uint32 keys above INT32_MAX");
+    }
+    if (v->page_rows != 1024 && v->page_rows != 2048 && v->page_rows != 4096 && v->page_rows != 8192)
+        return fail(ctx, GOLAP_EINVAL, "star: page_rows must be 1024, 2048, 4096 or 8192");
+    if (plan->chunk_begin > plan->chunk_end || plan->chunk_end > v->n_chunks)
+        return fail(ctx, GOLAP_EINVAL, "star: bad chunk range");
+    qh = QueryHost{};
+    qh.plan = *plan;
+    memset(&qd, 0, sizeof(qd));
+    // group domain, attributes in dimension order (first = most significant)
+    for (uint32_t d = 0; d < nd; ++d)
+        if (plan->dims[d].group_card) {
+            qh.lo.push_back(plan->dims[d].group_lo);
+            qh.card.push_back(plan->dims[d].group_card);
+            qh.domain *= plan->dims[d].group_card;
+            if (qh.domain >= MAX_GROUPS) return fail(ctx, GOLAP_EINVAL, "star: group domain too large");
+        }
+    qh.stride.resize(qh.card.size());
+    for (int a = (int)qh.card.size() - 1, s = 1; a >= 0; --a) {
+        qh.stride[a] = (uint64_t)s;
+        s *= (int)qh.card[a];
+    }
+    // driver: the dimension whose filter passes the smallest share of its key domain
+    double best = 2.0;
+    for (uint32_t d = 0; d < nd; ++d) {
+        const golap_hashmap *m = plan->dims[d].map;
+        const double sel = m->ht_size ? (double)m->filled / (double)m->ht_size : 1.0;
+        qh.sel *= sel;
+        qh.builds.emplace_back(m, m->build_id);
+        if (sel < best) {
+            best = sel;
+            qd.driver = d;
+        }
+    }
+    size_t a_idx = 0;
+    for (uint32_t d = 0; d < nd; ++d) {
+        const golap_star_dim &sd = plan->dims[d];
+        const golap_hashmap *m = sd.map;
+        if (int r = cs.index(ctx, sd.fk, &qd.fk_col[d])) return r;
+        const bool small = compact && m->small_log2;
+        qd.dim[d].slots = small ? m->d_small : m->d_slots;
+        qd.dim[d].log2cap = small ? m->small_log2 : m->log2cap;
+        qd.dim[d].group_lo = sd.group_lo;
+        qd.dim[d].group_card = sd.group_card;
+        qd.dim[d].group_stride = sd.group_card ? qh.stride[a_idx++] : 0;
+    }
+    if (int r = cs.index(ctx, v, &qd.val_col)) return r;
+    qd.n_dims = (uint8_t)nd;
+    qd.val_signed = v->type == GOLAP_I32;
+    // driver filter: dense exact bitmap, or the blocked Bloom filter of the map's keys
+    const golap_hashmap *dm = plan->dims[qd.driver].map;
+    if (dm->use_bitmap && (dm->krange + 31) / 32 <= (1u << HASH_FILTER_WORDS_LOG2)) {
+        qd.filter = DriverFilterView{dm->d_bitmap, dm->kmin, dm->krange, 0};
+        qd.filter_words = (dm->krange + 31) / 32;
+    } else {
+        if (int r = ensure_hfilter(ctx, dm)) return r;
+        qd.filter = DriverFilterView{dm->d_hfilter, 0, 0, HASH_FILTER_WORDS_LOG2};
+        qd.filter_words = 1u << HASH_FILTER_WORDS_LOG2;
+    }
+    const uint64_t ppc = v->chunk_rows / v->page_rows;
+    qd.chunk_begin = plan->chunk_begin;
+    qd.chunk_end = plan->chunk_end;
+    qd.page_begin = plan->chunk_begin * ppc;
+    qd.page_end = std::min<uint64_t>(v->n_pages, plan->chunk_end * ppc);
+    qd.chunk_mask = plan->dev_chunk_mask;
+    qd.n_groups = qh.domain;
+    return GOLAP_OK;
+}
+
+// LDS windows: a column gets one when some page of [pb, pe) is neither read in place nor
+// point-readable.
+uint32_t assign_windows(const ColumnSet &cs, uint64_t pb, uint64_t pe, int *win) {
+    std::lock_guard<std::mutex> g(g_extra_mu);
+    uint32_t n = 0;
+    for (size_t c = 0; c < cs.cols.size(); ++c) {
+        auto &ex = g_extra[cs.cols[c]];
+        win[c] = ex.window_prefix[pe] != ex.window_prefix[pb] ? (int)n++ : -1;
+    }
+    return n;
+}
+
+// chunks_run / chunks_pruned / comp_bytes of a plan over its slice, given the chunk flags
+// (empty = all run).
+void slice_stats(const golap_star_plan &plan, const std::vector<uint8_t> &run, golap_star_result *res) {
+    std::lock_guard<std::mutex> g(g_extra_mu);
+    for (uint64_t ch = plan.chunk_begin; ch < plan.chunk_end; ++ch) {
+        if (!run.empty() && !run[ch - plan.chunk_begin]) {
+            res->chunks_pruned++;
+            continue;
+        }
+        res->chunks_run++;
+        auto add = [&](const golap_column *c) {
+            auto &ex = g_extra[c];
+            res->comp_bytes += ex.chunk_payload[ch + 1] - ex.chunk_payload[ch];
+        };
+        for (uint32_t d = 0; d < plan.n_dims; ++d) add(plan.dims[d].fk);
+        add(plan.value);
+    }
+}
+
+int status_of_flags(golap_ctx *ctx, uint32_t flags, const char *what) {
+    if (flags & DEVERR_CORRUPT) return fail(ctx, GOLAP_ECORRUPT, std::string(what) + ": corrupt page");
+    if (flags & DEVERR_CAPACITY) return fail(ctx, GOLAP_ECAPACITY, std::string(what) + ": table full");
+    if (flags) return fail(ctx, GOLAP_EINVAL, std::string(what) + ": value outside its declared range");
+    return GOLAP_OK;
+}
+
+int ColumnSet::index(golap_ctx *ctx, const golap_column *c, uint8_t *idx) {
+    for (size_t i = 0; i < cols.size(); ++i)
+        if (cols[i] == c) {
+            *idx = (uint8_t)i;
+            return GOLAP_OK;
+        }
+    if (!cols.empty()) {
+        const golap_column *f = cols[0];
+        if (c->n_rows != f->n_rows || c->page_rows != f->page_rows || c->chunk_rows != f->chunk_rows)
+            return fail(ctx, GOLAP_EINVAL, "star: columns are not row-aligned");
+    }
+    if (cols.size() >= STAR_MAX_COLS) return fail(ctx, GOLAP_EINVAL, "star: more than 8 distinct fact columns");
+    *idx = (uint8_t)cols.size();
+    cols.push_back(c);
+    return GOLAP_OK;
+}
+
+}  // namespace starhost
+}  // namespace golap_impl
 
 extern "C" {
 
@@ -133,182 +310,69 @@ int golap_group_by(golap_ctx *ctx, const golap_group_key *keys, uint32_t n_keys,
 
 int golap_star_run(golap_ctx *ctx, const golap_star_plan *plan, golap_star_result *res,
                    int32_t *group_keys, int64_t *sums, uint64_t *counts, uint64_t max_groups) {
-    if (!ctx || !plan || !res || plan->n_dims < 1 || plan->n_dims > STAR_MAX_DIMS || !plan->value ||
-        plan->agg_mode > GOLAP_AGG_GLOBAL)
-        return GOLAP_EINVAL;
-    const uint32_t nd = plan->n_dims;
-    const golap_column *v = plan->value;
-    auto is32 = [](const golap_column *c) { return c->type == GOLAP_I32 || c->type == GOLAP_U32; };
-    if (v->ctx != ctx || !is32(v)) return fail(ctx, GOLAP_EINVAL, "star: value column must be 32-bit");
-    for (uint32_t d = 0; d < nd; ++d) {
-        const golap_star_dim &sd = plan->dims[d];
-        if (!sd.fk || !sd.map || sd.fk->ctx != ctx || sd.map->ctx != ctx || !sd.map->built)
-            return fail(ctx, GOLAP_EINVAL, "star: missing column or unbuilt map");
-        if (!is32(sd.fk)) return fail(ctx, GOLAP_EINVAL, "star: foreign keys must be 32-bit");
-        if (sd.fk->n_rows != v->n_rows || sd.fk->page_rows != v->page_rows || sd.fk->chunk_rows != v->chunk_rows)
-            return fail(ctx, GOLAP_EINVAL, "star: columns are not row-aligned");
-        if (sd.fk->type == GOLAP_U32)
-            for (int64_t mx : sd.fk->chunk_max)
-                if (mx > INT32_MAX) return fail(ctx, GOLAP_EINVAL, "star: uint32 keys above INT32_MAX");
-    }
-    if (v->page_rows != 1024 && v->page_rows != 2048 && v->page_rows != 4096 && v->page_rows != 8192)
-        return fail(ctx, GOLAP_EINVAL, "star: page_rows must be 1024, 2048, 4096 or 8192");
-    if (plan->chunk_begin > plan->chunk_end || plan->chunk_end > v->n_chunks)
-        return fail(ctx, GOLAP_EINVAL, "star: bad chunk range");
+    if (!ctx || !plan || !res) return GOLAP_EINVAL;
     Lock l(ctx->mu);
     if (int r = set_device(ctx)) return r;
     memset(res, 0, sizeof(*res));
-
-    // group domain, attributes in dimension order (first = most significant)
-    std::vector<int32_t> lo;
-    std::vector<uint32_t> card;
-    std::vector<uint32_t> gdim;
-    uint64_t domain = 1;
-    for (uint32_t d = 0; d < nd; ++d)
-        if (plan->dims[d].group_card) {
-            lo.push_back(plan->dims[d].group_lo);
-            card.push_back(plan->dims[d].group_card);
-            gdim.push_back(d);
-            domain *= plan->dims[d].group_card;
-            if (domain >= MAX_GROUPS) return fail(ctx, GOLAP_EINVAL, "star: group domain too large");
-        }
-    std::vector<uint64_t> stride(card.size());
-    for (int a = (int)card.size() - 1, s = 1; a >= 0; --a) {
-        stride[a] = (uint64_t)s;
-        s *= (int)card[a];
-    }
-
+    ColumnSet cs;
+    StarQueryDev qd;
+    QueryHost qh;
+    if (int r = prepare_query(ctx, plan, true, cs, qd, qh)) return r;
+    const golap_column *v = plan->value;
     StarLaunch L{};
-    L.n_dims = nd;
-    // driver: the dimension whose filter passes the smallest share of its key domain
-    double best = 2.0;
-    for (uint32_t d = 0; d < nd; ++d) {
-        const golap_hashmap *m = plan->dims[d].map;
-        const double sel = m->ht_size ? (double)m->filled / (double)m->ht_size : 1.0;
-        if (sel < best) {
-            best = sel;
-            L.driver = d;
-        }
-    }
-    const uint64_t ppc = v->chunk_rows / v->page_rows;
+    L.n_cols = (uint32_t)cs.cols.size();
+    for (uint32_t c = 0; c < L.n_cols; ++c) L.col[c] = view(cs.cols[c]);
+    L.n_windows = assign_windows(cs, qd.page_begin, qd.page_end, L.col_window);
     L.page_rows = v->page_rows;
+    L.pages_per_chunk = (uint32_t)(v->chunk_rows / v->page_rows);
     L.n_rows = v->n_rows;
-    L.page_begin = plan->chunk_begin * ppc;
-    L.page_end = std::min<uint64_t>(v->n_pages, plan->chunk_end * ppc);
-    L.chunk_mask = plan->dev_chunk_mask;
-    L.pages_per_chunk = (uint32_t)ppc;
-    uint32_t windows = 0;
-    auto window_of = [&](const golap_column *c) {
-        std::lock_guard<std::mutex> g(g_extra_mu);
-        auto &ex = g_extra[c];
-        return ex.window_prefix[L.page_end] != ex.window_prefix[L.page_begin] ? (int)windows++ : -1;
-    };
-    size_t a_idx = 0;
-    for (uint32_t d = 0; d < nd; ++d) {
-        const golap_star_dim &sd = plan->dims[d];
-        const golap_hashmap *m = sd.map;
-        L.fk[d] = view(sd.fk);
-        L.fk_window[d] = window_of(sd.fk);
-        L.dim[d].slots = m->small_log2 ? m->d_small : m->d_slots;
-        L.dim[d].log2cap = m->small_log2 ? m->small_log2 : m->log2cap;
-        L.dim[d].group_lo = sd.group_lo;
-        L.dim[d].group_card = sd.group_card;
-        L.dim[d].group_stride = sd.group_card ? stride[a_idx++] : 0;
-    }
-    L.val = view(v);
-    L.val_window = window_of(v);
-    L.val_signed = v->type == GOLAP_I32;
-    L.n_windows = windows;
-
-    // driver filter: dense exact bitmap, or the hashed one-bit filter of the map's keys
-    const golap_hashmap *dm = plan->dims[L.driver].map;
-    if (dm->use_bitmap && (dm->krange + 31) / 32 <= (1u << HASH_FILTER_WORDS_LOG2)) {
-        L.filter = dm->d_bitmap;
-        L.filter_words = (dm->krange + 31) / 32;
-        L.fkmin = dm->kmin;
-        L.fkrange = dm->krange;
-        L.fhash_log2 = 0;
-    } else {
-        const uint32_t words = 1u << HASH_FILTER_WORDS_LOG2;
-        if (!dm->d_hfilter) {
-            void *p = nullptr;
-            if (int r = dev_alloc(ctx, words * 4, &p)) return r;
-            dm->d_hfilter = static_cast<uint32_t *>(p);
-            dm->hfilter_build = 0;
-        }
-        if (dm->hfilter_build != dm->build_id) {
-            GOLAP_HIP_CHECK(ctx, hipMemsetAsync(dm->d_hfilter, 0, words * 4, ctx->stream));
-            GOLAP_HIP_CHECK(ctx, launch_hash_filter(ctx, dm->small_log2 ? dm->d_small : dm->d_slots,
-                                                    dm->small_log2 ? dm->small_log2 : dm->log2cap,
-                                                    dm->d_hfilter, HASH_FILTER_WORDS_LOG2));
-            dm->hfilter_build = dm->build_id;
-        }
-        L.filter = dm->d_hfilter;
-        L.filter_words = words;
-        L.fhash_log2 = HASH_FILTER_WORDS_LOG2;
-    }
-    L.n_groups = domain;
-    // AUTO: LDS partial tables pay off when many rows land in each group; estimate the
-    // qualifying rows from the maps' fill ratios (uniform foreign keys)
-    double expect = (double)(L.page_end - L.page_begin) * L.page_rows;
-    for (uint32_t d = 0; d < nd; ++d) {
-        const golap_hashmap *m = plan->dims[d].map;
-        expect *= m->ht_size ? (double)m->filled / (double)m->ht_size : 1.0;
-    }
+    L.n_pages = v->n_pages;
+    L.n_queries = 1;
+    L.n_items = qd.page_end - qd.page_begin;
+    L.filter_words = qd.filter_words;
+    // AUTO: LDS partial tables pay off when many rows land in each group
+    const double expect = (double)L.n_items * L.page_rows * qh.sel;
     uint32_t agg = plan->agg_mode;
-    if (agg == GOLAP_AGG_AUTO) agg = expect >= 16.0 * (double)domain && expect >= 65536.0 ? GOLAP_AGG_LDS : GOLAP_AGG_GLOBAL;
-    L.lds_gslots = choose_lds_slots(agg, plan->lds_group_slots, domain);
-    while (L.lds_gslots && star_lds_bytes(L.filter_words, L.page_rows, L.n_windows, L.lds_gslots) >
-                               (size_t)ctx->lds_max)
+    if (agg == GOLAP_AGG_AUTO)
+        agg = expect >= 16.0 * (double)qh.domain && expect >= 65536.0 ? GOLAP_AGG_LDS : GOLAP_AGG_GLOBAL;
+    L.lds_gslots = choose_lds_slots(agg, plan->lds_group_slots, qh.domain);
+    while (L.lds_gslots && star_lds_bytes(L.filter_words, L.page_rows, L.n_windows, L.lds_gslots) > (size_t)ctx->lds_max)
         L.lds_gslots >>= 1;
     if (star_lds_bytes(L.filter_words, L.page_rows, L.n_windows, L.lds_gslots) > (size_t)ctx->lds_max)
         return fail(ctx, GOLAP_EINVAL, "star: LDS budget exceeded (page_rows or coded columns)");
+    qd.use_lds = L.lds_gslots != 0;
+    L.group_bits = std::max<uint32_t>(1, bits_for(qh.domain));
     int64_t *d_sum;
     uint64_t *d_cnt;
-    if (int r = group_buffers(ctx, domain, &d_sum, &d_cnt)) return r;
-    L.g_sum = d_sum;
-    L.g_cnt = d_cnt;
-    L.rows_matched = reinterpret_cast<uint64_t *>(ctx->d_scratch + 16);
+    if (int r = group_buffers(ctx, qh.domain, &d_sum, &d_cnt)) return r;
+    qd.g_sum = d_sum;
+    qd.g_cnt = d_cnt;
+    L.q0 = qd;
     L.err = reinterpret_cast<uint32_t *>(ctx->d_scratch);
-    GOLAP_HIP_CHECK(ctx, hipMemsetAsync(ctx->d_scratch + 16, 0, 8, ctx->stream));
+    const golap_hashmap *dm = plan->dims[qd.driver].map;
     GOLAP_HIP_CHECK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    if (L.page_end > L.page_begin && dm->filled) GOLAP_HIP_CHECK(ctx, launch_star(ctx, L));
+    if (L.n_items && dm->filled) GOLAP_HIP_CHECK(ctx, launch_star(ctx, L, ctx->stream));
     GOLAP_HIP_CHECK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-    uint64_t host[3] = {0, 0, 0};
-    GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(host, ctx->d_scratch, 24, hipMemcpyDeviceToHost, ctx->stream));
+    uint32_t flags = 0;
+    GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(&flags, ctx->d_scratch, 4, hipMemcpyDeviceToHost, ctx->stream));
     std::vector<uint16_t> mask;
     if (plan->dev_chunk_mask && plan->chunk_end > plan->chunk_begin) {
-        mask.resize(plan->chunk_end);
-        GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(mask.data() + plan->chunk_begin, plan->dev_chunk_mask + plan->chunk_begin,
-                                            (plan->chunk_end - plan->chunk_begin) * 2, hipMemcpyDeviceToHost,
-                                            ctx->stream));
+        mask.resize(plan->chunk_end - plan->chunk_begin);
+        GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(mask.data(), plan->dev_chunk_mask + plan->chunk_begin, mask.size() * 2,
+                                            hipMemcpyDeviceToHost, ctx->stream));
     }
     GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    if ((uint32_t)host[0]) return error_flags_status(ctx, (uint32_t)host[0], "star");
+    if (flags) return error_flags_status(ctx, flags, "star");
     float ms = 0;
     GOLAP_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     res->kernel_ms = ms;
-    res->rows_matched = host[2];
-    res->driver_dim = L.driver;
+    res->driver_dim = qd.driver;
     res->lds_group_slots = L.lds_gslots;
-    {
-        std::lock_guard<std::mutex> g(g_extra_mu);
-        for (uint64_t ch = plan->chunk_begin; ch < plan->chunk_end; ++ch) {
-            if (!mask.empty() && !mask[ch]) {
-                res->chunks_pruned++;
-                continue;
-            }
-            res->chunks_run++;
-            auto add = [&](const golap_column *c) {
-                auto &ex = g_extra[c];
-                res->comp_bytes += ex.chunk_payload[ch + 1] - ex.chunk_payload[ch];
-            };
-            for (uint32_t d = 0; d < nd; ++d) add(plan->dims[d].fk);
-            add(v);
-        }
-    }
-    return emit_groups(ctx, domain, d_sum, d_cnt, lo, card, stride, &res->n_groups, &res->total_sum, group_keys,
-                       sums, counts, max_groups);
+    std::vector<uint8_t> run(mask.size());
+    for (size_t i = 0; i < mask.size(); ++i) run[i] = mask[i] != 0;
+    slice_stats(*plan, run, res);
+    return emit_groups(ctx, qh.domain, d_sum, d_cnt, qh.lo, qh.card, qh.stride, &res->n_groups, &res->total_sum,
+                       group_keys, sums, counts, max_groups, &res->rows_matched);
 }
 
 }  // extern "C"

@@ -31,6 +31,8 @@ struct golap_ctx {
     // dense group-by result buffers (sum int64, count uint64), grown on demand
     void *d_groups = nullptr;
     uint64_t groups_cap = 0;
+    // autotuned batch launch configurations by batch shape
+    std::unordered_map<std::string, golap_batch_config> tune_cache;
 };
 
 struct golap_column {
@@ -135,31 +137,73 @@ struct StarDimProbe {
     uint64_t group_stride;
 };
 
-struct StarLaunch {
-    ColView fk[STAR_MAX_DIMS];
-    int fk_window[STAR_MAX_DIMS];
-    ColView val;
-    int val_window;
-    uint32_t val_signed;
-    StarDimProbe dim[STAR_MAX_DIMS];
-    uint32_t n_dims, driver;
-    const uint32_t *filter;
-    uint32_t filter_words;
+// Device view of a driver filter (see stardev::filter_test).
+struct DriverFilterView {
+    const uint32_t *f;
     int32_t fkmin;
-    uint32_t fkrange, fhash_log2;
-    uint32_t page_rows;
-    uint64_t n_rows, page_begin, page_end;
-    const uint16_t *chunk_mask;
-    uint32_t pages_per_chunk;
-    uint32_t n_windows;
+    uint32_t fkrange;
+    uint32_t fhash_log2;
+};
+
+constexpr uint32_t STAR_MAX_COLS = 8;  // fact columns one star launch can read
+
+// One query of the persistent star kernel. Its columns are indexes into the launch's fact
+// columns, so queries over different foreign keys of one fact table share a launch.
+struct StarQueryDev {
+    StarDimProbe dim[STAR_MAX_DIMS];
+    uint8_t fk_col[STAR_MAX_DIMS];
+    uint8_t val_col;
+    uint8_t n_dims;
+    uint8_t val_signed;
+    uint8_t use_lds;                // aggregate in the LDS table (else straight to global atomics)
+    uint32_t driver;                // dimension whose filter is tested first
+    DriverFilterView filter;
+    uint32_t filter_words;
+    uint32_t pad0;
+    uint64_t page_begin, page_end;  // fact slice in pages (static schedule)
+    uint64_t chunk_begin, chunk_end;
+    const uint16_t *chunk_mask;     // zone-map flags per chunk, may be null
+    const int64_t *zmin, *zmax;     // zone predicate zlo <= zmax[c] && zhi >= zmin[c]; zmin null = none
+    int64_t zlo, zhi;
+    uint64_t item_begin;            // static schedule: pages of the queries before this one
     uint64_t n_groups;
-    uint32_t lds_gslots;
     int64_t *g_sum;
     uint64_t *g_cnt;
-    uint64_t *rows_matched;
+};
+
+struct StarLaunch {
+    ColView col[STAR_MAX_COLS];
+    int col_window[STAR_MAX_COLS];
+    uint32_t n_cols;
+    uint32_t page_rows;
+    uint64_t n_rows, n_pages;
+    uint32_t pages_per_chunk;
+    uint32_t n_windows;
+    // static schedule: q0 alone (queries == nullptr) or a device array; n_items pages in all
+    StarQueryDev q0;
+    const StarQueryDev *queries;
+    uint32_t n_queries;
+    uint64_t n_items;
+    // queue schedule (items != nullptr): device-built (chunk, query << 8 | part) items,
+    // queue[0] = next item, queue[1] = item count; an item covers pages_per_item pages
+    const uint2 *items;
+    uint32_t *queue;
+    uint32_t pages_per_item;
+    uint32_t filter_words;        // LDS words for the driver filter, 0 = read it from global memory
+    uint32_t lds_gslots;
+    uint32_t group_bits;
+    uint32_t max_blocks_per_cu;   // 0 = occupancy limit
     uint32_t *err;
 };
-hipError_t launch_star(golap_ctx *ctx, const StarLaunch &l);
+// prepare_only: set the kernel's LDS attribute and check the launch without launching (graph
+// capture then records the launch alone).
+hipError_t launch_star(golap_ctx *ctx, const StarLaunch &l, hipStream_t stream, bool prepare_only = false);
+// Batch work list: one thread per (query, chunk of its slice); surviving chunks (chunk mask and
+// zone predicate) emit parts items each into items, counted in queue[1].
+hipError_t launch_batch_items(const StarQueryDev *queries, uint32_t n_queries, const uint64_t *chunk_off,
+                              uint64_t total_chunks, uint32_t parts, uint2 *items, uint32_t *queue,
+                              hipStream_t stream);
+hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t stream);
 size_t star_lds_bytes(uint32_t filter_words, uint32_t page_rows, uint32_t n_windows, uint32_t lds_gslots);
 hipError_t launch_hash_filter(golap_ctx *ctx, const uint64_t *slots, uint32_t log2cap, uint32_t *filter,
                               uint32_t words_log2);

@@ -191,6 +191,15 @@ __global__ void k_fill_zero(uint64_t *p, uint64_t n) {
         p[i] = 0;
 }
 
+// Zeroing by a kernel rather than hipMemsetAsync, so that captured graphs hold kernel nodes only.
+hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t stream) {
+    if (bytes % 8 || reinterpret_cast<uintptr_t>(p) % 8) return hipErrorInvalidValue;
+    const uint64_t n = bytes / 8;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((n + BLOCK - 1) / BLOCK, 4096);
+    hipLaunchKernelGGL(k_fill_zero, dim3(std::max(grid, 1u)), dim3(BLOCK), 0, stream, static_cast<uint64_t *>(p), n);
+    return hipGetLastError();
+}
+
 hipError_t launch_hash_clear(golap_ctx *ctx, uint64_t *slots, uint64_t n) {
     const uint32_t grid = (uint32_t)std::min<uint64_t>((n + BLOCK - 1) / BLOCK, 4096);
     hipLaunchKernelGGL(k_fill_zero, dim3(std::max(grid, 1u)), dim3(BLOCK), 0, ctx->stream, slots, n);

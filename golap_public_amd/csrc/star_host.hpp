@@ -1,0 +1,49 @@
+// Host-side helpers of the star plans shared by the single-query entry point (capi_star.cpp)
+// and the batched form (capi_batch.cpp).
+#pragma once
+#include <cstdint>
+#include <utility>
+#include <vector>
+
+#include "common.hpp"
+
+namespace golap_impl {
+namespace starhost {
+
+constexpr uint64_t MAX_GROUPS = 1ull << 31;
+constexpr uint32_t LDS_GROUP_SLOTS_MAX = 4096;  // 64 KiB of LDS (key, count, int64 sum)
+
+// Distinct fact columns of one launch; all row-aligned with the first.
+struct ColumnSet {
+    std::vector<const golap_column *> cols;
+    int index(golap_ctx *ctx, const golap_column *c, uint8_t *idx);
+};
+
+// Host side of one prepared query.
+struct QueryHost {
+    golap_star_plan plan{};
+    std::vector<int32_t> lo;
+    std::vector<uint32_t> card;
+    std::vector<uint64_t> stride;
+    uint64_t domain = 1;
+    double sel = 1.0;  // expected share of fact rows that match (uniform foreign keys)
+    std::vector<std::pair<const golap_hashmap *, uint64_t>> builds;  // maps and their build ids
+};
+
+uint32_t pow2_at_least(uint64_t v);
+uint32_t bits_for(uint64_t n);
+uint32_t choose_lds_slots(uint32_t agg_mode, uint32_t requested, uint64_t n_groups);
+int group_buffers(golap_ctx *ctx, uint64_t n, int64_t **sum, uint64_t **cnt);
+int emit_groups(golap_ctx *ctx, uint64_t n, const int64_t *d_sum, const uint64_t *d_cnt,
+                const std::vector<int32_t> &lo, const std::vector<uint32_t> &card,
+                const std::vector<uint64_t> &stride, uint64_t *n_groups, int64_t *total, int32_t *keys,
+                int64_t *sums, uint64_t *counts, uint64_t max_groups, uint64_t *rows = nullptr);
+int prepare_query(golap_ctx *ctx, const golap_star_plan *plan, bool compact, ColumnSet &cs, StarQueryDev &qd,
+                  QueryHost &qh);
+uint32_t assign_windows(const ColumnSet &cs, uint64_t pb, uint64_t pe, int *win);
+void slice_stats(const golap_star_plan &plan, const std::vector<uint8_t> &run, golap_star_result *res);
+int status_of_flags(golap_ctx *ctx, uint32_t flags, const char *what);
+bool is_int_type(uint32_t t);
+
+}  // namespace starhost
+}  // namespace golap_impl

@@ -264,6 +264,60 @@ typedef struct {
 int golap_star_run(golap_ctx *ctx, const golap_star_plan *plan, golap_star_result *res,
                    int32_t *group_keys, int64_t *sums, uint64_t *counts, uint64_t max_groups);
 
+/* ------------------------------------------------------------------ batched star plans
+ * Many star plans over one fact table in one persistent fused launch. The reference runs one
+ * plan per call, with a worker loop over chunks (query2/query3, 02.hpp:54-307, 03.hpp:79-677),
+ * and prunes with MetaChecker before the loop (01.hpp:305-332). Here each query of the batch
+ * brings its own dimension maps, chunk range, optional chunk mask and optional zone-map range
+ * predicate (check_mmmeta semantics, metadata.cuh:539-547: a chunk survives iff
+ * lo <= max && hi >= min on zone_col). One run is three kernels:
+ *   1. zero the result arena;
+ *   2. build the work list: one entry per surviving (query, chunk part);
+ *   3. the persistent star kernel, whose workgroups take work-list items with one atomic each.
+ * The three are captured once into a hipGraph that every run replays.
+ * Every query's results equal golap_star_run of its plan alone. The fact columns of all queries
+ * (foreign keys and values, at most 8 distinct columns) must be row-aligned.
+ * golap_star_batch_tune times candidate launch configurations and keeps the fastest (autotuner):
+ *   - pages per work item (tile grain);
+ *   - LDS group-table slots and the LDS-versus-global aggregation cutover;
+ *   - the driver filter in LDS or in L2;
+ *   - compact or full probe tables;
+ *   - workgroups per CU.
+ * The chosen configuration is cached in the context for batches of the same shape. */
+typedef struct golap_star_batch golap_star_batch;
+typedef struct {
+    golap_star_plan plan;           /* plan.lds_group_slots is ignored: see golap_batch_config */
+    const golap_column *zone_col;   /* NULL: no zone-map predicate */
+    int64_t zone_lo, zone_hi;
+} golap_batch_query;
+typedef struct {
+    uint32_t pages_per_item;     /* pages of one chunk per work item, 1..pages per chunk */
+    uint32_t lds_group_slots;    /* LDS group table (power of two <= 4096), 0 = global atomics */
+    uint32_t lds_cutover;        /* AUTO queries aggregate in LDS when their expected matching
+                                    rows reach lds_cutover x their group count */
+    uint32_t filter_lds;         /* 1: copy each query's driver filter into LDS */
+    uint32_t compact_tables;     /* 1: probe the compact tables (2 x filled slots) */
+    uint32_t max_blocks_per_cu;  /* 0 = occupancy limit */
+} golap_batch_config;
+typedef struct {
+    double ms;                   /* device time of the run (graph or plain launches) */
+    uint64_t items;              /* work items built on the device */
+    uint64_t pages;              /* page tiles of the surviving chunks, all queries */
+    uint64_t comp_bytes;         /* compressed bytes of the surviving pages, all queries */
+} golap_batch_run_info;
+int golap_star_batch_create(golap_ctx *ctx, const golap_batch_query *queries, uint32_t n_queries,
+                            golap_star_batch **out);
+int golap_star_batch_config_get(golap_star_batch *b, golap_batch_config *cfg);
+int golap_star_batch_config_set(golap_star_batch *b, const golap_batch_config *cfg);
+/* use_graph 1: replay the captured graph; 0: the same three kernels as plain launches. */
+int golap_star_batch_run(golap_star_batch *b, uint32_t use_graph, golap_batch_run_info *info);
+/* Times each candidate reps times (after one warm-up), keeps and caches the fastest. */
+int golap_star_batch_tune(golap_star_batch *b, uint32_t reps, golap_batch_config *best, double *best_ms);
+/* Results of query q of the last run, as golap_star_run returns them (kernel_ms = the run's). */
+int golap_star_batch_result(golap_star_batch *b, uint32_t q, golap_star_result *res, int32_t *group_keys,
+                            int64_t *sums, uint64_t *counts, uint64_t max_groups);
+int golap_star_batch_free(golap_star_batch *b);
+
 /* ------------------------------------------------------------------ synthetic data
  * Seeded column generators of the benchmark workloads (the reference seeds its generator with
  * util::uniform_int, core/util.hpp:177-181; these use a SplitMix64 counter hash so row ranges
