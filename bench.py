@@ -33,8 +33,10 @@ def log(*a):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["q1", "star"], default="q1",
+    ap.add_argument("--workload", choices=["q1", "star", "batch"], default="q1",
                     help="q1: fused filter/probe/sum (the headline line); star: star join + group-by")
+    ap.add_argument("--queries", type=int, default=64, help="batch workload: queries per batch")
+    ap.add_argument("--zone-rows", type=int, default=65536, help="batch workload: rows per zone-map chunk")
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--rows", type=int, default=600_000_000)
@@ -174,6 +176,166 @@ def run_star(args):
     ctx.close()
 
 
+def star_setup(args, chunk_rows):
+    """The star schema of the star workloads: 600M fact rows (date_key sorted, bit-packed delta;
+    customer, supplier, part keys uniform and revenue, LZ4), dimensions of 2,556 / 2,000,000 /
+    200,000 / 1,400,000 rows with region (5), nation (25) and brand (1,000) attributes."""
+    import golap_public_amd as g
+    S = dict(n_cust=2_000_000, n_supp=200_000, n_part=1_400_000)
+    key, year, _ = g.gen_date_dim(2556)
+    log(f"generating {args.rows} star rows")
+    fact = g.gen_star_fact(args.seed, args.rows, args.rows, key, S["n_cust"], S["n_supp"], S["n_part"])
+    S["nation"] = g.gen_uniform_i32(args.seed, 20, S["n_cust"], 0, 24)
+    S["region"] = g.gen_uniform_i32(args.seed, 21, S["n_supp"], 0, 4)
+    S["brand"] = g.gen_uniform_i32(args.seed, 22, S["n_part"], 1, 1000)
+    log("encoding")
+    enc = dict(page_rows=args.page_rows, chunk_rows=chunk_rows, lz4_min_match=args.min_match)
+    hosts = [g.HostColumn.encode(fact[0], codec=g.CODEC_DBP, **enc)] + \
+            [g.HostColumn.encode(c, codec=g.CODEC_LZ4, **enc) for c in fact[1:]]
+    del fact
+    ctx = g.Context(0)
+    S["cols"] = [ctx.upload(h) for h in hosts]
+    S["comp"] = sum(h.desc().payload_bytes for h in hosts)
+    del hosts
+    dev = lambda a: ctx.to_device(np.ascontiguousarray(a))
+    S.update(key=key, year=year, d_key=dev(key), d_year=dev(year.astype(np.int32)),
+             c_key=dev(np.arange(S["n_cust"], dtype=np.int32)), c_nat=dev(S["nation"]),
+             s_key=dev(np.arange(S["n_supp"], dtype=np.int32)), s_reg=dev(S["region"]),
+             p_key=dev(np.arange(S["n_part"], dtype=np.int32)), p_brand=dev(S["brand"]))
+    log(f"compressed payload {S['comp']/1e9:.3f} GB for {args.rows*20/1e9:.3f} GB raw")
+    return ctx, S
+
+
+def batch_queries(args, S, nch):
+    """Seeded query mix: slice of log-uniform size in [1M, 600M] rows at a uniform start; date
+    range covering 5..50 % of the 2,556 days (the sorted fact then prunes 50..95 % of the zones);
+    supplier region, customer nation and a brand range of 1..10 brands drawn uniformly."""
+    rng = np.random.default_rng(args.seed)
+    qs = []
+    zr = args.zone_rows
+    for _ in range(args.queries):
+        rows = float(np.exp(rng.uniform(np.log(1e6), np.log(min(6e8, args.rows)))))
+        nc = max(1, min(nch, int(round(rows / zr))))
+        cb = int(rng.integers(0, nch - nc + 1))
+        width = int(2556 * rng.uniform(0.05, 0.5))
+        d0 = int(rng.integers(0, 2556 - width + 1))
+        bw = int(rng.integers(1, 11))
+        b0 = int(rng.integers(1, 1000 - bw + 2))
+        qs.append(dict(cb=cb, ce=cb + nc, dlo=int(S["key"][d0]), dhi=int(S["key"][d0 + width - 1]),
+                       region=int(rng.integers(0, 5)), nation=int(rng.integers(0, 25)), blo=b0, bhi=b0 + bw - 1))
+    return qs
+
+
+def run_batch(args):
+    """64 star queries over the date-sorted 600M-row star schema with min/max zone maps per
+    65,536-row chunk: one persistent fused launch under a hipGraph (autotuned configuration)
+    against one-query-at-a-time plain launches (zone-map check + golap_star_run per query),
+    dimension maps resident for both. Every query's groups are compared between the two."""
+    import golap_public_amd as g
+    ctx, S = star_setup(args, args.zone_rows)
+    cols = S["cols"]
+    nch = ctx.column_desc(cols[0]).n_chunks
+    qs = batch_queries(args, S, nch)
+    log(f"{len(qs)} queries, {nch} zones; building their dimension maps")
+    maps = []
+    t0 = time.perf_counter()
+    for q in qs:
+        m = [ctx.hashmap(2556, S["d_key"], g.I32), ctx.hashmap(S["n_cust"], S["c_key"], g.I32),
+             ctx.hashmap(S["n_supp"], S["s_key"], g.I32), ctx.hashmap(S["n_part"], S["p_key"], g.I32)]
+        ctx.hashmap_build(m[0], 2556, [(S["d_key"], g.I32, q["dlo"], q["dhi"])], S["d_year"])
+        ctx.hashmap_build(m[1], S["n_cust"], [(S["c_nat"], g.I32, q["nation"], q["nation"])])
+        ctx.hashmap_build(m[2], S["n_supp"], [(S["s_reg"], g.I32, q["region"], q["region"])])
+        ctx.hashmap_build(m[3], S["n_part"], [(S["p_brand"], g.I32, q["blo"], q["bhi"])], S["p_brand"])
+        maps.append(m)
+    ctx.sync()
+    build_ms = (time.perf_counter() - t0) * 1e3
+    dims_of = lambda i: [(cols[0], maps[i][0], 1992, 7), (cols[1], maps[i][1], 0, 0),
+                         (cols[2], maps[i][2], 0, 0), (cols[3], maps[i][3], 1, 1000)]
+    flags = ctx.alloc(nch * 2)
+
+    def plain_all():
+        out = []
+        for i, q in enumerate(qs):
+            ctx.minmax_check(cols[0], q["dlo"], q["dhi"], flags)
+            out.append(ctx.star_run(dims_of(i), cols[4], chunk_begin=q["cb"], chunk_end=q["ce"],
+                                    dev_chunk_mask=flags))
+        return out
+
+    entries = [dict(dims=dims_of(i), value=cols[4], chunk_begin=q["cb"], chunk_end=q["ce"],
+                    zone=(cols[0], q["dlo"], q["dhi"])) for i, q in enumerate(qs)]
+    b = ctx.star_batch(entries)
+    default_cfg = b.config().as_dict()
+    info = b.run()
+    t0 = time.perf_counter()
+    best, tuned_ms = b.tune(reps=3)
+    tune_s = time.perf_counter() - t0
+    log(f"autotuner: {default_cfg} -> {best.as_dict()} ({tuned_ms:.3f} ms, {tune_s:.1f} s)")
+    for _ in range(args.warmup):
+        b.run()
+        plain_all()
+    # batched: graph replay + every query's groups back on the host
+    dev_ms, walls = [], []
+    for _ in range(args.steps):
+        t1 = time.perf_counter()
+        info = b.run(use_graph=True)
+        res = [b.result(i) for i in range(len(qs))]
+        walls.append(time.perf_counter() - t1)
+        dev_ms.append(info.ms)
+    plain_walls, plain_dev = [], []
+    for _ in range(max(1, args.steps // 4)):
+        t1 = time.perf_counter()
+        ref = plain_all()
+        plain_walls.append(time.perf_counter() - t1)
+        plain_dev.append(sum(r[0].kernel_ms for r in ref))
+    for i in range(len(qs)):
+        assert all(np.array_equal(x, y) for x, y in zip(res[i][1:], ref[i][1:])), f"query {i} differs"
+    wall = statistics.median(walls)
+    pwall = statistics.median(plain_walls)
+    dms = statistics.median(dev_ms)
+    surv_rows = info.pages * args.page_rows
+    log(f"batch: {dms:.3f} ms device, {wall*1e3:.3f} ms with results; plain: {pwall*1e3:.3f} ms "
+        f"({statistics.median(plain_dev):.3f} ms kernels); items {info.items}, surviving pages {info.pages}")
+    gbps = info.comp_bytes / (dms * 1e-3) / 1e9
+    out = {"metric": "queries/s (64-query batch, one persistent fused launch, 1 GPU)",
+           "value": len(qs) / wall, "unit": "queries/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": wall * 1e3, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "int64", "data": "synthetic: SplitMix64-seeded star schema, seeded queries",
+           "config": {"workload": "star_batch_64q_600M", "rows": args.rows, "zone_rows": args.zone_rows,
+                      "page_rows": args.page_rows, "queries": len(qs), "zones": nch,
+                      "surviving_pages": info.pages, "surviving_rows": surv_rows, "work_items": info.items,
+                      "comp_bytes": info.comp_bytes, "device_ms": dms,
+                      "fact_rows_per_s": surv_rows / (dms * 1e-3),
+                      "plain_queries_per_s": len(qs) / pwall, "plain_ms": pwall * 1e3,
+                      "plain_kernel_ms": statistics.median(plain_dev),
+                      "speedup_vs_plain": pwall / wall, "map_build_ms_all_queries": build_ms,
+                      "tuned_config": best.as_dict(), "default_config": default_cfg, "tune_s": tune_s},
+           "roofline": {"bound": "hbm", "achieved": gbps, "peak": 8000.0, "unit": "GB/s", "frac": gbps / 8000.0,
+                        "traffic": None}}
+    if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        threads = len(os.sched_getaffinity(0))
+        sample = min(args.rows, 60_000_000)
+        f2 = oracle.gen_star_fact(args.seed, sample, args.rows, S["key"], S["n_cust"], S["n_supp"], S["n_part"])
+        q = qs[0]
+        t0 = time.perf_counter()
+        oracle.star_groupby([(S["key"], [(S["key"], q["dlo"], q["dhi"])], S["year"].astype(np.int32)),
+                             (np.arange(S["n_cust"], dtype=np.int32), [(S["nation"], q["nation"], q["nation"])], None),
+                             (np.arange(S["n_supp"], dtype=np.int32), [(S["region"], q["region"], q["region"])], None),
+                             (np.arange(S["n_part"], dtype=np.int32), [(S["brand"], q["blo"], q["bhi"])], S["brand"])],
+                            list(f2[:4]), f2[4], threads=threads)
+        dt = time.perf_counter() - t0
+        rows_per_q = surv_rows / len(qs)
+        out["cpu_baseline"] = {"value": sample / dt / rows_per_q, "unit": "queries/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"star plan of 02inmem.hpp over the first {sample} decoded rows "
+                                         f"({sample/dt:.3g} rows/s), scaled to this mix's mean "
+                                         f"{rows_per_q:.3g} surviving rows per query"}
+    print(json.dumps(out), flush=True)
+    b.free()
+    ctx.close()
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -182,6 +344,8 @@ def main():
         sys.exit(2)
     if args.workload == "star":
         return run_star(args)
+    if args.workload == "batch":
+        return run_batch(args)
     import golap_public_amd as g
 
     t_setup = time.perf_counter()

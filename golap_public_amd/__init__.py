@@ -15,7 +15,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "golap_hip.h")
 
 OK, EINVAL, ENOMEM, EHIP, ECORRUPT, ECAPACITY, EBUDGET, ENODEV = 0, -1, -2, -3, -4, -5, -6, -7
 I8, U8, I16, U16, I32, U32, I64 = 1, 2, 3, 4, 5, 6, 7
-CODEC_RAW, CODEC_LZ4, CODEC_DBP, CODEC_RLE, CODEC_AUTO = 0, 1, 2, 3, 255
+CODEC_RAW, CODEC_LZ4, CODEC_DBP, CODEC_RLE, CODEC_FOR, CODEC_AUTO = 0, 1, 2, 3, 4, 255
 PAGE_LITERAL_ONLY = 0x80
 
 NP_TYPE = {I8: np.int8, U8: np.uint8, I16: np.int16, U16: np.uint16, I32: np.int32,
@@ -71,6 +71,21 @@ class StarResult(C.Structure):
 
 
 AGG_AUTO, AGG_LDS, AGG_GLOBAL = 0, 1, 2
+
+
+class ScanPred(C.Structure):
+    _fields_ = [("col", u32), ("lo", i64), ("hi", i64)]
+
+
+class ScanKey(C.Structure):
+    _fields_ = [("col", u32), ("base", i64), ("div", u32), ("card", u32), ("out_lo", C.c_int32)]
+
+
+class ScanPlan(C.Structure):
+    _fields_ = [("cols", vp * 8), ("n_cols", u32), ("preds", ScanPred * 4), ("n_preds", u32),
+                ("keys", ScanKey * 3), ("n_keys", u32), ("value_col", C.c_int32), ("chunk_begin", u64),
+                ("chunk_end", u64), ("dev_chunk_mask", vp), ("agg_mode", u32), ("zone_prune", u32),
+                ("lds_group_slots", u32)]
 
 
 class BatchQuery(C.Structure):
@@ -133,6 +148,7 @@ _PROTOS = {
     "golap_group_by": (C.c_int, [vp, C.POINTER(GroupKey), u32, vp, u32, u64, u32, C.POINTER(u64), vp, vp, vp,
                                  u64]),
     "golap_star_run": (C.c_int, [vp, C.POINTER(StarPlan), C.POINTER(StarResult), vp, vp, vp, u64]),
+    "golap_scan_run": (C.c_int, [vp, C.POINTER(ScanPlan), C.POINTER(StarResult), vp, vp, vp, u64]),
     "golap_star_batch_create": (C.c_int, [vp, C.POINTER(BatchQuery), u32, C.POINTER(vp)]),
     "golap_star_batch_config_get": (C.c_int, [vp, C.POINTER(BatchConfig)]),
     "golap_star_batch_config_set": (C.c_int, [vp, C.POINTER(BatchConfig)]),
@@ -141,6 +157,7 @@ _PROTOS = {
     "golap_star_batch_result": (C.c_int, [vp, u32, C.POINTER(StarResult), vp, vp, vp, u64]),
     "golap_star_batch_free": (C.c_int, [vp]),
     "golap_gen_star_fact": (C.c_int, [u64, u64, u64, u64, vp, u32, u32, u32, u32, vp, vp, vp, vp, vp, u32]),
+    "golap_gen_trip": (C.c_int, [u64, u64, u64, u64, vp, vp, vp, vp, vp, u32]),
     "golap_gen_uniform_i32": (C.c_int, [u64, u32, u64, C.c_int32, C.c_int32, vp, u32]),
     "golap_gen_date_dim": (C.c_int, [u32, vp, vp, vp]),
     "golap_gen_q1_fact": (C.c_int, [u64, u64, u64, vp, u32, vp, vp, vp, vp, u32]),
@@ -168,6 +185,14 @@ class GolapError(RuntimeError):
     def __init__(self, status, msg=""):
         super().__init__(f"status {status} ({load().golap_status_string(status).decode()}) {msg}")
         self.status = status
+
+
+def _domain(dims):
+    n = 1
+    for d in dims:
+        if d[3]:
+            n *= int(d[3])
+    return n
 
 
 def _ptr(a):
@@ -409,10 +434,13 @@ class Context:
         return p
 
     def star_run(self, dims, value, chunk_begin=0, chunk_end=None, dev_chunk_mask=None, agg_mode=AGG_AUTO,
-                 lds_group_slots=0, max_groups=1 << 20):
-        """dims: list of (fk column, map, group_lo, group_card). Returns (result, keys, sums, counts)."""
+                 lds_group_slots=0, max_groups=None):
+        """dims: list of (fk column, map, group_lo, group_card). Returns (result, keys, sums, counts).
+        max_groups defaults to the group domain (the product of the group cards)."""
         p = self.star_plan(dims, value, chunk_begin, chunk_end, dev_chunk_mask, agg_mode, lds_group_slots)
         na = sum(1 for d in dims if d[3])
+        if max_groups is None:
+            max_groups = _domain(dims)
         kout = np.zeros(max_groups * max(na, 1), np.int32)
         sums = np.zeros(max_groups, np.int64)
         cnts = np.zeros(max_groups, np.uint64)
@@ -421,6 +449,43 @@ class Context:
                                            max_groups), "star_run")
         n = r.n_groups
         return r, kout[:n * na].reshape(n, na), sums[:n], cnts[:n]
+
+    def scan_plan(self, cols, preds=(), keys=(), value_col=-1, chunk_begin=0, chunk_end=None, dev_chunk_mask=None,
+                  agg_mode=AGG_AUTO, zone_prune=1, lds_group_slots=0):
+        """cols: device columns; preds: [(col, lo, hi)]; keys: [(col, base, div, card, out_lo)]."""
+        if chunk_end is None:
+            chunk_end = self.column_desc(cols[0]).n_chunks
+        p = ScanPlan()
+        for i, c in enumerate(cols):
+            p.cols[i] = c
+        p.n_cols = len(cols)
+        for i, pr in enumerate(preds):
+            p.preds[i] = ScanPred(*pr)
+        p.n_preds = len(preds)
+        for i, k in enumerate(keys):
+            p.keys[i] = ScanKey(*k)
+        p.n_keys = len(keys)
+        p.value_col, p.chunk_begin, p.chunk_end = value_col, chunk_begin, chunk_end
+        p.dev_chunk_mask, p.agg_mode, p.zone_prune, p.lds_group_slots = dev_chunk_mask, agg_mode, zone_prune, \
+            lds_group_slots
+        return p
+
+    def scan_run(self, cols, preds=(), keys=(), value_col=-1, max_groups=None, **kw):
+        """Returns (result, keys[n, k], sums, counts); max_groups defaults to the group domain."""
+        p = self.scan_plan(cols, preds, keys, value_col, **kw)
+        if max_groups is None:
+            max_groups = 1
+            for k in keys:
+                max_groups *= int(k[3])
+        nk = len(keys)
+        kout = np.zeros(max_groups * max(nk, 1), np.int32)
+        sums = np.zeros(max_groups, np.int64)
+        cnts = np.zeros(max_groups, np.uint64)
+        r = StarResult()
+        self.check(self.lib.golap_scan_run(self.h, C.byref(p), C.byref(r), _ptr(kout), _ptr(sums), _ptr(cnts),
+                                           max_groups), "scan_run")
+        n = r.n_groups
+        return r, kout[:n * nk].reshape(n, nk), sums[:n], cnts[:n]
 
     def star_batch(self, queries):
         """queries: list of dicts with star_plan() arguments plus optional zone=(column, lo, hi)."""
@@ -444,7 +509,7 @@ class StarBatch:
     def __init__(self, ctx, queries):
         self.ctx, self.h = ctx, None
         arr = (BatchQuery * len(queries))()
-        self.n_attrs = []
+        self.n_attrs, self.domains = [], []
         for i, q in enumerate(queries):
             q = dict(q)
             zone = q.pop("zone", None)
@@ -452,6 +517,7 @@ class StarBatch:
             if zone is not None:
                 arr[i].zone_col, arr[i].zone_lo, arr[i].zone_hi = zone
             self.n_attrs.append(sum(1 for d in q["dims"] if d[3]))
+            self.domains.append(_domain(q["dims"]))
         h = vp()
         ctx.check(ctx.lib.golap_star_batch_create(ctx.h, arr, len(queries), C.byref(h)), "star_batch_create")
         self.h = h.value
@@ -478,8 +544,10 @@ class StarBatch:
         self.ctx.check(self.ctx.lib.golap_star_batch_tune(self.h, reps, C.byref(c), C.byref(ms)), "batch_tune")
         return c, ms.value
 
-    def result(self, q, max_groups=1 << 20):
+    def result(self, q, max_groups=None):
         na = self.n_attrs[q]
+        if max_groups is None:
+            max_groups = self.domains[q]
         kout = np.zeros(max_groups * max(na, 1), np.int32)
         sums = np.zeros(max_groups, np.int64)
         cnts = np.zeros(max_groups, np.uint64)
@@ -528,6 +596,22 @@ def gen_star_fact(seed, n, n_total, date_keys, n_cust, n_supp, n_part, row0=0, t
                                     n_part, *[_ptr(c) for c in cols], threads)
     if st != OK:
         raise GolapError(st, "gen_star_fact")
+    return cols
+
+
+TRIP_T0, TRIP_SPAN = 1704067200, 31622400
+
+
+def gen_trip(seed, n, n_total=None, row0=0, threads=0):
+    """Trip-table columns (pickup_ts int64 sorted over 2024, passenger uint8 Zipf(1.2) 1..6, zone
+    uint16 1..265, fare_cents int32 lognormal(7.2, 0.6) in [250, 50000], distance_m int32
+    lognormal(8, 0.8)) for rows [row0, row0+n) of n_total."""
+    n_total = n if n_total is None else n_total
+    cols = (np.empty(n, np.int64), np.empty(n, np.uint8), np.empty(n, np.uint16), np.empty(n, np.int32),
+            np.empty(n, np.int32))
+    st = load().golap_gen_trip(seed, row0, n, n_total, *[_ptr(c) for c in cols], threads)
+    if st != OK:
+        raise GolapError(st, "gen_trip")
     return cols
 
 

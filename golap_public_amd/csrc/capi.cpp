@@ -256,12 +256,12 @@ int golap_host_column_assemble(uint32_t type, uint64_t n_rows, uint32_t page_row
     std::vector<const uint8_t *> ptrs(c->n_pages);
     for (uint64_t p = 0; p < c->n_pages; ++p) {
         uint8_t cd = codec[p] & golap_fmt::PAGE_CODEC_MASK;
-        if (cd > golap_fmt::RLE) return GOLAP_EINVAL;
+        if (cd > golap_fmt::FOR) return GOLAP_EINVAL;
         if (csize[p] && !pages[p]) return GOLAP_EINVAL;
         uint64_t rows = std::min<uint64_t>(page_rows, n_rows - p * page_rows);
         if (cd == golap_fmt::LZ4 && golap_host::lz4_is_literal_only(pages[p], csize[p], rows * c->elem))
             cd |= golap_fmt::PAGE_LITERAL_ONLY;
-        if (cd == golap_fmt::RLE || (cd == golap_fmt::DBP && csize[p] >= 32 && pages[p][20] == 0))
+        if (cd == golap_fmt::RLE || cd == golap_fmt::FOR || (cd == golap_fmt::DBP && csize[p] >= 32 && pages[p][20] == 0))
             cd |= golap_fmt::PAGE_POINT;
         c->page_codec[p] = cd;
         ptrs[p] = pages[p];

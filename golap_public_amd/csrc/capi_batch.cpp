@@ -44,6 +44,8 @@ struct golap_star_batch {
     uint2 *d_items = nullptr;
     uint8_t *d_arena = nullptr;  // [0, 256): queue head, item count, error word; then results
     uint64_t arena_bytes = 0;
+    uint8_t *h_arena = nullptr;  // pinned host mirror, filled once per run on the first result call
+    bool h_valid = false;
     std::vector<uint64_t> res_off;  // per query: byte offset of its sums (counts follow)
     golap_batch_config cfg{};
     hipGraph_t graph = nullptr;
@@ -150,7 +152,8 @@ StarLaunch launch_of(const golap_star_batch *b, const golap_batch_config &c) {
 int enqueue(golap_star_batch *b, hipStream_t s) {
     golap_ctx *ctx = b->ctx;
     GOLAP_HIP_CHECK(ctx, launch_fill_zero(b->d_arena, b->arena_bytes, s));
-    GOLAP_HIP_CHECK(ctx, launch_batch_items(b->d_q, (uint32_t)b->q.size(), b->d_chunk_off, b->chunk_off.back(),
+    GOLAP_HIP_CHECK(ctx, launch_batch_items(b->d_q, (uint32_t)sizeof(StarQueryDev), (uint32_t)b->q.size(),
+                                            b->d_chunk_off, b->chunk_off.back(),
                                             parts_of(b, b->cfg.pages_per_item), b->d_items,
                                             reinterpret_cast<uint32_t *>(b->d_arena), s));
     GOLAP_HIP_CHECK(ctx, launch_star(ctx, launch_of(b, b->cfg), s));
@@ -212,6 +215,7 @@ void free_batch(golap_star_batch *b) {
     dev_release(b->ctx, b->d_chunk_off);
     dev_release(b->ctx, b->d_items);
     dev_release(b->ctx, b->d_arena);
+    if (b->h_arena) (void)hipHostFree(b->h_arena);
     delete b;
 }
 
@@ -241,10 +245,7 @@ int golap_star_batch_create(golap_ctx *ctx, const golap_batch_query *queries, ui
             if (zc->ctx != ctx || zc->n_rows != b->q[i].plan.value->n_rows ||
                 zc->chunk_rows != b->q[i].plan.value->chunk_rows || !zc->d_chunk_min)
                 return bail(fail(ctx, GOLAP_EINVAL, "batch: zone column not row-aligned with the fact table"));
-            b->qd[i].zmin = zc->d_chunk_min;
-            b->qd[i].zmax = zc->d_chunk_max;
-            b->qd[i].zlo = b->q[i].zone_lo;
-            b->qd[i].zhi = b->q[i].zone_hi;
+            b->qd[i].s.zone[0] = ZonePred{zc->d_chunk_min, zc->d_chunk_max, b->q[i].zone_lo, b->q[i].zone_hi};
         }
         b->max_filter_words = std::max(b->max_filter_words, b->qd[i].filter_words);
         b->max_domain = std::max(b->max_domain, b->qh[i].domain);
@@ -257,16 +258,16 @@ int golap_star_batch_create(golap_ctx *ctx, const golap_batch_query *queries, ui
     uint64_t pb = ~0ull, pe = 0;
     b->chunk_off.assign(1, 0);
     for (uint32_t i = 0; i < n_queries; ++i) {
-        pb = std::min(pb, b->qd[i].page_begin);
-        pe = std::max(pe, b->qd[i].page_end);
-        b->chunk_off.push_back(b->chunk_off.back() + (b->qd[i].chunk_end - b->qd[i].chunk_begin));
+        pb = std::min(pb, b->qd[i].s.page_begin);
+        pe = std::max(pe, b->qd[i].s.page_end);
+        b->chunk_off.push_back(b->chunk_off.back() + (b->qd[i].s.chunk_end - b->qd[i].s.chunk_begin));
     }
     b->n_windows = assign_windows(b->cs, std::min(pb, pe), pe, b->windows);
     // host view of the surviving chunks: the same test as k_batch_items
     b->survive.resize(n_queries);
     b->surv_pages.assign(n_queries, 0);
     for (uint32_t i = 0; i < n_queries; ++i) {
-        const StarQueryDev &d = b->qd[i];
+        const SliceDev &d = b->qd[i].s;
         const uint64_t nch = d.chunk_end - d.chunk_begin;
         std::vector<uint16_t> mask;
         if (d.chunk_mask && nch) {
@@ -302,6 +303,10 @@ int golap_star_batch_create(golap_ctx *ctx, const golap_batch_query *queries, ui
     void *p = nullptr;
     if (int r = dev_alloc(ctx, off, &p)) return bail(r);
     b->d_arena = static_cast<uint8_t *>(p);
+    if (hipHostMalloc(reinterpret_cast<void **>(&b->h_arena), off, hipHostMallocDefault) != hipSuccess) {
+        b->h_arena = nullptr;
+        return bail(fail(ctx, GOLAP_ENOMEM, "batch: pinned result mirror"));
+    }
     if (int r = dev_alloc(ctx, n_queries * sizeof(StarQueryDev), &p)) return bail(r);
     b->d_q = static_cast<StarQueryDev *>(p);
     if (int r = dev_alloc(ctx, b->chunk_off.size() * 8, &p)) return bail(r);
@@ -361,6 +366,7 @@ int golap_star_batch_run(golap_star_batch *b, uint32_t use_graph, golap_batch_ru
     GOLAP_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     b->last_ms = ms;
     b->ran = false;
+    b->h_valid = false;
     if (int r = status_of_flags(ctx, hdr[2], "batch")) return r;
     b->ran = true;
     if (info) {
@@ -469,10 +475,16 @@ int golap_star_batch_result(golap_star_batch *b, uint32_t q, golap_star_result *
     res->driver_dim = b->qd[q].driver;
     res->lds_group_slots = b->qd[q].use_lds ? b->cfg.lds_group_slots : 0;
     slice_stats(b->q[q].plan, b->survive[q], res);
-    const int64_t *d_sum = reinterpret_cast<const int64_t *>(b->d_arena + b->res_off[q]);
-    const uint64_t *d_cnt = reinterpret_cast<const uint64_t *>(b->d_arena + b->res_off[q] + h.domain * 8);
-    return emit_groups(ctx, h.domain, d_sum, d_cnt, h.lo, h.card, h.stride, &res->n_groups, &res->total_sum,
-                       group_keys, sums, counts, max_groups, &res->rows_matched);
+    if (!b->h_valid) {  // every query's results in one copy
+        GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(b->h_arena + ARENA_HEADER, b->d_arena + ARENA_HEADER,
+                                            b->arena_bytes - ARENA_HEADER, hipMemcpyDeviceToHost, ctx->stream));
+        GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        b->h_valid = true;
+    }
+    const int64_t *hs = reinterpret_cast<const int64_t *>(b->h_arena + b->res_off[q]);
+    const uint64_t *hc = reinterpret_cast<const uint64_t *>(b->h_arena + b->res_off[q] + h.domain * 8);
+    return emit_groups_host(ctx, h.domain, hs, hc, h.lo, h.card, h.stride, &res->n_groups, &res->total_sum,
+                            group_keys, sums, counts, max_groups, &res->rows_matched);
 }
 
 int golap_star_batch_free(golap_star_batch *b) {

@@ -1,0 +1,204 @@
+// C ABI: fused scan + filter + group-by plans over typed columns (include/golap_hip.h). Host side
+// of the trip-table plans: validate the plan, compute the key strides and division constants,
+// place LDS windows for coded columns, and launch the persistent scan kernel.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+#include "format.hpp"
+#include "star_host.hpp"
+
+using namespace golap_impl;
+using namespace golap_impl::starhost;
+using Lock = std::lock_guard<std::recursive_mutex>;
+
+namespace golap_impl {
+namespace starhost {
+
+int prepare_scan(golap_ctx *ctx, const golap_scan_plan *p, ScanQueryDev &qd, ScanHost &h) {
+    if (!p || p->n_cols < 1 || p->n_cols > SCAN_MAX_COLS || p->n_preds > SCAN_MAX_PREDS ||
+        p->n_keys > SCAN_MAX_KEYS || p->agg_mode > GOLAP_AGG_GLOBAL || p->zone_prune > 1)
+        return fail(ctx, GOLAP_EINVAL, "scan: bad plan");
+    const golap_column *f = p->cols[0];
+    for (uint32_t c = 0; c < p->n_cols; ++c) {
+        const golap_column *col = p->cols[c];
+        if (!col || col->ctx != ctx || !is_int_type(col->type))
+            return fail(ctx, GOLAP_EINVAL, "scan: missing or non-integer column");
+        if (col->n_rows != f->n_rows || col->page_rows != f->page_rows || col->chunk_rows != f->chunk_rows)
+            return fail(ctx, GOLAP_EINVAL, "scan: columns are not row-aligned");
+    }
+    if (f->page_rows > 8192) return fail(ctx, GOLAP_EINVAL, "scan: page_rows above 8192");
+    if (p->chunk_begin > p->chunk_end || p->chunk_end > f->n_chunks)
+        return fail(ctx, GOLAP_EINVAL, "scan: bad chunk range");
+    if (p->value_col >= (int32_t)p->n_cols || p->value_col < -1) return fail(ctx, GOLAP_EINVAL, "scan: bad value column");
+    h = ScanHost{};
+    h.plan = *p;
+    memset(&qd, 0, sizeof(qd));
+    qd.n_preds = p->n_preds;
+    for (uint32_t i = 0; i < p->n_preds; ++i) {
+        if (p->preds[i].col >= p->n_cols) return fail(ctx, GOLAP_EINVAL, "scan: bad predicate column");
+        qd.pred[i] = ScanPredDev{p->preds[i].col, 0, p->preds[i].lo, p->preds[i].hi};
+        qd.colmask |= 1u << p->preds[i].col;
+    }
+    qd.n_keys = p->n_keys;
+    for (uint32_t i = 0; i < p->n_keys; ++i) {
+        const golap_scan_key &k = p->keys[i];
+        if (k.col >= p->n_cols || !k.div || !k.card) return fail(ctx, GOLAP_EINVAL, "scan: bad key");
+        if ((uint64_t)k.card * k.div >= (1ull << 32)) return fail(ctx, GOLAP_EINVAL, "scan: key card * div >= 2^32");
+        h.domain *= k.card;
+        if (h.domain >= MAX_GROUPS) return fail(ctx, GOLAP_EINVAL, "scan: group domain too large");
+        h.lo.push_back(k.out_lo);
+        h.card.push_back(k.card);
+        qd.key[i].col = k.col;
+        qd.key[i].card = k.card;
+        qd.key[i].base = k.base;
+        qd.key[i].span = (uint64_t)k.card * k.div;
+        qd.key[i].div = golap_fmt::udiv32_magic(k.div);
+        qd.colmask |= 1u << k.col;
+    }
+    h.stride.resize(p->n_keys);
+    for (int i = (int)p->n_keys - 1, s = 1; i >= 0; --i) {
+        h.stride[i] = (uint64_t)s;
+        qd.key[i].stride = (uint64_t)s;
+        s *= (int)p->keys[i].card;
+    }
+    qd.val_col = p->value_col;
+    if (p->value_col >= 0) qd.colmask |= 1u << p->value_col;
+    const uint64_t ppc = f->chunk_rows / f->page_rows;
+    qd.s.chunk_begin = p->chunk_begin;
+    qd.s.chunk_end = p->chunk_end;
+    qd.s.page_begin = p->chunk_begin * ppc;
+    qd.s.page_end = std::min<uint64_t>(f->n_pages, p->chunk_end * ppc);
+    qd.s.chunk_mask = p->dev_chunk_mask;
+    if (p->zone_prune)
+        for (uint32_t z = 0; z < std::min<uint32_t>(p->n_preds, SLICE_MAX_ZONES); ++z) {
+            const golap_column *zc = p->cols[p->preds[z].col];
+            qd.s.zone[z] = ZonePred{zc->d_chunk_min, zc->d_chunk_max, p->preds[z].lo, p->preds[z].hi};
+        }
+    qd.n_groups = h.domain;
+    return GOLAP_OK;
+}
+
+// Host evaluation of which chunks of the slice survive (the device work-list test).
+std::vector<uint8_t> scan_survivors(const golap_scan_plan &p, const std::vector<uint16_t> &mask) {
+    std::vector<uint8_t> run(p.chunk_end - p.chunk_begin);
+    for (uint64_t k = 0; k < run.size(); ++k) {
+        const uint64_t c = p.chunk_begin + k;
+        bool pass = mask.empty() || mask[k];
+        if (p.zone_prune)
+            for (uint32_t z = 0; z < std::min<uint32_t>(p.n_preds, SLICE_MAX_ZONES) && pass; ++z) {
+                const golap_column *zc = p.cols[p.preds[z].col];
+                pass = p.preds[z].lo <= zc->chunk_max[c] && p.preds[z].hi >= zc->chunk_min[c];
+            }
+        run[k] = pass;
+    }
+    return run;
+}
+
+void scan_stats(const golap_scan_plan &p, uint32_t colmask, const std::vector<uint8_t> &run, golap_star_result *res) {
+    std::lock_guard<std::mutex> g(g_extra_mu);
+    for (uint64_t k = 0; k < run.size(); ++k) {
+        const uint64_t ch = p.chunk_begin + k;
+        if (!run[k]) {
+            res->chunks_pruned++;
+            continue;
+        }
+        res->chunks_run++;
+        for (uint32_t c = 0; c < p.n_cols; ++c)
+            if ((colmask >> c) & 1u) {
+                auto &ex = g_extra[p.cols[c]];
+                res->comp_bytes += ex.chunk_payload[ch + 1] - ex.chunk_payload[ch];
+            }
+    }
+}
+
+// LDS windows of the columns a set of queries reads over pages [pb, pe): byte offsets (16-byte
+// aligned, page_rows * elem each); returns the total.
+uint32_t scan_windows(const golap_scan_plan &p, uint32_t colmask, uint64_t pb, uint64_t pe, int32_t *woff) {
+    std::lock_guard<std::mutex> g(g_extra_mu);
+    uint32_t bytes = 0;
+    for (uint32_t c = 0; c < p.n_cols; ++c) {
+        woff[c] = -1;
+        if (!((colmask >> c) & 1u)) continue;
+        auto &ex = g_extra[p.cols[c]];
+        if (ex.window_prefix[pe] == ex.window_prefix[pb]) continue;
+        woff[c] = (int32_t)bytes;
+        bytes += (p.cols[c]->page_rows * p.cols[c]->elem + 15) & ~15u;
+    }
+    return bytes;
+}
+
+// LDS group slots for a scan launch: the requested or a default size, halved until the launch
+// fits the LDS budget (and, for the default, two workgroups per CU when it can).
+uint32_t scan_slots(golap_ctx *ctx, uint32_t agg_mode, uint32_t requested, uint64_t domain, uint32_t win_bytes) {
+    if (agg_mode == GOLAP_AGG_GLOBAL) return 0;
+    uint32_t s = requested ? std::min(pow2_at_least(requested), LDS_GROUP_SLOTS_MAX)
+                           : (uint32_t)std::min<uint64_t>(pow2_at_least(std::max<uint64_t>(2 * domain, 64)), 2048);
+    const size_t budget = requested ? (size_t)ctx->lds_max : std::max<size_t>((size_t)ctx->lds_max / 2, win_bytes + 1024);
+    while (s && scan_lds_bytes(win_bytes, s) > budget) s >>= 1;
+    return s;
+}
+
+}  // namespace starhost
+}  // namespace golap_impl
+
+extern "C" {
+
+int golap_scan_run(golap_ctx *ctx, const golap_scan_plan *plan, golap_star_result *res, int32_t *group_keys,
+                   int64_t *sums, uint64_t *counts, uint64_t max_groups) {
+    if (!ctx || !plan || !res) return GOLAP_EINVAL;
+    Lock l(ctx->mu);
+    if (int r = set_device(ctx)) return r;
+    memset(res, 0, sizeof(*res));
+    ScanQueryDev qd;
+    ScanHost h;
+    if (int r = prepare_scan(ctx, plan, qd, h)) return r;
+    const golap_column *f = plan->cols[0];
+    ScanLaunch L{};
+    L.n_cols = plan->n_cols;
+    int32_t woff[SCAN_MAX_COLS];
+    L.win_bytes = scan_windows(*plan, qd.colmask, qd.s.page_begin, qd.s.page_end, woff);
+    for (uint32_t c = 0; c < plan->n_cols; ++c) L.col[c] = ScanColView{view(plan->cols[c]), plan->cols[c]->type, woff[c]};
+    L.page_rows = f->page_rows;
+    L.pages_per_chunk = (uint32_t)(f->chunk_rows / f->page_rows);
+    L.n_rows = f->n_rows;
+    L.n_pages = f->n_pages;
+    L.n_queries = 1;
+    L.n_items = qd.s.page_end - qd.s.page_begin;
+    L.lds_gslots = scan_slots(ctx, plan->agg_mode, plan->lds_group_slots, h.domain, L.win_bytes);
+    if (scan_lds_bytes(L.win_bytes, L.lds_gslots) > (size_t)ctx->lds_max)
+        return fail(ctx, GOLAP_EINVAL, "scan: LDS budget exceeded (page_rows or coded columns)");
+    L.flush_fill = L.lds_gslots / 2;
+    L.group_bits = std::max<uint32_t>(1, bits_for(h.domain));
+    qd.use_lds = L.lds_gslots != 0;
+    int64_t *d_sum;
+    uint64_t *d_cnt;
+    if (int r = group_buffers(ctx, h.domain, &d_sum, &d_cnt)) return r;
+    qd.g_sum = d_sum;
+    qd.g_cnt = d_cnt;
+    L.q0 = qd;
+    L.err = reinterpret_cast<uint32_t *>(ctx->d_scratch);
+    GOLAP_HIP_CHECK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    if (L.n_items) GOLAP_HIP_CHECK(ctx, launch_scan(ctx, L, ctx->stream));
+    GOLAP_HIP_CHECK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    uint32_t flags = 0;
+    GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(&flags, ctx->d_scratch, 4, hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<uint16_t> mask;
+    if (plan->dev_chunk_mask && plan->chunk_end > plan->chunk_begin) {
+        mask.resize(plan->chunk_end - plan->chunk_begin);
+        GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(mask.data(), plan->dev_chunk_mask + plan->chunk_begin, mask.size() * 2,
+                                            hipMemcpyDeviceToHost, ctx->stream));
+    }
+    GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (flags) return error_flags_status(ctx, flags, "scan");
+    float ms = 0;
+    GOLAP_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    res->kernel_ms = ms;
+    res->lds_group_slots = L.lds_gslots;
+    scan_stats(*plan, qd.colmask, scan_survivors(*plan, mask), res);
+    return emit_groups(ctx, h.domain, d_sum, d_cnt, h.lo, h.card, h.stride, &res->n_groups, &res->total_sum,
+                       group_keys, sums, counts, max_groups, &res->rows_matched);
+}
+
+}  // extern "C"

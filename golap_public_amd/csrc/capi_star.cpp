@@ -48,17 +48,12 @@ uint32_t choose_lds_slots(uint32_t agg_mode, uint32_t requested, uint64_t n_grou
     return (uint32_t)std::min<uint64_t>(pow2_at_least(std::max<uint64_t>(n_groups * 2, 64)), LDS_GROUP_SLOTS_MAX);
 }
 
-// Copies the dense result back and emits the groups with rows, in index (= key tuple) order.
+// Emits the groups with rows of a dense host result, in index (= key tuple) order.
 // *rows (optional) = the sum of the counts = the rows that reached the aggregation.
-int emit_groups(golap_ctx *ctx, uint64_t n, const int64_t *d_sum, const uint64_t *d_cnt,
-                const std::vector<int32_t> &lo, const std::vector<uint32_t> &card,
-                const std::vector<uint64_t> &stride, uint64_t *n_groups, int64_t *total,
-                int32_t *keys, int64_t *sums, uint64_t *counts, uint64_t max_groups, uint64_t *rows) {
-    std::vector<int64_t> hs(n);
-    std::vector<uint64_t> hc(n);
-    GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(hs.data(), d_sum, n * 8, hipMemcpyDeviceToHost, ctx->stream));
-    GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(hc.data(), d_cnt, n * 8, hipMemcpyDeviceToHost, ctx->stream));
-    GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+int emit_groups_host(golap_ctx *ctx, uint64_t n, const int64_t *hs, const uint64_t *hc,
+                     const std::vector<int32_t> &lo, const std::vector<uint32_t> &card,
+                     const std::vector<uint64_t> &stride, uint64_t *n_groups, int64_t *total, int32_t *keys,
+                     int64_t *sums, uint64_t *counts, uint64_t max_groups, uint64_t *rows) {
     uint64_t g = 0, nr = 0;
     int64_t t = 0;
     const size_t na = card.size();
@@ -79,6 +74,20 @@ int emit_groups(golap_ctx *ctx, uint64_t n, const int64_t *d_sum, const uint64_t
     if (rows) *rows = nr;
     if (g > max_groups) return fail(ctx, GOLAP_ECAPACITY, "more groups than max_groups");
     return GOLAP_OK;
+}
+
+// Copies a dense device result back and emits it.
+int emit_groups(golap_ctx *ctx, uint64_t n, const int64_t *d_sum, const uint64_t *d_cnt,
+                const std::vector<int32_t> &lo, const std::vector<uint32_t> &card,
+                const std::vector<uint64_t> &stride, uint64_t *n_groups, int64_t *total,
+                int32_t *keys, int64_t *sums, uint64_t *counts, uint64_t max_groups, uint64_t *rows) {
+    std::vector<int64_t> hs(n);
+    std::vector<uint64_t> hc(n);
+    GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(hs.data(), d_sum, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+    GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(hc.data(), d_cnt, n * 8, hipMemcpyDeviceToHost, ctx->stream));
+    GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return emit_groups_host(ctx, n, hs.data(), hc.data(), lo, card, stride, n_groups, total, keys, sums, counts,
+                            max_groups, rows);
 }
 
 bool is_int_type(uint32_t t) { return t >= GOLAP_I8 && t <= GOLAP_I64; }
@@ -190,11 +199,11 @@ int prepare_query(golap_ctx *ctx, const golap_star_plan *plan, bool compact, Col
         qd.filter_words = 1u << HASH_FILTER_WORDS_LOG2;
     }
     const uint64_t ppc = v->chunk_rows / v->page_rows;
-    qd.chunk_begin = plan->chunk_begin;
-    qd.chunk_end = plan->chunk_end;
-    qd.page_begin = plan->chunk_begin * ppc;
-    qd.page_end = std::min<uint64_t>(v->n_pages, plan->chunk_end * ppc);
-    qd.chunk_mask = plan->dev_chunk_mask;
+    qd.s.chunk_begin = plan->chunk_begin;
+    qd.s.chunk_end = plan->chunk_end;
+    qd.s.page_begin = plan->chunk_begin * ppc;
+    qd.s.page_end = std::min<uint64_t>(v->n_pages, plan->chunk_end * ppc);
+    qd.s.chunk_mask = plan->dev_chunk_mask;
     qd.n_groups = qh.domain;
     return GOLAP_OK;
 }
@@ -322,13 +331,13 @@ int golap_star_run(golap_ctx *ctx, const golap_star_plan *plan, golap_star_resul
     StarLaunch L{};
     L.n_cols = (uint32_t)cs.cols.size();
     for (uint32_t c = 0; c < L.n_cols; ++c) L.col[c] = view(cs.cols[c]);
-    L.n_windows = assign_windows(cs, qd.page_begin, qd.page_end, L.col_window);
+    L.n_windows = assign_windows(cs, qd.s.page_begin, qd.s.page_end, L.col_window);
     L.page_rows = v->page_rows;
     L.pages_per_chunk = (uint32_t)(v->chunk_rows / v->page_rows);
     L.n_rows = v->n_rows;
     L.n_pages = v->n_pages;
     L.n_queries = 1;
-    L.n_items = qd.page_end - qd.page_begin;
+    L.n_items = qd.s.page_end - qd.s.page_begin;
     L.filter_words = qd.filter_words;
     // AUTO: LDS partial tables pay off when many rows land in each group
     const double expect = (double)L.n_items * L.page_rows * qh.sel;

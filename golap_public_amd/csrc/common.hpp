@@ -10,6 +10,7 @@
 
 #include "../../include/golap_hip.h"
 #include "host_column.hpp"
+#include "udiv.hpp"
 
 // Device-side error flags raised by kernels (bit-or into ctx scratch word 0).
 enum : uint32_t { DEVERR_CORRUPT = 1u, DEVERR_CAPACITY = 2u, DEVERR_KEYRANGE = 4u };
@@ -146,10 +147,29 @@ struct DriverFilterView {
 };
 
 constexpr uint32_t STAR_MAX_COLS = 8;  // fact columns one star launch can read
+constexpr uint32_t SLICE_MAX_ZONES = 2;
+
+// Zone-map predicate of check_mmmeta (core/metadata.cuh:539-547): chunk c may hold matching
+// rows iff lo <= zmax[c] && hi >= zmin[c].
+struct ZonePred {
+    const int64_t *zmin, *zmax;  // zmin null = unused
+    int64_t lo, hi;
+};
+
+// A query's fact slice; the first member of every query descriptor, so one work-list builder
+// serves every query kind.
+struct SliceDev {
+    uint64_t chunk_begin, chunk_end;
+    uint64_t page_begin, page_end;  // the same slice in pages (static schedule)
+    const uint16_t *chunk_mask;     // zone-map flags per chunk, may be null
+    ZonePred zone[SLICE_MAX_ZONES];
+    uint64_t item_begin;            // static schedule: pages of the queries before this one
+};
 
 // One query of the persistent star kernel. Its columns are indexes into the launch's fact
 // columns, so queries over different foreign keys of one fact table share a launch.
 struct StarQueryDev {
+    SliceDev s;
     StarDimProbe dim[STAR_MAX_DIMS];
     uint8_t fk_col[STAR_MAX_DIMS];
     uint8_t val_col;
@@ -160,12 +180,6 @@ struct StarQueryDev {
     DriverFilterView filter;
     uint32_t filter_words;
     uint32_t pad0;
-    uint64_t page_begin, page_end;  // fact slice in pages (static schedule)
-    uint64_t chunk_begin, chunk_end;
-    const uint16_t *chunk_mask;     // zone-map flags per chunk, may be null
-    const int64_t *zmin, *zmax;     // zone predicate zlo <= zmax[c] && zhi >= zmin[c]; zmin null = none
-    int64_t zlo, zhi;
-    uint64_t item_begin;            // static schedule: pages of the queries before this one
     uint64_t n_groups;
     int64_t *g_sum;
     uint64_t *g_cnt;
@@ -199,14 +213,71 @@ struct StarLaunch {
 // capture then records the launch alone).
 hipError_t launch_star(golap_ctx *ctx, const StarLaunch &l, hipStream_t stream, bool prepare_only = false);
 // Batch work list: one thread per (query, chunk of its slice); surviving chunks (chunk mask and
-// zone predicate) emit parts items each into items, counted in queue[1].
-hipError_t launch_batch_items(const StarQueryDev *queries, uint32_t n_queries, const uint64_t *chunk_off,
+// zone predicates) emit parts items each into items, counted in queue[1]. Query q's slice is
+// the SliceDev at qbase + q * qstride.
+hipError_t launch_batch_items(const void *qbase, uint32_t qstride, uint32_t n_queries, const uint64_t *chunk_off,
                               uint64_t total_chunks, uint32_t parts, uint2 *items, uint32_t *queue,
                               hipStream_t stream);
 hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t stream);
 size_t star_lds_bytes(uint32_t filter_words, uint32_t page_rows, uint32_t n_windows, uint32_t lds_gslots);
 hipError_t launch_hash_filter(golap_ctx *ctx, const uint64_t *slots, uint32_t log2cap, uint32_t *filter,
                               uint32_t words_log2);
+
+// ---- scan + group-by over typed columns (trip-table plans)
+constexpr uint32_t SCAN_MAX_COLS = 8, SCAN_MAX_PREDS = 4, SCAN_MAX_KEYS = 3;
+
+struct ScanColView {
+    ColView c;
+    uint32_t type;
+    int32_t woff;  // byte offset of the column's LDS window, -1 = none
+};
+struct ScanPredDev {
+    uint32_t col, pad;
+    int64_t lo, hi;
+};
+// Computed group key k = (v - base) / div over [0, card): span = card * div < 2^32, so the
+// offset fits 32 bits and divides by the multiply-high method.
+struct ScanKeyDev {
+    uint32_t col, card;
+    int64_t base;
+    uint64_t span;
+    golap_fmt::UDiv32 div;
+    uint32_t pad;
+    uint64_t stride;
+};
+struct ScanQueryDev {
+    SliceDev s;
+    ScanPredDev pred[SCAN_MAX_PREDS];
+    ScanKeyDev key[SCAN_MAX_KEYS];
+    uint32_t n_preds, n_keys;
+    int32_t val_col;     // -1: COUNT only
+    uint32_t use_lds;
+    uint32_t colmask;    // fact columns the query reads
+    uint32_t pad;
+    uint64_t n_groups;
+    int64_t *g_sum;
+    uint64_t *g_cnt;
+};
+struct ScanLaunch {
+    ScanColView col[SCAN_MAX_COLS];
+    uint32_t n_cols;
+    uint32_t page_rows, pages_per_chunk;
+    uint64_t n_rows, n_pages;
+    uint32_t win_bytes;  // LDS bytes of all windows
+    ScanQueryDev q0;     // static schedule with queries == nullptr
+    const ScanQueryDev *queries;
+    uint32_t n_queries;
+    uint64_t n_items;    // static schedule: pages of all queries
+    const uint2 *items;  // queue schedule (as StarLaunch)
+    uint32_t *queue;
+    uint32_t pages_per_item;
+    uint32_t lds_gslots, group_bits;
+    uint32_t flush_fill;  // LDS table flushed to global at an item end when it holds more keys
+    uint32_t max_blocks_per_cu;
+    uint32_t *err;
+};
+hipError_t launch_scan(golap_ctx *ctx, const ScanLaunch &l, hipStream_t stream, bool prepare_only = false);
+size_t scan_lds_bytes(uint32_t win_bytes, uint32_t lds_gslots);
 
 struct GroupByLaunch {
     const void *key[4];

@@ -148,6 +148,34 @@ __device__ inline int64_t dbp_decode_wave(const uint8_t *__restrict__ src, uint3
     return n;
 }
 
+// FOR page header check; returns rows or -1.
+__device__ __forceinline__ int64_t for_check(const uint8_t *src, uint32_t csize, uint32_t elem_want) {
+    if (csize < 32) return -1;
+    const uint32_t n = ld_u32_unaligned(src + 16), bw = src[20], elem = src[21];
+    if (bw > golap_fmt::FOR_MAX_BW || (elem_want ? elem != elem_want : (elem != 1 && elem != 2 && elem != 4 && elem != 8)))
+        return -1;
+    if (golap_fmt::for_page_bytes(n, bw) > csize) return -1;
+    return n;
+}
+// Row r of a checked FOR page (payload = src + 32): one unaligned 8-byte read.
+__device__ __forceinline__ uint64_t for_value(const uint8_t *payload, uint64_t base, uint32_t bw, uint32_t r) {
+    const uint64_t bit = (uint64_t)r * bw;
+    const uint64_t x = ld_u64_unaligned(payload + (bit >> 3)) >> (bit & 7);
+    return base + (bw ? (x & (~0ull >> (64 - bw))) : 0ull);
+}
+// FOR page -> dst. Returns rows or -1.
+__device__ inline int64_t for_decode_wave(const uint8_t *__restrict__ src, uint32_t csize,
+                                          uint8_t *__restrict__ dst, uint32_t cap) {
+    const int64_t n = for_check(src, csize, 0);
+    if (n < 0) return -1;
+    const uint32_t bw = src[20], elem = src[21];
+    if ((uint64_t)n * elem > cap) return -1;
+    const uint64_t base = ld_u64_unaligned(src);
+    for (uint32_t r = lane_id(); r < (uint32_t)n; r += WAVE)
+        store_elem(dst + (uint64_t)r * elem, for_value(src + 32, base, bw, r), elem);
+    return n;
+}
+
 // RLE page -> dst. Returns rows or -1.
 __device__ inline int64_t rle_decode_wave(const uint8_t *__restrict__ src, uint32_t csize,
                                           uint8_t *__restrict__ dst, uint32_t cap) {
@@ -187,6 +215,10 @@ __device__ inline int64_t page_decode_wave(uint8_t codec, const uint8_t *__restr
     }
     case golap_fmt::RLE: {
         int64_t r = rle_decode_wave(src, csize, dst, cap);
+        return r < 0 ? -1 : r * elem;
+    }
+    case golap_fmt::FOR: {
+        int64_t r = for_decode_wave(src, csize, dst, cap);
         return r < 0 ? -1 : r * elem;
     }
     default: return -1;

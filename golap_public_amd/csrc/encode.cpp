@@ -140,6 +140,47 @@ uint64_t dbp_encode_page(const uint8_t *src, uint64_t rows, uint32_t type, uint8
     return sizeof(h) + words * 4;
 }
 
+// ------------------------------------------------------------------------------------ FOR
+uint64_t for_bound(uint64_t rows) { return golap_fmt::for_page_bytes(rows, golap_fmt::FOR_MAX_BW); }
+
+// Frame of reference: the page minimum plus bw-bit offsets; 0 if the range needs more than
+// FOR_MAX_BW bits.
+uint64_t for_encode_page(const uint8_t *src, uint64_t rows, uint32_t type, uint8_t *dst) {
+    const uint32_t elem = type_size(type);
+    if (!rows) return 0;
+    int64_t mn = load_value(src, type), mx = mn;
+    for (uint64_t i = 1; i < rows; ++i) {
+        const int64_t v = load_value(src + i * elem, type);
+        mn = std::min(mn, v);
+        mx = std::max(mx, v);
+    }
+    const unsigned __int128 range = (unsigned __int128)((__int128)mx - mn);
+    uint32_t bw = 0;
+    while (bw < 64 && (range >> bw) != 0) ++bw;
+    if (bw > golap_fmt::FOR_MAX_BW) return 0;
+    DbpHeader h{};
+    h.base = mn;
+    h.n = (uint32_t)rows;
+    h.bw = (uint8_t)bw;
+    h.elem = (uint8_t)elem;
+    memcpy(dst, &h, sizeof(h));
+    const uint64_t total = golap_fmt::for_page_bytes(rows, bw);
+    uint8_t *pk = dst + sizeof(h);
+    memset(pk, 0, total - sizeof(h));
+    if (bw)
+        for (uint64_t i = 0; i < rows; ++i) {
+            const uint64_t off = (uint64_t)((__int128)load_value(src + i * elem, type) - mn);
+            const uint64_t bit = i * bw;
+            for (uint32_t b = 0; b < bw;) {  // byte-wise, LSB first
+                const uint64_t pos = bit + b;
+                const uint32_t take = std::min<uint32_t>(bw - b, 8 - (uint32_t)(pos & 7));
+                pk[pos >> 3] |= (uint8_t)(((off >> b) & ((1u << take) - 1)) << (pos & 7));
+                b += take;
+            }
+        }
+    return total;
+}
+
 // ------------------------------------------------------------------------------------ RLE
 uint64_t rle_bound(uint64_t rows, uint32_t elem) {
     return sizeof(RleHeader) + ((rows * elem + 3) & ~3ull) + 4 * rows;
@@ -179,7 +220,7 @@ struct PageOut {
     uint8_t codec;   // codec | PAGE_LITERAL_ONLY
 };
 
-// Codes one page; for AUTO the smallest of LZ4 / DBP / RLE wins (ties keep the earlier).
+// Codes one page; for AUTO the smallest of LZ4 / DBP / RLE / FOR wins (ties keep the earlier).
 void code_page(const uint8_t *src, uint64_t rows, uint32_t type, uint32_t elem, uint32_t codec,
                uint32_t min_match, int64_t *ht, int64_t base, std::vector<uint8_t> &tmp,
                std::vector<uint8_t> &buf, PageOut &po) {
@@ -220,7 +261,11 @@ void code_page(const uint8_t *src, uint64_t rows, uint32_t type, uint32_t elem, 
         cand.resize(rle_bound(rows, elem));
         consider(rle_encode_page(src, rows, elem, cand.data()), (uint8_t)(RLE | PAGE_POINT));
     }
-    if (best == UINT64_MAX) {  // DBP requested but the page does not fit 32-bit deltas
+    if (codec == golap_fmt::FOR || codec == GOLAP_CODEC_AUTO) {
+        cand.resize(for_bound(rows));
+        consider(for_encode_page(src, rows, type, cand.data()), (uint8_t)(golap_fmt::FOR | PAGE_POINT));
+    }
+    if (best == UINT64_MAX) {  // DBP / FOR requested but the page's deltas or range are too wide
         cand.resize(lz4_bound(nbytes));
         uint64_t sz = lz4_encode_block(src, nbytes, cand.data(), min_match, ht, base);
         // a block with a match can have exactly the literal-only size: parse, do not compare sizes
@@ -241,7 +286,7 @@ int encode_column(const void *values, uint64_t n_rows, uint32_t type, const gola
         return GOLAP_EINVAL;
     }
     if ((uint64_t)o.page_rows * elem > (1u << 20)) { err = "encode: page larger than 1 MiB"; return GOLAP_EINVAL; }
-    if (o.codec != RAW && o.codec != LZ4 && o.codec != DBP && o.codec != RLE &&
+    if (o.codec != RAW && o.codec != LZ4 && o.codec != DBP && o.codec != RLE && o.codec != golap_fmt::FOR &&
         o.codec != GOLAP_CODEC_AUTO) { err = "encode: unknown codec"; return GOLAP_EINVAL; }
     if (n_rows && !values) { err = "encode: null values"; return GOLAP_EINVAL; }
     const uint8_t *src = static_cast<const uint8_t *>(values);

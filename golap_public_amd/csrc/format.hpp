@@ -14,6 +14,8 @@
 //             match-len bytes; the last sequence has literals only.
 //   DBP  (2)  frame-of-reference delta bit-packing (sorted or slowly varying integers).
 //   RLE  (3)  run-length encoding (value, exclusive end row) pairs.
+//   FOR  (4)  frame of reference: v[r] = base + the r-th bw-bit field of a little-endian bit
+//             stream (unsorted low-range integers); every row is readable on its own.
 // Flag bits above the codec: PAGE_POINT (0x40, see below) and PAGE_LITERAL_ONLY (0x80), which
 // marks an LZ4 page made of a single literal run; such a page's payload starts lz4_literal_header_bytes(n_bytes) bytes after the page start and the
 // encoder places it 16-byte aligned in the column buffer, so a scan reads it in place.
@@ -22,11 +24,12 @@
 
 namespace golap_fmt {
 
-enum Codec : uint8_t { RAW = 0, LZ4 = 1, DBP = 2, RLE = 3 };
+enum Codec : uint8_t { RAW = 0, LZ4 = 1, DBP = 2, RLE = 3, FOR = 4 };
 constexpr uint8_t PAGE_CODEC_MASK = 0x0f;
 constexpr uint8_t PAGE_LITERAL_ONLY = 0x80;
 // Bit 6 (PAGE_POINT): any row of the page can be read without decoding the page: every RLE
-// page (binary search over the run ends) and DBP pages with bw == 0 (v[r] = base + r*min_delta).
+// page (binary search over the run ends), every FOR page (one unaligned 8-byte read) and DBP
+// pages with bw == 0 (v[r] = base + r*min_delta).
 constexpr uint8_t PAGE_POINT = 0x40;
 
 constexpr uint32_t LZ4_MINMATCH = 4;
@@ -59,6 +62,12 @@ struct RleHeader {
     uint8_t pad[7];
 };
 static_assert(sizeof(RleHeader) == 16, "RleHeader layout");
+
+// FOR page header (32 bytes, the DbpHeader layout with min_delta unused) followed by
+// ceil(n*bw/8) bytes of bw-bit fields (field r at bits [r*bw, r*bw + bw) of the byte stream,
+// LSB first) and 8 zero bytes, so an 8-byte read at byte (r*bw)/8 never leaves the page.
+constexpr uint32_t FOR_MAX_BW = 56;
+__host__ __device__ inline uint64_t for_page_bytes(uint64_t n, uint32_t bw) { return 32 + (n * bw + 7) / 8 + 8; }
 
 constexpr uint32_t PAGE_ALIGN = 16;
 

@@ -257,13 +257,13 @@ __global__ __launch_bounds__(BLOCK) void k_star(StarArgs a) {
         uint32_t q = 0;
         for (uint64_t it = i0; it < i1; ++it) {
             if (a.queries)
-                while (q + 1 < a.n_queries && it >= a.queries[q + 1].item_begin) ++q;
+                while (q + 1 < a.n_queries && it >= a.queries[q + 1].s.item_begin) ++q;
             if (q != cur) {
                 star_set_query(a, S, q, lfilter);
                 cur = q;
             }
-            const uint64_t p = S.q.page_begin + (it - S.q.item_begin);
-            if (S.q.chunk_mask && S.q.chunk_mask[p / a.pages_per_chunk] == 0) continue;
+            const uint64_t p = S.q.s.page_begin + (it - S.q.s.item_begin);
+            if (!slice_keeps(S.q.s, p / a.pages_per_chunk)) continue;
             star_tile<PER>(a, S, q, p, lfilter, cand, win, T);
         }
     } else {
@@ -367,9 +367,9 @@ hipError_t launch_star(golap_ctx *ctx, const StarLaunch &l, hipStream_t stream, 
 // Batch work list. Wave w handles the flattened chunk indexes [64w, 64w + 64) of the queries'
 // slices (chunk_off: prefix of slice lengths in chunks); survivors are compacted with a wave
 // ballot and one global atomic per wave, then write `parts` items each.
-__global__ __launch_bounds__(BLOCK) void k_batch_items(const StarQueryDev *qs, uint32_t nq, const uint64_t *chunk_off,
-                                                       uint64_t total, uint32_t parts, uint2 *items,
-                                                       uint32_t *queue) {
+__global__ __launch_bounds__(BLOCK) void k_batch_items(const uint8_t *qbase, uint32_t qstride, uint32_t nq,
+                                                       const uint64_t *chunk_off, uint64_t total, uint32_t parts,
+                                                       uint2 *items, uint32_t *queue) {
     const uint32_t lane = threadIdx.x % WAVE;
     const uint64_t waves = (uint64_t)gridDim.x * (BLOCK / WAVE);
     for (uint64_t w = (uint64_t)blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE; w * WAVE < total; w += waves) {
@@ -385,10 +385,9 @@ __global__ __launch_bounds__(BLOCK) void k_batch_items(const StarQueryDev *qs, u
                 else hi = mid;
             }
             q = lo;
-            const StarQueryDev &Q = qs[q];
+            const SliceDev &Q = *reinterpret_cast<const SliceDev *>(qbase + (size_t)q * qstride);
             c = Q.chunk_begin + (i - chunk_off[q]);
-            pass = !Q.chunk_mask || Q.chunk_mask[c];
-            if (pass && Q.zmin) pass = Q.zlo <= Q.zmax[c] && Q.zhi >= Q.zmin[c];
+            pass = slice_keeps(Q, c);
         }
         const uint64_t ball = __ballot(pass);
         uint32_t base = 0;
@@ -402,15 +401,15 @@ __global__ __launch_bounds__(BLOCK) void k_batch_items(const StarQueryDev *qs, u
     }
 }
 
-hipError_t launch_batch_items(const StarQueryDev *queries, uint32_t n_queries, const uint64_t *chunk_off,
+hipError_t launch_batch_items(const void *qbase, uint32_t qstride, uint32_t n_queries, const uint64_t *chunk_off,
                               uint64_t total_chunks, uint32_t parts, uint2 *items, uint32_t *queue,
                               hipStream_t stream) {
     if (!n_queries || !parts || parts > 256 || n_queries > (1u << 24)) return hipErrorInvalidValue;
     if (!total_chunks) return hipSuccess;
     const uint64_t waves = (total_chunks + WAVE - 1) / WAVE;
     const uint32_t grid = (uint32_t)std::min<uint64_t>((waves + BLOCK / WAVE - 1) / (BLOCK / WAVE), 4096);
-    hipLaunchKernelGGL(k_batch_items, dim3(grid), dim3(BLOCK), 0, stream, queries, n_queries, chunk_off,
-                       total_chunks, parts, items, queue);
+    hipLaunchKernelGGL(k_batch_items, dim3(grid), dim3(BLOCK), 0, stream, static_cast<const uint8_t *>(qbase),
+                       qstride, n_queries, chunk_off, total_chunks, parts, items, queue);
     return hipGetLastError();
 }
 

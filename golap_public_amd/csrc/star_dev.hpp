@@ -41,18 +41,27 @@ __device__ __forceinline__ uint32_t fmix32(uint32_t h) {
 __device__ __forceinline__ uint32_t bloom_word(uint32_t h, uint32_t wlog2) { return h >> (32 - wlog2); }
 __device__ __forceinline__ uint32_t bloom_bits(uint32_t h) { return (1u << (h & 31)) | (1u << ((h >> 5) & 31)); }
 
+// Chunk c of a query's slice holds candidate rows: its mask flag is set and every zone
+// predicate can hold (check_mmmeta, core/metadata.cuh:539-547).
+__device__ __forceinline__ bool slice_keeps(const SliceDev &s, uint64_t c) {
+    bool pass = !s.chunk_mask || s.chunk_mask[c];
+    for (uint32_t z = 0; z < SLICE_MAX_ZONES; ++z)
+        if (pass && s.zone[z].zmin) pass = s.zone[z].lo <= s.zone[z].zmax[c] && s.zone[z].hi >= s.zone[z].zmin[c];
+    return pass;
+}
+
 struct StarCol {
     ColView c;
     int window;  // LDS window index when some page of the column in range needs decoding, else -1
 };
 
 // How one 32-bit column is read at a tile row.
-enum : uint32_t { ACC_DIRECT = 0, ACC_WINDOW = 1, ACC_AFFINE = 2, ACC_RLE = 3, ACC_BAD = 4 };
+enum : uint32_t { ACC_DIRECT = 0, ACC_WINDOW = 1, ACC_AFFINE = 2, ACC_RLE = 3, ACC_FOR = 4, ACC_BAD = 5 };
 struct ColAccess {
-    const uint8_t *base;  // DIRECT: payload in HBM; WINDOW: LDS window; RLE: page start
-    int64_t a, b;         // AFFINE: v[r] = a + r * b
+    const uint8_t *base;  // DIRECT: payload in HBM; WINDOW: LDS window; RLE: page start; FOR: fields
+    int64_t a, b;         // AFFINE: v[r] = a + r * b; FOR: a = base
     uint32_t mode;
-    uint32_t nruns;       // RLE
+    uint32_t nruns;       // RLE: runs; FOR: bits per field
 };
 
 // Sets up the access to column sc for page p (rows rows); wwin: its LDS window or null.
@@ -73,6 +82,15 @@ __device__ inline ColAccess col_access(const StarCol &sc, uint64_t p, uint32_t r
             x.b = (int64_t)ld_u64_unaligned(src + 8);
             x.mode = ACC_AFFINE;
         }
+    } else if ((codec & PAGE_CODEC_MASK) == golap_fmt::FOR) {
+        if (for_check(src, csize, 4) != (int64_t)rows) {
+            x.mode = ACC_BAD;
+        } else {
+            x.base = src + 32;
+            x.a = (int64_t)ld_u64_unaligned(src);
+            x.nruns = src[20];
+            x.mode = ACC_FOR;
+        }
     } else if ((codec & PAGE_CODEC_MASK) == golap_fmt::RLE) {
         const uint32_t nruns = ld_u32_unaligned(src), n = ld_u32_unaligned(src + 4);
         const bool ok = csize >= 16 && src[8] == 4 && n == rows && nruns >= 1 &&
@@ -91,6 +109,7 @@ __device__ __forceinline__ int32_t col_value(const ColAccess &x, uint32_t r) {
     int32_t v;
     switch (x.mode) {
     case ACC_AFFINE: return (int32_t)(uint32_t)(uint64_t)(x.a + (int64_t)r * x.b);
+    case ACC_FOR: return (int32_t)(uint32_t)for_value(x.base, (uint64_t)x.a, x.nruns, r);
     case ACC_RLE: {
         // first run whose exclusive end is above r
         const uint8_t *ends = x.base + 16 + 4ull * x.nruns;

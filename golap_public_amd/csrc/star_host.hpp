@@ -30,6 +30,15 @@ struct QueryHost {
     std::vector<std::pair<const golap_hashmap *, uint64_t>> builds;  // maps and their build ids
 };
 
+// Host side of one prepared scan plan.
+struct ScanHost {
+    golap_scan_plan plan{};
+    std::vector<int32_t> lo;
+    std::vector<uint32_t> card;
+    std::vector<uint64_t> stride;
+    uint64_t domain = 1;
+};
+
 uint32_t pow2_at_least(uint64_t v);
 uint32_t bits_for(uint64_t n);
 uint32_t choose_lds_slots(uint32_t agg_mode, uint32_t requested, uint64_t n_groups);
@@ -38,12 +47,22 @@ int emit_groups(golap_ctx *ctx, uint64_t n, const int64_t *d_sum, const uint64_t
                 const std::vector<int32_t> &lo, const std::vector<uint32_t> &card,
                 const std::vector<uint64_t> &stride, uint64_t *n_groups, int64_t *total, int32_t *keys,
                 int64_t *sums, uint64_t *counts, uint64_t max_groups, uint64_t *rows = nullptr);
+int emit_groups_host(golap_ctx *ctx, uint64_t n, const int64_t *hs, const uint64_t *hc,
+                     const std::vector<int32_t> &lo, const std::vector<uint32_t> &card,
+                     const std::vector<uint64_t> &stride, uint64_t *n_groups, int64_t *total, int32_t *keys,
+                     int64_t *sums, uint64_t *counts, uint64_t max_groups, uint64_t *rows = nullptr);
 int prepare_query(golap_ctx *ctx, const golap_star_plan *plan, bool compact, ColumnSet &cs, StarQueryDev &qd,
                   QueryHost &qh);
 uint32_t assign_windows(const ColumnSet &cs, uint64_t pb, uint64_t pe, int *win);
 void slice_stats(const golap_star_plan &plan, const std::vector<uint8_t> &run, golap_star_result *res);
 int status_of_flags(golap_ctx *ctx, uint32_t flags, const char *what);
 bool is_int_type(uint32_t t);
+
+int prepare_scan(golap_ctx *ctx, const golap_scan_plan *p, ScanQueryDev &qd, ScanHost &h);
+std::vector<uint8_t> scan_survivors(const golap_scan_plan &p, const std::vector<uint16_t> &mask);
+void scan_stats(const golap_scan_plan &p, uint32_t colmask, const std::vector<uint8_t> &run, golap_star_result *res);
+uint32_t scan_windows(const golap_scan_plan &p, uint32_t colmask, uint64_t pb, uint64_t pe, int32_t *woff);
+uint32_t scan_slots(golap_ctx *ctx, uint32_t agg_mode, uint32_t requested, uint64_t domain, uint32_t win_bytes);
 
 }  // namespace starhost
 }  // namespace golap_impl

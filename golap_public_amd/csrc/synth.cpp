@@ -4,6 +4,7 @@
 // * span) >> 32), the multiply-shift range reduction of util::uniform_int (core/util.hpp:177-181)
 // applied to a SplitMix64 counter hash, so any row range can be generated independently.
 #include <algorithm>
+#include <cmath>
 #include <cstdint>
 #include <thread>
 #include <vector>
@@ -23,6 +24,23 @@ inline uint32_t uniform(uint64_t seed, uint32_t col, uint64_t row, uint32_t lo, 
     const uint64_t r = splitmix64((seed * 0x9E3779B97F4A7C15ull) ^ ((uint64_t)col << 48) ^ row);
     const uint64_t span = (uint64_t)hi - lo + 1;
     return lo + (uint32_t)(((r >> 32) * span) >> 32);
+}
+
+// 53 random bits of (seed, column, row).
+inline uint64_t bits53(uint64_t seed, uint32_t col, uint64_t row) {
+    return splitmix64((seed * 0x9E3779B97F4A7C15ull) ^ ((uint64_t)col << 48) ^ row) >> 11;
+}
+
+// exp(mu + sigma * z) for a standard normal z drawn by Box-Muller from two columns, rounded
+// half up and clipped to [lo, hi].
+inline int32_t lognormal(uint64_t seed, uint32_t col, uint64_t row, double mu, double sigma, int64_t lo,
+                         int64_t hi) {
+    const double u1 = (double)(bits53(seed, col, row) + 1) * 0x1p-53;  // (0, 1]
+    const double u2 = (double)bits53(seed, col + 1, row) * 0x1p-53;     // [0, 1)
+    const double z = std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    const double x = std::floor(std::exp(mu + sigma * z) + 0.5);
+    const double c = std::min((double)hi, std::max((double)lo, x));
+    return (int32_t)(int64_t)c;
 }
 
 template <typename F>
@@ -104,6 +122,38 @@ int golap_gen_star_fact(uint64_t seed, uint64_t row0, uint64_t n, uint64_t n_tot
             if (supp_key) supp_key[i] = (int32_t)uniform(seed, 11, r, 0, n_supp - 1);
             if (part_key) part_key[i] = (int32_t)uniform(seed, 12, r, 0, n_part - 1);
             if (revenue) revenue[i] = (int32_t)uniform(seed, 13, r, 100, 5000000);
+        }
+    });
+    return GOLAP_OK;
+}
+
+// Trip-table rows [row0, row0 + n) of n_total.
+int golap_gen_trip(uint64_t seed, uint64_t row0, uint64_t n, uint64_t n_total, int64_t *pickup_ts,
+                   uint8_t *passenger, uint16_t *zone, int32_t *fare, int32_t *distance, uint32_t threads) {
+    if (pickup_ts && !n_total) return GOLAP_EINVAL;
+    // Zipf(s = 1.2) over 1..6 by inverse CDF on 53 bits: thresholds floor(cdf_k * 2^53)
+    double w[6], tot = 0;
+    for (int k = 0; k < 6; ++k) tot += (w[k] = std::pow((double)(k + 1), -1.2));
+    uint64_t thr[5];
+    double cum = 0;
+    for (int k = 0; k < 5; ++k) {
+        cum += w[k];
+        thr[k] = (uint64_t)std::floor(cum / tot * 0x1p53);
+    }
+    parallel_rows(n, threads, [&](uint64_t b, uint64_t e) {
+        for (uint64_t i = b; i < e; ++i) {
+            const uint64_t r = row0 + i;
+            if (pickup_ts)
+                pickup_ts[i] = GOLAP_TRIP_T0 + (int64_t)((unsigned __int128)r * GOLAP_TRIP_SPAN / n_total);
+            if (passenger) {
+                const uint64_t u = bits53(seed, 30, r);
+                uint8_t p = 1;
+                for (int k = 0; k < 5; ++k) p += u >= thr[k];
+                passenger[i] = p;
+            }
+            if (zone) zone[i] = (uint16_t)uniform(seed, 31, r, 1, 265);
+            if (fare) fare[i] = lognormal(seed, 32, r, 7.2, 0.6, 250, 50000);
+            if (distance) distance[i] = lognormal(seed, 34, r, 8.0, 0.8, 1, INT32_MAX);
         }
     });
     return GOLAP_OK;

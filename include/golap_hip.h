@@ -43,7 +43,8 @@ enum golap_type {
 /* Page codecs (CompInfo::comp_algo, core/comp.cuh:35-78). */
 enum golap_codec {
     GOLAP_CODEC_RAW = 0, GOLAP_CODEC_LZ4 = 1, GOLAP_CODEC_DBP = 2, GOLAP_CODEC_RLE = 3,
-    GOLAP_CODEC_AUTO = 255 /* smallest of LZ4/DBP/RLE per column */
+    GOLAP_CODEC_FOR = 4,   /* bit-packed frame of reference (point-readable) */
+    GOLAP_CODEC_AUTO = 255 /* smallest of LZ4/DBP/RLE/FOR per page */
 };
 
 typedef struct golap_ctx golap_ctx;
@@ -264,6 +265,49 @@ typedef struct {
 int golap_star_run(golap_ctx *ctx, const golap_star_plan *plan, golap_star_result *res,
                    int32_t *group_keys, int64_t *sums, uint64_t *counts, uint64_t max_groups);
 
+/* ------------------------------------------------------------------ scan + group-by plans
+ * Filter + group-by over row-aligned typed fact columns, fused with decompression:
+ *   - the range predicates of pipeline_q1_lineorder (usecases/ssb/queries/01.cuh:26-54,
+ *     inclusive bounds) on any columns;
+ *   - the grouping of group_by_agg_2col/4col with HashAggregate::add (core/group.cuh:14-36,
+ *     core/dev_structs.cuh:209-244), here over keys computed per row:
+ *     k = (value - base) / div in [0, card). For example the hour of a timestamp, or a zone id
+ *     with div 1. A qualifying row whose key is outside its range is GOLAP_EINVAL;
+ *   - SUM(value column) as int64 and COUNT per group, returned like golap_group_by, with each
+ *     key reported as out_lo + k.
+ * The product of the cards must stay below 2^31 and card * div below 2^32.
+ * With zone_prune, chunks whose zone map fails a predicate on the first two predicate columns
+ * are skipped (check_mmmeta semantics, core/metadata.cuh:539-547). */
+#define GOLAP_SCAN_MAX_COLS 8
+typedef struct {
+    uint32_t col;          /* index into cols */
+    int64_t lo, hi;
+} golap_scan_pred;
+typedef struct {
+    uint32_t col;
+    int64_t base;
+    uint32_t div;          /* >= 1 */
+    uint32_t card;
+    int32_t out_lo;
+} golap_scan_key;
+typedef struct {
+    const golap_column *cols[GOLAP_SCAN_MAX_COLS];
+    uint32_t n_cols;
+    golap_scan_pred preds[4];
+    uint32_t n_preds;
+    golap_scan_key keys[3];
+    uint32_t n_keys;
+    int32_t value_col;     /* index into cols, -1 = COUNT only */
+    uint64_t chunk_begin, chunk_end;
+    const uint16_t *dev_chunk_mask;
+    uint32_t agg_mode;     /* golap_agg_mode (AUTO = LDS tables, flushed as they fill) */
+    uint32_t zone_prune;
+    uint32_t lds_group_slots; /* 0 = library choice */
+} golap_scan_plan;
+/* golap_star_result fields as for star plans; driver_dim is 0. */
+int golap_scan_run(golap_ctx *ctx, const golap_scan_plan *plan, golap_star_result *res, int32_t *group_keys,
+                   int64_t *sums, uint64_t *counts, uint64_t max_groups);
+
 /* ------------------------------------------------------------------ batched star plans
  * Many star plans over one fact table in one persistent fused launch. The reference runs one
  * plan per call, with a worker loop over chunks (query2/query3, 02.hpp:54-307, 03.hpp:79-677),
@@ -335,6 +379,15 @@ int golap_gen_star_fact(uint64_t seed, uint64_t row0, uint64_t n, uint64_t n_tot
                         const int32_t *date_keys, uint32_t n_dates, uint32_t n_cust,
                         uint32_t n_supp, uint32_t n_part, int32_t *date_key, int32_t *cust_key,
                         int32_t *supp_key, int32_t *part_key, int32_t *revenue, uint32_t threads);
+/* Trip-table rows [row0, row0+n) of n_total: pickup_ts (unix seconds) sorted over the 366 days
+ * of 2024, row r at GOLAP_TRIP_T0 + floor(r * GOLAP_TRIP_SPAN / n_total); passenger_count
+ * Zipf(s = 1.2) over 1..6 (inverse CDF on 53 random bits); zone uniform over 1..265; fare_cents
+ * lognormal(mu 7.2, sigma 0.6) and distance_m lognormal(mu 8, sigma 0.8), normal deviates by
+ * Box-Muller, rounded half up, clipped to [250, 50000] and [1, INT32_MAX]. */
+#define GOLAP_TRIP_T0 1704067200LL /* 2024-01-01 00:00:00 UTC */
+#define GOLAP_TRIP_SPAN 31622400LL /* 366 days in seconds */
+int golap_gen_trip(uint64_t seed, uint64_t row0, uint64_t n, uint64_t n_total, int64_t *pickup_ts,
+                   uint8_t *passenger, uint16_t *zone, int32_t *fare, int32_t *distance, uint32_t threads);
 /* out[i] uniform over [lo, hi], rows [0, n), generator column id col. */
 int golap_gen_uniform_i32(uint64_t seed, uint32_t col, uint64_t n, int32_t lo, int32_t hi,
                           int32_t *out, uint32_t threads);

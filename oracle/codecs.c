@@ -91,6 +91,27 @@ int64_t oracle_dbp_decode(const uint8_t *src, uint64_t csize, uint8_t *dst, uint
     return n;
 }
 
+/* FOR page: the 32-byte header of a delta-bitpack page (base = frame of reference, min_delta
+ * unused), then n fields of bw bits (bw <= 56), field r at bits [r*bw, r*bw+bw) of a
+ * little-endian byte stream, then 8 zero bytes; v[r] = base + field r (mod 2^64). */
+int64_t oracle_for_decode(const uint8_t *src, uint64_t csize, uint8_t *dst, uint64_t cap) {
+    if (csize < 32) return -1;
+    uint64_t base = rd_le(src, 8);
+    uint32_t n = (uint32_t)rd_le(src + 16, 4), bw = src[20], elem = src[21];
+    if (bw > 56 || (elem != 1 && elem != 2 && elem != 4 && elem != 8)) return -1;
+    if ((uint64_t)n * elem > cap || 32 + ((uint64_t)n * bw + 7) / 8 + 8 > csize) return -1;
+    const uint8_t *pk = src + 32;
+    for (uint32_t r = 0; r < n; ++r) {
+        uint64_t v = 0;
+        for (uint32_t b = 0; b < bw; ++b) {  /* bit by bit: the plainest reading of the format */
+            uint64_t pos = (uint64_t)r * bw + b;
+            v |= (uint64_t)((pk[pos >> 3] >> (pos & 7)) & 1u) << b;
+        }
+        wr_le(dst + (uint64_t)r * elem, base + v, elem);
+    }
+    return n;
+}
+
 /* RLE page: 16-byte header {u32 nruns, u32 n, u8 elem, 7 pad}, values[nruns] of elem bytes
  * padded to 4, then u32 ends[nruns] (exclusive end row of each run, ends[nruns-1] == n). */
 int64_t oracle_rle_decode(const uint8_t *src, uint64_t csize, uint8_t *dst, uint64_t cap) {
@@ -121,6 +142,7 @@ int64_t oracle_page_decode(uint32_t codec, const uint8_t *src, uint64_t csize, u
     case 1: return oracle_lz4_decode(src, csize, dst, cap);
     case 2: return oracle_dbp_decode(src, csize, dst, cap);
     case 3: return oracle_rle_decode(src, csize, dst, cap);
+    case 4: return oracle_for_decode(src, csize, dst, cap);
     default: return -1;
     }
 }

@@ -67,6 +67,9 @@ void oracle_gen_star_fact(uint64_t seed, uint64_t row0, uint64_t n, uint64_t n_t
                           const int32_t *date_keys, uint32_t n_dates, uint32_t n_cust,
                           uint32_t n_supp, uint32_t n_part, int32_t *date_key, int32_t *cust_key,
                           int32_t *supp_key, int32_t *part_key, int32_t *revenue);
+/* Trip table rows [row0, row0+n) of n_total (definition in include/golap_hip.h, golap_gen_trip). */
+void oracle_gen_trip(uint64_t seed, uint64_t row0, uint64_t n, uint64_t n_total, int64_t *pickup_ts,
+                     uint8_t *passenger, uint16_t *zone, int32_t *fare, int32_t *distance);
 /* out[i] uniform over [lo, hi] (column col), rows [0, n). */
 void oracle_gen_uniform_i32(uint64_t seed, uint32_t col, uint64_t n, int32_t lo, int32_t hi,
                             int32_t *out);
@@ -80,7 +83,8 @@ int64_t oracle_lz4_decode(const uint8_t *src, uint64_t csize, uint8_t *dst, uint
  * Return rows written or -1. */
 int64_t oracle_dbp_decode(const uint8_t *src, uint64_t csize, uint8_t *dst, uint64_t cap);
 int64_t oracle_rle_decode(const uint8_t *src, uint64_t csize, uint8_t *dst, uint64_t cap);
-/* Decodes one page of any codec id (0 raw, 1 lz4, 2 delta-bitpack, 3 rle). */
+int64_t oracle_for_decode(const uint8_t *src, uint64_t csize, uint8_t *dst, uint64_t cap);
+/* Decodes one page of any codec id (0 raw, 1 lz4, 2 delta-bitpack, 3 rle, 4 frame of reference). */
 int64_t oracle_page_decode(uint32_t codec, const uint8_t *src, uint64_t csize, uint8_t *dst,
                            uint64_t cap);
 
@@ -170,6 +174,30 @@ int64_t oracle_star_groupby(const oracle_dim *dims, uint32_t n_dims, const int32
 int64_t oracle_groupby(const int32_t *const *keys, uint32_t n_keys, const int64_t *val,
                        uint64_t n, uint64_t max_groups, int32_t *keys_out, int64_t *sums_out,
                        uint64_t *counts_out);
+
+/* Scan + filter + group-by over typed columns (the trip-table plans). A row qualifies if
+ * lo <= col <= hi for every predicate (inclusive bounds like the reference's range predicates,
+ * usecases/ssb/queries/01.cuh:26-54). Its group is the mixed-radix index (first key most
+ * significant) of k_i = floor((col_i - base_i) / div_i), each required in [0, card_i): the
+ * grouping of group_by_agg_2col/4col with HashAggregate::add (core/group.cuh:14-36,
+ * core/dev_structs.cuh:209-244) over computed attributes. SUM(value column) as wrapping int64
+ * and COUNT per group; val_col < 0 sums nothing. Output: groups with rows in index order, key
+ * tuple out_lo_i + k_i. Returns the number of groups, -1 if a qualifying row's key is out of
+ * range, -2 if there are more than max_groups groups. types: golap_type ids (1..7). */
+typedef struct {
+    uint32_t col;
+    int64_t lo, hi;
+} oracle_scan_pred;
+typedef struct {
+    uint32_t col;
+    int64_t base;
+    uint32_t div, card;
+    int32_t out_lo;
+} oracle_scan_key;
+int64_t oracle_scan_groupby(uint64_t n, const void *const *cols, const uint32_t *types,
+                            const oracle_scan_pred *preds, uint32_t n_preds, const oracle_scan_key *keys,
+                            uint32_t n_keys, int32_t val_col, uint32_t threads, uint64_t max_groups,
+                            int32_t *keys_out, int64_t *sums_out, uint64_t *counts_out);
 
 /* ------------------------------------------------------------------ zone maps
  * MinMaxMeta::check_pred (core/metadata.cuh:195-211) / check_mmmeta (:539-547): a chunk qualifies

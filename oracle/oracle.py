@@ -61,6 +61,8 @@ def load():
             "oracle_gen_q1_fact": (None, [u64, u64, u64, vp, u32, vp, vp, vp, vp]),
             "oracle_gen_date_dim": (None, [u32, vp, vp, vp]),
             "oracle_gen_star_fact": (None, [u64, u64, u64, u64, vp, u32, u32, u32, u32, vp, vp, vp, vp, vp]),
+            "oracle_gen_trip": (None, [u64, u64, u64, u64, vp, vp, vp, vp, vp]),
+            "oracle_scan_groupby": (i64, [u64, vp, vp, vp, u32, vp, u32, C.c_int32, u32, u64, vp, vp, vp]),
             "oracle_gen_uniform_i32": (None, [u64, u32, u64, C.c_int32, C.c_int32, vp]),
             "oracle_lz4_decode": (i64, [vp, u64, vp, u64]),
             "oracle_dbp_decode": (i64, [vp, u64, vp, u64]),
@@ -147,6 +149,14 @@ def gen_star_fact(seed, n, n_total, date_keys, n_cust, n_supp, n_part, row0=0):
     cols = tuple(np.empty(n, np.int32) for _ in range(5))
     load().oracle_gen_star_fact(seed, row0, n, n_total, _p(date_keys), date_keys.size, n_cust, n_supp, n_part,
                                 *[_p(c) for c in cols])
+    return cols
+
+
+def gen_trip(seed, n, n_total=None, row0=0):
+    n_total = n if n_total is None else n_total
+    cols = (np.empty(n, np.int64), np.empty(n, np.uint8), np.empty(n, np.uint16), np.empty(n, np.int32),
+            np.empty(n, np.int32))
+    load().oracle_gen_trip(seed, row0, n, n_total, *[_p(c) for c in cols])
     return cols
 
 
@@ -262,6 +272,41 @@ def groupby(keys, val, max_groups=1 << 22):
     if n < 0:
         return None
     return keys_out[:n * len(ks)].reshape(n, len(ks)), sums[:n], counts[:n]
+
+
+class ScanPred(C.Structure):
+    _fields_ = [("col", u32), ("lo", i64), ("hi", i64)]
+
+
+class ScanKey(C.Structure):
+    _fields_ = [("col", u32), ("base", i64), ("div", u32), ("card", u32), ("out_lo", C.c_int32)]
+
+
+TYPE_IDS = {np.dtype(np.int8): 1, np.dtype(np.uint8): 2, np.dtype(np.int16): 3, np.dtype(np.uint16): 4,
+            np.dtype(np.int32): 5, np.dtype(np.uint32): 6, np.dtype(np.int64): 7}
+
+
+def scan_groupby(cols, preds, keys, val_col, threads=1, max_groups=1 << 24):
+    """cols: list of numpy columns; preds: [(col, lo, hi)]; keys: [(col, base, div, card, out_lo)];
+    val_col: index or -1. Returns (keys[n, k], sums, counts) or None if a key is out of range."""
+    cs = [np.ascontiguousarray(c) for c in cols]
+    cp = (vp * len(cs))(*[c.ctypes.data for c in cs])
+    ty = np.array([TYPE_IDS[c.dtype] for c in cs], np.uint32)
+    pa = (ScanPred * max(len(preds), 1))(*[ScanPred(*p) for p in preds])
+    ka = (ScanKey * max(len(keys), 1))(*[ScanKey(*k) for k in keys])
+    nk = max(len(keys), 1)
+    dom = 1
+    for k in keys:
+        dom *= k[3]
+    max_groups = min(max_groups, dom)
+    keys_out = np.zeros(max_groups * nk, np.int32)
+    sums = np.zeros(max_groups, np.int64)
+    counts = np.zeros(max_groups, np.uint64)
+    n = load().oracle_scan_groupby(cs[0].size, C.cast(cp, vp), _p(ty), C.cast(pa, vp), len(preds), C.cast(ka, vp),
+                                   len(keys), val_col, threads, max_groups, _p(keys_out), _p(sums), _p(counts))
+    if n < 0:
+        return None
+    return keys_out[:n * len(keys)].reshape(n, len(keys)), sums[:n], counts[:n]
 
 
 def minmax_check(mins, maxs, lo, hi):

@@ -440,3 +440,119 @@ void oracle_minmax_build_i32(const int32_t *col, uint64_t n, uint64_t chunk_rows
         maxs[c] = mx;
     }
 }
+
+/* ------------------------------------------------------------------ scan + group-by */
+static int64_t typed_at(const void *col, uint32_t type, uint64_t i) {
+    switch (type) {
+    case 1: return ((const int8_t *)col)[i];
+    case 2: return ((const uint8_t *)col)[i];
+    case 3: return ((const int16_t *)col)[i];
+    case 4: return ((const uint16_t *)col)[i];
+    case 5: return ((const int32_t *)col)[i];
+    case 6: return ((const uint32_t *)col)[i];
+    default: return ((const int64_t *)col)[i];
+    }
+}
+
+typedef struct {
+    uint64_t n, r0, r1, domain;
+    const void *const *cols;
+    const uint32_t *types;
+    const oracle_scan_pred *preds;
+    uint32_t n_preds;
+    const oracle_scan_key *keys;
+    uint32_t n_keys;
+    int32_t val_col;
+    uint64_t *sums, *counts; /* thread-local dense arrays, or shared ones updated atomically */
+    int shared;
+    int bad;
+} scan_job;
+
+static void *scan_worker(void *arg) {
+    scan_job *j = (scan_job *)arg;
+    for (uint64_t i = j->r0; i < j->r1; ++i) {
+        int ok = 1;
+        for (uint32_t p = 0; p < j->n_preds && ok; ++p) {
+            int64_t v = typed_at(j->cols[j->preds[p].col], j->types[j->preds[p].col], i);
+            ok = v >= j->preds[p].lo && v <= j->preds[p].hi;
+        }
+        if (!ok) continue;
+        uint64_t g = 0;
+        for (uint32_t k = 0; k < j->n_keys; ++k) {
+            const oracle_scan_key *K = &j->keys[k];
+            __int128 d = (__int128)typed_at(j->cols[K->col], j->types[K->col], i) - K->base;
+            if (d < 0 || d / K->div >= K->card) {
+                j->bad = 1;
+                return NULL;
+            }
+            g = g * K->card + (uint64_t)(d / K->div);
+        }
+        uint64_t v = j->val_col >= 0 ? (uint64_t)typed_at(j->cols[j->val_col], j->types[j->val_col], i) : 0;
+        if (j->shared) {
+            __atomic_fetch_add(&j->sums[g], v, __ATOMIC_RELAXED);
+            __atomic_fetch_add(&j->counts[g], 1, __ATOMIC_RELAXED);
+        } else {
+            j->sums[g] += v;
+            j->counts[g] += 1;
+        }
+    }
+    return NULL;
+}
+
+int64_t oracle_scan_groupby(uint64_t n, const void *const *cols, const uint32_t *types,
+                            const oracle_scan_pred *preds, uint32_t n_preds, const oracle_scan_key *keys,
+                            uint32_t n_keys, int32_t val_col, uint32_t threads, uint64_t max_groups,
+                            int32_t *keys_out, int64_t *sums_out, uint64_t *counts_out) {
+    uint64_t domain = 1;
+    for (uint32_t k = 0; k < n_keys; ++k) {
+        if (!keys[k].div || !keys[k].card) return -1;
+        domain *= keys[k].card;
+    }
+    if (threads < 1) threads = 1;
+    if (threads > 64) threads = 64;
+    const int shared = domain * 16 * threads > (512ull << 20);
+    const uint32_t copies = shared ? 1 : threads;
+    uint64_t *sums = calloc(domain * copies, 8), *counts = calloc(domain * copies, 8);
+    scan_job jobs[64];
+    pthread_t th[64];
+    for (uint32_t t = 0; t < threads; ++t) {
+        scan_job j = {n, n * t / threads, n * (t + 1) / threads, domain, cols, types, preds, n_preds, keys,
+                      n_keys, val_col, sums + (shared ? 0 : domain * t), counts + (shared ? 0 : domain * t),
+                      shared, 0};
+        jobs[t] = j;
+        pthread_create(&th[t], NULL, scan_worker, &jobs[t]);
+    }
+    int bad = 0;
+    for (uint32_t t = 0; t < threads; ++t) {
+        pthread_join(th[t], NULL);
+        bad |= jobs[t].bad;
+    }
+    int64_t out = 0;
+    if (bad) {
+        out = -1;
+    } else {
+        for (uint64_t g = 0; g < domain; ++g) {
+            uint64_t s = 0, c = 0;
+            for (uint32_t t = 0; t < copies; ++t) {
+                s += sums[domain * t + g];
+                c += counts[domain * t + g];
+            }
+            if (!c) continue;
+            if ((uint64_t)out >= max_groups) {
+                out = -2;
+                break;
+            }
+            uint64_t rest = g;
+            for (int k = (int)n_keys - 1; k >= 0; --k) {
+                keys_out[(uint64_t)out * n_keys + k] = keys[k].out_lo + (int32_t)(rest % keys[k].card);
+                rest /= keys[k].card;
+            }
+            sums_out[out] = (int64_t)s;
+            counts_out[out] = c;
+            ++out;
+        }
+    }
+    free(sums);
+    free(counts);
+    return out;
+}

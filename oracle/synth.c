@@ -6,6 +6,8 @@
  * same multiply-shift range reduction as util::uniform_int (core/util.hpp:177-181) applied to a
  * counter hash instead of an LCG stream.
  */
+#include <math.h>
+
 #include "oracle.h"
 
 /* SplitMix64 finaliser (Steele, Lea, Flood, "Fast splittable pseudorandom number generators",
@@ -75,6 +77,50 @@ void oracle_gen_star_fact(uint64_t seed, uint64_t row0, uint64_t n, uint64_t n_t
         if (supp_key) supp_key[i] = (int32_t)oracle_uniform(seed, 11, r, 0, n_supp - 1);
         if (part_key) part_key[i] = (int32_t)oracle_uniform(seed, 12, r, 0, n_part - 1);
         if (revenue) revenue[i] = (int32_t)oracle_uniform(seed, 13, r, 100, 5000000);
+    }
+}
+
+/* Trip-table columns: the same definitions as the product generator, restated. */
+static uint64_t bits53(uint64_t seed, uint32_t col, uint64_t row) {
+    return oracle_splitmix64((seed * 0x9E3779B97F4A7C15ull) ^ ((uint64_t)col << 48) ^ row) >> 11;
+}
+
+static int32_t lognormal(uint64_t seed, uint32_t col, uint64_t row, double mu, double sigma, double lo,
+                         double hi) {
+    double u1 = (double)(bits53(seed, col, row) + 1) * 0x1p-53;
+    double u2 = (double)bits53(seed, col + 1, row) * 0x1p-53;
+    double z = sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+    double x = floor(exp(mu + sigma * z) + 0.5);
+    if (x < lo) x = lo;
+    if (x > hi) x = hi;
+    return (int32_t)(int64_t)x;
+}
+
+void oracle_gen_trip(uint64_t seed, uint64_t row0, uint64_t n, uint64_t n_total, int64_t *pickup_ts,
+                     uint8_t *passenger, uint16_t *zone, int32_t *fare, int32_t *distance) {
+    double w[6], tot = 0, cum = 0;
+    uint64_t thr[5];
+    for (int k = 0; k < 6; ++k) {
+        w[k] = pow((double)(k + 1), -1.2);
+        tot += w[k];
+    }
+    for (int k = 0; k < 5; ++k) {
+        cum += w[k];
+        thr[k] = (uint64_t)floor(cum / tot * 0x1p53);
+    }
+    for (uint64_t i = 0; i < n; ++i) {
+        uint64_t r = row0 + i;
+        if (pickup_ts)
+            pickup_ts[i] = 1704067200LL + (int64_t)((unsigned __int128)r * 31622400u / n_total);
+        if (passenger) {
+            uint64_t u = bits53(seed, 30, r);
+            uint8_t p = 1;
+            for (int k = 0; k < 5; ++k) p += u >= thr[k];
+            passenger[i] = p;
+        }
+        if (zone) zone[i] = (uint16_t)oracle_uniform(seed, 31, r, 1, 265);
+        if (fare) fare[i] = lognormal(seed, 32, r, 7.2, 0.6, 250, 50000);
+        if (distance) distance[i] = lognormal(seed, 34, r, 8.0, 0.8, 1, 2147483647.0);
     }
 }
 

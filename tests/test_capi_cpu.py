@@ -58,3 +58,20 @@ def test_star_generators_match_oracle(orc, golap):
         assert np.array_equal(x, y)
     assert np.all(np.diff(a[0]) >= 0)  # date_key sorted
     assert np.array_equal(golap.gen_uniform_i32(9, 21, 50_000, -3, 24), orc.gen_uniform_i32(9, 21, 50_000, -3, 24))
+
+
+def test_trip_generators_match_oracle(orc, golap):
+    import numpy as np
+    a = golap.gen_trip(11, 200_000, 1_500_000, row0=700_001, threads=3)
+    b = orc.gen_trip(11, 200_000, 1_500_000, row0=700_001)
+    for x, y in zip(a, b):
+        assert x.dtype == y.dtype and np.array_equal(x, y)
+    ts, pas, zone, fare, dist = a
+    assert np.all(np.diff(ts) >= 0) and ts[0] >= golap.TRIP_T0 and ts[-1] < golap.TRIP_T0 + golap.TRIP_SPAN
+    assert pas.min() >= 1 and pas.max() <= 6 and zone.min() >= 1 and zone.max() <= 265
+    assert fare.min() >= 250 and fare.max() <= 50000 and dist.min() >= 1
+    # Zipf(1.2) over 1..6: P(1) = 1 / sum k^-1.2 = 0.4645; lognormal medians exp(mu)
+    assert abs(np.mean(pas == 1) - 1 / sum(k ** -1.2 for k in range(1, 7))) < 0.005
+    assert abs(np.median(fare) / np.exp(7.2) - 1) < 0.02 and abs(np.median(dist) / np.exp(8.0) - 1) < 0.02
+    # log of a lognormal is normal with the stated sigma (fare's clip trims only the far tails)
+    assert abs(np.std(np.log(dist)) - 0.8) < 0.01

@@ -39,7 +39,7 @@ def check_roundtrip(orc, golap, values, **kw):
 
 @pytest.mark.parametrize("dtype", [np.uint8, np.int16, np.int32, np.uint32, np.int64])
 @pytest.mark.parametrize("kind", ["uniform", "lowcard", "sorted", "runs", "const"])
-@pytest.mark.parametrize("codec", [1, 2, 3, 255])
+@pytest.mark.parametrize("codec", [1, 2, 3, 4, 255])
 def test_encode_decode_roundtrip(orc, golap, dtype, kind, codec):
     n = 3 * 4096 + 777  # ragged last page and chunk
     values = gen(kind, n, dtype)
@@ -113,12 +113,47 @@ def test_closed_forms(golap):
     v = np.cumsum(rng(1).integers(0, 2, 4096)).astype(np.int32)
     a = golap.HostColumn.encode(v, codec=2, page_rows=4096, chunk_rows=4096).arrays()
     assert a["page_csize"][0] == 32 + 4 * ((4095 + 31) // 32)
-    # AUTO chooses the smallest of the three
+    # AUTO chooses the smallest of the four
     sizes = {}
-    for c in (1, 2, 3):
+    for c in (1, 2, 3, 4):
         sizes[c] = int(golap.HostColumn.encode(v, codec=c, page_rows=4096, chunk_rows=4096).arrays()["page_csize"][0])
     auto = golap.HostColumn.encode(v, codec=255, page_rows=4096, chunk_rows=4096).arrays()
     assert int(auto["page_csize"][0]) == min(sizes.values())
+
+
+def test_for_closed_forms(golap):
+    # range exactly 2^9 - 1 -> 9-bit fields: header + ceil(4096*9/8) + 8 padding bytes
+    v = (100 + rng(2).integers(0, 512, 4096)).astype(np.int32)
+    v[0], v[1] = 100, 611
+    a = golap.HostColumn.encode(v, codec=4, page_rows=4096, chunk_rows=4096).arrays()
+    off = int(a["page_off"][0])
+    assert a["page_codec"][0] & 0x0f == 4 and a["page_codec"][0] & 0x40  # point-readable
+    assert a["page_csize"][0] == 32 + 4096 * 9 // 8 + 8
+    assert a["comp"][off + 20] == 9 and int(a["comp"][off:off + 8].view(np.int64)[0]) == 100
+    # constant -> 0-bit fields
+    a = golap.HostColumn.encode(np.full(4096, -5, np.int16), codec=4, page_rows=4096, chunk_rows=4096).arrays()
+    assert a["page_csize"][0] == 32 + 8
+    # a range above 2^56 does not fit: the page falls back to LZ4
+    w = np.array([0, 1 << 60] * 2048, np.int64)
+    a = golap.HostColumn.encode(w, codec=4, page_rows=4096, chunk_rows=4096).arrays()
+    assert a["page_codec"][0] & 0x0f == 1
+
+
+def test_for_known_answer(orc, golap):
+    """Fields [5, 0, 7, 1] of 3 bits over base 10, LSB first: bytes 0xC5 0x03 (hand-computed)."""
+    page = np.zeros(32 + 2 + 8, np.uint8)
+    page[0:8] = np.array([10], np.int64).view(np.uint8)
+    page[16:20] = np.array([4], np.uint32).view(np.uint8)
+    page[20], page[21] = 3, 4
+    page[32], page[33] = 0xC5, 0x03
+    r, out = orc.page_decode(4, page, 16)
+    assert r == 4 and list(out[:16].view(np.int32)) == [15, 10, 17, 11]
+    a = golap.HostColumn.encode(np.array([15, 10, 17, 11], np.int32), codec=4, page_rows=64,
+                                chunk_rows=64).arrays()
+    off = int(a["page_off"][0])
+    assert bytes(a["comp"][off:off + a["page_csize"][0]]) == page.tobytes()
+    # a truncated page is rejected
+    assert orc.page_decode(4, page[:-1], 16)[0] == -1
 
 
 def test_assemble_from_liblz4_pages(orc, golap, lz4lib):

@@ -1,0 +1,122 @@
+"""Fused scan + filter + group-by over the trip table (range predicates of 01.cuh:26-54,
+grouping of group_by_agg_2col/4col + HashAggregate::add, core/group.cuh:14-36 and
+core/dev_structs.cuh:209-244, over computed keys: zone x hour of the pickup timestamp) against
+the oracle's restatement."""
+import numpy as np
+import pytest
+
+from gpu_util import say
+
+pytestmark = pytest.mark.gpu
+
+T0 = 1704067200
+
+
+def trip_cols(golap, ctx, n, seed, codec=255, page_rows=4096, chunk_rows=65536):
+    host = golap.gen_trip(seed, n)
+    dev = [ctx.upload(golap.HostColumn.encode(c, codec=codec, page_rows=page_rows, chunk_rows=chunk_rows))
+           for c in host]
+    return host, dev
+
+
+def template(t, h0, h1, full_hours=False):
+    """(preds, keys, value_col) of the four request templates over hours [h0, h1] of the year.
+    Columns: 0 pickup_ts, 1 passenger, 2 zone, 3 fare, 4 distance."""
+    lo, hi = T0 + h0 * 3600, T0 + (h1 + 1) * 3600 - 1
+    hb, hc = (0, 8784) if full_hours else (h0, h1 - h0 + 1)
+    zone = (2, 1, 1, 265, 1)
+    hour = (0, T0 + hb * 3600, 3600, hc, hb)
+    if t == 0:
+        return [(0, lo, hi)], [zone, hour], 3
+    if t == 1:
+        return [(0, lo, hi), (1, 2, 6)], [zone, hour], 4
+    if t == 2:
+        return [(0, lo, hi), (3, 1000, 5000)], [hour, zone], 4
+    return [(0, lo, hi), (4, 5000, 1 << 30)], [(1, 1, 1, 6, 1), zone], -1
+
+
+def check(orc, ctx, host, dev, preds, keys, vc, **kw):
+    r, k, s, c = ctx.scan_run(dev, preds, keys, vc, **kw)
+    cb, ce = kw.get("chunk_begin", 0), kw.get("chunk_end")
+    rows = slice(cb * 65536, None if ce is None else ce * 65536)
+    w = orc.scan_groupby([h[rows] for h in host], preds, keys, vc, threads=4)
+    assert w is not None
+    wk, ws, wc = w
+    assert np.array_equal(k, wk) and np.array_equal(s, ws) and np.array_equal(c, wc)
+    assert r.rows_matched == int(wc.sum()) and r.total_sum == int(ws.astype(np.uint64).sum().astype(np.int64))
+    return r
+
+
+@pytest.fixture(scope="module")
+def trip(golap, ctx):
+    host, dev = trip_cols(golap, ctx, 3_000_000, seed=5)
+    yield host, dev
+    for d in dev:
+        ctx.free_column(d)
+
+
+@pytest.mark.parametrize("t", range(4))
+@pytest.mark.parametrize("agg", [0, 2])
+def test_scan_templates_match_oracle(orc, golap, ctx, trip, t, agg):
+    host, dev = trip
+    # 3M rows span the year: hours 1000..1700 hold about 240k rows
+    preds, keys, vc = template(t, 1000, 1700)
+    say("scan template", t, "agg", agg)
+    r = check(orc, ctx, host, dev, preds, keys, vc, agg_mode=agg)
+    assert r.chunks_pruned > r.chunks_run  # the sorted timestamp prunes most zones
+
+
+def test_scan_full_year_groups_spill(orc, golap, ctx, trip):
+    """Zone x hour over the whole year: 2,327,760 groups, LDS tables flushed and spilled."""
+    host, dev = trip
+    preds, keys, vc = template(0, 0, 8783, full_hours=True)
+    r = check(orc, ctx, host, dev, preds, keys, vc)
+    assert r.lds_group_slots > 0 and r.n_groups > 1_000_000
+
+
+@pytest.mark.parametrize("slots", [64, 512, 4096])
+def test_scan_small_lds_tables_agree(orc, golap, ctx, trip, slots):
+    host, dev = trip
+    preds, keys, vc = template(1, 200, 5000)
+    check(orc, ctx, host, dev, preds, keys, vc, agg_mode=1, lds_group_slots=slots)
+
+
+def test_scan_slice_mask_and_no_prune_agree(orc, golap, ctx, trip):
+    host, dev = trip
+    preds, keys, vc = template(2, 3000, 3300)
+    a = check(orc, ctx, host, dev, preds, keys, vc, chunk_begin=3, chunk_end=40)
+    b = check(orc, ctx, host, dev, preds, keys, vc, chunk_begin=3, chunk_end=40, zone_prune=0)
+    assert a.chunks_run < b.chunks_run and b.chunks_pruned == 0
+    nch = ctx.column_desc(dev[0]).n_chunks
+    flags = ctx.alloc(nch * 2)
+    try:
+        ctx.minmax_check(dev[0], preds[0][1], preds[0][2], flags)
+        c = check(orc, ctx, host, dev, preds, keys, vc, chunk_begin=3, chunk_end=40, zone_prune=0,
+                  dev_chunk_mask=flags)
+        assert c.chunks_run == a.chunks_run
+    finally:
+        ctx.free(flags)
+
+
+@pytest.mark.parametrize("codec", [0, 1, 2, 4])
+def test_scan_codecs(orc, golap, ctx, codec):
+    host, dev = trip_cols(golap, ctx, 400_000, seed=9, codec=codec, page_rows=2048, chunk_rows=16384)
+    try:
+        for t in range(4):
+            preds, keys, vc = template(t, 100, 8000)
+            r, k, s, c = ctx.scan_run(dev, preds, keys, vc)
+            wk, ws, wc = orc.scan_groupby(list(host), preds, keys, vc, threads=4)
+            assert np.array_equal(k, wk) and np.array_equal(s, ws) and np.array_equal(c, wc), (codec, t)
+    finally:
+        for d in dev:
+            ctx.free_column(d)
+
+
+def test_scan_key_out_of_range_is_an_error(golap, ctx, trip):
+    host, dev = trip
+    preds, keys, vc = template(0, 1000, 1700)
+    keys = [keys[0], (0, T0 + 1000 * 3600, 3600, 100, 1000)]  # hours 1100..1700 fall outside
+    with pytest.raises(golap.GolapError):
+        ctx.scan_run(dev, preds, keys, vc)
+    r = ctx.scan_run(dev, *template(0, 1000, 1700))  # the context still works
+    assert r[0].n_groups > 0
