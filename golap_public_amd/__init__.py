@@ -6,6 +6,7 @@ entry point; if the library is missing, load() raises.
 """
 import ctypes as C
 import os
+import threading
 
 import numpy as np
 
@@ -88,8 +89,21 @@ class ScanPlan(C.Structure):
                 ("lds_group_slots", u32)]
 
 
-class BatchQuery(C.Structure):
+class StarQuery(C.Structure):
     _fields_ = [("plan", StarPlan), ("zone_col", vp), ("zone_lo", i64), ("zone_hi", i64)]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("hbm_budget", u64), ("max_batch", u32), ("slots", u32), ("emit_threads", u32),
+                ("batch_window_us", u32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", u64), ("completed", u64), ("refused", u64), ("batches", u64),
+                ("peak_admitted_bytes", u64), ("mean_batch", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 class BatchConfig(C.Structure):
@@ -149,14 +163,20 @@ _PROTOS = {
                                  u64]),
     "golap_star_run": (C.c_int, [vp, C.POINTER(StarPlan), C.POINTER(StarResult), vp, vp, vp, u64]),
     "golap_scan_run": (C.c_int, [vp, C.POINTER(ScanPlan), C.POINTER(StarResult), vp, vp, vp, u64]),
-    "golap_star_batch_create": (C.c_int, [vp, C.POINTER(BatchQuery), u32, C.POINTER(vp)]),
-    "golap_star_batch_config_get": (C.c_int, [vp, C.POINTER(BatchConfig)]),
-    "golap_star_batch_config_set": (C.c_int, [vp, C.POINTER(BatchConfig)]),
-    "golap_star_batch_run": (C.c_int, [vp, u32, C.POINTER(BatchRunInfo)]),
-    "golap_star_batch_tune": (C.c_int, [vp, u32, C.POINTER(BatchConfig), C.POINTER(C.c_double)]),
-    "golap_star_batch_result": (C.c_int, [vp, u32, C.POINTER(StarResult), vp, vp, vp, u64]),
-    "golap_star_batch_free": (C.c_int, [vp]),
-    "golap_gen_star_fact": (C.c_int, [u64, u64, u64, u64, vp, u32, u32, u32, u32, vp, vp, vp, vp, vp, u32]),
+    "golap_batch_create_star": (C.c_int, [vp, C.POINTER(StarQuery), u32, C.POINTER(vp)]),
+    "golap_batch_create_scan": (C.c_int, [vp, C.POINTER(ScanPlan), u32, C.POINTER(vp)]),
+    "golap_batch_config_get": (C.c_int, [vp, C.POINTER(BatchConfig)]),
+    "golap_batch_config_set": (C.c_int, [vp, C.POINTER(BatchConfig)]),
+    "golap_batch_run": (C.c_int, [vp, u32, C.POINTER(BatchRunInfo)]),
+    "golap_batch_tune": (C.c_int, [vp, u32, C.POINTER(BatchConfig), C.POINTER(C.c_double)]),
+    "golap_batch_result": (C.c_int, [vp, u32, C.POINTER(StarResult), vp, vp, vp, u64]),
+    "golap_batch_free": (C.c_int, [vp]),
+    "golap_server_create": (C.c_int, [vp, C.POINTER(ServerOpts), C.POINTER(vp)]),
+    "golap_server_submit_star": (C.c_int, [vp, C.POINTER(StarQuery), C.POINTER(u64)]),
+    "golap_server_submit_scan": (C.c_int, [vp, C.POINTER(ScanPlan), C.POINTER(u64)]),
+    "golap_server_wait": (C.c_int, [vp, u64, C.POINTER(StarResult), vp, vp, vp, u64, C.POINTER(C.c_double)]),
+    "golap_server_get_stats": (C.c_int, [vp, C.POINTER(ServerStats)]),
+    "golap_server_destroy": (C.c_int, [vp]),
     "golap_gen_trip": (C.c_int, [u64, u64, u64, u64, vp, vp, vp, vp, vp, u32]),
     "golap_gen_uniform_i32": (C.c_int, [u64, u32, u64, C.c_int32, C.c_int32, vp, u32]),
     "golap_gen_date_dim": (C.c_int, [u32, vp, vp, vp]),
@@ -489,7 +509,15 @@ class Context:
 
     def star_batch(self, queries):
         """queries: list of dicts with star_plan() arguments plus optional zone=(column, lo, hi)."""
-        return StarBatch(self, queries)
+        return Batch(self, "star", queries)
+
+    def server(self, **opts):
+        """Serving loop over this context (golap_server_*); opts: ServerOpts fields."""
+        return Server(self, **opts)
+
+    def scan_batch(self, plans):
+        """plans: list of dicts with scan_plan() arguments."""
+        return Batch(self, "scan", plans)
 
     # plans
     def q1_run(self, date, disc, qty, price, date_map, disc_rng, qty_rng, price_rng=(-2**31, 2**31 - 1),
@@ -503,45 +531,58 @@ class Context:
         return r
 
 
-class StarBatch:
-    """Many star plans over one fact table in one persistent launch (golap_star_batch_*)."""
+class Batch:
+    """Many star or scan plans over one fact table in one persistent launch (golap_batch_*)."""
 
-    def __init__(self, ctx, queries):
-        self.ctx, self.h = ctx, None
-        arr = (BatchQuery * len(queries))()
+    def __init__(self, ctx, kind, queries):
+        self.ctx, self.h, self.kind = ctx, None, kind
         self.n_attrs, self.domains = [], []
-        for i, q in enumerate(queries):
-            q = dict(q)
-            zone = q.pop("zone", None)
-            arr[i].plan = ctx.star_plan(**q)
-            if zone is not None:
-                arr[i].zone_col, arr[i].zone_lo, arr[i].zone_hi = zone
-            self.n_attrs.append(sum(1 for d in q["dims"] if d[3]))
-            self.domains.append(_domain(q["dims"]))
         h = vp()
-        ctx.check(ctx.lib.golap_star_batch_create(ctx.h, arr, len(queries), C.byref(h)), "star_batch_create")
+        if kind == "star":
+            arr = (StarQuery * len(queries))()
+            for i, q in enumerate(queries):
+                q = dict(q)
+                zone = q.pop("zone", None)
+                arr[i].plan = ctx.star_plan(**q)
+                if zone is not None:
+                    arr[i].zone_col, arr[i].zone_lo, arr[i].zone_hi = zone
+                self.n_attrs.append(sum(1 for d in q["dims"] if d[3]))
+                self.domains.append(_domain(q["dims"]))
+            st = ctx.lib.golap_batch_create_star(ctx.h, arr, len(queries), C.byref(h))
+        else:
+            arr = (ScanPlan * len(queries))()
+            for i, q in enumerate(queries):
+                arr[i] = ctx.scan_plan(**q)
+                keys = q.get("keys", ())
+                self.n_attrs.append(len(keys))
+                d = 1
+                for k in keys:
+                    d *= int(k[3])
+                self.domains.append(d)
+            st = ctx.lib.golap_batch_create_scan(ctx.h, arr, len(queries), C.byref(h))
+        ctx.check(st, "batch_create")
         self.h = h.value
         self.n = len(queries)
 
     def config(self):
         c = BatchConfig()
-        self.ctx.check(self.ctx.lib.golap_star_batch_config_get(self.h, C.byref(c)), "batch_config_get")
+        self.ctx.check(self.ctx.lib.golap_batch_config_get(self.h, C.byref(c)), "batch_config_get")
         return c
 
     def set_config(self, **kw):
         c = self.config()
         for k, v in kw.items():
             setattr(c, k, v)
-        self.ctx.check(self.ctx.lib.golap_star_batch_config_set(self.h, C.byref(c)), "batch_config_set")
+        self.ctx.check(self.ctx.lib.golap_batch_config_set(self.h, C.byref(c)), "batch_config_set")
 
     def run(self, use_graph=True):
         info = BatchRunInfo()
-        self.ctx.check(self.ctx.lib.golap_star_batch_run(self.h, int(use_graph), C.byref(info)), "batch_run")
+        self.ctx.check(self.ctx.lib.golap_batch_run(self.h, int(use_graph), C.byref(info)), "batch_run")
         return info
 
     def tune(self, reps=3):
         c, ms = BatchConfig(), C.c_double()
-        self.ctx.check(self.ctx.lib.golap_star_batch_tune(self.h, reps, C.byref(c), C.byref(ms)), "batch_tune")
+        self.ctx.check(self.ctx.lib.golap_batch_tune(self.h, reps, C.byref(c), C.byref(ms)), "batch_tune")
         return c, ms.value
 
     def result(self, q, max_groups=None):
@@ -552,14 +593,14 @@ class StarBatch:
         sums = np.zeros(max_groups, np.int64)
         cnts = np.zeros(max_groups, np.uint64)
         r = StarResult()
-        self.ctx.check(self.ctx.lib.golap_star_batch_result(self.h, q, C.byref(r), _ptr(kout), _ptr(sums),
-                                                            _ptr(cnts), max_groups), "batch_result")
+        self.ctx.check(self.ctx.lib.golap_batch_result(self.h, q, C.byref(r), _ptr(kout), _ptr(sums),
+                                                       _ptr(cnts), max_groups), "batch_result")
         n = r.n_groups
         return r, kout[:n * na].reshape(n, na), sums[:n], cnts[:n]
 
     def free(self):
         if self.h:
-            self.ctx.check(self.ctx.lib.golap_star_batch_free(self.h), "batch_free")
+            self.ctx.check(self.ctx.lib.golap_batch_free(self.h), "batch_free")
             self.h = None
 
     def __enter__(self):
@@ -567,6 +608,73 @@ class StarBatch:
 
     def __exit__(self, *a):
         self.free()
+
+
+class Server:
+    """Request loop with HBM-budget admission and continuous batching (golap_server_*)."""
+
+    def __init__(self, ctx, **opts):
+        self.ctx, self.h = ctx, None
+        o = ServerOpts()
+        for k, v in opts.items():
+            setattr(o, k, v)
+        h = vp()
+        ctx.check(ctx.lib.golap_server_create(ctx.h, C.byref(o), C.byref(h)), "server_create")
+        self.h = h.value
+        self.shape = {}
+        self._lock = threading.Lock()
+
+    def submit_scan(self, **plan):
+        p = self.ctx.scan_plan(**plan)
+        t = u64()
+        self.ctx.check(self.ctx.lib.golap_server_submit_scan(self.h, C.byref(p), C.byref(t)), "server_submit")
+        keys = plan.get("keys", ())
+        d = 1
+        for k in keys:
+            d *= int(k[3])
+        with self._lock:
+            self.shape[t.value] = (len(keys), d)
+        return t.value
+
+    def submit_star(self, zone=None, **query):
+        q = StarQuery()
+        q.plan = self.ctx.star_plan(**query)
+        if zone is not None:
+            q.zone_col, q.zone_lo, q.zone_hi = zone
+        t = u64()
+        self.ctx.check(self.ctx.lib.golap_server_submit_star(self.h, C.byref(q), C.byref(t)), "server_submit")
+        with self._lock:
+            self.shape[t.value] = (sum(1 for d in query["dims"] if d[3]), _domain(query["dims"]))
+        return t.value
+
+    def wait(self, ticket):
+        """(result, keys, sums, counts, latency_ms)."""
+        with self._lock:
+            na, dom = self.shape.pop(ticket)
+        kout = np.zeros(dom * max(na, 1), np.int32)
+        sums = np.zeros(dom, np.int64)
+        cnts = np.zeros(dom, np.uint64)
+        r, lat = StarResult(), C.c_double()
+        self.ctx.check(self.ctx.lib.golap_server_wait(self.h, ticket, C.byref(r), _ptr(kout), _ptr(sums), _ptr(cnts),
+                                                      dom, C.byref(lat)), "server_wait")
+        n = r.n_groups
+        return r, kout[:n * na].reshape(n, na), sums[:n], cnts[:n], lat.value
+
+    def stats(self):
+        st = ServerStats()
+        self.ctx.check(self.ctx.lib.golap_server_get_stats(self.h, C.byref(st)), "server_stats")
+        return st.as_dict()
+
+    def close(self):
+        if self.h:
+            self.ctx.check(self.ctx.lib.golap_server_destroy(self.h), "server_destroy")
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 def gen_date_dim(n_days):

@@ -1,0 +1,103 @@
+// Batched plans (internal interface shared by the batch C ABI and the serving loop).
+//
+// A batch holds many queries of one kind (star join + group-by, or scan + group-by) over one
+// fact table and runs them as three kernels on one stream:
+//   1. zero the result arena;
+//   2. build the work list;
+//   3. the persistent fused kernel.
+// A BatchSlot owns the stream, the events and the device and pinned buffers a batch runs in.
+// The public batch API gives each batch its own slot on the context stream. The serving loop
+// keeps a few slots with their own streams and reuses their buffers across batches, so batches
+// on different slots overlap on the device.
+#pragma once
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+#include "star_host.hpp"
+
+namespace golap_impl {
+
+struct BatchSlot {
+    golap_ctx *ctx = nullptr;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, dep = nullptr;
+    uint8_t *d_arena = nullptr, *h_arena = nullptr, *d_q = nullptr;
+    uint64_t *d_chunk_off = nullptr;
+    uint2 *d_items = nullptr;
+    uint64_t arena_cap = 0, q_cap = 0, off_cap = 0, items_cap = 0;
+    std::string err;
+
+    // own_stream: a new stream (serving slots); otherwise the context stream.
+    int init(golap_ctx *c, bool own);
+    // Grows the buffers to at least these sizes (bytes, bytes, entries, entries).
+    int reserve(uint64_t arena, uint64_t qbytes, uint64_t offs, uint64_t items);
+    void release();
+};
+
+enum class BatchKind { STAR, SCAN };
+
+}  // namespace golap_impl
+
+struct golap_batch {
+    golap_ctx *ctx = nullptr;
+    golap_impl::BatchKind kind = golap_impl::BatchKind::STAR;
+    golap_impl::BatchSlot *slot = nullptr;
+    bool own_slot = false;
+    // star queries
+    std::vector<golap_star_query> sq;
+    std::vector<golap_impl::starhost::QueryHost> qh;
+    std::vector<golap_impl::StarQueryDev> qd;
+    uint32_t max_filter_words = 0;
+    // scan queries
+    std::vector<golap_impl::starhost::ScanHost> sh;
+    std::vector<golap_impl::ScanQueryDev> sd;
+    uint32_t win_bytes = 0;
+    // fact columns of the launch and their LDS windows
+    golap_impl::starhost::ColumnSet cs;
+    int windows[golap_impl::STAR_MAX_COLS] = {};
+    uint32_t n_windows = 0;
+    // slices and work list
+    uint32_t page_rows = 0, ppc = 0;
+    uint64_t n_rows = 0, n_pages = 0;
+    std::vector<uint64_t> chunk_off;
+    std::vector<std::vector<uint8_t>> survive;  // host view of surviving chunks per query
+    std::vector<uint64_t> surv_pages;
+    uint64_t total_pages = 0, total_bytes = 0;
+    uint64_t max_domain = 1;
+    std::vector<uint64_t> res_off;
+    uint64_t arena_bytes = 0;
+    golap_batch_config cfg{};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::string shape;
+    bool ran = false, h_valid = false;
+    double last_ms = 0;
+    std::string err;
+
+    uint32_t n_queries() const { return (uint32_t)(kind == golap_impl::BatchKind::STAR ? sq.size() : sh.size()); }
+};
+
+namespace golap_impl {
+
+// Creation validates and prepares every query (call with the context lock held). slot null:
+// the batch gets its own slot on the context stream. capture: record the hipGraph (public
+// batches; one-shot serving batches launch plainly).
+int batch_create_star(golap_ctx *ctx, const golap_star_query *q, uint32_t n, BatchSlot *slot, bool capture,
+                      golap_batch **out);
+int batch_create_scan(golap_ctx *ctx, const golap_scan_plan *q, uint32_t n, BatchSlot *slot, bool capture,
+                      golap_batch **out);
+// Enqueues one run on the slot stream after the context stream's pending work (lock held).
+int batch_enqueue(golap_batch *b, bool use_graph);
+// Waits for the run and checks its error word; no context lock needed.
+int batch_finish(golap_batch *b, golap_batch_run_info *info);
+// Query q's groups of the last finished run; no context lock needed.
+int batch_result(golap_batch *b, uint32_t q, golap_star_result *res, int32_t *keys, int64_t *sums, uint64_t *counts,
+                 uint64_t max_groups);
+// Device bytes a query adds to a batch (results, work items, descriptor): the serving loop's
+// admission unit.
+uint64_t batch_query_bytes(uint64_t domain, uint64_t slice_chunks, uint32_t pages_per_chunk, bool star);
+void batch_free(golap_batch *b);
+
+}  // namespace golap_impl

@@ -1,67 +1,51 @@
-// C ABI: batched star plans (include/golap_hip.h). Many star-join + group-by queries over one
-// fact table run as one persistent fused launch:
+// Batched plans (batch.hpp, include/golap_hip.h). Many star or scan queries over one fact table
+// run as one persistent fused launch:
 //   1. a fill kernel zeroes the result arena (work-list counters, error word, every query's
 //      dense SUM/COUNT arrays);
-//   2. k_batch_items evaluates each query's chunk mask and zone-map predicate per chunk of its
+//   2. k_batch_items evaluates each query's chunk mask and zone-map predicates per chunk of its
 //      slice (check_mmmeta semantics, core/metadata.cuh:539-547) and writes the surviving
 //      (chunk, query, part) items;
-//   3. k_star (queue schedule) lets workgroups take items with one atomic each.
-// The three kernels are captured once into a hipGraph and replayed per run. The launch
-// configuration is chosen by a coordinate-descent autotuner that times candidates on the batch
-// itself and caches the winner in the context by batch shape.
+//   3. k_star or k_scan (queue schedule) lets workgroups take items with one atomic each.
+// Public batches capture the three kernels into a hipGraph once and replay it per run. The
+// launch configuration is chosen by a coordinate-descent autotuner that times candidates on the
+// batch itself and caches the winner in the context by batch shape.
+#include "batch.hpp"
+
 #include <algorithm>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
-
-#include "common.hpp"
-#include "star_host.hpp"
 
 using namespace golap_impl;
 using namespace golap_impl::starhost;
 using Lock = std::lock_guard<std::recursive_mutex>;
 
-struct golap_star_batch {
-    golap_ctx *ctx = nullptr;
-    std::vector<golap_batch_query> q;
-    std::vector<QueryHost> qh;
-    std::vector<StarQueryDev> qd;  // descriptors as last uploaded
-    ColumnSet cs;
-    int windows[STAR_MAX_COLS] = {};
-    uint32_t n_windows = 0;
-    uint32_t page_rows = 0, ppc = 0;
-    uint64_t n_rows = 0, n_pages = 0;
-    std::vector<uint64_t> chunk_off;  // prefix of slice lengths in chunks, n + 1 entries
-    // host view of which chunks survive (chunk masks read at creation, zone maps), per query
-    std::vector<std::vector<uint8_t>> survive;
-    std::vector<uint64_t> surv_pages;
-    uint64_t total_pages = 0, total_bytes = 0;
-    uint32_t max_filter_words = 0;
-    uint64_t max_domain = 1;
-    // device
-    StarQueryDev *d_q = nullptr;
-    uint64_t *d_chunk_off = nullptr;
-    uint2 *d_items = nullptr;
-    uint8_t *d_arena = nullptr;  // [0, 256): queue head, item count, error word; then results
-    uint64_t arena_bytes = 0;
-    uint8_t *h_arena = nullptr;  // pinned host mirror, filled once per run on the first result call
-    bool h_valid = false;
-    std::vector<uint64_t> res_off;  // per query: byte offset of its sums (counts follow)
-    golap_batch_config cfg{};
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    std::string shape;
-    bool ran = false;
-    double last_ms = 0;
-};
+#define BATCH_HIP_CHECK(b, expr)                                                                                 \
+    do {                                                                                                         \
+        hipError_t e__ = (expr);                                                                                 \
+        if (e__ != hipSuccess) return bfail((b), GOLAP_EHIP, std::string(#expr ": ") + hipGetErrorString(e__)); \
+    } while (0)
 
 namespace {
 
-constexpr uint64_t ARENA_HEADER = 256;
+constexpr uint64_t ARENA_HEADER = 256;  // queue head, item count, error word
 
-uint32_t parts_of(const golap_star_batch *b, uint32_t ppi) { return (b->ppc + ppi - 1) / ppi; }
+int bfail(golap_batch *b, int code, const std::string &msg) {
+    b->err = msg;
+    return code;
+}
 
-void drop_graph(golap_star_batch *b) {
+uint32_t parts_of(const golap_batch *b, uint32_t ppi) { return (b->ppc + ppi - 1) / ppi; }
+uint32_t desc_size(const golap_batch *b) {
+    return b->kind == BatchKind::STAR ? (uint32_t)sizeof(StarQueryDev) : (uint32_t)sizeof(ScanQueryDev);
+}
+const SliceDev &slice_of(const golap_batch *b, uint32_t i) { return b->kind == BatchKind::STAR ? b->qd[i].s : b->sd[i].s; }
+uint64_t domain_of(const golap_batch *b, uint32_t i) {
+    return b->kind == BatchKind::STAR ? b->qh[i].domain : b->sh[i].domain;
+}
+
+void drop_graph(golap_batch *b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     b->exec = nullptr;
@@ -69,305 +53,474 @@ void drop_graph(golap_star_batch *b) {
 }
 
 // Batches of the same shape share tuned configurations.
-std::string shape_of(const golap_star_batch *b) {
+std::string shape_of(const golap_batch *b) {
     auto bucket = [](uint64_t v) { return bits_for(std::max<uint64_t>(v, 1)); };
-    return std::to_string(b->page_rows) + "/" + std::to_string(b->ppc) + "/" + std::to_string(b->n_windows) + "/" +
-           std::to_string(b->cs.cols.size()) + "/" + std::to_string(bucket(b->q.size())) + "/" +
+    return std::string(b->kind == BatchKind::STAR ? "star/" : "scan/") + std::to_string(b->page_rows) + "/" +
+           std::to_string(b->ppc) + "/" + std::to_string(b->n_windows) + "/" + std::to_string(b->win_bytes) + "/" +
+           std::to_string(b->cs.cols.size()) + "/" + std::to_string(bucket(b->n_queries())) + "/" +
            std::to_string(bucket(b->total_pages)) + "/" + std::to_string(bucket(b->max_domain));
 }
 
-uint32_t group_bits_of(const golap_star_batch *b) { return std::max<uint32_t>(1, bits_for(b->max_domain)); }
+uint32_t group_bits_of(const golap_batch *b) { return std::max<uint32_t>(1, bits_for(b->max_domain)); }
 
 // LDS group keys are (query << group_bits | group) and must stay below the empty marker.
-bool lds_keys_fit(const golap_star_batch *b) { return bits_for(b->q.size()) + group_bits_of(b) <= 31; }
+bool lds_keys_fit(const golap_batch *b) { return bits_for(b->n_queries()) + group_bits_of(b) <= 31; }
 
-int check_config(const golap_star_batch *b, const golap_batch_config &c) {
-    golap_ctx *ctx = b->ctx;
+size_t lds_bytes(const golap_batch *b, const golap_batch_config &c) {
+    if (b->kind == BatchKind::STAR)
+        return star_lds_bytes(c.filter_lds ? b->max_filter_words : 0, b->page_rows, b->n_windows, c.lds_group_slots);
+    return scan_lds_bytes(b->win_bytes, c.lds_group_slots);
+}
+
+int check_config(golap_batch *b, const golap_batch_config &c) {
     if (c.pages_per_item < 1 || c.pages_per_item > b->ppc || parts_of(b, c.pages_per_item) > 256)
-        return fail(ctx, GOLAP_EINVAL, "batch: pages_per_item out of range");
+        return bfail(b, GOLAP_EINVAL, "batch: pages_per_item out of range");
     if (c.lds_group_slots && (c.lds_group_slots & (c.lds_group_slots - 1) || c.lds_group_slots > LDS_GROUP_SLOTS_MAX))
-        return fail(ctx, GOLAP_EINVAL, "batch: lds_group_slots must be 0 or a power of two <= 4096");
+        return bfail(b, GOLAP_EINVAL, "batch: lds_group_slots must be 0 or a power of two <= 4096");
     if (c.lds_group_slots && !lds_keys_fit(b))
-        return fail(ctx, GOLAP_EINVAL, "batch: too many queries x groups for LDS group keys");
-    if (c.filter_lds > 1 || c.compact_tables > 1) return fail(ctx, GOLAP_EINVAL, "batch: bad flag");
-    const uint32_t fw = c.filter_lds ? b->max_filter_words : 0;
-    if (star_lds_bytes(fw, b->page_rows, b->n_windows, c.lds_group_slots) > (size_t)ctx->lds_max)
-        return fail(ctx, GOLAP_EINVAL, "batch: LDS budget exceeded");
+        return bfail(b, GOLAP_EINVAL, "batch: too many queries x groups for LDS group keys");
+    if (c.filter_lds > 1 || c.compact_tables > 1) return bfail(b, GOLAP_EINVAL, "batch: bad flag");
+    if (lds_bytes(b, c) > (size_t)b->ctx->lds_max) return bfail(b, GOLAP_EINVAL, "batch: LDS budget exceeded");
     return GOLAP_OK;
 }
 
-// Descriptors for a configuration: probe tables (compact or full) and the per-query choice of
-// LDS or global aggregation; uploaded to the device.
-int upload_descriptors(golap_star_batch *b, const golap_batch_config &c) {
-    golap_ctx *ctx = b->ctx;
-    const uint32_t nq = (uint32_t)b->q.size();
+// Descriptors for a configuration (star: compact or full probe tables; every query: LDS or
+// global aggregation and its result arrays), uploaded on the slot stream.
+int upload_descriptors(golap_batch *b, const golap_batch_config &c) {
+    const uint32_t nq = b->n_queries();
     for (uint32_t i = 0; i < nq; ++i) {
-        StarQueryDev &d = b->qd[i];
-        const golap_star_plan &p = b->q[i].plan;
-        for (uint32_t k = 0; k < p.n_dims; ++k) {
-            const golap_hashmap *m = p.dims[k].map;
-            const bool small = c.compact_tables && m->small_log2;
-            d.dim[k].slots = small ? m->d_small : m->d_slots;
-            d.dim[k].log2cap = small ? m->small_log2 : m->log2cap;
+        const uint64_t dom = domain_of(b, i);
+        int64_t *gs = reinterpret_cast<int64_t *>(b->slot->d_arena + b->res_off[i]);
+        uint64_t *gc = reinterpret_cast<uint64_t *>(b->slot->d_arena + b->res_off[i] + dom * 8);
+        if (b->kind == BatchKind::STAR) {
+            StarQueryDev &d = b->qd[i];
+            const golap_star_plan &p = b->sq[i].plan;
+            for (uint32_t k = 0; k < p.n_dims; ++k) {
+                const golap_hashmap *m = p.dims[k].map;
+                const bool small = c.compact_tables && m->small_log2;
+                d.dim[k].slots = small ? m->d_small : m->d_slots;
+                d.dim[k].log2cap = small ? m->small_log2 : m->log2cap;
+            }
+            const double expect = (double)b->surv_pages[i] * b->page_rows * b->qh[i].sel;
+            const bool lds = p.agg_mode == GOLAP_AGG_LDS ||
+                             (p.agg_mode == GOLAP_AGG_AUTO && expect >= (double)c.lds_cutover * (double)dom);
+            d.use_lds = lds && c.lds_group_slots;
+            d.g_sum = gs;
+            d.g_cnt = gc;
+        } else {
+            ScanQueryDev &d = b->sd[i];
+            const uint32_t am = b->sh[i].plan.agg_mode;
+            const double expect = (double)b->surv_pages[i] * b->page_rows;
+            const bool lds = am == GOLAP_AGG_LDS ||
+                             (am == GOLAP_AGG_AUTO && expect >= (double)c.lds_cutover * (double)dom);
+            d.use_lds = lds && c.lds_group_slots;
+            d.g_sum = gs;
+            d.g_cnt = gc;
         }
-        const double expect = (double)b->surv_pages[i] * b->page_rows * b->qh[i].sel;
-        bool lds = p.agg_mode == GOLAP_AGG_LDS ||
-                   (p.agg_mode == GOLAP_AGG_AUTO && expect >= (double)c.lds_cutover * (double)b->qh[i].domain);
-        d.use_lds = lds && c.lds_group_slots;
-        d.g_sum = reinterpret_cast<int64_t *>(b->d_arena + b->res_off[i]);
-        d.g_cnt = reinterpret_cast<uint64_t *>(b->d_arena + b->res_off[i] + b->qh[i].domain * 8);
     }
-    GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(b->d_q, b->qd.data(), nq * sizeof(StarQueryDev), hipMemcpyHostToDevice,
-                                        ctx->stream));
-    GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    const void *src = b->kind == BatchKind::STAR ? (const void *)b->qd.data() : (const void *)b->sd.data();
+    BATCH_HIP_CHECK(b, hipMemcpyAsync(b->slot->d_q, src, (size_t)nq * desc_size(b), hipMemcpyHostToDevice,
+                                      b->slot->stream));
+    BATCH_HIP_CHECK(b, hipStreamSynchronize(b->slot->stream));
     return GOLAP_OK;
 }
 
-StarLaunch launch_of(const golap_star_batch *b, const golap_batch_config &c) {
-    StarLaunch L{};
-    L.n_cols = (uint32_t)b->cs.cols.size();
-    for (uint32_t i = 0; i < L.n_cols; ++i) {
-        L.col[i] = view(b->cs.cols[i]);
-        L.col_window[i] = b->windows[i];
+hipError_t launch_kernel(golap_batch *b, hipStream_t s, bool prepare_only) {
+    const golap_batch_config &c = b->cfg;
+    BatchSlot *sl = b->slot;
+    if (b->kind == BatchKind::STAR) {
+        StarLaunch L{};
+        L.n_cols = (uint32_t)b->cs.cols.size();
+        for (uint32_t i = 0; i < L.n_cols; ++i) {
+            L.col[i] = view(b->cs.cols[i]);
+            L.col_window[i] = b->windows[i];
+        }
+        L.n_windows = b->n_windows;
+        L.page_rows = b->page_rows;
+        L.pages_per_chunk = b->ppc;
+        L.n_rows = b->n_rows;
+        L.n_pages = b->n_pages;
+        L.queries = reinterpret_cast<const StarQueryDev *>(sl->d_q);
+        L.n_queries = b->n_queries();
+        L.items = sl->d_items;
+        L.queue = reinterpret_cast<uint32_t *>(sl->d_arena);
+        L.pages_per_item = c.pages_per_item;
+        L.filter_words = c.filter_lds ? b->max_filter_words : 0;
+        L.lds_gslots = c.lds_group_slots;
+        L.group_bits = group_bits_of(b);
+        L.max_blocks_per_cu = c.max_blocks_per_cu;
+        L.err = reinterpret_cast<uint32_t *>(sl->d_arena + 8);
+        return launch_star(b->ctx, L, s, prepare_only);
     }
-    L.n_windows = b->n_windows;
+    ScanLaunch L{};
+    L.n_cols = (uint32_t)b->cs.cols.size();
+    for (uint32_t i = 0; i < L.n_cols; ++i)
+        L.col[i] = ScanColView{view(b->cs.cols[i]), b->cs.cols[i]->type, b->windows[i]};
     L.page_rows = b->page_rows;
     L.pages_per_chunk = b->ppc;
     L.n_rows = b->n_rows;
     L.n_pages = b->n_pages;
-    L.queries = b->d_q;
-    L.n_queries = (uint32_t)b->q.size();
-    L.items = b->d_items;
-    L.queue = reinterpret_cast<uint32_t *>(b->d_arena);
+    L.win_bytes = b->win_bytes;
+    L.queries = reinterpret_cast<const ScanQueryDev *>(sl->d_q);
+    L.n_queries = b->n_queries();
+    L.items = sl->d_items;
+    L.queue = reinterpret_cast<uint32_t *>(sl->d_arena);
     L.pages_per_item = c.pages_per_item;
-    L.filter_words = c.filter_lds ? b->max_filter_words : 0;
     L.lds_gslots = c.lds_group_slots;
     L.group_bits = group_bits_of(b);
+    L.flush_fill = c.lds_group_slots / 2;
     L.max_blocks_per_cu = c.max_blocks_per_cu;
-    L.err = reinterpret_cast<uint32_t *>(b->d_arena + 8);
-    return L;
+    L.err = reinterpret_cast<uint32_t *>(sl->d_arena + 8);
+    return launch_scan(b->ctx, L, s, prepare_only);
 }
 
 // The run's three kernels on one stream.
-int enqueue(golap_star_batch *b, hipStream_t s) {
-    golap_ctx *ctx = b->ctx;
-    GOLAP_HIP_CHECK(ctx, launch_fill_zero(b->d_arena, b->arena_bytes, s));
-    GOLAP_HIP_CHECK(ctx, launch_batch_items(b->d_q, (uint32_t)sizeof(StarQueryDev), (uint32_t)b->q.size(),
-                                            b->d_chunk_off, b->chunk_off.back(),
-                                            parts_of(b, b->cfg.pages_per_item), b->d_items,
-                                            reinterpret_cast<uint32_t *>(b->d_arena), s));
-    GOLAP_HIP_CHECK(ctx, launch_star(ctx, launch_of(b, b->cfg), s));
+int enqueue_kernels(golap_batch *b, hipStream_t s) {
+    BatchSlot *sl = b->slot;
+    BATCH_HIP_CHECK(b, launch_fill_zero(sl->d_arena, b->arena_bytes, s));
+    BATCH_HIP_CHECK(b, launch_batch_items(sl->d_q, desc_size(b), b->n_queries(), sl->d_chunk_off, b->chunk_off.back(),
+                                          parts_of(b, b->cfg.pages_per_item), sl->d_items,
+                                          reinterpret_cast<uint32_t *>(sl->d_arena), s));
+    BATCH_HIP_CHECK(b, launch_kernel(b, s, false));
     return GOLAP_OK;
 }
 
-int capture(golap_star_batch *b) {
-    golap_ctx *ctx = b->ctx;
+int capture(golap_batch *b) {
     drop_graph(b);
-    GOLAP_HIP_CHECK(ctx, launch_star(ctx, launch_of(b, b->cfg), ctx->stream, true));
-    GOLAP_HIP_CHECK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    const int r = enqueue(b, ctx->stream);
+    hipStream_t s = b->slot->stream;
+    BATCH_HIP_CHECK(b, launch_kernel(b, s, true));  // kernel attributes outside the capture
+    BATCH_HIP_CHECK(b, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    const int r = enqueue_kernels(b, s);
     hipGraph_t g = nullptr;
-    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+    const hipError_t e = hipStreamEndCapture(s, &g);
     if (r) {
         if (g) (void)hipGraphDestroy(g);
         return r;
     }
-    GOLAP_HIP_CHECK(ctx, e);
+    BATCH_HIP_CHECK(b, e);
     b->graph = g;
-    GOLAP_HIP_CHECK(ctx, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    BATCH_HIP_CHECK(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
     return GOLAP_OK;
 }
 
-int apply_config(golap_star_batch *b, const golap_batch_config &c) {
+int apply_config(golap_batch *b, const golap_batch_config &c, bool with_graph) {
     if (int r = check_config(b, c)) return r;
     if (int r = upload_descriptors(b, c)) return r;
     b->cfg = c;
     b->ran = false;
-    return capture(b);
+    return with_graph ? capture(b) : GOLAP_OK;
 }
 
-golap_batch_config default_config(const golap_star_batch *b) {
+golap_batch_config default_config(golap_batch *b) {
     golap_batch_config c{};
     c.pages_per_item = b->ppc;
     while (parts_of(b, c.pages_per_item) > 256) ++c.pages_per_item;
-    c.lds_cutover = 16;
-    c.filter_lds = 1;
+    c.lds_cutover = b->kind == BatchKind::STAR ? 16 : 0;
+    c.filter_lds = b->kind == BatchKind::STAR;
     c.compact_tables = 1;
     c.max_blocks_per_cu = 0;
-    c.lds_group_slots = lds_keys_fit(b) ? LDS_GROUP_SLOTS_MAX : 0;
-    while (c.lds_group_slots &&
-           star_lds_bytes(b->max_filter_words, b->page_rows, b->n_windows, c.lds_group_slots) > (size_t)b->ctx->lds_max)
-        c.lds_group_slots >>= 1;
+    if (b->kind == BatchKind::STAR) {
+        c.lds_group_slots = lds_keys_fit(b) ? LDS_GROUP_SLOTS_MAX : 0;
+        while (c.lds_group_slots && lds_bytes(b, c) > (size_t)b->ctx->lds_max) c.lds_group_slots >>= 1;
+    } else {
+        // two workgroups per CU when the windows allow
+        c.lds_group_slots = lds_keys_fit(b) ? 2048 : 0;
+        while (c.lds_group_slots && lds_bytes(b, c) > std::max<size_t>(b->ctx->lds_max / 2, b->win_bytes + 1024))
+            c.lds_group_slots >>= 1;
+    }
     return c;
 }
 
-int check_builds(golap_star_batch *b) {
+int check_builds(golap_batch *b) {
     for (const QueryHost &h : b->qh)
         for (const auto &mb : h.builds)
             if (mb.first->build_id != mb.second)
-                return fail(b->ctx, GOLAP_EINVAL, "batch: a dimension map was rebuilt after the batch was created");
+                return bfail(b, GOLAP_EINVAL, "batch: a dimension map was rebuilt after the batch was created");
     return GOLAP_OK;
 }
 
-void free_batch(golap_star_batch *b) {
-    drop_graph(b);
-    dev_release(b->ctx, b->d_q);
-    dev_release(b->ctx, b->d_chunk_off);
-    dev_release(b->ctx, b->d_items);
-    dev_release(b->ctx, b->d_arena);
-    if (b->h_arena) (void)hipHostFree(b->h_arena);
-    delete b;
-}
-
-}  // namespace
-
-extern "C" {
-
-int golap_star_batch_create(golap_ctx *ctx, const golap_batch_query *queries, uint32_t n_queries,
-                            golap_star_batch **out) {
-    if (!ctx || !queries || !n_queries || !out || n_queries > (1u << 24)) return GOLAP_EINVAL;
-    *out = nullptr;
-    Lock l(ctx->mu);
-    if (int r = set_device(ctx)) return r;
-    auto *b = new golap_star_batch();
-    b->ctx = ctx;
-    b->q.assign(queries, queries + n_queries);
-    b->qh.resize(n_queries);
-    b->qd.resize(n_queries);
-    auto bail = [&](int r) {
-        free_batch(b);
-        return r;
-    };
-    for (uint32_t i = 0; i < n_queries; ++i) {
-        if (int r = prepare_query(ctx, &b->q[i].plan, true, b->cs, b->qd[i], b->qh[i])) return bail(r);
-        const golap_column *zc = b->q[i].zone_col;
-        if (zc) {
-            if (zc->ctx != ctx || zc->n_rows != b->q[i].plan.value->n_rows ||
-                zc->chunk_rows != b->q[i].plan.value->chunk_rows || !zc->d_chunk_min)
-                return bail(fail(ctx, GOLAP_EINVAL, "batch: zone column not row-aligned with the fact table"));
-            b->qd[i].s.zone[0] = ZonePred{zc->d_chunk_min, zc->d_chunk_max, b->q[i].zone_lo, b->q[i].zone_hi};
-        }
-        b->max_filter_words = std::max(b->max_filter_words, b->qd[i].filter_words);
-        b->max_domain = std::max(b->max_domain, b->qh[i].domain);
-    }
+// Slices, survivors (host view), arena layout, buffers and the first configuration.
+int finish_create(golap_ctx *ctx, golap_batch *b, bool with_graph) {
+    const uint32_t nq = b->n_queries();
     const golap_column *f = b->cs.cols[0];
     b->page_rows = f->page_rows;
     b->ppc = (uint32_t)(f->chunk_rows / f->page_rows);
     b->n_rows = f->n_rows;
     b->n_pages = f->n_pages;
-    uint64_t pb = ~0ull, pe = 0;
     b->chunk_off.assign(1, 0);
-    for (uint32_t i = 0; i < n_queries; ++i) {
-        pb = std::min(pb, b->qd[i].s.page_begin);
-        pe = std::max(pe, b->qd[i].s.page_end);
-        b->chunk_off.push_back(b->chunk_off.back() + (b->qd[i].s.chunk_end - b->qd[i].s.chunk_begin));
+    for (uint32_t i = 0; i < nq; ++i) {
+        const SliceDev &s = slice_of(b, i);
+        b->chunk_off.push_back(b->chunk_off.back() + (s.chunk_end - s.chunk_begin));
+        b->max_domain = std::max(b->max_domain, domain_of(b, i));
     }
-    b->n_windows = assign_windows(b->cs, std::min(pb, pe), pe, b->windows);
-    // host view of the surviving chunks: the same test as k_batch_items
-    b->survive.resize(n_queries);
-    b->surv_pages.assign(n_queries, 0);
-    for (uint32_t i = 0; i < n_queries; ++i) {
-        const SliceDev &d = b->qd[i].s;
+    b->survive.resize(nq);
+    b->surv_pages.assign(nq, 0);
+    for (uint32_t i = 0; i < nq; ++i) {
+        const SliceDev &d = slice_of(b, i);
         const uint64_t nch = d.chunk_end - d.chunk_begin;
         std::vector<uint16_t> mask;
         if (d.chunk_mask && nch) {
             mask.resize(nch);
-            if (hipMemcpyAsync(mask.data(), d.chunk_mask + d.chunk_begin, nch * 2, hipMemcpyDeviceToHost,
-                               ctx->stream) != hipSuccess ||
-                hipStreamSynchronize(ctx->stream) != hipSuccess)
-                return bail(fail(ctx, GOLAP_EHIP, "batch: reading a chunk mask failed"));
-        }
-        const golap_column *zc = b->q[i].zone_col;
-        auto &sv = b->survive[i];
-        sv.resize(nch);
-        for (uint64_t k = 0; k < nch; ++k) {
-            const uint64_t c = d.chunk_begin + k;
-            bool pass = mask.empty() || mask[k];
-            if (pass && zc) pass = b->q[i].zone_lo <= zc->chunk_max[c] && b->q[i].zone_hi >= zc->chunk_min[c];
-            sv[k] = pass;
-            if (pass) b->surv_pages[i] += std::min<uint64_t>(b->n_pages, (c + 1) * b->ppc) - c * b->ppc;
+            GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(mask.data(), d.chunk_mask + d.chunk_begin, nch * 2,
+                                                hipMemcpyDeviceToHost, ctx->stream));
+            GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
         }
         golap_star_result tmp{};
-        slice_stats(b->q[i].plan, sv, &tmp);
+        if (b->kind == BatchKind::STAR) {
+            const golap_star_query &q = b->sq[i];
+            auto &sv = b->survive[i];
+            sv.resize(nch);
+            for (uint64_t k = 0; k < nch; ++k) {
+                const uint64_t c = d.chunk_begin + k;
+                bool pass = mask.empty() || mask[k];
+                if (pass && q.zone_col) pass = q.zone_lo <= q.zone_col->chunk_max[c] && q.zone_hi >= q.zone_col->chunk_min[c];
+                sv[k] = pass;
+            }
+            slice_stats(q.plan, sv, &tmp);
+        } else {
+            b->survive[i] = scan_survivors(b->sh[i].plan, mask);
+            scan_stats(b->sh[i].plan, b->sh[i].colmask, b->survive[i], &tmp);
+        }
+        for (uint64_t k = 0; k < nch; ++k)
+            if (b->survive[i][k]) {
+                const uint64_t c = d.chunk_begin + k;
+                b->surv_pages[i] += std::min<uint64_t>(b->n_pages, (c + 1) * b->ppc) - c * b->ppc;
+            }
         b->total_pages += b->surv_pages[i];
         b->total_bytes += tmp.comp_bytes;
     }
-    // device buffers
     uint64_t off = ARENA_HEADER;
-    b->res_off.resize(n_queries);
-    for (uint32_t i = 0; i < n_queries; ++i) {
+    b->res_off.resize(nq);
+    for (uint32_t i = 0; i < nq; ++i) {
         b->res_off[i] = off;
-        off += b->qh[i].domain * 16;
+        off += domain_of(b, i) * 16;
     }
     b->arena_bytes = off;
-    void *p = nullptr;
-    if (int r = dev_alloc(ctx, off, &p)) return bail(r);
-    b->d_arena = static_cast<uint8_t *>(p);
-    if (hipHostMalloc(reinterpret_cast<void **>(&b->h_arena), off, hipHostMallocDefault) != hipSuccess) {
-        b->h_arena = nullptr;
-        return bail(fail(ctx, GOLAP_ENOMEM, "batch: pinned result mirror"));
-    }
-    if (int r = dev_alloc(ctx, n_queries * sizeof(StarQueryDev), &p)) return bail(r);
-    b->d_q = static_cast<StarQueryDev *>(p);
-    if (int r = dev_alloc(ctx, b->chunk_off.size() * 8, &p)) return bail(r);
-    b->d_chunk_off = static_cast<uint64_t *>(p);
-    const uint64_t items_cap = std::max<uint64_t>(1, b->chunk_off.back() * b->ppc);
-    if (items_cap >= (1ull << 32)) return bail(fail(ctx, GOLAP_EINVAL, "batch: too many work items"));
-    if (int r = dev_alloc(ctx, items_cap * sizeof(uint2), &p)) return bail(r);
-    b->d_items = static_cast<uint2 *>(p);
-    if (hipMemcpyAsync(b->d_chunk_off, b->chunk_off.data(), b->chunk_off.size() * 8, hipMemcpyHostToDevice,
-                       ctx->stream) != hipSuccess ||
-        hipStreamSynchronize(ctx->stream) != hipSuccess)
-        return bail(fail(ctx, GOLAP_EHIP, "batch: upload failed"));
+    const uint64_t items = std::max<uint64_t>(1, b->chunk_off.back() * b->ppc);
+    if (items >= (1ull << 32)) return fail(ctx, GOLAP_EINVAL, "batch: too many work items");
+    if (int r = b->slot->reserve(off, (uint64_t)nq * desc_size(b), b->chunk_off.size(), items))
+        return fail(ctx, r, b->slot->err);
+    GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(b->slot->d_chunk_off, b->chunk_off.data(), b->chunk_off.size() * 8,
+                                        hipMemcpyHostToDevice, b->slot->stream));
     b->shape = shape_of(b);
     golap_batch_config c = default_config(b);
     auto it = ctx->tune_cache.find(b->shape);
     if (it != ctx->tune_cache.end() && check_config(b, it->second) == GOLAP_OK) c = it->second;
-    if (int r = apply_config(b, c)) return bail(r);
+    if (int r = apply_config(b, c, with_graph)) return fail(ctx, r, b->err);
+    return GOLAP_OK;
+}
+
+golap_batch *new_batch(golap_ctx *ctx, BatchKind kind, BatchSlot *slot, int *st) {
+    auto *b = new golap_batch();
+    b->ctx = ctx;
+    b->kind = kind;
+    *st = GOLAP_OK;
+    if (slot) {
+        b->slot = slot;
+    } else {
+        b->slot = new BatchSlot();
+        b->own_slot = true;
+        if (int r = b->slot->init(ctx, false)) *st = fail(ctx, r, b->slot->err);
+    }
+    return b;
+}
+
+}  // namespace
+
+namespace golap_impl {
+
+int BatchSlot::init(golap_ctx *c, bool own) {
+    ctx = c;
+    own_stream = own;
+    if (own) {
+        if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) {
+            err = "batch slot: stream";
+            return GOLAP_EHIP;
+        }
+    } else {
+        stream = c->stream;
+    }
+    if (hipEventCreate(&ev0) != hipSuccess || hipEventCreate(&ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&dep, hipEventDisableTiming) != hipSuccess) {
+        err = "batch slot: events";
+        return GOLAP_EHIP;
+    }
+    return GOLAP_OK;
+}
+
+int BatchSlot::reserve(uint64_t arena, uint64_t qbytes, uint64_t offs, uint64_t items) {
+    auto grow = [&](auto *&p, uint64_t &cap, uint64_t need, uint64_t unit) -> int {
+        if (need <= cap) return GOLAP_OK;
+        const uint64_t want = std::max(need, cap * 2);
+        dev_release(ctx, p);
+        p = nullptr;
+        cap = 0;
+        void *q = nullptr;
+        if (int r = dev_alloc(ctx, want * unit, &q)) {
+            err = ctx->err;
+            return r;
+        }
+        p = static_cast<std::remove_reference_t<decltype(p)>>(q);
+        cap = want;
+        return GOLAP_OK;
+    };
+    const bool arena_grows = arena > arena_cap;
+    if (int r = grow(d_arena, arena_cap, arena, 1)) return r;
+    if (arena_grows && h_arena) {
+        (void)hipHostFree(h_arena);
+        h_arena = nullptr;
+    }
+    if (!h_arena && hipHostMalloc(reinterpret_cast<void **>(&h_arena), arena_cap, hipHostMallocDefault) != hipSuccess) {
+        h_arena = nullptr;
+        err = "batch slot: pinned result mirror";
+        return GOLAP_ENOMEM;
+    }
+    if (int r = grow(d_q, q_cap, qbytes, 1)) return r;
+    if (int r = grow(d_chunk_off, off_cap, offs, 8)) return r;
+    if (int r = grow(d_items, items_cap, items, sizeof(uint2))) return r;
+    return GOLAP_OK;
+}
+
+void BatchSlot::release() {
+    dev_release(ctx, d_arena);
+    dev_release(ctx, d_q);
+    dev_release(ctx, d_chunk_off);
+    dev_release(ctx, d_items);
+    if (h_arena) (void)hipHostFree(h_arena);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (dep) (void)hipEventDestroy(dep);
+    if (own_stream && stream) (void)hipStreamDestroy(stream);
+    d_arena = h_arena = d_q = nullptr;
+    d_chunk_off = nullptr;
+    d_items = nullptr;
+    ev0 = ev1 = dep = nullptr;
+    stream = nullptr;
+    arena_cap = q_cap = off_cap = items_cap = 0;
+}
+
+uint64_t batch_query_bytes(uint64_t domain, uint64_t slice_chunks, uint32_t pages_per_chunk, bool star) {
+    return domain * 16 + slice_chunks * pages_per_chunk * sizeof(uint2) + 8 +
+           (star ? sizeof(StarQueryDev) : sizeof(ScanQueryDev));
+}
+
+int batch_create_star(golap_ctx *ctx, const golap_star_query *q, uint32_t n, BatchSlot *slot, bool with_graph,
+                      golap_batch **out) {
+    *out = nullptr;
+    int st;
+    golap_batch *b = new_batch(ctx, BatchKind::STAR, slot, &st);
+    auto bail = [&](int r) {
+        batch_free(b);
+        return r;
+    };
+    if (st) return bail(st);
+    b->sq.assign(q, q + n);
+    b->qh.resize(n);
+    b->qd.resize(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (int r = prepare_query(ctx, &b->sq[i].plan, true, b->cs, b->qd[i], b->qh[i])) return bail(r);
+        const golap_column *zc = b->sq[i].zone_col;
+        if (zc) {
+            if (zc->ctx != ctx || zc->n_rows != b->sq[i].plan.value->n_rows ||
+                zc->chunk_rows != b->sq[i].plan.value->chunk_rows || !zc->d_chunk_min)
+                return bail(fail(ctx, GOLAP_EINVAL, "batch: zone column not row-aligned with the fact table"));
+            b->qd[i].s.zone[0] = ZonePred{zc->d_chunk_min, zc->d_chunk_max, b->sq[i].zone_lo, b->sq[i].zone_hi};
+        }
+        b->max_filter_words = std::max(b->max_filter_words, b->qd[i].filter_words);
+    }
+    uint64_t pb = ~0ull, pe = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        pb = std::min(pb, b->qd[i].s.page_begin);
+        pe = std::max(pe, b->qd[i].s.page_end);
+    }
+    b->n_windows = assign_windows(b->cs, std::min(pb, pe), pe, b->windows);
+    if (int r = finish_create(ctx, b, with_graph)) return bail(r);
     *out = b;
     return GOLAP_OK;
 }
 
-int golap_star_batch_config_get(golap_star_batch *b, golap_batch_config *cfg) {
-    if (!b || !cfg) return GOLAP_EINVAL;
-    *cfg = b->cfg;
-    return GOLAP_OK;
-}
-
-int golap_star_batch_config_set(golap_star_batch *b, const golap_batch_config *cfg) {
-    if (!b || !cfg) return GOLAP_EINVAL;
-    Lock l(b->ctx->mu);
-    if (int r = set_device(b->ctx)) return r;
-    const golap_batch_config old = b->cfg;
-    if (int r = apply_config(b, *cfg)) {
-        (void)apply_config(b, old);
+int batch_create_scan(golap_ctx *ctx, const golap_scan_plan *q, uint32_t n, BatchSlot *slot, bool with_graph,
+                      golap_batch **out) {
+    *out = nullptr;
+    int st;
+    golap_batch *b = new_batch(ctx, BatchKind::SCAN, slot, &st);
+    auto bail = [&](int r) {
+        batch_free(b);
         return r;
+    };
+    if (st) return bail(st);
+    b->sh.resize(n);
+    b->sd.resize(n);
+    uint64_t pb = ~0ull, pe = 0;
+    uint32_t used = 0;  // launch columns read by some query
+    for (uint32_t i = 0; i < n; ++i) {
+        if (int r = prepare_scan(ctx, &q[i], b->sd[i], b->sh[i])) return bail(r);
+        // plan-local column indexes -> indexes into the launch's columns
+        uint8_t map[SCAN_MAX_COLS];
+        for (uint32_t c = 0; c < q[i].n_cols; ++c)
+            if (int r = b->cs.index(ctx, q[i].cols[c], &map[c])) return bail(r);
+        ScanQueryDev &d = b->sd[i];
+        for (uint32_t k = 0; k < d.n_preds; ++k) d.pred[k].col = map[d.pred[k].col];
+        for (uint32_t k = 0; k < d.n_keys; ++k) d.key[k].col = map[d.key[k].col];
+        if (d.val_col >= 0) d.val_col = map[d.val_col];
+        uint32_t m = 0;
+        for (uint32_t c = 0; c < q[i].n_cols; ++c)
+            if ((d.colmask >> c) & 1u) m |= 1u << map[c];
+        d.colmask = m;
+        used |= m;
+        pb = std::min(pb, d.s.page_begin);
+        pe = std::max(pe, d.s.page_end);
     }
+    // LDS windows of the used columns that have some coded page in the union of the slices
+    {
+        std::lock_guard<std::mutex> g(g_extra_mu);
+        pb = std::min(pb, pe);
+        for (size_t c = 0; c < b->cs.cols.size(); ++c) {
+            b->windows[c] = -1;
+            if (!((used >> c) & 1u)) continue;
+            auto &ex = g_extra[b->cs.cols[c]];
+            if (ex.window_prefix[pe] == ex.window_prefix[pb]) continue;
+            b->windows[c] = (int)b->win_bytes;
+            b->win_bytes += (b->cs.cols[c]->page_rows * b->cs.cols[c]->elem + 15) & ~15u;
+            ++b->n_windows;
+        }
+    }
+    if (int r = finish_create(ctx, b, with_graph)) return bail(r);
+    *out = b;
     return GOLAP_OK;
 }
 
-int golap_star_batch_run(golap_star_batch *b, uint32_t use_graph, golap_batch_run_info *info) {
-    if (!b) return GOLAP_EINVAL;
-    golap_ctx *ctx = b->ctx;
-    Lock l(ctx->mu);
-    if (int r = set_device(ctx)) return r;
+int batch_enqueue(golap_batch *b, bool use_graph) {
     if (int r = check_builds(b)) return r;
-    GOLAP_HIP_CHECK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    if (use_graph) {
-        GOLAP_HIP_CHECK(ctx, hipGraphLaunch(b->exec, ctx->stream));
-    } else if (int r = enqueue(b, ctx->stream)) {
+    BatchSlot *sl = b->slot;
+    if (sl->stream != b->ctx->stream) {  // after the context stream's pending work (masks, builds)
+        BATCH_HIP_CHECK(b, hipEventRecord(sl->dep, b->ctx->stream));
+        BATCH_HIP_CHECK(b, hipStreamWaitEvent(sl->stream, sl->dep, 0));
+    }
+    BATCH_HIP_CHECK(b, hipEventRecord(sl->ev0, sl->stream));
+    if (use_graph && b->exec) {
+        BATCH_HIP_CHECK(b, hipGraphLaunch(b->exec, sl->stream));
+    } else if (int r = enqueue_kernels(b, sl->stream)) {
         return r;
     }
-    GOLAP_HIP_CHECK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
-    uint32_t hdr[4] = {0, 0, 0, 0};
-    GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(hdr, b->d_arena, 16, hipMemcpyDeviceToHost, ctx->stream));
-    GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    float ms = 0;
-    GOLAP_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-    b->last_ms = ms;
+    BATCH_HIP_CHECK(b, hipEventRecord(sl->ev1, sl->stream));
     b->ran = false;
     b->h_valid = false;
-    if (int r = status_of_flags(ctx, hdr[2], "batch")) return r;
+    return GOLAP_OK;
+}
+
+int batch_finish(golap_batch *b, golap_batch_run_info *info) {
+    BatchSlot *sl = b->slot;
+    uint32_t *hdr = reinterpret_cast<uint32_t *>(sl->h_arena);
+    BATCH_HIP_CHECK(b, hipMemcpyAsync(hdr, sl->d_arena, 16, hipMemcpyDeviceToHost, sl->stream));
+    BATCH_HIP_CHECK(b, hipStreamSynchronize(sl->stream));
+    float ms = 0;
+    BATCH_HIP_CHECK(b, hipEventElapsedTime(&ms, sl->ev0, sl->ev1));
+    b->last_ms = ms;
+    const uint32_t flags = hdr[2];
+    if (flags & DEVERR_CORRUPT) return bfail(b, GOLAP_ECORRUPT, "batch: corrupt page");
+    if (flags & DEVERR_CAPACITY) return bfail(b, GOLAP_ECAPACITY, "batch: table full");
+    if (flags) return bfail(b, GOLAP_EINVAL, "batch: value outside its declared range");
     b->ran = true;
     if (info) {
         info->ms = ms;
@@ -378,7 +531,107 @@ int golap_star_batch_run(golap_star_batch *b, uint32_t use_graph, golap_batch_ru
     return GOLAP_OK;
 }
 
-int golap_star_batch_tune(golap_star_batch *b, uint32_t reps, golap_batch_config *best, double *best_ms) {
+int batch_result(golap_batch *b, uint32_t q, golap_star_result *res, int32_t *keys, int64_t *sums, uint64_t *counts,
+                 uint64_t max_groups) {
+    if (!b->ran) return bfail(b, GOLAP_EINVAL, "batch: no completed run since creation or reconfiguration");
+    BatchSlot *sl = b->slot;
+    if (!b->h_valid) {  // every query's results in one copy
+        BATCH_HIP_CHECK(b, hipMemcpyAsync(sl->h_arena + ARENA_HEADER, sl->d_arena + ARENA_HEADER,
+                                          b->arena_bytes - ARENA_HEADER, hipMemcpyDeviceToHost, sl->stream));
+        BATCH_HIP_CHECK(b, hipStreamSynchronize(sl->stream));
+        b->h_valid = true;
+    }
+    memset(res, 0, sizeof(*res));
+    res->kernel_ms = b->last_ms;
+    const std::vector<int32_t> *lo;
+    const std::vector<uint32_t> *card;
+    const std::vector<uint64_t> *stride;
+    if (b->kind == BatchKind::STAR) {
+        res->driver_dim = b->qd[q].driver;
+        res->lds_group_slots = b->qd[q].use_lds ? b->cfg.lds_group_slots : 0;
+        slice_stats(b->sq[q].plan, b->survive[q], res);
+        lo = &b->qh[q].lo;
+        card = &b->qh[q].card;
+        stride = &b->qh[q].stride;
+    } else {
+        res->lds_group_slots = b->sd[q].use_lds ? b->cfg.lds_group_slots : 0;
+        scan_stats(b->sh[q].plan, b->sh[q].colmask, b->survive[q], res);
+        lo = &b->sh[q].lo;
+        card = &b->sh[q].card;
+        stride = &b->sh[q].stride;
+    }
+    const uint64_t dom = domain_of(b, q);
+    const int64_t *hs = reinterpret_cast<const int64_t *>(sl->h_arena + b->res_off[q]);
+    const uint64_t *hc = reinterpret_cast<const uint64_t *>(sl->h_arena + b->res_off[q] + dom * 8);
+    uint64_t g = 0;
+    int64_t total = 0;
+    const int r = emit_groups_host(nullptr, dom, hs, hc, *lo, *card, *stride, &g, &total, keys, sums, counts,
+                                   max_groups, &res->rows_matched);
+    res->n_groups = g;
+    res->total_sum = total;
+    if (r) return bfail(b, r, "more groups than max_groups");
+    return GOLAP_OK;
+}
+
+void batch_free(golap_batch *b) {
+    if (!b) return;
+    drop_graph(b);
+    if (b->own_slot && b->slot) {
+        b->slot->release();
+        delete b->slot;
+    }
+    delete b;
+}
+
+}  // namespace golap_impl
+
+extern "C" {
+
+int golap_batch_create_star(golap_ctx *ctx, const golap_star_query *queries, uint32_t n_queries, golap_batch **out) {
+    if (!ctx || !queries || !n_queries || !out || n_queries > (1u << 24)) return GOLAP_EINVAL;
+    Lock l(ctx->mu);
+    if (int r = set_device(ctx)) return r;
+    return batch_create_star(ctx, queries, n_queries, nullptr, true, out);
+}
+
+int golap_batch_create_scan(golap_ctx *ctx, const golap_scan_plan *plans, uint32_t n_plans, golap_batch **out) {
+    if (!ctx || !plans || !n_plans || !out || n_plans > (1u << 24)) return GOLAP_EINVAL;
+    Lock l(ctx->mu);
+    if (int r = set_device(ctx)) return r;
+    return batch_create_scan(ctx, plans, n_plans, nullptr, true, out);
+}
+
+int golap_batch_config_get(golap_batch *b, golap_batch_config *cfg) {
+    if (!b || !cfg) return GOLAP_EINVAL;
+    *cfg = b->cfg;
+    return GOLAP_OK;
+}
+
+int golap_batch_config_set(golap_batch *b, const golap_batch_config *cfg) {
+    if (!b || !cfg) return GOLAP_EINVAL;
+    golap_ctx *ctx = b->ctx;
+    Lock l(ctx->mu);
+    if (int r = set_device(ctx)) return r;
+    const golap_batch_config old = b->cfg;
+    if (int r = apply_config(b, *cfg, true)) {
+        const std::string msg = b->err;
+        (void)apply_config(b, old, true);
+        return fail(ctx, r, msg);
+    }
+    return GOLAP_OK;
+}
+
+int golap_batch_run(golap_batch *b, uint32_t use_graph, golap_batch_run_info *info) {
+    if (!b) return GOLAP_EINVAL;
+    golap_ctx *ctx = b->ctx;
+    Lock l(ctx->mu);
+    if (int r = set_device(ctx)) return r;
+    if (int r = batch_enqueue(b, use_graph != 0)) return fail(ctx, r, b->err);
+    if (int r = batch_finish(b, info)) return fail(ctx, r, b->err);
+    return GOLAP_OK;
+}
+
+int golap_batch_tune(golap_batch *b, uint32_t reps, golap_batch_config *best, double *best_ms) {
     if (!b) return GOLAP_EINVAL;
     golap_ctx *ctx = b->ctx;
     Lock l(ctx->mu);
@@ -386,12 +639,12 @@ int golap_star_batch_tune(golap_star_batch *b, uint32_t reps, golap_batch_config
     reps = std::max<uint32_t>(reps, 1);
     auto time_cfg = [&](const golap_batch_config &c, double *ms) -> int {
         if (check_config(b, c)) return GOLAP_EINVAL;
-        if (int r = apply_config(b, c)) return r;
-        if (int r = golap_star_batch_run(b, 1, nullptr)) return r;  // warm-up
+        if (int r = apply_config(b, c, true)) return r;
+        if (int r = golap_batch_run(b, 1, nullptr)) return r;  // warm-up
         std::vector<double> t;
         for (uint32_t i = 0; i < reps; ++i) {
             golap_batch_run_info info{};
-            if (int r = golap_star_batch_run(b, 1, &info)) return r;
+            if (int r = golap_batch_run(b, 1, &info)) return r;
             t.push_back(info.ms);
         }
         std::sort(t.begin(), t.end());
@@ -400,9 +653,11 @@ int golap_star_batch_tune(golap_star_batch *b, uint32_t reps, golap_batch_config
     };
     golap_batch_config cur = b->cfg;
     double cur_ms = 0;
-    if (int r = time_cfg(cur, &cur_ms)) return r;
+    if (int r = time_cfg(cur, &cur_ms)) return fail(ctx, r, b->err);
+    const bool star = b->kind == BatchKind::STAR;
     // coordinate descent over the knobs, in order of expected effect
-    std::vector<std::vector<golap_batch_config>> axes;
+    enum Axis { LDS, GRAIN, FILTER, TABLES, OCC };
+    std::vector<std::pair<Axis, std::vector<golap_batch_config>>> axes;
     {
         std::vector<golap_batch_config> ax;
         for (uint32_t slots : {0u, 1024u, 2048u, 4096u})
@@ -413,7 +668,7 @@ int golap_star_batch_tune(golap_star_batch *b, uint32_t reps, golap_batch_config
                 c.lds_cutover = cut;
                 ax.push_back(c);
             }
-        axes.push_back(ax);
+        axes.emplace_back(LDS, ax);
     }
     {
         std::vector<golap_batch_config> ax;
@@ -422,10 +677,16 @@ int golap_star_batch_tune(golap_star_batch *b, uint32_t reps, golap_batch_config
             c.pages_per_item = ppi;
             ax.push_back(c);
         }
-        axes.push_back(ax);
+        axes.emplace_back(GRAIN, ax);
     }
-    axes.push_back({golap_batch_config{0, 0, 0, 1, 0, 0}, golap_batch_config{0, 0, 0, 0, 0, 0}});  // filter
-    axes.push_back({golap_batch_config{0, 0, 0, 0, 1, 0}, golap_batch_config{0, 0, 0, 0, 0, 0}});  // tables
+    if (star) {
+        golap_batch_config on{}, off{};
+        on.filter_lds = 1;
+        axes.emplace_back(FILTER, std::vector<golap_batch_config>{on, off});
+        on = off = golap_batch_config{};
+        on.compact_tables = 1;
+        axes.emplace_back(TABLES, std::vector<golap_batch_config>{on, off});
+    }
     {
         std::vector<golap_batch_config> ax;
         for (uint32_t mb : {0u, 1u, 2u, 3u}) {
@@ -433,66 +694,51 @@ int golap_star_batch_tune(golap_star_batch *b, uint32_t reps, golap_batch_config
             c.max_blocks_per_cu = mb;
             ax.push_back(c);
         }
-        axes.push_back(ax);
+        axes.emplace_back(OCC, ax);
     }
-    for (size_t a = 0; a < axes.size(); ++a)
-        for (const golap_batch_config &v : axes[a]) {
+    for (const auto &axis : axes)
+        for (const golap_batch_config &v : axis.second) {
             golap_batch_config c = cur;
-            switch (a) {
-            case 0: c.lds_group_slots = v.lds_group_slots; c.lds_cutover = v.lds_cutover; break;
-            case 1: c.pages_per_item = v.pages_per_item; break;
-            case 2: c.filter_lds = v.filter_lds; break;
-            case 3: c.compact_tables = v.compact_tables; break;
-            default: c.max_blocks_per_cu = v.max_blocks_per_cu; break;
+            switch (axis.first) {
+            case LDS: c.lds_group_slots = v.lds_group_slots; c.lds_cutover = v.lds_cutover; break;
+            case GRAIN: c.pages_per_item = v.pages_per_item; break;
+            case FILTER: c.filter_lds = v.filter_lds; break;
+            case TABLES: c.compact_tables = v.compact_tables; break;
+            case OCC: c.max_blocks_per_cu = v.max_blocks_per_cu; break;
             }
             if (!memcmp(&c, &cur, sizeof(c))) continue;
             double ms = 0;
             const int r = time_cfg(c, &ms);
             if (r == GOLAP_EINVAL) continue;  // not valid for this batch (LDS budget, item limits)
-            if (r) return r;
+            if (r) return fail(ctx, r, b->err);
             if (ms < cur_ms * 0.98) {  // keep the incumbent on ties (timing noise)
                 cur = c;
                 cur_ms = ms;
             }
         }
-    if (int r = apply_config(b, cur)) return r;
+    if (int r = apply_config(b, cur, true)) return fail(ctx, r, b->err);
     ctx->tune_cache[b->shape] = cur;
     if (best) *best = cur;
     if (best_ms) *best_ms = cur_ms;
     return GOLAP_OK;
 }
 
-int golap_star_batch_result(golap_star_batch *b, uint32_t q, golap_star_result *res, int32_t *group_keys,
-                            int64_t *sums, uint64_t *counts, uint64_t max_groups) {
-    if (!b || !res || q >= b->q.size()) return GOLAP_EINVAL;
+int golap_batch_result(golap_batch *b, uint32_t q, golap_star_result *res, int32_t *group_keys, int64_t *sums,
+                       uint64_t *counts, uint64_t max_groups) {
+    if (!b || !res || q >= b->n_queries()) return GOLAP_EINVAL;
     golap_ctx *ctx = b->ctx;
     Lock l(ctx->mu);
-    if (!b->ran) return fail(ctx, GOLAP_EINVAL, "batch: no completed run since creation or reconfiguration");
     if (int r = set_device(ctx)) return r;
-    memset(res, 0, sizeof(*res));
-    const QueryHost &h = b->qh[q];
-    res->kernel_ms = b->last_ms;
-    res->driver_dim = b->qd[q].driver;
-    res->lds_group_slots = b->qd[q].use_lds ? b->cfg.lds_group_slots : 0;
-    slice_stats(b->q[q].plan, b->survive[q], res);
-    if (!b->h_valid) {  // every query's results in one copy
-        GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(b->h_arena + ARENA_HEADER, b->d_arena + ARENA_HEADER,
-                                            b->arena_bytes - ARENA_HEADER, hipMemcpyDeviceToHost, ctx->stream));
-        GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-        b->h_valid = true;
-    }
-    const int64_t *hs = reinterpret_cast<const int64_t *>(b->h_arena + b->res_off[q]);
-    const uint64_t *hc = reinterpret_cast<const uint64_t *>(b->h_arena + b->res_off[q] + h.domain * 8);
-    return emit_groups_host(ctx, h.domain, hs, hc, h.lo, h.card, h.stride, &res->n_groups, &res->total_sum,
-                            group_keys, sums, counts, max_groups, &res->rows_matched);
+    if (int r = batch_result(b, q, res, group_keys, sums, counts, max_groups)) return fail(ctx, r, b->err);
+    return GOLAP_OK;
 }
 
-int golap_star_batch_free(golap_star_batch *b) {
+int golap_batch_free(golap_batch *b) {
     if (!b) return GOLAP_EINVAL;
     golap_ctx *ctx = b->ctx;
     Lock l(ctx->mu);
     if (int r = set_device(ctx)) return r;
-    free_batch(b);
+    batch_free(b);
     return GOLAP_OK;
 }
 

@@ -77,6 +77,7 @@ int prepare_scan(golap_ctx *ctx, const golap_scan_plan *p, ScanQueryDev &qd, Sca
             qd.s.zone[z] = ZonePred{zc->d_chunk_min, zc->d_chunk_max, p->preds[z].lo, p->preds[z].hi};
         }
     qd.n_groups = h.domain;
+    h.colmask = qd.colmask;
     return GOLAP_OK;
 }
 

@@ -54,20 +54,27 @@ int emit_groups_host(golap_ctx *ctx, uint64_t n, const int64_t *hs, const uint64
                      const std::vector<int32_t> &lo, const std::vector<uint32_t> &card,
                      const std::vector<uint64_t> &stride, uint64_t *n_groups, int64_t *total, int32_t *keys,
                      int64_t *sums, uint64_t *counts, uint64_t max_groups, uint64_t *rows) {
+    (void)stride;  // index i = sum of digit[a] * stride[a]: the odometer below walks the digits
     uint64_t g = 0, nr = 0;
     int64_t t = 0;
     const size_t na = card.size();
+    std::vector<uint32_t> digit(na, 0);
     for (uint64_t i = 0; i < n; ++i) {
-        if (!hc[i]) continue;
-        t += hs[i];
-        nr += hc[i];
-        if (g < max_groups) {
-            for (size_t a = 0; a < na && keys; ++a)
-                keys[g * na + a] = lo[a] + (int32_t)((i / stride[a]) % card[a]);
-            if (sums) sums[g] = hs[i];
-            if (counts) counts[g] = hc[i];
+        if (hc[i]) {
+            t += hs[i];
+            nr += hc[i];
+            if (g < max_groups) {
+                if (keys)
+                    for (size_t a = 0; a < na; ++a) keys[g * na + a] = lo[a] + (int32_t)digit[a];
+                if (sums) sums[g] = hs[i];
+                if (counts) counts[g] = hc[i];
+            }
+            ++g;
         }
-        ++g;
+        for (size_t a = na; a-- > 0;) {  // next index: increment the last digit, carry left
+            if (++digit[a] < card[a]) break;
+            digit[a] = 0;
+        }
     }
     *n_groups = g;
     if (total) *total = t;

@@ -33,6 +33,7 @@ struct QueryHost {
 // Host side of one prepared scan plan.
 struct ScanHost {
     golap_scan_plan plan{};
+    uint32_t colmask = 0;  // columns read, as indexes of plan.cols
     std::vector<int32_t> lo;
     std::vector<uint32_t> card;
     std::vector<uint64_t> stride;

@@ -308,59 +308,108 @@ typedef struct {
 int golap_scan_run(golap_ctx *ctx, const golap_scan_plan *plan, golap_star_result *res, int32_t *group_keys,
                    int64_t *sums, uint64_t *counts, uint64_t max_groups);
 
-/* ------------------------------------------------------------------ batched star plans
- * Many star plans over one fact table in one persistent fused launch. The reference runs one
- * plan per call, with a worker loop over chunks (query2/query3, 02.hpp:54-307, 03.hpp:79-677),
- * and prunes with MetaChecker before the loop (01.hpp:305-332). Here each query of the batch
- * brings its own dimension maps, chunk range, optional chunk mask and optional zone-map range
- * predicate (check_mmmeta semantics, metadata.cuh:539-547: a chunk survives iff
- * lo <= max && hi >= min on zone_col). One run is three kernels:
+/* ------------------------------------------------------------------ batched plans
+ * Many plans of one kind over one fact table in one persistent fused launch. The reference
+ * runs one plan per call, with a worker loop over chunks (query2/query3, 02.hpp:54-307,
+ * 03.hpp:79-677), and prunes with MetaChecker before the loop (01.hpp:305-332).
+ * Each query brings:
+ *   - a star query: its own dimension maps, chunk range, optional chunk mask and optional
+ *     zone-map range predicate on zone_col;
+ *   - a scan plan: its own predicates and keys, and zone pruning on its first two predicates.
+ * The zone-map test is check_mmmeta (metadata.cuh:539-547): a chunk survives iff
+ * lo <= max && hi >= min.
+ * One run is three kernels:
  *   1. zero the result arena;
  *   2. build the work list: one entry per surviving (query, chunk part);
- *   3. the persistent star kernel, whose workgroups take work-list items with one atomic each.
+ *   3. the persistent kernel, whose workgroups take work-list items with one atomic each.
  * The three are captured once into a hipGraph that every run replays.
- * Every query's results equal golap_star_run of its plan alone. The fact columns of all queries
- * (foreign keys and values, at most 8 distinct columns) must be row-aligned.
- * golap_star_batch_tune times candidate launch configurations and keeps the fastest (autotuner):
+ * Every query's results equal golap_star_run / golap_scan_run of its plan alone. All queries'
+ * fact columns (at most 8 distinct columns) must be row-aligned.
+ * golap_batch_tune times candidate launch configurations and keeps the fastest (autotuner):
  *   - pages per work item (tile grain);
  *   - LDS group-table slots and the LDS-versus-global aggregation cutover;
- *   - the driver filter in LDS or in L2;
- *   - compact or full probe tables;
+ *   - star plans: the driver filter in LDS or in L2, and compact or full probe tables;
  *   - workgroups per CU.
- * The chosen configuration is cached in the context for batches of the same shape. */
-typedef struct golap_star_batch golap_star_batch;
+ * The chosen configuration is cached in the context for batches of the same shape.
+ * The maps and columns a batch reads must outlive it. A star batch fails to run once one of
+ * its maps has been rebuilt. */
+typedef struct golap_batch golap_batch;
 typedef struct {
     golap_star_plan plan;           /* plan.lds_group_slots is ignored: see golap_batch_config */
     const golap_column *zone_col;   /* NULL: no zone-map predicate */
     int64_t zone_lo, zone_hi;
-} golap_batch_query;
+} golap_star_query;
 typedef struct {
     uint32_t pages_per_item;     /* pages of one chunk per work item, 1..pages per chunk */
     uint32_t lds_group_slots;    /* LDS group table (power of two <= 4096), 0 = global atomics */
     uint32_t lds_cutover;        /* AUTO queries aggregate in LDS when their expected matching
                                     rows reach lds_cutover x their group count */
-    uint32_t filter_lds;         /* 1: copy each query's driver filter into LDS */
-    uint32_t compact_tables;     /* 1: probe the compact tables (2 x filled slots) */
+    uint32_t filter_lds;         /* star: 1 = copy each query's driver filter into LDS */
+    uint32_t compact_tables;     /* star: 1 = probe the compact tables (2 x filled slots) */
     uint32_t max_blocks_per_cu;  /* 0 = occupancy limit */
 } golap_batch_config;
 typedef struct {
     double ms;                   /* device time of the run (graph or plain launches) */
     uint64_t items;              /* work items built on the device */
     uint64_t pages;              /* page tiles of the surviving chunks, all queries */
-    uint64_t comp_bytes;         /* compressed bytes of the surviving pages, all queries */
+    uint64_t comp_bytes;         /* compressed bytes of the surviving pages' columns, all queries */
 } golap_batch_run_info;
-int golap_star_batch_create(golap_ctx *ctx, const golap_batch_query *queries, uint32_t n_queries,
-                            golap_star_batch **out);
-int golap_star_batch_config_get(golap_star_batch *b, golap_batch_config *cfg);
-int golap_star_batch_config_set(golap_star_batch *b, const golap_batch_config *cfg);
+int golap_batch_create_star(golap_ctx *ctx, const golap_star_query *queries, uint32_t n_queries,
+                            golap_batch **out);
+int golap_batch_create_scan(golap_ctx *ctx, const golap_scan_plan *plans, uint32_t n_plans,
+                            golap_batch **out);
+int golap_batch_config_get(golap_batch *b, golap_batch_config *cfg);
+int golap_batch_config_set(golap_batch *b, const golap_batch_config *cfg);
 /* use_graph 1: replay the captured graph; 0: the same three kernels as plain launches. */
-int golap_star_batch_run(golap_star_batch *b, uint32_t use_graph, golap_batch_run_info *info);
+int golap_batch_run(golap_batch *b, uint32_t use_graph, golap_batch_run_info *info);
 /* Times each candidate reps times (after one warm-up), keeps and caches the fastest. */
-int golap_star_batch_tune(golap_star_batch *b, uint32_t reps, golap_batch_config *best, double *best_ms);
-/* Results of query q of the last run, as golap_star_run returns them (kernel_ms = the run's). */
-int golap_star_batch_result(golap_star_batch *b, uint32_t q, golap_star_result *res, int32_t *group_keys,
-                            int64_t *sums, uint64_t *counts, uint64_t max_groups);
-int golap_star_batch_free(golap_star_batch *b);
+int golap_batch_tune(golap_batch *b, uint32_t reps, golap_batch_config *best, double *best_ms);
+/* Results of query q of the last run, as the single-query entry points return them. */
+int golap_batch_result(golap_batch *b, uint32_t q, golap_star_result *res, int32_t *group_keys,
+                       int64_t *sums, uint64_t *counts, uint64_t max_groups);
+int golap_batch_free(golap_batch *b);
+
+/* ------------------------------------------------------------------ serving loop
+ * No counterpart in the reference, which runs one plan per call on its caller's thread.
+ * A request loop over one context:
+ *   - Submitted star queries and scan plans wait in arrival order.
+ *   - They are admitted while the device bytes of admitted, unfinished requests (result arrays,
+ *     work lists, descriptors) stay within hbm_budget.
+ *   - As soon as a slot is free, the admitted requests of one kind form a batch, run as one
+ *     persistent fused launch on that slot's own stream (golap_batch_*).
+ *   - Up to `slots` batches are on the device at once; requests arriving meanwhile form the next
+ *     batch (continuous batching).
+ *   - The dimension maps and compressed columns a request names stay resident across requests
+ *     (the caller builds them once).
+ *   - Results are the single-query entry points' results, emitted by a pool of host threads.
+ *   - golap_server_wait blocks until the request is done and reports its latency from submit
+ *     to results ready.
+ * A request whose own device bytes exceed hbm_budget is refused with GOLAP_EBUDGET; an invalid
+ * plan is refused at submit with the single-query entry point's status. */
+typedef struct golap_server golap_server;
+typedef struct {
+    uint64_t hbm_budget;       /* bytes; 0 = 4 GiB */
+    uint32_t max_batch;        /* requests per launch; 0 = 64 */
+    uint32_t slots;            /* launches in flight, one stream each; 0 = 2 */
+    uint32_t emit_threads;     /* host threads that emit results; 0 = 8 */
+    uint32_t batch_window_us;  /* with a free slot and fewer than max_batch requests waiting,
+                                  wait this long for more arrivals before launching; 0 = none */
+} golap_server_opts;
+typedef struct {
+    uint64_t submitted, completed, refused, batches;
+    uint64_t peak_admitted_bytes;
+    double mean_batch;         /* requests per launch */
+} golap_server_stats;
+int golap_server_create(golap_ctx *ctx, const golap_server_opts *opts, golap_server **out);
+int golap_server_submit_star(golap_server *s, const golap_star_query *q, uint64_t *ticket);
+int golap_server_submit_scan(golap_server *s, const golap_scan_plan *p, uint64_t *ticket);
+/* Blocks until the request is done; output as golap_star_run / golap_scan_run. The ticket is
+ * released when this returns. latency_ms may be NULL. */
+int golap_server_wait(golap_server *s, uint64_t ticket, golap_star_result *res, int32_t *group_keys,
+                      int64_t *sums, uint64_t *counts, uint64_t max_groups, double *latency_ms);
+int golap_server_get_stats(golap_server *s, golap_server_stats *out);
+/* Finishes the submitted requests (unwaited results are dropped) and stops the loop. */
+int golap_server_destroy(golap_server *s);
 
 /* ------------------------------------------------------------------ synthetic data
  * Seeded column generators of the benchmark workloads (the reference seeds its generator with

@@ -120,3 +120,69 @@ def test_scan_key_out_of_range_is_an_error(golap, ctx, trip):
         ctx.scan_run(dev, preds, keys, vc)
     r = ctx.scan_run(dev, *template(0, 1000, 1700))  # the context still works
     assert r[0].n_groups > 0
+
+
+def scan_mix(rng, n):
+    """Seeded trip requests: template 0..3 over an hour range of log-uniform length."""
+    out = []
+    for i in range(n):
+        t = int(rng.integers(0, 4))
+        span = int(np.exp(rng.uniform(0, np.log(8784))))
+        h0 = int(rng.integers(0, 8784 - span + 1))
+        out.append((t, h0, h0 + span - 1, bool(i % 5 == 0)))
+    return out
+
+
+def test_scan_batch_matches_single_and_oracle(orc, golap, ctx, trip):
+    host, dev = trip
+    mix = scan_mix(np.random.default_rng(21), 16)
+    plans = []
+    for t, h0, h1, full in mix:
+        preds, keys, vc = template(t, h0, h1, full_hours=full)
+        plans.append(dict(cols=dev, preds=preds, keys=keys, value_col=vc))
+    with ctx.scan_batch(plans) as b:
+        b.run()
+        for i, p in enumerate(plans):
+            got = b.result(i)
+            ref = ctx.scan_run(**p)
+            assert all(np.array_equal(x, y) for x, y in zip(got[1:], ref[1:])), i
+            assert (got[0].chunks_run, got[0].comp_bytes, got[0].rows_matched) == \
+                   (ref[0].chunks_run, ref[0].comp_bytes, ref[0].rows_matched), i
+            if i < 3:
+                wk, ws, wc = orc.scan_groupby(list(host), p["preds"], p["keys"], p["value_col"], threads=4)
+                assert np.array_equal(got[1], wk) and np.array_equal(got[2], ws) and np.array_equal(got[3], wc)
+        want = [b.result(i) for i in range(len(plans))]
+        for cfg in [dict(lds_group_slots=0), dict(lds_group_slots=256, pages_per_item=1),
+                    dict(lds_group_slots=4096, max_blocks_per_cu=1, pages_per_item=4)]:
+            say("scan batch config", cfg)
+            b.set_config(**cfg)
+            for rep in range(2):
+                b.run(use_graph=rep == 1)
+                for i in range(len(plans)):
+                    got = b.result(i)
+                    assert all(np.array_equal(x, y) for x, y in zip(got[1:], want[i][1:])), (cfg, rep, i)
+        c, ms = b.tune(reps=2)
+        say("scan batch tuned", c.as_dict(), ms)
+        b.run()
+        for i in range(len(plans)):
+            got = b.result(i)
+            assert all(np.array_equal(x, y) for x, y in zip(got[1:], want[i][1:])), i
+
+
+def test_scan_batch_mixed_column_sets(orc, golap, ctx, trip):
+    """Plans that list different subsets of the table's columns share one launch."""
+    host, dev = trip
+    ts, pas, zone, fare, dist = dev
+    plans = [dict(cols=[ts, zone, fare], preds=[(0, T0, T0 + 86400 * 30)], keys=[(1, 1, 1, 265, 1)], value_col=2),
+             dict(cols=[dist, ts], preds=[(1, T0 + 86400 * 100, T0 + 86400 * 200), (0, 1000, 9000)],
+                  keys=[(1, T0 + 86400 * 100, 86400, 101, 100)], value_col=0),
+             dict(cols=[pas, zone], preds=[(0, 3, 6)], keys=[(0, 1, 1, 6, 1), (1, 1, 1, 265, 1)], value_col=-1)]
+    with ctx.scan_batch(plans) as b:
+        b.run()
+        for i, p in enumerate(plans):
+            got = b.result(i)
+            ref = ctx.scan_run(**p)
+            assert all(np.array_equal(x, y) for x, y in zip(got[1:], ref[1:])), i
+            hosts = [host[dev.index(c)] for c in p["cols"]]
+            wk, ws, wc = orc.scan_groupby(hosts, p["preds"], p["keys"], p["value_col"], threads=4)
+            assert np.array_equal(got[1], wk) and np.array_equal(got[2], ws) and np.array_equal(got[3], wc)
