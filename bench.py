@@ -33,10 +33,15 @@ def log(*a):
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--workload", choices=["q1", "star", "batch"], default="q1",
+    ap.add_argument("--workload", choices=["q1", "star", "batch", "serve"], default="q1",
                     help="q1: fused filter/probe/sum (the headline line); star: star join + group-by")
     ap.add_argument("--queries", type=int, default=64, help="batch workload: queries per batch")
     ap.add_argument("--zone-rows", type=int, default=65536, help="batch workload: rows per zone-map chunk")
+    ap.add_argument("--trip-rows", type=int, default=1_500_000_000, help="serve workload: trip-table rows")
+    ap.add_argument("--requests", type=int, default=1000, help="serve workload: arrivals per run")
+    ap.add_argument("--qps", type=float, default=0.0,
+                    help="serve workload: arrival rate; 0 = 4 x the measured plain one-at-a-time rate")
+    ap.add_argument("--budget-mb", type=int, default=4096, help="serve workload: HBM admission budget")
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--rows", type=int, default=600_000_000)
@@ -336,6 +341,154 @@ def run_batch(args):
     ctx.close()
 
 
+TRIP_HOURS = 8784
+
+
+def trip_request(rng, t0):
+    """One of four request templates over a time range of log-uniform length (1 hour .. 1 year):
+    0 SUM(fare), 1 passengers >= 2 SUM(distance), 2 fare in [1000, 5000] SUM(distance),
+    3 distance >= 5000 COUNT by (passenger_count, zone); the others group by zone x hour of
+    the range (up to 265 x 8,784 = 2,327,760 groups). Columns: 0 ts, 1 passenger, 2 zone,
+    3 fare, 4 distance."""
+    t = int(rng.integers(0, 4))
+    span = int(np.exp(rng.uniform(0, np.log(TRIP_HOURS))))
+    h0 = int(rng.integers(0, TRIP_HOURS - span + 1))
+    lo, hi = t0 + h0 * 3600, t0 + (h0 + span) * 3600 - 1
+    zone, hour = (2, 1, 1, 265, 1), (0, t0 + h0 * 3600, 3600, span, h0)
+    if t == 0:
+        return dict(preds=[(0, lo, hi)], keys=[zone, hour], value_col=3)
+    if t == 1:
+        return dict(preds=[(0, lo, hi), (1, 2, 6)], keys=[zone, hour], value_col=4)
+    if t == 2:
+        return dict(preds=[(0, lo, hi), (3, 1000, 5000)], keys=[hour, zone], value_col=4)
+    return dict(preds=[(0, lo, hi), (4, 5000, 1 << 30)], keys=[(1, 1, 1, 6, 1), zone], value_col=-1)
+
+
+def run_serve(args):
+    """Serving form on the trip table: Poisson arrivals at a fixed rate drawn from four request
+    templates, served by the request loop (HBM-budget admission, continuous batching into
+    persistent fused launches on two streams), against one-query-at-a-time plain launches
+    (golap_scan_run) at the same arrival times. Latency = arrival to results on the host."""
+    import ctypes as C
+    import threading
+    import golap_public_amd as g
+    n = args.trip_rows
+    log(f"generating {n} trip rows")
+    t_setup = time.perf_counter()
+    cols = g.gen_trip(args.seed, n)
+    log(f"encoding ({time.perf_counter()-t_setup:.1f} s)")
+    enc = dict(page_rows=args.page_rows, chunk_rows=args.zone_rows, lz4_min_match=args.min_match)
+    codecs = [g.CODEC_DBP, g.CODEC_LZ4, g.CODEC_FOR, g.CODEC_FOR, g.CODEC_FOR]
+    hosts = []
+    for c, k in zip(cols, codecs):
+        hosts.append(g.HostColumn.encode(c, codec=k, **enc))
+    del cols
+    ctx = g.Context(0)
+    dev = [ctx.upload(h) for h in hosts]
+    comp = sum(h.desc().payload_bytes for h in hosts)
+    del hosts
+    log(f"compressed {comp/1e9:.3f} GB for {n*19/1e9:.3f} GB raw; setup {time.perf_counter()-t_setup:.1f} s")
+    rng = np.random.default_rng(args.seed)
+    reqs = [dict(cols=dev, **trip_request(rng, g.TRIP_T0)) for _ in range(args.requests)]
+    plans = [ctx.scan_plan(**r) for r in reqs]
+    doms = [int(np.prod([k[3] for k in r["keys"]])) for r in reqs]
+    maxd = max(doms)
+    lib = ctx.lib
+
+    def bufs():
+        return (np.empty(maxd * 2, np.int32), np.empty(maxd, np.int64), np.empty(maxd, np.uint64))
+
+    pb = bufs()
+
+    def plain(i):
+        r = g.StarResult()
+        ctx.check(lib.golap_scan_run(ctx.h, C.byref(plans[i]), C.byref(r), g._ptr(pb[0]), g._ptr(pb[1]),
+                                     g._ptr(pb[2]), doms[i]), "scan_run")
+        return r
+
+    for i in range(min(8, len(reqs))):
+        plain(i)
+    # plain one-at-a-time capacity, back to back
+    k = min(len(reqs), 200)
+    t0 = time.perf_counter()
+    for i in range(k):
+        plain(i)
+    plain_rate = k / (time.perf_counter() - t0)
+    qps = args.qps or 4.0 * plain_rate
+    arrivals = np.cumsum(np.random.default_rng(args.seed + 1).exponential(1.0 / qps, len(reqs)))
+    log(f"plain capacity {plain_rate:.1f} q/s; offered load {qps:.1f} q/s, {len(reqs)} requests")
+
+    # served
+    opts = dict(hbm_budget=args.budget_mb << 20, slots=2, max_batch=64, emit_threads=8)
+    lat_srv = [0.0] * len(reqs)
+    res_srv = [None] * len(reqs)
+    with ctx.server(**opts) as srv:
+        tickets = [None] * len(reqs)
+
+        def waiter(w, nw):
+            b = bufs()
+            for i in range(w, len(reqs), nw):
+                while tickets[i] is None:
+                    time.sleep(0.0001)
+                r, lat = g.StarResult(), C.c_double()
+                ctx.check(lib.golap_server_wait(srv.h, tickets[i], C.byref(r), g._ptr(b[0]), g._ptr(b[1]),
+                                                g._ptr(b[2]), doms[i], C.byref(lat)), "server_wait")
+                lat_srv[i] = lat.value + sub_delay[i]
+                res_srv[i] = (r.n_groups, r.total_sum, r.rows_matched)
+
+        sub_delay = [0.0] * len(reqs)
+        nw = 8
+        ws = [threading.Thread(target=waiter, args=(w, nw)) for w in range(nw)]
+        for w in ws:
+            w.start()
+        start = time.perf_counter()
+        for i in range(len(reqs)):
+            now = time.perf_counter() - start
+            if arrivals[i] > now:
+                time.sleep(arrivals[i] - now)
+            t = C.c_uint64()
+            ctx.check(lib.golap_server_submit_scan(srv.h, C.byref(plans[i]), C.byref(t)), "server_submit")
+            sub_delay[i] = max(0.0, (time.perf_counter() - start) - arrivals[i]) * 1e3  # late submits count
+            tickets[i] = t.value
+        for w in ws:
+            w.join()
+        srv_wall = time.perf_counter() - start
+        st = srv.stats()
+    # plain at the same arrivals: one worker, arrival order
+    lat_pl = [0.0] * len(reqs)
+    start = time.perf_counter()
+    for i in range(len(reqs)):
+        now = time.perf_counter() - start
+        if arrivals[i] > now:
+            time.sleep(arrivals[i] - now)
+        r = plain(i)
+        lat_pl[i] = (time.perf_counter() - start - arrivals[i]) * 1e3
+        assert (r.n_groups, r.total_sum, r.rows_matched) == res_srv[i], f"request {i} differs"
+    pl_wall = time.perf_counter() - start
+    srv_qps, pl_qps = len(reqs) / srv_wall, len(reqs) / pl_wall
+    p = lambda a, q: float(np.percentile(np.array(a), q))
+    log(f"served {srv_qps:.1f} q/s p50 {p(lat_srv, 50):.2f} ms p99 {p(lat_srv, 99):.2f} ms; plain {pl_qps:.1f} q/s "
+        f"p50 {p(lat_pl, 50):.2f} ms p99 {p(lat_pl, 99):.2f} ms; stats {st}")
+    gbps = st["comp_bytes"] / (st["device_ms"] * 1e-3) / 1e9 if st["device_ms"] else 0.0
+    out = {"metric": "queries/s + p99 latency (served trip-table requests, Poisson arrivals, 1 GPU)",
+           "value": srv_qps, "unit": "queries/s", "n_gpus": 1, "steps": len(reqs), "warmup": 8,
+           "ms_per_step": srv_wall * 1e3 / len(reqs), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "int64", "data": "synthetic: SplitMix64-seeded trip table, seeded requests",
+           "config": {"workload": "serve_trip_1.5B", "rows": n, "zone_rows": args.zone_rows, "page_rows": args.page_rows,
+                      "codecs": "ts DBP, passenger LZ4, zone/fare/distance FOR", "comp_bytes_table": comp,
+                      "offered_qps": qps, "requests": len(reqs), "served_qps": srv_qps,
+                      "served_p50_ms": p(lat_srv, 50), "served_p99_ms": p(lat_srv, 99),
+                      "plain_qps": pl_qps, "plain_p50_ms": p(lat_pl, 50), "plain_p99_ms": p(lat_pl, 99),
+                      "plain_capacity_qps": plain_rate, "speedup_vs_plain": srv_qps / pl_qps,
+                      "server": st, "server_opts": opts},
+           "roofline": {"bound": "hbm", "achieved": gbps, "peak": 8000.0, "unit": "GB/s", "frac": gbps / 8000.0,
+                        "traffic": None}}
+    print(json.dumps(out), flush=True)
+    for d in dev:
+        ctx.free_column(d)
+    ctx.close()
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -346,6 +499,8 @@ def main():
         return run_star(args)
     if args.workload == "batch":
         return run_batch(args)
+    if args.workload == "serve":
+        return run_serve(args)
     import golap_public_amd as g
 
     t_setup = time.perf_counter()

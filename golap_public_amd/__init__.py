@@ -100,7 +100,8 @@ class ServerOpts(C.Structure):
 
 class ServerStats(C.Structure):
     _fields_ = [("submitted", u64), ("completed", u64), ("refused", u64), ("batches", u64),
-                ("peak_admitted_bytes", u64), ("mean_batch", C.c_double)]
+                ("peak_admitted_bytes", u64), ("mean_batch", C.c_double), ("device_ms", C.c_double),
+                ("comp_bytes", u64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

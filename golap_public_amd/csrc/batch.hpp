@@ -57,6 +57,7 @@ struct golap_batch {
     // fact columns of the launch and their LDS windows
     golap_impl::starhost::ColumnSet cs;
     int windows[golap_impl::STAR_MAX_COLS] = {};
+    int dslots[golap_impl::STAR_MAX_COLS] = {};  // scan: register slots of delta-bitpack columns
     uint32_t n_windows = 0;
     // slices and work list
     uint32_t page_rows = 0, ppc = 0;

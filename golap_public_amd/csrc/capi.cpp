@@ -349,11 +349,14 @@ int golap_column_upload(golap_ctx *ctx, const golap_host_column *h, golap_column
     ColumnExtra ex;
     ex.nondirect_prefix.resize(c->n_pages + 1, 0);
     ex.window_prefix.resize(c->n_pages + 1, 0);
+    ex.dbp_prefix.resize(c->n_pages + 1, 0);
     for (uint64_t i = 0; i < c->n_pages; ++i) {
         const bool direct = page_direct_host(c->page_codec[i]);
         ex.nondirect_prefix[i + 1] = ex.nondirect_prefix[i] + (direct ? 0 : 1);
         const bool point = (c->page_codec[i] & golap_fmt::PAGE_POINT) != 0;
         ex.window_prefix[i + 1] = ex.window_prefix[i] + (direct || point ? 0 : 1);
+        const bool dbp = (c->page_codec[i] & golap_fmt::PAGE_CODEC_MASK) == golap_fmt::DBP && !point;
+        ex.dbp_prefix[i + 1] = ex.dbp_prefix[i] + (dbp ? 1 : 0);
     }
     ex.chunk_payload.resize(c->n_chunks + 1, 0);
     const uint64_t ppc = c->chunk_rows / c->page_rows;

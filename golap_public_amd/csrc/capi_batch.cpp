@@ -155,7 +155,7 @@ hipError_t launch_kernel(golap_batch *b, hipStream_t s, bool prepare_only) {
     ScanLaunch L{};
     L.n_cols = (uint32_t)b->cs.cols.size();
     for (uint32_t i = 0; i < L.n_cols; ++i)
-        L.col[i] = ScanColView{view(b->cs.cols[i]), b->cs.cols[i]->type, b->windows[i]};
+        L.col[i] = ScanColView{view(b->cs.cols[i]), b->cs.cols[i]->type, b->windows[i], b->dslots[i], 0};
     L.page_rows = b->page_rows;
     L.pages_per_chunk = b->ppc;
     L.n_rows = b->n_rows;
@@ -471,18 +471,15 @@ int batch_create_scan(golap_ctx *ctx, const golap_scan_plan *q, uint32_t n, Batc
         pb = std::min(pb, d.s.page_begin);
         pe = std::max(pe, d.s.page_end);
     }
-    // LDS windows of the used columns that have some coded page in the union of the slices
+    // register slots and LDS windows of the used columns over the union of the slices
     {
-        std::lock_guard<std::mutex> g(g_extra_mu);
-        pb = std::min(pb, pe);
+        int32_t woff[SCAN_MAX_COLS], dslot[SCAN_MAX_COLS];
+        b->win_bytes = scan_layout(b->cs.cols.data(), (uint32_t)b->cs.cols.size(), used, std::min(pb, pe), pe, woff,
+                                   dslot);
         for (size_t c = 0; c < b->cs.cols.size(); ++c) {
-            b->windows[c] = -1;
-            if (!((used >> c) & 1u)) continue;
-            auto &ex = g_extra[b->cs.cols[c]];
-            if (ex.window_prefix[pe] == ex.window_prefix[pb]) continue;
-            b->windows[c] = (int)b->win_bytes;
-            b->win_bytes += (b->cs.cols[c]->page_rows * b->cs.cols[c]->elem + 15) & ~15u;
-            ++b->n_windows;
+            b->windows[c] = woff[c];
+            b->dslots[c] = dslot[c];
+            b->n_windows += woff[c] >= 0;
         }
     }
     if (int r = finish_create(ctx, b, with_graph)) return bail(r);

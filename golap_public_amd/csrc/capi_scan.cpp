@@ -114,18 +114,28 @@ void scan_stats(const golap_scan_plan &p, uint32_t colmask, const std::vector<ui
     }
 }
 
-// LDS windows of the columns a set of queries reads over pages [pb, pe): byte offsets (16-byte
-// aligned, page_rows * elem each); returns the total.
-uint32_t scan_windows(const golap_scan_plan &p, uint32_t colmask, uint64_t pb, uint64_t pe, int32_t *woff) {
+// Register slots and LDS windows of the columns (cols[c], read when bit c of colmask is set)
+// over pages [pb, pe): the first SCAN_DBP_SLOTS columns with delta-bitpack pages decode them in
+// registers; a column gets a window (16-byte aligned, page_rows * elem bytes) when it has other
+// coded pages, or delta pages without a slot. Returns the window bytes.
+uint32_t scan_layout(const golap_column *const *cols, uint32_t n_cols, uint32_t colmask, uint64_t pb, uint64_t pe,
+                     int32_t *woff, int32_t *dslot) {
     std::lock_guard<std::mutex> g(g_extra_mu);
-    uint32_t bytes = 0;
-    for (uint32_t c = 0; c < p.n_cols; ++c) {
+    uint32_t bytes = 0, slots = 0;
+    for (uint32_t c = 0; c < n_cols; ++c) {
         woff[c] = -1;
+        dslot[c] = -1;
         if (!((colmask >> c) & 1u)) continue;
-        auto &ex = g_extra[p.cols[c]];
-        if (ex.window_prefix[pe] == ex.window_prefix[pb]) continue;
+        auto &ex = g_extra[cols[c]];
+        const uint32_t dbp = ex.dbp_prefix[pe] - ex.dbp_prefix[pb];
+        uint32_t other = ex.window_prefix[pe] - ex.window_prefix[pb] - dbp;
+        if (dbp) {
+            if (slots < SCAN_DBP_SLOTS) dslot[c] = (int32_t)slots++;
+            else other += dbp;
+        }
+        if (!other) continue;
         woff[c] = (int32_t)bytes;
-        bytes += (p.cols[c]->page_rows * p.cols[c]->elem + 15) & ~15u;
+        bytes += (cols[c]->page_rows * cols[c]->elem + 15) & ~15u;
     }
     return bytes;
 }
@@ -158,9 +168,10 @@ int golap_scan_run(golap_ctx *ctx, const golap_scan_plan *plan, golap_star_resul
     const golap_column *f = plan->cols[0];
     ScanLaunch L{};
     L.n_cols = plan->n_cols;
-    int32_t woff[SCAN_MAX_COLS];
-    L.win_bytes = scan_windows(*plan, qd.colmask, qd.s.page_begin, qd.s.page_end, woff);
-    for (uint32_t c = 0; c < plan->n_cols; ++c) L.col[c] = ScanColView{view(plan->cols[c]), plan->cols[c]->type, woff[c]};
+    int32_t woff[SCAN_MAX_COLS], dslot[SCAN_MAX_COLS];
+    L.win_bytes = scan_layout(plan->cols, plan->n_cols, qd.colmask, qd.s.page_begin, qd.s.page_end, woff, dslot);
+    for (uint32_t c = 0; c < plan->n_cols; ++c)
+        L.col[c] = ScanColView{view(plan->cols[c]), plan->cols[c]->type, woff[c], dslot[c], 0};
     L.page_rows = f->page_rows;
     L.pages_per_chunk = (uint32_t)(f->chunk_rows / f->page_rows);
     L.n_rows = f->n_rows;

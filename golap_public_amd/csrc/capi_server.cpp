@@ -216,6 +216,11 @@ void completion_loop(golap_server *s, Slot *slot) {
         golap_batch_run_info info{};
         int r = batch_finish(b, &info);
         std::string err = r ? b->err : std::string();
+        if (!r) {
+            std::lock_guard<std::mutex> lk(s->m);
+            s->stats.device_ms += info.ms;
+            s->stats.comp_bytes += info.comp_bytes;
+        }
         if (!r && !reqs.empty()) {  // the whole arena back in one copy, before the parallel emission
             golap_star_result tmp{};
             r = batch_result(b, 0, &tmp, nullptr, nullptr, nullptr, 0);

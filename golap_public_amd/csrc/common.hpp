@@ -102,6 +102,7 @@ inline int fail(golap_ctx *ctx, int code, const std::string &msg) {
 struct ColumnExtra {
     std::vector<uint32_t> nondirect_prefix;  // pages that cannot be read in place, prefix count
     std::vector<uint32_t> window_prefix;     // pages neither in place nor point-readable
+    std::vector<uint32_t> dbp_prefix;        // delta-bitpack pages that are not point-readable
     std::vector<uint64_t> chunk_payload;     // payload bytes per chunk, prefix
 };
 extern std::mutex g_extra_mu;
@@ -226,10 +227,15 @@ hipError_t launch_hash_filter(golap_ctx *ctx, const uint64_t *slots, uint32_t lo
 // ---- scan + group-by over typed columns (trip-table plans)
 constexpr uint32_t SCAN_MAX_COLS = 8, SCAN_MAX_PREDS = 4, SCAN_MAX_KEYS = 3;
 
+// Delta-bitpack pages of up to SCAN_DBP_SLOTS columns of a launch are decoded in registers (a
+// running sum per thread); other coded pages go through an LDS window.
+constexpr uint32_t SCAN_DBP_SLOTS = 2;
 struct ScanColView {
     ColView c;
     uint32_t type;
-    int32_t woff;  // byte offset of the column's LDS window, -1 = none
+    int32_t woff;   // byte offset of the column's LDS window, -1 = none
+    int32_t dslot;  // register slot for its delta-bitpack pages, -1 = none
+    uint32_t pad;
 };
 struct ScanPredDev {
     uint32_t col, pad;

@@ -62,7 +62,8 @@ bool is_int_type(uint32_t t);
 int prepare_scan(golap_ctx *ctx, const golap_scan_plan *p, ScanQueryDev &qd, ScanHost &h);
 std::vector<uint8_t> scan_survivors(const golap_scan_plan &p, const std::vector<uint16_t> &mask);
 void scan_stats(const golap_scan_plan &p, uint32_t colmask, const std::vector<uint8_t> &run, golap_star_result *res);
-uint32_t scan_windows(const golap_scan_plan &p, uint32_t colmask, uint64_t pb, uint64_t pe, int32_t *woff);
+uint32_t scan_layout(const golap_column *const *cols, uint32_t n_cols, uint32_t colmask, uint64_t pb, uint64_t pe,
+                     int32_t *woff, int32_t *dslot);
 uint32_t scan_slots(golap_ctx *ctx, uint32_t agg_mode, uint32_t requested, uint64_t domain, uint32_t win_bytes);
 
 }  // namespace starhost

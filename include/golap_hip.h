@@ -399,6 +399,8 @@ typedef struct {
     uint64_t submitted, completed, refused, batches;
     uint64_t peak_admitted_bytes;
     double mean_batch;         /* requests per launch */
+    double device_ms;          /* device time of all launches (events around each batch) */
+    uint64_t comp_bytes;       /* compressed bytes of the surviving pages read, all batches */
 } golap_server_stats;
 int golap_server_create(golap_ctx *ctx, const golap_server_opts *opts, golap_server **out);
 int golap_server_submit_star(golap_server *s, const golap_star_query *q, uint64_t *ticket);
