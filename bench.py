@@ -181,6 +181,27 @@ def run_star(args):
     ctx.close()
 
 
+def calibrate(ctx, args):
+    """Three device copies of --calibrate-copy bytes in the profiled process: rocprofv3's
+    FETCH_SIZE / WRITE_SIZE of that copy give the factors that turn the counters into bytes."""
+    if args.calibrate_copy:
+        a_, b_ = ctx.alloc(args.calibrate_copy), ctx.alloc(args.calibrate_copy)
+        ctx.memset(a_, 1, args.calibrate_copy)
+        for _ in range(3):
+            ctx.copy(b_, a_, args.calibrate_copy)
+        ctx.free(a_)
+        ctx.free(b_)
+        log(f"calibration: 3 copies of {args.calibrate_copy} bytes")
+
+
+def traffic_of(name):
+    """HBM bytes per launch of a workload's dominant kernel from its committed rocprof summary."""
+    f = os.path.join(ROOT, "profiles", name + "_traffic.json")
+    if os.path.exists(f):
+        return json.load(open(f))["hbm_bytes_per_launch"]
+    return None
+
+
 def star_setup(args, chunk_rows):
     """The star schema of the star workloads: 600M fact rows (date_key sorted, bit-packed delta;
     customer, supplier, part keys uniform and revenue, LZ4), dimensions of 2,556 / 2,000,000 /
@@ -315,7 +336,8 @@ def run_batch(args):
                       "speedup_vs_plain": pwall / wall, "map_build_ms_all_queries": build_ms,
                       "tuned_config": best.as_dict(), "default_config": default_cfg, "tune_s": tune_s},
            "roofline": {"bound": "hbm", "achieved": gbps, "peak": 8000.0, "unit": "GB/s", "frac": gbps / 8000.0,
-                        "traffic": None}}
+                        "traffic": traffic_of("star_batch_64q_600M")}}
+    calibrate(ctx, args)
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import oracle
@@ -482,7 +504,8 @@ def run_serve(args):
                       "plain_capacity_qps": plain_rate, "speedup_vs_plain": srv_qps / pl_qps,
                       "server": st, "server_opts": opts},
            "roofline": {"bound": "hbm", "achieved": gbps, "peak": 8000.0, "unit": "GB/s", "frac": gbps / 8000.0,
-                        "traffic": None}}
+                        "traffic": traffic_of("serve_trip_1.5B")}}
+    calibrate(ctx, args)
     print(json.dumps(out), flush=True)
     for d in dev:
         ctx.free_column(d)
@@ -564,14 +587,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
     }
-    if args.calibrate_copy:
-        a_, b_ = ctx.alloc(args.calibrate_copy), ctx.alloc(args.calibrate_copy)
-        ctx.memset(a_, 1, args.calibrate_copy)
-        for _ in range(3):
-            ctx.copy(b_, a_, args.calibrate_copy)
-        ctx.free(a_)
-        ctx.free(b_)
-        log(f"calibration: 3 copies of {args.calibrate_copy} bytes")
+    calibrate(ctx, args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args, hosts, key, year, r.sum)
     print(json.dumps(out), flush=True)

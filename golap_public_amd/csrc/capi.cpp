@@ -127,6 +127,7 @@ int golap_ctx_destroy(golap_ctx *ctx) {
         (void)hipStreamSynchronize(ctx->stream);
         for (auto &kv : ctx->allocs) (void)hipFree(kv.first);
         ctx->allocs.clear();
+        if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
         (void)hipEventDestroy(ctx->ev0);
         (void)hipEventDestroy(ctx->ev1);
         (void)hipStreamDestroy(ctx->stream);

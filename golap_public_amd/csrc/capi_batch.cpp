@@ -740,3 +740,43 @@ int golap_batch_free(golap_batch *b) {
 }
 
 }  // extern "C"
+
+namespace golap_impl {
+
+int solo_worklist(golap_ctx *ctx, const void *desc, uint32_t desc_size, uint32_t pages_per_chunk, const void **d_desc,
+                  uint32_t **queue, uint2 **items) {
+    if (desc_size > 1024) return fail(ctx, GOLAP_EINVAL, "solo work list: descriptor too large");
+    const SliceDev &s = *static_cast<const SliceDev *>(desc);
+    const uint64_t nch = s.chunk_end - s.chunk_begin;
+    if (nch > ctx->solo_items_cap) {
+        dev_release(ctx, ctx->d_solo_items);
+        ctx->d_solo_items = nullptr;
+        ctx->solo_items_cap = 0;
+        if (int r = dev_alloc(ctx, std::max<uint64_t>(nch, 1024) * sizeof(uint2), &ctx->d_solo_items)) return r;
+        ctx->solo_items_cap = std::max<uint64_t>(nch, 1024);
+    }
+    uint8_t *dd = ctx->d_scratch + 1024;
+    uint64_t *off = reinterpret_cast<uint64_t *>(ctx->d_scratch + 2048);
+    uint32_t *q = reinterpret_cast<uint32_t *>(ctx->d_scratch + 2304);
+    if (!ctx->h_stage && hipHostMalloc(reinterpret_cast<void **>(&ctx->h_stage), 4096, hipHostMallocDefault) != hipSuccess) {
+        ctx->h_stage = nullptr;
+        return fail(ctx, GOLAP_ENOMEM, "solo work list: pinned staging");
+    }
+    // the staging page is reused by the next call only after this call's final synchronisation
+    GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    memcpy(ctx->h_stage, desc, desc_size);
+    const uint64_t host_off[2] = {0, nch};
+    memcpy(ctx->h_stage + 1024, host_off, sizeof(host_off));
+    GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(dd, ctx->h_stage, desc_size, hipMemcpyHostToDevice, ctx->stream));
+    GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(off, ctx->h_stage + 1024, sizeof(host_off), hipMemcpyHostToDevice, ctx->stream));
+    GOLAP_HIP_CHECK(ctx, launch_fill_zero(q, 8, ctx->stream));
+    uint2 *it = static_cast<uint2 *>(ctx->d_solo_items);
+    GOLAP_HIP_CHECK(ctx, launch_batch_items(dd, desc_size, 1, off, nch, 1, it, q, ctx->stream));
+    (void)pages_per_chunk;
+    *d_desc = dd;
+    *queue = q;
+    *items = it;
+    return GOLAP_OK;
+}
+
+}  // namespace golap_impl

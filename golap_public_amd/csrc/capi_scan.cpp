@@ -189,9 +189,20 @@ int golap_scan_run(golap_ctx *ctx, const golap_scan_plan *plan, golap_star_resul
     if (int r = group_buffers(ctx, h.domain, &d_sum, &d_cnt)) return r;
     qd.g_sum = d_sum;
     qd.g_cnt = d_cnt;
-    L.q0 = qd;
     L.err = reinterpret_cast<uint32_t *>(ctx->d_scratch);
     GOLAP_HIP_CHECK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    // with pruning, the surviving chunks may sit anywhere in the slice: a device-built work list
+    // keeps every workgroup busy (the static page ranges would leave most of them idle)
+    const bool pruned = plan->dev_chunk_mask || (plan->zone_prune && plan->n_preds);
+    if (pruned && L.n_items) {
+        const void *dq;
+        if (int r = solo_worklist(ctx, &qd, sizeof(qd), L.pages_per_chunk, &dq, &L.queue, const_cast<uint2 **>(&L.items)))
+            return r;
+        L.queries = static_cast<const ScanQueryDev *>(dq);
+        L.pages_per_item = L.pages_per_chunk;
+    } else {
+        L.q0 = qd;
+    }
     if (L.n_items) GOLAP_HIP_CHECK(ctx, launch_scan(ctx, L, ctx->stream));
     GOLAP_HIP_CHECK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     uint32_t flags = 0;

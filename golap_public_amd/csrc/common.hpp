@@ -24,7 +24,9 @@ struct golap_ctx {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::recursive_mutex mu;
     std::string err;
-    // Device scratch: word 0 error flags; [8,16) int64 sum; [16,24) u64 count; rest free.
+    // Device scratch: word 0 error flags; [8,16) int64 sum; [16,24) u64 count; [64,80) hash
+    // build; [1024,2048) a single query's descriptor, [2048,2064) its chunk offsets, [2304,2312)
+    // its work-list counters (pruned single-query plans).
     uint8_t *d_scratch = nullptr;
     static constexpr uint64_t SCRATCH_BYTES = 4096;
     uint64_t allocated = 0;
@@ -34,6 +36,10 @@ struct golap_ctx {
     uint64_t groups_cap = 0;
     // autotuned batch launch configurations by batch shape
     std::unordered_map<std::string, golap_batch_config> tune_cache;
+    // work list of pruned single-query plans, grown on demand
+    void *d_solo_items = nullptr;
+    uint64_t solo_items_cap = 0;
+    uint8_t *h_stage = nullptr;  // pinned host staging for small uploads (4 KiB)
 };
 
 struct golap_column {
@@ -220,6 +226,13 @@ hipError_t launch_batch_items(const void *qbase, uint32_t qstride, uint32_t n_qu
                               uint64_t total_chunks, uint32_t parts, uint2 *items, uint32_t *queue,
                               hipStream_t stream);
 hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t stream);
+
+// Work list of one pruned query (a batch of one): its descriptor (desc_size <= 1024 bytes,
+// beginning with its SliceDev) goes to the context scratch, the surviving chunks' items to a
+// context buffer. On return *d_desc, *queue and *items are the device pointers for the queue
+// schedule with pages_per_item = pages per chunk.
+int solo_worklist(golap_ctx *ctx, const void *desc, uint32_t desc_size, uint32_t pages_per_chunk, const void **d_desc,
+                  uint32_t **queue, uint2 **items);
 size_t star_lds_bytes(uint32_t filter_words, uint32_t page_rows, uint32_t n_windows, uint32_t lds_gslots);
 hipError_t launch_hash_filter(golap_ctx *ctx, const uint64_t *slots, uint32_t log2cap, uint32_t *filter,
                               uint32_t words_log2);
