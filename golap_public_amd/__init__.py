@@ -101,7 +101,7 @@ class ServerOpts(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [("submitted", u64), ("completed", u64), ("refused", u64), ("batches", u64),
                 ("peak_admitted_bytes", u64), ("mean_batch", C.c_double), ("device_ms", C.c_double),
-                ("comp_bytes", u64)]
+                ("comp_bytes", u64), ("shared_comp_bytes", u64)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
@@ -116,7 +116,8 @@ class BatchConfig(C.Structure):
 
 
 class BatchRunInfo(C.Structure):
-    _fields_ = [("ms", C.c_double), ("items", u64), ("pages", u64), ("comp_bytes", u64)]
+    _fields_ = [("ms", C.c_double), ("items", u64), ("pages", u64), ("comp_bytes", u64),
+                ("shared_comp_bytes", u64)]
 
 
 class Q1Result(C.Structure):

@@ -25,14 +25,15 @@ struct BatchSlot {
     hipEvent_t ev0 = nullptr, ev1 = nullptr, dep = nullptr;
     uint8_t *d_arena = nullptr, *h_arena = nullptr, *d_q = nullptr;
     uint64_t *d_chunk_off = nullptr;
-    uint2 *d_items = nullptr;
-    uint64_t arena_cap = 0, q_cap = 0, off_cap = 0, items_cap = 0;
+    uint2 *d_items = nullptr;   // star work list
+    uint4 *d_items4 = nullptr;  // scan work list
+    uint64_t arena_cap = 0, q_cap = 0, off_cap = 0, items_cap = 0, items4_cap = 0;
     std::string err;
 
     // own_stream: a new stream (serving slots); otherwise the context stream.
     int init(golap_ctx *c, bool own);
-    // Grows the buffers to at least these sizes (bytes, bytes, entries, entries).
-    int reserve(uint64_t arena, uint64_t qbytes, uint64_t offs, uint64_t items);
+    // Grows the buffers to at least these sizes (bytes, bytes, entries, entries, entries).
+    int reserve(uint64_t arena, uint64_t qbytes, uint64_t offs, uint64_t items, uint64_t items4);
     void release();
 };
 
@@ -66,6 +67,8 @@ struct golap_batch {
     std::vector<std::vector<uint8_t>> survive;  // host view of surviving chunks per query
     std::vector<uint64_t> surv_pages;
     uint64_t total_pages = 0, total_bytes = 0;
+    // scan batches: union of the slices, surviving (chunk, plan word) pairs, bytes read once
+    uint64_t union_begin = 0, union_end = 0, union_pairs = 0, shared_bytes = 0;
     uint64_t max_domain = 1;
     std::vector<uint64_t> res_off;
     uint64_t arena_bytes = 0;

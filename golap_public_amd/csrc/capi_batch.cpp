@@ -75,9 +75,10 @@ size_t lds_bytes(const golap_batch *b, const golap_batch_config &c) {
 int check_config(golap_batch *b, const golap_batch_config &c) {
     if (c.pages_per_item < 1 || c.pages_per_item > b->ppc || parts_of(b, c.pages_per_item) > 256)
         return bfail(b, GOLAP_EINVAL, "batch: pages_per_item out of range");
-    if (c.lds_group_slots && (c.lds_group_slots & (c.lds_group_slots - 1) || c.lds_group_slots > LDS_GROUP_SLOTS_MAX))
-        return bfail(b, GOLAP_EINVAL, "batch: lds_group_slots must be 0 or a power of two <= 4096");
-    if (c.lds_group_slots && !lds_keys_fit(b))
+    const uint32_t max_slots = b->kind == BatchKind::STAR ? LDS_GROUP_SLOTS_MAX : SCAN_AGG_MAX;
+    if (c.lds_group_slots && (c.lds_group_slots & (c.lds_group_slots - 1) || c.lds_group_slots > max_slots))
+        return bfail(b, GOLAP_EINVAL, "batch: lds_group_slots must be 0 or a power of two <= 4096 (star) / 8192 (scan)");
+    if (c.lds_group_slots && b->kind == BatchKind::STAR && !lds_keys_fit(b))
         return bfail(b, GOLAP_EINVAL, "batch: too many queries x groups for LDS group keys");
     if (c.filter_lds > 1 || c.compact_tables > 1) return bfail(b, GOLAP_EINVAL, "batch: bad flag");
     if (lds_bytes(b, c) > (size_t)b->ctx->lds_max) return bfail(b, GOLAP_EINVAL, "batch: LDS budget exceeded");
@@ -108,12 +109,9 @@ int upload_descriptors(golap_batch *b, const golap_batch_config &c) {
             d.g_sum = gs;
             d.g_cnt = gc;
         } else {
+            // dense LDS partials per (plan, item) whenever the plan's local domain fits
             ScanQueryDev &d = b->sd[i];
-            const uint32_t am = b->sh[i].plan.agg_mode;
-            const double expect = (double)b->surv_pages[i] * b->page_rows;
-            const bool lds = am == GOLAP_AGG_LDS ||
-                             (am == GOLAP_AGG_AUTO && expect >= (double)c.lds_cutover * (double)dom);
-            d.use_lds = lds && c.lds_group_slots;
+            d.use_lds = b->sh[i].plan.agg_mode != GOLAP_AGG_GLOBAL && c.lds_group_slots;
             d.g_sum = gs;
             d.g_cnt = gc;
         }
@@ -161,14 +159,12 @@ hipError_t launch_kernel(golap_batch *b, hipStream_t s, bool prepare_only) {
     L.n_rows = b->n_rows;
     L.n_pages = b->n_pages;
     L.win_bytes = b->win_bytes;
+    L.agg_entries = c.lds_group_slots;
     L.queries = reinterpret_cast<const ScanQueryDev *>(sl->d_q);
     L.n_queries = b->n_queries();
-    L.items = sl->d_items;
+    L.items = sl->d_items4;
     L.queue = reinterpret_cast<uint32_t *>(sl->d_arena);
     L.pages_per_item = c.pages_per_item;
-    L.lds_gslots = c.lds_group_slots;
-    L.group_bits = group_bits_of(b);
-    L.flush_fill = c.lds_group_slots / 2;
     L.max_blocks_per_cu = c.max_blocks_per_cu;
     L.err = reinterpret_cast<uint32_t *>(sl->d_arena + 8);
     return launch_scan(b->ctx, L, s, prepare_only);
@@ -178,9 +174,15 @@ hipError_t launch_kernel(golap_batch *b, hipStream_t s, bool prepare_only) {
 int enqueue_kernels(golap_batch *b, hipStream_t s) {
     BatchSlot *sl = b->slot;
     BATCH_HIP_CHECK(b, launch_fill_zero(sl->d_arena, b->arena_bytes, s));
-    BATCH_HIP_CHECK(b, launch_batch_items(sl->d_q, desc_size(b), b->n_queries(), sl->d_chunk_off, b->chunk_off.back(),
-                                          parts_of(b, b->cfg.pages_per_item), sl->d_items,
-                                          reinterpret_cast<uint32_t *>(sl->d_arena), s));
+    if (b->kind == BatchKind::STAR)
+        BATCH_HIP_CHECK(b, launch_batch_items(sl->d_q, desc_size(b), b->n_queries(), sl->d_chunk_off,
+                                              b->chunk_off.back(), parts_of(b, b->cfg.pages_per_item), sl->d_items,
+                                              reinterpret_cast<uint32_t *>(sl->d_arena), s));
+    else
+        BATCH_HIP_CHECK(b, launch_scan_items(reinterpret_cast<const ScanQueryDev *>(sl->d_q), b->n_queries(),
+                                             b->union_begin, b->union_end - b->union_begin,
+                                             parts_of(b, b->cfg.pages_per_item), sl->d_items4,
+                                             reinterpret_cast<uint32_t *>(sl->d_arena), s));
     BATCH_HIP_CHECK(b, launch_kernel(b, s, false));
     return GOLAP_OK;
 }
@@ -223,10 +225,11 @@ golap_batch_config default_config(golap_batch *b) {
         c.lds_group_slots = lds_keys_fit(b) ? LDS_GROUP_SLOTS_MAX : 0;
         while (c.lds_group_slots && lds_bytes(b, c) > (size_t)b->ctx->lds_max) c.lds_group_slots >>= 1;
     } else {
-        // two workgroups per CU when the windows allow
-        c.lds_group_slots = lds_keys_fit(b) ? 2048 : 0;
+        // dense partials for several plans per item; items small enough to fill the device
+        c.lds_group_slots = 4096;
         while (c.lds_group_slots && lds_bytes(b, c) > std::max<size_t>(b->ctx->lds_max / 2, b->win_bytes + 1024))
             c.lds_group_slots >>= 1;
+        c.pages_per_item = scan_pages_per_item(b->ctx, b->union_pairs, b->ppc);
     }
     return c;
 }
@@ -289,6 +292,42 @@ int finish_create(golap_ctx *ctx, golap_batch *b, bool with_graph) {
         b->total_pages += b->surv_pages[i];
         b->total_bytes += tmp.comp_bytes;
     }
+    if (b->kind == BatchKind::SCAN) {
+        // shared scan: the union of the slices, the surviving (chunk, plan word) pairs and the
+        // bytes read once per chunk for all plans that keep it
+        b->union_begin = ~0ull;
+        b->union_end = 0;
+        for (uint32_t i = 0; i < nq; ++i) {
+            const SliceDev &d = slice_of(b, i);
+            if (d.chunk_end <= d.chunk_begin) continue;
+            b->union_begin = std::min(b->union_begin, d.chunk_begin);
+            b->union_end = std::max(b->union_end, d.chunk_end);
+        }
+        if (b->union_begin > b->union_end) b->union_begin = b->union_end = 0;
+        const uint64_t un = b->union_end - b->union_begin;
+        const uint32_t nw = (nq + 63) / 64;
+        std::vector<uint32_t> cmask(un * nw, 0);  // columns read per (chunk, word); bit 31: kept
+        for (uint32_t i = 0; i < nq; ++i) {
+            const SliceDev &d = slice_of(b, i);
+            for (uint64_t k = 0; k < d.chunk_end - d.chunk_begin; ++k)
+                if (b->survive[i][k]) cmask[(d.chunk_begin + k - b->union_begin) * nw + i / 64] |= b->sd[i].colmask | (1u << 31);
+        }
+        b->union_pairs = 0;
+        b->shared_bytes = 0;
+        std::lock_guard<std::mutex> g(g_extra_mu);
+        for (uint64_t k = 0; k < un; ++k)
+            for (uint32_t w = 0; w < nw; ++w) {
+                const uint32_t m = cmask[k * nw + w];
+                if (!m) continue;
+                ++b->union_pairs;
+                for (size_t c = 0; c < b->cs.cols.size(); ++c)
+                    if ((m >> c) & 1u) {
+                        const auto &ex = g_extra[b->cs.cols[c]];
+                        const uint64_t ch = b->union_begin + k;
+                        b->shared_bytes += ex.chunk_payload[ch + 1] - ex.chunk_payload[ch];
+                    }
+            }
+    }
     uint64_t off = ARENA_HEADER;
     b->res_off.resize(nq);
     for (uint32_t i = 0; i < nq; ++i) {
@@ -298,7 +337,10 @@ int finish_create(golap_ctx *ctx, golap_batch *b, bool with_graph) {
     b->arena_bytes = off;
     const uint64_t items = std::max<uint64_t>(1, b->chunk_off.back() * b->ppc);
     if (items >= (1ull << 32)) return fail(ctx, GOLAP_EINVAL, "batch: too many work items");
-    if (int r = b->slot->reserve(off, (uint64_t)nq * desc_size(b), b->chunk_off.size(), items))
+    const uint64_t items4 = b->kind == BatchKind::SCAN ? std::max<uint64_t>(1, b->union_pairs * b->ppc) : 0;
+    if (items4 >= (1ull << 32)) return fail(ctx, GOLAP_EINVAL, "batch: too many work items");
+    if (int r = b->slot->reserve(off, (uint64_t)nq * desc_size(b), b->chunk_off.size(), b->kind == BatchKind::STAR ? items : 1,
+                                 items4))
         return fail(ctx, r, b->slot->err);
     GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(b->slot->d_chunk_off, b->chunk_off.data(), b->chunk_off.size() * 8,
                                         hipMemcpyHostToDevice, b->slot->stream));
@@ -348,7 +390,7 @@ int BatchSlot::init(golap_ctx *c, bool own) {
     return GOLAP_OK;
 }
 
-int BatchSlot::reserve(uint64_t arena, uint64_t qbytes, uint64_t offs, uint64_t items) {
+int BatchSlot::reserve(uint64_t arena, uint64_t qbytes, uint64_t offs, uint64_t items, uint64_t items4) {
     auto grow = [&](auto *&p, uint64_t &cap, uint64_t need, uint64_t unit) -> int {
         if (need <= cap) return GOLAP_OK;
         const uint64_t want = std::max(need, cap * 2);
@@ -364,6 +406,7 @@ int BatchSlot::reserve(uint64_t arena, uint64_t qbytes, uint64_t offs, uint64_t 
         cap = want;
         return GOLAP_OK;
     };
+    int r4 = GOLAP_OK;
     const bool arena_grows = arena > arena_cap;
     if (int r = grow(d_arena, arena_cap, arena, 1)) return r;
     if (arena_grows && h_arena) {
@@ -378,6 +421,7 @@ int BatchSlot::reserve(uint64_t arena, uint64_t qbytes, uint64_t offs, uint64_t 
     if (int r = grow(d_q, q_cap, qbytes, 1)) return r;
     if (int r = grow(d_chunk_off, off_cap, offs, 8)) return r;
     if (int r = grow(d_items, items_cap, items, sizeof(uint2))) return r;
+    if (items4 && (r4 = grow(d_items4, items4_cap, items4, sizeof(uint4)))) return r4;
     return GOLAP_OK;
 }
 
@@ -386,6 +430,7 @@ void BatchSlot::release() {
     dev_release(ctx, d_q);
     dev_release(ctx, d_chunk_off);
     dev_release(ctx, d_items);
+    dev_release(ctx, d_items4);
     if (h_arena) (void)hipHostFree(h_arena);
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
@@ -394,9 +439,10 @@ void BatchSlot::release() {
     d_arena = h_arena = d_q = nullptr;
     d_chunk_off = nullptr;
     d_items = nullptr;
+    d_items4 = nullptr;
     ev0 = ev1 = dep = nullptr;
     stream = nullptr;
-    arena_cap = q_cap = off_cap = items_cap = 0;
+    arena_cap = q_cap = off_cap = items_cap = items4_cap = 0;
 }
 
 uint64_t batch_query_bytes(uint64_t domain, uint64_t slice_chunks, uint32_t pages_per_chunk, bool star) {
@@ -524,6 +570,7 @@ int batch_finish(golap_batch *b, golap_batch_run_info *info) {
         info->items = hdr[1];
         info->pages = b->total_pages;
         info->comp_bytes = b->total_bytes;
+        info->shared_comp_bytes = b->kind == BatchKind::SCAN ? b->shared_bytes : b->total_bytes;
     }
     return GOLAP_OK;
 }
@@ -657,9 +704,10 @@ int golap_batch_tune(golap_batch *b, uint32_t reps, golap_batch_config *best, do
     std::vector<std::pair<Axis, std::vector<golap_batch_config>>> axes;
     {
         std::vector<golap_batch_config> ax;
-        for (uint32_t slots : {0u, 1024u, 2048u, 4096u})
+        for (uint32_t slots : {0u, 1024u, 2048u, 4096u, 8192u})
             for (uint32_t cut : {0u, 4u, 16u, 256u}) {
-                if (!slots && cut) continue;
+                if ((!slots || !star) && cut) continue;  // scan plans have no cutover
+                if (star && slots > LDS_GROUP_SLOTS_MAX) continue;
                 golap_batch_config c{};
                 c.lds_group_slots = slots;
                 c.lds_cutover = cut;

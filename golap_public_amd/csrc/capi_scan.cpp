@@ -38,7 +38,8 @@ int prepare_scan(golap_ctx *ctx, const golap_scan_plan *p, ScanQueryDev &qd, Sca
     qd.n_preds = p->n_preds;
     for (uint32_t i = 0; i < p->n_preds; ++i) {
         if (p->preds[i].col >= p->n_cols) return fail(ctx, GOLAP_EINVAL, "scan: bad predicate column");
-        qd.pred[i] = ScanPredDev{p->preds[i].col, 0, p->preds[i].lo, p->preds[i].hi};
+        const golap_column *pc = p->cols[p->preds[i].col];
+        qd.pred[i] = ScanPredDev{p->preds[i].col, 0, p->preds[i].lo, p->preds[i].hi, pc->d_chunk_min, pc->d_chunk_max};
         qd.colmask |= 1u << p->preds[i].col;
     }
     qd.n_keys = p->n_keys;
@@ -55,6 +56,8 @@ int prepare_scan(golap_ctx *ctx, const golap_scan_plan *p, ScanQueryDev &qd, Sca
         qd.key[i].base = k.base;
         qd.key[i].span = (uint64_t)k.card * k.div;
         qd.key[i].div = golap_fmt::udiv32_magic(k.div);
+        qd.key[i].zmin = p->cols[k.col]->d_chunk_min;
+        qd.key[i].zmax = p->cols[k.col]->d_chunk_max;
         qd.colmask |= 1u << k.col;
     }
     h.stride.resize(p->n_keys);
@@ -140,15 +143,26 @@ uint32_t scan_layout(const golap_column *const *cols, uint32_t n_cols, uint32_t 
     return bytes;
 }
 
-// LDS group slots for a scan launch: the requested or a default size, halved until the launch
-// fits the LDS budget (and, for the default, two workgroups per CU when it can).
-uint32_t scan_slots(golap_ctx *ctx, uint32_t agg_mode, uint32_t requested, uint64_t domain, uint32_t win_bytes) {
+// Dense LDS aggregation entries of a scan launch: 0 for global aggregation, else the requested
+// or a default count (a power of two), halved until the launch fits the LDS budget (and, for
+// the default, leaves room for several workgroups per CU).
+uint32_t scan_agg_entries(golap_ctx *ctx, uint32_t agg_mode, uint32_t requested, uint32_t win_bytes) {
     if (agg_mode == GOLAP_AGG_GLOBAL) return 0;
-    uint32_t s = requested ? std::min(pow2_at_least(requested), LDS_GROUP_SLOTS_MAX)
-                           : (uint32_t)std::min<uint64_t>(pow2_at_least(std::max<uint64_t>(2 * domain, 64)), 2048);
+    uint32_t s = requested ? std::min(pow2_at_least(requested), SCAN_AGG_MAX) : 2048u;
     const size_t budget = requested ? (size_t)ctx->lds_max : std::max<size_t>((size_t)ctx->lds_max / 2, win_bytes + 1024);
     while (s && scan_lds_bytes(win_bytes, s) > budget) s >>= 1;
     return s;
+}
+
+// Pages per work item: whole chunks when there are enough surviving chunks to fill the device,
+// else chunk parts, so a short slice still spreads over every CU.
+uint32_t scan_pages_per_item(golap_ctx *ctx, uint64_t surviving_chunks, uint32_t ppc) {
+    const uint64_t want = (uint64_t)ctx->cus * 4;
+    if (!surviving_chunks || surviving_chunks >= want) return ppc;
+    const uint64_t parts = std::min<uint64_t>(ppc, (want + surviving_chunks - 1) / surviving_chunks);
+    uint32_t ppi = (uint32_t)((ppc + parts - 1) / parts);
+    while ((ppc + ppi - 1) / ppi > 256) ++ppi;
+    return std::max<uint32_t>(ppi, 1);
 }
 
 }  // namespace starhost
@@ -165,7 +179,20 @@ int golap_scan_run(golap_ctx *ctx, const golap_scan_plan *plan, golap_star_resul
     ScanQueryDev qd;
     ScanHost h;
     if (int r = prepare_scan(ctx, plan, qd, h)) return r;
+    static_assert(sizeof(ScanQueryDev) <= 1024, "scan descriptor staged in 1 KiB");
     const golap_column *f = plan->cols[0];
+    const uint64_t nch = plan->chunk_end - plan->chunk_begin;
+    // surviving chunks on the host (the device work list applies the same test): item grain
+    std::vector<uint16_t> mask;
+    if (plan->dev_chunk_mask && nch) {
+        mask.resize(nch);
+        GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(mask.data(), plan->dev_chunk_mask + plan->chunk_begin, nch * 2,
+                                            hipMemcpyDeviceToHost, ctx->stream));
+        GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    const std::vector<uint8_t> run = scan_survivors(*plan, mask);
+    uint64_t surv = 0;
+    for (uint8_t v : run) surv += v;
     ScanLaunch L{};
     L.n_cols = plan->n_cols;
     int32_t woff[SCAN_MAX_COLS], dslot[SCAN_MAX_COLS];
@@ -177,49 +204,59 @@ int golap_scan_run(golap_ctx *ctx, const golap_scan_plan *plan, golap_star_resul
     L.n_rows = f->n_rows;
     L.n_pages = f->n_pages;
     L.n_queries = 1;
-    L.n_items = qd.s.page_end - qd.s.page_begin;
-    L.lds_gslots = scan_slots(ctx, plan->agg_mode, plan->lds_group_slots, h.domain, L.win_bytes);
-    if (scan_lds_bytes(L.win_bytes, L.lds_gslots) > (size_t)ctx->lds_max)
+    L.agg_entries = scan_agg_entries(ctx, plan->agg_mode, plan->lds_group_slots, L.win_bytes);
+    if (scan_lds_bytes(L.win_bytes, L.agg_entries) > (size_t)ctx->lds_max)
         return fail(ctx, GOLAP_EINVAL, "scan: LDS budget exceeded (page_rows or coded columns)");
-    L.flush_fill = L.lds_gslots / 2;
-    L.group_bits = std::max<uint32_t>(1, bits_for(h.domain));
-    qd.use_lds = L.lds_gslots != 0;
+    qd.use_lds = L.agg_entries != 0;
+    L.pages_per_item = scan_pages_per_item(ctx, surv, L.pages_per_chunk);
+    const uint32_t parts = (L.pages_per_chunk + L.pages_per_item - 1) / L.pages_per_item;
     int64_t *d_sum;
     uint64_t *d_cnt;
     if (int r = group_buffers(ctx, h.domain, &d_sum, &d_cnt)) return r;
     qd.g_sum = d_sum;
     qd.g_cnt = d_cnt;
     L.err = reinterpret_cast<uint32_t *>(ctx->d_scratch);
-    GOLAP_HIP_CHECK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
-    // with pruning, the surviving chunks may sit anywhere in the slice: a device-built work list
-    // keeps every workgroup busy (the static page ranges would leave most of them idle)
-    const bool pruned = plan->dev_chunk_mask || (plan->zone_prune && plan->n_preds);
-    if (pruned && L.n_items) {
-        const void *dq;
-        if (int r = solo_worklist(ctx, &qd, sizeof(qd), L.pages_per_chunk, &dq, &L.queue, const_cast<uint2 **>(&L.items)))
-            return r;
-        L.queries = static_cast<const ScanQueryDev *>(dq);
-        L.pages_per_item = L.pages_per_chunk;
-    } else {
-        L.q0 = qd;
+    // work list: the descriptor and the queue counters in the context scratch, the items in a
+    // context buffer
+    const uint64_t n_items = std::max<uint64_t>(nch * parts, 1);
+    if (n_items > ctx->scan_items_cap) {
+        dev_release(ctx, ctx->d_scan_items);
+        ctx->d_scan_items = nullptr;
+        ctx->scan_items_cap = 0;
+        const uint64_t cap = std::max<uint64_t>(n_items, 4096);
+        if (int r = dev_alloc(ctx, cap * sizeof(uint4), &ctx->d_scan_items)) return r;
+        ctx->scan_items_cap = cap;
     }
-    if (L.n_items) GOLAP_HIP_CHECK(ctx, launch_scan(ctx, L, ctx->stream));
+    if (!ctx->h_stage && hipHostMalloc(reinterpret_cast<void **>(&ctx->h_stage), 4096, hipHostMallocDefault) != hipSuccess) {
+        ctx->h_stage = nullptr;
+        return fail(ctx, GOLAP_ENOMEM, "scan: pinned staging");
+    }
+    ScanQueryDev *dq = reinterpret_cast<ScanQueryDev *>(ctx->d_scratch + 1024);
+    uint32_t *queue = reinterpret_cast<uint32_t *>(ctx->d_scratch + 2304);
+    // the staging page is reused by the next call only after this call's final synchronisation
+    GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    memcpy(ctx->h_stage, &qd, sizeof(qd));
+    GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(dq, ctx->h_stage, sizeof(qd), hipMemcpyHostToDevice, ctx->stream));
+    GOLAP_HIP_CHECK(ctx, launch_fill_zero(queue, 8, ctx->stream));
+    L.queries = dq;
+    L.queue = queue;
+    L.items = static_cast<const uint4 *>(ctx->d_scan_items);
+    GOLAP_HIP_CHECK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    if (nch) {
+        GOLAP_HIP_CHECK(ctx, launch_scan_items(dq, 1, plan->chunk_begin, nch, parts, static_cast<uint4 *>(ctx->d_scan_items),
+                                               queue, ctx->stream));
+        GOLAP_HIP_CHECK(ctx, launch_scan(ctx, L, ctx->stream));
+    }
     GOLAP_HIP_CHECK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     uint32_t flags = 0;
     GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(&flags, ctx->d_scratch, 4, hipMemcpyDeviceToHost, ctx->stream));
-    std::vector<uint16_t> mask;
-    if (plan->dev_chunk_mask && plan->chunk_end > plan->chunk_begin) {
-        mask.resize(plan->chunk_end - plan->chunk_begin);
-        GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(mask.data(), plan->dev_chunk_mask + plan->chunk_begin, mask.size() * 2,
-                                            hipMemcpyDeviceToHost, ctx->stream));
-    }
     GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     if (flags) return error_flags_status(ctx, flags, "scan");
     float ms = 0;
     GOLAP_HIP_CHECK(ctx, hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     res->kernel_ms = ms;
-    res->lds_group_slots = L.lds_gslots;
-    scan_stats(*plan, qd.colmask, scan_survivors(*plan, mask), res);
+    res->lds_group_slots = L.agg_entries;
+    scan_stats(*plan, qd.colmask, run, res);
     return emit_groups(ctx, h.domain, d_sum, d_cnt, h.lo, h.card, h.stride, &res->n_groups, &res->total_sum,
                        group_keys, sums, counts, max_groups, &res->rows_matched);
 }

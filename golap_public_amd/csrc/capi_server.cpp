@@ -220,6 +220,7 @@ void completion_loop(golap_server *s, Slot *slot) {
             std::lock_guard<std::mutex> lk(s->m);
             s->stats.device_ms += info.ms;
             s->stats.comp_bytes += info.comp_bytes;
+            s->stats.shared_comp_bytes += info.shared_comp_bytes;
         }
         if (!r && !reqs.empty()) {  // the whole arena back in one copy, before the parallel emission
             golap_star_result tmp{};

@@ -39,6 +39,9 @@ struct golap_ctx {
     // work list of pruned single-query plans, grown on demand
     void *d_solo_items = nullptr;
     uint64_t solo_items_cap = 0;
+    // work list of single scan plans (uint4 items), grown on demand
+    void *d_scan_items = nullptr;
+    uint64_t scan_items_cap = 0;
     uint8_t *h_stage = nullptr;  // pinned host staging for small uploads (4 KiB)
 };
 
@@ -253,6 +256,7 @@ struct ScanColView {
 struct ScanPredDev {
     uint32_t col, pad;
     int64_t lo, hi;
+    const int64_t *zmin, *zmax;  // the column's zone maps: a chunk inside [lo, hi] skips the test
 };
 // Computed group key k = (v - base) / div over [0, card): span = card * div < 2^32, so the
 // offset fits 32 bits and divides by the multiply-high method.
@@ -263,6 +267,7 @@ struct ScanKeyDev {
     golap_fmt::UDiv32 div;
     uint32_t pad;
     uint64_t stride;
+    const int64_t *zmin, *zmax;  // the column's zone maps: the key's range within a chunk
 };
 struct ScanQueryDev {
     SliceDev s;
@@ -277,26 +282,32 @@ struct ScanQueryDev {
     int64_t *g_sum;
     uint64_t *g_cnt;
 };
+// Shared-scan launch: work items are (chunk, part, query word) with a 64-bit mask of the
+// queries of that word that need the chunk; a workgroup reads the item's pages once for all of
+// them. queue[0] = next item, queue[1] = item count (device-built by launch_scan_items).
 struct ScanLaunch {
     ScanColView col[SCAN_MAX_COLS];
     uint32_t n_cols;
     uint32_t page_rows, pages_per_chunk;
     uint64_t n_rows, n_pages;
-    uint32_t win_bytes;  // LDS bytes of all windows
-    ScanQueryDev q0;     // static schedule with queries == nullptr
+    uint32_t win_bytes;    // LDS bytes of all windows
+    uint32_t agg_entries;  // dense LDS aggregation entries (SUM int64 + COUNT u32), 0 = global atomics
     const ScanQueryDev *queries;
     uint32_t n_queries;
-    uint64_t n_items;    // static schedule: pages of all queries
-    const uint2 *items;  // queue schedule (as StarLaunch)
+    const uint4 *items;
     uint32_t *queue;
     uint32_t pages_per_item;
-    uint32_t lds_gslots, group_bits;
-    uint32_t flush_fill;  // LDS table flushed to global at an item end when it holds more keys
     uint32_t max_blocks_per_cu;
     uint32_t *err;
 };
+constexpr uint32_t SCAN_AGG_MAX = 8192;  // 96 KiB of LDS
 hipError_t launch_scan(golap_ctx *ctx, const ScanLaunch &l, hipStream_t stream, bool prepare_only = false);
-size_t scan_lds_bytes(uint32_t win_bytes, uint32_t lds_gslots);
+// Work list of a scan launch: one thread per (chunk in [c0, c0 + nch), query word) sets the
+// bits of the word's queries whose slice keeps the chunk and, if any, appends `parts` items
+// {chunk, part | word << 8, mask lo, mask hi}.
+hipError_t launch_scan_items(const ScanQueryDev *queries, uint32_t n_queries, uint64_t c0, uint64_t nch,
+                             uint32_t parts, uint4 *items, uint32_t *queue, hipStream_t stream);
+size_t scan_lds_bytes(uint32_t win_bytes, uint32_t agg_entries);
 
 struct GroupByLaunch {
     const void *key[4];

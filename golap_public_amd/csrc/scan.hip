@@ -1,23 +1,25 @@
-// Fused scan + filter + group-by over paged typed columns (gfx950): the trip-table plans.
+// Fused shared scan + filter + group-by over paged typed columns (gfx950): the trip-table plans.
 //
 // The reference groups with group_by_agg_2col / 4col over decompressed attribute columns and
 // HashAggregate::add (core/group.cuh:14-36, core/dev_structs.cuh:209-244) after its range
-// predicates (usecases/ssb/queries/01.cuh:26-54). Here one persistent kernel reads the
-// compressed pages of every column a query uses:
-//   - in place (raw and literal-only LZ4 pages), by point access (frame-of-reference and RLE
-//     pages, delta pages with bw 0), in registers (delta pages), or decoded into an LDS window
-//     (other coded pages, by one wave);
-//   - thread t owns rows [t*PER, t*PER + PER) of the page and works column by column over
-//     four of them at a time, with wave-uniform column plans, so the loads of one column are
-//     independent and issue back to back; delta-bitpack pages of up to two columns are decoded
-//     in registers (running sums started from a block scan of the per-thread delta totals);
-//   - range predicates on any column; group keys computed per row as (v - base) / div with a
-//     magic-number division (hour of a timestamp, zone id), mixed-radix group index;
-//   - SUM/COUNT into an LDS-private hashed table keyed by (query, group) that spills rows to
-//     global int64 atomics when a slot run is full, and is flushed to the global dense result
-//     when it fills up at a work-item end (sorted time columns give every stretch of rows its
-//     own small set of groups); when all passing rows of a wave share one group, the wave adds
-//     their sum with a single update.
+// predicates (usecases/ssb/queries/01.cuh:26-54), one plan per launch. Here one persistent
+// kernel serves any number of plans over one table:
+//   - a work item is a chunk part with a 64-bit mask of the plans (of one word of 64) that
+//     need the chunk; the workgroup reads the item's pages once for all of them;
+//   - per (plan, item), the zone maps of the chunk give every key its local range (a sorted
+//     timestamp's hour spans one or two values in a chunk, a zone id all 265) and tell which
+//     predicates hold for the whole chunk (skipped) and which keys are constant (hoisted);
+//   - SUM/COUNT go into a dense LDS array over the local key ranges (no hashing, no probing),
+//     flushed to the plan's dense global result once per item; a plan whose local domain does
+//     not fit, or that asks for global aggregation, adds straight to global int64 atomics;
+//     when all passing rows of a wave share one group, the wave adds their sum in one update;
+//   - plans whose partials do not fit the LDS array together run in several passes over the
+//     item (the pages are then re-read from L2 / MALL);
+//   - pages are read in place (raw and literal-only LZ4 pages), by point access (frame-of-
+//     reference and RLE pages, delta pages with bw 0), in registers (delta pages: running sums
+//     started from a block scan of the per-thread delta totals), or decoded into an LDS window
+//     (other coded pages, by one wave); thread t owns rows [t*PER, t*PER + PER) of a page and
+//     works on four of them at a time.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -37,20 +39,14 @@ struct ScanArgs {
     uint32_t n_cols;
     uint32_t page_rows, pages_per_chunk;
     uint64_t n_rows, n_pages;
-    uint32_t win_bytes;
-    uint32_t n_queries;
-    ScanQueryDev q0;
+    uint32_t win_bytes, agg_entries;
     const ScanQueryDev *queries;
-    uint64_t n_items;
-    const uint2 *items;
+    uint32_t n_queries;
+    const uint4 *items;
     uint32_t *queue;
     uint32_t pages_per_item;
-    uint32_t lds_gslots, group_bits, flush_fill;
     uint32_t *err;
 };
-
-constexpr uint32_t SQ_WORDS = sizeof(ScanQueryDev) / 4;
-static_assert(sizeof(ScanQueryDev) % 8 == 0, "query descriptor copied as words");
 
 // How one typed column is read in the current tile. Offsets, not pointers: global addresses
 // are rebuilt from the kernel argument's column base (so loads stay global loads) and window
@@ -61,15 +57,41 @@ struct TPlan {
     int64_t a, b;  // AFFINE: v[r] = a + r*b; FOR: a = frame of reference; RLE: b = value bytes (padded);
                    // DBP: a = v[0], b = minimum delta
     uint32_t mode;
-    uint32_t aux;  // RLE: runs; FOR, DBP: bits per field
+    uint32_t aux;  // RLE: runs; FOR, DBP: bits per field; DIRECT: payload 16-byte aligned
+};
+
+constexpr uint32_t PASS_MAX = 16;        // plans evaluated together (their item views in LDS)
+constexpr uint32_t NO_LDS = 0xffffffffu;
+
+// A plan's view of the current work item. Predicates that hold for the whole chunk are left
+// out; keys constant over the chunk contribute cg to the global group index; the others are
+// listed first with their local range [klo, klo + lcard) and local stride.
+struct ItemQuery {
+    int64_t plo[SCAN_MAX_PREDS], phi[SCAN_MAX_PREDS];
+    int64_t kbase[SCAN_MAX_KEYS];
+    int64_t *g_sum;
+    uint64_t *g_cnt;
+    uint32_t kspan[SCAN_MAX_KEYS];
+    uint32_t km[SCAN_MAX_KEYS], ks[SCAN_MAX_KEYS];  // division magic: m; s1 | s2 << 8
+    uint32_t kgs[SCAN_MAX_KEYS];                     // global stride
+    uint32_t klo[SCAN_MAX_KEYS], lcard[SCAN_MAX_KEYS], lstride[SCAN_MAX_KEYS];
+    uint32_t pcol, kcol;  // column of predicate / key i in bits [8i, 8i + 8)
+    uint32_t np, nk;      // predicates tested and keys computed per row
+    int32_t vcol;         // -1: COUNT only
+    uint32_t cg;
+    uint32_t lbase, L;    // dense partials at [lbase, lbase + L); lbase NO_LDS: global atomics
+    uint32_t colmask;     // columns read per row
+    uint32_t pad;
 };
 
 struct ScanShared {
-    ScanQueryDev q;
     ScanColView cols[SCAN_MAX_COLS];  // the launch's columns: indexed at run time from LDS, not kernarg
     TPlan tp[SCAN_MAX_COLS];
     uint64_t wtot[SCAN_DBP_SLOTS][BLOCK / WAVE];
-    uint32_t tile_bad, fill, item;
+    ItemQuery iq[PASS_MAX];
+    uint32_t qidx[WAVE], qlen[WAVE];  // the item's live plans and their local domains (0: global)
+    uint32_t lbase[PASS_MAX];
+    uint32_t nlive, pass_end, pass_used, tile_bad, item;
 };
 
 __device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
@@ -82,23 +104,6 @@ __device__ __forceinline__ uint64_t uni64(uint64_t x) {
     return ((uint64_t)uni((uint32_t)(x >> 32)) << 32) | uni((uint32_t)x);
 }
 
-__device__ __forceinline__ uint32_t elem_of(uint32_t type) {
-    return type <= GOLAP_U8 ? 1u : type <= GOLAP_U16 ? 2u : type <= GOLAP_U32 ? 4u : 8u;
-}
-
-// The low elem bytes of raw as a value of type (sign or zero extension).
-__device__ __forceinline__ int64_t typed(uint64_t raw, uint32_t type) {
-    switch (type) {
-    case GOLAP_I8: return (int8_t)raw;
-    case GOLAP_U8: return (uint8_t)raw;
-    case GOLAP_I16: return (int16_t)raw;
-    case GOLAP_U16: return (uint16_t)raw;
-    case GOLAP_I32: return (int32_t)raw;
-    case GOLAP_U32: return (uint32_t)raw;
-    default: return (int64_t)raw;
-    }
-}
-
 __device__ __forceinline__ uint64_t load_elem(const uint8_t *p, uint32_t elem) {
     switch (elem) {
     case 1: return *p;
@@ -107,13 +112,6 @@ __device__ __forceinline__ uint64_t load_elem(const uint8_t *p, uint32_t elem) {
     default: return *reinterpret_cast<const uint64_t *>(p);
     }
 }
-__device__ __forceinline__ uint64_t gload_elem(const __attribute__((address_space(1))) uint8_t *p, uint32_t elem) {
-    switch (elem) {
-    case 1: return *p;
-    case 2: return *reinterpret_cast<const __attribute__((address_space(1))) uint16_t *>(p);
-    case 4: return *reinterpret_cast<const __attribute__((address_space(1))) uint32_t *>(p);
-    default: return *reinterpret_cast<const __attribute__((address_space(1))) uint64_t *>(p);
-    }
 }
 
 // Access plan of column v for page p; woff: its LDS window offset or -1.
@@ -129,6 +127,7 @@ __device__ inline TPlan tcol_plan(const ScanColView &v, uint64_t p, uint32_t row
         const uint32_t hdr = direct_hdr(codec, rows * elem);
         x.off = off + hdr;
         x.mode = (csize == hdr + rows * elem && !(reinterpret_cast<uintptr_t>(src + hdr) % elem)) ? TA_DIRECT : TA_BAD;
+        x.aux = !(reinterpret_cast<uintptr_t>(src + hdr) % 16);  // vector loads of a thread's rows
     } else if (cd == golap_fmt::DBP && (codec & golap_fmt::PAGE_POINT)) {
         if (csize < 32 || src[20] != 0 || src[21] != elem || ld_u32_unaligned(src + 16) != rows) {
             x.mode = TA_BAD;
@@ -178,8 +177,6 @@ __device__ inline TPlan tcol_plan(const ScanColView &v, uint64_t p, uint32_t row
     return x;
 }
 
-constexpr int NR = 4;  // rows of a thread processed together (independent loads per column)
-
 // Uniform view of a register-decoded delta-bitpack column in the current tile.
 struct DbpReg {
     const __attribute__((address_space(1))) uint32_t *pk;  // packed fields
@@ -197,57 +194,143 @@ __device__ __forceinline__ uint64_t dbp_delta(const DbpReg &d, uint32_t i) {
     return d.mind + p;
 }
 
-// Raw values of column c (wave-uniform) at the thread's rows r[j] whose bit is set in valid;
-// dv holds the current group's values of the register-decoded columns.
-__device__ __forceinline__ void fetch(const ScanArgs &a, const ScanShared &S, const uint8_t *lds, uint32_t c,
-                                      const uint32_t (&r)[NR], uint32_t valid, uint64_t (&raw)[NR],
-                                      const uint64_t (&dv)[SCAN_DBP_SLOTS][NR]) {
+// Shift that sign-extends a value of `type` held in its low bytes (0: none needed).
+__device__ __forceinline__ uint32_t sext_shift(uint32_t type, uint32_t elem) {
+    const bool sgn = type == GOLAP_I8 || type == GOLAP_I16 || type == GOLAP_I32;
+    return sgn ? 64u - 8u * elem : 0u;
+}
+__device__ __forceinline__ int64_t sext(uint64_t raw, uint32_t sh) { return sh ? ((int64_t)(raw << sh) >> sh) : (int64_t)raw; }
+
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(1))) u32x4 gvec4;
+typedef const __attribute__((address_space(1))) u32x2 gvec2;
+typedef const __attribute__((address_space(1))) uint32_t gu32;
+typedef const __attribute__((address_space(1))) uint64_t gu64;
+
+// Loads at a uniform base plus a 32-bit byte offset (the scalar-base form of global loads).
+__device__ __forceinline__ uint32_t gld32(gbyte *base, uint32_t off) { return *reinterpret_cast<gu32 *>(base + off); }
+__device__ __forceinline__ uint64_t gld64(gbyte *base, uint32_t off) { return *reinterpret_cast<gu64 *>(base + off); }
+__device__ __forceinline__ uint64_t gld_elem(gbyte *base, uint32_t off, uint32_t elem) {
+    switch (elem) {
+    case 1: return base[off];
+    case 2: return *reinterpret_cast<const __attribute__((address_space(1))) uint16_t *>(base + off);
+    case 4: return gld32(base, off);
+    default: return gld64(base, off);
+    }
+}
+
+// Values (as int64) of column c (wave-uniform) at the thread's rows r0 + j, j < HH; a row past
+// rmax (the page's last row) reads row rmax instead (its value is never used). Frame-of-reference,
+// delta and affine pages give the exact int64 values; in-place, window and run-length reads are
+// sign- or zero-extended from their element width. drs: the delta slots' values at row r0 - 1
+// (at row 0: the page's first value).
+template <int HH>
+__device__ __forceinline__ void fetch_col(const ScanShared &S, const uint8_t *lds, uint32_t c, uint32_t r0, uint32_t rmax,
+                                          int64_t (&out)[HH], const DbpReg (&dr)[SCAN_DBP_SLOTS],
+                                          const uint64_t (&drs)[SCAN_DBP_SLOTS]) {
     const uint32_t mode = uni(S.tp[c].mode);
-    if (mode == TA_DBP) {
+    if (mode == TA_DBP) {  // running sums of the packed deltas
         const uint32_t slot = uni((uint32_t)S.cols[c].dslot);
+        const DbpReg d = slot ? dr[1] : dr[0];
+        uint64_t run = slot ? drs[1] : drs[0];
 #pragma unroll
-        for (int j = 0; j < NR; ++j) raw[j] = slot ? dv[1][j] : dv[0][j];
+        for (int j = 0; j < HH; ++j) {
+            const uint32_t r = r0 + j;
+            if (r > 0 && r <= rmax) run += dbp_delta(d, r);
+            out[j] = (int64_t)run;
+        }
         return;
     }
     const uint32_t elem = uni(S.cols[c].c.elem);
+    const uint32_t sh = sext_shift(uni(S.cols[c].type), elem);
     const uint64_t off = uni64(S.tp[c].off);
     const uint8_t *comp = reinterpret_cast<const uint8_t *>(uni64(reinterpret_cast<uint64_t>(S.cols[c].c.comp)));
     switch (mode) {
     case TA_DIRECT: {
         gbyte *base = gmem(comp) + off;
+        if (r0 + HH - 1 <= rmax && uni(S.tp[c].aux)) {
+            // HH consecutive rows of a 16-byte aligned payload (r0 % HH == 0): vector loads
+            static_assert(HH == 8 || HH == 16, "vector loads of 8 or 16 rows");
+            gbyte *vb = base + (size_t)r0 * elem;
+            if (elem == 1 && HH == 8) {
+                const u32x2 x = *reinterpret_cast<gvec2 *>(vb);
 #pragma unroll
-        for (int j = 0; j < NR; ++j) raw[j] = ((valid >> j) & 1u) ? gload_elem(base + (size_t)r[j] * elem, elem) : 0;
+                for (int j = 0; j < HH; ++j) out[j] = sext(((j < 4 ? x.x : x.y) >> (8 * (j & 3))) & 0xffu, sh);
+            } else if (elem == 1) {
+                const u32x4 x = *reinterpret_cast<gvec4 *>(vb);
+                const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+                for (int j = 0; j < HH; ++j) out[j] = sext((w[(j >> 2) & 3] >> (8 * (j & 3))) & 0xffu, sh);
+            } else if (elem == 2) {
+#pragma unroll
+                for (int q = 0; q < HH / 8; ++q) {
+                    const u32x4 x = reinterpret_cast<gvec4 *>(vb)[q];
+                    const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) out[8 * q + j] = sext((w[j >> 1] >> (16 * (j & 1))) & 0xffffu, sh);
+                }
+            } else if (elem == 4) {
+#pragma unroll
+                for (int q = 0; q < HH / 4; ++q) {
+                    const u32x4 x = reinterpret_cast<gvec4 *>(vb)[q];
+                    out[4 * q] = sext(x.x, sh);
+                    out[4 * q + 1] = sext(x.y, sh);
+                    out[4 * q + 2] = sext(x.z, sh);
+                    out[4 * q + 3] = sext(x.w, sh);
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < HH / 2; ++q) {
+                    const u32x4 x = reinterpret_cast<gvec4 *>(vb)[q];
+                    out[2 * q] = (int64_t)(((uint64_t)x.y << 32) | x.x);
+                    out[2 * q + 1] = (int64_t)(((uint64_t)x.w << 32) | x.z);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < HH; ++j) out[j] = sext(gld_elem(base, min(r0 + j, rmax) * elem, elem), sh);
+        }
         break;
     }
     case TA_WINDOW: {
         const uint8_t *base = lds + off;
 #pragma unroll
-        for (int j = 0; j < NR; ++j) raw[j] = ((valid >> j) & 1u) ? load_elem(base + (size_t)r[j] * elem, elem) : 0;
+        for (int j = 0; j < HH; ++j) out[j] = sext(load_elem(base + (size_t)min(r0 + j, rmax) * elem, elem), sh);
         break;
     }
     case TA_FOR: {
-        const __attribute__((address_space(1))) uint64_t *w =
-            reinterpret_cast<const __attribute__((address_space(1))) uint64_t *>(gmem(comp) + off);
         const uint32_t bw = uni(S.tp[c].aux);
-        const uint64_t ref = uni64((uint64_t)S.tp[c].a);
-        const uint64_t mask = bw ? (~0ull >> (64 - bw)) : 0ull;
+        const int64_t ref = (int64_t)uni64((uint64_t)S.tp[c].a);
+        if (bw <= 32) {
+            // field r at bits [r*bw, r*bw + bw): two dwords and a funnel shift (the payload is
+            // followed by 8 zero bytes, so the second dword is always inside the page)
+            gbyte *w = gmem(comp) + off;
+            const uint32_t mask = bw == 32 ? 0xffffffffu : ((1u << bw) - 1u);
 #pragma unroll
-        for (int j = 0; j < NR; ++j) {
-            uint64_t x = 0;
-            if (((valid >> j) & 1u) && bw) {
-                const uint64_t bit = (uint64_t)r[j] * bw;
-                const uint32_t sh = (uint32_t)(bit & 63);
-                x = w[bit >> 6] >> sh;
-                if (sh + bw > 64) x |= w[(bit >> 6) + 1] << (64 - sh);  // the field ends in the next word
+            for (int j = 0; j < HH; ++j) {
+                const uint32_t bit = min(r0 + j, rmax) * bw;
+                const uint32_t lo = gld32(w, (bit >> 5) << 2), hi = gld32(w, ((bit >> 5) << 2) + 4);
+                out[j] = ref + (int64_t)(__builtin_amdgcn_alignbit(hi, lo, bit & 31u) & mask);
             }
-            raw[j] = ref + (x & mask);
+        } else {
+            gbyte *w = gmem(comp) + off;
+            const uint64_t mask = ~0ull >> (64 - bw);
+#pragma unroll
+            for (int j = 0; j < HH; ++j) {
+                const uint32_t bit = min(r0 + j, rmax) * bw;  // < 2^32: rows <= 8192, bw <= 56
+                const uint32_t s = bit & 63;
+                uint64_t x = gld64(w, (bit >> 6) << 3) >> s;
+                if (s + bw > 64) x |= gld64(w, ((bit >> 6) << 3) + 8) << (64 - s);  // the field ends in the next word
+                out[j] = ref + (int64_t)(x & mask);
+            }
         }
         break;
     }
     case TA_AFFINE: {
         const uint64_t a0 = uni64((uint64_t)S.tp[c].a), b0 = uni64((uint64_t)S.tp[c].b);
 #pragma unroll
-        for (int j = 0; j < NR; ++j) raw[j] = a0 + (uint64_t)r[j] * b0;
+        for (int j = 0; j < HH; ++j) out[j] = (int64_t)(a0 + (uint64_t)(r0 + j) * b0);
         break;
     }
     default: {  // RLE: first run whose exclusive end is above the row
@@ -255,18 +338,17 @@ __device__ __forceinline__ void fetch(const ScanArgs &a, const ScanShared &S, co
         const uint32_t nruns = uni(S.tp[c].aux);
         const uint64_t vbytes = uni64((uint64_t)S.tp[c].b);
 #pragma unroll
-        for (int j = 0; j < NR; ++j) {
-            raw[j] = 0;
-            if (!((valid >> j) & 1u)) continue;
+        for (int j = 0; j < HH; ++j) {
+            const uint32_t r = min(r0 + j, rmax);
             uint32_t lo = 0, hi = nruns - 1;
             while (lo < hi) {
                 const uint32_t mid = (lo + hi) >> 1;
-                if (ld_u32_unaligned(pg + 16 + vbytes + 4ull * mid) > r[j]) hi = mid;
+                if (ld_u32_unaligned(pg + 16 + vbytes + 4ull * mid) > r) hi = mid;
                 else lo = mid + 1;
             }
             uint64_t v = 0;
             for (uint32_t b = 0; b < elem; ++b) v |= (uint64_t)pg[16 + (uint64_t)lo * elem + b] << (8 * b);
-            raw[j] = v;
+            out[j] = sext(v, sh);
         }
         break;
     }
@@ -311,61 +393,121 @@ __device__ __forceinline__ bool dbp_decode_block(const uint8_t *src, uint32_t cs
     return true;
 }
 
-// group_add_lds for n rows of one key with sum v, counting the keys the table holds (fill) for
-// the flush decision.
-__device__ __forceinline__ bool group_add_counted(const GroupTable &T, uint32_t key, int64_t v, uint32_t n,
-                                                  uint32_t *fill) {
-    const uint32_t mask = T.slots - 1;
-    uint32_t h = (hash32(key) >> 7) & mask;
-    for (int i = 0; i < GROUP_PROBES; ++i) {
-        uint32_t k = T.gkey[h];
-        if (k == GROUP_EMPTY) {
-            const uint32_t old = atomicCAS(&T.gkey[h], GROUP_EMPTY, key);
-            if (old == GROUP_EMPTY) atomicAdd(fill, 1u);
-            k = old == GROUP_EMPTY ? key : old;
-        }
-        if (k == key) {
-            atomicAdd(&T.gsum[h], (unsigned long long)v);
-            atomicAdd(&T.gcnt[h], n);
-            return true;
-        }
-        h = (h + 1) & mask;
+// Predicate P on chunk c: 0 fails for every row, 1 holds for every row, 2 must be tested.
+__device__ __forceinline__ uint32_t pred_state(const ScanPredDev &P, uint64_t c) {
+    const int64_t zmin = P.zmin[c], zmax = P.zmax[c];
+    if (zmax < P.lo || zmin > P.hi) return 0;
+    return (zmin >= P.lo && zmax <= P.hi) ? 1 : 2;
+}
+
+// Key K on chunk c: its values' key indexes lie in [klo, khi]; inside: every row's key is in
+// range; false when no row's key is (a passing row is then an error, found per row).
+__device__ __forceinline__ bool key_range(const ScanKeyDev &K, uint64_t c, uint32_t &klo, uint32_t &khi, bool &inside) {
+    const int64_t zmin = K.zmin[c], zmax = K.zmax[c];
+    // offsets of the chunk's extreme values from base, clamped at 0
+    const uint64_t ulo = zmin <= K.base ? 0 : (uint64_t)zmin - (uint64_t)K.base;
+    const uint64_t uhi = zmax < K.base ? 0 : (uint64_t)zmax - (uint64_t)K.base;
+    inside = zmin >= K.base && uhi < K.span;
+    klo = khi = 0;
+    if (zmax < K.base || ulo >= K.span) return false;
+    klo = golap_fmt::udiv32((uint32_t)ulo, K.div);
+    khi = golap_fmt::udiv32((uint32_t)min<uint64_t>(uhi, K.span - 1), K.div);
+    return true;
+}
+
+// Plan Q on chunk c: false when a predicate fails for the whole chunk; *L = the local domain
+// of the keys that are not constant over the chunk, 0 when the plan aggregates in global
+// memory (it asks for that, a key's chunk range leaves its span, or the domain exceeds
+// agg_entries).
+__device__ bool item_domain(const ScanQueryDev &Q, uint64_t c, uint32_t agg_entries, uint32_t *L) {
+    for (uint32_t i = 0; i < Q.n_preds; ++i)
+        if (!pred_state(Q.pred[i], c)) return false;
+    uint64_t n = 1;
+    bool local = Q.use_lds != 0;
+    for (uint32_t i = 0; i < Q.n_keys; ++i) {
+        uint32_t klo, khi;
+        bool inside;
+        local &= key_range(Q.key[i], c, klo, khi, inside);
+        if (!(inside && klo == khi)) n *= khi - klo + 1;
     }
-    return false;
+    *L = (local && n <= agg_entries) ? (uint32_t)n : 0u;
+    return true;
 }
 
-// Moves every LDS entry into its query's dense global result and empties the table.
-__device__ __forceinline__ void scan_flush(const ScanArgs &a, const GroupTable &T, ScanShared &S) {
-    __syncthreads();
-    const uint32_t gmask = (1u << a.group_bits) - 1;
-    for (uint32_t i = threadIdx.x; i < T.slots; i += BLOCK) {
-        const uint32_t k = T.gkey[i];
-        if (k == GROUP_EMPTY) continue;
-        const ScanQueryDev *Q = a.queries ? a.queries + (k >> a.group_bits) : &a.q0;
-        atomicAdd(reinterpret_cast<unsigned long long *>(&Q->g_sum[k & gmask]), T.gsum[i]);
-        atomicAdd(reinterpret_cast<unsigned long long *>(&Q->g_cnt[k & gmask]), (unsigned long long)T.gcnt[i]);
-        T.gkey[i] = GROUP_EMPTY;
-        T.gcnt[i] = 0;
-        T.gsum[i] = 0;
+// Plan Q's view of chunk c (written by one thread; lbase is set by the caller). Predicates
+// that hold for the whole chunk are left out; keys constant over the chunk go into cg.
+__device__ void item_query(const ScanQueryDev &Q, uint64_t c, uint32_t agg_entries, ItemQuery &o) {
+    o.g_sum = Q.g_sum;
+    o.g_cnt = Q.g_cnt;
+    o.vcol = Q.val_col;
+    o.colmask = Q.val_col >= 0 ? 1u << Q.val_col : 0u;
+    uint32_t np = 0, pcol = 0;
+    for (uint32_t i = 0; i < Q.n_preds; ++i) {
+        const ScanPredDev &P = Q.pred[i];
+        if (pred_state(P, c) != 2) continue;
+        o.plo[np] = P.lo;
+        o.phi[np] = P.hi;
+        pcol |= P.col << (8 * np);
+        o.colmask |= 1u << P.col;
+        ++np;
     }
-    if (threadIdx.x == 0) S.fill = 0;
-    __syncthreads();
+    o.np = np;
+    o.pcol = pcol;
+    uint32_t nk = 0, kcol = 0, cg = 0;
+    for (uint32_t i = 0; i < Q.n_keys; ++i) {
+        const ScanKeyDev &K = Q.key[i];
+        uint32_t klo, khi;
+        bool inside;
+        (void)key_range(K, c, klo, khi, inside);
+        if (inside && klo == khi) {  // constant over the chunk
+            cg += klo * (uint32_t)K.stride;
+            continue;
+        }
+        o.kbase[nk] = K.base;
+        o.kspan[nk] = (uint32_t)K.span;
+        o.km[nk] = K.div.m;
+        o.ks[nk] = K.div.s1 | (K.div.s2 << 8);
+        o.kgs[nk] = (uint32_t)K.stride;
+        o.klo[nk] = klo;
+        o.lcard[nk] = khi - klo + 1;
+        kcol |= K.col << (8 * nk);
+        o.colmask |= 1u << K.col;
+        ++nk;
+    }
+    o.nk = nk;
+    o.kcol = kcol;
+    o.cg = cg;
+    uint32_t st = 1;
+    for (int n = (int)nk - 1; n >= 0; --n) {
+        o.lstride[n] = st;
+        st *= o.lcard[n];
+    }
+    uint32_t L = 0;
+    (void)item_domain(Q, c, agg_entries, &L);
+    o.L = L;
 }
 
-__device__ __forceinline__ void scan_set_query(const ScanArgs &a, ScanShared &S, uint32_t q) {
-    __syncthreads();
-    const uint32_t *src = reinterpret_cast<const uint32_t *>(a.queries ? a.queries + q : &a.q0);
-    uint32_t *dst = reinterpret_cast<uint32_t *>(&S.q);
-    for (uint32_t i = threadIdx.x; i < SQ_WORDS; i += BLOCK) dst[i] = src[i];
-    __syncthreads();
+// Adds the sum and count of rows of one group to dense partial e, or to the global result g.
+__device__ __forceinline__ void agg_lds(unsigned long long *lsum, uint32_t *lcnt, uint32_t e, int64_t v, uint32_t n,
+                                        bool sum) {
+    if (sum) atomicAdd(&lsum[e], (unsigned long long)v);
+    atomicAdd(&lcnt[e], n);
+}
+__device__ __forceinline__ void agg_global(const ItemQuery &Q, uint32_t g, int64_t v, uint32_t n, bool sum) {
+    if (sum) atomicAdd(reinterpret_cast<unsigned long long *>(&Q.g_sum[g]), (unsigned long long)v);
+    atomicAdd(reinterpret_cast<unsigned long long *>(&Q.g_cnt[g]), (unsigned long long)n);
 }
 
-template <int PER>
-__device__ __forceinline__ void scan_tile(const ScanArgs &a, ScanShared &S, uint32_t q, uint64_t p, uint8_t *lds,
-                                          const GroupTable &T) {
+// One page of the item for the plans of the current pass: column plans and window decodes of
+// the columns they read, then the page in sub-tiles of BLOCK * HH rows (thread t owns HH
+// consecutive rows of each); delta-bitpack columns with a register slot are decoded by running
+// sums whose start values come from a block scan of the per-thread delta totals.
+constexpr int HH = 8;
+constexpr uint32_t SUBTILE = BLOCK * HH;
+__device__ __forceinline__ void scan_page(const ScanArgs &a, ScanShared &S, uint64_t p, uint32_t used, uint32_t npass,
+                                          uint8_t *lds, unsigned long long *lsum, uint32_t *lcnt) {
     const uint32_t wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const uint32_t rows = (uint32_t)min<uint64_t>(a.page_rows, a.n_rows - p * a.page_rows);
-    const uint32_t used = S.q.colmask;
     if (threadIdx.x == 0) S.tile_bad = 0;
     if (threadIdx.x < a.n_cols && ((used >> threadIdx.x) & 1u)) S.tp[threadIdx.x] = tcol_plan(S.cols[threadIdx.x], p, rows);
     __syncthreads();
@@ -376,64 +518,19 @@ __device__ __forceinline__ void scan_tile(const ScanArgs &a, ScanShared &S, uint
         __syncthreads();
         return;
     }
-    // thread t owns rows [t*PER, t*PER + PER): delta-bitpack columns with a register slot are
-    // decoded by running sums; every thread's start value comes from a block scan of the
-    // per-thread delta totals
-    const uint32_t row0 = threadIdx.x * PER, row1 = min(rows, row0 + PER);
-    DbpReg dr[SCAN_DBP_SLOTS];
-    uint64_t run[SCAN_DBP_SLOTS], tot[SCAN_DBP_SLOTS], incl[SCAN_DBP_SLOTS];
-    uint32_t dmask = 0;  // active slots (uniform)
-#pragma unroll
-    for (uint32_t sl = 0; sl < SCAN_DBP_SLOTS; ++sl) {
-        run[sl] = tot[sl] = incl[sl] = 0;
-        dr[sl] = DbpReg{nullptr, 0, 0, 0, 0};
-    }
-    for (uint32_t c = 0; c < a.n_cols; ++c) {
-        if (!((used >> c) & 1u) || uni(S.tp[c].mode) != TA_DBP) continue;
-        const uint32_t sl = uni((uint32_t)S.cols[c].dslot);
-        const uint8_t *comp = reinterpret_cast<const uint8_t *>(uni64(reinterpret_cast<uint64_t>(S.cols[c].c.comp)));
-        DbpReg d;
-        d.pk = reinterpret_cast<const __attribute__((address_space(1))) uint32_t *>(gmem(comp) + uni64(S.tp[c].off) + 32);
-        d.bw = uni(S.tp[c].aux);
-        d.mind = uni64((uint64_t)S.tp[c].b);
-        d.fmask = d.bw == 32 ? 0xffffffffull : ((1ull << d.bw) - 1);
-        d.words = rows ? (((uint64_t)(rows - 1) * d.bw + 31) / 32) : 0;
-        uint64_t t = 0;
-        for (uint32_t i = max(row0, 1u); i < row1; ++i) t += dbp_delta(d, i);
-        const uint64_t in = wave_incl_scan_u64(t);
-        if (lane == WAVE - 1) S.wtot[sl][wave] = in;
-        if (sl == 0) {
-            dr[0] = d;
-            tot[0] = t;
-            incl[0] = in;
-            run[0] = uni64((uint64_t)S.tp[c].a);
-        } else {
-            dr[1] = d;
-            tot[1] = t;
-            incl[1] = in;
-            run[1] = uni64((uint64_t)S.tp[c].a);
-        }
-        dmask |= 1u << sl;
-    }
-    // coded pages without a slot: delta pages by the whole block, others by one wave each
-    bool any_wave = false;
+    // coded pages without a register slot: delta pages by the whole block, others by one wave each
+    bool any_wave = false, any_window = false;
     for (uint32_t c = 0; c < a.n_cols; ++c) {
         if (!((used >> c) & 1u) || S.tp[c].mode != TA_WINDOW) continue;
+        any_window = true;
         const ColView &cv = S.cols[c].c;
         if ((cv.page_codec[p] & PAGE_CODEC_MASK) == golap_fmt::DBP) {
-            __syncthreads();  // slot 0's totals are read before a block decode reuses wtot[0]
-            if ((dmask & 1u) && !(dmask & 4u)) {
-                run[0] += incl[0] - tot[0];
-                for (uint32_t w = 0; w < wave; ++w) run[0] += S.wtot[0][w];
-                incl[0] = tot[0] = 0;
-                dmask |= 4u;  // slot 0 start value taken
-            }
-            __syncthreads();
             if (!dbp_decode_block(cv.comp + cv.page_off[p], cv.page_csize[p], lds + S.tp[c].off, rows, cv.elem, S) &&
                 threadIdx.x == 0) {
                 S.tile_bad = 1;
                 raise_err(a.err, DEVERR_CORRUPT);
             }
+            __syncthreads();  // wtot is reused by the next block decode
         } else {
             any_wave = true;
         }
@@ -454,107 +551,182 @@ __device__ __forceinline__ void scan_tile(const ScanArgs &a, ScanShared &S, uint
             }
         }
     }
-    __syncthreads();
+    if (any_window) __syncthreads();
+    if (S.tile_bad) {  // uniform (read after the barrier)
+        __syncthreads();
+        return;
+    }
+    // register-decoded delta columns of the page
+    DbpReg dr[SCAN_DBP_SLOTS];
+    uint64_t carry[SCAN_DBP_SLOTS];  // value before the sub-tile's first row (uniform)
+    uint32_t dmask = 0;
 #pragma unroll
     for (uint32_t sl = 0; sl < SCAN_DBP_SLOTS; ++sl) {
-        if (!((dmask >> sl) & 1u) || (sl == 0 && (dmask & 4u))) continue;
-        run[sl] += incl[sl] - tot[sl];
-        for (uint32_t w = 0; w < wave; ++w) run[sl] += S.wtot[sl][w];
+        dr[sl] = DbpReg{nullptr, 0, 0, 0, 0};
+        carry[sl] = 0;
     }
-    if (!S.tile_bad) {
-        const uint32_t np = uni(S.q.n_preds), nk = uni(S.q.n_keys);
-        const int32_t vc = (int32_t)uni((uint32_t)S.q.val_col);
-        const bool use_lds = uni(S.q.use_lds) && T.slots;
-        const uint32_t qkey = q << a.group_bits;
-        int64_t *g_sum = reinterpret_cast<int64_t *>(uni64(reinterpret_cast<uint64_t>(S.q.g_sum)));
-        uint64_t *g_cnt = reinterpret_cast<uint64_t *>(uni64(reinterpret_cast<uint64_t>(S.q.g_cnt)));
-        for (int j0 = 0; j0 < PER; j0 += NR) {
-            uint32_t r[NR];
-            uint32_t valid = 0;
+    for (uint32_t c = 0; c < a.n_cols; ++c) {
+        if (!((used >> c) & 1u) || uni(S.tp[c].mode) != TA_DBP) continue;
+        const uint32_t sl = uni((uint32_t)S.cols[c].dslot);
+        const uint8_t *comp = reinterpret_cast<const uint8_t *>(uni64(reinterpret_cast<uint64_t>(S.cols[c].c.comp)));
+        DbpReg d;
+        d.pk = reinterpret_cast<const __attribute__((address_space(1))) uint32_t *>(gmem(comp) + uni64(S.tp[c].off) + 32);
+        d.bw = uni(S.tp[c].aux);
+        d.mind = uni64((uint64_t)S.tp[c].b);
+        d.fmask = d.bw == 32 ? 0xffffffffull : ((1ull << d.bw) - 1);
+        d.words = rows ? (((uint64_t)(rows - 1) * d.bw + 31) / 32) : 0;
+        if (sl == 0) {
+            dr[0] = d;
+            carry[0] = uni64((uint64_t)S.tp[c].a);
+        } else {
+            dr[1] = d;
+            carry[1] = uni64((uint64_t)S.tp[c].a);
+        }
+        dmask |= 1u << sl;
+    }
+    const uint32_t rmax = rows - 1;
+    for (uint32_t sub = 0; sub * SUBTILE < rows; ++sub) {
+        const uint32_t row0 = sub * SUBTILE + threadIdx.x * HH, row1 = min(rows, row0 + HH);
+        uint64_t run[SCAN_DBP_SLOTS];
 #pragma unroll
-            for (int j = 0; j < NR; ++j) {
-                r[j] = row0 + (uint32_t)(j0 + j);
-                valid |= (uint32_t)(r[j] < rows) << j;
-            }
-            // this group's values of the register-decoded columns
-            uint64_t dv[SCAN_DBP_SLOTS][NR];
+        for (uint32_t sl = 0; sl < SCAN_DBP_SLOTS; ++sl) run[sl] = carry[sl];
+        if (dmask) {  // uniform
+            uint64_t tot[SCAN_DBP_SLOTS], incl[SCAN_DBP_SLOTS];
 #pragma unroll
             for (uint32_t sl = 0; sl < SCAN_DBP_SLOTS; ++sl) {
-                const bool on = (dmask >> sl) & 1u;
+                tot[sl] = incl[sl] = 0;
+                if (!((dmask >> sl) & 1u)) continue;
+                for (uint32_t i = max(row0, 1u); i < row1; ++i) tot[sl] += dbp_delta(dr[sl], i);
+                incl[sl] = wave_incl_scan_u64(tot[sl]);
+                if (lane == WAVE - 1) S.wtot[sl][wave] = incl[sl];
+            }
+            __syncthreads();
 #pragma unroll
-                for (int j = 0; j < NR; ++j) {
-                    if (on && ((valid >> j) & 1u) && r[j] > 0) run[sl] += dbp_delta(dr[sl], r[j]);
-                    dv[sl][j] = run[sl];
+            for (uint32_t sl = 0; sl < SCAN_DBP_SLOTS; ++sl) {
+                if (!((dmask >> sl) & 1u)) continue;
+                run[sl] += incl[sl] - tot[sl];
+                for (uint32_t w = 0; w < BLOCK / WAVE; ++w) {
+                    if (w < wave) run[sl] += S.wtot[sl][w];
+                    carry[sl] += S.wtot[sl][w];
                 }
             }
-            uint32_t pass = valid;
-            uint64_t raw[NR];
-            for (uint32_t i = 0; i < np; ++i) {
-                const uint32_t c = uni(S.q.pred[i].col);
-                const uint32_t ty = uni(S.cols[c].type);
-                const int64_t lo = (int64_t)uni64((uint64_t)S.q.pred[i].lo), hi = (int64_t)uni64((uint64_t)S.q.pred[i].hi);
-                fetch(a, S, lds, c, r, pass, raw, dv);
+            __syncthreads();  // wtot is rewritten by the next sub-tile
+        }
+        // Plan by plan, column use by column use over the thread's HH rows: the plan's and the
+        // column's parameters stay uniform across them, every use issues HH independent loads,
+        // and rows are carried between uses as bit masks (pass) and partial group indexes (g
+        // global, e local).
+        for (uint32_t k = 0; k < npass; ++k) {
+            const ItemQuery &Q = S.iq[k];
+            const uint32_t np = uni(Q.np), nk = uni(Q.nk), pcol = uni(Q.pcol), kcol = uni(Q.kcol);
+            const int32_t vc = (int32_t)uni((uint32_t)Q.vcol);
+            const bool with_sum = vc >= 0;
+            const uint32_t nu = np + nk + 1;  // the last use is the value (0 for COUNT only)
+            const uint32_t lbase = uni(Q.lbase), cg = uni(Q.cg);
+            const bool one = lbase != NO_LDS && uni(Q.L) == 1u;  // every row of the item in one group
+            uint32_t pass = 0;
 #pragma unroll
-                for (int j = 0; j < NR; ++j) {
-                    const int64_t v = typed(raw[j], ty);
-                    if (v < lo || v > hi) pass &= ~(1u << j);
-                }
+            for (int j = 0; j < HH; ++j) pass |= (uint32_t)(row0 + j <= rmax) << j;
+            uint32_t inl = pass;
+            uint32_t g[HH], e[HH];
+#pragma unroll
+            for (int j = 0; j < HH; ++j) {
+                g[j] = cg;
+                e[j] = 0;
             }
-            if (__ballot(pass != 0) == 0) continue;  // wave-uniform
-            uint32_t g[NR];
+            for (uint32_t u = 0; u < nu; ++u) {
+                const bool last = u + 1 == nu;
+                if (__ballot(pass != 0) == 0) break;  // wave-uniform: no row passes
+                int64_t x[HH];
+                if (!last || with_sum) {
+                    const uint32_t c = u < np ? (pcol >> (8 * u)) & 0xffu
+                                              : u < np + nk ? (kcol >> (8 * (u - np))) & 0xffu : (uint32_t)vc;
+                    fetch_col<HH>(S, lds, c, row0, rmax, x, dr, run);
+                } else {
 #pragma unroll
-            for (int j = 0; j < NR; ++j) g[j] = 0;
-            for (uint32_t i = 0; i < nk; ++i) {
-                const uint32_t c = uni(S.q.key[i].col);
-                const uint32_t ty = uni(S.cols[c].type);
-                const uint64_t base = uni64((uint64_t)S.q.key[i].base), span = uni64(S.q.key[i].span);
-                golap_fmt::UDiv32 dvz;
-                dvz.m = uni(S.q.key[i].div.m);
-                dvz.s1 = uni(S.q.key[i].div.s1);
-                dvz.s2 = uni(S.q.key[i].div.s2);
-                const uint32_t stride = uni((uint32_t)S.q.key[i].stride);
-                fetch(a, S, lds, c, r, pass, raw, dv);
+                    for (int j = 0; j < HH; ++j) x[j] = 0;
+                }
+                if (u < np) {
+                    const int64_t lo = (int64_t)uni64((uint64_t)Q.plo[u]), hi = (int64_t)uni64((uint64_t)Q.phi[u]);
 #pragma unroll
-                for (int j = 0; j < NR; ++j) {
-                    const uint64_t u = (uint64_t)typed(raw[j], ty) - base;
-                    if (((pass >> j) & 1u) && u >= span) {
-                        raise_err(a.err, DEVERR_KEYRANGE);
-                        pass &= ~(1u << j);
+                    for (int j = 0; j < HH; ++j)
+                        if (x[j] < lo || x[j] > hi) pass &= ~(1u << j);
+                } else if (!last) {
+                    const uint32_t i = u - np;
+                    const uint64_t base = uni64((uint64_t)Q.kbase[i]);
+                    const uint32_t span = uni(Q.kspan[i]), gs = uni(Q.kgs[i]), klo = uni(Q.klo[i]);
+                    const uint32_t lcard = uni(Q.lcard[i]), ls = uni(Q.lstride[i]), ks = uni(Q.ks[i]);
+                    golap_fmt::UDiv32 dvz;
+                    dvz.m = uni(Q.km[i]);
+                    dvz.s1 = ks & 0xffu;
+                    dvz.s2 = ks >> 8;
+                    uint32_t bad = 0;
+#pragma unroll
+                    for (int j = 0; j < HH; ++j) {
+                        const uint64_t uo = (uint64_t)x[j] - base;
+                        bad |= (uint32_t)(uo >= span) << j;
+                        const uint32_t kk = golap_fmt::udiv32((uint32_t)uo, dvz);
+                        g[j] += kk * gs;
+                        const uint32_t lk = kk - klo;
+                        inl &= ~((uint32_t)(lk >= lcard) << j);
+                        e[j] += lk * ls;
                     }
-                    g[j] += golap_fmt::udiv32((uint32_t)u, dvz) * stride;
-                }
-            }
-            if (vc >= 0) {
-                const uint32_t ty = uni(S.cols[vc].type);
-                fetch(a, S, lds, (uint32_t)vc, r, pass, raw, dv);
+                    if (bad & pass) {  // a qualifying row's key outside its declared range
+                        raise_err(a.err, DEVERR_KEYRANGE);
+                        pass &= ~bad;
+                    }
+                } else {
+                    // aggregate the passing rows: x holds their values
+                    inl &= pass;
+                    const uint32_t outside = pass & ~inl;  // key outside the local range: global
+                    if (one) {  // the wave's rows in one update
+                        int64_t tsum = 0;
+                        uint32_t tcnt = 0;
 #pragma unroll
-                for (int j = 0; j < NR; ++j) raw[j] = (uint64_t)typed(raw[j], ty);
-            } else {
+                        for (int j = 0; j < HH; ++j)
+                            if ((inl >> j) & 1u) {
+                                tsum += x[j];
+                                ++tcnt;
+                            }
+                        if (with_sum) {
 #pragma unroll
-                for (int j = 0; j < NR; ++j) raw[j] = 0;
-            }
+                            for (int o = WAVE / 2; o > 0; o >>= 1) tsum += __shfl_xor(tsum, o, WAVE);
+                        }
 #pragma unroll
-            for (int j = 0; j < NR; ++j) {
-                const bool act = (pass >> j) & 1u;
-                const uint64_t am = __ballot(act);
-                if (!am) continue;
-                const uint32_t leader = (uint32_t)__ffsll((long long)am) - 1;
-                const uint32_t g0 = __shfl(g[j], leader, WAVE);
-                if (__ballot(act && g[j] != g0) == 0) {
-                    // every passing row of the wave is in one group (sorted keys): one update
-                    int64_t sum = act ? (int64_t)raw[j] : 0;
+                        for (int o = WAVE / 2; o > 0; o >>= 1) tcnt += __shfl_xor(tcnt, o, WAVE);
+                        if (lane == 0 && tcnt) agg_lds(lsum, lcnt, lbase, tsum, tcnt, with_sum);
+                    } else if (lbase != NO_LDS) {
 #pragma unroll
-                    for (int o = WAVE / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o, WAVE);
-                    if (lane == leader) {
-                        const uint32_t cnt = (uint32_t)__popcll(am);
-                        if (!use_lds || !group_add_counted(T, qkey | g0, sum, cnt, &S.fill)) {
-                            atomicAdd(reinterpret_cast<unsigned long long *>(&g_sum[g0]), (unsigned long long)sum);
-                            atomicAdd(reinterpret_cast<unsigned long long *>(&g_cnt[g0]), (unsigned long long)cnt);
+                        for (int j = 0; j < HH; ++j)
+                            if ((inl >> j) & 1u) agg_lds(lsum, lcnt, lbase + e[j], x[j], 1u, with_sum);
+                    }
+                    if (lbase != NO_LDS) {
+                        if (__ballot(outside != 0)) {
+#pragma unroll
+                            for (int j = 0; j < HH; ++j)
+                                if ((outside >> j) & 1u) agg_global(Q, g[j], x[j], 1u, with_sum);
+                        }
+                    } else {
+                        // global aggregation: rows of a wave that share one group (sorted keys) add once
+#pragma unroll
+                        for (int j = 0; j < HH; ++j) {
+                            const bool act = (pass >> j) & 1u;
+                            const uint64_t am = __ballot(act);
+                            if (!am) continue;
+                            const uint32_t leader = (uint32_t)__ffsll((long long)am) - 1;
+                            const uint32_t g0 = __builtin_amdgcn_readlane(g[j], leader);
+                            if (__ballot(act && g[j] != g0) == 0) {
+                                int64_t sum = act ? x[j] : 0;
+                                if (with_sum) {
+#pragma unroll
+                                    for (int o = WAVE / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o, WAVE);
+                                }
+                                if (lane == leader) agg_global(Q, g0, sum, (uint32_t)__popcll(am), with_sum);
+                            } else if (act) {
+                                agg_global(Q, g[j], x[j], 1u, with_sum);
+                            }
                         }
                     }
-                } else if (act) {
-                    if (!use_lds || !group_add_counted(T, qkey | g[j], (int64_t)raw[j], 1u, &S.fill))
-                        group_add_global(g_sum, g_cnt, g[j], (int64_t)raw[j]);
                 }
             }
         }
@@ -562,70 +734,143 @@ __device__ __forceinline__ void scan_tile(const ScanArgs &a, ScanShared &S, uint
     __syncthreads();
 }
 
-// Persistent scan kernel. Static schedule: the queries' pages flattened in query order, workgroup
-// b takes [b*n/G, (b+1)*n/G). Queue schedule: device-built (chunk, query, part) items taken with
+// Moves the pass's dense partials into their plans' global results.
+__device__ __forceinline__ void pass_flush(ScanShared &S, uint32_t npass, const unsigned long long *lsum,
+                                           const uint32_t *lcnt) {
+    __syncthreads();
+    for (uint32_t k = 0; k < npass; ++k) {
+        const ItemQuery &Q = S.iq[k];
+        if (Q.lbase == NO_LDS) continue;
+        const bool with_sum = Q.vcol >= 0;
+        for (uint32_t e = threadIdx.x; e < Q.L; e += BLOCK) {
+            const uint32_t n = lcnt[Q.lbase + e];
+            if (!n) continue;
+            uint32_t rem = e, g = Q.cg;
+            for (uint32_t i = 0; i < Q.nk; ++i) {
+                const uint32_t d = rem / Q.lstride[i];
+                rem -= d * Q.lstride[i];
+                g += (Q.klo[i] + d) * Q.kgs[i];
+            }
+            agg_global(Q, g, (int64_t)lsum[Q.lbase + e], n, with_sum);
+        }
+    }
+    __syncthreads();
+}
+
+// Persistent shared-scan kernel: workgroups take (chunk part, plan word, plan mask) items with
 // one atomic each.
-template <int PER, bool QUEUE>
-__global__ __launch_bounds__(BLOCK) void k_scan(ScanArgs a) {
+__global__ __launch_bounds__(BLOCK, 4) void k_scan(ScanArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     __shared__ ScanShared S;
-    GroupTable T;
-    T.slots = a.lds_gslots;
-    T.gkey = reinterpret_cast<uint32_t *>(lds + a.win_bytes);
-    T.gcnt = T.gkey + a.lds_gslots;
-    T.gsum = reinterpret_cast<unsigned long long *>(lds + a.win_bytes + (size_t)a.lds_gslots * 8);
-    for (uint32_t i = threadIdx.x; i < a.lds_gslots; i += BLOCK) {
-        T.gkey[i] = GROUP_EMPTY;
-        T.gcnt[i] = 0;
-        T.gsum[i] = 0;
-    }
+    unsigned long long *lsum = reinterpret_cast<unsigned long long *>(lds + a.win_bytes);
+    uint32_t *lcnt = reinterpret_cast<uint32_t *>(lds + a.win_bytes + (size_t)a.agg_entries * 8);
+    const uint32_t lane = threadIdx.x % WAVE;
 #pragma unroll
     for (uint32_t i = 0; i < SCAN_MAX_COLS; ++i)
         if (threadIdx.x == i) S.cols[i] = a.col[i];
-    if (threadIdx.x == 0) S.fill = 0;
-    uint32_t cur = 0xffffffffu;
-    if (!QUEUE) {
-        const uint64_t i0 = a.n_items * blockIdx.x / gridDim.x, i1 = a.n_items * (blockIdx.x + 1) / gridDim.x;
-        uint32_t q = 0;
-        for (uint64_t it = i0; it < i1; ++it) {
-            if (a.queries)
-                while (q + 1 < a.n_queries && it >= a.queries[q + 1].s.item_begin) ++q;
-            if (q != cur) {
-                scan_set_query(a, S, q);
-                cur = q;
+    const uint32_t n_items = a.queue[1];
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) S.item = atomicAdd(&a.queue[0], 1u);
+        __syncthreads();
+        const uint32_t it = S.item;
+        if (it >= n_items) break;
+        const uint4 w = a.items[it];
+        const uint64_t c = w.x;
+        const uint32_t part = w.y & 0xffu, word = w.y >> 8;
+        const uint64_t mask = (uint64_t)w.z | ((uint64_t)w.w << 32);
+        // the item's live plans (wave 0, one lane per plan of the word)
+        if (threadIdx.x < WAVE) {
+            const uint32_t q = word * WAVE + lane;
+            bool live = false;
+            uint32_t L = 0;
+            if (((mask >> lane) & 1u) && q < a.n_queries) live = item_domain(a.queries[q], c, a.agg_entries, &L);
+            const uint64_t b = __ballot(live);
+            if (live) {
+                const uint32_t rank = (uint32_t)__popcll(lane ? (b << (64 - lane)) : 0ull);
+                S.qidx[rank] = q;
+                S.qlen[rank] = L;
             }
-            const uint64_t p = S.q.s.page_begin + (it - S.q.s.item_begin);
-            if (!slice_keeps(S.q.s, p / a.pages_per_chunk)) continue;
-            scan_tile<PER>(a, S, q, p, lds, T);
-            if (T.slots && S.fill > a.flush_fill) scan_flush(a, T, S);  // S.fill read after the tile's barrier
+            if (lane == 0) S.nlive = (uint32_t)__popcll(b);
         }
-    } else {
-        const uint32_t n_items = a.queue[1];
-        for (;;) {
-            __syncthreads();
-            if (threadIdx.x == 0) S.item = atomicAdd(&a.queue[0], 1u);
-            __syncthreads();
-            const uint32_t it = S.item;
-            if (it >= n_items) break;
-            const uint2 w = a.items[it];
-            const uint32_t q = w.y >> 8, part = w.y & 0xff;
-            if (q != cur) {
-                scan_set_query(a, S, q);
-                cur = q;
+        __syncthreads();
+        const uint32_t nlive = S.nlive;
+        const uint64_t c0 = c * a.pages_per_chunk;
+        const uint64_t pb = c0 + (uint64_t)part * a.pages_per_item;
+        const uint64_t pe = min(min(pb + a.pages_per_item, c0 + a.pages_per_chunk), a.n_pages);
+        for (uint32_t k0 = 0; k0 < nlive;) {
+            // a pass: the next plans, up to PASS_MAX, whose partials fit the LDS array together
+            if (threadIdx.x == 0) {
+                uint32_t k1 = k0, used = 0;
+                while (k1 < nlive && k1 - k0 < PASS_MAX) {
+                    const uint32_t L = S.qlen[k1];
+                    if (L && used + L > a.agg_entries) break;
+                    S.lbase[k1 - k0] = L ? used : NO_LDS;
+                    used += L;
+                    ++k1;
+                }
+                S.pass_end = k1;
+                S.pass_used = used;
             }
-            const uint64_t c0 = (uint64_t)w.x * a.pages_per_chunk;
-            const uint64_t pb = c0 + (uint64_t)part * a.pages_per_item;
-            const uint64_t pe = min(min(pb + a.pages_per_item, c0 + a.pages_per_chunk), a.n_pages);
-            for (uint64_t p = pb; p < pe; ++p) scan_tile<PER>(a, S, q, p, lds, T);
-            if (T.slots && S.fill > a.flush_fill) scan_flush(a, T, S);
+            __syncthreads();
+            const uint32_t k1 = S.pass_end, npass = k1 - k0, entries = S.pass_used;
+            if (threadIdx.x < npass) {
+                ItemQuery &o = S.iq[threadIdx.x];
+                item_query(a.queries[S.qidx[k0 + threadIdx.x]], c, a.agg_entries, o);
+                o.lbase = S.lbase[threadIdx.x];
+            }
+            for (uint32_t e = threadIdx.x; e < entries; e += BLOCK) {
+                lsum[e] = 0;
+                lcnt[e] = 0;
+            }
+            __syncthreads();
+            uint32_t used = 0;
+            for (uint32_t k = 0; k < npass; ++k) used |= S.iq[k].colmask;
+            for (uint64_t p = pb; p < pe; ++p) scan_page(a, S, p, used, npass, lds, lsum, lcnt);
+            pass_flush(S, npass, lsum, lcnt);
+            k0 = k1;
         }
     }
-    if (T.slots) scan_flush(a, T, S);
+}
+
+// Scan work list: wave w handles (chunk, word) pairs [64w, 64w + 64); pairs with a plan that
+// keeps the chunk are compacted with a wave ballot and one global atomic per wave.
+__global__ __launch_bounds__(BLOCK) void k_scan_items(const ScanQueryDev *qs, uint32_t nq, uint64_t c0, uint64_t nch,
+                                                      uint32_t parts, uint4 *items, uint32_t *queue) {
+    const uint32_t lane = threadIdx.x % WAVE;
+    const uint32_t nw = (nq + WAVE - 1) / WAVE;
+    const uint64_t total = nch * nw;
+    const uint64_t waves = (uint64_t)gridDim.x * (BLOCK / WAVE);
+    for (uint64_t wv = (uint64_t)blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE; wv * WAVE < total; wv += waves) {
+        const uint64_t t = wv * WAVE + lane;
+        uint64_t mask = 0;
+        uint64_t c = 0;
+        uint32_t word = 0;
+        if (t < total) {
+            c = c0 + t / nw;
+            word = (uint32_t)(t % nw);
+            const uint32_t qe = min(nq, (word + 1) * (uint32_t)WAVE);
+            for (uint32_t q = word * WAVE; q < qe; ++q) {
+                const SliceDev &s = qs[q].s;
+                if (c >= s.chunk_begin && c < s.chunk_end && slice_keeps(s, c)) mask |= 1ull << (q - word * WAVE);
+            }
+        }
+        const uint64_t ball = __ballot(mask != 0);
+        uint32_t base = 0;
+        if (lane == 0 && ball) base = atomicAdd(&queue[1], (uint32_t)__popcll(ball) * parts);
+        base = __shfl(base, 0, WAVE);
+        if (mask) {
+            const uint64_t below = lane ? (ball << (64 - lane)) : 0ull;
+            const uint32_t slot = base + (uint32_t)__popcll(below) * parts;
+            for (uint32_t t2 = 0; t2 < parts; ++t2)
+                items[slot + t2] = make_uint4((uint32_t)c, t2 | (word << 8), (uint32_t)mask, (uint32_t)(mask >> 32));
+        }
+    }
 }
 
 template <typename K>
 hipError_t launch_scan_t(golap_ctx *ctx, K kernel, const ScanArgs &a, size_t lds, hipStream_t stream,
-                         uint32_t max_blocks_per_cu, uint64_t max_grid, bool prepare_only) {
+                         uint32_t max_blocks_per_cu, bool prepare_only) {
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -636,7 +881,7 @@ hipError_t launch_scan_t(golap_ctx *ctx, K kernel, const ScanArgs &a, size_t lds
     if (e != hipSuccess) return e;
     occ = std::max(occ, 1);
     if (max_blocks_per_cu) occ = std::min<int>(occ, (int)max_blocks_per_cu);
-    const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>(max_grid, (uint64_t)ctx->cus * occ));
+    const uint64_t grid = std::max<uint64_t>(1, (uint64_t)ctx->cus * occ);
     if (prepare_only) return hipSuccess;
     hipLaunchKernelGGL(kernel, dim3((uint32_t)grid), dim3(BLOCK), lds, stream, a);
     return hipGetLastError();
@@ -644,12 +889,12 @@ hipError_t launch_scan_t(golap_ctx *ctx, K kernel, const ScanArgs &a, size_t lds
 
 }  // namespace
 
-size_t scan_lds_bytes(uint32_t win_bytes, uint32_t lds_gslots) { return (size_t)win_bytes + (size_t)lds_gslots * 16; }
+size_t scan_lds_bytes(uint32_t win_bytes, uint32_t agg_entries) { return (size_t)win_bytes + (size_t)agg_entries * 12; }
 
 hipError_t launch_scan(golap_ctx *ctx, const ScanLaunch &l, hipStream_t stream, bool prepare_only) {
-    if (l.n_cols > SCAN_MAX_COLS || (!l.queries && l.n_queries != 1) || (l.items && !l.queue) ||
-        (l.items && (!l.pages_per_item || l.pages_per_item > l.pages_per_chunk)) || l.win_bytes % 16 ||
-        (l.lds_gslots & (l.lds_gslots - 1)) || l.group_bits > 31)
+    if (l.n_cols > SCAN_MAX_COLS || !l.queries || !l.n_queries || !l.items || !l.queue || !l.pages_per_item ||
+        l.pages_per_item > l.pages_per_chunk || l.win_bytes % 16 || l.agg_entries > SCAN_AGG_MAX ||
+        l.page_rows > 32 * BLOCK)
         return hipErrorInvalidValue;
     ScanArgs a{};
     for (uint32_t c = 0; c < l.n_cols; ++c) a.col[c] = l.col[c];
@@ -659,30 +904,29 @@ hipError_t launch_scan(golap_ctx *ctx, const ScanLaunch &l, hipStream_t stream, 
     a.n_rows = l.n_rows;
     a.n_pages = l.n_pages;
     a.win_bytes = l.win_bytes;
-    a.n_queries = l.n_queries;
-    a.q0 = l.q0;
+    a.agg_entries = l.agg_entries;
     a.queries = l.queries;
-    a.n_items = l.n_items;
+    a.n_queries = l.n_queries;
     a.items = l.items;
     a.queue = l.queue;
     a.pages_per_item = l.pages_per_item;
-    a.lds_gslots = l.lds_gslots;
-    a.group_bits = l.group_bits;
-    a.flush_fill = l.flush_fill;
     a.err = l.err;
-    if (!l.items && !a.n_items) return hipSuccess;
-    const size_t lds = scan_lds_bytes(l.win_bytes, l.lds_gslots);
+    const size_t lds = scan_lds_bytes(l.win_bytes, l.agg_entries);
     if (lds > (size_t)ctx->lds_max) return hipErrorInvalidConfiguration;
-    const uint64_t max_grid = l.items ? ~0ull : a.n_items;
     const uint32_t mb = l.max_blocks_per_cu;
-#define GOLAP_SCAN_CASE(PER)                                                                        \
-    return l.items ? launch_scan_t(ctx, k_scan<PER, true>, a, lds, stream, mb, max_grid, prepare_only) \
-                   : launch_scan_t(ctx, k_scan<PER, false>, a, lds, stream, mb, max_grid, prepare_only);
-    // rows per thread: page_rows / BLOCK rounded up to a multiple of NR
-    if (l.page_rows <= 8 * BLOCK) { GOLAP_SCAN_CASE(8) }
-    if (l.page_rows <= 16 * BLOCK) { GOLAP_SCAN_CASE(16) }
-    GOLAP_SCAN_CASE(32)
-#undef GOLAP_SCAN_CASE
+    return launch_scan_t(ctx, k_scan, a, lds, stream, mb, prepare_only);
+}
+
+hipError_t launch_scan_items(const ScanQueryDev *queries, uint32_t n_queries, uint64_t c0, uint64_t nch,
+                             uint32_t parts, uint4 *items, uint32_t *queue, hipStream_t stream) {
+    if (!n_queries || !parts || parts > 256 || n_queries > (1u << 24)) return hipErrorInvalidValue;
+    const uint64_t total = nch * ((n_queries + WAVE - 1) / WAVE);
+    if (!total) return hipSuccess;
+    const uint64_t waves = (total + WAVE - 1) / WAVE;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((waves + BLOCK / WAVE - 1) / (BLOCK / WAVE), 4096);
+    hipLaunchKernelGGL(k_scan_items, dim3(grid), dim3(BLOCK), 0, stream, queries, n_queries, c0, nch, parts, items,
+                       queue);
+    return hipGetLastError();
 }
 
 }  // namespace golap_impl

@@ -353,6 +353,8 @@ typedef struct {
     uint64_t items;              /* work items built on the device */
     uint64_t pages;              /* page tiles of the surviving chunks, all queries */
     uint64_t comp_bytes;         /* compressed bytes of the surviving pages' columns, all queries */
+    uint64_t shared_comp_bytes;  /* scan batches: the bytes read once per chunk for all queries that
+                                    keep it (union of their columns); star batches: comp_bytes */
 } golap_batch_run_info;
 int golap_batch_create_star(golap_ctx *ctx, const golap_star_query *queries, uint32_t n_queries,
                             golap_batch **out);
@@ -400,7 +402,8 @@ typedef struct {
     uint64_t peak_admitted_bytes;
     double mean_batch;         /* requests per launch */
     double device_ms;          /* device time of all launches (events around each batch) */
-    uint64_t comp_bytes;       /* compressed bytes of the surviving pages read, all batches */
+    uint64_t comp_bytes;       /* compressed bytes of the surviving pages, summed over requests */
+    uint64_t shared_comp_bytes; /* the bytes the shared launches read (per batch, once per chunk) */
 } golap_server_stats;
 int golap_server_create(golap_ctx *ctx, const golap_server_opts *opts, golap_server **out);
 int golap_server_submit_star(golap_server *s, const golap_star_query *q, uint64_t *ticket);
