@@ -69,7 +69,7 @@ bool lds_keys_fit(const golap_batch *b) { return bits_for(b->n_queries()) + grou
 size_t lds_bytes(const golap_batch *b, const golap_batch_config &c) {
     if (b->kind == BatchKind::STAR)
         return star_lds_bytes(c.filter_lds ? b->max_filter_words : 0, b->page_rows, b->n_windows, c.lds_group_slots);
-    return scan_lds_bytes(b->win_bytes, c.lds_group_slots);
+    return scan_lds_bytes(b->win_bytes, c.lds_group_slots, scan_page_waves(b->ctx, b->win_bytes, c.lds_group_slots));
 }
 
 int check_config(golap_batch *b, const golap_batch_config &c) {
@@ -81,7 +81,8 @@ int check_config(golap_batch *b, const golap_batch_config &c) {
     if (c.lds_group_slots && b->kind == BatchKind::STAR && !lds_keys_fit(b))
         return bfail(b, GOLAP_EINVAL, "batch: too many queries x groups for LDS group keys");
     if (c.filter_lds > 1 || c.compact_tables > 1) return bfail(b, GOLAP_EINVAL, "batch: bad flag");
-    if (lds_bytes(b, c) > (size_t)b->ctx->lds_max) return bfail(b, GOLAP_EINVAL, "batch: LDS budget exceeded");
+    const size_t budget = b->kind == BatchKind::SCAN ? scan_lds_budget(b->ctx) : (size_t)b->ctx->lds_max;
+    if (lds_bytes(b, c) > budget) return bfail(b, GOLAP_EINVAL, "batch: LDS budget exceeded");
     return GOLAP_OK;
 }
 
@@ -160,6 +161,7 @@ hipError_t launch_kernel(golap_batch *b, hipStream_t s, bool prepare_only) {
     L.n_pages = b->n_pages;
     L.win_bytes = b->win_bytes;
     L.agg_entries = c.lds_group_slots;
+    L.page_waves = scan_page_waves(b->ctx, b->win_bytes, c.lds_group_slots);
     L.queries = reinterpret_cast<const ScanQueryDev *>(sl->d_q);
     L.n_queries = b->n_queries();
     L.items = sl->d_items4;
@@ -227,7 +229,8 @@ golap_batch_config default_config(golap_batch *b) {
     } else {
         // dense partials for several plans per item; items small enough to fill the device
         c.lds_group_slots = 4096;
-        while (c.lds_group_slots && lds_bytes(b, c) > std::max<size_t>(b->ctx->lds_max / 2, b->win_bytes + 1024))
+        while (c.lds_group_slots &&
+               lds_bytes(b, c) > std::max<size_t>(scan_lds_budget(b->ctx) / 2, scan_lds_bytes(b->win_bytes, 0, 4) + 1024))
             c.lds_group_slots >>= 1;
         c.pages_per_item = scan_pages_per_item(b->ctx, b->union_pairs, b->ppc);
     }

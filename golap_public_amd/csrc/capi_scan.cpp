@@ -2,6 +2,7 @@
 // of the trip-table plans: validate the plan, compute the key strides and division constants,
 // place LDS windows for coded columns, and launch the persistent scan kernel.
 #include <algorithm>
+#include <cstdint>
 #include <cstring>
 #include <vector>
 
@@ -47,6 +48,8 @@ int prepare_scan(golap_ctx *ctx, const golap_scan_plan *p, ScanQueryDev &qd, Sca
         const golap_scan_key &k = p->keys[i];
         if (k.col >= p->n_cols || !k.div || !k.card) return fail(ctx, GOLAP_EINVAL, "scan: bad key");
         if ((uint64_t)k.card * k.div >= (1ull << 32)) return fail(ctx, GOLAP_EINVAL, "scan: key card * div >= 2^32");
+        if (k.base > INT64_MAX - (int64_t)((uint64_t)k.card * k.div - 1))
+            return fail(ctx, GOLAP_EINVAL, "scan: key range beyond int64");
         h.domain *= k.card;
         if (h.domain >= MAX_GROUPS) return fail(ctx, GOLAP_EINVAL, "scan: group domain too large");
         h.lo.push_back(k.out_lo);
@@ -143,14 +146,23 @@ uint32_t scan_layout(const golap_column *const *cols, uint32_t n_cols, uint32_t 
     return bytes;
 }
 
+// Waves per workgroup that take pages (each needs its own windows): all of them when their
+// windows fit beside agg_entries dense partial entries, else fewer.
+uint32_t scan_page_waves(golap_ctx *ctx, uint32_t win_bytes, uint32_t agg_entries) {
+    uint32_t w = 4;
+    while (w > 1 && scan_lds_bytes(win_bytes, agg_entries, w) > scan_lds_budget(ctx)) w >>= 1;
+    return w;
+}
+
 // Dense LDS aggregation entries of a scan launch: 0 for global aggregation, else the requested
 // or a default count (a power of two), halved until the launch fits the LDS budget (and, for
 // the default, leaves room for several workgroups per CU).
-uint32_t scan_agg_entries(golap_ctx *ctx, uint32_t agg_mode, uint32_t requested, uint32_t win_bytes) {
+uint32_t scan_agg_entries(golap_ctx *ctx, uint32_t agg_mode, uint32_t requested, uint32_t win_bytes, uint32_t page_waves) {
     if (agg_mode == GOLAP_AGG_GLOBAL) return 0;
     uint32_t s = requested ? std::min(pow2_at_least(requested), SCAN_AGG_MAX) : 2048u;
-    const size_t budget = requested ? (size_t)ctx->lds_max : std::max<size_t>((size_t)ctx->lds_max / 2, win_bytes + 1024);
-    while (s && scan_lds_bytes(win_bytes, s) > budget) s >>= 1;
+    const size_t budget = requested ? scan_lds_budget(ctx)
+                                    : std::max<size_t>(scan_lds_budget(ctx) / 2, scan_lds_bytes(win_bytes, 0, page_waves) + 1024);
+    while (s && scan_lds_bytes(win_bytes, s, page_waves) > budget) s >>= 1;
     return s;
 }
 
@@ -161,6 +173,7 @@ uint32_t scan_pages_per_item(golap_ctx *ctx, uint64_t surviving_chunks, uint32_t
     if (!surviving_chunks || surviving_chunks >= want) return ppc;
     const uint64_t parts = std::min<uint64_t>(ppc, (want + surviving_chunks - 1) / surviving_chunks);
     uint32_t ppi = (uint32_t)((ppc + parts - 1) / parts);
+    ppi = std::min(ppc, std::max(ppi, 4u));  // a page for each wave of the workgroup
     while ((ppc + ppi - 1) / ppi > 256) ++ppi;
     return std::max<uint32_t>(ppi, 1);
 }
@@ -204,8 +217,9 @@ int golap_scan_run(golap_ctx *ctx, const golap_scan_plan *plan, golap_star_resul
     L.n_rows = f->n_rows;
     L.n_pages = f->n_pages;
     L.n_queries = 1;
-    L.agg_entries = scan_agg_entries(ctx, plan->agg_mode, plan->lds_group_slots, L.win_bytes);
-    if (scan_lds_bytes(L.win_bytes, L.agg_entries) > (size_t)ctx->lds_max)
+    L.page_waves = scan_page_waves(ctx, L.win_bytes, 1024);
+    L.agg_entries = scan_agg_entries(ctx, plan->agg_mode, plan->lds_group_slots, L.win_bytes, L.page_waves);
+    if (scan_lds_bytes(L.win_bytes, L.agg_entries, L.page_waves) > scan_lds_budget(ctx))
         return fail(ctx, GOLAP_EINVAL, "scan: LDS budget exceeded (page_rows or coded columns)");
     qd.use_lds = L.agg_entries != 0;
     L.pages_per_item = scan_pages_per_item(ctx, surv, L.pages_per_chunk);

@@ -290,8 +290,9 @@ struct ScanLaunch {
     uint32_t n_cols;
     uint32_t page_rows, pages_per_chunk;
     uint64_t n_rows, n_pages;
-    uint32_t win_bytes;    // LDS bytes of all windows
+    uint32_t win_bytes;    // LDS bytes of one wave's windows
     uint32_t agg_entries;  // dense LDS aggregation entries (SUM int64 + COUNT u32), 0 = global atomics
+    uint32_t page_waves;   // waves of a workgroup that take pages, each with its own windows (1..4)
     const ScanQueryDev *queries;
     uint32_t n_queries;
     const uint4 *items;
@@ -307,7 +308,9 @@ hipError_t launch_scan(golap_ctx *ctx, const ScanLaunch &l, hipStream_t stream, 
 // {chunk, part | word << 8, mask lo, mask hi}.
 hipError_t launch_scan_items(const ScanQueryDev *queries, uint32_t n_queries, uint64_t c0, uint64_t nch,
                              uint32_t parts, uint4 *items, uint32_t *queue, hipStream_t stream);
-size_t scan_lds_bytes(uint32_t win_bytes, uint32_t agg_entries);
+size_t scan_lds_bytes(uint32_t win_bytes, uint32_t agg_entries, uint32_t page_waves);
+// Dynamic LDS a scan launch may use (the device limit less the kernel's own LDS).
+size_t scan_lds_budget(golap_ctx *ctx);
 
 struct GroupByLaunch {
     const void *key[4];

@@ -13,13 +13,20 @@
 //     flushed to the plan's dense global result once per item; a plan whose local domain does
 //     not fit, or that asks for global aggregation, adds straight to global int64 atomics;
 //     when all passing rows of a wave share one group, the wave adds their sum in one update;
+//   - plans that evaluate the chunk's rows identically (same tested predicates, same local key
+//     functions with every key in range, same value column: typically one request template
+//     over different time ranges) share one set of partials, computed once and flushed into
+//     each plan's result with its own group mapping;
 //   - plans whose partials do not fit the LDS array together run in several passes over the
 //     item (the pages are then re-read from L2 / MALL);
-//   - pages are read in place (raw and literal-only LZ4 pages), by point access (frame-of-
-//     reference and RLE pages, delta pages with bw 0), in registers (delta pages: running sums
-//     started from a block scan of the per-thread delta totals), or decoded into an LDS window
-//     (other coded pages, by one wave); thread t owns rows [t*PER, t*PER + PER) of a page and
-//     works on four of them at a time.
+//   - the waves of a workgroup take the item's pages round robin and work on a page alone:
+//     column plans, window decodes and delta-bitpack prefix sums are per wave, so no workgroup
+//     barrier stands between pages (only the item's set-up and flush synchronise the block);
+//   - a page is read in place (raw and literal-only LZ4 pages), by point access (frame-of-
+//     reference and RLE pages, delta pages with bw 0), by running sums in registers (delta
+//     pages; start values from a wave prefix scan of per-lane delta totals), or from an LDS
+//     window the wave decodes it into (other coded pages); lane l owns HH consecutive rows of
+//     every sub-tile of WAVE * HH rows, and every column use reads all HH of them at once.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -39,7 +46,8 @@ struct ScanArgs {
     uint32_t n_cols;
     uint32_t page_rows, pages_per_chunk;
     uint64_t n_rows, n_pages;
-    uint32_t win_bytes, agg_entries;
+    uint32_t win_bytes, agg_entries;  // windows of one wave; dense partial entries
+    uint32_t page_waves;              // waves that take pages (each with its own windows)
     const ScanQueryDev *queries;
     uint32_t n_queries;
     const uint4 *items;
@@ -48,20 +56,22 @@ struct ScanArgs {
     uint32_t *err;
 };
 
-// How one typed column is read in the current tile. Offsets, not pointers: global addresses
-// are rebuilt from the kernel argument's column base (so loads stay global loads) and window
-// addresses from the LDS array.
+// How one typed column is read in the current page. off: byte offset of the data in the
+// column buffer (windows: in the wave's LDS windows).
 enum : uint32_t { TA_DIRECT = 0, TA_WINDOW = 1, TA_AFFINE = 2, TA_RLE = 3, TA_FOR = 4, TA_DBP = 5, TA_BAD = 6 };
 struct TPlan {
-    uint64_t off;  // DIRECT: payload; FOR: fields; RLE, DBP: page start (byte offsets into comp); WINDOW: LDS offset
+    uint64_t off;  // DIRECT: payload; FOR: fields; RLE, DBP: page start; WINDOW: LDS offset
     int64_t a, b;  // AFFINE: v[r] = a + r*b; FOR: a = frame of reference; RLE: b = value bytes (padded);
                    // DBP: a = v[0], b = minimum delta
     uint32_t mode;
     uint32_t aux;  // RLE: runs; FOR, DBP: bits per field; DIRECT: payload 16-byte aligned
 };
 
-constexpr uint32_t PASS_MAX = 16;        // plans evaluated together (their item views in LDS)
+constexpr uint32_t PASS_MAX = 16;  // plans evaluated together (their item views in LDS)
 constexpr uint32_t NO_LDS = 0xffffffffu;
+constexpr uint32_t NWAVE = BLOCK / WAVE;
+constexpr int HH = 8;                        // rows of a lane per sub-tile
+constexpr uint32_t SUBTILE = WAVE * HH;      // rows of a wave's sub-tile
 
 // A plan's view of the current work item. Predicates that hold for the whole chunk are left
 // out; keys constant over the chunk contribute cg to the global group index; the others are
@@ -81,51 +91,73 @@ struct ItemQuery {
     uint32_t cg;
     uint32_t lbase, L;    // dense partials at [lbase, lbase + L); lbase NO_LDS: global atomics
     uint32_t colmask;     // columns read per row
-    uint32_t pad;
+    // Sharing: plans whose rows evaluate identically on the chunk (same tested predicates, same
+    // local key functions, every key inside its range, same value column) add into one set of
+    // partials, computed by the first of them (leader) and flushed into every one's result.
+    int64_t korg[SCAN_MAX_KEYS];  // value where local key index 0 starts (kbase + klo * div)
+    uint32_t sig;                 // hash of the evaluation signature
+    uint32_t shareable;
+    uint32_t leader;              // pass index of the plan whose partials this plan uses
+    uint32_t members;             // leaders: other plans use these partials
 };
 
 struct ScanShared {
     ScanColView cols[SCAN_MAX_COLS];  // the launch's columns: indexed at run time from LDS, not kernarg
-    TPlan tp[SCAN_MAX_COLS];
-    uint64_t wtot[SCAN_DBP_SLOTS][BLOCK / WAVE];
+    TPlan tp[NWAVE][SCAN_MAX_COLS];   // each wave's plans of its current page
     ItemQuery iq[PASS_MAX];
     uint32_t qidx[WAVE], qlen[WAVE];  // the item's live plans and their local domains (0: global)
-    uint32_t lbase[PASS_MAX];
-    uint32_t nlive, pass_end, pass_used, tile_bad, item;
+    uint32_t nlive, pass_end, pass_used, item;
 };
 
 __device__ __forceinline__ uint32_t uni(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
-
-// A column base read back from LDS is a generic pointer; it always points to device memory, and
-// loads through the global address space are plain global loads.
-typedef const __attribute__((address_space(1))) uint8_t gbyte;
-__device__ __forceinline__ gbyte *gmem(const uint8_t *p) { return (gbyte *)p; }
 __device__ __forceinline__ uint64_t uni64(uint64_t x) {
     return ((uint64_t)uni((uint32_t)(x >> 32)) << 32) | uni((uint32_t)x);
 }
 
-__device__ __forceinline__ uint64_t load_elem(const uint8_t *p, uint32_t elem) {
+// Address-space views: column buffers are global memory, windows are LDS. Loads at a uniform
+// base plus a 32-bit byte offset compile to the scalar-base form of global loads and to plain
+// LDS reads.
+typedef const __attribute__((address_space(1))) uint8_t gbyte;
+typedef const __attribute__((address_space(3))) uint8_t lbyte;
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ gbyte *gmem(const uint8_t *p) { return (gbyte *)p; }
+
+template <typename T>
+__device__ __forceinline__ T ldt(gbyte *base, uint32_t off) {
+    return *reinterpret_cast<const __attribute__((address_space(1))) T *>(base + off);
+}
+template <typename T>
+__device__ __forceinline__ T ldt(lbyte *base, uint32_t off) {
+    return *reinterpret_cast<const __attribute__((address_space(3))) T *>(base + off);
+}
+template <typename B>
+__device__ __forceinline__ uint64_t ld_elem(B *base, uint32_t off, uint32_t elem) {
     switch (elem) {
-    case 1: return *p;
-    case 2: return *reinterpret_cast<const uint16_t *>(p);
-    case 4: return *reinterpret_cast<const uint32_t *>(p);
-    default: return *reinterpret_cast<const uint64_t *>(p);
+    case 1: return ldt<uint8_t>(base, off);
+    case 2: return ldt<uint16_t>(base, off);
+    case 4: return ldt<uint32_t>(base, off);
+    default: return ldt<uint64_t>(base, off);
     }
 }
-}
 
-// Access plan of column v for page p; woff: its LDS window offset or -1.
-__device__ inline TPlan tcol_plan(const ScanColView &v, uint64_t p, uint32_t rows) {
+// Shift that sign-extends a value of `type` held in its low bytes (0: none needed).
+__device__ __forceinline__ uint32_t sext_shift(uint32_t type, uint32_t elem) {
+    const bool sgn = type == GOLAP_I8 || type == GOLAP_I16 || type == GOLAP_I32;
+    return sgn ? 64u - 8u * elem : 0u;
+}
+__device__ __forceinline__ int64_t sext(uint64_t raw, uint32_t sh) { return sh ? ((int64_t)(raw << sh) >> sh) : (int64_t)raw; }
+
+// Access plan of column v for a page whose bytes start at src (global or LDS, generic pointer)
+// at offset base_off from the column's base, with the catalog's codec and size.
+__device__ inline TPlan plan_page(const ScanColView &v, const uint8_t *src, uint64_t base_off, uint8_t codec, uint32_t csize,
+                                  uint32_t rows) {
     TPlan x{};
     const uint32_t elem = v.c.elem;
-    const uint8_t codec = v.c.page_codec[p];
-    const uint64_t off = v.c.page_off[p];
-    const uint8_t *src = v.c.comp + off;
-    const uint32_t csize = v.c.page_csize[p];
     const uint32_t cd = codec & PAGE_CODEC_MASK;
     if (is_direct(codec)) {
         const uint32_t hdr = direct_hdr(codec, rows * elem);
-        x.off = off + hdr;
+        x.off = base_off + hdr;
         x.mode = (csize == hdr + rows * elem && !(reinterpret_cast<uintptr_t>(src + hdr) % elem)) ? TA_DIRECT : TA_BAD;
         x.aux = !(reinterpret_cast<uintptr_t>(src + hdr) % 16);  // vector loads of a thread's rows
     } else if (cd == golap_fmt::DBP && (codec & golap_fmt::PAGE_POINT)) {
@@ -137,11 +169,11 @@ __device__ inline TPlan tcol_plan(const ScanColView &v, uint64_t p, uint32_t row
             x.mode = TA_AFFINE;
         }
     } else if (cd == golap_fmt::FOR) {
-        // fields read as aligned 8-byte words: the payload must be 8-byte aligned
+        // fields read as aligned 4- or 8-byte words: the payload must be 8-byte aligned
         if (for_check(src, csize, elem) != (int64_t)rows || (reinterpret_cast<uintptr_t>(src) & 7)) {
             x.mode = TA_BAD;
         } else {
-            x.off = off + 32;
+            x.off = base_off + 32;
             x.a = (int64_t)ld_u64_unaligned(src);
             x.aux = src[20];
             x.mode = TA_FOR;
@@ -151,7 +183,7 @@ __device__ inline TPlan tcol_plan(const ScanColView &v, uint64_t p, uint32_t row
         const uint64_t vbytes = ((uint64_t)nruns * elem + 3) & ~3ull;
         const bool ok = csize >= 16 && src[8] == elem && ld_u32_unaligned(src + 4) == rows && nruns >= 1 &&
                         16ull + vbytes + 4ull * nruns <= csize;
-        x.off = off;
+        x.off = base_off;
         x.aux = nruns;
         x.b = (int64_t)vbytes;
         x.mode = ok ? TA_RLE : TA_BAD;
@@ -162,7 +194,7 @@ __device__ inline TPlan tcol_plan(const ScanColView &v, uint64_t p, uint32_t row
             (reinterpret_cast<uintptr_t>(src) & 3)) {
             x.mode = TA_BAD;
         } else {
-            x.off = off;
+            x.off = base_off;
             x.a = (int64_t)ld_u64_unaligned(src);
             x.b = (int64_t)ld_u64_unaligned(src + 8);
             x.aux = bw;
@@ -177,62 +209,39 @@ __device__ inline TPlan tcol_plan(const ScanColView &v, uint64_t p, uint32_t row
     return x;
 }
 
-// Uniform view of a register-decoded delta-bitpack column in the current tile.
+// Register-decoded delta-bitpack column of the current page (uniform); B: its address space.
+template <typename B>
 struct DbpReg {
-    const __attribute__((address_space(1))) uint32_t *pk;  // packed fields
+    B *pk;  // packed fields
     uint64_t mind, fmask, words;
     uint32_t bw;
 };
 // v[i] - v[i-1] of row i >= 1.
-__device__ __forceinline__ uint64_t dbp_delta(const DbpReg &d, uint32_t i) {
+template <typename B>
+__device__ __forceinline__ uint64_t dbp_delta(const DbpReg<B> &d, uint32_t i) {
     uint64_t p = 0;
     if (d.bw) {
-        const uint64_t bit = (uint64_t)(i - 1) * d.bw, w = bit >> 5;
-        const uint64_t lo = d.pk[w], hi = (w + 1 < d.words) ? d.pk[w + 1] : 0;
+        const uint32_t bit = (i - 1) * d.bw, w = bit >> 5;
+        const uint64_t lo = ldt<uint32_t>(d.pk, w << 2), hi = (w + 1 < d.words) ? ldt<uint32_t>(d.pk, (w + 1) << 2) : 0;
         p = ((lo | (hi << 32)) >> (bit & 31)) & d.fmask;
     }
     return d.mind + p;
 }
 
-// Shift that sign-extends a value of `type` held in its low bytes (0: none needed).
-__device__ __forceinline__ uint32_t sext_shift(uint32_t type, uint32_t elem) {
-    const bool sgn = type == GOLAP_I8 || type == GOLAP_I16 || type == GOLAP_I32;
-    return sgn ? 64u - 8u * elem : 0u;
-}
-__device__ __forceinline__ int64_t sext(uint64_t raw, uint32_t sh) { return sh ? ((int64_t)(raw << sh) >> sh) : (int64_t)raw; }
-
-typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-typedef const __attribute__((address_space(1))) u32x4 gvec4;
-typedef const __attribute__((address_space(1))) u32x2 gvec2;
-typedef const __attribute__((address_space(1))) uint32_t gu32;
-typedef const __attribute__((address_space(1))) uint64_t gu64;
-
-// Loads at a uniform base plus a 32-bit byte offset (the scalar-base form of global loads).
-__device__ __forceinline__ uint32_t gld32(gbyte *base, uint32_t off) { return *reinterpret_cast<gu32 *>(base + off); }
-__device__ __forceinline__ uint64_t gld64(gbyte *base, uint32_t off) { return *reinterpret_cast<gu64 *>(base + off); }
-__device__ __forceinline__ uint64_t gld_elem(gbyte *base, uint32_t off, uint32_t elem) {
-    switch (elem) {
-    case 1: return base[off];
-    case 2: return *reinterpret_cast<const __attribute__((address_space(1))) uint16_t *>(base + off);
-    case 4: return gld32(base, off);
-    default: return gld64(base, off);
-    }
-}
-
 // Values (as int64) of column c (wave-uniform) at the thread's rows r0 + j, j < HH; a row past
-// rmax (the page's last row) reads row rmax instead (its value is never used). Frame-of-reference,
-// delta and affine pages give the exact int64 values; in-place, window and run-length reads are
-// sign- or zero-extended from their element width. drs: the delta slots' values at row r0 - 1
-// (at row 0: the page's first value).
-template <int HH>
-__device__ __forceinline__ void fetch_col(const ScanShared &S, const uint8_t *lds, uint32_t c, uint32_t r0, uint32_t rmax,
-                                          int64_t (&out)[HH], const DbpReg (&dr)[SCAN_DBP_SLOTS],
-                                          const uint64_t (&drs)[SCAN_DBP_SLOTS]) {
-    const uint32_t mode = uni(S.tp[c].mode);
+// rmax (the page's last row) reads row rmax instead (its value is never used). base: the data
+// base the plans' offsets refer to; windows are in the wave's LDS windows (lds). Frame-of-reference, delta and
+// affine pages give the exact int64 values; in-place, window and run-length reads are sign- or
+// zero-extended from their element width. drs: the delta slots' values at row r0 - 1 (at row
+// 0: the page's first value).
+template <typename B>
+__device__ __forceinline__ void fetch_col(const TPlan *tp, const ScanColView *cols, B *base, lbyte *lds, uint32_t c,
+                                          uint32_t r0, uint32_t rmax, int64_t (&out)[HH],
+                                          const DbpReg<B> (&dr)[SCAN_DBP_SLOTS], const uint64_t (&drs)[SCAN_DBP_SLOTS]) {
+    const uint32_t mode = uni(tp[c].mode);
     if (mode == TA_DBP) {  // running sums of the packed deltas
-        const uint32_t slot = uni((uint32_t)S.cols[c].dslot);
-        const DbpReg d = slot ? dr[1] : dr[0];
+        const uint32_t slot = uni((uint32_t)cols[c].dslot);
+        const DbpReg<B> d = slot ? dr[1] : dr[0];
         uint64_t run = slot ? drs[1] : drs[0];
 #pragma unroll
         for (int j = 0; j < HH; ++j) {
@@ -242,38 +251,29 @@ __device__ __forceinline__ void fetch_col(const ScanShared &S, const uint8_t *ld
         }
         return;
     }
-    const uint32_t elem = uni(S.cols[c].c.elem);
-    const uint32_t sh = sext_shift(uni(S.cols[c].type), elem);
-    const uint64_t off = uni64(S.tp[c].off);
-    const uint8_t *comp = reinterpret_cast<const uint8_t *>(uni64(reinterpret_cast<uint64_t>(S.cols[c].c.comp)));
+    const uint32_t elem = uni(cols[c].c.elem);
+    const uint32_t sh = sext_shift(uni(cols[c].type), elem);
+    const uint32_t off = (uint32_t)uni64(tp[c].off);
+    B *b = base + off;
     switch (mode) {
     case TA_DIRECT: {
-        gbyte *base = gmem(comp) + off;
-        if (r0 + HH - 1 <= rmax && uni(S.tp[c].aux)) {
+        if (r0 + HH - 1 <= rmax && uni(tp[c].aux)) {
             // HH consecutive rows of a 16-byte aligned payload (r0 % HH == 0): vector loads
-            static_assert(HH == 8 || HH == 16, "vector loads of 8 or 16 rows");
-            gbyte *vb = base + (size_t)r0 * elem;
-            if (elem == 1 && HH == 8) {
-                const u32x2 x = *reinterpret_cast<gvec2 *>(vb);
+            static_assert(HH == 8, "vector loads of 8 rows");
+            const uint32_t vo = r0 * elem;
+            if (elem == 1) {
+                const u32x2 x = ldt<u32x2>(b, vo);
 #pragma unroll
                 for (int j = 0; j < HH; ++j) out[j] = sext(((j < 4 ? x.x : x.y) >> (8 * (j & 3))) & 0xffu, sh);
-            } else if (elem == 1) {
-                const u32x4 x = *reinterpret_cast<gvec4 *>(vb);
+            } else if (elem == 2) {
+                const u32x4 x = ldt<u32x4>(b, vo);
                 const uint32_t w[4] = {x.x, x.y, x.z, x.w};
 #pragma unroll
-                for (int j = 0; j < HH; ++j) out[j] = sext((w[(j >> 2) & 3] >> (8 * (j & 3))) & 0xffu, sh);
-            } else if (elem == 2) {
-#pragma unroll
-                for (int q = 0; q < HH / 8; ++q) {
-                    const u32x4 x = reinterpret_cast<gvec4 *>(vb)[q];
-                    const uint32_t w[4] = {x.x, x.y, x.z, x.w};
-#pragma unroll
-                    for (int j = 0; j < 8; ++j) out[8 * q + j] = sext((w[j >> 1] >> (16 * (j & 1))) & 0xffffu, sh);
-                }
+                for (int j = 0; j < HH; ++j) out[j] = sext((w[j >> 1] >> (16 * (j & 1))) & 0xffffu, sh);
             } else if (elem == 4) {
 #pragma unroll
-                for (int q = 0; q < HH / 4; ++q) {
-                    const u32x4 x = reinterpret_cast<gvec4 *>(vb)[q];
+                for (int q = 0; q < 2; ++q) {
+                    const u32x4 x = ldt<u32x4>(b, vo + 16 * q);
                     out[4 * q] = sext(x.x, sh);
                     out[4 * q + 1] = sext(x.y, sh);
                     out[4 * q + 2] = sext(x.z, sh);
@@ -281,116 +281,75 @@ __device__ __forceinline__ void fetch_col(const ScanShared &S, const uint8_t *ld
                 }
             } else {
 #pragma unroll
-                for (int q = 0; q < HH / 2; ++q) {
-                    const u32x4 x = reinterpret_cast<gvec4 *>(vb)[q];
+                for (int q = 0; q < 4; ++q) {
+                    const u32x4 x = ldt<u32x4>(b, vo + 16 * q);
                     out[2 * q] = (int64_t)(((uint64_t)x.y << 32) | x.x);
                     out[2 * q + 1] = (int64_t)(((uint64_t)x.w << 32) | x.z);
                 }
             }
         } else {
 #pragma unroll
-            for (int j = 0; j < HH; ++j) out[j] = sext(gld_elem(base, min(r0 + j, rmax) * elem, elem), sh);
+            for (int j = 0; j < HH; ++j) out[j] = sext(ld_elem(b, min(r0 + j, rmax) * elem, elem), sh);
         }
         break;
     }
     case TA_WINDOW: {
-        const uint8_t *base = lds + off;
+        lbyte *w = lds + off;
 #pragma unroll
-        for (int j = 0; j < HH; ++j) out[j] = sext(load_elem(base + (size_t)min(r0 + j, rmax) * elem, elem), sh);
+        for (int j = 0; j < HH; ++j) out[j] = sext(ld_elem(w, min(r0 + j, rmax) * elem, elem), sh);
         break;
     }
     case TA_FOR: {
-        const uint32_t bw = uni(S.tp[c].aux);
-        const int64_t ref = (int64_t)uni64((uint64_t)S.tp[c].a);
+        const uint32_t bw = uni(tp[c].aux);
+        const int64_t ref = (int64_t)uni64((uint64_t)tp[c].a);
         if (bw <= 32) {
             // field r at bits [r*bw, r*bw + bw): two dwords and a funnel shift (the payload is
             // followed by 8 zero bytes, so the second dword is always inside the page)
-            gbyte *w = gmem(comp) + off;
             const uint32_t mask = bw == 32 ? 0xffffffffu : ((1u << bw) - 1u);
 #pragma unroll
             for (int j = 0; j < HH; ++j) {
                 const uint32_t bit = min(r0 + j, rmax) * bw;
-                const uint32_t lo = gld32(w, (bit >> 5) << 2), hi = gld32(w, ((bit >> 5) << 2) + 4);
+                const uint32_t lo = ldt<uint32_t>(b, (bit >> 5) << 2), hi = ldt<uint32_t>(b, ((bit >> 5) << 2) + 4);
                 out[j] = ref + (int64_t)(__builtin_amdgcn_alignbit(hi, lo, bit & 31u) & mask);
             }
         } else {
-            gbyte *w = gmem(comp) + off;
             const uint64_t mask = ~0ull >> (64 - bw);
 #pragma unroll
             for (int j = 0; j < HH; ++j) {
                 const uint32_t bit = min(r0 + j, rmax) * bw;  // < 2^32: rows <= 8192, bw <= 56
                 const uint32_t s = bit & 63;
-                uint64_t x = gld64(w, (bit >> 6) << 3) >> s;
-                if (s + bw > 64) x |= gld64(w, ((bit >> 6) << 3) + 8) << (64 - s);  // the field ends in the next word
+                uint64_t x = ldt<uint64_t>(b, (bit >> 6) << 3) >> s;
+                if (s + bw > 64) x |= ldt<uint64_t>(b, ((bit >> 6) << 3) + 8) << (64 - s);  // the field ends in the next word
                 out[j] = ref + (int64_t)(x & mask);
             }
         }
         break;
     }
     case TA_AFFINE: {
-        const uint64_t a0 = uni64((uint64_t)S.tp[c].a), b0 = uni64((uint64_t)S.tp[c].b);
+        const uint64_t a0 = uni64((uint64_t)tp[c].a), b0 = uni64((uint64_t)tp[c].b);
 #pragma unroll
         for (int j = 0; j < HH; ++j) out[j] = (int64_t)(a0 + (uint64_t)(r0 + j) * b0);
         break;
     }
     default: {  // RLE: first run whose exclusive end is above the row
-        const uint8_t *pg = comp + off;
-        const uint32_t nruns = uni(S.tp[c].aux);
-        const uint64_t vbytes = uni64((uint64_t)S.tp[c].b);
+        const uint32_t nruns = uni(tp[c].aux);
+        const uint32_t vbytes = (uint32_t)uni64((uint64_t)tp[c].b);
 #pragma unroll
         for (int j = 0; j < HH; ++j) {
             const uint32_t r = min(r0 + j, rmax);
             uint32_t lo = 0, hi = nruns - 1;
             while (lo < hi) {
                 const uint32_t mid = (lo + hi) >> 1;
-                if (ld_u32_unaligned(pg + 16 + vbytes + 4ull * mid) > r) hi = mid;
+                if (ldt<uint32_t>(b, 16 + vbytes + 4 * mid) > r) hi = mid;
                 else lo = mid + 1;
             }
             uint64_t v = 0;
-            for (uint32_t b = 0; b < elem; ++b) v |= (uint64_t)pg[16 + (uint64_t)lo * elem + b] << (8 * b);
+            for (uint32_t k = 0; k < elem; ++k) v |= (uint64_t)ldt<uint8_t>(b, 16 + lo * elem + k) << (8 * k);
             out[j] = sext(v, sh);
         }
         break;
     }
     }
-}
-
-// Decodes a delta-bitpack page (rows rows of elem bytes) into the LDS window with the whole
-// block: thread t sums the deltas of rows [t*k, t*k + k), a block scan of the per-thread totals
-// gives every thread its start value. Uniform; false if the header does not fit.
-__device__ __forceinline__ bool dbp_decode_block(const uint8_t *src, uint32_t csize, uint8_t *dst, uint32_t rows, uint32_t elem,
-                                 ScanShared &S) {
-    const uint32_t n = ld_u32_unaligned(src + 16), bw = src[20], e = src[21];
-    const uint64_t words = n ? (((uint64_t)(n - 1) * bw + 31) / 32) : 0;
-    if (csize < 32 || n != rows || e != elem || bw > 32 || 32 + words * 4 > csize ||
-        (reinterpret_cast<uintptr_t>(src) & 3))
-        return false;
-    const uint64_t base = ld_u64_unaligned(src), mind = ld_u64_unaligned(src + 8);
-    const uint32_t *pk = reinterpret_cast<const uint32_t *>(src + 32);
-    const uint64_t fmask = bw == 32 ? 0xffffffffull : ((1ull << bw) - 1);
-    const uint32_t k = (rows + BLOCK - 1) / BLOCK, r0 = threadIdx.x * k, r1 = min(rows, r0 + k);
-    auto delta = [&](uint32_t i) -> uint64_t {  // v[i] - v[i-1], i >= 1
-        uint64_t p = 0;
-        if (bw) {
-            const uint64_t bit = (uint64_t)(i - 1) * bw, w = bit >> 5;
-            const uint64_t lo = pk[w], hi = (w + 1 < words) ? pk[w + 1] : 0;
-            p = ((lo | (hi << 32)) >> (bit & 31)) & fmask;
-        }
-        return mind + p;
-    };
-    uint64_t tot = 0;
-    for (uint32_t i = max(r0, 1u); i < r1; ++i) tot += delta(i);
-    const uint64_t incl = wave_incl_scan_u64(tot);
-    const uint32_t wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    if (lane == WAVE - 1) S.wtot[0][wave] = incl;
-    __syncthreads();
-    uint64_t run = base + incl - tot;
-    for (uint32_t w = 0; w < wave; ++w) run += S.wtot[0][w];
-    for (uint32_t i = r0; i < r1; ++i) {
-        if (i > 0) run += delta(i);
-        store_elem(dst + (size_t)i * elem, run, elem);
-    }
-    return true;
 }
 
 // Predicate P on chunk c: 0 fails for every row, 1 holds for every row, 2 must be tested.
@@ -454,6 +413,7 @@ __device__ void item_query(const ScanQueryDev &Q, uint64_t c, uint32_t agg_entri
     o.np = np;
     o.pcol = pcol;
     uint32_t nk = 0, kcol = 0, cg = 0;
+    bool shareable = true;
     for (uint32_t i = 0; i < Q.n_keys; ++i) {
         const ScanKeyDev &K = Q.key[i];
         uint32_t klo, khi;
@@ -464,6 +424,8 @@ __device__ void item_query(const ScanQueryDev &Q, uint64_t c, uint32_t agg_entri
             continue;
         }
         o.kbase[nk] = K.base;
+        o.korg[nk] = K.base + (int64_t)klo * (int64_t)(K.span / K.card);
+        shareable &= inside;
         o.kspan[nk] = (uint32_t)K.span;
         o.km[nk] = K.div.m;
         o.ks[nk] = K.div.s1 | (K.div.s2 << 8);
@@ -485,6 +447,37 @@ __device__ void item_query(const ScanQueryDev &Q, uint64_t c, uint32_t agg_entri
     uint32_t L = 0;
     (void)item_domain(Q, c, agg_entries, &L);
     o.L = L;
+    o.shareable = shareable && L;
+    // FNV-1a over the signature fields
+    uint32_t hsh = 2166136261u;
+    auto mix = [&](uint64_t v) {
+        for (int b = 0; b < 8; ++b) hsh = (hsh ^ (uint32_t)((v >> (8 * b)) & 0xffu)) * 16777619u;
+    };
+    mix(np | ((uint64_t)nk << 8) | ((uint64_t)(uint32_t)o.vcol << 16) | ((uint64_t)L << 32));
+    mix(pcol | ((uint64_t)kcol << 32));
+    for (uint32_t i = 0; i < np; ++i) {
+        mix((uint64_t)o.plo[i]);
+        mix((uint64_t)o.phi[i]);
+    }
+    for (uint32_t i = 0; i < nk; ++i) {
+        mix((uint64_t)o.korg[i]);
+        mix(o.km[i] | ((uint64_t)o.ks[i] << 32));
+        mix(o.lcard[i]);
+    }
+    o.sig = hsh;
+}
+
+// Whether plans a and b (both shareable) evaluate every row of the chunk identically.
+__device__ bool same_rows(const ItemQuery &x, const ItemQuery &y) {
+    if (x.sig != y.sig || x.np != y.np || x.nk != y.nk || x.pcol != y.pcol || x.kcol != y.kcol || x.vcol != y.vcol ||
+        x.L != y.L)
+        return false;
+    for (uint32_t i = 0; i < x.np; ++i)
+        if (x.plo[i] != y.plo[i] || x.phi[i] != y.phi[i]) return false;
+    for (uint32_t i = 0; i < x.nk; ++i)
+        if (x.korg[i] != y.korg[i] || x.km[i] != y.km[i] || x.ks[i] != y.ks[i] || x.lcard[i] != y.lcard[i])
+            return false;
+    return true;
 }
 
 // Adds the sum and count of rows of one group to dense partial e, or to the global result g.
@@ -498,126 +491,88 @@ __device__ __forceinline__ void agg_global(const ItemQuery &Q, uint32_t g, int64
     atomicAdd(reinterpret_cast<unsigned long long *>(&Q.g_cnt[g]), (unsigned long long)n);
 }
 
-// One page of the item for the plans of the current pass: column plans and window decodes of
-// the columns they read, then the page in sub-tiles of BLOCK * HH rows (thread t owns HH
-// consecutive rows of each); delta-bitpack columns with a register slot are decoded by running
-// sums whose start values come from a block scan of the per-thread delta totals.
-constexpr int HH = 8;
-constexpr uint32_t SUBTILE = BLOCK * HH;
-__device__ __forceinline__ void scan_page(const ScanArgs &a, ScanShared &S, uint64_t p, uint32_t used, uint32_t npass,
-                                          uint8_t *lds, unsigned long long *lsum, uint32_t *lcnt) {
+// Page p of the item for the plans of the pass, processed by one wave alone: column plans
+// (wave-private), window decodes into the wave's windows (wlds), delta-bitpack start values
+// from wave prefix scans, then plan by plan and column use by column use over the lane's HH
+// rows of every sub-tile: the plan's and the column's parameters stay uniform across them,
+// every use issues HH independent loads, and rows are carried between uses as bit masks (pass)
+// and partial group indexes (g global, e local).
+__device__ void wave_page(const ScanArgs &a, ScanShared &S, uint64_t p, uint32_t used, uint32_t npass, uint8_t *wlds,
+                          unsigned long long *lsum, uint32_t *lcnt) {
     const uint32_t wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     const uint32_t rows = (uint32_t)min<uint64_t>(a.page_rows, a.n_rows - p * a.page_rows);
-    if (threadIdx.x == 0) S.tile_bad = 0;
-    if (threadIdx.x < a.n_cols && ((used >> threadIdx.x) & 1u)) S.tp[threadIdx.x] = tcol_plan(S.cols[threadIdx.x], p, rows);
-    __syncthreads();
+    TPlan *tp = S.tp[wave];
+    if (lane < a.n_cols && ((used >> lane) & 1u)) {
+        const ScanColView &v = S.cols[lane];
+        const uint64_t off = v.c.page_off[p];
+        tp[lane] = plan_page(v, v.c.comp + off, off, v.c.page_codec[p], v.c.page_csize[p], rows);
+    }
+    // the wave's own LDS writes are visible to its later LDS reads (in order within a wave)
     bool bad = false;
-    for (uint32_t c = 0; c < a.n_cols; ++c) bad |= ((used >> c) & 1u) && S.tp[c].mode == TA_BAD;
-    if (bad) {  // uniform
-        if (threadIdx.x == 0) raise_err(a.err, DEVERR_CORRUPT);
-        __syncthreads();
+    for (uint32_t c = 0; c < a.n_cols; ++c) bad |= ((used >> c) & 1u) && tp[c].mode == TA_BAD;
+    if (bad) {  // wave-uniform
+        if (lane == 0) raise_err(a.err, DEVERR_CORRUPT);
         return;
     }
-    // coded pages without a register slot: delta pages by the whole block, others by one wave each
-    bool any_wave = false, any_window = false;
     for (uint32_t c = 0; c < a.n_cols; ++c) {
-        if (!((used >> c) & 1u) || S.tp[c].mode != TA_WINDOW) continue;
-        any_window = true;
+        if (!((used >> c) & 1u) || tp[c].mode != TA_WINDOW) continue;
         const ColView &cv = S.cols[c].c;
-        if ((cv.page_codec[p] & PAGE_CODEC_MASK) == golap_fmt::DBP) {
-            if (!dbp_decode_block(cv.comp + cv.page_off[p], cv.page_csize[p], lds + S.tp[c].off, rows, cv.elem, S) &&
-                threadIdx.x == 0) {
-                S.tile_bad = 1;
-                raise_err(a.err, DEVERR_CORRUPT);
-            }
-            __syncthreads();  // wtot is reused by the next block decode
-        } else {
-            any_wave = true;
+        const uint32_t want = rows * cv.elem;
+        const int64_t got = page_decode_wave(cv.page_codec[p], cv.comp + cv.page_off[p], cv.page_csize[p],
+                                             wlds + tp[c].off, want, cv.elem);
+        if (got != (int64_t)want) {  // wave-uniform
+            if (lane == 0) raise_err(a.err, DEVERR_CORRUPT);
+            return;
         }
-    }
-    if (any_wave) {
-        uint32_t k = 0;
-        for (uint32_t c = 0; c < a.n_cols; ++c) {
-            if (!((used >> c) & 1u) || S.tp[c].mode != TA_WINDOW) continue;
-            const ColView &cv = S.cols[c].c;
-            if ((cv.page_codec[p] & PAGE_CODEC_MASK) == golap_fmt::DBP) continue;
-            if (k++ % (BLOCK / WAVE) != wave) continue;
-            const uint32_t want = rows * cv.elem;
-            const int64_t got = page_decode_wave(cv.page_codec[p], cv.comp + cv.page_off[p], cv.page_csize[p],
-                                                 lds + S.tp[c].off, want, cv.elem);
-            if (got != (int64_t)want && lane == 0) {
-                S.tile_bad = 1;
-                raise_err(a.err, DEVERR_CORRUPT);
-            }
-        }
-    }
-    if (any_window) __syncthreads();
-    if (S.tile_bad) {  // uniform (read after the barrier)
-        __syncthreads();
-        return;
     }
     // register-decoded delta columns of the page
-    DbpReg dr[SCAN_DBP_SLOTS];
+    DbpReg<gbyte> dr[SCAN_DBP_SLOTS];
     uint64_t carry[SCAN_DBP_SLOTS];  // value before the sub-tile's first row (uniform)
     uint32_t dmask = 0;
 #pragma unroll
     for (uint32_t sl = 0; sl < SCAN_DBP_SLOTS; ++sl) {
-        dr[sl] = DbpReg{nullptr, 0, 0, 0, 0};
+        dr[sl] = DbpReg<gbyte>{nullptr, 0, 0, 0, 0};
         carry[sl] = 0;
     }
     for (uint32_t c = 0; c < a.n_cols; ++c) {
-        if (!((used >> c) & 1u) || uni(S.tp[c].mode) != TA_DBP) continue;
+        if (!((used >> c) & 1u) || uni(tp[c].mode) != TA_DBP) continue;
         const uint32_t sl = uni((uint32_t)S.cols[c].dslot);
-        const uint8_t *comp = reinterpret_cast<const uint8_t *>(uni64(reinterpret_cast<uint64_t>(S.cols[c].c.comp)));
-        DbpReg d;
-        d.pk = reinterpret_cast<const __attribute__((address_space(1))) uint32_t *>(gmem(comp) + uni64(S.tp[c].off) + 32);
-        d.bw = uni(S.tp[c].aux);
-        d.mind = uni64((uint64_t)S.tp[c].b);
+        DbpReg<gbyte> d;
+        d.pk = gmem(reinterpret_cast<const uint8_t *>(uni64(reinterpret_cast<uint64_t>(S.cols[c].c.comp)))) +
+               uni64(tp[c].off) + 32;
+        d.bw = uni(tp[c].aux);
+        d.mind = uni64((uint64_t)tp[c].b);
         d.fmask = d.bw == 32 ? 0xffffffffull : ((1ull << d.bw) - 1);
         d.words = rows ? (((uint64_t)(rows - 1) * d.bw + 31) / 32) : 0;
         if (sl == 0) {
             dr[0] = d;
-            carry[0] = uni64((uint64_t)S.tp[c].a);
+            carry[0] = uni64((uint64_t)tp[c].a);
         } else {
             dr[1] = d;
-            carry[1] = uni64((uint64_t)S.tp[c].a);
+            carry[1] = uni64((uint64_t)tp[c].a);
         }
         dmask |= 1u << sl;
     }
+    auto col_base = [&](uint32_t c) -> gbyte * {
+        return gmem(reinterpret_cast<const uint8_t *>(uni64(reinterpret_cast<uint64_t>(S.cols[c].c.comp))));
+    };
     const uint32_t rmax = rows - 1;
     for (uint32_t sub = 0; sub * SUBTILE < rows; ++sub) {
-        const uint32_t row0 = sub * SUBTILE + threadIdx.x * HH, row1 = min(rows, row0 + HH);
+        const uint32_t row0 = sub * SUBTILE + lane * HH, row1 = min(rows, row0 + HH);
         uint64_t run[SCAN_DBP_SLOTS];
 #pragma unroll
-        for (uint32_t sl = 0; sl < SCAN_DBP_SLOTS; ++sl) run[sl] = carry[sl];
-        if (dmask) {  // uniform
-            uint64_t tot[SCAN_DBP_SLOTS], incl[SCAN_DBP_SLOTS];
-#pragma unroll
-            for (uint32_t sl = 0; sl < SCAN_DBP_SLOTS; ++sl) {
-                tot[sl] = incl[sl] = 0;
-                if (!((dmask >> sl) & 1u)) continue;
-                for (uint32_t i = max(row0, 1u); i < row1; ++i) tot[sl] += dbp_delta(dr[sl], i);
-                incl[sl] = wave_incl_scan_u64(tot[sl]);
-                if (lane == WAVE - 1) S.wtot[sl][wave] = incl[sl];
-            }
-            __syncthreads();
-#pragma unroll
-            for (uint32_t sl = 0; sl < SCAN_DBP_SLOTS; ++sl) {
-                if (!((dmask >> sl) & 1u)) continue;
-                run[sl] += incl[sl] - tot[sl];
-                for (uint32_t w = 0; w < BLOCK / WAVE; ++w) {
-                    if (w < wave) run[sl] += S.wtot[sl][w];
-                    carry[sl] += S.wtot[sl][w];
-                }
-            }
-            __syncthreads();  // wtot is rewritten by the next sub-tile
+        for (uint32_t sl = 0; sl < SCAN_DBP_SLOTS; ++sl) {
+            run[sl] = carry[sl];
+            if (!((dmask >> sl) & 1u)) continue;  // uniform
+            uint64_t tot = 0;
+            for (uint32_t i = max(row0, 1u); i < row1; ++i) tot += dbp_delta(dr[sl], i);
+            const uint64_t incl = wave_incl_scan_u64(tot);
+            run[sl] += incl - tot;
+            carry[sl] += uni64(__shfl(incl, WAVE - 1, WAVE));
         }
-        // Plan by plan, column use by column use over the thread's HH rows: the plan's and the
-        // column's parameters stay uniform across them, every use issues HH independent loads,
-        // and rows are carried between uses as bit masks (pass) and partial group indexes (g
-        // global, e local).
         for (uint32_t k = 0; k < npass; ++k) {
             const ItemQuery &Q = S.iq[k];
+            if (uni(Q.leader) != k) continue;  // adds into its leader's partials
             const uint32_t np = uni(Q.np), nk = uni(Q.nk), pcol = uni(Q.pcol), kcol = uni(Q.kcol);
             const int32_t vc = (int32_t)uni((uint32_t)Q.vcol);
             const bool with_sum = vc >= 0;
@@ -641,7 +596,7 @@ __device__ __forceinline__ void scan_page(const ScanArgs &a, ScanShared &S, uint
                 if (!last || with_sum) {
                     const uint32_t c = u < np ? (pcol >> (8 * u)) & 0xffu
                                               : u < np + nk ? (kcol >> (8 * (u - np))) & 0xffu : (uint32_t)vc;
-                    fetch_col<HH>(S, lds, c, row0, rmax, x, dr, run);
+                    fetch_col<gbyte>(tp, S.cols, col_base(c), (lbyte *)wlds, c, row0, rmax, x, dr, run);
                 } else {
 #pragma unroll
                     for (int j = 0; j < HH; ++j) x[j] = 0;
@@ -653,27 +608,28 @@ __device__ __forceinline__ void scan_page(const ScanArgs &a, ScanShared &S, uint
                         if (x[j] < lo || x[j] > hi) pass &= ~(1u << j);
                 } else if (!last) {
                     const uint32_t i = u - np;
-                    const uint64_t base = uni64((uint64_t)Q.kbase[i]);
+                    const uint64_t kb = uni64((uint64_t)Q.kbase[i]);
                     const uint32_t span = uni(Q.kspan[i]), gs = uni(Q.kgs[i]), klo = uni(Q.klo[i]);
                     const uint32_t lcard = uni(Q.lcard[i]), ls = uni(Q.lstride[i]), ks = uni(Q.ks[i]);
                     golap_fmt::UDiv32 dvz;
                     dvz.m = uni(Q.km[i]);
                     dvz.s1 = ks & 0xffu;
                     dvz.s2 = ks >> 8;
-                    uint32_t bad = 0;
+                    const bool div1 = dvz.m == 1 && ks == 0;
+                    uint32_t bad_rows = 0;
 #pragma unroll
                     for (int j = 0; j < HH; ++j) {
-                        const uint64_t uo = (uint64_t)x[j] - base;
-                        bad |= (uint32_t)(uo >= span) << j;
-                        const uint32_t kk = golap_fmt::udiv32((uint32_t)uo, dvz);
+                        const uint64_t uo = (uint64_t)x[j] - kb;
+                        bad_rows |= (uint32_t)(uo >= span) << j;
+                        const uint32_t kk = div1 ? (uint32_t)uo : golap_fmt::udiv32((uint32_t)uo, dvz);
                         g[j] += kk * gs;
                         const uint32_t lk = kk - klo;
                         inl &= ~((uint32_t)(lk >= lcard) << j);
                         e[j] += lk * ls;
                     }
-                    if (bad & pass) {  // a qualifying row's key outside its declared range
+                    if (bad_rows & pass) {  // a qualifying row's key outside its declared range
                         raise_err(a.err, DEVERR_KEYRANGE);
-                        pass &= ~bad;
+                        pass &= ~bad_rows;
                     }
                 } else {
                     // aggregate the passing rows: x holds their values
@@ -702,6 +658,9 @@ __device__ __forceinline__ void scan_page(const ScanArgs &a, ScanShared &S, uint
                     }
                     if (lbase != NO_LDS) {
                         if (__ballot(outside != 0)) {
+                            // zone maps bound every row's key; a plan that shares its partials
+                            // cannot send rows elsewhere
+                            if (uni(Q.members) && lane == 0) raise_err(a.err, DEVERR_CORRUPT);
 #pragma unroll
                             for (int j = 0; j < HH; ++j)
                                 if ((outside >> j) & 1u) agg_global(Q, g[j], x[j], 1u, with_sum);
@@ -731,7 +690,6 @@ __device__ __forceinline__ void scan_page(const ScanArgs &a, ScanShared &S, uint
             }
         }
     }
-    __syncthreads();
 }
 
 // Moves the pass's dense partials into their plans' global results.
@@ -758,13 +716,14 @@ __device__ __forceinline__ void pass_flush(ScanShared &S, uint32_t npass, const 
 }
 
 // Persistent shared-scan kernel: workgroups take (chunk part, plan word, plan mask) items with
-// one atomic each.
-__global__ __launch_bounds__(BLOCK, 4) void k_scan(ScanArgs a) {
+// one atomic each; the waves share the item's dense partials and split its pages.
+__global__ __launch_bounds__(BLOCK, 2) void k_scan(ScanArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     __shared__ ScanShared S;
-    unsigned long long *lsum = reinterpret_cast<unsigned long long *>(lds + a.win_bytes);
-    uint32_t *lcnt = reinterpret_cast<uint32_t *>(lds + a.win_bytes + (size_t)a.agg_entries * 8);
-    const uint32_t lane = threadIdx.x % WAVE;
+    const uint32_t wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    uint8_t *wlds = lds + wave * a.win_bytes;
+    unsigned long long *lsum = reinterpret_cast<unsigned long long *>(lds + a.page_waves * a.win_bytes);
+    uint32_t *lcnt = reinterpret_cast<uint32_t *>(lds + a.page_waves * a.win_bytes + (size_t)a.agg_entries * 8);
 #pragma unroll
     for (uint32_t i = 0; i < SCAN_MAX_COLS; ++i)
         if (threadIdx.x == i) S.cols[i] = a.col[i];
@@ -799,34 +758,47 @@ __global__ __launch_bounds__(BLOCK, 4) void k_scan(ScanArgs a) {
         const uint64_t pb = c0 + (uint64_t)part * a.pages_per_item;
         const uint64_t pe = min(min(pb + a.pages_per_item, c0 + a.pages_per_chunk), a.n_pages);
         for (uint32_t k0 = 0; k0 < nlive;) {
-            // a pass: the next plans, up to PASS_MAX, whose partials fit the LDS array together
+            // a pass: the views of the next plans, up to PASS_MAX; plans that share a leader's
+            // partials take no LDS, the others are admitted while their partials fit
+            const uint32_t nview = min(nlive - k0, PASS_MAX);
+            if (threadIdx.x < nview) item_query(a.queries[S.qidx[k0 + threadIdx.x]], c, a.agg_entries, S.iq[threadIdx.x]);
+            __syncthreads();
             if (threadIdx.x == 0) {
-                uint32_t k1 = k0, used = 0;
-                while (k1 < nlive && k1 - k0 < PASS_MAX) {
-                    const uint32_t L = S.qlen[k1];
-                    if (L && used + L > a.agg_entries) break;
-                    S.lbase[k1 - k0] = L ? used : NO_LDS;
-                    used += L;
-                    ++k1;
+                uint32_t k = 0, used = 0;
+                for (; k < nview; ++k) {
+                    ItemQuery &Q = S.iq[k];
+                    Q.leader = k;
+                    Q.members = 0;
+                    if (Q.shareable)
+                        for (uint32_t j = 0; j < k; ++j)
+                            if (S.iq[j].leader == j && S.iq[j].shareable && same_rows(S.iq[j], Q)) {
+                                Q.leader = j;
+                                break;
+                            }
+                    if (Q.leader != k) {
+                        Q.lbase = S.iq[Q.leader].lbase;
+                        S.iq[Q.leader].members = 1;
+                        continue;
+                    }
+                    if (Q.L && used + Q.L > a.agg_entries) break;
+                    Q.lbase = Q.L ? used : NO_LDS;
+                    used += Q.L;
                 }
-                S.pass_end = k1;
+                S.pass_end = k0 + max(k, 1u);
                 S.pass_used = used;
             }
             __syncthreads();
             const uint32_t k1 = S.pass_end, npass = k1 - k0, entries = S.pass_used;
-            if (threadIdx.x < npass) {
-                ItemQuery &o = S.iq[threadIdx.x];
-                item_query(a.queries[S.qidx[k0 + threadIdx.x]], c, a.agg_entries, o);
-                o.lbase = S.lbase[threadIdx.x];
-            }
             for (uint32_t e = threadIdx.x; e < entries; e += BLOCK) {
                 lsum[e] = 0;
                 lcnt[e] = 0;
             }
             __syncthreads();
             uint32_t used = 0;
-            for (uint32_t k = 0; k < npass; ++k) used |= S.iq[k].colmask;
-            for (uint64_t p = pb; p < pe; ++p) scan_page(a, S, p, used, npass, lds, lsum, lcnt);
+            for (uint32_t k = 0; k < npass; ++k)
+                if (S.iq[k].leader == k) used |= S.iq[k].colmask;
+            if (wave < a.page_waves)
+                for (uint64_t p = pb + wave; p < pe; p += a.page_waves) wave_page(a, S, p, used, npass, wlds, lsum, lcnt);
             pass_flush(S, npass, lsum, lcnt);
             k0 = k1;
         }
@@ -868,33 +840,44 @@ __global__ __launch_bounds__(BLOCK) void k_scan_items(const ScanQueryDev *qs, ui
     }
 }
 
-template <typename K>
-hipError_t launch_scan_t(golap_ctx *ctx, K kernel, const ScanArgs &a, size_t lds, hipStream_t stream,
-                         uint32_t max_blocks_per_cu, bool prepare_only) {
+hipError_t launch_scan_t(golap_ctx *ctx, const ScanArgs &a, size_t lds, hipStream_t stream, uint32_t max_blocks_per_cu,
+                         bool prepare_only) {
     if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_scan),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
     int occ = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, BLOCK, lds);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_scan, BLOCK, lds);
     if (e != hipSuccess) return e;
     occ = std::max(occ, 1);
     if (max_blocks_per_cu) occ = std::min<int>(occ, (int)max_blocks_per_cu);
     const uint64_t grid = std::max<uint64_t>(1, (uint64_t)ctx->cus * occ);
     if (prepare_only) return hipSuccess;
-    hipLaunchKernelGGL(kernel, dim3((uint32_t)grid), dim3(BLOCK), lds, stream, a);
+    hipLaunchKernelGGL(k_scan, dim3((uint32_t)grid), dim3(BLOCK), lds, stream, a);
     return hipGetLastError();
 }
 
 }  // namespace
 
-size_t scan_lds_bytes(uint32_t win_bytes, uint32_t agg_entries) { return (size_t)win_bytes + (size_t)agg_entries * 12; }
+size_t scan_lds_budget(golap_ctx *ctx) {
+    static size_t static_lds = ~(size_t)0;  // the kernel's own LDS (ScanShared), the same on every device
+    if (static_lds == ~(size_t)0) {
+        hipFuncAttributes fa{};
+        static_lds = hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(k_scan)) == hipSuccess ? fa.sharedSizeBytes
+                                                                                                      : 16384;
+    }
+    return (size_t)ctx->lds_max > static_lds ? (size_t)ctx->lds_max - static_lds : 0;
+}
+
+size_t scan_lds_bytes(uint32_t win_bytes, uint32_t agg_entries, uint32_t page_waves) {
+    return (size_t)page_waves * win_bytes + (size_t)agg_entries * 12;
+}
 
 hipError_t launch_scan(golap_ctx *ctx, const ScanLaunch &l, hipStream_t stream, bool prepare_only) {
     if (l.n_cols > SCAN_MAX_COLS || !l.queries || !l.n_queries || !l.items || !l.queue || !l.pages_per_item ||
-        l.pages_per_item > l.pages_per_chunk || l.win_bytes % 16 || l.agg_entries > SCAN_AGG_MAX ||
-        l.page_rows > 32 * BLOCK)
+        l.pages_per_item > l.pages_per_chunk || l.win_bytes % 16 || l.agg_entries > SCAN_AGG_MAX || l.page_rows > 8192 ||
+        !l.page_waves || l.page_waves > NWAVE)
         return hipErrorInvalidValue;
     ScanArgs a{};
     for (uint32_t c = 0; c < l.n_cols; ++c) a.col[c] = l.col[c];
@@ -905,16 +888,16 @@ hipError_t launch_scan(golap_ctx *ctx, const ScanLaunch &l, hipStream_t stream, 
     a.n_pages = l.n_pages;
     a.win_bytes = l.win_bytes;
     a.agg_entries = l.agg_entries;
+    a.page_waves = l.page_waves;
     a.queries = l.queries;
     a.n_queries = l.n_queries;
     a.items = l.items;
     a.queue = l.queue;
     a.pages_per_item = l.pages_per_item;
     a.err = l.err;
-    const size_t lds = scan_lds_bytes(l.win_bytes, l.agg_entries);
-    if (lds > (size_t)ctx->lds_max) return hipErrorInvalidConfiguration;
-    const uint32_t mb = l.max_blocks_per_cu;
-    return launch_scan_t(ctx, k_scan, a, lds, stream, mb, prepare_only);
+    const size_t lds = scan_lds_bytes(l.win_bytes, l.agg_entries, l.page_waves);
+    if (lds > scan_lds_budget(ctx)) return hipErrorInvalidConfiguration;
+    return launch_scan_t(ctx, a, lds, stream, l.max_blocks_per_cu, prepare_only);
 }
 
 hipError_t launch_scan_items(const ScanQueryDev *queries, uint32_t n_queries, uint64_t c0, uint64_t nch,

@@ -64,7 +64,8 @@ std::vector<uint8_t> scan_survivors(const golap_scan_plan &p, const std::vector<
 void scan_stats(const golap_scan_plan &p, uint32_t colmask, const std::vector<uint8_t> &run, golap_star_result *res);
 uint32_t scan_layout(const golap_column *const *cols, uint32_t n_cols, uint32_t colmask, uint64_t pb, uint64_t pe,
                      int32_t *woff, int32_t *dslot);
-uint32_t scan_agg_entries(golap_ctx *ctx, uint32_t agg_mode, uint32_t requested, uint32_t win_bytes);
+uint32_t scan_page_waves(golap_ctx *ctx, uint32_t win_bytes, uint32_t agg_entries);
+uint32_t scan_agg_entries(golap_ctx *ctx, uint32_t agg_mode, uint32_t requested, uint32_t win_bytes, uint32_t page_waves);
 uint32_t scan_pages_per_item(golap_ctx *ctx, uint64_t surviving_chunks, uint32_t ppc);
 
 }  // namespace starhost
