@@ -366,15 +366,11 @@ def run_batch(args):
 TRIP_HOURS = 8784
 
 
-def trip_request(rng, t0):
-    """One of four request templates over a time range of log-uniform length (1 hour .. 1 year):
-    0 SUM(fare), 1 passengers >= 2 SUM(distance), 2 fare in [1000, 5000] SUM(distance),
-    3 distance >= 5000 COUNT by (passenger_count, zone); the others group by zone x hour of
-    the range (up to 265 x 8,784 = 2,327,760 groups). Columns: 0 ts, 1 passenger, 2 zone,
-    3 fare, 4 distance."""
-    t = int(rng.integers(0, 4))
-    span = int(np.exp(rng.uniform(0, np.log(TRIP_HOURS))))
-    h0 = int(rng.integers(0, TRIP_HOURS - span + 1))
+def trip_template(t, h0, span, t0):
+    """Request template t over hours [h0, h0 + span) of the year: 0 SUM(fare), 1 passengers >= 2
+    SUM(distance), 2 fare in [1000, 5000] SUM(distance), 3 distance >= 5000 COUNT by
+    (passenger_count, zone); the others group by zone x hour of the range (up to 265 x 8,784 =
+    2,327,760 groups). Columns: 0 ts, 1 passenger, 2 zone, 3 fare, 4 distance."""
     lo, hi = t0 + h0 * 3600, t0 + (h0 + span) * 3600 - 1
     zone, hour = (2, 1, 1, 265, 1), (0, t0 + h0 * 3600, 3600, span, h0)
     if t == 0:
@@ -384,6 +380,15 @@ def trip_request(rng, t0):
     if t == 2:
         return dict(preds=[(0, lo, hi), (3, 1000, 5000)], keys=[hour, zone], value_col=4)
     return dict(preds=[(0, lo, hi), (4, 5000, 1 << 30)], keys=[(1, 1, 1, 6, 1), zone], value_col=-1)
+
+
+def trip_request(rng, t0):
+    """A seeded request: a uniform template over a time range of log-uniform length
+    (1 hour .. 1 year)."""
+    t = int(rng.integers(0, 4))
+    span = int(np.exp(rng.uniform(0, np.log(TRIP_HOURS))))
+    h0 = int(rng.integers(0, TRIP_HOURS - span + 1))
+    return trip_template(t, h0, span, t0)
 
 
 def run_serve(args):
@@ -404,6 +409,8 @@ def run_serve(args):
     hosts = []
     for c, k in zip(cols, codecs):
         hosts.append(g.HostColumn.encode(c, codec=k, **enc))
+    cpu_rows = 0 if args.no_cpu_baseline else min(n, 150_000_000)
+    cpu_cols = [c[:cpu_rows].copy() for c in cols]  # the CPU baseline's bounded sample
     del cols
     ctx = g.Context(0)
     dev = [ctx.upload(h) for h in hosts]
@@ -457,8 +464,10 @@ def run_serve(args):
                                                 g._ptr(b[2]), doms[i], C.byref(lat)), "server_wait")
                 lat_srv[i] = lat.value + sub_delay[i]
                 res_srv[i] = (r.n_groups, r.total_sum, r.rows_matched)
+                chunks_srv[i] = r.chunks_run
 
         sub_delay = [0.0] * len(reqs)
+        chunks_srv = [0] * len(reqs)
         nw = 8
         ws = [threading.Thread(target=waiter, args=(w, nw)) for w in range(nw)]
         for w in ws:
@@ -507,6 +516,31 @@ def run_serve(args):
            "roofline": {"bound": "hbm", "achieved": gbps, "peak": 8000.0, "unit": "GB/s", "frac": gbps / 8000.0,
                         "traffic": traffic_of("serve_trip_1.5B")}}
     calibrate(ctx, args)
+    if cpu_rows:
+        # the four templates over the whole year on the first cpu_rows rows (every row passes the
+        # time filter), median of 3, scaled to the mix's mean scanned rows per request
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle
+        threads = len(os.sched_getaffinity(0))
+        per_t = []
+        for t in range(4):
+            q = trip_template(t, 0, TRIP_HOURS, g.TRIP_T0)
+            preds, keys = q["preds"], q["keys"]
+            ts = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                w = oracle.scan_groupby(cpu_cols, preds, keys, q["value_col"], threads=threads)
+                ts.append(time.perf_counter() - t0)
+                assert w is not None
+            per_t.append(statistics.median(ts))
+        rows_s = 4 * cpu_rows / sum(per_t)
+        rows_per_req = statistics.mean(chunks_srv) * args.zone_rows
+        out["cpu_baseline"] = {"value": rows_s / rows_per_req, "unit": "queries/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"the four request templates over the first {cpu_rows} decoded rows "
+                                         f"({rows_s:.3g} rows/s, median of 3), scaled to this run's mean "
+                                         f"{rows_per_req:.3g} scanned rows per request"}
+    del cpu_cols
     print(json.dumps(out), flush=True)
     for d in dev:
         ctx.free_column(d)
