@@ -323,7 +323,9 @@ int golap_column_upload(golap_ctx *ctx, const golap_host_column *h, golap_column
     c->chunk_min = h->chunk_min;
     c->chunk_max = h->chunk_max;
     void *p[6];
-    const uint64_t sz[6] = {h->comp.size(), h->n_pages * 8, h->n_pages * 4, h->n_pages,
+    // the page data is followed by COMP_TAIL zero bytes: kernels read packed fields as dword
+    // pairs, whose second dword may lie past the last page
+    const uint64_t sz[6] = {h->comp.size() + COMP_TAIL, h->n_pages * 8, h->n_pages * 4, h->n_pages,
                             h->n_chunks * 8, h->n_chunks * 8};
     const void *src[6] = {h->comp.data(), h->page_off.data(), h->page_csize.data(),
                           h->page_codec.data(), h->chunk_min.data(), h->chunk_max.data()};
@@ -332,12 +334,12 @@ int golap_column_upload(golap_ctx *ctx, const golap_host_column *h, golap_column
             for (int j = 0; j < i; ++j) dev_release(ctx, p[j]);
             return r;
         }
-        if (sz[i]) {
-            hipError_t e = hipMemcpyAsync(p[i], src[i], sz[i], hipMemcpyHostToDevice, ctx->stream);
-            if (e != hipSuccess) {
-                for (int j = 0; j <= i; ++j) dev_release(ctx, p[j]);
-                return fail(ctx, GOLAP_EHIP, std::string("upload: ") + hipGetErrorString(e));
-            }
+        const uint64_t n = i == 0 ? sz[i] - COMP_TAIL : sz[i];
+        hipError_t e = i == 0 ? hipMemsetAsync(static_cast<uint8_t *>(p[i]) + n, 0, COMP_TAIL, ctx->stream) : hipSuccess;
+        if (e == hipSuccess && n) e = hipMemcpyAsync(p[i], src[i], n, hipMemcpyHostToDevice, ctx->stream);
+        if (e != hipSuccess) {
+            for (int j = 0; j <= i; ++j) dev_release(ctx, p[j]);
+            return fail(ctx, GOLAP_EHIP, std::string("upload: ") + hipGetErrorString(e));
         }
     }
     GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));

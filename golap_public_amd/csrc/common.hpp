@@ -242,10 +242,12 @@ hipError_t launch_hash_filter(golap_ctx *ctx, const uint64_t *slots, uint32_t lo
 
 // ---- scan + group-by over typed columns (trip-table plans)
 constexpr uint32_t SCAN_MAX_COLS = 8, SCAN_MAX_PREDS = 4, SCAN_MAX_KEYS = 3;
+constexpr uint64_t COMP_TAIL = 16;  // zero bytes after a device column's page data
 
 // Delta-bitpack pages of up to SCAN_DBP_SLOTS columns of a launch are decoded in registers (a
-// running sum per thread); other coded pages go through an LDS window.
-constexpr uint32_t SCAN_DBP_SLOTS = 2;
+// running sum per thread); other coded pages, and delta pages of further columns, go through an
+// LDS window.
+constexpr uint32_t SCAN_DBP_SLOTS = 1;
 struct ScanColView {
     ColView c;
     uint32_t type;

@@ -213,19 +213,22 @@ __device__ inline TPlan plan_page(const ScanColView &v, const uint8_t *src, uint
 template <typename B>
 struct DbpReg {
     B *pk;  // packed fields
-    uint64_t mind, fmask, words;
+    uint64_t mind;
     uint32_t bw;
 };
-// v[i] - v[i-1] of row i >= 1.
+// The delta fields of a lane's HH rows r0 + j of a sub-tile, decoded once per sub-tile and
+// shared by every use of the column: p[j] = packed field of row r0 + j (the delta v[r] - v[r-1]
+// is mind + p[j]); valid bit j: the row has a delta (1 <= r0 + j <= rmax).
+struct DbpRows {
+    uint32_t p[SCAN_DBP_SLOTS][HH];
+    uint32_t valid;
+};
+// Field extraction at bit offset `bit` (fields of bw <= 32 bits; the column buffer extends at
+// least 8 bytes past any page, so the second dword is always readable).
 template <typename B>
-__device__ __forceinline__ uint64_t dbp_delta(const DbpReg<B> &d, uint32_t i) {
-    uint64_t p = 0;
-    if (d.bw) {
-        const uint32_t bit = (i - 1) * d.bw, w = bit >> 5;
-        const uint64_t lo = ldt<uint32_t>(d.pk, w << 2), hi = (w + 1 < d.words) ? ldt<uint32_t>(d.pk, (w + 1) << 2) : 0;
-        p = ((lo | (hi << 32)) >> (bit & 31)) & d.fmask;
-    }
-    return d.mind + p;
+__device__ __forceinline__ uint32_t packed_field(B *pk, uint32_t bit, uint32_t mask) {
+    const u32x2 w = ldt<u32x2>(pk, (bit >> 5) << 2);
+    return __builtin_amdgcn_alignbit(w.y, w.x, bit & 31u) & mask;
 }
 
 // Values (as int64) of column c (wave-uniform) at the thread's rows r0 + j, j < HH; a row past
@@ -237,16 +240,25 @@ __device__ __forceinline__ uint64_t dbp_delta(const DbpReg<B> &d, uint32_t i) {
 template <typename B>
 __device__ __forceinline__ void fetch_col(const TPlan *tp, const ScanColView *cols, B *base, lbyte *lds, uint32_t c,
                                           uint32_t r0, uint32_t rmax, int64_t (&out)[HH],
-                                          const DbpReg<B> (&dr)[SCAN_DBP_SLOTS], const uint64_t (&drs)[SCAN_DBP_SLOTS]) {
+                                          const DbpReg<B> (&dr)[SCAN_DBP_SLOTS], const DbpRows &dp,
+                                          const uint64_t (&drs)[SCAN_DBP_SLOTS]) {
     const uint32_t mode = uni(tp[c].mode);
-    if (mode == TA_DBP) {  // running sums of the packed deltas
+    if (mode == TA_DBP) {  // running sums of the sub-tile's decoded deltas
         const uint32_t slot = uni((uint32_t)cols[c].dslot);
-        const DbpReg<B> d = slot ? dr[1] : dr[0];
-        uint64_t run = slot ? drs[1] : drs[0];
+        uint64_t mind = dr[0].mind, run = drs[0];
+#pragma unroll
+        for (uint32_t t = 1; t < SCAN_DBP_SLOTS; ++t)
+            if (slot == t) {
+                mind = dr[t].mind;
+                run = drs[t];
+            }
 #pragma unroll
         for (int j = 0; j < HH; ++j) {
-            const uint32_t r = r0 + j;
-            if (r > 0 && r <= rmax) run += dbp_delta(d, r);
+            uint32_t p = dp.p[0][j];
+#pragma unroll
+            for (uint32_t t = 1; t < SCAN_DBP_SLOTS; ++t)
+                if (slot == t) p = dp.p[t][j];
+            if ((dp.valid >> j) & 1u) run += mind + p;
             out[j] = (int64_t)run;
         }
         return;
@@ -306,12 +318,9 @@ __device__ __forceinline__ void fetch_col(const TPlan *tp, const ScanColView *co
             // field r at bits [r*bw, r*bw + bw): two dwords and a funnel shift (the payload is
             // followed by 8 zero bytes, so the second dword is always inside the page)
             const uint32_t mask = bw == 32 ? 0xffffffffu : ((1u << bw) - 1u);
+            const uint32_t b0 = __umul24(r0, bw), bmax = rmax * bw;  // rows < 2^24
 #pragma unroll
-            for (int j = 0; j < HH; ++j) {
-                const uint32_t bit = min(r0 + j, rmax) * bw;
-                const uint32_t lo = ldt<uint32_t>(b, (bit >> 5) << 2), hi = ldt<uint32_t>(b, ((bit >> 5) << 2) + 4);
-                out[j] = ref + (int64_t)(__builtin_amdgcn_alignbit(hi, lo, bit & 31u) & mask);
-            }
+            for (int j = 0; j < HH; ++j) out[j] = ref + (int64_t)packed_field(b, min(b0 + j * bw, bmax), mask);
         } else {
             const uint64_t mask = ~0ull >> (64 - bw);
 #pragma unroll
@@ -491,6 +500,34 @@ __device__ __forceinline__ void agg_global(const ItemQuery &Q, uint32_t g, int64
     atomicAdd(reinterpret_cast<unsigned long long *>(&Q.g_cnt[g]), (unsigned long long)n);
 }
 
+// Key i of plan Q at the lane's HH rows with values x: kk[j] = (x[j] - base) / div; returns
+// the rows whose value lies outside [base, base + span) (their kk is meaningless).
+__device__ __forceinline__ uint32_t key_index(const ItemQuery &Q, uint32_t i, const int64_t (&x)[HH], uint32_t (&kk)[HH]) {
+    const uint64_t kb = uni64((uint64_t)Q.kbase[i]);
+    const uint32_t span = uni(Q.kspan[i]), ks = uni(Q.ks[i]);
+    golap_fmt::UDiv32 dv;
+    dv.m = uni(Q.km[i]);
+    dv.s1 = ks & 0xffu;
+    dv.s2 = ks >> 8;
+    uint32_t bad = 0;
+    if (dv.m == 1 && ks == 0) {  // divisor 1
+#pragma unroll
+        for (int j = 0; j < HH; ++j) {
+            const uint64_t uo = (uint64_t)x[j] - kb;
+            bad |= (uint32_t)(uo >= span) << j;
+            kk[j] = (uint32_t)uo;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < HH; ++j) {
+            const uint64_t uo = (uint64_t)x[j] - kb;
+            bad |= (uint32_t)(uo >= span) << j;
+            kk[j] = golap_fmt::udiv32((uint32_t)uo, dv);
+        }
+    }
+    return bad;
+}
+
 // Page p of the item for the plans of the pass, processed by one wave alone: column plans
 // (wave-private), window decodes into the wave's windows (wlds), delta-bitpack start values
 // from wave prefix scans, then plan by plan and column use by column use over the lane's HH
@@ -531,7 +568,7 @@ __device__ void wave_page(const ScanArgs &a, ScanShared &S, uint64_t p, uint32_t
     uint32_t dmask = 0;
 #pragma unroll
     for (uint32_t sl = 0; sl < SCAN_DBP_SLOTS; ++sl) {
-        dr[sl] = DbpReg<gbyte>{nullptr, 0, 0, 0, 0};
+        dr[sl] = DbpReg<gbyte>{nullptr, 0, 0};
         carry[sl] = 0;
     }
     for (uint32_t c = 0; c < a.n_cols; ++c) {
@@ -542,15 +579,12 @@ __device__ void wave_page(const ScanArgs &a, ScanShared &S, uint64_t p, uint32_t
                uni64(tp[c].off) + 32;
         d.bw = uni(tp[c].aux);
         d.mind = uni64((uint64_t)tp[c].b);
-        d.fmask = d.bw == 32 ? 0xffffffffull : ((1ull << d.bw) - 1);
-        d.words = rows ? (((uint64_t)(rows - 1) * d.bw + 31) / 32) : 0;
-        if (sl == 0) {
-            dr[0] = d;
-            carry[0] = uni64((uint64_t)tp[c].a);
-        } else {
-            dr[1] = d;
-            carry[1] = uni64((uint64_t)tp[c].a);
-        }
+#pragma unroll
+        for (uint32_t t = 0; t < SCAN_DBP_SLOTS; ++t)
+            if (sl == t) {
+                dr[t] = d;
+                carry[t] = uni64((uint64_t)tp[c].a);
+            }
         dmask |= 1u << sl;
     }
     auto col_base = [&](uint32_t c) -> gbyte * {
@@ -558,14 +592,33 @@ __device__ void wave_page(const ScanArgs &a, ScanShared &S, uint64_t p, uint32_t
     };
     const uint32_t rmax = rows - 1;
     for (uint32_t sub = 0; sub * SUBTILE < rows; ++sub) {
-        const uint32_t row0 = sub * SUBTILE + lane * HH, row1 = min(rows, row0 + HH);
+        const uint32_t row0 = sub * SUBTILE + lane * HH;
+        // delta fields of the lane's rows, once per sub-tile; start values by a wave prefix scan
+        DbpRows dp;
+        dp.valid = 0;
+#pragma unroll
+        for (int j = 0; j < HH; ++j) dp.valid |= (uint32_t)(row0 + j >= 1 && row0 + j <= rmax) << j;
         uint64_t run[SCAN_DBP_SLOTS];
 #pragma unroll
         for (uint32_t sl = 0; sl < SCAN_DBP_SLOTS; ++sl) {
             run[sl] = carry[sl];
+#pragma unroll
+            for (int j = 0; j < HH; ++j) dp.p[sl][j] = 0;
             if (!((dmask >> sl) & 1u)) continue;  // uniform
+            const uint32_t bw = dr[sl].bw;
             uint64_t tot = 0;
-            for (uint32_t i = max(row0, 1u); i < row1; ++i) tot += dbp_delta(dr[sl], i);
+            if (bw) {
+                const uint32_t mask = bw == 32 ? 0xffffffffu : ((1u << bw) - 1u);
+                // field of row r >= 1 at bit (r - 1) * bw
+                const uint32_t b0 = __umul24(row0, bw) - bw;
+#pragma unroll
+                for (int j = 0; j < HH; ++j)
+                    if ((dp.valid >> j) & 1u) {
+                        dp.p[sl][j] = packed_field(dr[sl].pk, b0 + j * bw, mask);
+                        tot += dp.p[sl][j];
+                    }
+            }
+            tot += (uint64_t)__popc(dp.valid) * dr[sl].mind;
             const uint64_t incl = wave_incl_scan_u64(tot);
             run[sl] += incl - tot;
             carry[sl] += uni64(__shfl(incl, WAVE - 1, WAVE));
@@ -578,17 +631,17 @@ __device__ void wave_page(const ScanArgs &a, ScanShared &S, uint64_t p, uint32_t
             const bool with_sum = vc >= 0;
             const uint32_t nu = np + nk + 1;  // the last use is the value (0 for COUNT only)
             const uint32_t lbase = uni(Q.lbase), cg = uni(Q.cg);
-            const bool one = lbase != NO_LDS && uni(Q.L) == 1u;  // every row of the item in one group
+            const bool glob = lbase == NO_LDS;
+            const bool one = !glob && uni(Q.L) == 1u;  // every row of the item in one group
             uint32_t pass = 0;
 #pragma unroll
             for (int j = 0; j < HH; ++j) pass |= (uint32_t)(row0 + j <= rmax) << j;
             uint32_t inl = pass;
-            uint32_t g[HH], e[HH];
+            // group index of the row: global (g, from cg) for global plans, local (e, from 0)
+            // for plans with dense partials
+            uint32_t ge[HH];
 #pragma unroll
-            for (int j = 0; j < HH; ++j) {
-                g[j] = cg;
-                e[j] = 0;
-            }
+            for (int j = 0; j < HH; ++j) ge[j] = glob ? cg : 0u;
             for (uint32_t u = 0; u < nu; ++u) {
                 const bool last = u + 1 == nu;
                 if (__ballot(pass != 0) == 0) break;  // wave-uniform: no row passes
@@ -596,36 +649,34 @@ __device__ void wave_page(const ScanArgs &a, ScanShared &S, uint64_t p, uint32_t
                 if (!last || with_sum) {
                     const uint32_t c = u < np ? (pcol >> (8 * u)) & 0xffu
                                               : u < np + nk ? (kcol >> (8 * (u - np))) & 0xffu : (uint32_t)vc;
-                    fetch_col<gbyte>(tp, S.cols, col_base(c), (lbyte *)wlds, c, row0, rmax, x, dr, run);
+                    fetch_col<gbyte>(tp, S.cols, col_base(c), (lbyte *)wlds, c, row0, rmax, x, dr, dp, run);
                 } else {
 #pragma unroll
                     for (int j = 0; j < HH; ++j) x[j] = 0;
                 }
                 if (u < np) {
                     const int64_t lo = (int64_t)uni64((uint64_t)Q.plo[u]), hi = (int64_t)uni64((uint64_t)Q.phi[u]);
+                    const uint64_t w = (uint64_t)hi - (uint64_t)lo;
 #pragma unroll
                     for (int j = 0; j < HH; ++j)
-                        if (x[j] < lo || x[j] > hi) pass &= ~(1u << j);
+                        if ((uint64_t)x[j] - (uint64_t)lo > w) pass &= ~(1u << j);
                 } else if (!last) {
                     const uint32_t i = u - np;
-                    const uint64_t kb = uni64((uint64_t)Q.kbase[i]);
-                    const uint32_t span = uni(Q.kspan[i]), gs = uni(Q.kgs[i]), klo = uni(Q.klo[i]);
-                    const uint32_t lcard = uni(Q.lcard[i]), ls = uni(Q.lstride[i]), ks = uni(Q.ks[i]);
-                    golap_fmt::UDiv32 dvz;
-                    dvz.m = uni(Q.km[i]);
-                    dvz.s1 = ks & 0xffu;
-                    dvz.s2 = ks >> 8;
-                    const bool div1 = dvz.m == 1 && ks == 0;
-                    uint32_t bad_rows = 0;
+                    uint32_t kk[HH];
+                    const uint32_t bad_rows = key_index(Q, i, x, kk);
+                    if (glob) {
+                        const uint32_t gs = uni(Q.kgs[i]);
 #pragma unroll
-                    for (int j = 0; j < HH; ++j) {
-                        const uint64_t uo = (uint64_t)x[j] - kb;
-                        bad_rows |= (uint32_t)(uo >= span) << j;
-                        const uint32_t kk = div1 ? (uint32_t)uo : golap_fmt::udiv32((uint32_t)uo, dvz);
-                        g[j] += kk * gs;
-                        const uint32_t lk = kk - klo;
-                        inl &= ~((uint32_t)(lk >= lcard) << j);
-                        e[j] += lk * ls;
+                        for (int j = 0; j < HH; ++j) ge[j] += kk[j] * gs;
+                    } else {
+                        // local index: lcard, lstride <= the LDS entries (< 2^24)
+                        const uint32_t klo = uni(Q.klo[i]), lcard = uni(Q.lcard[i]), ls = uni(Q.lstride[i]);
+#pragma unroll
+                        for (int j = 0; j < HH; ++j) {
+                            const uint32_t lk = kk[j] - klo;
+                            inl &= ~((uint32_t)(lk >= lcard) << j);
+                            ge[j] += __umul24(lk, ls);
+                        }
                     }
                     if (bad_rows & pass) {  // a qualifying row's key outside its declared range
                         raise_err(a.err, DEVERR_KEYRANGE);
@@ -651,19 +702,32 @@ __device__ void wave_page(const ScanArgs &a, ScanShared &S, uint64_t p, uint32_t
 #pragma unroll
                         for (int o = WAVE / 2; o > 0; o >>= 1) tcnt += __shfl_xor(tcnt, o, WAVE);
                         if (lane == 0 && tcnt) agg_lds(lsum, lcnt, lbase, tsum, tcnt, with_sum);
-                    } else if (lbase != NO_LDS) {
+                    } else if (!glob) {
 #pragma unroll
                         for (int j = 0; j < HH; ++j)
-                            if ((inl >> j) & 1u) agg_lds(lsum, lcnt, lbase + e[j], x[j], 1u, with_sum);
+                            if ((inl >> j) & 1u) agg_lds(lsum, lcnt, lbase + ge[j], x[j], 1u, with_sum);
                     }
-                    if (lbase != NO_LDS) {
+                    if (!glob) {
                         if (__ballot(outside != 0)) {
                             // zone maps bound every row's key; a plan that shares its partials
                             // cannot send rows elsewhere
                             if (uni(Q.members) && lane == 0) raise_err(a.err, DEVERR_CORRUPT);
+                            // global group index of these rows: the key columns once more
+#pragma unroll
+                            for (int j = 0; j < HH; ++j) ge[j] = cg;
+                            for (uint32_t i = 0; i < nk; ++i) {
+                                const uint32_t c = (kcol >> (8 * i)) & 0xffu;
+                                int64_t xk[HH];
+                                fetch_col<gbyte>(tp, S.cols, col_base(c), (lbyte *)wlds, c, row0, rmax, xk, dr, dp, run);
+                                uint32_t kk[HH];
+                                (void)key_index(Q, i, xk, kk);
+                                const uint32_t gs = uni(Q.kgs[i]);
+#pragma unroll
+                                for (int j = 0; j < HH; ++j) ge[j] += kk[j] * gs;
+                            }
 #pragma unroll
                             for (int j = 0; j < HH; ++j)
-                                if ((outside >> j) & 1u) agg_global(Q, g[j], x[j], 1u, with_sum);
+                                if ((outside >> j) & 1u) agg_global(Q, ge[j], x[j], 1u, with_sum);
                         }
                     } else {
                         // global aggregation: rows of a wave that share one group (sorted keys) add once
@@ -673,8 +737,8 @@ __device__ void wave_page(const ScanArgs &a, ScanShared &S, uint64_t p, uint32_t
                             const uint64_t am = __ballot(act);
                             if (!am) continue;
                             const uint32_t leader = (uint32_t)__ffsll((long long)am) - 1;
-                            const uint32_t g0 = __builtin_amdgcn_readlane(g[j], leader);
-                            if (__ballot(act && g[j] != g0) == 0) {
+                            const uint32_t g0 = __builtin_amdgcn_readlane(ge[j], leader);
+                            if (__ballot(act && ge[j] != g0) == 0) {
                                 int64_t sum = act ? x[j] : 0;
                                 if (with_sum) {
 #pragma unroll
@@ -682,7 +746,7 @@ __device__ void wave_page(const ScanArgs &a, ScanShared &S, uint64_t p, uint32_t
                                 }
                                 if (lane == leader) agg_global(Q, g0, sum, (uint32_t)__popcll(am), with_sum);
                             } else if (act) {
-                                agg_global(Q, g[j], x[j], 1u, with_sum);
+                                agg_global(Q, ge[j], x[j], 1u, with_sum);
                             }
                         }
                     }
