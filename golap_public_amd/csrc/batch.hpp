@@ -69,6 +69,7 @@ struct golap_batch {
     uint64_t total_pages = 0, total_bytes = 0;
     // scan batches: union of the slices, surviving (chunk, plan word) pairs, bytes read once
     uint64_t union_begin = 0, union_end = 0, union_pairs = 0, shared_bytes = 0;
+    uint64_t max_local = 0;  // largest local key domain of a plan on a surviving chunk
     uint64_t max_domain = 1;
     std::vector<uint64_t> res_off;
     uint64_t arena_bytes = 0;

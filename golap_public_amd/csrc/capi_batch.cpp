@@ -168,6 +168,8 @@ hipError_t launch_kernel(golap_batch *b, hipStream_t s, bool prepare_only) {
     L.queue = reinterpret_cast<uint32_t *>(sl->d_arena);
     L.pages_per_item = c.pages_per_item;
     L.max_blocks_per_cu = c.max_blocks_per_cu;
+    L.rows_per_lane = scan_rows_per_lane(b->ctx, b->union_pairs * b->ppc,
+                                         scan_lds_bytes(L.win_bytes, L.agg_entries, L.page_waves));
     L.err = reinterpret_cast<uint32_t *>(sl->d_arena + 8);
     return launch_scan(b->ctx, L, s, prepare_only);
 }
@@ -227,11 +229,12 @@ golap_batch_config default_config(golap_batch *b) {
         c.lds_group_slots = lds_keys_fit(b) ? LDS_GROUP_SLOTS_MAX : 0;
         while (c.lds_group_slots && lds_bytes(b, c) > (size_t)b->ctx->lds_max) c.lds_group_slots >>= 1;
     } else {
-        // dense partials for several plans per item; items small enough to fill the device
-        c.lds_group_slots = 4096;
-        while (c.lds_group_slots &&
-               lds_bytes(b, c) > std::max<size_t>(scan_lds_budget(b->ctx) / 2, scan_lds_bytes(b->win_bytes, 0, 4) + 1024))
-            c.lds_group_slots >>= 1;
+        // dense partials for several plans per item (4096 entries: about four request
+        // templates' zone x hour partials per pass); items small enough to fill the device
+        c.lds_group_slots = scan_agg_entries(b->ctx, GOLAP_AGG_AUTO, 0, b->win_bytes,
+                                             scan_page_waves(b->ctx, b->win_bytes, 1024),
+                                             b->sh.size() > 1 ? std::max(4096u, scan_default_entries(b->max_local))
+                                                              : scan_default_entries(b->max_local));
         c.pages_per_item = scan_pages_per_item(b->ctx, b->union_pairs, b->ppc);
     }
     return c;
@@ -286,6 +289,7 @@ int finish_create(golap_ctx *ctx, golap_batch *b, bool with_graph) {
         } else {
             b->survive[i] = scan_survivors(b->sh[i].plan, mask);
             scan_stats(b->sh[i].plan, b->sh[i].colmask, b->survive[i], &tmp);
+            b->max_local = std::max(b->max_local, scan_local_domain(b->sh[i].plan, b->sd[i], b->survive[i]));
         }
         for (uint64_t k = 0; k < nch; ++k)
             if (b->survive[i][k]) {

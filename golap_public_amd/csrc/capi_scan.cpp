@@ -155,15 +155,48 @@ uint32_t scan_page_waves(golap_ctx *ctx, uint32_t win_bytes, uint32_t agg_entrie
 }
 
 // Dense LDS aggregation entries of a scan launch: 0 for global aggregation, else the requested
-// or a default count (a power of two), halved until the launch fits the LDS budget (and, for
-// the default, leaves room for several workgroups per CU).
-uint32_t scan_agg_entries(golap_ctx *ctx, uint32_t agg_mode, uint32_t requested, uint32_t win_bytes, uint32_t page_waves) {
+// count or the default `dflt` (powers of two), halved until the launch fits the LDS budget.
+uint32_t scan_agg_entries(golap_ctx *ctx, uint32_t agg_mode, uint32_t requested, uint32_t win_bytes, uint32_t page_waves,
+                          uint32_t dflt) {
     if (agg_mode == GOLAP_AGG_GLOBAL) return 0;
-    uint32_t s = requested ? std::min(pow2_at_least(requested), SCAN_AGG_MAX) : 2048u;
-    const size_t budget = requested ? scan_lds_budget(ctx)
-                                    : std::max<size_t>(scan_lds_budget(ctx) / 2, scan_lds_bytes(win_bytes, 0, page_waves) + 1024);
-    while (s && scan_lds_bytes(win_bytes, s, page_waves) > budget) s >>= 1;
+    uint32_t s = std::min(pow2_at_least(requested ? requested : dflt), SCAN_AGG_MAX);
+    while (s && scan_lds_bytes(win_bytes, s, page_waves) > scan_lds_budget(ctx)) s >>= 1;
     return s;
+}
+
+// Largest local key domain of plan p over its surviving chunks (run): the dense partial
+// entries the kernel gives the plan on a work item, from the key columns' zone maps as the
+// kernel computes them (0 when every chunk aggregates in global memory).
+uint64_t scan_local_domain(const golap_scan_plan &p, const ScanQueryDev &qd, const std::vector<uint8_t> &run) {
+    uint64_t best = 0;
+    for (uint64_t k = 0; k < run.size(); ++k) {
+        if (!run[k]) continue;
+        const uint64_t c = p.chunk_begin + k;
+        uint64_t n = 1;
+        for (uint32_t i = 0; i < qd.n_keys && n; ++i) {
+            const ScanKeyDev &K = qd.key[i];  // key i of p (its column index may be a batch's)
+            const golap_column *col = p.cols[p.keys[i].col];
+            const int64_t zmin = col->chunk_min[c], zmax = col->chunk_max[c];
+            const uint64_t ulo = zmin <= K.base ? 0 : (uint64_t)zmin - (uint64_t)K.base;
+            const uint64_t uhi = zmax < K.base ? 0 : (uint64_t)zmax - (uint64_t)K.base;
+            if (zmax < K.base || ulo >= K.span) {
+                n = 0;
+                break;
+            }
+            const uint64_t div = K.span / K.card;
+            const uint64_t klo = ulo / div, khi = std::min<uint64_t>(uhi, K.span - 1) / div;
+            const bool inside = zmin >= K.base && uhi < K.span;
+            if (!(inside && klo == khi)) n *= khi - klo + 1;
+        }
+        best = std::max(best, n);
+    }
+    return best;
+}
+
+// Default dense entries for plans whose largest local domain is maxl: at least 1024 (four
+// workgroups per CU fit beside small windows), else enough for one such plan.
+uint32_t scan_default_entries(uint64_t maxl) {
+    return std::min<uint32_t>(SCAN_AGG_MAX, pow2_at_least(std::max<uint64_t>(maxl, 1024)));
 }
 
 // Pages per work item: whole chunks when there are enough surviving chunks to fill the device,
@@ -218,11 +251,14 @@ int golap_scan_run(golap_ctx *ctx, const golap_scan_plan *plan, golap_star_resul
     L.n_pages = f->n_pages;
     L.n_queries = 1;
     L.page_waves = scan_page_waves(ctx, L.win_bytes, 1024);
-    L.agg_entries = scan_agg_entries(ctx, plan->agg_mode, plan->lds_group_slots, L.win_bytes, L.page_waves);
+    L.agg_entries = scan_agg_entries(ctx, plan->agg_mode, plan->lds_group_slots, L.win_bytes, L.page_waves,
+                                     scan_default_entries(scan_local_domain(*plan, qd, run)));
     if (scan_lds_bytes(L.win_bytes, L.agg_entries, L.page_waves) > scan_lds_budget(ctx))
         return fail(ctx, GOLAP_EINVAL, "scan: LDS budget exceeded (page_rows or coded columns)");
     qd.use_lds = L.agg_entries != 0;
     L.pages_per_item = scan_pages_per_item(ctx, surv, L.pages_per_chunk);
+    L.rows_per_lane = scan_rows_per_lane(ctx, surv * L.pages_per_chunk,
+                                         scan_lds_bytes(L.win_bytes, L.agg_entries, L.page_waves));
     const uint32_t parts = (L.pages_per_chunk + L.pages_per_item - 1) / L.pages_per_item;
     int64_t *d_sum;
     uint64_t *d_cnt;

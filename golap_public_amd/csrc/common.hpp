@@ -301,6 +301,7 @@ struct ScanLaunch {
     uint32_t *queue;
     uint32_t pages_per_item;
     uint32_t max_blocks_per_cu;
+    uint32_t rows_per_lane;  // 8: short launches (fewer steps per page); 4: long ones (more workgroups per CU)
     uint32_t *err;
 };
 constexpr uint32_t SCAN_AGG_MAX = 8192;  // 96 KiB of LDS
@@ -313,6 +314,9 @@ hipError_t launch_scan_items(const ScanQueryDev *queries, uint32_t n_queries, ui
 size_t scan_lds_bytes(uint32_t win_bytes, uint32_t agg_entries, uint32_t page_waves);
 // Dynamic LDS a scan launch may use (the device limit less the kernel's own LDS).
 size_t scan_lds_budget(golap_ctx *ctx);
+// Rows per lane of a scan launch over `pages` surviving pages with `lds` bytes of dynamic LDS.
+uint32_t scan_rows_per_lane(golap_ctx *ctx, uint64_t pages, size_t lds);
+
 
 struct GroupByLaunch {
     const void *key[4];

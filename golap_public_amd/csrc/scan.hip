@@ -70,8 +70,8 @@ struct TPlan {
 constexpr uint32_t PASS_MAX = 16;  // plans evaluated together (their item views in LDS)
 constexpr uint32_t NO_LDS = 0xffffffffu;
 constexpr uint32_t NWAVE = BLOCK / WAVE;
-constexpr int HH = 8;                        // rows of a lane per sub-tile
-constexpr uint32_t SUBTILE = WAVE * HH;      // rows of a wave's sub-tile
+// HH (template parameter): rows of a lane per sub-tile of WAVE * HH rows; 8 for short launches
+// (fewer sub-tiles per page), 4 for long ones (fewer registers: four workgroups per CU).
 
 // A plan's view of the current work item. Predicates that hold for the whole chunk are left
 // out; keys constant over the chunk contribute cg to the global group index; the others are
@@ -219,6 +219,7 @@ struct DbpReg {
 // The delta fields of a lane's HH rows r0 + j of a sub-tile, decoded once per sub-tile and
 // shared by every use of the column: p[j] = packed field of row r0 + j (the delta v[r] - v[r-1]
 // is mind + p[j]); valid bit j: the row has a delta (1 <= r0 + j <= rmax).
+template <int HH>
 struct DbpRows {
     uint32_t p[SCAN_DBP_SLOTS][HH];
     uint32_t valid;
@@ -237,10 +238,10 @@ __device__ __forceinline__ uint32_t packed_field(B *pk, uint32_t bit, uint32_t m
 // affine pages give the exact int64 values; in-place, window and run-length reads are sign- or
 // zero-extended from their element width. drs: the delta slots' values at row r0 - 1 (at row
 // 0: the page's first value).
-template <typename B>
+template <int HH, typename B>
 __device__ __forceinline__ void fetch_col(const TPlan *tp, const ScanColView *cols, B *base, lbyte *lds, uint32_t c,
                                           uint32_t r0, uint32_t rmax, int64_t (&out)[HH],
-                                          const DbpReg<B> (&dr)[SCAN_DBP_SLOTS], const DbpRows &dp,
+                                          const DbpReg<B> (&dr)[SCAN_DBP_SLOTS], const DbpRows<HH> &dp,
                                           const uint64_t (&drs)[SCAN_DBP_SLOTS]) {
     const uint32_t mode = uni(tp[c].mode);
     if (mode == TA_DBP) {  // running sums of the sub-tile's decoded deltas
@@ -270,34 +271,45 @@ __device__ __forceinline__ void fetch_col(const TPlan *tp, const ScanColView *co
     switch (mode) {
     case TA_DIRECT: {
         if (r0 + HH - 1 <= rmax && uni(tp[c].aux)) {
-            // HH consecutive rows of a 16-byte aligned payload (r0 % HH == 0): vector loads
-            static_assert(HH == 8, "vector loads of 8 rows");
+            // HH consecutive rows of a 16-byte aligned payload (r0 % HH == 0): HH * elem bytes
+            // in dwords w, loaded as 16-byte vectors where they fill one
+            static_assert(HH == 4 || HH == 8, "row vectors of 4 or 8 rows");
             const uint32_t vo = r0 * elem;
-            if (elem == 1) {
-                const u32x2 x = ldt<u32x2>(b, vo);
+            uint32_t w[2 * HH];
+            auto load = [&](int nw) {
+                if (nw == 1) {
+                    w[0] = ldt<uint32_t>(b, vo);
+                } else if (nw == 2) {
+                    const u32x2 v = ldt<u32x2>(b, vo);
+                    w[0] = v.x;
+                    w[1] = v.y;
+                } else {
 #pragma unroll
-                for (int j = 0; j < HH; ++j) out[j] = sext(((j < 4 ? x.x : x.y) >> (8 * (j & 3))) & 0xffu, sh);
+                    for (int q = 0; q < nw / 4; ++q) {
+                        const u32x4 v = ldt<u32x4>(b, vo + 16 * q);
+                        w[4 * q] = v.x;
+                        w[4 * q + 1] = v.y;
+                        w[4 * q + 2] = v.z;
+                        w[4 * q + 3] = v.w;
+                    }
+                }
+            };
+            if (elem == 1) {
+                load(HH / 4);
+#pragma unroll
+                for (int j = 0; j < HH; ++j) out[j] = sext((w[j >> 2] >> (8 * (j & 3))) & 0xffu, sh);
             } else if (elem == 2) {
-                const u32x4 x = ldt<u32x4>(b, vo);
-                const uint32_t w[4] = {x.x, x.y, x.z, x.w};
+                load(HH / 2);
 #pragma unroll
                 for (int j = 0; j < HH; ++j) out[j] = sext((w[j >> 1] >> (16 * (j & 1))) & 0xffffu, sh);
             } else if (elem == 4) {
+                load(HH);
 #pragma unroll
-                for (int q = 0; q < 2; ++q) {
-                    const u32x4 x = ldt<u32x4>(b, vo + 16 * q);
-                    out[4 * q] = sext(x.x, sh);
-                    out[4 * q + 1] = sext(x.y, sh);
-                    out[4 * q + 2] = sext(x.z, sh);
-                    out[4 * q + 3] = sext(x.w, sh);
-                }
+                for (int j = 0; j < HH; ++j) out[j] = sext(w[j], sh);
             } else {
+                load(2 * HH);
 #pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const u32x4 x = ldt<u32x4>(b, vo + 16 * q);
-                    out[2 * q] = (int64_t)(((uint64_t)x.y << 32) | x.x);
-                    out[2 * q + 1] = (int64_t)(((uint64_t)x.w << 32) | x.z);
-                }
+                for (int j = 0; j < HH; ++j) out[j] = (int64_t)(((uint64_t)w[2 * j + 1] << 32) | w[2 * j]);
             }
         } else {
 #pragma unroll
@@ -502,6 +514,7 @@ __device__ __forceinline__ void agg_global(const ItemQuery &Q, uint32_t g, int64
 
 // Key i of plan Q at the lane's HH rows with values x: kk[j] = (x[j] - base) / div; returns
 // the rows whose value lies outside [base, base + span) (their kk is meaningless).
+template <int HH>
 __device__ __forceinline__ uint32_t key_index(const ItemQuery &Q, uint32_t i, const int64_t (&x)[HH], uint32_t (&kk)[HH]) {
     const uint64_t kb = uni64((uint64_t)Q.kbase[i]);
     const uint32_t span = uni(Q.kspan[i]), ks = uni(Q.ks[i]);
@@ -534,6 +547,7 @@ __device__ __forceinline__ uint32_t key_index(const ItemQuery &Q, uint32_t i, co
 // rows of every sub-tile: the plan's and the column's parameters stay uniform across them,
 // every use issues HH independent loads, and rows are carried between uses as bit masks (pass)
 // and partial group indexes (g global, e local).
+template <int HH>
 __device__ void wave_page(const ScanArgs &a, ScanShared &S, uint64_t p, uint32_t used, uint32_t npass, uint8_t *wlds,
                           unsigned long long *lsum, uint32_t *lcnt) {
     const uint32_t wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
@@ -591,10 +605,10 @@ __device__ void wave_page(const ScanArgs &a, ScanShared &S, uint64_t p, uint32_t
         return gmem(reinterpret_cast<const uint8_t *>(uni64(reinterpret_cast<uint64_t>(S.cols[c].c.comp))));
     };
     const uint32_t rmax = rows - 1;
-    for (uint32_t sub = 0; sub * SUBTILE < rows; ++sub) {
-        const uint32_t row0 = sub * SUBTILE + lane * HH;
+    for (uint32_t sub = 0; sub * (WAVE * HH) < rows; ++sub) {
+        const uint32_t row0 = sub * (WAVE * HH) + lane * HH;
         // delta fields of the lane's rows, once per sub-tile; start values by a wave prefix scan
-        DbpRows dp;
+        DbpRows<HH> dp;
         dp.valid = 0;
 #pragma unroll
         for (int j = 0; j < HH; ++j) dp.valid |= (uint32_t)(row0 + j >= 1 && row0 + j <= rmax) << j;
@@ -649,7 +663,7 @@ __device__ void wave_page(const ScanArgs &a, ScanShared &S, uint64_t p, uint32_t
                 if (!last || with_sum) {
                     const uint32_t c = u < np ? (pcol >> (8 * u)) & 0xffu
                                               : u < np + nk ? (kcol >> (8 * (u - np))) & 0xffu : (uint32_t)vc;
-                    fetch_col<gbyte>(tp, S.cols, col_base(c), (lbyte *)wlds, c, row0, rmax, x, dr, dp, run);
+                    fetch_col<HH, gbyte>(tp, S.cols, col_base(c), (lbyte *)wlds, c, row0, rmax, x, dr, dp, run);
                 } else {
 #pragma unroll
                     for (int j = 0; j < HH; ++j) x[j] = 0;
@@ -718,7 +732,7 @@ __device__ void wave_page(const ScanArgs &a, ScanShared &S, uint64_t p, uint32_t
                             for (uint32_t i = 0; i < nk; ++i) {
                                 const uint32_t c = (kcol >> (8 * i)) & 0xffu;
                                 int64_t xk[HH];
-                                fetch_col<gbyte>(tp, S.cols, col_base(c), (lbyte *)wlds, c, row0, rmax, xk, dr, dp, run);
+                                fetch_col<HH, gbyte>(tp, S.cols, col_base(c), (lbyte *)wlds, c, row0, rmax, xk, dr, dp, run);
                                 uint32_t kk[HH];
                                 (void)key_index(Q, i, xk, kk);
                                 const uint32_t gs = uni(Q.kgs[i]);
@@ -781,7 +795,8 @@ __device__ __forceinline__ void pass_flush(ScanShared &S, uint32_t npass, const 
 
 // Persistent shared-scan kernel: workgroups take (chunk part, plan word, plan mask) items with
 // one atomic each; the waves share the item's dense partials and split its pages.
-__global__ __launch_bounds__(BLOCK, 2) void k_scan(ScanArgs a) {
+template <int HH>
+__global__ __launch_bounds__(BLOCK, HH == 4 ? 4 : 2) void k_scan(ScanArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     __shared__ ScanShared S;
     const uint32_t wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
@@ -862,7 +877,7 @@ __global__ __launch_bounds__(BLOCK, 2) void k_scan(ScanArgs a) {
             for (uint32_t k = 0; k < npass; ++k)
                 if (S.iq[k].leader == k) used |= S.iq[k].colmask;
             if (wave < a.page_waves)
-                for (uint64_t p = pb + wave; p < pe; p += a.page_waves) wave_page(a, S, p, used, npass, wlds, lsum, lcnt);
+                for (uint64_t p = pb + wave; p < pe; p += a.page_waves) wave_page<HH>(a, S, p, used, npass, wlds, lsum, lcnt);
             pass_flush(S, npass, lsum, lcnt);
             k0 = k1;
         }
@@ -904,34 +919,54 @@ __global__ __launch_bounds__(BLOCK) void k_scan_items(const ScanQueryDev *qs, ui
     }
 }
 
-hipError_t launch_scan_t(golap_ctx *ctx, const ScanArgs &a, size_t lds, hipStream_t stream, uint32_t max_blocks_per_cu,
-                         bool prepare_only) {
+template <typename K>
+hipError_t launch_scan_t(golap_ctx *ctx, K kernel, const ScanArgs &a, size_t lds, hipStream_t stream,
+                         uint32_t max_blocks_per_cu, bool prepare_only) {
     if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_scan),
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(kernel),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
     int occ = 0;
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_scan, BLOCK, lds);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, BLOCK, lds);
     if (e != hipSuccess) return e;
     occ = std::max(occ, 1);
     if (max_blocks_per_cu) occ = std::min<int>(occ, (int)max_blocks_per_cu);
     const uint64_t grid = std::max<uint64_t>(1, (uint64_t)ctx->cus * occ);
     if (prepare_only) return hipSuccess;
-    hipLaunchKernelGGL(k_scan, dim3((uint32_t)grid), dim3(BLOCK), lds, stream, a);
+    hipLaunchKernelGGL(kernel, dim3((uint32_t)grid), dim3(BLOCK), lds, stream, a);
     return hipGetLastError();
 }
 
 }  // namespace
 
-size_t scan_lds_budget(golap_ctx *ctx) {
-    static size_t static_lds = ~(size_t)0;  // the kernel's own LDS (ScanShared), the same on every device
+// The kernel's own LDS (ScanShared), the same on every device.
+static size_t scan_static_lds() {
+    static size_t static_lds = ~(size_t)0;
     if (static_lds == ~(size_t)0) {
         hipFuncAttributes fa{};
-        static_lds = hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(k_scan)) == hipSuccess ? fa.sharedSizeBytes
+        static_lds = hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(k_scan<4>)) == hipSuccess ? fa.sharedSizeBytes
                                                                                                       : 16384;
     }
-    return (size_t)ctx->lds_max > static_lds ? (size_t)ctx->lds_max - static_lds : 0;
+    return static_lds;
+}
+
+size_t scan_lds_budget(golap_ctx *ctx) {
+    const size_t st = scan_static_lds();
+    return (size_t)ctx->lds_max > st ? (size_t)ctx->lds_max - st : 0;
+}
+
+
+// 8 rows per lane when the pages do not give every wave of four workgroups per CU a page (each
+// wave's steps per page then bound the time) or when the LDS leaves no more workgroups per CU
+// for the 4-row kernel (its fewer registers buy nothing); else 4.
+uint32_t scan_rows_per_lane(golap_ctx *ctx, uint64_t pages, size_t lds) {
+    if (pages < (uint64_t)ctx->cus * 16) return 8;
+    int o4 = 0, o8 = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&o4, k_scan<4>, BLOCK, lds) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&o8, k_scan<8>, BLOCK, lds) != hipSuccess)
+        return 8;
+    return o4 > o8 ? 4u : 8u;
 }
 
 size_t scan_lds_bytes(uint32_t win_bytes, uint32_t agg_entries, uint32_t page_waves) {
@@ -941,7 +976,7 @@ size_t scan_lds_bytes(uint32_t win_bytes, uint32_t agg_entries, uint32_t page_wa
 hipError_t launch_scan(golap_ctx *ctx, const ScanLaunch &l, hipStream_t stream, bool prepare_only) {
     if (l.n_cols > SCAN_MAX_COLS || !l.queries || !l.n_queries || !l.items || !l.queue || !l.pages_per_item ||
         l.pages_per_item > l.pages_per_chunk || l.win_bytes % 16 || l.agg_entries > SCAN_AGG_MAX || l.page_rows > 8192 ||
-        !l.page_waves || l.page_waves > NWAVE)
+        !l.page_waves || l.page_waves > NWAVE || (l.rows_per_lane != 4 && l.rows_per_lane != 8))
         return hipErrorInvalidValue;
     ScanArgs a{};
     for (uint32_t c = 0; c < l.n_cols; ++c) a.col[c] = l.col[c];
@@ -961,7 +996,8 @@ hipError_t launch_scan(golap_ctx *ctx, const ScanLaunch &l, hipStream_t stream, 
     a.err = l.err;
     const size_t lds = scan_lds_bytes(l.win_bytes, l.agg_entries, l.page_waves);
     if (lds > scan_lds_budget(ctx)) return hipErrorInvalidConfiguration;
-    return launch_scan_t(ctx, a, lds, stream, l.max_blocks_per_cu, prepare_only);
+    return l.rows_per_lane == 8 ? launch_scan_t(ctx, k_scan<8>, a, lds, stream, l.max_blocks_per_cu, prepare_only)
+                                : launch_scan_t(ctx, k_scan<4>, a, lds, stream, l.max_blocks_per_cu, prepare_only);
 }
 
 hipError_t launch_scan_items(const ScanQueryDev *queries, uint32_t n_queries, uint64_t c0, uint64_t nch,

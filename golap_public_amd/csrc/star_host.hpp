@@ -65,7 +65,10 @@ void scan_stats(const golap_scan_plan &p, uint32_t colmask, const std::vector<ui
 uint32_t scan_layout(const golap_column *const *cols, uint32_t n_cols, uint32_t colmask, uint64_t pb, uint64_t pe,
                      int32_t *woff, int32_t *dslot);
 uint32_t scan_page_waves(golap_ctx *ctx, uint32_t win_bytes, uint32_t agg_entries);
-uint32_t scan_agg_entries(golap_ctx *ctx, uint32_t agg_mode, uint32_t requested, uint32_t win_bytes, uint32_t page_waves);
+uint32_t scan_agg_entries(golap_ctx *ctx, uint32_t agg_mode, uint32_t requested, uint32_t win_bytes, uint32_t page_waves,
+                          uint32_t dflt);
+uint64_t scan_local_domain(const golap_scan_plan &p, const ScanQueryDev &qd, const std::vector<uint8_t> &run);
+uint32_t scan_default_entries(uint64_t maxl);
 uint32_t scan_pages_per_item(golap_ctx *ctx, uint64_t surviving_chunks, uint32_t ppc);
 
 }  // namespace starhost
