@@ -356,6 +356,11 @@ int golap_star_run(golap_ctx *ctx, const golap_star_plan *plan, golap_star_resul
         L.lds_gslots >>= 1;
     if (star_lds_bytes(L.filter_words, L.page_rows, L.n_windows, L.lds_gslots) > (size_t)ctx->lds_max)
         return fail(ctx, GOLAP_EINVAL, "star: LDS budget exceeded (page_rows or coded columns)");
+    // the driver filter stays in global memory (L2) when pinning it in LDS would cost
+    // workgroups per CU: the kernel is latency-bound, and tiles in flight matter more
+    if (star_blocks_per_cu(L.page_rows, star_lds_bytes(0, L.page_rows, L.n_windows, L.lds_gslots)) >
+        star_blocks_per_cu(L.page_rows, star_lds_bytes(L.filter_words, L.page_rows, L.n_windows, L.lds_gslots)))
+        L.filter_words = 0;
     qd.use_lds = L.lds_gslots != 0;
     L.group_bits = std::max<uint32_t>(1, bits_for(qh.domain));
     int64_t *d_sum;

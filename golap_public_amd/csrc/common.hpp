@@ -237,6 +237,8 @@ hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t stream);
 int solo_worklist(golap_ctx *ctx, const void *desc, uint32_t desc_size, uint32_t pages_per_chunk, const void **d_desc,
                   uint32_t **queue, uint2 **items);
 size_t star_lds_bytes(uint32_t filter_words, uint32_t page_rows, uint32_t n_windows, uint32_t lds_gslots);
+// Workgroups per CU of the star kernel for pages of page_rows rows and `lds` bytes of dynamic LDS.
+uint32_t star_blocks_per_cu(uint32_t page_rows, size_t lds);
 hipError_t launch_hash_filter(golap_ctx *ctx, const uint64_t *slots, uint32_t log2cap, uint32_t *filter,
                               uint32_t words_log2);
 

@@ -322,6 +322,19 @@ static hipError_t launch_star_t(golap_ctx *ctx, K kernel, const StarArgs &a, siz
     return hipGetLastError();
 }
 
+uint32_t star_blocks_per_cu(uint32_t page_rows, size_t lds) {
+    int occ = 0;
+    hipError_t e = hipErrorInvalidValue;
+    switch (page_rows) {
+    case 1024: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_star<4, false>, BLOCK, lds); break;
+    case 2048: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_star<8, false>, BLOCK, lds); break;
+    case 4096: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_star<16, false>, BLOCK, lds); break;
+    case 8192: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_star<32, false>, BLOCK, lds); break;
+    default: break;
+    }
+    return e == hipSuccess ? (uint32_t)std::max(occ, 0) : 0u;
+}
+
 hipError_t launch_star(golap_ctx *ctx, const StarLaunch &l, hipStream_t stream, bool prepare_only) {
     if (l.n_cols > STAR_MAX_COLS || (!l.queries && l.n_queries != 1) || (l.items && !l.queue) ||
         (l.items && (!l.pages_per_item || l.pages_per_item > l.pages_per_chunk)))
