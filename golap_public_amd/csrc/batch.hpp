@@ -70,6 +70,10 @@ struct golap_batch {
     // scan batches: union of the slices, surviving (chunk, plan word) pairs, bytes read once
     uint64_t union_begin = 0, union_end = 0, union_pairs = 0, shared_bytes = 0;
     uint64_t max_local = 0;  // largest local key domain of a plan on a surviving chunk
+    // star batches: fact column mcol that every query probes, and each query's membership filter
+    // of its dimension on that column (64 entries per word of queries), for the shared filter
+    golap_impl::DriverFilterView *d_mfilt = nullptr;
+    int mcol = -1;
     uint64_t max_domain = 1;
     std::vector<uint64_t> res_off;
     uint64_t arena_bytes = 0;

@@ -61,12 +61,62 @@ std::string shape_of(const golap_batch *b) {
            std::to_string(bucket(b->total_pages)) + "/" + std::to_string(bucket(b->max_domain));
 }
 
+// Columns (of the batch's column set) query i reads.
+uint32_t colmask_of(const golap_batch *b, uint32_t i) {
+    if (b->kind == BatchKind::SCAN) return b->sd[i].colmask;
+    const StarQueryDev &d = b->qd[i];
+    uint32_t m = 1u << d.val_col;
+    for (uint32_t k = 0; k < d.n_dims; ++k) m |= 1u << d.fk_col[k];
+    return m;
+}
+
+// The shared star filter: a fact column every query probes (the most common driver among them)
+// and each query's membership filter of its dimension on that column. Left out when no column
+// qualifies.
+int build_shared_filter(golap_ctx *ctx, golap_batch *b) {
+    const uint32_t nq = b->n_queries(), ncols = (uint32_t)b->cs.cols.size();
+    std::vector<uint32_t> has(ncols, 0), drives(ncols, 0);
+    for (uint32_t i = 0; i < nq; ++i) {
+        const StarQueryDev &d = b->qd[i];
+        uint32_t seen = 0;
+        for (uint32_t k = 0; k < d.n_dims; ++k) seen |= 1u << d.fk_col[k];
+        for (uint32_t c = 0; c < ncols; ++c) has[c] += (seen >> c) & 1u;
+        drives[d.fk_col[d.driver]]++;
+    }
+    int mcol = -1;
+    for (uint32_t c = 0; c < ncols; ++c)
+        if (has[c] == nq && (mcol < 0 || drives[c] > drives[mcol])) mcol = (int)c;
+    if (mcol < 0) return GOLAP_OK;
+    std::vector<DriverFilterView> f(((nq + 63) / 64) * 64, DriverFilterView{});
+    for (uint32_t i = 0; i < nq; ++i) {
+        const StarQueryDev &d = b->qd[i];
+        for (uint32_t k = 0; k < d.n_dims; ++k)
+            if (d.fk_col[k] == (uint32_t)mcol) {
+                uint32_t words = 0;
+                if (int r = filter_view(ctx, b->sq[i].plan.dims[k].map, &f[i], &words)) return r;
+                break;
+            }
+    }
+    void *p = nullptr;
+    if (int r = dev_alloc(ctx, f.size() * sizeof(DriverFilterView), &p)) return r;
+    hipError_t e = hipMemcpyAsync(p, f.data(), f.size() * sizeof(DriverFilterView), hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);  // with the filters built on it
+    if (e != hipSuccess) {
+        dev_release(ctx, p);
+        return fail(ctx, GOLAP_EHIP, std::string("batch shared filter: ") + hipGetErrorString(e));
+    }
+    b->d_mfilt = static_cast<DriverFilterView *>(p);
+    b->mcol = mcol;
+    return GOLAP_OK;
+}
+
 uint32_t group_bits_of(const golap_batch *b) { return std::max<uint32_t>(1, bits_for(b->max_domain)); }
 
 // LDS group keys are (query << group_bits | group) and must stay below the empty marker.
 bool lds_keys_fit(const golap_batch *b) { return bits_for(b->n_queries()) + group_bits_of(b) <= 31; }
 
 size_t lds_bytes(const golap_batch *b, const golap_batch_config &c) {
+    if (b->kind == BatchKind::STAR && c.shared_filter) return star_lds_bytes(0, b->page_rows, b->n_windows, 0);
     if (b->kind == BatchKind::STAR)
         return star_lds_bytes(c.filter_lds ? b->max_filter_words : 0, b->page_rows, b->n_windows, c.lds_group_slots);
     return scan_lds_bytes(b->win_bytes, c.lds_group_slots, scan_page_waves(b->ctx, b->win_bytes, c.lds_group_slots));
@@ -80,7 +130,9 @@ int check_config(golap_batch *b, const golap_batch_config &c) {
         return bfail(b, GOLAP_EINVAL, "batch: lds_group_slots must be 0 or a power of two <= 4096 (star) / 8192 (scan)");
     if (c.lds_group_slots && b->kind == BatchKind::STAR && !lds_keys_fit(b))
         return bfail(b, GOLAP_EINVAL, "batch: too many queries x groups for LDS group keys");
-    if (c.filter_lds > 1 || c.compact_tables > 1) return bfail(b, GOLAP_EINVAL, "batch: bad flag");
+    if (c.filter_lds > 1 || c.compact_tables > 1 || c.shared_filter > 1) return bfail(b, GOLAP_EINVAL, "batch: bad flag");
+    if (c.shared_filter && (b->kind != BatchKind::STAR || !b->d_mfilt))
+        return bfail(b, GOLAP_EINVAL, "batch: no shared filter for this batch");
     const size_t budget = b->kind == BatchKind::SCAN ? scan_lds_budget(b->ctx) : (size_t)b->ctx->lds_max;
     if (lds_bytes(b, c) > budget) return bfail(b, GOLAP_EINVAL, "batch: LDS budget exceeded");
     return GOLAP_OK;
@@ -149,6 +201,11 @@ hipError_t launch_kernel(golap_batch *b, hipStream_t s, bool prepare_only) {
         L.group_bits = group_bits_of(b);
         L.max_blocks_per_cu = c.max_blocks_per_cu;
         L.err = reinterpret_cast<uint32_t *>(sl->d_arena + 8);
+        if (c.shared_filter) {
+            L.items4 = sl->d_items4;
+            L.mfilt = b->d_mfilt;
+            L.mcol = (uint32_t)b->mcol;
+        }
         return launch_star(b->ctx, L, s, prepare_only);
     }
     ScanLaunch L{};
@@ -178,7 +235,11 @@ hipError_t launch_kernel(golap_batch *b, hipStream_t s, bool prepare_only) {
 int enqueue_kernels(golap_batch *b, hipStream_t s) {
     BatchSlot *sl = b->slot;
     BATCH_HIP_CHECK(b, launch_fill_zero(sl->d_arena, b->arena_bytes, s));
-    if (b->kind == BatchKind::STAR)
+    if (b->kind == BatchKind::STAR && b->cfg.shared_filter)
+        BATCH_HIP_CHECK(b, launch_mask_items(sl->d_q, desc_size(b), b->n_queries(), b->union_begin,
+                                             b->union_end - b->union_begin, parts_of(b, b->cfg.pages_per_item),
+                                             sl->d_items4, reinterpret_cast<uint32_t *>(sl->d_arena), s));
+    else if (b->kind == BatchKind::STAR)
         BATCH_HIP_CHECK(b, launch_batch_items(sl->d_q, desc_size(b), b->n_queries(), sl->d_chunk_off,
                                               b->chunk_off.back(), parts_of(b, b->cfg.pages_per_item), sl->d_items,
                                               reinterpret_cast<uint32_t *>(sl->d_arena), s));
@@ -228,6 +289,7 @@ golap_batch_config default_config(golap_batch *b) {
     if (b->kind == BatchKind::STAR) {
         c.lds_group_slots = lds_keys_fit(b) ? LDS_GROUP_SLOTS_MAX : 0;
         while (c.lds_group_slots && lds_bytes(b, c) > (size_t)b->ctx->lds_max) c.lds_group_slots >>= 1;
+        c.shared_filter = 0;  // the tuner tries it
     } else {
         // dense partials for several plans per item (4096 entries: about four request
         // templates' zone x hour partials per pass); items small enough to fill the device
@@ -299,9 +361,9 @@ int finish_create(golap_ctx *ctx, golap_batch *b, bool with_graph) {
         b->total_pages += b->surv_pages[i];
         b->total_bytes += tmp.comp_bytes;
     }
-    if (b->kind == BatchKind::SCAN) {
-        // shared scan: the union of the slices, the surviving (chunk, plan word) pairs and the
-        // bytes read once per chunk for all plans that keep it
+    if (b->kind == BatchKind::SCAN || b->d_mfilt) {
+        // shared scan / shared star filter: the union of the slices, the surviving (chunk, plan
+        // word) pairs and the bytes read once per chunk for all plans that keep it
         b->union_begin = ~0ull;
         b->union_end = 0;
         for (uint32_t i = 0; i < nq; ++i) {
@@ -317,7 +379,7 @@ int finish_create(golap_ctx *ctx, golap_batch *b, bool with_graph) {
         for (uint32_t i = 0; i < nq; ++i) {
             const SliceDev &d = slice_of(b, i);
             for (uint64_t k = 0; k < d.chunk_end - d.chunk_begin; ++k)
-                if (b->survive[i][k]) cmask[(d.chunk_begin + k - b->union_begin) * nw + i / 64] |= b->sd[i].colmask | (1u << 31);
+                if (b->survive[i][k]) cmask[(d.chunk_begin + k - b->union_begin) * nw + i / 64] |= colmask_of(b, i) | (1u << 31);
         }
         b->union_pairs = 0;
         b->shared_bytes = 0;
@@ -344,7 +406,7 @@ int finish_create(golap_ctx *ctx, golap_batch *b, bool with_graph) {
     b->arena_bytes = off;
     const uint64_t items = std::max<uint64_t>(1, b->chunk_off.back() * b->ppc);
     if (items >= (1ull << 32)) return fail(ctx, GOLAP_EINVAL, "batch: too many work items");
-    const uint64_t items4 = b->kind == BatchKind::SCAN ? std::max<uint64_t>(1, b->union_pairs * b->ppc) : 0;
+    const uint64_t items4 = b->kind == BatchKind::SCAN || b->d_mfilt ? std::max<uint64_t>(1, b->union_pairs * b->ppc) : 0;
     if (items4 >= (1ull << 32)) return fail(ctx, GOLAP_EINVAL, "batch: too many work items");
     if (int r = b->slot->reserve(off, (uint64_t)nq * desc_size(b), b->chunk_off.size(), b->kind == BatchKind::STAR ? items : 1,
                                  items4))
@@ -487,6 +549,8 @@ int batch_create_star(golap_ctx *ctx, const golap_star_query *q, uint32_t n, Bat
         pe = std::max(pe, b->qd[i].s.page_end);
     }
     b->n_windows = assign_windows(b->cs, std::min(pb, pe), pe, b->windows);
+    if (n > 1)
+        if (int r = build_shared_filter(ctx, b)) return bail(r);
     if (int r = finish_create(ctx, b, with_graph)) return bail(r);
     *out = b;
     return GOLAP_OK;
@@ -627,6 +691,7 @@ int batch_result(golap_batch *b, uint32_t q, golap_star_result *res, int32_t *ke
 void batch_free(golap_batch *b) {
     if (!b) return;
     drop_graph(b);
+    if (b->d_mfilt) dev_release(b->ctx, b->d_mfilt);
     if (b->own_slot && b->slot) {
         b->slot->release();
         delete b->slot;
@@ -707,7 +772,7 @@ int golap_batch_tune(golap_batch *b, uint32_t reps, golap_batch_config *best, do
     if (int r = time_cfg(cur, &cur_ms)) return fail(ctx, r, b->err);
     const bool star = b->kind == BatchKind::STAR;
     // coordinate descent over the knobs, in order of expected effect
-    enum Axis { LDS, GRAIN, FILTER, TABLES, OCC };
+    enum Axis { SHARED, LDS, GRAIN, FILTER, TABLES, OCC };
     std::vector<std::pair<Axis, std::vector<golap_batch_config>>> axes;
     {
         std::vector<golap_batch_config> ax;
@@ -748,10 +813,21 @@ int golap_batch_tune(golap_batch *b, uint32_t reps, golap_batch_config *best, do
         }
         axes.emplace_back(OCC, ax);
     }
+    if (star && b->d_mfilt) {
+        // the shared filter last, against the tuned per-query schedule; then grain and
+        // occupancy once more for whichever schedule won
+        golap_batch_config on{}, off{};
+        on.shared_filter = 1;
+        const auto grain = axes[1], occ = axes.back();
+        axes.emplace_back(SHARED, std::vector<golap_batch_config>{on, off});
+        axes.push_back(grain);
+        axes.push_back(occ);
+    }
     for (const auto &axis : axes)
         for (const golap_batch_config &v : axis.second) {
             golap_batch_config c = cur;
             switch (axis.first) {
+            case SHARED: c.shared_filter = v.shared_filter; break;
             case LDS: c.lds_group_slots = v.lds_group_slots; c.lds_cutover = v.lds_cutover; break;
             case GRAIN: c.pages_per_item = v.pages_per_item; break;
             case FILTER: c.filter_lds = v.filter_lds; break;

@@ -127,6 +127,18 @@ int ensure_hfilter(golap_ctx *ctx, const golap_hashmap *dm) {
     return GOLAP_OK;
 }
 
+int filter_view(golap_ctx *ctx, const golap_hashmap *dm, DriverFilterView *out, uint32_t *words) {
+    if (dm->use_bitmap && (dm->krange + 31) / 32 <= (1u << HASH_FILTER_WORDS_LOG2)) {
+        *out = DriverFilterView{dm->d_bitmap, dm->kmin, dm->krange, 0};
+        *words = (dm->krange + 31) / 32;
+        return GOLAP_OK;
+    }
+    if (int r = ensure_hfilter(ctx, dm)) return r;
+    *out = DriverFilterView{dm->d_hfilter, 0, 0, HASH_FILTER_WORDS_LOG2};
+    *words = 1u << HASH_FILTER_WORDS_LOG2;
+    return GOLAP_OK;
+}
+
 // Validates a plan and fills its device descriptor (all but use_lds, item_begin and the result
 // pointers): fact column indexes, driver, probe tables, driver filter, slice and group strides.
 int prepare_query(golap_ctx *ctx, const golap_star_plan *plan, bool compact, ColumnSet &cs, StarQueryDev &qd,
@@ -196,15 +208,7 @@ int prepare_query(golap_ctx *ctx, const golap_star_plan *plan, bool compact, Col
     qd.n_dims = (uint8_t)nd;
     qd.val_signed = v->type == GOLAP_I32;
     // driver filter: dense exact bitmap, or the blocked Bloom filter of the map's keys
-    const golap_hashmap *dm = plan->dims[qd.driver].map;
-    if (dm->use_bitmap && (dm->krange + 31) / 32 <= (1u << HASH_FILTER_WORDS_LOG2)) {
-        qd.filter = DriverFilterView{dm->d_bitmap, dm->kmin, dm->krange, 0};
-        qd.filter_words = (dm->krange + 31) / 32;
-    } else {
-        if (int r = ensure_hfilter(ctx, dm)) return r;
-        qd.filter = DriverFilterView{dm->d_hfilter, 0, 0, HASH_FILTER_WORDS_LOG2};
-        qd.filter_words = 1u << HASH_FILTER_WORDS_LOG2;
-    }
+    if (int r = filter_view(ctx, plan->dims[qd.driver].map, &qd.filter, &qd.filter_words)) return r;
     const uint64_t ppc = v->chunk_rows / v->page_rows;
     qd.s.chunk_begin = plan->chunk_begin;
     qd.s.chunk_end = plan->chunk_end;

@@ -218,10 +218,17 @@ struct StarLaunch {
     uint32_t group_bits;
     uint32_t max_blocks_per_cu;   // 0 = occupancy limit
     uint32_t *err;
+    // shared-filter schedule (items4 != nullptr): items {chunk, part | word << 8, mask lo, mask
+    // hi} over the queries of one word of 64; a page's keys of fact column mcol are read once
+    // and tested against mfilt[q], the membership filter of query q's dimension on that column
+    const uint4 *items4;
+    const DriverFilterView *mfilt;
+    uint32_t mcol;
 };
 // prepare_only: set the kernel's LDS attribute and check the launch without launching (graph
 // capture then records the launch alone).
 hipError_t launch_star(golap_ctx *ctx, const StarLaunch &l, hipStream_t stream, bool prepare_only = false);
+
 // Batch work list: one thread per (query, chunk of its slice); surviving chunks (chunk mask and
 // zone predicates) emit parts items each into items, counted in queue[1]. Query q's slice is
 // the SliceDev at qbase + q * qstride.
@@ -312,6 +319,9 @@ hipError_t launch_scan(golap_ctx *ctx, const ScanLaunch &l, hipStream_t stream, 
 // bits of the word's queries whose slice keeps the chunk and, if any, appends `parts` items
 // {chunk, part | word << 8, mask lo, mask hi}.
 hipError_t launch_scan_items(const ScanQueryDev *queries, uint32_t n_queries, uint64_t c0, uint64_t nch,
+                             uint32_t parts, uint4 *items, uint32_t *queue, hipStream_t stream);
+// The same for any descriptors that start with their SliceDev (query q's at qbase + q * qstride).
+hipError_t launch_mask_items(const void *qbase, uint32_t qstride, uint32_t n_queries, uint64_t c0, uint64_t nch,
                              uint32_t parts, uint4 *items, uint32_t *queue, hipStream_t stream);
 size_t scan_lds_bytes(uint32_t win_bytes, uint32_t agg_entries, uint32_t page_waves);
 // Dynamic LDS a scan launch may use (the device limit less the kernel's own LDS).

@@ -886,8 +886,8 @@ __global__ __launch_bounds__(BLOCK, HH == 4 ? 4 : 2) void k_scan(ScanArgs a) {
 
 // Scan work list: wave w handles (chunk, word) pairs [64w, 64w + 64); pairs with a plan that
 // keeps the chunk are compacted with a wave ballot and one global atomic per wave.
-__global__ __launch_bounds__(BLOCK) void k_scan_items(const ScanQueryDev *qs, uint32_t nq, uint64_t c0, uint64_t nch,
-                                                      uint32_t parts, uint4 *items, uint32_t *queue) {
+__global__ __launch_bounds__(BLOCK) void k_mask_items(const uint8_t *qbase, uint32_t qstride, uint32_t nq, uint64_t c0,
+                                                      uint64_t nch, uint32_t parts, uint4 *items, uint32_t *queue) {
     const uint32_t lane = threadIdx.x % WAVE;
     const uint32_t nw = (nq + WAVE - 1) / WAVE;
     const uint64_t total = nch * nw;
@@ -902,7 +902,7 @@ __global__ __launch_bounds__(BLOCK) void k_scan_items(const ScanQueryDev *qs, ui
             word = (uint32_t)(t % nw);
             const uint32_t qe = min(nq, (word + 1) * (uint32_t)WAVE);
             for (uint32_t q = word * WAVE; q < qe; ++q) {
-                const SliceDev &s = qs[q].s;
+                const SliceDev &s = *reinterpret_cast<const SliceDev *>(qbase + (size_t)q * qstride);
                 if (c >= s.chunk_begin && c < s.chunk_end && slice_keeps(s, c)) mask |= 1ull << (q - word * WAVE);
             }
         }
@@ -1000,16 +1000,21 @@ hipError_t launch_scan(golap_ctx *ctx, const ScanLaunch &l, hipStream_t stream, 
                                 : launch_scan_t(ctx, k_scan<4>, a, lds, stream, l.max_blocks_per_cu, prepare_only);
 }
 
-hipError_t launch_scan_items(const ScanQueryDev *queries, uint32_t n_queries, uint64_t c0, uint64_t nch,
+hipError_t launch_mask_items(const void *qbase, uint32_t qstride, uint32_t n_queries, uint64_t c0, uint64_t nch,
                              uint32_t parts, uint4 *items, uint32_t *queue, hipStream_t stream) {
     if (!n_queries || !parts || parts > 256 || n_queries > (1u << 24)) return hipErrorInvalidValue;
     const uint64_t total = nch * ((n_queries + WAVE - 1) / WAVE);
     if (!total) return hipSuccess;
     const uint64_t waves = (total + WAVE - 1) / WAVE;
     const uint32_t grid = (uint32_t)std::min<uint64_t>((waves + BLOCK / WAVE - 1) / (BLOCK / WAVE), 4096);
-    hipLaunchKernelGGL(k_scan_items, dim3(grid), dim3(BLOCK), 0, stream, queries, n_queries, c0, nch, parts, items,
-                       queue);
+    hipLaunchKernelGGL(k_mask_items, dim3(grid), dim3(BLOCK), 0, stream, static_cast<const uint8_t *>(qbase), qstride,
+                       n_queries, c0, nch, parts, items, queue);
     return hipGetLastError();
+}
+
+hipError_t launch_scan_items(const ScanQueryDev *queries, uint32_t n_queries, uint64_t c0, uint64_t nch,
+                             uint32_t parts, uint4 *items, uint32_t *queue, hipStream_t stream) {
+    return launch_mask_items(queries, (uint32_t)sizeof(ScanQueryDev), n_queries, c0, nch, parts, items, queue, stream);
 }
 
 }  // namespace golap_impl

@@ -44,9 +44,16 @@ struct StarArgs {
     uint32_t lds_gslots;
     uint32_t group_bits;
     uint32_t *err;
+    // shared-filter schedule (see StarLaunch)
+    const uint4 *items4;
+    const DriverFilterView *mfilt;
+    uint32_t mcol;
+    uint32_t n_cols;
 };
 
 constexpr uint32_t QDESC_WORDS = sizeof(StarQueryDev) / 4;
+static_assert(offsetof(StarQueryDev, s) == 0 && offsetof(ScanQueryDev, s) == 0,
+              "work lists read a descriptor's slice at its start");
 static_assert(sizeof(StarQueryDev) % 8 == 0, "query descriptor copied as words");
 
 // LDS of one workgroup besides the dynamic arrays.
@@ -298,6 +305,197 @@ __global__ __launch_bounds__(BLOCK) void k_star(StarArgs a) {
     }
 }
 
+// Shared-filter star kernel (batches whose queries all probe one dimension column): work items
+// are (chunk part, word of 64 queries, mask of the queries that keep the chunk). Per page tile:
+//   1. the access plans of the columns the item's queries read;
+//   2. the shared column's keys are read once into registers and tested against the membership
+//      filter of every query of the item (each query's filter of its dimension on that column);
+//   3. rows that pass any filter are compacted into the LDS candidate list (as in k_star);
+//   4. a candidate row tests the filters again query by query, and for each query that passes
+//      probes every dimension of that query exactly (the shared one too, for its payload) and
+//      adds SUM/COUNT to the query's dense global result.
+// A page's catalog, shared-column reads, compaction and barriers are paid once however many
+// queries keep the chunk.
+struct SharedLds {
+    ColAccess acc[STAR_MAX_COLS];
+    uint32_t cand_n, tile_bad, item, colmask;
+};
+
+template <int PER>
+__global__ __launch_bounds__(BLOCK) void k_star_shared(StarArgs a) {
+    constexpr int G = PER / 4;
+    extern __shared__ __align__(16) uint8_t lds[];
+    __shared__ SharedLds S;
+    const uint32_t P = a.page_rows;
+    const uint32_t wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    uint16_t *cand = reinterpret_cast<uint16_t *>(lds);
+    uint8_t *win = lds + (((size_t)P * 2 + 15) & ~(size_t)15);
+    const uint32_t n_items = a.queue[1];
+    const uint32_t mcol = a.mcol;
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) S.item = atomicAdd(&a.queue[0], 1u);
+        __syncthreads();
+        const uint32_t it = S.item;
+        if (it >= n_items) break;
+        const uint4 w = a.items4[it];
+        const uint64_t c = w.x;
+        const uint32_t part = w.y & 0xffu, word = w.y >> 8;
+        const uint64_t mask = (uint64_t)w.z | ((uint64_t)w.w << 32);
+        const DriverFilterView *mf = a.mfilt + (size_t)word * WAVE;
+        // the columns the item's queries read (wave 0, one lane per query of the word)
+        if (threadIdx.x < WAVE) {
+            const uint32_t q = word * WAVE + lane;
+            uint32_t cm = 0;
+            if (((mask >> lane) & 1u) && q < a.n_queries) {
+                const StarQueryDev &Q = a.queries[q];
+                cm = 1u << Q.val_col;
+                for (uint32_t d = 0; d < Q.n_dims; ++d) cm |= 1u << Q.fk_col[d];
+            }
+#pragma unroll
+            for (int o = WAVE / 2; o > 0; o >>= 1) cm |= __shfl_xor(cm, o, WAVE);
+            if (lane == 0) S.colmask = cm | (1u << mcol);
+        }
+        const uint64_t c0 = c * a.pages_per_chunk;
+        const uint64_t pb = c0 + (uint64_t)part * a.pages_per_item;
+        const uint64_t pe = min(min(pb + a.pages_per_item, c0 + a.pages_per_chunk), a.n_pages);
+        for (uint64_t p = pb; p < pe; ++p) {
+            const uint32_t rows = (uint32_t)min<uint64_t>(P, a.n_rows - p * P);
+            __syncthreads();  // the previous tile's readers are done; colmask is visible
+            const uint32_t colmask = S.colmask;
+            if (threadIdx.x == 0) {
+                S.cand_n = 0;
+                S.tile_bad = 0;
+            }
+            if (threadIdx.x < a.n_cols && ((colmask >> threadIdx.x) & 1u)) {
+                const StarCol &sc = a.col[threadIdx.x];
+                S.acc[threadIdx.x] = col_access(sc, p, rows, sc.window >= 0 ? win + (size_t)sc.window * P * 4 : nullptr);
+            }
+            __syncthreads();
+            bool bad = false;
+            for (uint32_t k = 0; k < a.n_cols; ++k) bad |= ((colmask >> k) & 1u) && S.acc[k].mode == ACC_BAD;
+            if (bad) {  // uniform
+                if (threadIdx.x == 0) raise_err(a.err, DEVERR_CORRUPT);
+                continue;
+            }
+            const ColAccess dacc = S.acc[mcol];
+            if (dacc.mode == ACC_WINDOW) {
+                if (wave == 0) {
+                    const ColView &cv = a.col[mcol].c;
+                    const int64_t got = page_decode_wave(cv.page_codec[p], cv.comp + cv.page_off[p], cv.page_csize[p],
+                                                         const_cast<uint8_t *>(dacc.base), rows * 4, 4);
+                    if (got != (int64_t)rows * 4 && lane == 0) {
+                        S.tile_bad = 1;
+                        raise_err(a.err, DEVERR_CORRUPT);
+                    }
+                }
+                __syncthreads();
+            }
+            // the shared column's keys once, tested against every query's filter
+            uint32_t passbits = 0;
+            if (!S.tile_bad) {
+                int32_t kv[PER];
+                uint32_t valid = 0;
+                if (rows == (uint32_t)(PER * BLOCK) && dacc.mode <= ACC_WINDOW) {
+#pragma unroll
+                    for (int k = 0; k < G; ++k) {
+                        const int4 v = *reinterpret_cast<const int4 *>(dacc.base + 16 * ((size_t)threadIdx.x + (size_t)k * BLOCK));
+                        kv[4 * k] = v.x;
+                        kv[4 * k + 1] = v.y;
+                        kv[4 * k + 2] = v.z;
+                        kv[4 * k + 3] = v.w;
+                    }
+                    valid = PER == 32 ? 0xffffffffu : (1u << PER) - 1;
+                } else {
+#pragma unroll
+                    for (int k = 0; k < G; ++k)
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            const uint32_t r = 4 * (threadIdx.x + k * BLOCK) + j;
+                            kv[4 * k + j] = r < rows ? col_value(dacc, r) : 0;
+                            valid |= (uint32_t)(r < rows) << (4 * k + j);
+                        }
+                }
+                for (uint64_t m = mask; m; m &= m - 1) {  // uniform
+                    const DriverFilterView f = mf[__ffsll((long long)m) - 1];
+#pragma unroll
+                    for (int j = 0; j < PER; ++j) passbits |= (uint32_t)filter_test(f, kv[j]) << j;
+                }
+                passbits &= valid;
+            }
+            const uint32_t mine = __popc(passbits);
+            uint32_t incl = mine;
+#pragma unroll
+            for (int d = 1; d < WAVE; d <<= 1) {
+                const uint32_t t = __shfl_up(incl, d, WAVE);
+                if (lane >= (uint32_t)d) incl += t;
+            }
+            const uint32_t wtotal = __shfl(incl, WAVE - 1, WAVE);
+            uint32_t wbase = 0;
+            if (lane == WAVE - 1 && wtotal) wbase = atomicAdd(&S.cand_n, wtotal);
+            wbase = __shfl(wbase, WAVE - 1, WAVE);
+            uint32_t slot = wbase + incl - mine;
+            for (uint32_t bits = passbits; bits; bits &= bits - 1) {
+                const uint32_t b = __ffs(bits) - 1;
+                cand[slot++] = (uint16_t)(4 * (threadIdx.x + (b >> 2) * BLOCK) + (b & 3));
+            }
+            __syncthreads();
+            const uint32_t nc = S.cand_n;
+            if (nc == 0 || S.tile_bad) continue;
+            // other columns that need a decode: wave w decodes columns w, w+4, ...
+            for (uint32_t k = wave; k < a.n_cols; k += BLOCK / WAVE) {
+                if (k == mcol || !((colmask >> k) & 1u) || S.acc[k].mode != ACC_WINDOW) continue;
+                const ColView &cv = a.col[k].c;
+                const int64_t got = page_decode_wave(cv.page_codec[p], cv.comp + cv.page_off[p], cv.page_csize[p],
+                                                     const_cast<uint8_t *>(S.acc[k].base), rows * 4, 4);
+                if (got != (int64_t)rows * 4 && lane == 0) {
+                    S.tile_bad = 1;
+                    raise_err(a.err, DEVERR_CORRUPT);
+                }
+            }
+            __syncthreads();
+            if (S.tile_bad) continue;
+            for (uint32_t i = threadIdx.x; i < nc; i += BLOCK) {
+                const uint32_t r = cand[i];
+                const int32_t mk = col_value(S.acc[mcol], r);
+                for (uint64_t qm = mask; qm; qm &= qm - 1) {
+                    const uint32_t b = (uint32_t)__ffsll((long long)qm) - 1;
+                    if (!filter_test(mf[b], mk)) continue;
+                    const StarQueryDev &Q = a.queries[word * WAVE + b];
+                    const uint32_t nd = Q.n_dims;
+                    int32_t key[STAR_MAX_DIMS];
+                    uint32_t h[STAR_MAX_DIMS];
+                    uint64_t first[STAR_MAX_DIMS];
+                    for (uint32_t d = 0; d < nd; ++d) {
+                        key[d] = col_value(S.acc[Q.fk_col[d]], r);
+                        h[d] = hash32((uint32_t)key[d]) >> (32 - Q.dim[d].log2cap);
+                        first[d] = Q.dim[d].slots[h[d]];
+                    }
+                    uint64_t gidx = 0;
+                    bool ok = true;
+                    for (uint32_t d = 0; d < nd && ok; ++d) {
+                        const StarDimProbe &dd = Q.dim[d];
+                        uint32_t pay = 0;
+                        ok = probe_exact(dd.slots, dd.log2cap, key[d], first[d], h[d], pay);
+                        if (ok && dd.group_card) {
+                            const uint32_t gv = pay - (uint32_t)dd.group_lo;
+                            if (gv >= dd.group_card) {
+                                raise_err(a.err, DEVERR_KEYRANGE);
+                                ok = false;
+                            }
+                            gidx += (uint64_t)gv * dd.group_stride;
+                        }
+                    }
+                    if (!ok) continue;
+                    const int32_t raw = col_value(S.acc[Q.val_col], r);
+                    const int64_t v = Q.val_signed ? (int64_t)raw : (int64_t)(uint32_t)raw;
+                    group_add_global(Q.g_sum, Q.g_cnt, gidx, v);
+                }
+            }
+        }
+    }
+}
+
 size_t star_lds_bytes(uint32_t filter_words, uint32_t page_rows, uint32_t n_windows, uint32_t lds_gslots) {
     return (((size_t)filter_words * 4 + 15) & ~(size_t)15) + (((size_t)page_rows * 2 + 15) & ~(size_t)15) +
            (size_t)n_windows * page_rows * 4 + (size_t)lds_gslots * 16;
@@ -339,6 +537,9 @@ hipError_t launch_star(golap_ctx *ctx, const StarLaunch &l, hipStream_t stream, 
     if (l.n_cols > STAR_MAX_COLS || (!l.queries && l.n_queries != 1) || (l.items && !l.queue) ||
         (l.items && (!l.pages_per_item || l.pages_per_item > l.pages_per_chunk)))
         return hipErrorInvalidValue;
+    if (l.items4 && (!l.queries || !l.queue || !l.mfilt || l.mcol >= l.n_cols || !l.pages_per_item ||
+                     l.pages_per_item > l.pages_per_chunk))
+        return hipErrorInvalidValue;
     StarArgs a{};
     for (uint32_t c = 0; c < l.n_cols; ++c) a.col[c] = StarCol{l.col[c], l.col_window[c]};
     a.page_rows = l.page_rows;
@@ -357,10 +558,27 @@ hipError_t launch_star(golap_ctx *ctx, const StarLaunch &l, hipStream_t stream, 
     a.lds_gslots = l.lds_gslots;
     a.group_bits = l.group_bits;
     a.err = l.err;
+    a.items4 = l.items4;
+    a.mfilt = l.mfilt;
+    a.mcol = l.mcol;
+    a.n_cols = l.n_cols;
+    const uint32_t mb = l.max_blocks_per_cu;
+    if (l.items4) {  // shared filter: candidate list and windows only
+        a.filter_words = 0;
+        a.lds_gslots = 0;
+        const size_t lds = star_lds_bytes(0, l.page_rows, l.n_windows, 0);
+        if (lds > (size_t)ctx->lds_max) return hipErrorInvalidConfiguration;
+        switch (l.page_rows) {
+        case 1024: return launch_star_t(ctx, k_star_shared<4>, a, lds, stream, mb, ~0ull, prepare_only);
+        case 2048: return launch_star_t(ctx, k_star_shared<8>, a, lds, stream, mb, ~0ull, prepare_only);
+        case 4096: return launch_star_t(ctx, k_star_shared<16>, a, lds, stream, mb, ~0ull, prepare_only);
+        case 8192: return launch_star_t(ctx, k_star_shared<32>, a, lds, stream, mb, ~0ull, prepare_only);
+        default: return hipErrorInvalidValue;
+        }
+    }
     if (!l.items && !a.n_items) return hipSuccess;
     const size_t lds = star_lds_bytes(a.filter_words, l.page_rows, l.n_windows, l.lds_gslots);
     if (lds > (size_t)ctx->lds_max) return hipErrorInvalidConfiguration;
-    const uint32_t mb = l.max_blocks_per_cu;
     // the queue schedule's item count is only known on the device: fill the chip
     const uint64_t max_grid = l.items ? ~0ull : a.n_items;
 #define GOLAP_STAR_CASE(ROWS, PER)                                                                      \

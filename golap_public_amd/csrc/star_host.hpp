@@ -55,6 +55,9 @@ int emit_groups_host(golap_ctx *ctx, uint64_t n, const int64_t *hs, const uint64
 int prepare_query(golap_ctx *ctx, const golap_star_plan *plan, bool compact, ColumnSet &cs, StarQueryDev &qd,
                   QueryHost &qh);
 uint32_t assign_windows(const ColumnSet &cs, uint64_t pb, uint64_t pe, int *win);
+int ensure_hfilter(golap_ctx *ctx, const golap_hashmap *dm);
+// Membership filter of a map's keys: its dense bitmap when that is small, else its Bloom filter.
+int filter_view(golap_ctx *ctx, const golap_hashmap *dm, DriverFilterView *out, uint32_t *words);
 void slice_stats(const golap_star_plan &plan, const std::vector<uint8_t> &run, golap_star_result *res);
 int status_of_flags(golap_ctx *ctx, uint32_t flags, const char *what);
 bool is_int_type(uint32_t t);

@@ -347,6 +347,11 @@ typedef struct {
     uint32_t filter_lds;         /* star: 1 = copy each query's driver filter into LDS */
     uint32_t compact_tables;     /* star: 1 = probe the compact tables (2 x filled slots) */
     uint32_t max_blocks_per_cu;  /* 0 = occupancy limit */
+    uint32_t shared_filter;      /* star: 1 = test each page's rows once for all queries against a
+                                    membership table of one dimension column they all probe
+                                    (available when every query has that dimension and its
+                                    keys span at most 2^27 values); candidates then probe
+                                    every dimension per query, aggregating in global memory */
 } golap_batch_config;
 typedef struct {
     double ms;                   /* device time of the run (graph or plain launches) */

@@ -131,10 +131,21 @@ def test_batch_matches_single_and_oracle(orc, golap, ctx, fx):
                 if i < 4:
                     wk, ws, wc = fx.oracle(orc, q)
                     assert np.array_equal(got[1], wk) and np.array_equal(got[2], ws) and np.array_equal(got[3], wc)
-            # the device work list holds the surviving chunks' parts
-            parts = -(-16 // b.config().pages_per_item)
-            assert info.items == sum(b.result(i)[0].chunks_run for i in range(len(qs))) * parts
+            # the device work list holds the surviving chunks' parts: one item per (query,
+            # chunk part)
+            cfg = b.config()
+            assert cfg.shared_filter == 0
+            parts = -(-16 // cfg.pages_per_item)
+            runs = [b.result(i)[0].chunks_run for i in range(len(qs))]
+            assert info.items == sum(runs) * parts
             assert info.comp_bytes == sum(b.result(i)[0].comp_bytes for i in range(len(qs)))
+            # with the shared filter (every query probes the date column): one item per chunk
+            # part that any query keeps, and the same groups
+            b.set_config(shared_filter=1)
+            info = b.run()
+            assert info.items % parts == 0 and max(runs) * parts <= info.items < sum(runs) * parts
+            for i, q in enumerate(qs):
+                assert same(b.result(i), fx.single(q, flags)), ("shared filter", i)
     finally:
         ctx.free(flags)
 
@@ -157,10 +168,14 @@ def test_batch_graph_replays_and_plain_launches_agree(golap, ctx, fx):
 
 
 CONFIGS = [
-    dict(pages_per_item=1, lds_group_slots=0, filter_lds=0, compact_tables=0),
-    dict(pages_per_item=4, lds_group_slots=1024, lds_cutover=0, filter_lds=1, compact_tables=1),
-    dict(pages_per_item=16, lds_group_slots=4096, lds_cutover=0, max_blocks_per_cu=1),
-    dict(pages_per_item=2, lds_group_slots=2048, lds_cutover=1 << 30, filter_lds=0, max_blocks_per_cu=2),
+    dict(pages_per_item=1, lds_group_slots=0, filter_lds=0, compact_tables=0, shared_filter=0),
+    dict(pages_per_item=4, lds_group_slots=1024, lds_cutover=0, filter_lds=1, compact_tables=1, shared_filter=0),
+    dict(pages_per_item=16, lds_group_slots=4096, lds_cutover=0, max_blocks_per_cu=1, shared_filter=0),
+    dict(pages_per_item=2, lds_group_slots=2048, lds_cutover=1 << 30, filter_lds=0, max_blocks_per_cu=2,
+         shared_filter=0),
+    # one membership test per row for all queries (every query here probes the date dimension)
+    dict(pages_per_item=1, compact_tables=0, shared_filter=1),
+    dict(pages_per_item=16, compact_tables=1, max_blocks_per_cu=2, shared_filter=1),
 ]
 
 
