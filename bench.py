@@ -300,6 +300,15 @@ def run_batch(args):
     best, tuned_ms = b.tune(reps=3)
     tune_s = time.perf_counter() - t0
     log(f"autotuner: {default_cfg} -> {best.as_dict()} ({tuned_ms:.3f} ms, {tune_s:.1f} s)")
+    # the shared-filter schedule under the tuned grain and occupancy, for the record
+    shared_ms = None
+    try:
+        b.set_config(**dict(best.as_dict(), shared_filter=1))
+        shared_ms = statistics.median(b.run().ms for _ in range(3))
+    except g.GolapError:
+        pass
+    b.set_config(**best.as_dict())
+    log(f"shared-filter schedule: {shared_ms} ms")
     for _ in range(args.warmup):
         b.run()
         plain_all()
@@ -338,7 +347,8 @@ def run_batch(args):
                       "plain_queries_per_s": len(qs) / pwall, "plain_ms": pwall * 1e3,
                       "plain_kernel_ms": statistics.median(plain_dev),
                       "speedup_vs_plain": pwall / wall, "map_build_ms_all_queries": build_ms,
-                      "tuned_config": best.as_dict(), "default_config": default_cfg, "tune_s": tune_s},
+                      "tuned_config": best.as_dict(), "default_config": default_cfg, "tune_s": tune_s,
+                      "shared_filter_ms": shared_ms},
            "roofline": {"bound": "hbm", "achieved": gbps, "peak": 8000.0, "unit": "GB/s", "frac": gbps / 8000.0,
                         "traffic": traffic_of("star_batch_64q_600M")}}
     calibrate(ctx, args)
@@ -612,6 +622,15 @@ def main():
             traffic = None
     log(f"sum {r.sum} matches {r.matches}; {ms_per_step:.3f} ms/step, kernel {kernel_ms:.3f} ms, "
         f"{achieved:.0f} GB/s")
+    # what a plain streaming read of 4 GiB reaches on this device (best of 5), beside the peak
+    probe_bytes = 4 << 30
+    pbuf = ctx.alloc(probe_bytes)
+    ctx.memset(pbuf, 1, probe_bytes)
+    probe_ms = min(ctx.read_probe(pbuf, probe_bytes) for _ in range(6))
+    ctx.free(pbuf)
+    probe_gbps = probe_bytes / (probe_ms * 1e-3) / 1e9
+    log(f"plain streaming read: {probe_gbps:.0f} GB/s; the fused kernel's HBM traffic moves at "
+        f"{(traffic or algo_bytes) / (kernel_ms * 1e-3) / 1e9:.0f} GB/s")
 
     out = {
         "metric": METRIC, "value": value, "unit": "rows/s", "n_gpus": 1, "steps": args.steps,
@@ -623,7 +642,8 @@ def main():
                    "comp_bytes": r.comp_bytes, "raw_bytes": raw_bytes,
                    "plan": "year=1995, discount 4..6, quantity>=30, SUM(price*discount)",
                    "result_sum": r.sum, "result_matches": r.matches, "kernel_ms": kernel_ms,
-                   "device": name},
+                   "device": name, "read_probe_GBps": probe_gbps,
+                   "hbm_traffic_GBps": (traffic / (kernel_ms * 1e-3) / 1e9) if traffic else None},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
     }

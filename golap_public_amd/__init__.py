@@ -139,6 +139,7 @@ _PROTOS = {
     "golap_d2h": (C.c_int, [vp, vp, vp, u64]),
     "golap_memset": (C.c_int, [vp, vp, C.c_int, u64]),
     "golap_d2d": (C.c_int, [vp, vp, vp, u64]),
+    "golap_read_probe": (C.c_int, [vp, vp, u64, u32, C.POINTER(C.c_double)]),
     "golap_device_allocated": (u64, [vp]),
     "golap_encode_opts_default": (None, [C.POINTER(EncodeOpts)]),
     "golap_encode": (C.c_int, [vp, u64, u32, C.POINTER(EncodeOpts), C.POINTER(vp)]),
@@ -352,6 +353,12 @@ class Context:
 
     def copy(self, dst, src, nbytes):
         self.check(self.lib.golap_d2d(self.h, dst, src, nbytes), "d2d")
+
+    def read_probe(self, p, nbytes, blocks_per_cu=2):
+        """Device ms of one streaming read of nbytes at device pointer p."""
+        ms = C.c_double()
+        self.check(self.lib.golap_read_probe(self.h, p, nbytes, blocks_per_cu, C.byref(ms)), "read_probe")
+        return ms.value
 
     def sync(self):
         self.check(self.lib.golap_sync(self.h), "sync")

@@ -209,6 +209,23 @@ int golap_d2d(golap_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
 }
 uint64_t golap_device_allocated(golap_ctx *ctx) { return ctx ? ctx->allocated : 0; }
 
+int golap_read_probe(golap_ctx *ctx, const void *dev_buf, uint64_t bytes, uint32_t blocks_per_cu, double *ms) {
+    if (!ctx || !dev_buf || !ms || !bytes) return GOLAP_EINVAL;
+    Lock l(ctx->mu);
+    if (int r = set_device(ctx)) return r;
+    GOLAP_HIP_CHECK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
+    const hipError_t e = launch_read_probe(ctx, dev_buf, bytes, blocks_per_cu,
+                                           reinterpret_cast<uint32_t *>(ctx->d_scratch + 3072), ctx->stream);
+    if (e == hipErrorInvalidValue) return fail(ctx, GOLAP_EINVAL, "read_probe: bytes and buffer must be 16-byte aligned");
+    GOLAP_HIP_CHECK(ctx, e);
+    GOLAP_HIP_CHECK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
+    GOLAP_HIP_CHECK(ctx, hipEventSynchronize(ctx->ev1));
+    float t = 0;
+    GOLAP_HIP_CHECK(ctx, hipEventElapsedTime(&t, ctx->ev0, ctx->ev1));
+    *ms = t;
+    return GOLAP_OK;
+}
+
 // ------------------------------------------------------------------------------ write path
 void golap_encode_opts_default(golap_encode_opts *o) {
     if (!o) return;

@@ -236,6 +236,8 @@ hipError_t launch_batch_items(const void *qbase, uint32_t qstride, uint32_t n_qu
                               uint64_t total_chunks, uint32_t parts, uint2 *items, uint32_t *queue,
                               hipStream_t stream);
 hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t stream);
+hipError_t launch_read_probe(golap_ctx *ctx, const void *p, uint64_t bytes, uint32_t blocks_per_cu, uint32_t *sink,
+                             hipStream_t stream);
 
 // Work list of one pruned query (a batch of one): its descriptor (desc_size <= 1024 bytes,
 // beginning with its SliceDev) goes to the context scratch, the surviving chunks' items to a

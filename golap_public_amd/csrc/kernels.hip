@@ -200,6 +200,34 @@ hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t stream) {
     return hipGetLastError();
 }
 
+// Streaming read of n16 16-byte words, grid-stride with four loads in flight per thread: the
+// HBM read bandwidth a plain read kernel reaches, beside which the fused scans are reported.
+// The sink is written only for an impossible checksum, so the loads stay.
+__global__ __launch_bounds__(BLOCK) void k_read_probe(const uint4 *p, uint64_t n16, uint32_t *sink) {
+    uint32_t acc = 0;
+    const uint64_t step = (uint64_t)gridDim.x * BLOCK;
+    uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
+    for (; i + 3 * step < n16; i += 4 * step) {
+        const uint4 a = p[i], b = p[i + step], c = p[i + 2 * step], d = p[i + 3 * step];
+        acc ^= a.x ^ a.y ^ a.z ^ a.w ^ b.x ^ b.y ^ b.z ^ b.w ^ c.x ^ c.y ^ c.z ^ c.w ^ d.x ^ d.y ^ d.z ^ d.w;
+    }
+    for (; i < n16; i += step) {
+        const uint4 a = p[i];
+        acc ^= a.x ^ a.y ^ a.z ^ a.w;
+    }
+    if (acc == 0x9e3779b9u && threadIdx.x == 0xffffu) sink[0] = acc;
+}
+
+hipError_t launch_read_probe(golap_ctx *ctx, const void *p, uint64_t bytes, uint32_t blocks_per_cu, uint32_t *sink,
+                             hipStream_t stream) {
+    if (bytes % 16 || reinterpret_cast<uintptr_t>(p) % 16 || !blocks_per_cu || blocks_per_cu > 32)
+        return hipErrorInvalidValue;
+    const uint32_t grid = (uint32_t)ctx->cus * blocks_per_cu;
+    hipLaunchKernelGGL(k_read_probe, dim3(grid), dim3(BLOCK), 0, stream, static_cast<const uint4 *>(p), bytes / 16,
+                       sink);
+    return hipGetLastError();
+}
+
 hipError_t launch_hash_clear(golap_ctx *ctx, uint64_t *slots, uint64_t n) {
     const uint32_t grid = (uint32_t)std::min<uint64_t>((n + BLOCK - 1) / BLOCK, 4096);
     hipLaunchKernelGGL(k_fill_zero, dim3(std::max(grid, 1u)), dim3(BLOCK), 0, ctx->stream, slots, n);

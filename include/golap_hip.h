@@ -75,6 +75,10 @@ int golap_d2h(golap_ctx *ctx, void *dst, const void *dev_src, uint64_t bytes);
 int golap_memset(golap_ctx *ctx, void *dev_dst, int value, uint64_t bytes);
 /* DataCopy (core/access.hpp:16-67): device-to-device copy on the context's stream. */
 int golap_d2d(golap_ctx *ctx, void *dev_dst, const void *dev_src, uint64_t bytes);
+/* No counterpart in the reference: a streaming read of `bytes` (16-byte aligned) of device
+   memory by blocks_per_cu workgroups per CU; *ms = its device time. The HBM read bandwidth a
+   plain read kernel reaches, reported beside the fused scans' roofline fraction. */
+int golap_read_probe(golap_ctx *ctx, const void *dev_buf, uint64_t bytes, uint32_t blocks_per_cu, double *ms);
 uint64_t golap_device_allocated(golap_ctx *ctx);
 
 /* ------------------------------------------------------------------ write path
