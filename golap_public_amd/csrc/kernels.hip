@@ -202,8 +202,9 @@ hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t stream) {
 
 // Streaming read of n16 16-byte words, grid-stride with four loads in flight per thread: the
 // HBM read bandwidth a plain read kernel reaches, beside which the fused scans are reported.
-// The sink is written only for an impossible checksum, so the loads stay.
-__global__ __launch_bounds__(BLOCK) void k_read_probe(const uint4 *p, uint64_t n16, uint32_t *sink) {
+// The sink is written only when the checksum equals `never` (a value the host picks), so the
+// loads stay.
+__global__ __launch_bounds__(BLOCK) void k_read_probe(const uint4 *p, uint64_t n16, uint32_t *sink, uint32_t never) {
     uint32_t acc = 0;
     const uint64_t step = (uint64_t)gridDim.x * BLOCK;
     uint64_t i = (uint64_t)blockIdx.x * BLOCK + threadIdx.x;
@@ -215,7 +216,7 @@ __global__ __launch_bounds__(BLOCK) void k_read_probe(const uint4 *p, uint64_t n
         const uint4 a = p[i];
         acc ^= a.x ^ a.y ^ a.z ^ a.w;
     }
-    if (acc == 0x9e3779b9u && threadIdx.x == 0xffffu) sink[0] = acc;
+    if (acc == never) sink[0] = acc;
 }
 
 hipError_t launch_read_probe(golap_ctx *ctx, const void *p, uint64_t bytes, uint32_t blocks_per_cu, uint32_t *sink,
@@ -224,7 +225,7 @@ hipError_t launch_read_probe(golap_ctx *ctx, const void *p, uint64_t bytes, uint
         return hipErrorInvalidValue;
     const uint32_t grid = (uint32_t)ctx->cus * blocks_per_cu;
     hipLaunchKernelGGL(k_read_probe, dim3(grid), dim3(BLOCK), 0, stream, static_cast<const uint4 *>(p), bytes / 16,
-                       sink);
+                       sink, 0x9e3779b9u);
     return hipGetLastError();
 }
 
