@@ -186,3 +186,36 @@ def test_scan_batch_mixed_column_sets(orc, golap, ctx, trip):
             hosts = [host[dev.index(c)] for c in p["cols"]]
             wk, ws, wc = orc.scan_groupby(hosts, p["preds"], p["keys"], p["value_col"], threads=4)
             assert np.array_equal(got[1], wk) and np.array_equal(got[2], ws) and np.array_equal(got[3], wc)
+
+
+def test_scan_long_launch_four_row_form(orc, golap, ctx):
+    """A table with more pages than 16 per CU: long launches take the 4-rows-per-lane kernel
+    (single plans, and a batch whose dense entries leave room for four workgroups per CU)."""
+    cus, _ = ctx.info()
+    n = (cus * 16 + 256) * 4096  # pages beyond one per wave of four workgroups per CU
+    codecs = [golap.CODEC_DBP, golap.CODEC_LZ4, golap.CODEC_FOR, golap.CODEC_FOR, golap.CODEC_FOR]
+    host = golap.gen_trip(77, n)
+    dev = [ctx.upload(golap.HostColumn.encode(c, codec=k, page_rows=4096, chunk_rows=65536, lz4_min_match=16))
+           for c, k in zip(host, codecs)]
+    try:
+        plans = []
+        for t in range(4):
+            preds, keys, vc = template(t, 0, 8783, full_hours=True)
+            say("long scan template", t, "rows", n)
+            r, k, s, c = ctx.scan_run(dev, preds, keys, vc)
+            wk, ws, wc = orc.scan_groupby(list(host), preds, keys, vc, threads=8)
+            assert np.array_equal(k, wk) and np.array_equal(s, ws) and np.array_equal(c, wc), t
+            plans.append(dict(cols=dev, preds=preds, keys=keys, value_col=vc))
+        # mixed plans in one launch with 1024 dense entries (four workgroups per CU)
+        for t in range(4):
+            preds, keys, vc = template(t, 1000 * t, 1000 * t + 3000)
+            plans.append(dict(cols=dev, preds=preds, keys=keys, value_col=vc))
+        with ctx.scan_batch(plans) as b:
+            b.set_config(lds_group_slots=1024)
+            b.run()
+            for i, p in enumerate(plans):
+                got, ref = b.result(i), ctx.scan_run(**p)
+                assert all(np.array_equal(x, y) for x, y in zip(got[1:], ref[1:])), i
+    finally:
+        for d in dev:
+            ctx.free_column(d)
