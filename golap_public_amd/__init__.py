@@ -140,6 +140,8 @@ _PROTOS = {
     "golap_memset": (C.c_int, [vp, vp, C.c_int, u64]),
     "golap_d2d": (C.c_int, [vp, vp, vp, u64]),
     "golap_read_probe": (C.c_int, [vp, vp, u64, u32, C.POINTER(C.c_double)]),
+    "golap_hashmap_keys": (C.c_int, [vp, vp, vp, u64, C.POINTER(u64)]),
+    "golap_minmax_check_points": (C.c_int, [vp, vp, vp, u64, vp]),
     "golap_device_allocated": (u64, [vp]),
     "golap_encode_opts_default": (None, [C.POINTER(EncodeOpts)]),
     "golap_encode": (C.c_int, [vp, u64, u32, C.POINTER(EncodeOpts), C.POINTER(vp)]),
@@ -343,6 +345,10 @@ class Context:
         self.check(self.lib.golap_h2d(self.h, p, _ptr(arr), arr.nbytes), "h2d")
         return p
 
+    def h2d(self, p, arr):
+        arr = np.ascontiguousarray(arr)
+        self.check(self.lib.golap_h2d(self.h, p, _ptr(arr), arr.nbytes), "h2d")
+
     def to_host(self, p, n, dtype):
         out = np.empty(n, dtype=dtype)
         self.check(self.lib.golap_d2h(self.h, _ptr(out), p, out.nbytes), "d2h")
@@ -436,6 +442,16 @@ class Context:
 
     def combine_and(self, dev_a, dev_b, n):
         self.check(self.lib.golap_combine_and(self.h, dev_a, dev_b, n), "combine_and")
+
+    def hashmap_keys(self, m, dev_keys, cap):
+        """collect_hashmap_keys: the map's keys, ascending, into dev_keys; returns their count."""
+        n = u64()
+        self.check(self.lib.golap_hashmap_keys(self.h, m, dev_keys, cap, C.byref(n)), "hashmap_keys")
+        return n.value
+
+    def minmax_check_points(self, col, dev_keys, n, dev_flags):
+        """check_mmmeta_multiple_pts: chunk flags = some key in the chunk's [min, max]."""
+        self.check(self.lib.golap_minmax_check_points(self.h, col, dev_keys, n, dev_flags), "minmax_check_points")
 
     # group-by
     def group_by(self, keys, dev_val, val_type, n_rows, agg_mode=AGG_AUTO, max_groups=1 << 20):

@@ -632,6 +632,23 @@ int golap_minmax_check(golap_ctx *ctx, const golap_column *c, int64_t lo, int64_
     return GOLAP_OK;
 }
 
+int golap_hashmap_keys(golap_ctx *ctx, const golap_hashmap *m, int32_t *dev_keys, uint64_t cap, uint64_t *n) {
+    if (!ctx || !m || !n || m->ctx != ctx || (cap && !dev_keys)) return GOLAP_EINVAL;
+    Lock l(ctx->mu);
+    if (int r = set_device(ctx)) return r;
+    if (!m->built) return fail(ctx, GOLAP_EINVAL, "hashmap_keys: the map is not built");
+    return collect_sorted_keys(ctx, m, dev_keys, cap, n);
+}
+
+int golap_minmax_check_points(golap_ctx *ctx, const golap_column *c, const int32_t *dev_keys, uint64_t n,
+                              uint16_t *flags) {
+    if (!ctx || !c || !flags || (n && !dev_keys)) return GOLAP_EINVAL;
+    Lock l(ctx->mu);
+    if (int r = set_device(ctx)) return r;
+    GOLAP_HIP_CHECK(ctx, launch_check_points(ctx, c, dev_keys, n, flags));
+    return GOLAP_OK;
+}
+
 int golap_combine_and(golap_ctx *ctx, uint16_t *a, const uint16_t *b, uint64_t n) {
     if (!ctx || (n && (!a || !b))) return GOLAP_EINVAL;
     Lock l(ctx->mu);

@@ -236,6 +236,12 @@ hipError_t launch_batch_items(const void *qbase, uint32_t qstride, uint32_t n_qu
                               uint64_t total_chunks, uint32_t parts, uint2 *items, uint32_t *queue,
                               hipStream_t stream);
 hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t stream);
+// The keys map m holds, ascending, into keys (capacity cap); *n = their count (also when it
+// exceeds cap: then GOLAP_ECAPACITY). Synchronises the context stream.
+int collect_sorted_keys(golap_ctx *ctx, const golap_hashmap *m, int32_t *keys, uint64_t cap, uint64_t *n);
+// flags[c] = some key of the ascending array keys[0, n_keys) lies in [min[c], max[c]].
+hipError_t launch_check_points(golap_ctx *ctx, const golap_column *col, const int32_t *keys, uint64_t n_keys,
+                               uint16_t *flags);
 hipError_t launch_read_probe(golap_ctx *ctx, const void *p, uint64_t bytes, uint32_t blocks_per_cu, uint32_t *sink,
                              hipStream_t stream);
 

@@ -375,6 +375,22 @@ public:
                           dev_.get(), "combine_and");
         first_ = false;
     }
+    // Join-key pruning (usecases/ssb/queries/01.hpp:246-270): collect_hashmap_keys of the
+    // built map, check_mmmeta_multiple_pts of the foreign-key column against them, combine_and.
+    void add_join_keys(const HashMap &map, const CompInfo &fk) {
+        const uint64_t cap = std::max<uint64_t>(map.filled(), 1);
+        DeviceMem keys(dev_, Tag<int32_t>{}, cap);
+        uint64_t n = 0;
+        detail::check(golap_hashmap_keys(dev_.get(), map.get(), keys.ptr<int32_t>(), cap, &n), dev_.get(),
+                      "hashmap_keys");
+        uint16_t *dst = first_ ? combined.ptr<uint16_t>() : tmp.ptr<uint16_t>();
+        detail::check(golap_minmax_check_points(dev_.get(), fk.col.get(), keys.ptr<int32_t>(), n, dst), dev_.get(),
+                      "minmax_check_points");
+        if (!first_)
+            detail::check(golap_combine_and(dev_.get(), combined.ptr<uint16_t>(), tmp.ptr<uint16_t>(), chunk_num),
+                          dev_.get(), "combine_and");
+        first_ = false;
+    }
     const uint16_t *mask() const { return first_ ? nullptr : combined.ptr<uint16_t>(); }
     std::vector<uint16_t> to_host() const {
         std::vector<uint16_t> h(chunk_num, 1);

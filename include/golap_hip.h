@@ -184,6 +184,16 @@ int golap_hashmap_free(golap_ctx *ctx, golap_hashmap *map);
 int golap_minmax_check(golap_ctx *ctx, const golap_column *col, int64_t lo, int64_t hi,
                        uint16_t *dev_flags);
 int golap_combine_and(golap_ctx *ctx, uint16_t *dev_a, const uint16_t *dev_b, uint64_t n);
+/* Join-key pruning (usecases/ssb/queries/01.hpp:246-270).
+ * collect_hashmap_keys (core/dev_structs.cuh:136-150): the keys map m holds, written to
+ * dev_keys in ascending order (the reference appends them in arbitrary order); *n = their
+ * count. GOLAP_ECAPACITY (with *n set) when more than cap keys are held.
+ * check_mmmeta_multiple_pts (core/metadata.cuh:550-559): dev_flags[c] = 1 if some key of the
+ * ascending dev_keys[0, n) lies in [min[c], max[c]] of column col's zone map, else 0 (the
+ * reference only sets flags, on an array its caller zeroed). */
+int golap_hashmap_keys(golap_ctx *ctx, const golap_hashmap *m, int32_t *dev_keys, uint64_t cap, uint64_t *n);
+int golap_minmax_check_points(golap_ctx *ctx, const golap_column *col, const int32_t *dev_keys, uint64_t n,
+                              uint16_t *dev_flags);
 
 /* ------------------------------------------------------------------ Q1 plan
  * query1 (usecases/ssb/queries/01.hpp:53-449) with pipeline_q1_lineorder

@@ -87,6 +87,22 @@ int main() {
     REQUIRE(pruned.sum == want && pruned.matches == want_matches);
     REQUIRE(pruned.chunks_pruned == chunk_num - keep);
 
+    // join-key pruning (01.hpp:246-270): the date map's keys against the orderdate zone maps
+    golap::MetaChecker jchecker(dev, chunk_num);
+    jchecker.add_join_keys(date_map, ci_date);
+    auto jmask = jchecker.to_host();
+    uint64_t jkeep = 0;
+    for (uint64_t c = 0; c < chunk_num; ++c) {
+        bool want = false;  // some 1995 date key inside the chunk's [min, max]
+        for (int k = 0; k < 2556 && !want; ++k)
+            want = d_year[k] == 1995 && mm_date.check_pred_host(c, d_key[k], d_key[k]);
+        REQUIRE((jmask[c] != 0) == want);
+        jkeep += want;
+    }
+    auto jpruned = golap::query1(dev, ci_date, ci_disc, ci_qty, ci_price, date_map, 4, 6, 30, 50, &jchecker);
+    REQUIRE(jpruned.sum == want && jpruned.matches == want_matches);
+    REQUIRE(jpruned.chunks_pruned == chunk_num - jkeep);
+
     // select path: step every chunk of every column through a TableLoader
     std::vector<uint64_t> all(chunk_num);
     std::iota(all.begin(), all.end(), 0);
