@@ -107,6 +107,11 @@ class ServerStats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class ChunkMetaDesc(C.Structure):
+    _fields_ = [("n_chunks", u64), ("buckets", u32), ("bloom_bytes", u64), ("bloom_k", u64), ("bloom_n", u64),
+                ("mins", vp), ("maxs", vp), ("hist", vp), ("bloom_state", vp)]
+
+
 class BatchConfig(C.Structure):
     _fields_ = [("pages_per_item", u32), ("lds_group_slots", u32), ("lds_cutover", u32), ("filter_lds", u32),
                 ("compact_tables", u32), ("max_blocks_per_cu", u32), ("shared_filter", u32)]
@@ -141,6 +146,12 @@ _PROTOS = {
     "golap_d2d": (C.c_int, [vp, vp, vp, u64]),
     "golap_read_probe": (C.c_int, [vp, vp, u64, u32, C.POINTER(C.c_double)]),
     "golap_hashmap_keys": (C.c_int, [vp, vp, vp, u64, C.POINTER(u64)]),
+    "golap_eqhist_build": (C.c_int, [vp, vp, u32, u64, u64, u32, C.POINTER(vp)]),
+    "golap_eqhist_check": (C.c_int, [vp, vp, i64, i64, vp]),
+    "golap_bloommeta_build": (C.c_int, [vp, vp, u32, u64, u64, C.c_double, u64, C.POINTER(vp)]),
+    "golap_bloommeta_check": (C.c_int, [vp, vp, i64, vp]),
+    "golap_chunkmeta_info": (C.c_int, [vp, C.c_void_p]),
+    "golap_chunkmeta_free": (C.c_int, [vp, vp]),
     "golap_minmax_check_points": (C.c_int, [vp, vp, vp, u64, vp]),
     "golap_device_allocated": (u64, [vp]),
     "golap_encode_opts_default": (None, [C.POINTER(EncodeOpts)]),
@@ -442,6 +453,34 @@ class Context:
 
     def combine_and(self, dev_a, dev_b, n):
         self.check(self.lib.golap_combine_and(self.h, dev_a, dev_b, n), "combine_and")
+
+    def eqhist_build(self, dev_values, type_, n_rows, chunk_rows, buckets):
+        """EqHistogram per chunk of a device column; returns a handle."""
+        h = vp()
+        self.check(self.lib.golap_eqhist_build(self.h, dev_values, type_, n_rows, chunk_rows, buckets, C.byref(h)),
+                   "eqhist_build")
+        return h.value
+
+    def eqhist_check(self, h, lo, hi, dev_flags):
+        self.check(self.lib.golap_eqhist_check(self.h, h, lo, hi, dev_flags), "eqhist_check")
+
+    def bloommeta_build(self, dev_values, type_, n_rows, chunk_rows, p, m_bytes):
+        """BloomMeta per chunk of a device column; returns a handle."""
+        h = vp()
+        self.check(self.lib.golap_bloommeta_build(self.h, dev_values, type_, n_rows, chunk_rows, p, m_bytes,
+                                                  C.byref(h)), "bloommeta_build")
+        return h.value
+
+    def bloommeta_check(self, h, value, dev_flags):
+        self.check(self.lib.golap_bloommeta_check(self.h, h, value, dev_flags), "bloommeta_check")
+
+    def chunkmeta_desc(self, h):
+        d = ChunkMetaDesc()
+        self.check(self.lib.golap_chunkmeta_info(h, C.byref(d)), "chunkmeta_info")
+        return d
+
+    def chunkmeta_free(self, h):
+        self.check(self.lib.golap_chunkmeta_free(self.h, h), "chunkmeta_free")
 
     def hashmap_keys(self, m, dev_keys, cap):
         """collect_hashmap_keys: the map's keys, ascending, into dev_keys; returns their count."""

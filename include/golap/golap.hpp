@@ -149,11 +149,38 @@ inline uint32_t codec_of(const std::string &algo) {
     throw Error(GOLAP_EINVAL, "unknown comp_algo " + algo);
 }
 
+// EqHistogram (core/metadata.cuh:225-382) and BloomMeta (:386-537): per-chunk metadata built
+// on the device by prepare_compressed_device and checked by MetaChecker::add_hist / add_bloom.
+template <typename T>
+class EqHistogram {
+public:
+    explicit EqHistogram(uint64_t bucket_num) : bucket_num(bucket_num) {}
+    uint64_t bucket_num;
+    std::shared_ptr<golap_chunkmeta> meta;
+};
+template <typename T>
+class BloomMeta {
+public:
+    BloomMeta(double p, uint64_t m) : p(p), m(m) {}
+    double p;
+    uint64_t m;  // filter bytes per chunk
+    std::shared_ptr<golap_chunkmeta> meta;
+};
+
 // prepare_compressed_device (core/apps.cuh:257-275): encode `col` into row-aligned chunks of
-// ci.chunk_bytes, upload to HBM, fill ci.blocks and the zone map. Returns the encode time (ms).
+// ci.chunk_bytes, upload to HBM, fill ci.blocks and the zone map (and, when given, the
+// histogram and Bloom metadata of each chunk). Returns the encode time (ms).
+namespace detail {
+template <typename X>
+struct same {  // keeps a parameter out of template argument deduction (so nullptr binds)
+    using type = X;
+};
+}  // namespace detail
 template <typename T>
 double prepare_compressed_device(Device &dev, const T *col, uint64_t num_tuples, CompInfo &ci,
-                                 MinMaxMeta<T> *minmax = nullptr) {
+                                 typename detail::same<MinMaxMeta<T>>::type *minmax = nullptr,
+                                 typename detail::same<EqHistogram<T>>::type *histogram = nullptr,
+                                 typename detail::same<BloomMeta<T>>::type *bloommeta = nullptr) {
     golap_encode_opts o;
     golap_encode_opts_default(&o);
     o.codec = codec_of(ci.comp_algo);
@@ -189,6 +216,29 @@ double prepare_compressed_device(Device &dev, const T *col, uint64_t num_tuples,
         minmax->maxs.resize(d.n_chunks);
         golap_d2h(ctx, minmax->mins.data(), d.chunk_min, d.n_chunks * 8);
         golap_d2h(ctx, minmax->maxs.data(), d.chunk_max, d.n_chunks * 8);
+    }
+    if (histogram || bloommeta) {
+        // the values on the device, chunk by chunk as the column's chunks
+        void *v = nullptr;
+        detail::check(golap_dev_alloc(ctx, std::max<uint64_t>(num_tuples, 1) * sizeof(T), &v), ctx, "dev_alloc");
+        std::shared_ptr<void> guard(v, [ctx](void *q) { golap_dev_free(ctx, q); });
+        detail::check(golap_h2d(ctx, v, col, num_tuples * sizeof(T)), ctx, "h2d");
+        auto keep = [ctx](golap_chunkmeta *m) { return std::shared_ptr<golap_chunkmeta>(m, [ctx](golap_chunkmeta *q) {
+                                                      golap_chunkmeta_free(ctx, q);
+                                                  }); };
+        golap_chunkmeta *m = nullptr;
+        if (histogram) {
+            detail::check(golap_eqhist_build(ctx, v, detail::type_of<T>(), num_tuples, d.chunk_rows,
+                                             (uint32_t)histogram->bucket_num, &m),
+                          ctx, "eqhist_build");
+            histogram->meta = keep(m);
+        }
+        if (bloommeta) {
+            detail::check(golap_bloommeta_build(ctx, v, detail::type_of<T>(), num_tuples, d.chunk_rows, bloommeta->p,
+                                                bloommeta->m, &m),
+                          ctx, "bloommeta_build");
+            bloommeta->meta = keep(m);
+        }
     }
     return ms;
 }
@@ -370,10 +420,23 @@ public:
         uint16_t *dst = first_ ? combined.ptr<uint16_t>() : tmp.ptr<uint16_t>();
         detail::check(golap_minmax_check(dev_.get(), ci.col.get(), (int64_t)lo, (int64_t)hi, dst), dev_.get(),
                       "minmax_check");
-        if (!first_)
-            detail::check(golap_combine_and(dev_.get(), combined.ptr<uint16_t>(), tmp.ptr<uint16_t>(), chunk_num),
-                          dev_.get(), "combine_and");
-        first_ = false;
+        combine(dst);
+    }
+    // check_hist + combine_and (metadata.cuh:573-585, MetaChecker::add_hist :674-687)
+    template <typename T>
+    void add_hist(const EqHistogram<T> &h, T lo, T hi) {
+        uint16_t *dst = first_ ? combined.ptr<uint16_t>() : tmp.ptr<uint16_t>();
+        detail::check(golap_eqhist_check(dev_.get(), h.meta.get(), (int64_t)lo, (int64_t)hi, dst), dev_.get(),
+                      "eqhist_check");
+        combine(dst);
+    }
+    // check_bloom + combine_and (metadata.cuh:593-600, MetaChecker::add_bloom :689-702)
+    template <typename T>
+    void add_bloom(const BloomMeta<T> &b, T val) {
+        uint16_t *dst = first_ ? combined.ptr<uint16_t>() : tmp.ptr<uint16_t>();
+        detail::check(golap_bloommeta_check(dev_.get(), b.meta.get(), (int64_t)val, dst), dev_.get(),
+                      "bloommeta_check");
+        combine(dst);
     }
     // Join-key pruning (usecases/ssb/queries/01.hpp:246-270): collect_hashmap_keys of the
     // built map, check_mmmeta_multiple_pts of the foreign-key column against them, combine_and.
@@ -386,12 +449,16 @@ public:
         uint16_t *dst = first_ ? combined.ptr<uint16_t>() : tmp.ptr<uint16_t>();
         detail::check(golap_minmax_check_points(dev_.get(), fk.col.get(), keys.ptr<int32_t>(), n, dst), dev_.get(),
                       "minmax_check_points");
-        if (!first_)
-            detail::check(golap_combine_and(dev_.get(), combined.ptr<uint16_t>(), tmp.ptr<uint16_t>(), chunk_num),
-                          dev_.get(), "combine_and");
-        first_ = false;
+        combine(dst);
     }
     const uint16_t *mask() const { return first_ ? nullptr : combined.ptr<uint16_t>(); }
+    // ANDs the flags just written into tmp into the combined mask (the first check wrote it).
+    void combine(uint16_t *dst) {
+        if (!first_)
+            detail::check(golap_combine_and(dev_.get(), combined.ptr<uint16_t>(), dst, chunk_num), dev_.get(),
+                          "combine_and");
+        first_ = false;
+    }
     std::vector<uint16_t> to_host() const {
         std::vector<uint16_t> h(chunk_num, 1);
         if (!first_) combined.copy_to(h.data(), chunk_num);

@@ -184,6 +184,36 @@ int golap_hashmap_free(golap_ctx *ctx, golap_hashmap *map);
 int golap_minmax_check(golap_ctx *ctx, const golap_column *col, int64_t lo, int64_t hi,
                        uint16_t *dev_flags);
 int golap_combine_and(golap_ctx *ctx, uint16_t *dev_a, const uint16_t *dev_b, uint64_t n);
+/* Chunk metadata beyond min/max, built on the device from a column's values (dev_values: n_rows
+ * values of an integer golap_type) in chunks of chunk_rows rows, as prepare_compressed fills it
+ * per chunk (core/apps.cuh:212-224).
+ * EqHistogram (core/metadata.cuh:225-382): per chunk its min, max and bucket_num (1..8192)
+ * equal-width bucket counts, bucket = (v - min) * bucket_num / (max - min) in double, the
+ * maximum clamped to the last bucket. check_hist (:573-585): dev_flags[c] = 1 if a bucket from
+ * (lo - min) * inv to (hi - min) * inv (int64 truncation as in the reference; from the last
+ * bucket when lo == max) is non-empty; a constant chunk when lo <= v <= hi.
+ * BloomMeta (:386-537): a (p, m) Bloom filter of m_bytes (a multiple of 4, <= 65536) per chunk
+ * with n = ceil(8m ln(0.5^ln 2) / ln p) and k = ceil(8m ln 2 / n) hashes of value + i (the
+ * 64-bit MurmurHash3 finalizer modulo 8m). check_bloom (:593-600): dev_flags[c] = 1 if all k
+ * bits of value (taken as the column type) are set. Flags are written, not only set. */
+typedef struct golap_chunkmeta golap_chunkmeta;
+typedef struct {
+    uint64_t n_chunks;
+    uint32_t buckets;           /* histogram: buckets per chunk */
+    uint64_t bloom_bytes, bloom_k, bloom_n;
+    const int64_t *mins, *maxs; /* device: histogram chunk bounds */
+    const uint64_t *hist;       /* device: n_chunks x buckets counts */
+    const uint32_t *bloom_state;/* device: n_chunks x bloom_bytes / 4 words */
+} golap_chunkmeta_desc;
+int golap_eqhist_build(golap_ctx *ctx, const void *dev_values, uint32_t type, uint64_t n_rows, uint64_t chunk_rows,
+                       uint32_t bucket_num, golap_chunkmeta **out);
+int golap_eqhist_check(golap_ctx *ctx, const golap_chunkmeta *h, int64_t lo, int64_t hi, uint16_t *dev_flags);
+int golap_bloommeta_build(golap_ctx *ctx, const void *dev_values, uint32_t type, uint64_t n_rows,
+                          uint64_t chunk_rows, double p, uint64_t m_bytes, golap_chunkmeta **out);
+int golap_bloommeta_check(golap_ctx *ctx, const golap_chunkmeta *b, int64_t value, uint16_t *dev_flags);
+int golap_chunkmeta_info(const golap_chunkmeta *m, golap_chunkmeta_desc *out);
+int golap_chunkmeta_free(golap_ctx *ctx, golap_chunkmeta *m);
+
 /* Join-key pruning (usecases/ssb/queries/01.hpp:246-270).
  * collect_hashmap_keys (core/dev_structs.cuh:136-150): the keys map m holds, written to
  * dev_keys in ascending order (the reference appends them in arbitrary order); *n = their

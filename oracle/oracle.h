@@ -208,6 +208,18 @@ void oracle_minmax_check_i64(const int64_t *mins, const int64_t *maxs, uint64_t 
 void oracle_minmax_build_i32(const int32_t *col, uint64_t n, uint64_t chunk_rows, int64_t *mins,
                              int64_t *maxs);
 
+/* ------------------------------------------------------------------ chunk metadata (meta.c)
+ * EqHistogram (core/metadata.cuh:225-382) and BloomMeta (:386-537) with fill_hist / check_hist /
+ * fill_bloom / check_bloom (:561-603), over int64 column values in chunks of chunk_rows. */
+void oracle_eqhist_build(const int64_t *v, uint64_t n, uint64_t chunk_rows, uint64_t bucket_num, int64_t *mins,
+                         int64_t *maxs, uint64_t *hist);
+void oracle_eqhist_check(const int64_t *mins, const int64_t *maxs, const uint64_t *hist, uint64_t n_chunks,
+                         uint64_t bucket_num, int64_t lo, int64_t hi, uint16_t *out);
+void oracle_bloom_params(double p, uint64_t m, uint64_t *n, uint64_t *k);
+void oracle_bloom_build(const int64_t *v, uint64_t n, uint64_t chunk_rows, uint64_t m, uint64_t k, uint32_t *state);
+void oracle_bloom_check(const uint32_t *state, uint64_t n_chunks, uint64_t m, uint64_t k, int64_t value,
+                        uint16_t *out);
+
 #ifdef __cplusplus
 }
 #endif

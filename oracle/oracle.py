@@ -77,6 +77,11 @@ def load():
             "oracle_groupby": (i64, [vp, u32, vp, u64, u64, vp, vp, vp]),
             "oracle_minmax_check_i64": (None, [vp, vp, u64, i64, i64, vp]),
             "oracle_minmax_build_i32": (None, [vp, u64, u64, vp, vp]),
+            "oracle_eqhist_build": (None, [vp, u64, u64, u64, vp, vp, vp]),
+            "oracle_eqhist_check": (None, [vp, vp, vp, u64, u64, i64, i64, vp]),
+            "oracle_bloom_params": (None, [C.c_double, u64, C.POINTER(u64), C.POINTER(u64)]),
+            "oracle_bloom_build": (None, [vp, u64, u64, u64, u64, vp]),
+            "oracle_bloom_check": (None, [vp, u64, u64, u64, i64, vp]),
         }
         for name, (res, args) in protos.items():
             fn = getattr(lib, name)
@@ -324,3 +329,46 @@ def minmax_build(col, chunk_rows):
     maxs = np.zeros(nc, np.int64)
     load().oracle_minmax_build_i32(_p(col), col.size, chunk_rows, _p(mins), _p(maxs))
     return mins, maxs
+
+
+# ---- chunk metadata: EqHistogram / BloomMeta (core/metadata.cuh:225-537)
+def eqhist_build(values, chunk_rows, bucket_num):
+    """(mins, maxs, hist[chunks, buckets]) of the int column values (EqHistogram + fill_hist)."""
+    v = np.ascontiguousarray(values, np.int64)
+    nc = (v.size + chunk_rows - 1) // chunk_rows
+    mins, maxs = np.zeros(nc, np.int64), np.zeros(nc, np.int64)
+    hist = np.zeros(nc * bucket_num, np.uint64)
+    load().oracle_eqhist_build(_p(v), v.size, chunk_rows, bucket_num, _p(mins), _p(maxs), _p(hist))
+    return mins, maxs, hist.reshape(nc, bucket_num)
+
+
+def eqhist_check(mins, maxs, hist, lo, hi):
+    """check_hist flags of [lo, hi]."""
+    hist = np.ascontiguousarray(hist, np.uint64)
+    out = np.zeros(mins.size, np.uint16)
+    load().oracle_eqhist_check(_p(np.ascontiguousarray(mins, np.int64)), _p(np.ascontiguousarray(maxs, np.int64)),
+                               _p(hist), mins.size, hist.shape[1], lo, hi, _p(out))
+    return out
+
+
+def bloom_params(p, m):
+    """(n, k) of a (p, m) BloomMeta."""
+    n, k = u64(), u64()
+    load().oracle_bloom_params(p, m, C.byref(n), C.byref(k))
+    return n.value, k.value
+
+
+def bloom_build(values, chunk_rows, m, k):
+    """Per-chunk m-byte filters (as uint32 words) of the int column values (fill_bloom)."""
+    v = np.ascontiguousarray(values, np.int64)
+    nc = (v.size + chunk_rows - 1) // chunk_rows
+    state = np.zeros(nc * m // 4, np.uint32)
+    load().oracle_bloom_build(_p(v), v.size, chunk_rows, m, k, _p(state))
+    return state
+
+
+def bloom_check(state, n_chunks, m, k, value):
+    """check_bloom flags of one value."""
+    out = np.zeros(n_chunks, np.uint16)
+    load().oracle_bloom_check(_p(np.ascontiguousarray(state, np.uint32)), n_chunks, m, k, value, _p(out))
+    return out

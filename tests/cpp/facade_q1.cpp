@@ -50,8 +50,10 @@ int main() {
     golap::MinMaxMeta<int32_t> mm_date;
     golap::MinMaxMeta<uint8_t> mm_qty;
     golap::prepare_compressed_device(dev, date.data(), n, ci_date, &mm_date);
-    golap::prepare_compressed_device(dev, disc.data(), n, ci_disc);
-    golap::prepare_compressed_device(dev, qty.data(), n, ci_qty, &mm_qty);
+    golap::BloomMeta<uint8_t> bloom_disc(0.01, 256);
+    golap::EqHistogram<uint8_t> hist_qty(16);
+    golap::prepare_compressed_device(dev, disc.data(), n, ci_disc, nullptr, nullptr, &bloom_disc);
+    golap::prepare_compressed_device(dev, qty.data(), n, ci_qty, &mm_qty, &hist_qty);
     golap::prepare_compressed_device(dev, price.data(), n, ci_price);
     const uint64_t chunk_num = ci_date.blocks.size();
     REQUIRE(chunk_num == (n + tuples_per_chunk - 1) / tuples_per_chunk);  // apps_tests.cu:12-65
@@ -86,6 +88,28 @@ int main() {
     REQUIRE(full.sum == want && full.matches == want_matches);
     REQUIRE(pruned.sum == want && pruned.matches == want_matches);
     REQUIRE(pruned.chunks_pruned == chunk_num - keep);
+
+    // histogram and Bloom metadata (metadata.cuh:225-537, MetaChecker::add_hist / add_bloom)
+    golap::MetaChecker hchecker(dev, chunk_num);
+    hchecker.add_hist<uint8_t>(hist_qty, 30, 50);
+    hchecker.add_bloom<uint8_t>(bloom_disc, 5);
+    auto hmask = hchecker.to_host();
+    {
+        std::vector<int64_t> q64(qty.begin(), qty.end()), d64(disc.begin(), disc.end()), mins(chunk_num),
+            maxs(chunk_num);
+        std::vector<uint64_t> hist(chunk_num * 16);
+        std::vector<uint16_t> f1(chunk_num), f2(chunk_num);
+        oracle_eqhist_build(q64.data(), n, tuples_per_chunk, 16, mins.data(), maxs.data(), hist.data());
+        oracle_eqhist_check(mins.data(), maxs.data(), hist.data(), chunk_num, 16, 30, 50, f1.data());
+        uint64_t bn = 0, bk = 0;
+        oracle_bloom_params(0.01, 256, &bn, &bk);
+        std::vector<uint32_t> state(chunk_num * 256 / 4);
+        oracle_bloom_build(d64.data(), n, tuples_per_chunk, 256, bk, state.data());
+        oracle_bloom_check(state.data(), chunk_num, 256, bk, 5, f2.data());
+        for (uint64_t c = 0; c < chunk_num; ++c) REQUIRE((hmask[c] != 0) == (f1[c] && f2[c]));
+    }
+    auto hpruned = golap::query1(dev, ci_date, ci_disc, ci_qty, ci_price, date_map, 4, 6, 30, 50, &hchecker);
+    REQUIRE(hpruned.sum == want && hpruned.matches == want_matches);
 
     // join-key pruning (01.hpp:246-270): the date map's keys against the orderdate zone maps
     golap::MetaChecker jchecker(dev, chunk_num);
