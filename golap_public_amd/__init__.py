@@ -107,6 +107,14 @@ class ServerStats(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class LoadStats(C.Structure):
+    _fields_ = [("bytes", u64), ("segments", u64), ("direct_io", u32), ("read_ms", C.c_double),
+                ("total_ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class ChunkMetaDesc(C.Structure):
     _fields_ = [("n_chunks", u64), ("buckets", u32), ("bloom_bytes", u64), ("bloom_k", u64), ("bloom_n", u64),
                 ("mins", vp), ("maxs", vp), ("hist", vp), ("bloom_state", vp)]
@@ -146,6 +154,9 @@ _PROTOS = {
     "golap_d2d": (C.c_int, [vp, vp, vp, u64]),
     "golap_read_probe": (C.c_int, [vp, vp, u64, u32, C.POINTER(C.c_double)]),
     "golap_hashmap_keys": (C.c_int, [vp, vp, vp, u64, C.POINTER(u64)]),
+    "golap_host_column_save": (C.c_int, [vp, C.c_char_p]),
+    "golap_host_column_load": (C.c_int, [C.c_char_p, C.POINTER(vp)]),
+    "golap_column_load_file": (C.c_int, [vp, C.c_char_p, u64, u32, C.POINTER(vp), C.c_void_p]),
     "golap_eqhist_build": (C.c_int, [vp, vp, u32, u64, u64, u32, C.POINTER(vp)]),
     "golap_eqhist_check": (C.c_int, [vp, vp, i64, i64, vp]),
     "golap_bloommeta_build": (C.c_int, [vp, vp, u32, u64, u64, C.c_double, u64, C.POINTER(vp)]),
@@ -273,6 +284,20 @@ class HostColumn:
             raise GolapError(st, "assemble")
         return cls(out.value)
 
+    def save(self, path):
+        """Writes the column file (header, catalog, 4 KiB-aligned page data)."""
+        st = load().golap_host_column_save(self.h, str(path).encode())
+        if st != OK:
+            raise GolapError(st, "host_column_save")
+
+    @classmethod
+    def load_file(cls, path):
+        out = vp()
+        st = load().golap_host_column_load(str(path).encode(), C.byref(out))
+        if st != OK:
+            raise GolapError(st, "host_column_load")
+        return cls(out.value)
+
     def desc(self):
         d = ColumnDesc()
         st = load().golap_host_column_info(self.h, C.byref(d))
@@ -394,6 +419,14 @@ class Context:
             self.h, _ptr(values), values.size, TYPE_OF[values.dtype], C.byref(encode_opts(**kw)),
             C.byref(out), C.byref(ms)), "prepare_compressed_device")
         return out.value, ms.value
+
+    def load_file(self, path, segment_bytes=0, n_buffers=0):
+        """SSD -> HBM: the column file's pages through pinned buffers into a new device column.
+        Returns (column handle, LoadStats)."""
+        out, st = vp(), LoadStats()
+        self.check(self.lib.golap_column_load_file(self.h, str(path).encode(), segment_bytes, n_buffers, C.byref(out),
+                                                   C.byref(st)), "column_load_file")
+        return out.value, st
 
     def column_desc(self, col):
         d = ColumnDesc()

@@ -323,6 +323,13 @@ int golap_column_upload(golap_ctx *ctx, const golap_host_column *h, golap_column
     *out = nullptr;
     Lock l(ctx->mu);
     if (int r = set_device(ctx)) return r;
+    return golap_impl::create_device_column(ctx, h, h->comp.size(), h->comp.data(), out);
+}
+
+}  // extern "C"
+
+int golap_impl::create_device_column(golap_ctx *ctx, const golap_host_column *h, uint64_t comp_bytes,
+                                     const uint8_t *comp, golap_column **out) {
     std::unique_ptr<golap_column> c(new golap_column());
     c->ctx = ctx;
     c->type = h->type;
@@ -332,7 +339,7 @@ int golap_column_upload(golap_ctx *ctx, const golap_host_column *h, golap_column
     c->n_rows = h->n_rows;
     c->n_pages = h->n_pages;
     c->n_chunks = h->n_chunks;
-    c->comp_bytes = h->comp.size();
+    c->comp_bytes = comp_bytes;
     c->payload_bytes = h->payload_bytes;
     c->page_off = h->page_off;
     c->page_csize = h->page_csize;
@@ -342,9 +349,9 @@ int golap_column_upload(golap_ctx *ctx, const golap_host_column *h, golap_column
     void *p[6];
     // the page data is followed by COMP_TAIL zero bytes: kernels read packed fields as dword
     // pairs, whose second dword may lie past the last page
-    const uint64_t sz[6] = {h->comp.size() + COMP_TAIL, h->n_pages * 8, h->n_pages * 4, h->n_pages,
+    const uint64_t sz[6] = {comp_bytes + COMP_TAIL, h->n_pages * 8, h->n_pages * 4, h->n_pages,
                             h->n_chunks * 8, h->n_chunks * 8};
-    const void *src[6] = {h->comp.data(), h->page_off.data(), h->page_csize.data(),
+    const void *src[6] = {comp, h->page_off.data(), h->page_csize.data(),
                           h->page_codec.data(), h->chunk_min.data(), h->chunk_max.data()};
     for (int i = 0; i < 6; ++i) {
         if (int r = dev_alloc(ctx, sz[i], &p[i])) {
@@ -353,7 +360,8 @@ int golap_column_upload(golap_ctx *ctx, const golap_host_column *h, golap_column
         }
         const uint64_t n = i == 0 ? sz[i] - COMP_TAIL : sz[i];
         hipError_t e = i == 0 ? hipMemsetAsync(static_cast<uint8_t *>(p[i]) + n, 0, COMP_TAIL, ctx->stream) : hipSuccess;
-        if (e == hipSuccess && n) e = hipMemcpyAsync(p[i], src[i], n, hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess && n && src[i])  // no page data: the caller fills d_comp
+            e = hipMemcpyAsync(p[i], src[i], n, hipMemcpyHostToDevice, ctx->stream);
         if (e != hipSuccess) {
             for (int j = 0; j <= i; ++j) dev_release(ctx, p[j]);
             return fail(ctx, GOLAP_EHIP, std::string("upload: ") + hipGetErrorString(e));
@@ -392,6 +400,8 @@ int golap_column_upload(golap_ctx *ctx, const golap_host_column *h, golap_column
     *out = c.release();
     return GOLAP_OK;
 }
+
+extern "C" {
 
 int golap_column_free(golap_ctx *ctx, golap_column *c) {
     if (!ctx || !c) return GOLAP_EINVAL;

@@ -118,6 +118,11 @@ extern std::mutex g_extra_mu;
 extern std::unordered_map<const golap_column *, ColumnExtra> g_extra;
 bool page_direct_host(uint8_t codec);
 int dev_alloc(golap_ctx *ctx, uint64_t bytes, void **p);
+// A device column from a host column's catalog and zone maps, with comp_bytes of page data
+// copied from comp, or only allocated (and left for the caller to fill) when comp is null.
+// Call with the context lock held.
+int create_device_column(golap_ctx *ctx, const golap_host_column *h, uint64_t comp_bytes, const uint8_t *comp,
+                         golap_column **out);
 void dev_release(golap_ctx *ctx, void *p);
 int set_device(golap_ctx *ctx);
 int check_device_errors(golap_ctx *ctx, const char *what);

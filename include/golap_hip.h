@@ -128,6 +128,27 @@ int golap_host_column_info(const golap_host_column *col, golap_column_desc *out)
 /* Upload an encoded column to HBM (prepare_compressed_device's device-resident result,
  * core/apps.cuh:257-275; the configs keep compressed chunks resident instead of on SSD). */
 int golap_column_upload(golap_ctx *ctx, const golap_host_column *col, golap_column **out);
+
+/* ------------------------------------------------------------------ column files, SSD -> HBM
+ * DenseWriter (core/access.hpp:116-157) / .dat serialization (core/hl/serialization.hpp:60-126)
+ * analogue: a column file holds a header, the catalog and zone maps, then the page data as one
+ * 4 KiB-aligned region. golap_host_column_load reads it back to the host.
+ * golap_column_load_file: the SSD -> CPU -> GPU path (SSD2CPU2GPU, core/apps.cuh:315-320;
+ * VarLoader, core/access.hpp:189-435): O_DIRECT preads of segment_bytes (rounded up to 4 KiB;
+ * 0 = 64 MiB) into a ring of n_buffers (2..16; 0 = 4) pinned buffers on a reader thread,
+ * overlapped with hipMemcpyAsync of the filled buffers into the new device column. File
+ * systems without O_DIRECT are read with buffered preads (stats->direct_io = 0). */
+typedef struct {
+    uint64_t bytes;       /* page data moved to HBM */
+    uint64_t segments;
+    uint32_t direct_io;   /* 1: O_DIRECT reads */
+    double read_ms;       /* time inside preads (reader thread) */
+    double total_ms;      /* open to device column ready */
+} golap_load_stats;
+int golap_host_column_save(const golap_host_column *col, const char *path);
+int golap_host_column_load(const char *path, golap_host_column **out);
+int golap_column_load_file(golap_ctx *ctx, const char *path, uint64_t segment_bytes, uint32_t n_buffers,
+                           golap_column **out, golap_load_stats *stats);
 int golap_column_free(golap_ctx *ctx, golap_column *col);
 int golap_column_info(const golap_column *col, golap_column_desc *out);
 /* Encode + upload; *comp_ms receives the host encode time like prepare_compressed_device's
