@@ -138,6 +138,20 @@ class Q1Result(C.Structure):
                 ("comp_bytes", u64), ("kernel_ms", C.c_double)]
 
 
+class Q1FilePlan(C.Structure):
+    _fields_ = [("paths", C.c_char_p * 4), ("date_map", vp), ("disc_lo", i64), ("disc_hi", i64), ("qty_lo", i64),
+                ("qty_hi", i64), ("price_lo", i64), ("price_hi", i64), ("host_chunk_mask", vp),
+                ("host_chunk_mask_len", u64), ("window_chunks", u32), ("zone_prune", u32)]
+
+
+class StreamStats(C.Structure):
+    _fields_ = [("windows", u64), ("windows_skipped", u64), ("bytes_read", u64), ("bytes_copied", u64),
+                ("direct_io", u32), ("window_chunks", u32), ("read_ms", C.c_double), ("total_ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 _PROTOS = {
     "golap_api_version": (C.c_int, []),
     "golap_status_string": (C.c_char_p, [C.c_int]),
@@ -186,6 +200,7 @@ _PROTOS = {
     "golap_minmax_check": (C.c_int, [vp, vp, i64, i64, vp]),
     "golap_combine_and": (C.c_int, [vp, vp, vp, u64]),
     "golap_q1_run": (C.c_int, [vp, C.POINTER(Q1Plan), C.POINTER(Q1Result)]),
+    "golap_q1_run_files": (C.c_int, [vp, C.POINTER(Q1FilePlan), C.POINTER(Q1Result), C.POINTER(StreamStats)]),
     "golap_group_by": (C.c_int, [vp, C.POINTER(GroupKey), u32, vp, u32, u64, u32, C.POINTER(u64), vp, vp, vp,
                                  u64]),
     "golap_star_run": (C.c_int, [vp, C.POINTER(StarPlan), C.POINTER(StarResult), vp, vp, vp, u64]),
@@ -626,6 +641,24 @@ class Context:
         r = Q1Result()
         self.check(self.lib.golap_q1_run(self.h, C.byref(p), C.byref(r)), "q1_run")
         return r
+
+    def q1_run_files(self, paths, date_map, disc_rng, qty_rng, price_rng=(-2**31, 2**31 - 1),
+                     host_chunk_mask=None, window_chunks=0, zone_prune=True):
+        """Q1 streamed from four column files (date, disc, qty, price) in windows of chunks;
+        host_chunk_mask: optional uint16 per chunk (0 = pruned, never read).
+        Returns (Q1Result, StreamStats)."""
+        if len(paths) != 4:
+            raise ValueError("q1_run_files: four column files (date, disc, qty, price)")
+        mask = None
+        if host_chunk_mask is not None:
+            mask = np.ascontiguousarray(host_chunk_mask, dtype=np.uint16)
+        p = Q1FilePlan((C.c_char_p * 4)(*[str(x).encode() for x in paths]), date_map, disc_rng[0], disc_rng[1],
+                       qty_rng[0], qty_rng[1], price_rng[0], price_rng[1],
+                       None if mask is None else mask.ctypes.data, 0 if mask is None else mask.size,
+                       window_chunks, 1 if zone_prune else 0)
+        r, st = Q1Result(), StreamStats()
+        self.check(self.lib.golap_q1_run_files(self.h, C.byref(p), C.byref(r), C.byref(st)), "q1_run_files")
+        return r, st
 
 
 class Batch:

@@ -374,6 +374,13 @@ int golap_impl::create_device_column(golap_ctx *ctx, const golap_host_column *h,
     c->d_page_codec = static_cast<uint8_t *>(p[3]);
     c->d_chunk_min = static_cast<int64_t *>(p[4]);
     c->d_chunk_max = static_cast<int64_t *>(p[5]);
+    register_extra(c.get());
+    *out = c.release();
+    return GOLAP_OK;
+}
+
+// The planning tables of a column from its host catalog (page kinds, payload per chunk).
+void golap_impl::register_extra(const golap_column *c) {
     ColumnExtra ex;
     ex.nondirect_prefix.resize(c->n_pages + 1, 0);
     ex.window_prefix.resize(c->n_pages + 1, 0);
@@ -393,11 +400,34 @@ int golap_impl::create_device_column(golap_ctx *ctx, const golap_host_column *h,
         for (uint64_t i = ch * ppc; i < std::min<uint64_t>(c->n_pages, (ch + 1) * ppc); ++i) s += c->page_csize[i];
         ex.chunk_payload[ch + 1] = ex.chunk_payload[ch] + s;
     }
-    {
-        std::lock_guard<std::mutex> g(g_extra_mu);
-        g_extra[c.get()] = std::move(ex);
-    }
-    *out = c.release();
+    std::lock_guard<std::mutex> g(g_extra_mu);
+    g_extra[c] = std::move(ex);
+}
+
+int golap_impl::set_column_window(golap_ctx *ctx, golap_column *c, const golap_host_column &w, uint64_t comp_bytes,
+                                  hipStream_t stream) {
+    c->type = w.type;
+    c->elem = w.elem;
+    c->page_rows = w.page_rows;
+    c->chunk_rows = w.chunk_rows;
+    c->n_rows = w.n_rows;
+    c->n_pages = w.n_pages;
+    c->n_chunks = w.n_chunks;
+    c->comp_bytes = comp_bytes;
+    c->payload_bytes = w.payload_bytes;
+    c->page_off = w.page_off;
+    c->page_csize = w.page_csize;
+    c->page_codec = w.page_codec;
+    c->chunk_min = w.chunk_min;
+    c->chunk_max = w.chunk_max;
+    hipError_t e = hipMemcpyAsync(c->d_page_off, w.page_off.data(), w.n_pages * 8, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->d_page_csize, w.page_csize.data(), w.n_pages * 4, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->d_page_codec, w.page_codec.data(), w.n_pages, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->d_chunk_min, w.chunk_min.data(), w.n_chunks * 8, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->d_chunk_max, w.chunk_max.data(), w.n_chunks * 8, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) return fail(ctx, GOLAP_EHIP, std::string("column window: ") + hipGetErrorString(e));
+    register_extra(c);
     return GOLAP_OK;
 }
 

@@ -123,6 +123,13 @@ int dev_alloc(golap_ctx *ctx, uint64_t bytes, void **p);
 // Call with the context lock held.
 int create_device_column(golap_ctx *ctx, const golap_host_column *h, uint64_t comp_bytes, const uint8_t *comp,
                          golap_column **out);
+// The host planning tables of a column, from its host catalog.
+void register_extra(const golap_column *c);
+// Points device column c (created with catalog capacity for at least w's pages and chunks) at a
+// window w of a larger column: catalog and zone maps copied (synchronously, on stream), planning
+// tables rebuilt; the caller fills d_comp with comp_bytes of page data.
+int set_column_window(golap_ctx *ctx, golap_column *c, const golap_host_column &w, uint64_t comp_bytes,
+                      hipStream_t stream);
 void dev_release(golap_ctx *ctx, void *p);
 int set_device(golap_ctx *ctx);
 int check_device_errors(golap_ctx *ctx, const char *what);

@@ -8,8 +8,9 @@
 //
 // Here a column file holds one paged column: a header, its catalog and zone maps, then the
 // column's page data as one 4 KiB-aligned region (pages back to back, so each chunk's pages
-// are contiguous). The loader reads that region in large segments with O_DIRECT preads into a
-// ring of pinned buffers on a reader thread while the calling thread copies filled buffers to
+// are contiguous). The loader reads that region in large segments with O_DIRECT preads (each
+// segment in 8 MiB pieces by up to 8 threads) into a ring of pinned buffers on a reader thread
+// while the calling thread copies filled buffers to
 // HBM with hipMemcpyAsync: disk reads and PCIe copies overlap, and each buffer is reused once
 // its copy has completed (an event per buffer). Files on file systems without O_DIRECT are
 // read with buffered preads instead (reported in the statistics).
@@ -19,6 +20,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -28,23 +30,16 @@
 #include <thread>
 #include <vector>
 
+#include "column_file.hpp"
 #include "common.hpp"
 #include "host_column.hpp"
 
-namespace {
+namespace golap_file {
 
+namespace {
 constexpr char MAGIC[8] = {'G', 'O', 'L', 'A', 'P', 'C', 'O', 'L'};
 constexpr uint32_t VERSION = 1;
-constexpr uint64_t ALIGN = 4096;
-
-struct FileHeader {
-    char magic[8];
-    uint32_t version, type, elem, page_rows;
-    uint64_t chunk_rows, n_rows, n_pages, n_chunks, comp_bytes, payload_bytes;
-    uint64_t data_offset;  // 4 KiB-aligned start of the page data
-    uint64_t file_bytes;   // data_offset + comp_bytes rounded up to 4 KiB
-};
-static_assert(sizeof(FileHeader) == 88, "file header layout");
+}  // namespace
 
 uint64_t align_up(uint64_t v) { return (v + ALIGN - 1) / ALIGN * ALIGN; }
 
@@ -127,9 +122,39 @@ bool read_catalog(int fd, golap_host_column &h, FileHeader &fh, std::string &err
     return true;
 }
 
-}  // namespace
+void split_read(std::vector<FileRead> &out, int dfd, int bfd, std::atomic<bool> *direct, uint64_t at, uint64_t len,
+                uint8_t *dst) {
+    for (uint64_t x = 0; x < len; x += READ_PIECE)
+        out.push_back({dfd, bfd, direct, at + x, std::min<uint64_t>(READ_PIECE, len - x), dst + x});
+}
+
+bool read_parallel(const std::vector<FileRead> &reads) {
+    std::atomic<size_t> next{0};
+    std::atomic<bool> ok{true};
+    auto work = [&] {
+        for (size_t i = next++; i < reads.size() && ok; i = next++) {
+            const FileRead &r = reads[i];
+            bool done = false;
+            if (*r.direct) {
+                done = read_all(r.dfd, r.dst, r.len, r.at);
+                if (!done) *r.direct = false;
+            }
+            if (!done) done = read_all(r.bfd, r.dst, r.len, r.at);
+            if (!done) ok = false;
+        }
+    };
+    const unsigned n = (unsigned)std::min<size_t>(READ_THREADS, reads.size());
+    std::vector<std::thread> pool;
+    for (unsigned t = 1; t < n; ++t) pool.emplace_back(work);
+    work();
+    for (auto &t : pool) t.join();
+    return ok;
+}
+
+}  // namespace golap_file
 
 using namespace golap_impl;
+using namespace golap_file;
 using Lock = std::lock_guard<std::recursive_mutex>;
 
 extern "C" {
@@ -197,11 +222,15 @@ int golap_column_load_file(golap_ctx *ctx, const char *path, uint64_t segment_by
     segment_bytes = segment_bytes ? align_up(segment_bytes) : (64ull << 20);
     n_buffers = std::min<uint32_t>(std::max<uint32_t>(n_buffers ? n_buffers : 4, 2), 16);
     const auto t0 = std::chrono::steady_clock::now();
-    int fd = open(path, O_RDONLY | O_DIRECT);
-    bool direct = fd >= 0;
-    if (fd < 0) fd = open(path, O_RDONLY);  // file systems without O_DIRECT
-    if (fd < 0) return fail(ctx, GOLAP_EINVAL, std::string("load_file: cannot open ") + path);
-    std::unique_ptr<int, void (*)(int *)> fd_guard(&fd, [](int *f) { close(*f); });
+    struct Fds {
+        int dfd, bfd;  // O_DIRECT (-1 on file systems without it) and buffered
+        ~Fds() {
+            if (dfd >= 0) close(dfd);
+            if (bfd >= 0) close(bfd);
+        }
+    } fds{open(path, O_RDONLY | O_DIRECT), open(path, O_RDONLY)};
+    std::atomic<bool> direct{fds.dfd >= 0};
+    if (fds.bfd < 0) return fail(ctx, GOLAP_EINVAL, std::string("load_file: cannot open ") + path);
     Lock l(ctx->mu);
     if (int r = set_device(ctx)) return r;
     // header and catalog through a pinned, aligned buffer (O_DIRECT needs aligned transfers)
@@ -264,19 +293,11 @@ int golap_column_load_file(golap_ctx *ctx, const char *path, uint64_t segment_by
             const uint64_t off = fh.data_offset + t * segment_bytes;
             const uint64_t want = std::min<uint64_t>(segment_bytes, data_end - off);
             const auto r0 = std::chrono::steady_clock::now();
-            // O_DIRECT transfers whole 4 KiB blocks: the file is padded to a block boundary.
-            // A file system that accepts O_DIRECT at open but refuses the reads (stacked file
-            // systems) is read buffered from then on.
-            bool ok = read_all(fd, buf[t % n_buffers], direct ? align_up(want) : want, off);
-            if (!ok && direct) {
-                const int bfd = open(path, O_RDONLY);
-                if (bfd >= 0) {
-                    close(fd);
-                    fd = bfd;
-                    direct = false;
-                    ok = read_all(fd, buf[t % n_buffers], want, off);
-                }
-            }
+            // whole 4 KiB blocks (the file is padded to a block boundary), in pieces read by a
+            // few threads at once
+            std::vector<FileRead> reads;
+            split_read(reads, fds.dfd, fds.bfd, &direct, off, align_up(want), buf[t % n_buffers]);
+            const bool ok = read_parallel(reads);
             const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - r0).count();
             std::lock_guard<std::mutex> lk(m);
             read_ms += ms;

@@ -268,6 +268,37 @@ typedef struct {
 } golap_q1_result;
 int golap_q1_run(golap_ctx *ctx, const golap_q1_plan *plan, golap_q1_result *res);
 
+/* Q1 streamed from column files (query1's SSD -> GPU pipeline, usecases/ssb/queries/01.hpp:
+ * 350-424, with VarLoader reads, core/access.hpp:189-260, and skip_step for pruned blocks,
+ * 01.hpp:368-378): the four files (paths: date, disc, qty, price; row-aligned) are scanned in
+ * windows of window_chunks chunks (0: about 64 MiB of pages per window). Reader threads
+ * pread the next window into pinned buffers (O_DIRECT where the file system allows it) while
+ * the current window is copied to HBM and scanned; chunks that are pruned are never read: those
+ * whose host_chunk_mask flag (optional, one per chunk) is 0 and, with zone_prune set, those whose
+ * disc, qty or price zone map misses the predicate. The result equals golap_q1_run over the same
+ * columns resident in HBM; kernel_ms sums the windows' kernels. */
+typedef struct {
+    const char *paths[4];
+    const golap_hashmap *date_map;
+    int64_t disc_lo, disc_hi, qty_lo, qty_hi, price_lo, price_hi;
+    const uint16_t *host_chunk_mask;
+    uint64_t host_chunk_mask_len;  /* entries in host_chunk_mask, at least the files' chunk count */
+    uint32_t window_chunks;
+    uint32_t zone_prune;
+} golap_q1_file_plan;
+typedef struct {
+    uint64_t windows;          /* windows scanned */
+    uint64_t windows_skipped;  /* windows with every chunk pruned (nothing read) */
+    uint64_t bytes_read;       /* bytes read from the files, 4 KiB-aligned reads included */
+    uint64_t bytes_copied;     /* page bytes copied to HBM */
+    uint32_t direct_io;        /* 1: every read was O_DIRECT */
+    uint32_t window_chunks;    /* window size used */
+    double read_ms;            /* time inside preads (reader thread) */
+    double total_ms;           /* open to result */
+} golap_stream_stats;
+int golap_q1_run_files(golap_ctx *ctx, const golap_q1_file_plan *plan, golap_q1_result *res,
+                       golap_stream_stats *stats);
+
 /* ------------------------------------------------------------------ group-by
  * Results of the grouping entry points are dense over a declared group domain: each grouping
  * attribute a has a range [lo, lo + card); the group index is the mixed-radix number of the

@@ -25,6 +25,7 @@ def test_null_arguments_are_rejected(golap):
     out = ctypes.c_void_p()
     assert lib.golap_encode(None, 10, 99, None, ctypes.byref(out)) == golap.EINVAL
     assert lib.golap_q1_run(None, None, None) == golap.EINVAL
+    assert lib.golap_q1_run_files(None, None, None, None) == golap.EINVAL
 
 
 def test_no_device_reports_enodev(golap):
