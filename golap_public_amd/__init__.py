@@ -170,6 +170,7 @@ _PROTOS = {
     "golap_hashmap_keys": (C.c_int, [vp, vp, vp, u64, C.POINTER(u64)]),
     "golap_host_column_save": (C.c_int, [vp, C.c_char_p]),
     "golap_host_column_load": (C.c_int, [C.c_char_p, C.POINTER(vp)]),
+    "golap_encode_device": (C.c_int, [vp, vp, u64, u32, C.c_void_p, C.POINTER(vp), C.POINTER(C.c_double)]),
     "golap_column_load_file": (C.c_int, [vp, C.c_char_p, u64, u32, C.POINTER(vp), C.c_void_p]),
     "golap_eqhist_build": (C.c_int, [vp, vp, u32, u64, u64, u32, C.POINTER(vp)]),
     "golap_eqhist_check": (C.c_int, [vp, vp, i64, i64, vp]),
@@ -433,6 +434,14 @@ class Context:
         self.check(self.lib.golap_prepare_compressed_device(
             self.h, _ptr(values), values.size, TYPE_OF[values.dtype], C.byref(encode_opts(**kw)),
             C.byref(out), C.byref(ms)), "prepare_compressed_device")
+        return out.value, ms.value
+
+    def encode_device(self, dev_values, n_rows, type_, **kw):
+        """prepare_compressed on the device: values already in HBM coded into a new device
+        column, byte-identical to HostColumn.encode + upload. Returns (column, kernel_ms)."""
+        out, ms = vp(), C.c_double()
+        self.check(self.lib.golap_encode_device(self.h, dev_values, n_rows, type_, C.byref(encode_opts(**kw)),
+                                                C.byref(out), C.byref(ms)), "encode_device")
         return out.value, ms.value
 
     def load_file(self, path, segment_bytes=0, n_buffers=0):

@@ -157,6 +157,15 @@ int golap_prepare_compressed_device(golap_ctx *ctx, const void *host_values, uin
                                     uint32_t type, const golap_encode_opts *opts,
                                     golap_column **out, double *comp_ms);
 
+/* prepare_compressed's compression on the device (ChunkCompressor, core/comp.cuh:81-149, in
+ * the chain of core/apps.cuh:148-255; nvcomp there): n_rows values of `type` already in HBM
+ * (dev_values, 16-byte aligned) coded into a new device column, zone maps included. The column
+ * is byte-identical to golap_encode + golap_column_upload of the same values with the same
+ * options: same codec per page, same LZ4 parse, same layout. Pages up to 64 KiB (1 MiB for RAW
+ * and RLE). *kernel_ms = device time of the coding, zone-map and placement kernels. */
+int golap_encode_device(golap_ctx *ctx, const void *dev_values, uint64_t n_rows, uint32_t type,
+                        const golap_encode_opts *opts, golap_column **out, double *kernel_ms);
+
 /* ------------------------------------------------------------------ decompression
  * ChunkDecompressor::step (core/comp.cuh:171-195), nvcomp decompress (core/comp.cuh:186):
  * decode chunks [chunk_begin, chunk_end) into dev_out, which holds

@@ -26,6 +26,7 @@ def test_null_arguments_are_rejected(golap):
     assert lib.golap_encode(None, 10, 99, None, ctypes.byref(out)) == golap.EINVAL
     assert lib.golap_q1_run(None, None, None) == golap.EINVAL
     assert lib.golap_q1_run_files(None, None, None, None) == golap.EINVAL
+    assert lib.golap_encode_device(None, None, 0, golap.I32, None, None, None) == golap.EINVAL
 
 
 def test_no_device_reports_enodev(golap):
