@@ -72,8 +72,8 @@ __device__ __forceinline__ void store_typed(uint8_t *p, uint32_t elem, uint32_t 
     }
 }
 
-// Four bytes at byte x of a 4-byte aligned page: two aligned dwords funnelled by x & 3 bytes
-// (v_alignbyte_b32 shifts the 64-bit pair right by 8 * its third operand).
+// Four bytes at byte x of a 4-byte aligned page (staged in LDS): two aligned dwords funnelled by
+// x & 3 bytes (v_alignbyte_b32 shifts the 64-bit pair right by 8 * its third operand).
 __device__ __forceinline__ uint32_t rd32(const uint8_t *p, uint32_t x) {
     const uint32_t *w = reinterpret_cast<const uint32_t *>(p + (x & ~3u));
     return __builtin_amdgcn_alignbyte(w[1], w[0], x & 3u);
@@ -129,15 +129,15 @@ __device__ __forceinline__ void copy_bytes(uint8_t *dst, const uint8_t *src, uin
 
 // The host encoder's greedy parse (one hash probe per position, acceleration after 64 misses,
 // matches shorter than min_match not emitted), run by the whole wave with identical control
-// flow. The hash table holds page positions; HT_EMPTY entries and entries not below ip are
-// rejected exactly as the host rejects entries of earlier pages. A literal-only block is
-// written `shift` bytes into the slot so that its payload is 16-byte aligned there, as it will
-// be in the column. Returns the block size; *literal_only set.
-__device__ uint32_t lz4_page(const uint8_t *src, uint32_t n, uint8_t *out, uint32_t min_match, uint16_t *ht,
-                             bool &literal_only) {
+// flow over the page staged in LDS (src). The hash table holds page positions; HT_EMPTY
+// entries and entries not below ip are rejected exactly as the host rejects entries of earlier
+// pages. The workgroup is one wave, whose LDS accesses complete in issue order, so the probe
+// needs no barrier between reading and rewriting a table entry. A literal-only block is written
+// `shift` bytes into the slot so that its payload is 16-byte aligned there, as it will be in the
+// column. Returns the block size; *literal_only set.
+__device__ __forceinline__ uint32_t lz4_page(const uint8_t *src, uint32_t n, uint8_t *out, uint32_t min_match,
+                                             uint16_t *ht, bool &literal_only) {
     const uint32_t lane = threadIdx.x;
-    for (uint32_t i = lane; i < HT_SIZE; i += PW) ht[i] = (uint16_t)HT_EMPTY;
-    __syncthreads();
     if (min_match < LZ4_MINMATCH) min_match = LZ4_MINMATCH;
     uint32_t op = 0, anchor = 0, ip = 0;
     if (n >= LZ4_MFLIMIT + 1) {
@@ -147,9 +147,7 @@ __device__ uint32_t lz4_page(const uint8_t *src, uint32_t n, uint8_t *out, uint3
             const uint32_t seq = rd32(src, ip);
             const uint32_t h = (seq * 2654435761u) >> (32 - HT_LOG);
             const uint32_t ref = ht[h];
-            __syncthreads();
             ht[h] = (uint16_t)ip;  // every lane writes the same value
-            __syncthreads();
             if (ref != HT_EMPTY && ref < ip && ip - ref <= LZ4_MAX_OFFSET && rd32(src, ref) == seq) {
                 uint32_t len = 4;
                 for (;;) {  // first position where the bytes differ or the match must end
@@ -357,9 +355,29 @@ __device__ void raw_write(const uint8_t *src, uint32_t n, uint8_t *out) {
     for (uint32_t i = n16 * 16 + threadIdx.x; i < n; i += PW) out[i] = src[i];
 }
 
+// Dynamic LDS of the LZ4 variant: the hash table, then the page being parsed.
+constexpr uint32_t HT_BYTES = HT_SIZE * 2;
+uint32_t encode_lds_bytes(uint64_t page_bytes) { return HT_BYTES + (uint32_t)((page_bytes + 15) / 16 * 16); }
+
+// The page's bytes into LDS and the hash table emptied (the wave's loads and stores complete
+// before the barrier's wait).
+__device__ __forceinline__ void stage_page(const uint8_t *src, uint32_t n, uint8_t *page, uint16_t *ht) {
+    const uint4 empty = make_uint4(0xffffffffu, 0xffffffffu, 0xffffffffu, 0xffffffffu);
+    uint4 *h16 = reinterpret_cast<uint4 *>(ht);
+    for (uint32_t i = threadIdx.x; i < HT_BYTES / 16; i += PW) h16[i] = empty;
+    const uint32_t n16 = n / 16;
+    const uint4 *s16 = reinterpret_cast<const uint4 *>(src);
+    uint4 *p16 = reinterpret_cast<uint4 *>(page);
+    for (uint32_t i = threadIdx.x; i < n16; i += PW) p16[i] = s16[i];
+    for (uint32_t i = n16 * 16 + threadIdx.x; i < n; i += PW) page[i] = src[i];
+    __syncthreads();
+}
+
 template <bool LZ>
 __global__ __launch_bounds__(PW) void k_encode_pages(EncodeArgs a) {
-    __shared__ uint16_t ht[LZ ? HT_SIZE : 1];
+    extern __shared__ __align__(16) uint8_t lds[];
+    uint16_t *ht = reinterpret_cast<uint16_t *>(lds);
+    uint8_t *page = lds + HT_BYTES;
     for (uint64_t p = blockIdx.x; p < a.n_pages; p += gridDim.x) {
         const uint64_t r0 = p * a.page_rows;
         const uint32_t rows = (uint32_t)min<uint64_t>(a.page_rows, a.n_rows - r0);
@@ -380,7 +398,8 @@ __global__ __launch_bounds__(PW) void k_encode_pages(EncodeArgs a) {
             uint32_t pick = 0xff;  // codec whose bytes must be written (LZ4 is written as it is coded)
             if (LZ && (c == LZ4 || auto_ || (c == DBP && !s.dbp_ok) || (c == FOR && !s.for_ok))) {
                 bool lit_only = false;
-                best = lz4_page(src, nbytes, out, a.min_match, ht, lit_only);
+                stage_page(src, nbytes, page, ht);
+                best = lz4_page(page, nbytes, out, a.min_match, ht, lit_only);
                 code = (uint8_t)(LZ4 | (lit_only ? PAGE_LITERAL_ONLY : 0));
                 pick = LZ4;
             }
@@ -507,6 +526,8 @@ extern "C" int golap_encode_device(golap_ctx *ctx, const void *dev_values, uint6
     const bool lz = o.codec != RAW && o.codec != RLE;  // DBP and FOR fall back to LZ4 per page
     if (page_bytes > (1u << 20) || (lz && page_bytes > LZ4_DEV_MAX_PAGE))
         return fail(ctx, GOLAP_EINVAL, "encode_device: pages up to 64 KiB (1 MiB for RAW and RLE)");
+    if (lz && encode_lds_bytes(page_bytes) > (uint64_t)ctx->lds_max)
+        return fail(ctx, GOLAP_EINVAL, "encode_device: page and hash table exceed the device's LDS");
     if (n_rows && (!dev_values || reinterpret_cast<uintptr_t>(dev_values) % 16))
         return fail(ctx, GOLAP_EINVAL, "encode_device: values must be 16-byte aligned device memory");
     Lock l(ctx->mu);
@@ -558,9 +579,15 @@ extern "C" int golap_encode_device(golap_ctx *ctx, const void *dev_values, uint6
         const uint32_t cgrid = (uint32_t)std::min<uint64_t>(h.n_chunks, (uint64_t)ctx->cus * 8);
         e = hipEventRecord(ctx->ev0, ctx->stream);
         if (e == hipSuccess) {
-            if (lz) hipLaunchKernelGGL(k_encode_pages<true>, dim3(grid), dim3(PW), 0, ctx->stream, a);
-            else hipLaunchKernelGGL(k_encode_pages<false>, dim3(grid), dim3(PW), 0, ctx->stream, a);
-            e = hipGetLastError();
+            const uint32_t lds = lz ? encode_lds_bytes(page_bytes) : 0;
+            if (lds > 65536)
+                e = hipFuncSetAttribute(reinterpret_cast<const void *>(k_encode_pages<true>),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e == hipSuccess) {
+                if (lz) hipLaunchKernelGGL(k_encode_pages<true>, dim3(grid), dim3(PW), lds, ctx->stream, a);
+                else hipLaunchKernelGGL(k_encode_pages<false>, dim3(grid), dim3(PW), 0, ctx->stream, a);
+                e = hipGetLastError();
+            }
         }
         if (e == hipSuccess) {
             hipLaunchKernelGGL(k_chunk_minmax, dim3(cgrid), dim3(BLOCK), 0, ctx->stream, a.src, type, n_rows,
