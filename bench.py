@@ -169,6 +169,22 @@ def run_star(args):
            "roofline": {"bound": "hbm", "achieved": res.comp_bytes / (kernel_ms * 1e-3) / 1e9, "peak": 8000.0,
                         "unit": "GB/s", "frac": res.comp_bytes / (kernel_ms * 1e-3) / 1e9 / 8000.0,
                         "traffic": traffic_of("star_groupby_600M")}}
+    # co-execution: the customer dimension joined on the host (the query 3 variants that keep
+    # the customer table on the CPU), same groups as the all-device plan
+    hm = g.HostHashMap(np.arange(n_cust, dtype=np.int32))
+    hm.build([(nation, 7, 7)])
+    dims = [(cols[0], maps[0], 1992, 7), (cols[1], None, 0, 0), (cols[2], maps[2], 0, 0), (cols[3], maps[3], 1, 1000)]
+    cx = [ctx.star_run_hostdim(dims, cols[4], 1, hm) for _ in range(3)]
+    same = all(np.array_equal(a, b) for a, b in zip(cx[-1][1:4], (keys, sums, counts)))
+    cst = cx[-1][4]
+    log(f"co-execution (customer on the host): {statistics.median(c[4].total_ms for c in cx):.1f} ms, "
+        f"{cst.rounds} rounds, {cst.emitted} rows handed over, host {cst.host_ms:.1f} ms, same groups {same}")
+    if not same:
+        raise SystemExit("star: co-execution groups differ from the device plan's")
+    out["config"]["coexec_customer_on_host"] = {
+        "ms": statistics.median(c[4].total_ms for c in cx), "device_span_ms": cst.device_ms,
+        "host_ms": cst.host_ms, "rounds": cst.rounds, "rows_to_host": cst.emitted, "same_groups": same}
+    hm.free()
     calibrate(ctx, args)
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))

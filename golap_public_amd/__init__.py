@@ -138,6 +138,18 @@ class Q1Result(C.Structure):
                 ("comp_bytes", u64), ("kernel_ms", C.c_double)]
 
 
+class HostPred(C.Structure):
+    _fields_ = [("col", vp), ("type", u32), ("lo", i64), ("hi", i64)]
+
+
+class CoexecStats(C.Structure):
+    _fields_ = [("rounds", u64), ("emitted", u64), ("host_ms", C.c_double), ("device_ms", C.c_double),
+                ("total_ms", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class Q1FilePlan(C.Structure):
     _fields_ = [("paths", C.c_char_p * 4), ("date_map", vp), ("disc_lo", i64), ("disc_hi", i64), ("qty_lo", i64),
                 ("qty_hi", i64), ("price_lo", i64), ("price_hi", i64), ("host_chunk_mask", vp),
@@ -205,6 +217,13 @@ _PROTOS = {
     "golap_group_by": (C.c_int, [vp, C.POINTER(GroupKey), u32, vp, u32, u64, u32, C.POINTER(u64), vp, vp, vp,
                                  u64]),
     "golap_star_run": (C.c_int, [vp, C.POINTER(StarPlan), C.POINTER(StarResult), vp, vp, vp, u64]),
+    "golap_host_hashmap_create": (C.c_int, [u64, vp, C.POINTER(vp)]),
+    "golap_host_hashmap_build": (C.c_int, [vp, u64, C.c_void_p, u32, vp, u32]),
+    "golap_host_hashmap_probe": (C.c_int, [vp, vp, u64, vp, C.POINTER(u64)]),
+    "golap_host_hashmap_filled": (C.c_int, [vp, C.POINTER(u64)]),
+    "golap_host_hashmap_free": (C.c_int, [vp]),
+    "golap_star_run_hostdim": (C.c_int, [vp, C.POINTER(StarPlan), u32, vp, u32, C.POINTER(StarResult),
+                                         C.POINTER(CoexecStats), vp, vp, vp, u64]),
     "golap_scan_run": (C.c_int, [vp, C.POINTER(ScanPlan), C.POINTER(StarResult), vp, vp, vp, u64]),
     "golap_batch_create_star": (C.c_int, [vp, C.POINTER(StarQuery), u32, C.POINTER(vp)]),
     "golap_batch_create_scan": (C.c_int, [vp, C.POINTER(ScanPlan), u32, C.POINTER(vp)]),
@@ -268,6 +287,57 @@ def encode_opts(codec=CODEC_LZ4, page_rows=4096, chunk_rows=262144, lz4_min_matc
     o.codec, o.page_rows, o.chunk_rows, o.lz4_min_match, o.threads = (
         codec, page_rows, chunk_rows, lz4_min_match, threads)
     return o
+
+
+class HostHashMap:
+    """HostHashMap (core/host_structs.hpp:9-78) over a host int32 key column."""
+
+    def __init__(self, keys, ht_size=None):
+        self.keys = np.ascontiguousarray(keys, dtype=np.int32)
+        h = vp()
+        rc = load().golap_host_hashmap_create(ht_size or max(1, self.keys.size), _ptr(self.keys), C.byref(h))
+        if rc:
+            raise GolapError(rc, "host_hashmap_create")
+        self.h = h.value
+
+    def build(self, preds=(), payload=None, threads=0, n_rows=None):
+        """preds: [(host numpy column, lo, hi)]; payload: int32 per row or None (store rids)."""
+        n = self.keys.size if n_rows is None else n_rows
+        cols = [np.ascontiguousarray(c) for c, _, _ in preds]
+        arr = (HostPred * max(1, len(preds)))()
+        for i, (c, (_, lo, hi)) in enumerate(zip(cols, preds)):
+            arr[i] = HostPred(c.ctypes.data, TYPE_OF[c.dtype], lo, hi)
+        pay = None if payload is None else np.ascontiguousarray(payload, dtype=np.int32)
+        rc = load().golap_host_hashmap_build(self.h, n, C.cast(arr, C.c_void_p), len(preds),
+                                             None if pay is None else _ptr(pay), threads)
+        if rc:
+            raise GolapError(rc, "host_hashmap_build")
+
+    def probe(self, keys):
+        keys = np.ascontiguousarray(keys, dtype=np.int32)
+        out = np.zeros(keys.size, np.uint64)
+        hits = u64()
+        rc = load().golap_host_hashmap_probe(self.h, _ptr(keys), keys.size, _ptr(out), C.byref(hits))
+        if rc:
+            raise GolapError(rc, "host_hashmap_probe")
+        return out, hits.value
+
+    @property
+    def filled(self):
+        f = u64()
+        load().golap_host_hashmap_filled(self.h, C.byref(f))
+        return f.value
+
+    def free(self):
+        if self.h:
+            load().golap_host_hashmap_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class HostColumn:
@@ -590,6 +660,25 @@ class Context:
                                            max_groups), "star_run")
         n = r.n_groups
         return r, kout[:n * na].reshape(n, na), sums[:n], cnts[:n]
+
+    def star_run_hostdim(self, dims, value, host_dim, hmap, threads=0, chunk_begin=0, chunk_end=None,
+                         dev_chunk_mask=None, max_groups=None):
+        """A star plan with dims[host_dim] joined on the host through hmap (a HostHashMap; that
+        dim's map entry is ignored). Returns (result, keys, sums, counts, CoexecStats)."""
+        dims = [(d[0], d[1] if i != host_dim else None, d[2], d[3]) for i, d in enumerate(dims)]
+        p = self.star_plan(dims, value, chunk_begin, chunk_end, dev_chunk_mask, AGG_GLOBAL, 0)
+        na = sum(1 for d in dims if d[3])
+        if max_groups is None:
+            max_groups = _domain(dims)
+        kout = np.zeros(max_groups * max(na, 1), np.int32)
+        sums = np.zeros(max_groups, np.int64)
+        cnts = np.zeros(max_groups, np.uint64)
+        r, st = StarResult(), CoexecStats()
+        self.check(self.lib.golap_star_run_hostdim(self.h, C.byref(p), host_dim, hmap.h, threads, C.byref(r),
+                                                   C.byref(st), _ptr(kout), _ptr(sums), _ptr(cnts), max_groups),
+                   "star_run_hostdim")
+        n = r.n_groups
+        return r, kout[:n * na].reshape(n, na), sums[:n], cnts[:n], st
 
     def scan_plan(self, cols, preds=(), keys=(), value_col=-1, chunk_begin=0, chunk_end=None, dev_chunk_mask=None,
                   agg_mode=AGG_AUTO, zone_prune=1, lds_group_slots=0):

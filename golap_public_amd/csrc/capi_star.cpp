@@ -142,7 +142,7 @@ int filter_view(golap_ctx *ctx, const golap_hashmap *dm, DriverFilterView *out, 
 // Validates a plan and fills its device descriptor (all but use_lds, item_begin and the result
 // pointers): fact column indexes, driver, probe tables, driver filter, slice and group strides.
 int prepare_query(golap_ctx *ctx, const golap_star_plan *plan, bool compact, ColumnSet &cs, StarQueryDev &qd,
-                  QueryHost &qh) {
+                  QueryHost &qh, int host_dim) {
     if (!plan || plan->n_dims < 1 || plan->n_dims > STAR_MAX_DIMS || !plan->value || plan->agg_mode > GOLAP_AGG_GLOBAL)
         return fail(ctx, GOLAP_EINVAL, "star: bad plan");
     const uint32_t nd = plan->n_dims;
@@ -151,7 +151,8 @@ int prepare_query(golap_ctx *ctx, const golap_star_plan *plan, bool compact, Col
     if (v->ctx != ctx || !is32(v)) return fail(ctx, GOLAP_EINVAL, "star: value column must be 32-bit");
     for (uint32_t d = 0; d < nd; ++d) {
         const golap_star_dim &sd = plan->dims[d];
-        if (!sd.fk || !sd.map || sd.fk->ctx != ctx || sd.map->ctx != ctx || !sd.map->built)
+        const bool on_host = (int)d == host_dim;  // joined on the host: no device map
+        if (!sd.fk || sd.fk->ctx != ctx || (!on_host && (!sd.map || sd.map->ctx != ctx || !sd.map->built)))
             return fail(ctx, GOLAP_EINVAL, "star: missing column or unbuilt map");
         if (!is32(sd.fk)) return fail(ctx, GOLAP_EINVAL, "star: foreign keys must be 32-bit");
         if (sd.fk->n_rows != v->n_rows || sd.fk->page_rows != v->page_rows || sd.fk->chunk_rows != v->chunk_rows)
@@ -183,6 +184,7 @@ int prepare_query(golap_ctx *ctx, const golap_star_plan *plan, bool compact, Col
     // driver: the dimension whose filter passes the smallest share of its key domain
     double best = 2.0;
     for (uint32_t d = 0; d < nd; ++d) {
+        if ((int)d == host_dim) continue;
         const golap_hashmap *m = plan->dims[d].map;
         const double sel = m->ht_size ? (double)m->filled / (double)m->ht_size : 1.0;
         qh.sel *= sel;
@@ -197,9 +199,11 @@ int prepare_query(golap_ctx *ctx, const golap_star_plan *plan, bool compact, Col
         const golap_star_dim &sd = plan->dims[d];
         const golap_hashmap *m = sd.map;
         if (int r = cs.index(ctx, sd.fk, &qd.fk_col[d])) return r;
-        const bool small = compact && m->small_log2;
-        qd.dim[d].slots = small ? m->d_small : m->d_slots;
-        qd.dim[d].log2cap = small ? m->small_log2 : m->log2cap;
+        if ((int)d != host_dim) {
+            const bool small = compact && m->small_log2;
+            qd.dim[d].slots = small ? m->d_small : m->d_slots;
+            qd.dim[d].log2cap = small ? m->small_log2 : m->log2cap;
+        }
         qd.dim[d].group_lo = sd.group_lo;
         qd.dim[d].group_card = sd.group_card;
         qd.dim[d].group_stride = sd.group_card ? qh.stride[a_idx++] : 0;

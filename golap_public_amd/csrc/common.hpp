@@ -236,6 +236,13 @@ struct StarLaunch {
     const uint4 *items4;
     const DriverFilterView *mfilt;
     uint32_t mcol;
+    // emit mode (emit != nullptr; one query, no LDS group table): rows passing every dimension
+    // but emit_dim are appended to emit as {emit_dim's key, group index, value lo, value hi};
+    // emit_n counts them, appends beyond emit_cap are dropped and flagged DEVERR_CAPACITY
+    int4 *emit;
+    uint32_t *emit_n;
+    uint64_t emit_cap;
+    uint32_t emit_dim;
 };
 // prepare_only: set the kernel's LDS attribute and check the launch without launching (graph
 // capture then records the launch alone).

@@ -49,6 +49,11 @@ struct StarArgs {
     const DriverFilterView *mfilt;
     uint32_t mcol;
     uint32_t n_cols;
+    // emit mode (see StarLaunch)
+    int4 *emit;
+    uint32_t *emit_n;
+    uint64_t emit_cap;
+    uint32_t emit_dim;
 };
 
 constexpr uint32_t QDESC_WORDS = sizeof(StarQueryDev) / 4;
@@ -64,9 +69,67 @@ struct StarShared {
     uint32_t item;
 };
 
+// Emit mode: candidates probe every dimension but emit_dim exactly; the rows that pass all of
+// them are appended as {emit_dim's key, group index over the probed dimensions, value lo, hi}
+// with one atomic per wave (ballot + prefix within the wave). Appends beyond emit_cap are
+// dropped and flagged.
+__device__ __forceinline__ void star_emit(const StarArgs &a, StarShared &S, const uint16_t *cand, uint32_t nc) {
+    const uint32_t nd = S.q.n_dims, drv = S.q.driver, hd = a.emit_dim;
+    const uint32_t lane = threadIdx.x % WAVE;
+    for (uint32_t i0 = 0; i0 < nc; i0 += BLOCK) {  // every lane takes every round: ballots stay whole
+        const uint32_t i = i0 + threadIdx.x;
+        bool ok = i < nc;
+        int32_t hkey = 0;
+        uint64_t gidx = 0;
+        int64_t v = 0;
+        if (ok) {
+            const uint32_t r = cand[i];
+            uint32_t pay = 0;
+            const StarDimProbe &dd = S.q.dim[drv];
+            const int32_t dk = col_value(S.acc[drv], r);
+            const uint32_t h0 = hash32((uint32_t)dk) >> (32 - dd.log2cap);
+            ok = probe_exact(dd.slots, dd.log2cap, dk, dd.slots[h0], h0, pay);
+            for (uint32_t d = 0; d < nd && ok; ++d) {
+                const int32_t key = col_value(S.acc[d], r);
+                if (d == hd) {
+                    hkey = key;
+                    continue;
+                }
+                const StarDimProbe &pd = S.q.dim[d];
+                const uint32_t h = hash32((uint32_t)key) >> (32 - pd.log2cap);
+                ok = probe_exact(pd.slots, pd.log2cap, key, pd.slots[h], h, pay);
+                if (ok && pd.group_card) {
+                    const uint32_t gv = pay - (uint32_t)pd.group_lo;
+                    if (gv >= pd.group_card) {
+                        raise_err(a.err, DEVERR_KEYRANGE);
+                        ok = false;
+                    }
+                    gidx += (uint64_t)gv * pd.group_stride;
+                }
+            }
+            const int32_t raw = col_value(S.acc[nd], r);
+            v = S.q.val_signed ? (int64_t)raw : (int64_t)(uint32_t)raw;
+        }
+        const uint64_t m = __ballot(ok);
+        if (!m) continue;
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(a.emit_n, (uint32_t)__popcll(m));
+        base = __shfl(base, 0, WAVE);
+        const uint64_t pos = (uint64_t)base + (uint64_t)__popcll(m & ((1ull << lane) - 1));
+        if (ok) {
+            if (pos < a.emit_cap)
+                a.emit[pos] = make_int4(hkey, (int32_t)(uint32_t)gidx, (int32_t)(uint32_t)(uint64_t)v,
+                                        (int32_t)(uint32_t)((uint64_t)v >> 32));
+            else
+                raise_err(a.err, DEVERR_CAPACITY);
+        }
+    }
+}
+
 // One page tile of one query: driver filter, compaction, decode of the other coded columns
-// for tiles with candidates, exact probes, group index, aggregation.
-template <int PER>
+// for tiles with candidates, exact probes, group index, aggregation (or, in emit mode, the
+// rows handed to the host).
+template <int PER, bool EMIT>
 __device__ __forceinline__ void star_tile(const StarArgs &a, StarShared &S, uint32_t q, uint64_t p,
                                           const uint32_t *lfilter, uint16_t *cand, uint8_t *win,
                                           const GroupTable &T) {
@@ -165,7 +228,9 @@ __device__ __forceinline__ void star_tile(const StarArgs &a, StarShared &S, uint
         }
     }
     __syncthreads();
-    if (!S.tile_bad) {
+    if (EMIT) {
+        if (!S.tile_bad) star_emit(a, S, cand, nc);
+    } else if (!S.tile_bad) {
         const bool use_lds = S.q.use_lds && T.slots;
         const uint32_t qkey = q << a.group_bits;
         for (uint32_t i = threadIdx.x; i < nc; i += BLOCK) {
@@ -234,7 +299,7 @@ __device__ __forceinline__ void star_set_query(const StarArgs &a, StarShared &S,
 // part) items with one atomic each, which balances slices and pruning of very uneven size.
 // The LDS group table is keyed by (query, group); it is merged into every query's dense global
 // result once at the end.
-template <int PER, bool QUEUE>
+template <int PER, bool QUEUE, bool EMIT>
 __global__ __launch_bounds__(BLOCK) void k_star(StarArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     __shared__ StarShared S;
@@ -271,7 +336,7 @@ __global__ __launch_bounds__(BLOCK) void k_star(StarArgs a) {
             }
             const uint64_t p = S.q.s.page_begin + (it - S.q.s.item_begin);
             if (!slice_keeps(S.q.s, p / a.pages_per_chunk)) continue;
-            star_tile<PER>(a, S, q, p, lfilter, cand, win, T);
+            star_tile<PER, EMIT>(a, S, q, p, lfilter, cand, win, T);
         }
     } else {
         const uint32_t n_items = a.queue[1];
@@ -290,7 +355,7 @@ __global__ __launch_bounds__(BLOCK) void k_star(StarArgs a) {
             const uint64_t c0 = (uint64_t)w.x * a.pages_per_chunk;
             const uint64_t pb = c0 + (uint64_t)part * a.pages_per_item;
             const uint64_t pe = min(min(pb + a.pages_per_item, c0 + a.pages_per_chunk), a.n_pages);
-            for (uint64_t p = pb; p < pe; ++p) star_tile<PER>(a, S, q, p, lfilter, cand, win, T);
+            for (uint64_t p = pb; p < pe; ++p) star_tile<PER, EMIT>(a, S, q, p, lfilter, cand, win, T);
         }
     }
     // merge the LDS-private table into every query's global dense result
@@ -524,10 +589,10 @@ uint32_t star_blocks_per_cu(uint32_t page_rows, size_t lds) {
     int occ = 0;
     hipError_t e = hipErrorInvalidValue;
     switch (page_rows) {
-    case 1024: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_star<4, false>, BLOCK, lds); break;
-    case 2048: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_star<8, false>, BLOCK, lds); break;
-    case 4096: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_star<16, false>, BLOCK, lds); break;
-    case 8192: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_star<32, false>, BLOCK, lds); break;
+    case 1024: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_star<4, false, false>, BLOCK, lds); break;
+    case 2048: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_star<8, false, false>, BLOCK, lds); break;
+    case 4096: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_star<16, false, false>, BLOCK, lds); break;
+    case 8192: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_star<32, false, false>, BLOCK, lds); break;
     default: break;
     }
     return e == hipSuccess ? (uint32_t)std::max(occ, 0) : 0u;
@@ -562,6 +627,11 @@ hipError_t launch_star(golap_ctx *ctx, const StarLaunch &l, hipStream_t stream, 
     a.mfilt = l.mfilt;
     a.mcol = l.mcol;
     a.n_cols = l.n_cols;
+    a.emit = l.emit;
+    a.emit_n = l.emit_n;
+    a.emit_cap = l.emit_cap;
+    a.emit_dim = l.emit_dim;
+    if (l.emit && (l.items4 || l.n_queries != 1 || !l.emit_n || l.lds_gslots)) return hipErrorInvalidValue;
     const uint32_t mb = l.max_blocks_per_cu;
     if (l.items4) {  // shared filter: candidate list and windows only
         a.filter_words = 0;
@@ -581,10 +651,13 @@ hipError_t launch_star(golap_ctx *ctx, const StarLaunch &l, hipStream_t stream, 
     if (lds > (size_t)ctx->lds_max) return hipErrorInvalidConfiguration;
     // the queue schedule's item count is only known on the device: fill the chip
     const uint64_t max_grid = l.items ? ~0ull : a.n_items;
-#define GOLAP_STAR_CASE(ROWS, PER)                                                                      \
-    case ROWS:                                                                                          \
-        return l.items ? launch_star_t(ctx, k_star<PER, true>, a, lds, stream, mb, max_grid, prepare_only) \
-                       : launch_star_t(ctx, k_star<PER, false>, a, lds, stream, mb, max_grid, prepare_only);
+#define GOLAP_STAR_CASE(ROWS, PER)                                                                              \
+    case ROWS:                                                                                                  \
+        if (l.emit)                                                                                             \
+            return l.items ? launch_star_t(ctx, k_star<PER, true, true>, a, lds, stream, mb, max_grid, prepare_only)   \
+                           : launch_star_t(ctx, k_star<PER, false, true>, a, lds, stream, mb, max_grid, prepare_only); \
+        return l.items ? launch_star_t(ctx, k_star<PER, true, false>, a, lds, stream, mb, max_grid, prepare_only)      \
+                       : launch_star_t(ctx, k_star<PER, false, false>, a, lds, stream, mb, max_grid, prepare_only);
     switch (l.page_rows) {
         GOLAP_STAR_CASE(1024, 4)
         GOLAP_STAR_CASE(2048, 8)

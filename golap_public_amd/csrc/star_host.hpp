@@ -52,8 +52,10 @@ int emit_groups_host(golap_ctx *ctx, uint64_t n, const int64_t *hs, const uint64
                      const std::vector<int32_t> &lo, const std::vector<uint32_t> &card,
                      const std::vector<uint64_t> &stride, uint64_t *n_groups, int64_t *total, int32_t *keys,
                      int64_t *sums, uint64_t *counts, uint64_t max_groups, uint64_t *rows = nullptr);
+// host_dim >= 0: that dimension is joined on the host (emit mode): its map may be null, it is
+// never the driver and the kernel does not probe it.
 int prepare_query(golap_ctx *ctx, const golap_star_plan *plan, bool compact, ColumnSet &cs, StarQueryDev &qd,
-                  QueryHost &qh);
+                  QueryHost &qh, int host_dim = -1);
 uint32_t assign_windows(const ColumnSet &cs, uint64_t pb, uint64_t pe, int *win);
 int ensure_hfilter(golap_ctx *ctx, const golap_hashmap *dm);
 // Membership filter of a map's keys: its dense bitmap when that is small, else its Bloom filter.

@@ -370,6 +370,48 @@ typedef struct {
 int golap_star_run(golap_ctx *ctx, const golap_star_plan *plan, golap_star_result *res,
                    int32_t *group_keys, int64_t *sums, uint64_t *counts, uint64_t max_groups);
 
+/* ------------------------------------------------------------------ CPU/GPU co-execution
+ * HostHashMap (core/host_structs.hpp:9-78): a host-side table over a caller-owned int32 key
+ * column (which must stay valid until the build returns). build inserts the rows of
+ * [0, n_rows) that pass every predicate (inclusive bounds on host columns), storing payload[r]
+ * (or the rid r when payload is NULL), with up to `threads` threads (0 = up to 16);
+ * GOLAP_ECAPACITY if more than ht_size rows qualify. probe writes the stored payload (its 32
+ * bits, zero-extended) or rid per key, UINT64_MAX for a miss. */
+typedef struct golap_host_hashmap golap_host_hashmap;
+typedef struct {
+    const void *col;  /* host column */
+    uint32_t type;
+    int64_t lo, hi;
+} golap_host_pred;
+int golap_host_hashmap_create(uint64_t ht_size, const int32_t *keys, golap_host_hashmap **out);
+int golap_host_hashmap_build(golap_host_hashmap *map, uint64_t n_rows, const golap_host_pred *preds,
+                             uint32_t n_preds, const int32_t *payload, uint32_t threads);
+int golap_host_hashmap_probe(const golap_host_hashmap *map, const int32_t *keys, uint64_t n, uint64_t *out,
+                             uint64_t *hits);
+int golap_host_hashmap_filled(const golap_host_hashmap *map, uint64_t *filled);
+int golap_host_hashmap_free(golap_host_hashmap *map);
+
+/* A star plan with dimension host_dim joined on the host: the query 3 variants that keep the
+ * customer table on the CPU (usecases/ssb/queries/03a.hpp, 03c.hpp, 03d.hpp; 03c.hpp:189-488,
+ * HostHashAggregate, core/host_structs.hpp:82-152). plan->dims[host_dim].map is ignored (may
+ * be NULL); its fk column, group_lo and group_card apply, with hmap's payload as the group
+ * attribute. The GPU probes every other dimension exactly and hands the passing rows
+ * {host key, group index, value} to the host in rounds of chunks; host threads (`threads`,
+ * 0 = up to 16) probe hmap and aggregate while the GPU runs the next round. Results, groups
+ * and result fields as golap_star_run (kernel_ms = device span of the rounds); at least one
+ * dimension must stay on the device. */
+typedef struct {
+    uint64_t rounds;
+    uint64_t emitted;   /* fact rows handed to the host */
+    double host_ms;     /* host probe + aggregation time */
+    double device_ms;   /* first round's launch to the last round's end */
+    double total_ms;
+} golap_coexec_stats;
+int golap_star_run_hostdim(golap_ctx *ctx, const golap_star_plan *plan, uint32_t host_dim,
+                           const golap_host_hashmap *hmap, uint32_t threads, golap_star_result *res,
+                           golap_coexec_stats *stats, int32_t *group_keys, int64_t *sums, uint64_t *counts,
+                           uint64_t max_groups);
+
 /* ------------------------------------------------------------------ scan + group-by plans
  * Filter + group-by over row-aligned typed fact columns, fused with decompression:
  *   - the range predicates of pipeline_q1_lineorder (usecases/ssb/queries/01.cuh:26-54,
