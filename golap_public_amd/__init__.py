@@ -143,7 +143,7 @@ class HostPred(C.Structure):
 
 
 class CoexecStats(C.Structure):
-    _fields_ = [("rounds", u64), ("emitted", u64), ("host_ms", C.c_double), ("device_ms", C.c_double),
+    _fields_ = [("rounds", u64), ("splits", u64), ("emitted", u64), ("host_ms", C.c_double), ("device_ms", C.c_double),
                 ("total_ms", C.c_double)]
 
     def as_dict(self):

@@ -238,7 +238,7 @@ struct StarLaunch {
     uint32_t mcol;
     // emit mode (emit != nullptr; one query, no LDS group table): rows passing every dimension
     // but emit_dim are appended to emit as {emit_dim's key, group index, value lo, value hi};
-    // emit_n counts them, appends beyond emit_cap are dropped and flagged DEVERR_CAPACITY
+    // emit_n counts them all, appends beyond emit_cap are dropped
     int4 *emit;
     uint32_t *emit_n;
     uint64_t emit_cap;

@@ -22,6 +22,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
+#include <deque>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -213,12 +214,13 @@ int golap_star_run_hostdim(golap_ctx *ctx, const golap_star_plan *plan, uint32_t
     }
     const uint64_t dom = qh.domain;
 
-    // rounds of whole chunks, at most ROUND_ROWS fact rows each: a round's emitted rows fit its buffer
-    constexpr uint64_t ROUND_ROWS = 8ull << 20;
+    // rounds of whole chunks, about ROUND_ROWS fact rows each, handing over at most `cap` rows:
+    // a round that passes more is halved and run again (one chunk always fits: cap >= its rows)
+    constexpr uint64_t ROUND_ROWS = 64ull << 20, EMIT_CAP = 4ull << 20;
     const uint64_t cb = plan->chunk_begin, ce = plan->chunk_end;
     const uint64_t rpc = v->chunk_rows, cpr = std::max<uint64_t>(1, ROUND_ROWS / rpc);
     const uint64_t rounds = (ce - cb + cpr - 1) / cpr;
-    const uint64_t cap = std::min<uint64_t>(cpr * rpc, std::max<uint64_t>(1, (ce - cb) * rpc));
+    const uint64_t cap = std::min<uint64_t>(std::max<uint64_t>(EMIT_CAP, rpc), std::max<uint64_t>(1, (ce - cb) * rpc));
     const golap_hashmap *dm = plan->dims[qd.driver].map;
     void *dbuf[2] = {nullptr, nullptr};
     int4 *hbuf[2] = {nullptr, nullptr};
@@ -325,8 +327,8 @@ int golap_star_run_hostdim(golap_ctx *ctx, const golap_star_plan *plan, uint32_t
         host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
     };
 
-    auto launch_round = [&](uint64_t k) -> hipError_t {
-        const uint64_t c0 = cb + k * cpr, c1 = std::min(ce, c0 + cpr);
+    // the round over chunks [c0, c1) into buffer `slot`
+    auto launch_range = [&](uint64_t c0, uint64_t c1, int slot) -> hipError_t {
         const uint64_t ppc = L.pages_per_chunk;
         StarQueryDev q = qd;
         q.s.chunk_begin = c0;
@@ -335,31 +337,53 @@ int golap_star_run_hostdim(golap_ctx *ctx, const golap_star_plan *plan, uint32_t
         q.s.page_end = std::min<uint64_t>(v->n_pages, c1 * ppc);
         L.q0 = q;
         L.n_items = q.s.page_end - q.s.page_begin;
-        L.emit = static_cast<int4 *>(dbuf[k & 1]);
-        L.emit_n = static_cast<uint32_t *>(dcount) + (k & 1);
+        L.emit = static_cast<int4 *>(dbuf[slot]);
+        L.emit_n = static_cast<uint32_t *>(dcount) + slot;
         L.emit_cap = cap;
         hipError_t e = hipMemsetAsync(L.emit_n, 0, 4, ctx->stream);
         if (e == hipSuccess) e = launch_star(ctx, L, ctx->stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(hcount + (k & 1), L.emit_n, 4, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(hcount + slot, L.emit_n, 4, hipMemcpyDeviceToHost, ctx->stream);
         return e;
     };
 
     hipError_t e = hipMemsetAsync(ctx->d_scratch, 0, 4, ctx->stream);
     if (e == hipSuccess) e = hipEventRecord(ctx->ev0, ctx->stream);
     if (any) {
-        if (e == hipSuccess) e = launch_round(0);
-        for (uint64_t k = 0; k < rounds && e == hipSuccess; ++k) {
-            e = hipStreamSynchronize(ctx->stream);  // round k's kernel and count
+        std::deque<std::pair<uint64_t, uint64_t>> todo;
+        for (uint64_t k = 0; k < rounds; ++k) todo.emplace_back(cb + k * cpr, std::min(ce, cb + (k + 1) * cpr));
+        int slot = 0;
+        std::pair<uint64_t, uint64_t> cur = todo.front();
+        todo.pop_front();
+        if (e == hipSuccess) e = launch_range(cur.first, cur.second, slot);
+        while (e == hipSuccess) {
+            e = hipStreamSynchronize(ctx->stream);  // the current round's kernel and count
             if (e != hipSuccess) break;
-            const uint64_t n = std::min<uint64_t>(hcount[k & 1], cap);
-            st.emitted += hcount[k & 1];
-            if (n) e = hipMemcpyAsync(hbuf[k & 1], dbuf[k & 1], n * sizeof(int4), hipMemcpyDeviceToHost, ctx->stream);
+            const uint64_t n = hcount[slot];
+            if (n > cap) {  // more rows than the buffer holds: run the halves instead
+                const uint64_t mid = cur.first + (cur.second - cur.first) / 2;
+                todo.emplace_front(mid, cur.second);
+                cur.second = mid;
+                st.splits++;
+                e = launch_range(cur.first, cur.second, slot);
+                continue;
+            }
+            st.emitted += n;
+            if (n) e = hipMemcpyAsync(hbuf[slot], dbuf[slot], n * sizeof(int4), hipMemcpyDeviceToHost, ctx->stream);
             if (e == hipSuccess) e = hipEventRecord(copied, ctx->stream);
-            // round k+1 on the device while the host aggregates round k
-            if (e == hipSuccess && k + 1 < rounds) e = launch_round(k + 1);
+            // the next round on the device while the host aggregates this one
+            const bool more = !todo.empty();
+            std::pair<uint64_t, uint64_t> next{0, 0};
+            if (more) {
+                next = todo.front();
+                todo.pop_front();
+                if (e == hipSuccess) e = launch_range(next.first, next.second, slot ^ 1);
+            }
             if (e == hipSuccess) e = hipEventSynchronize(copied);
-            if (e == hipSuccess) aggregate(hbuf[k & 1], n);
+            if (e == hipSuccess) aggregate(hbuf[slot], n);
             st.rounds++;
+            if (!more) break;
+            cur = next;
+            slot ^= 1;
         }
     }
     if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);

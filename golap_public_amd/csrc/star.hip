@@ -72,7 +72,7 @@ struct StarShared {
 // Emit mode: candidates probe every dimension but emit_dim exactly; the rows that pass all of
 // them are appended as {emit_dim's key, group index over the probed dimensions, value lo, hi}
 // with one atomic per wave (ballot + prefix within the wave). Appends beyond emit_cap are
-// dropped and flagged.
+// dropped; emit_n still counts them, so the host sees the overflow and reruns smaller ranges.
 __device__ __forceinline__ void star_emit(const StarArgs &a, StarShared &S, const uint16_t *cand, uint32_t nc) {
     const uint32_t nd = S.q.n_dims, drv = S.q.driver, hd = a.emit_dim;
     const uint32_t lane = threadIdx.x % WAVE;
@@ -116,13 +116,9 @@ __device__ __forceinline__ void star_emit(const StarArgs &a, StarShared &S, cons
         if (lane == 0) base = atomicAdd(a.emit_n, (uint32_t)__popcll(m));
         base = __shfl(base, 0, WAVE);
         const uint64_t pos = (uint64_t)base + (uint64_t)__popcll(m & ((1ull << lane) - 1));
-        if (ok) {
-            if (pos < a.emit_cap)
-                a.emit[pos] = make_int4(hkey, (int32_t)(uint32_t)gidx, (int32_t)(uint32_t)(uint64_t)v,
-                                        (int32_t)(uint32_t)((uint64_t)v >> 32));
-            else
-                raise_err(a.err, DEVERR_CAPACITY);
-        }
+        if (ok && pos < a.emit_cap)
+            a.emit[pos] = make_int4(hkey, (int32_t)(uint32_t)gidx, (int32_t)(uint32_t)(uint64_t)v,
+                                    (int32_t)(uint32_t)((uint64_t)v >> 32));
     }
 }
 

@@ -402,6 +402,7 @@ int golap_host_hashmap_free(golap_host_hashmap *map);
  * dimension must stay on the device. */
 typedef struct {
     uint64_t rounds;
+    uint64_t splits;    /* rounds halved because they passed more rows than one hand-over holds */
     uint64_t emitted;   /* fact rows handed to the host */
     double host_ms;     /* host probe + aggregation time */
     double device_ms;   /* first round's launch to the last round's end */

@@ -11,23 +11,23 @@ from test_gpu_star import build_map, star_data
 pytestmark = pytest.mark.gpu
 
 
-def setup(ctx, golap, D, codec=1, chunk_rows=65536, reg=1, nat=7, brange=(100, 199)):
+def setup(ctx, golap, D, codec=1, chunk_rows=65536, reg=(1, 1), nat=(7, 7), brange=(100, 199)):
     enc = dict(page_rows=4096, chunk_rows=chunk_rows, lz4_min_match=16)
     cols = {name: ctx.upload(golap.HostColumn.encode(D[name], codec=255 if name == "date" else codec, **enc))
             for name in ("date", "cust", "supp", "part", "rev")}
     maps, bufs = [], []
     for key, n_dim, preds, pay in (
             (D["d_key"], 2556, [], D["d_year"]),
-            (D["c_key"], D["c_key"].size, [(D["c_nation"], nat, nat)], None),
-            (D["s_key"], D["s_key"].size, [(D["s_region"], reg, reg)], None),
+            (D["c_key"], D["c_key"].size, [(D["c_nation"], nat[0], nat[1])], None),
+            (D["s_key"], D["s_key"].size, [(D["s_region"], reg[0], reg[1])], None),
             (D["p_key"], D["p_key"].size, [(D["p_brand"], brange[0], brange[1])], D["p_brand"])):
         m, b = build_map(ctx, golap, key, n_dim, preds, pay)
         maps.append(m)
         bufs += b
     dims = [(cols["date"], maps[0], 1992, 7), (cols["cust"], maps[1], 0, 0), (cols["supp"], maps[2], 0, 0),
             (cols["part"], maps[3], 1, 1000)]
-    host_specs = [(D["d_key"], [], D["d_year"]), (D["c_key"], [(D["c_nation"], nat, nat)], None),
-                  (D["s_key"], [(D["s_region"], reg, reg)], None),
+    host_specs = [(D["d_key"], [], D["d_year"]), (D["c_key"], [(D["c_nation"], nat[0], nat[1])], None),
+                  (D["s_key"], [(D["s_region"], reg[0], reg[1])], None),
                   (D["p_key"], [(D["p_brand"], brange[0], brange[1])], D["p_brand"])]
     return cols, maps, bufs, dims, host_specs
 
@@ -68,10 +68,14 @@ def test_hostdim_equals_device_plan(orc, golap, ctx, host_dim, threads):
         release(ctx, cols, maps, bufs)
 
 
-def test_hostdim_rounds_slices_and_masks(orc, golap, ctx):
-    """Many rounds (8M-row rounds over 20M rows), a chunk slice, a chunk mask."""
+@pytest.mark.parametrize("selective", [True, False])
+def test_hostdim_rounds_slices_and_masks(orc, golap, ctx, selective):
+    """A chunk slice and a chunk mask; with every predicate passing, 20M rows hand over more
+    than one round's 4M-row buffer, so rounds are halved until they fit."""
     D = star_data(golap, 20_000_000, seed=31, n_cust=200_000)
-    cols, maps, bufs, dims, specs = setup(ctx, golap, D, chunk_rows=262144, reg=2, nat=4, brange=(1, 1000))
+    pick = dict(reg=(2, 2), nat=(4, 4), brange=(1, 1000)) if selective else \
+        dict(reg=(0, 4), nat=(0, 24), brange=(1, 1000))
+    cols, maps, bufs, dims, specs = setup(ctx, golap, D, chunk_rows=262144, **pick)
     key, preds, pay = specs[1]
     hm = golap.HostHashMap(key)
     hm.build(preds, payload=pay)
@@ -89,7 +93,11 @@ def test_hostdim_rounds_slices_and_masks(orc, golap, ctx):
             assert (got[0].chunks_run, got[0].chunks_pruned) == (want[0].chunks_run, want[0].chunks_pruned)
             assert got[4].rounds >= 1
         full = ctx.star_run_hostdim(dims, cols["rev"], 1, hm)
-        assert full[4].rounds == -(-nch // 32)
+        if selective:
+            assert full[4].rounds == 1 and full[4].splits == 0
+        else:
+            assert full[4].emitted == full[0].rows_matched == 20_000_000
+            assert full[4].splits >= 3 and full[4].rounds == full[4].splits + 1
         # an empty host map: nothing qualifies
         hm0 = golap.HostHashMap(key)
         hm0.build([(D["c_nation"], 99, 99)])
