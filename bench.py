@@ -337,7 +337,7 @@ def run_batch(args):
     for _ in range(args.steps):
         t1 = time.perf_counter()
         info = b.run(use_graph=True)
-        res = [b.result(i) for i in range(len(qs))]
+        res = b.results()
         walls.append(time.perf_counter() - t1)
         dev_ms.append(info.ms)
     plain_walls, plain_dev = [], []

@@ -222,6 +222,7 @@ _PROTOS = {
     "golap_host_hashmap_probe": (C.c_int, [vp, vp, u64, vp, C.POINTER(u64)]),
     "golap_host_hashmap_filled": (C.c_int, [vp, C.POINTER(u64)]),
     "golap_host_hashmap_free": (C.c_int, [vp]),
+    "golap_batch_results": (C.c_int, [vp, vp, vp, vp, vp, vp, u64]),
     "golap_star_run_hostdim": (C.c_int, [vp, C.POINTER(StarPlan), u32, vp, u32, C.POINTER(StarResult),
                                          C.POINTER(CoexecStats), vp, vp, vp, u64]),
     "golap_scan_run": (C.c_int, [vp, C.POINTER(ScanPlan), C.POINTER(StarResult), vp, vp, vp, u64]),
@@ -825,6 +826,33 @@ class Batch:
                                                        _ptr(cnts), max_groups), "batch_result")
         n = r.n_groups
         return r, kout[:n * na].reshape(n, na), sums[:n], cnts[:n]
+
+    def results(self, max_total=None):
+        """Every query's (result, keys, sums, counts) of the last run, from one call."""
+        nq = len(self.n_attrs)
+        if max_total is None:
+            max_total = max(1, min(sum(self.domains), 1 << 22))
+        na_max = max([1] + list(self.n_attrs))
+        while True:
+            res = (StarResult * nq)()
+            off = np.zeros(nq + 1, np.uint64)
+            keys = np.empty(max_total * na_max, np.int32)
+            sums = np.empty(max_total, np.int64)
+            cnts = np.empty(max_total, np.uint64)
+            rc = self.ctx.lib.golap_batch_results(self.h, C.cast(res, C.c_void_p), _ptr(off), _ptr(keys), _ptr(sums),
+                                                  _ptr(cnts), max_total)
+            if rc == ECAPACITY and int(off[-1]) > max_total:
+                max_total = int(off[-1])
+                continue
+            self.ctx.check(rc, "batch_results")
+            break
+        out, ko = [], 0
+        for q in range(nq):
+            g0, g1, na = int(off[q]), int(off[q + 1]), self.n_attrs[q]
+            n = g1 - g0
+            out.append((res[q], keys[ko:ko + n * na].reshape(n, na), sums[g0:g1], cnts[g0:g1]))
+            ko += n * na
+        return out
 
     def free(self):
         if self.h:

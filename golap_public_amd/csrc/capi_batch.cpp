@@ -861,6 +861,34 @@ int golap_batch_result(golap_batch *b, uint32_t q, golap_star_result *res, int32
     return GOLAP_OK;
 }
 
+int golap_batch_results(golap_batch *b, golap_star_result *res, uint64_t *group_off, int32_t *group_keys,
+                        int64_t *sums, uint64_t *counts, uint64_t max_total_groups) {
+    if (!b || !res || !group_off) return GOLAP_EINVAL;
+    golap_ctx *ctx = b->ctx;
+    Lock l(ctx->mu);
+    if (int r = set_device(ctx)) return r;
+    const uint32_t nq = b->n_queries();
+    uint64_t off = 0, koff = 0;
+    group_off[0] = 0;
+    for (uint32_t q = 0; q < nq; ++q) {
+        const uint64_t room = off < max_total_groups ? max_total_groups - off : 0;
+        const size_t na = b->kind == BatchKind::STAR ? b->qh[q].card.size() : b->sh[q].card.size();
+        const int r = batch_result(b, q, &res[q], group_keys ? group_keys + koff : nullptr, sums ? sums + off : nullptr,
+                                   counts ? counts + off : nullptr, room);
+        if (r && r != GOLAP_ECAPACITY) return fail(ctx, r, b->err);
+        off += res[q].n_groups;
+        koff += res[q].n_groups * na;
+        group_off[q + 1] = off;
+        if (off > max_total_groups) {  // offsets of the rest still reported
+            group_keys = nullptr;
+            sums = nullptr;
+            counts = nullptr;
+        }
+    }
+    if (off > max_total_groups) return fail(ctx, GOLAP_ECAPACITY, "batch_results: more groups than max_total_groups");
+    return GOLAP_OK;
+}
+
 int golap_batch_free(golap_batch *b) {
     if (!b) return GOLAP_EINVAL;
     golap_ctx *ctx = b->ctx;

@@ -105,6 +105,10 @@ std::vector<uint8_t> scan_survivors(const golap_scan_plan &p, const std::vector<
 
 void scan_stats(const golap_scan_plan &p, uint32_t colmask, const std::vector<uint8_t> &run, golap_star_result *res) {
     std::lock_guard<std::mutex> g(g_extra_mu);
+    // per-chunk payload prefixes of the columns read, looked up once
+    std::vector<const uint64_t *> pf;
+    for (uint32_t c = 0; c < p.n_cols; ++c)
+        if ((colmask >> c) & 1u) pf.push_back(g_extra[p.cols[c]].chunk_payload.data());
     for (uint64_t k = 0; k < run.size(); ++k) {
         const uint64_t ch = p.chunk_begin + k;
         if (!run[k]) {
@@ -112,11 +116,7 @@ void scan_stats(const golap_scan_plan &p, uint32_t colmask, const std::vector<ui
             continue;
         }
         res->chunks_run++;
-        for (uint32_t c = 0; c < p.n_cols; ++c)
-            if ((colmask >> c) & 1u) {
-                auto &ex = g_extra[p.cols[c]];
-                res->comp_bytes += ex.chunk_payload[ch + 1] - ex.chunk_payload[ch];
-            }
+        for (const uint64_t *f : pf) res->comp_bytes += f[ch + 1] - f[ch];
     }
 }
 

@@ -54,27 +54,26 @@ int emit_groups_host(golap_ctx *ctx, uint64_t n, const int64_t *hs, const uint64
                      const std::vector<int32_t> &lo, const std::vector<uint32_t> &card,
                      const std::vector<uint64_t> &stride, uint64_t *n_groups, int64_t *total, int32_t *keys,
                      int64_t *sums, uint64_t *counts, uint64_t max_groups, uint64_t *rows) {
-    (void)stride;  // index i = sum of digit[a] * stride[a]: the odometer below walks the digits
+    (void)stride;  // index i = sum of digit[a] * stride[a], the last attribute varying fastest
     uint64_t g = 0, nr = 0;
     int64_t t = 0;
     const size_t na = card.size();
-    std::vector<uint32_t> digit(na, 0);
     for (uint64_t i = 0; i < n; ++i) {
-        if (hc[i]) {
-            t += hs[i];
-            nr += hc[i];
-            if (g < max_groups) {
-                if (keys)
-                    for (size_t a = 0; a < na; ++a) keys[g * na + a] = lo[a] + (int32_t)digit[a];
-                if (sums) sums[g] = hs[i];
-                if (counts) counts[g] = hc[i];
+        if (!hc[i]) continue;  // most of a dense domain is empty: digits only for groups with rows
+        t += hs[i];
+        nr += hc[i];
+        if (g < max_groups) {
+            if (keys) {
+                uint64_t rem = i;
+                for (size_t a = na; a-- > 0;) {
+                    keys[g * na + a] = lo[a] + (int32_t)(rem % card[a]);
+                    rem /= card[a];
+                }
             }
-            ++g;
+            if (sums) sums[g] = hs[i];
+            if (counts) counts[g] = hc[i];
         }
-        for (size_t a = na; a-- > 0;) {  // next index: increment the last digit, carry left
-            if (++digit[a] < card[a]) break;
-            digit[a] = 0;
-        }
+        ++g;
     }
     *n_groups = g;
     if (total) *total = t;
@@ -239,18 +238,18 @@ uint32_t assign_windows(const ColumnSet &cs, uint64_t pb, uint64_t pe, int *win)
 // (empty = all run).
 void slice_stats(const golap_star_plan &plan, const std::vector<uint8_t> &run, golap_star_result *res) {
     std::lock_guard<std::mutex> g(g_extra_mu);
+    // per-chunk payload prefixes of the plan's columns, looked up once
+    const uint64_t *pf[STAR_MAX_DIMS + 1];
+    uint32_t n = 0;
+    for (uint32_t d = 0; d < plan.n_dims; ++d) pf[n++] = g_extra[plan.dims[d].fk].chunk_payload.data();
+    pf[n++] = g_extra[plan.value].chunk_payload.data();
     for (uint64_t ch = plan.chunk_begin; ch < plan.chunk_end; ++ch) {
         if (!run.empty() && !run[ch - plan.chunk_begin]) {
             res->chunks_pruned++;
             continue;
         }
         res->chunks_run++;
-        auto add = [&](const golap_column *c) {
-            auto &ex = g_extra[c];
-            res->comp_bytes += ex.chunk_payload[ch + 1] - ex.chunk_payload[ch];
-        };
-        for (uint32_t d = 0; d < plan.n_dims; ++d) add(plan.dims[d].fk);
-        add(plan.value);
+        for (uint32_t c = 0; c < n; ++c) res->comp_bytes += pf[c][ch + 1] - pf[c][ch];
     }
 }
 

@@ -522,6 +522,12 @@ int golap_batch_tune(golap_batch *b, uint32_t reps, golap_batch_config *best, do
 /* Results of query q of the last run, as the single-query entry points return them. */
 int golap_batch_result(golap_batch *b, uint32_t q, golap_star_result *res, int32_t *group_keys,
                        int64_t *sums, uint64_t *counts, uint64_t max_groups);
+/* Results of every query of the last run in one call: query q's groups are entries
+ * [group_off[q], group_off[q+1]) of sums / counts (group_off has n_queries + 1 entries), its keys
+ * the next n_groups x n_attrs int32 of group_keys after the previous queries' keys.
+ * GOLAP_ECAPACITY (with group_off and res complete) if more than max_total_groups in all. */
+int golap_batch_results(golap_batch *b, golap_star_result *res, uint64_t *group_off, int32_t *group_keys,
+                        int64_t *sums, uint64_t *counts, uint64_t max_total_groups);
 int golap_batch_free(golap_batch *b);
 
 /* ------------------------------------------------------------------ serving loop

@@ -162,6 +162,12 @@ def test_batch_graph_replays_and_plain_launches_agree(golap, ctx, fx):
                 b.run(use_graph=True)
                 for i in range(len(qs)):
                     assert same(b.result(i), want[i]), (rep, i)
+                # every query's groups from one call equal the per-query ones, also when the
+                # first buffer is too small
+                for got, w in zip(b.results(), want):
+                    assert same(got, w) and got[0].n_groups == w[0].n_groups
+                for got, w in zip(b.results(max_total=3), want):
+                    assert same(got, w)
                 fx.single(qs[rep], flags)  # eager work of the program between replays
     finally:
         ctx.free(flags)
