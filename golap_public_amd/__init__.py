@@ -305,6 +305,8 @@ class HostHashMap:
         """preds: [(host numpy column, lo, hi)]; payload: int32 per row or None (store rids)."""
         n = self.keys.size if n_rows is None else n_rows
         cols = [np.ascontiguousarray(c) for c, _, _ in preds]
+        if n > self.keys.size or any(c.size < n for c in cols) or (payload is not None and len(payload) < n):
+            raise ValueError("HostHashMap.build: keys, predicate columns and payload need n_rows entries")
         arr = (HostPred * max(1, len(preds)))()
         for i, (c, (_, lo, hi)) in enumerate(zip(cols, preds)):
             arr[i] = HostPred(c.ctypes.data, TYPE_OF[c.dtype], lo, hi)

@@ -375,13 +375,17 @@ int golap_q1_run_files(golap_ctx *ctx, const golap_q1_file_plan *plan, golap_q1_
                 break;
             }
         }
-        // device slot s held window i - 2, whose scan returned in the previous iteration
+        // device slot s held window i - 2, whose scan returned in the previous iteration; the
+        // four catalogs first (each waits for its own small copies), then the page data, so the
+        // data copies run behind the scan of window i - 1
         hipError_t e = hipSuccess;
         for (int k = 0; k < NCOL && status == GOLAP_OK; ++k) {
             golap_host_column wc;
             window_catalog(f[k].h, w.c0, w.c1, w.base[k], wc);
             status = set_column_window(ctx, dcol[s][k], wc, w.bytes[k], cs);
-            if (status) break;
+        }
+        if (status) break;
+        for (int k = 0; k < NCOL; ++k) {
             uint8_t *d = dcol[s][k]->d_comp;
             for (const Run &run : w.runs) {
                 uint64_t lo, hi;
@@ -394,7 +398,6 @@ int golap_q1_run_files(golap_ctx *ctx, const golap_q1_file_plan *plan, golap_q1_
             // dword pairs read past the window's last page land in zeros
             if (e == hipSuccess) e = hipMemsetAsync(d + w.bytes[k], 0, COMP_TAIL, cs);
         }
-        if (status) break;
         hmask[s].assign(w.c1 - w.c0, 0);
         for (uint64_t ch = w.c0; ch < w.c1; ++ch) hmask[s][ch - w.c0] = keep[ch];
         if (e == hipSuccess) e = hipMemcpyAsync(dmask[s], hmask[s].data(), hmask[s].size() * 2, hipMemcpyHostToDevice, cs);

@@ -25,3 +25,5 @@ def test_host_hashmap_matches_device_semantics(golap):
     m3 = golap.HostHashMap(keys, ht_size=10)  # more rows than ht_size qualify
     with pytest.raises(golap.GolapError):
         m3.build()
+    with pytest.raises(ValueError):  # a predicate column shorter than the keys
+        golap.HostHashMap(keys).build([(attr[:10], 0, 5)])
