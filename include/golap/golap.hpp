@@ -5,12 +5,14 @@
 //
 // What differs from the reference, by design:
 //  * errors throw golap::Error (the reference aborts in checkCudaErrors);
-//  * compressed chunks stay resident in HBM (no SSD loaders, no cuFile);
+//  * compressed chunks stay resident in HBM, or stream from column files through pinned host
+//    memory (query1_files); no cuFile;
 //  * the fused plans (query1, star) take whole chunk ranges in one launch instead of one
 //    kernel per chunk driven from host worker threads; the per-chunk Op protocol remains for
 //    callers that step chunks themselves (ChunkDecompressor, Collector, TableLoader).
 #pragma once
 #include <algorithm>
+#include <array>
 #include <cstdint>
 #include <memory>
 #include <stdexcept>
@@ -176,21 +178,12 @@ struct same {  // keeps a parameter out of template argument deduction (so nullp
     using type = X;
 };
 }  // namespace detail
+namespace detail {
+// A device column c of num_tuples values into ci (catalog -> BlockInfo list), with the zone maps
+// into minmax and the histogram / Bloom metadata built from the values at dev_values.
 template <typename T>
-double prepare_compressed_device(Device &dev, const T *col, uint64_t num_tuples, CompInfo &ci,
-                                 typename detail::same<MinMaxMeta<T>>::type *minmax = nullptr,
-                                 typename detail::same<EqHistogram<T>>::type *histogram = nullptr,
-                                 typename detail::same<BloomMeta<T>>::type *bloommeta = nullptr) {
-    golap_encode_opts o;
-    golap_encode_opts_default(&o);
-    o.codec = codec_of(ci.comp_algo);
-    o.page_rows = (uint32_t)ci.page_rows;
-    o.chunk_rows = (uint32_t)(ci.chunk_bytes / sizeof(T));
-    o.lz4_min_match = ci.lz4_min_match;
-    golap_column *c = nullptr;
-    double ms = 0;
-    detail::check(golap_prepare_compressed_device(dev.get(), col, num_tuples, detail::type_of<T>(), &o, &c, &ms),
-                  dev.get(), "prepare_compressed_device");
+void adopt_column(Device &dev, golap_column *c, uint64_t num_tuples, CompInfo &ci, MinMaxMeta<T> *minmax,
+                  EqHistogram<T> *histogram, BloomMeta<T> *bloommeta, const void *dev_values) {
     golap_ctx *ctx = dev.get();
     ci.col = std::shared_ptr<golap_column>(c, [ctx](golap_column *p) { golap_column_free(ctx, p); });
     ci.uncomp_bytes = num_tuples * sizeof(T);
@@ -217,29 +210,77 @@ double prepare_compressed_device(Device &dev, const T *col, uint64_t num_tuples,
         golap_d2h(ctx, minmax->mins.data(), d.chunk_min, d.n_chunks * 8);
         golap_d2h(ctx, minmax->maxs.data(), d.chunk_max, d.n_chunks * 8);
     }
+    auto keep = [ctx](golap_chunkmeta *m) {
+        return std::shared_ptr<golap_chunkmeta>(m, [ctx](golap_chunkmeta *q) { golap_chunkmeta_free(ctx, q); });
+    };
+    golap_chunkmeta *m = nullptr;
+    if (histogram) {
+        check(golap_eqhist_build(ctx, dev_values, type_of<T>(), num_tuples, d.chunk_rows,
+                                 (uint32_t)histogram->bucket_num, &m),
+              ctx, "eqhist_build");
+        histogram->meta = keep(m);
+    }
+    if (bloommeta) {
+        check(golap_bloommeta_build(ctx, dev_values, type_of<T>(), num_tuples, d.chunk_rows, bloommeta->p,
+                                    bloommeta->m, &m),
+              ctx, "bloommeta_build");
+        bloommeta->meta = keep(m);
+    }
+}
+
+template <typename T>
+golap_encode_opts encode_opts_of(const CompInfo &ci) {
+    golap_encode_opts o;
+    golap_encode_opts_default(&o);
+    o.codec = codec_of(ci.comp_algo);
+    o.page_rows = (uint32_t)ci.page_rows;
+    o.chunk_rows = (uint32_t)(ci.chunk_bytes / sizeof(T));
+    o.lz4_min_match = ci.lz4_min_match;
+    return o;
+}
+}  // namespace detail
+
+// prepare_compressed_device (core/apps.cuh:257-275) for host values: coded on the host, uploaded;
+// returns the coding time in ms.
+template <typename T>
+double prepare_compressed_device(Device &dev, const T *col, uint64_t num_tuples, CompInfo &ci,
+                                 typename detail::same<MinMaxMeta<T>>::type *minmax = nullptr,
+                                 typename detail::same<EqHistogram<T>>::type *histogram = nullptr,
+                                 typename detail::same<BloomMeta<T>>::type *bloommeta = nullptr) {
+    const golap_encode_opts o = detail::encode_opts_of<T>(ci);
+    golap_column *c = nullptr;
+    double ms = 0;
+    detail::check(golap_prepare_compressed_device(dev.get(), col, num_tuples, detail::type_of<T>(), &o, &c, &ms),
+                  dev.get(), "prepare_compressed_device");
+    golap_ctx *ctx = dev.get();
     if (histogram || bloommeta) {
-        // the values on the device, chunk by chunk as the column's chunks
         void *v = nullptr;
         detail::check(golap_dev_alloc(ctx, std::max<uint64_t>(num_tuples, 1) * sizeof(T), &v), ctx, "dev_alloc");
         std::shared_ptr<void> guard(v, [ctx](void *q) { golap_dev_free(ctx, q); });
         detail::check(golap_h2d(ctx, v, col, num_tuples * sizeof(T)), ctx, "h2d");
-        auto keep = [ctx](golap_chunkmeta *m) { return std::shared_ptr<golap_chunkmeta>(m, [ctx](golap_chunkmeta *q) {
-                                                      golap_chunkmeta_free(ctx, q);
-                                                  }); };
-        golap_chunkmeta *m = nullptr;
-        if (histogram) {
-            detail::check(golap_eqhist_build(ctx, v, detail::type_of<T>(), num_tuples, d.chunk_rows,
-                                             (uint32_t)histogram->bucket_num, &m),
-                          ctx, "eqhist_build");
-            histogram->meta = keep(m);
-        }
-        if (bloommeta) {
-            detail::check(golap_bloommeta_build(ctx, v, detail::type_of<T>(), num_tuples, d.chunk_rows, bloommeta->p,
-                                                bloommeta->m, &m),
-                          ctx, "bloommeta_build");
-            bloommeta->meta = keep(m);
-        }
+        detail::adopt_column<T>(dev, c, num_tuples, ci, minmax, histogram, bloommeta, v);
+    } else {
+        detail::adopt_column<T>(dev, c, num_tuples, ci, minmax, nullptr, nullptr, nullptr);
     }
+    return ms;
+}
+
+// The same for values already in HBM (the reference's Column in device memory, coded by nvcomp
+// there): coded on the device, byte-identical to the host path; returns the device time in ms.
+// Call with the value type: prepare_compressed_device<int32_t>(dev, values, n, ci, &minmax).
+template <typename T>
+double prepare_compressed_device(Device &dev, const DeviceMem &dev_col, uint64_t num_tuples, CompInfo &ci,
+                                 typename detail::same<MinMaxMeta<T>>::type *minmax = nullptr,
+                                 typename detail::same<EqHistogram<T>>::type *histogram = nullptr,
+                                 typename detail::same<BloomMeta<T>>::type *bloommeta = nullptr) {
+    if (dev_col.size_bytes() < num_tuples * sizeof(T))
+        throw Error(GOLAP_EINVAL, "prepare_compressed_device: buffer smaller than num_tuples values");
+    const golap_encode_opts o = detail::encode_opts_of<T>(ci);
+    golap_column *c = nullptr;
+    double ms = 0;
+    detail::check(golap_encode_device(dev.get(), dev_col.ptr<T>(), num_tuples, detail::type_of<T>(), &o, &c, &ms),
+                  dev.get(), "encode_device");
+    detail::adopt_column<T>(dev, c, num_tuples, ci, minmax, histogram, bloommeta, dev_col.ptr<T>());
     return ms;
 }
 
@@ -405,6 +446,45 @@ uint64_t hash_join_count(HashMap &m, const KEY *dev_probe, uint64_t num) {
     return hits;
 }
 
+// HostHashMap (core/host_structs.hpp:9-78) over a host int32 key column; build inserts the rows
+// passing every predicate (PredInfo over host columns), storing a payload (or the rid).
+class HostHashMap {
+public:
+    HostHashMap(uint64_t ht_size, const int32_t *table) : ht_size(ht_size) {
+        detail::check(golap_host_hashmap_create(ht_size, table, &m_), nullptr, "host_hashmap_create");
+    }
+    ~HostHashMap() {
+        if (m_) golap_host_hashmap_free(m_);
+    }
+    HostHashMap(const HostHashMap &) = delete;
+    HostHashMap &operator=(const HostHashMap &) = delete;
+    template <typename... PREDS>
+    void build(uint64_t num_tuples, const int32_t *payload, const PREDS &...preds) {
+        std::vector<golap_pred> v;
+        detail::collect_preds(v, preds...);
+        std::vector<golap_host_pred> hp;
+        for (const golap_pred &p : v) hp.push_back(golap_host_pred{p.dev_col, p.type, p.lo, p.hi});
+        detail::check(golap_host_hashmap_build(m_, num_tuples, hp.data(), (uint32_t)hp.size(), payload, 0), nullptr,
+                      "host_hashmap_build");
+    }
+    // HostHashMap::probe: payload (or rid) of the key, UINT64_MAX when absent
+    uint64_t probe(int32_t key) const {
+        uint64_t out = 0;
+        detail::check(golap_host_hashmap_probe(m_, &key, 1, &out, nullptr), nullptr, "host_hashmap_probe");
+        return out;
+    }
+    uint64_t filled() const {
+        uint64_t f = 0;
+        golap_host_hashmap_filled(m_, &f);
+        return f;
+    }
+    golap_host_hashmap *get() const { return m_; }
+    uint64_t ht_size;
+
+private:
+    golap_host_hashmap *m_ = nullptr;
+};
+
 // ---------------------------------------------------------------- pruning (metadata.cuh)
 // MetaChecker (core/metadata.cuh:638-698): AND of zone-map predicates into one u16 chunk mask.
 class MetaChecker {
@@ -540,6 +620,61 @@ inline GroupResult star_query(Device &dev, const std::vector<StarDim> &dims, con
     out.counts.resize(r.n_groups);
     for (uint64_t g = 0; g < r.n_groups; ++g) out.keys.emplace_back(k.begin() + g * na, k.begin() + (g + 1) * na);
     return out;
+}
+
+// The query 3 variants with one dimension's table on the CPU (03a.hpp, 03c.hpp, 03d.hpp):
+// dims[host_dim] is joined through host_map (its map entry may be null), the others on the
+// device; the groups equal star_query's.
+inline GroupResult star_query_hostdim(Device &dev, const std::vector<StarDim> &dims, uint32_t host_dim,
+                                      const HostHashMap &host_map, const CompInfo &value,
+                                      const MetaChecker *checker = nullptr, uint64_t max_groups = 1 << 20,
+                                      uint32_t threads = 0) {
+    golap_star_plan p{};
+    p.n_dims = (uint32_t)dims.size();
+    uint32_t na = 0;
+    for (size_t i = 0; i < dims.size(); ++i) {
+        p.dims[i] = golap_star_dim{dims[i].fk->col.get(), dims[i].map ? dims[i].map->get() : nullptr,
+                                   dims[i].group_lo, dims[i].group_card};
+        na += dims[i].group_card ? 1 : 0;
+    }
+    p.value = value.col.get();
+    p.chunk_end = value.blocks.size();
+    p.dev_chunk_mask = checker ? checker->mask() : nullptr;
+    golap_star_result r{};
+    std::vector<int32_t> k(max_groups * std::max(1u, na));
+    GroupResult out;
+    out.sums.resize(max_groups);
+    out.counts.resize(max_groups);
+    detail::check(golap_star_run_hostdim(dev.get(), &p, host_dim, host_map.get(), threads, &r, nullptr, k.data(),
+                                         out.sums.data(), out.counts.data(), max_groups),
+                  dev.get(), "star_query_hostdim");
+    out.sums.resize(r.n_groups);
+    out.counts.resize(r.n_groups);
+    for (uint64_t g = 0; g < r.n_groups; ++g) out.keys.emplace_back(k.begin() + g * na, k.begin() + (g + 1) * na);
+    return out;
+}
+
+// query1 from SSD (01.hpp:350-424): the four columns' files (orderdate, discount, quantity,
+// extendedprice) streamed in windows of chunks, pruned chunks never read; the same result as
+// query1 over the resident columns.
+inline Q1Result query1_files(Device &dev, const std::array<std::string, 4> &paths, const HashMap &date_map,
+                             int64_t disc_lo, int64_t disc_hi, int64_t qty_lo, int64_t qty_hi,
+                             int64_t price_lo = INT32_MIN, int64_t price_hi = INT32_MAX, uint32_t window_chunks = 0,
+                             golap_stream_stats *stats = nullptr) {
+    golap_q1_file_plan p{};
+    for (int i = 0; i < 4; ++i) p.paths[i] = paths[i].c_str();
+    p.date_map = date_map.get();
+    p.disc_lo = disc_lo;
+    p.disc_hi = disc_hi;
+    p.qty_lo = qty_lo;
+    p.qty_hi = qty_hi;
+    p.price_lo = price_lo;
+    p.price_hi = price_hi;
+    p.window_chunks = window_chunks;
+    p.zone_prune = 1;
+    golap_q1_result r{};
+    detail::check(golap_q1_run_files(dev.get(), &p, &r, stats), dev.get(), "query1_files");
+    return Q1Result{r.sum, r.matches, r.chunks_run, r.chunks_pruned, r.comp_bytes, r.kernel_ms};
 }
 
 }  // namespace golap

@@ -2,12 +2,18 @@
 // (usecases/ssb/queries/01.hpp:53-449): prepare_compressed_device per column with MinMaxMeta,
 // HashMap + hash_map_build_pred_other on the date dimension, MetaChecker pruning, the fused
 // query1, and a TableLoader<DecompressEnv> stepping every chunk (the select path,
-// core/hl/select.cuh:11-83). Results are checked against the oracle (oracle/liboracle.so).
+// core/hl/select.cuh:11-83); also a column coded on the device, a star plan with one dimension
+// joined through a HostHashMap, and query1 streamed from column files. Results are checked
+// against the oracle (oracle/liboracle.so).
 // Prints "OK" and exits 0 on success.
+#include <unistd.h>
+
 #include <algorithm>
+#include <array>
 #include <cstdio>
 #include <cstring>
 #include <numeric>
+#include <string>
 #include <vector>
 
 #include "../../include/golap/golap.hpp"
@@ -126,6 +132,67 @@ int main() {
     auto jpruned = golap::query1(dev, ci_date, ci_disc, ci_qty, ci_price, date_map, 4, 6, 30, 50, &jchecker);
     REQUIRE(jpruned.sum == want && jpruned.matches == want_matches);
     REQUIRE(jpruned.chunks_pruned == chunk_num - jkeep);
+
+    // values already in HBM, coded on the device (prepare_compressed_device over a device
+    // Column): the same blocks, zone maps and query result as the host-coded column
+    golap::DeviceMem dvals(dev, golap::Tag<int32_t>{}, n);
+    dvals.copy_from(date.data(), n);
+    golap::CompInfo ci_date_dev;
+    ci_date_dev.chunk_bytes = tuples_per_chunk * 4;
+    ci_date_dev.lz4_min_match = 16;
+    golap::MinMaxMeta<int32_t> mm_date_dev;
+    golap::prepare_compressed_device<int32_t>(dev, dvals, n, ci_date_dev, &mm_date_dev);
+    REQUIRE(ci_date_dev.blocks.size() == chunk_num);
+    for (uint64_t c = 0; c < chunk_num; ++c)
+        REQUIRE(ci_date_dev.blocks[c].offset == ci_date.blocks[c].offset &&
+                ci_date_dev.blocks[c].payload == ci_date.blocks[c].payload);
+    REQUIRE(mm_date_dev.mins == mm_date.mins && mm_date_dev.maxs == mm_date.maxs);
+    auto dq = golap::query1(dev, ci_date_dev, ci_disc, ci_qty, ci_price, date_map, 4, 6, 30, 50);
+    REQUIRE(dq.sum == want && dq.matches == want_matches);
+
+    // one dimension joined on the host (the query 3 variants with a table on the CPU): group by
+    // year through a device map and by month through a HostHashMap over the date keys
+    std::vector<int32_t> year32(d_year.begin(), d_year.end()), month32(d_month.begin(), d_month.end());
+    golap::DeviceMem dyi(dev, golap::Tag<int32_t>{}, 2556), dmi(dev, golap::Tag<int32_t>{}, 2556);
+    dyi.copy_from(year32.data(), 2556);
+    dmi.copy_from(month32.data(), 2556);
+    golap::HashMap ymap(dev, 2556, dk.ptr<int32_t>()), mmap(dev, 2556, dk.ptr<int32_t>());
+    golap::hash_map_build_pred_payload(ymap, 2556, dyi.ptr<int32_t>());
+    golap::hash_map_build_pred_payload(mmap, 2556, dmi.ptr<int32_t>());
+    golap::HostHashMap hmonth(2556, d_key.data());
+    hmonth.build(2556, month32.data());
+    REQUIRE(hmonth.filled() == 2556 && hmonth.probe(d_key[100]) == (uint64_t)month32[100]);
+    REQUIRE(hmonth.probe(-5) == UINT64_MAX);
+    std::vector<golap::StarDim> sd = {{&ci_date, &ymap, 1992, 7}, {&ci_date, &mmap, 1, 12}};
+    auto g_dev = golap::star_query(dev, sd, ci_price);
+    sd[1].map = nullptr;
+    auto g_host = golap::star_query_hostdim(dev, sd, 1, hmonth, ci_price);
+    REQUIRE(!g_dev.sums.empty() && g_dev.keys == g_host.keys && g_dev.sums == g_host.sums &&
+            g_dev.counts == g_host.counts);
+
+    // query1 streamed from column files (query1 from SSD, 01.hpp:350-424)
+    {
+        golap_encode_opts o;
+        golap_encode_opts_default(&o);
+        o.codec = GOLAP_CODEC_LZ4;
+        o.lz4_min_match = 16;
+        std::array<std::string, 4> paths;
+        const void *vals[4] = {date.data(), disc.data(), qty.data(), price.data()};
+        const uint32_t types[4] = {GOLAP_I32, GOLAP_U8, GOLAP_U8, GOLAP_I32};
+        for (int i = 0; i < 4; ++i) {
+            o.chunk_rows = (uint32_t)tuples_per_chunk;
+            golap_host_column *h = nullptr;
+            REQUIRE(golap_encode(vals[i], n, types[i], &o, &h) == GOLAP_OK);
+            paths[i] = "/tmp/golap_facade_" + std::to_string(getpid()) + "_" + std::to_string(i) + ".gcol";
+            REQUIRE(golap_host_column_save(h, paths[i].c_str()) == GOLAP_OK);
+            golap_host_column_free(h);
+        }
+        golap_stream_stats st{};
+        auto fq = golap::query1_files(dev, paths, date_map, 4, 6, 30, 50, INT32_MIN, INT32_MAX, 3, &st);
+        for (auto &p : paths) std::remove(p.c_str());
+        REQUIRE(fq.sum == want && fq.matches == want_matches);
+        REQUIRE(st.windows + st.windows_skipped == (chunk_num + 2) / 3);
+    }
 
     // select path: step every chunk of every column through a TableLoader
     std::vector<uint64_t> all(chunk_num);
