@@ -234,6 +234,27 @@ uint32_t assign_windows(const ColumnSet &cs, uint64_t pb, uint64_t pe, int *win)
     return n;
 }
 
+// Windows and prefix-sum regions of a single plan's columns over pages [pb, pe). A column gets a
+// window when some page there is neither read in place, point-readable nor delta-bitpacked, or
+// when it has delta-bitpack pages and is the driver's column (every row of it is tested, so a
+// full decode is cheaper); other delta-bitpack pages are point-read through per-tile prefix sums
+// (anchor[c] = 1 + region). Returns the windows.
+uint32_t assign_access(const ColumnSet &cs, uint64_t pb, uint64_t pe, int driver_col, int *win, int *anchor,
+                       uint32_t *n_anchors) {
+    std::lock_guard<std::mutex> g(g_extra_mu);
+    uint32_t n = 0, na = 0;
+    for (size_t c = 0; c < cs.cols.size(); ++c) {
+        auto &ex = g_extra[cs.cols[c]];
+        const uint32_t coded = ex.window_prefix[pe] - ex.window_prefix[pb];
+        const uint32_t dbp = ex.dbp_prefix[pe] - ex.dbp_prefix[pb];
+        const bool needs_window = coded > dbp || (dbp && (int)c == driver_col);
+        win[c] = needs_window ? (int)n++ : -1;
+        anchor[c] = !needs_window && dbp ? (int)++na : 0;
+    }
+    *n_anchors = na;
+    return n;
+}
+
 // chunks_run / chunks_pruned / comp_bytes of a plan over its slice, given the chunk flags
 // (empty = all run).
 void slice_stats(const golap_star_plan &plan, const std::vector<uint8_t> &run, golap_star_result *res) {
@@ -345,7 +366,8 @@ int golap_star_run(golap_ctx *ctx, const golap_star_plan *plan, golap_star_resul
     StarLaunch L{};
     L.n_cols = (uint32_t)cs.cols.size();
     for (uint32_t c = 0; c < L.n_cols; ++c) L.col[c] = view(cs.cols[c]);
-    L.n_windows = assign_windows(cs, qd.s.page_begin, qd.s.page_end, L.col_window);
+    L.n_windows = assign_access(cs, qd.s.page_begin, qd.s.page_end, qd.fk_col[qd.driver], L.col_window, L.col_anchor,
+                                &L.n_anchors);
     L.page_rows = v->page_rows;
     L.pages_per_chunk = (uint32_t)(v->chunk_rows / v->page_rows);
     L.n_rows = v->n_rows;
@@ -359,14 +381,15 @@ int golap_star_run(golap_ctx *ctx, const golap_star_plan *plan, golap_star_resul
     if (agg == GOLAP_AGG_AUTO)
         agg = expect >= 16.0 * (double)qh.domain && expect >= 65536.0 ? GOLAP_AGG_LDS : GOLAP_AGG_GLOBAL;
     L.lds_gslots = choose_lds_slots(agg, plan->lds_group_slots, qh.domain);
-    while (L.lds_gslots && star_lds_bytes(L.filter_words, L.page_rows, L.n_windows, L.lds_gslots) > (size_t)ctx->lds_max)
+    const uint32_t na = L.n_anchors;
+    while (L.lds_gslots && star_lds_bytes(L.filter_words, L.page_rows, L.n_windows, L.lds_gslots, na) > (size_t)ctx->lds_max)
         L.lds_gslots >>= 1;
-    if (star_lds_bytes(L.filter_words, L.page_rows, L.n_windows, L.lds_gslots) > (size_t)ctx->lds_max)
+    if (star_lds_bytes(L.filter_words, L.page_rows, L.n_windows, L.lds_gslots, na) > (size_t)ctx->lds_max)
         return fail(ctx, GOLAP_EINVAL, "star: LDS budget exceeded (page_rows or coded columns)");
     // the driver filter stays in global memory (L2) when pinning it in LDS would cost
     // workgroups per CU: the kernel is latency-bound, and tiles in flight matter more
-    if (star_blocks_per_cu(L.page_rows, star_lds_bytes(0, L.page_rows, L.n_windows, L.lds_gslots)) >
-        star_blocks_per_cu(L.page_rows, star_lds_bytes(L.filter_words, L.page_rows, L.n_windows, L.lds_gslots)))
+    if (star_blocks_per_cu(L.page_rows, star_lds_bytes(0, L.page_rows, L.n_windows, L.lds_gslots, na)) >
+        star_blocks_per_cu(L.page_rows, star_lds_bytes(L.filter_words, L.page_rows, L.n_windows, L.lds_gslots, na)))
         L.filter_words = 0;
     qd.use_lds = L.lds_gslots != 0;
     L.group_bits = std::max<uint32_t>(1, bits_for(qh.domain));

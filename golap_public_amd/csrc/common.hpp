@@ -210,6 +210,11 @@ struct StarQueryDev {
 struct StarLaunch {
     ColView col[STAR_MAX_COLS];
     int col_window[STAR_MAX_COLS];
+    // 1 + the LDS prefix-sum region of a column read without a window whose delta-bitpack
+    // pages the tiles point-read; 0 = none (the k_star schedules only; the shared-filter kernel
+    // decodes such pages into windows)
+    int col_anchor[STAR_MAX_COLS];
+    uint32_t n_anchors;
     uint32_t n_cols;
     uint32_t page_rows;
     uint64_t n_rows, n_pages;
@@ -270,7 +275,8 @@ hipError_t launch_read_probe(golap_ctx *ctx, const void *p, uint64_t bytes, uint
 // schedule with pages_per_item = pages per chunk.
 int solo_worklist(golap_ctx *ctx, const void *desc, uint32_t desc_size, uint32_t pages_per_chunk, const void **d_desc,
                   uint32_t **queue, uint2 **items);
-size_t star_lds_bytes(uint32_t filter_words, uint32_t page_rows, uint32_t n_windows, uint32_t lds_gslots);
+size_t star_lds_bytes(uint32_t filter_words, uint32_t page_rows, uint32_t n_windows, uint32_t lds_gslots,
+                      uint32_t n_anchors = 0);
 // Workgroups per CU of the star kernel for pages of page_rows rows and `lds` bytes of dynamic LDS.
 uint32_t star_blocks_per_cu(uint32_t page_rows, size_t lds);
 hipError_t launch_hash_filter(golap_ctx *ctx, const uint64_t *slots, uint32_t log2cap, uint32_t *filter,

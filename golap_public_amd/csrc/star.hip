@@ -33,6 +33,7 @@ struct StarArgs {
     uint32_t pages_per_chunk;
     uint64_t n_rows, n_pages;
     uint32_t n_windows;
+    uint32_t n_anchors;  // LDS prefix-sum regions (page_rows / 32 words each)
     uint32_t n_queries;
     StarQueryDev q0;
     const StarQueryDev *queries;
@@ -127,7 +128,7 @@ __device__ __forceinline__ void star_emit(const StarArgs &a, StarShared &S, cons
 // rows handed to the host).
 template <int PER, bool EMIT>
 __device__ __forceinline__ void star_tile(const StarArgs &a, StarShared &S, uint32_t q, uint64_t p,
-                                          const uint32_t *lfilter, uint16_t *cand, uint8_t *win,
+                                          const uint32_t *lfilter, uint16_t *cand, uint8_t *win, uint32_t *ancs,
                                           const GroupTable &T) {
     constexpr int G = PER / 4;
     const uint32_t P = a.page_rows;
@@ -140,7 +141,8 @@ __device__ __forceinline__ void star_tile(const StarArgs &a, StarShared &S, uint
     }
     if (threadIdx.x < ncols) {
         const StarCol &sc = a.col[threadIdx.x < nd ? S.q.fk_col[threadIdx.x] : S.q.val_col];
-        S.acc[threadIdx.x] = col_access(sc, p, rows, sc.window >= 0 ? win + (size_t)sc.window * P * 4 : nullptr);
+        S.acc[threadIdx.x] = col_access(sc, p, rows, sc.window >= 0 ? win + (size_t)sc.window * P * 4 : nullptr,
+                                        sc.anchors >= 0 ? ancs + (size_t)sc.anchors * (P / 32) : nullptr);
     }
     __syncthreads();
     bool bad = false;
@@ -212,8 +214,13 @@ __device__ __forceinline__ void star_tile(const StarArgs &a, StarShared &S, uint
         __syncthreads();
         return;
     }
-    // other columns that need a decode: wave w decodes columns w, w+4, ...
+    // other columns that need a decode: wave w decodes columns w, w+4, ... (delta-bitpack
+    // pages without a window: their prefix sums only)
     for (uint32_t c = wave; c < ncols; c += BLOCK / WAVE) {
+        if (S.acc[c].mode == ACC_DBPA) {
+            dbp_anchors_wave(S.acc[c], rows);
+            continue;
+        }
         if (c == drv || S.acc[c].mode != ACC_WINDOW) continue;
         const ColView &cv = a.col[c < nd ? S.q.fk_col[c] : S.q.val_col].c;
         const int64_t got = page_decode_wave(cv.page_codec[p], cv.comp + cv.page_off[p], cv.page_csize[p],
@@ -307,6 +314,8 @@ __global__ __launch_bounds__(BLOCK) void k_star(StarArgs a) {
     off += ((size_t)P * 2 + 15) & ~(size_t)15;
     uint8_t *win = lds + off;
     off += (size_t)a.n_windows * P * 4;
+    uint32_t *ancs = reinterpret_cast<uint32_t *>(lds + off);
+    off += (size_t)a.n_anchors * (P / 32) * 4;
     GroupTable T;
     T.slots = a.lds_gslots;
     T.gkey = reinterpret_cast<uint32_t *>(lds + off);
@@ -332,7 +341,7 @@ __global__ __launch_bounds__(BLOCK) void k_star(StarArgs a) {
             }
             const uint64_t p = S.q.s.page_begin + (it - S.q.s.item_begin);
             if (!slice_keeps(S.q.s, p / a.pages_per_chunk)) continue;
-            star_tile<PER, EMIT>(a, S, q, p, lfilter, cand, win, T);
+            star_tile<PER, EMIT>(a, S, q, p, lfilter, cand, win, ancs, T);
         }
     } else {
         const uint32_t n_items = a.queue[1];
@@ -351,7 +360,7 @@ __global__ __launch_bounds__(BLOCK) void k_star(StarArgs a) {
             const uint64_t c0 = (uint64_t)w.x * a.pages_per_chunk;
             const uint64_t pb = c0 + (uint64_t)part * a.pages_per_item;
             const uint64_t pe = min(min(pb + a.pages_per_item, c0 + a.pages_per_chunk), a.n_pages);
-            for (uint64_t p = pb; p < pe; ++p) star_tile<PER, EMIT>(a, S, q, p, lfilter, cand, win, T);
+            for (uint64_t p = pb; p < pe; ++p) star_tile<PER, EMIT>(a, S, q, p, lfilter, cand, win, ancs, T);
         }
     }
     // merge the LDS-private table into every query's global dense result
@@ -473,7 +482,7 @@ __global__ __launch_bounds__(BLOCK) void k_star_shared(StarArgs a) {
 #pragma unroll
                         for (int j = 0; j < 4; ++j) {
                             const uint32_t r = 4 * (threadIdx.x + k * BLOCK) + j;
-                            kv[4 * k + j] = r < rows ? col_value(dacc, r) : 0;
+                            kv[4 * k + j] = r < rows ? col_value<false>(dacc, r) : 0;
                             valid |= (uint32_t)(r < rows) << (4 * k + j);
                         }
                 }
@@ -518,7 +527,7 @@ __global__ __launch_bounds__(BLOCK) void k_star_shared(StarArgs a) {
             if (S.tile_bad) continue;
             for (uint32_t i = threadIdx.x; i < nc; i += BLOCK) {
                 const uint32_t r = cand[i];
-                const int32_t mk = col_value(S.acc[mcol], r);
+                const int32_t mk = col_value<false>(S.acc[mcol], r);
                 for (uint64_t qm = mask; qm; qm &= qm - 1) {
                     const uint32_t b = (uint32_t)__ffsll((long long)qm) - 1;
                     if (!filter_test(mf[b], mk)) continue;
@@ -528,7 +537,7 @@ __global__ __launch_bounds__(BLOCK) void k_star_shared(StarArgs a) {
                     uint32_t h[STAR_MAX_DIMS];
                     uint64_t first[STAR_MAX_DIMS];
                     for (uint32_t d = 0; d < nd; ++d) {
-                        key[d] = col_value(S.acc[Q.fk_col[d]], r);
+                        key[d] = col_value<false>(S.acc[Q.fk_col[d]], r);
                         h[d] = hash32((uint32_t)key[d]) >> (32 - Q.dim[d].log2cap);
                         first[d] = Q.dim[d].slots[h[d]];
                     }
@@ -548,7 +557,7 @@ __global__ __launch_bounds__(BLOCK) void k_star_shared(StarArgs a) {
                         }
                     }
                     if (!ok) continue;
-                    const int32_t raw = col_value(S.acc[Q.val_col], r);
+                    const int32_t raw = col_value<false>(S.acc[Q.val_col], r);
                     const int64_t v = Q.val_signed ? (int64_t)raw : (int64_t)(uint32_t)raw;
                     group_add_global(Q.g_sum, Q.g_cnt, gidx, v);
                 }
@@ -557,9 +566,10 @@ __global__ __launch_bounds__(BLOCK) void k_star_shared(StarArgs a) {
     }
 }
 
-size_t star_lds_bytes(uint32_t filter_words, uint32_t page_rows, uint32_t n_windows, uint32_t lds_gslots) {
+size_t star_lds_bytes(uint32_t filter_words, uint32_t page_rows, uint32_t n_windows, uint32_t lds_gslots,
+                      uint32_t n_anchors) {
     return (((size_t)filter_words * 4 + 15) & ~(size_t)15) + (((size_t)page_rows * 2 + 15) & ~(size_t)15) +
-           (size_t)n_windows * page_rows * 4 + (size_t)lds_gslots * 16;
+           (size_t)n_windows * page_rows * 4 + (size_t)n_anchors * (page_rows / 32) * 4 + (size_t)lds_gslots * 16;
 }
 
 template <typename K>
@@ -602,12 +612,13 @@ hipError_t launch_star(golap_ctx *ctx, const StarLaunch &l, hipStream_t stream, 
                      l.pages_per_item > l.pages_per_chunk))
         return hipErrorInvalidValue;
     StarArgs a{};
-    for (uint32_t c = 0; c < l.n_cols; ++c) a.col[c] = StarCol{l.col[c], l.col_window[c]};
+    for (uint32_t c = 0; c < l.n_cols; ++c) a.col[c] = StarCol{l.col[c], l.col_window[c], l.col_anchor[c] - 1};
     a.page_rows = l.page_rows;
     a.pages_per_chunk = l.pages_per_chunk;
     a.n_rows = l.n_rows;
     a.n_pages = l.n_pages;
     a.n_windows = l.n_windows;
+    a.n_anchors = l.n_anchors;
     a.n_queries = l.n_queries;
     a.q0 = l.q0;
     a.queries = l.queries;
@@ -628,6 +639,7 @@ hipError_t launch_star(golap_ctx *ctx, const StarLaunch &l, hipStream_t stream, 
     a.emit_cap = l.emit_cap;
     a.emit_dim = l.emit_dim;
     if (l.emit && (l.items4 || l.n_queries != 1 || !l.emit_n || l.lds_gslots)) return hipErrorInvalidValue;
+    if (l.items4 && l.n_anchors) return hipErrorInvalidValue;  // the shared-filter kernel decodes into windows
     const uint32_t mb = l.max_blocks_per_cu;
     if (l.items4) {  // shared filter: candidate list and windows only
         a.filter_words = 0;
@@ -643,7 +655,7 @@ hipError_t launch_star(golap_ctx *ctx, const StarLaunch &l, hipStream_t stream, 
         }
     }
     if (!l.items && !a.n_items) return hipSuccess;
-    const size_t lds = star_lds_bytes(a.filter_words, l.page_rows, l.n_windows, l.lds_gslots);
+    const size_t lds = star_lds_bytes(a.filter_words, l.page_rows, l.n_windows, l.lds_gslots, l.n_anchors);
     if (lds > (size_t)ctx->lds_max) return hipErrorInvalidConfiguration;
     // the queue schedule's item count is only known on the device: fill the chip
     const uint64_t max_grid = l.items ? ~0ull : a.n_items;

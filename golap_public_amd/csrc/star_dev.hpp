@@ -52,20 +52,34 @@ __device__ __forceinline__ bool slice_keeps(const SliceDev &s, uint64_t c) {
 
 struct StarCol {
     ColView c;
-    int window;  // LDS window index when some page of the column in range needs decoding, else -1
+    int window;   // LDS window index when some page of the column in range needs decoding, else -1
+    int anchors;  // without a window: LDS prefix-sum region for its delta-bitpack pages, else -1
 };
 
 // How one 32-bit column is read at a tile row.
-enum : uint32_t { ACC_DIRECT = 0, ACC_WINDOW = 1, ACC_AFFINE = 2, ACC_RLE = 3, ACC_FOR = 4, ACC_BAD = 5 };
+enum : uint32_t { ACC_DIRECT = 0, ACC_WINDOW = 1, ACC_AFFINE = 2, ACC_RLE = 3, ACC_FOR = 4, ACC_BAD = 5, ACC_DBPA = 6 };
 struct ColAccess {
-    const uint8_t *base;  // DIRECT: payload in HBM; WINDOW: LDS window; RLE: page start; FOR: fields
-    int64_t a, b;         // AFFINE: v[r] = a + r * b; FOR: a = base
+    const uint8_t *base;  // DIRECT: payload in HBM; WINDOW: LDS window; RLE: page start; FOR, DBPA: fields
+    int64_t a, b;         // AFFINE: v[r] = a + r * b; FOR: a = base; DBPA: a = v[0], b = min delta
     uint32_t mode;
-    uint32_t nruns;       // RLE: runs; FOR: bits per field
+    uint32_t nruns;       // RLE: runs; FOR, DBPA: bits per field
+    uint32_t *anc;        // DBPA: LDS prefix sums of the fields, one per 32 rows
+    uint32_t words;       // DBPA: packed words
 };
 
-// Sets up the access to column sc for page p (rows rows); wwin: its LDS window or null.
-__device__ inline ColAccess col_access(const StarCol &sc, uint64_t p, uint32_t rows, uint8_t *wwin) {
+// Delta-bitpack field j (row j + 1's delta minus the minimum) of a page's packed words.
+__device__ __forceinline__ uint32_t dbp_field(const uint32_t *pk, uint32_t words, uint32_t bw, uint32_t j) {
+    const uint64_t bit = (uint64_t)j * bw;
+    const uint32_t w = (uint32_t)(bit >> 5);
+    const uint64_t lo = pk[w], hi = w + 1 < words ? pk[w + 1] : 0;
+    return (uint32_t)(((lo | (hi << 32)) >> (bit & 31)) & (bw == 32 ? 0xffffffffull : ((1ull << bw) - 1)));
+}
+
+// Sets up the access to column sc for page p (rows rows); wwin: its LDS window or null; anc: its
+// LDS prefix-sum region or null. A delta-bitpack page that is not point-readable is read
+// through its window when it has one, else through prefix sums the tile computes (ACC_DBPA).
+__device__ inline ColAccess col_access(const StarCol &sc, uint64_t p, uint32_t rows, uint8_t *wwin,
+                                       uint32_t *anc = nullptr) {
     ColAccess x{};
     const uint8_t codec = sc.c.page_codec[p];
     const uint8_t *src = sc.c.comp + sc.c.page_off[p];
@@ -98,6 +112,21 @@ __device__ inline ColAccess col_access(const StarCol &sc, uint64_t p, uint32_t r
         x.base = src;
         x.nruns = nruns;
         x.mode = ok ? ACC_RLE : ACC_BAD;
+    } else if ((codec & PAGE_CODEC_MASK) == golap_fmt::DBP && !wwin && anc) {
+        const uint32_t bw = src[20];
+        const uint32_t words = rows ? (uint32_t)(((uint64_t)(rows - 1) * bw + 31) / 32) : 0;
+        if (csize < 32 || bw == 0 || bw > 32 || src[21] != 4 || ld_u32_unaligned(src + 16) != rows ||
+            32ull + 4ull * words > csize || (reinterpret_cast<uintptr_t>(src) & 3)) {
+            x.mode = ACC_BAD;
+        } else {
+            x.base = src + 32;
+            x.a = (int64_t)ld_u64_unaligned(src);
+            x.b = (int64_t)ld_u64_unaligned(src + 8);
+            x.nruns = bw;
+            x.words = words;
+            x.anc = anc;
+            x.mode = ACC_DBPA;
+        }
     } else {
         x.base = wwin;
         x.mode = wwin ? ACC_WINDOW : ACC_BAD;
@@ -105,11 +134,55 @@ __device__ inline ColAccess col_access(const StarCol &sc, uint64_t p, uint32_t r
     return x;
 }
 
+// ACC_DBPA set-up by one wave: anc[g] = sum of the fields of rows below 32g (mod 2^32), for the
+// ceil(rows / 32) groups of the page.
+__device__ inline void dbp_anchors_wave(const ColAccess &x, uint32_t rows) {
+    const uint32_t lane = threadIdx.x % WAVE;
+    const uint32_t *pk = reinterpret_cast<const uint32_t *>(x.base);
+    const uint32_t bw = x.nruns, nf = rows ? rows - 1 : 0, groups = (rows + 31) / 32;
+    uint32_t carry = 0;
+    for (uint32_t g0 = 0; g0 < groups; g0 += WAVE) {
+        const uint32_t g = g0 + lane;
+        uint32_t s = 0;
+        if (g < groups && 32 * g < nf) {
+            const uint32_t j1 = min(nf, 32 * g + 32);
+            if (bw == 1) {
+                const uint32_t k = j1 - 32 * g;
+                s = __popc(pk[g] & (k == 32 ? 0xffffffffu : ((1u << k) - 1)));
+            } else {
+                for (uint32_t j = 32 * g; j < j1; ++j) s += dbp_field(pk, x.words, bw, j);
+            }
+        }
+        uint32_t incl = s;
+        for (int d = 1; d < WAVE; d <<= 1) {
+            const uint32_t t = __shfl_up(incl, d, WAVE);
+            if (lane >= (uint32_t)d) incl += t;
+        }
+        if (g < groups) x.anc[g] = carry + incl - s;
+        carry += __shfl(incl, WAVE - 1, WAVE);
+    }
+}
+
+// DBPA = false: a kernel that never sets up ACC_DBPA (no prefix-sum regions) leaves its code out.
+template <bool DBPA = true>
 __device__ __forceinline__ int32_t col_value(const ColAccess &x, uint32_t r) {
     int32_t v;
-    switch (x.mode) {
+    switch (DBPA ? x.mode : (x.mode == ACC_DBPA ? ACC_BAD : x.mode)) {
     case ACC_AFFINE: return (int32_t)(uint32_t)(uint64_t)(x.a + (int64_t)r * x.b);
     case ACC_FOR: return (int32_t)(uint32_t)for_value(x.base, (uint64_t)x.a, x.nruns, r);
+    case ACC_DBPA: {
+        if (!DBPA) return 0;
+        // v[r] = v[0] + r * min_delta + (fields of rows 1..r): the group's prefix plus the rest
+        const uint32_t g = r >> 5, j0 = g << 5;
+        const uint32_t *pk = reinterpret_cast<const uint32_t *>(x.base);
+        uint32_t s = x.anc[g];
+        if (x.nruns == 1) {
+            if (r > j0) s += __popc(pk[g] & ((1u << (r - j0)) - 1));
+        } else {
+            for (uint32_t j = j0; j < r; ++j) s += dbp_field(pk, x.words, x.nruns, j);
+        }
+        return (int32_t)((uint32_t)(uint64_t)x.a + r * (uint32_t)(uint64_t)x.b + s);
+    }
     case ACC_RLE: {
         // first run whose exclusive end is above r
         const uint8_t *ends = x.base + 16 + 4ull * x.nruns;

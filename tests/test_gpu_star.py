@@ -47,7 +47,7 @@ QUERIES = [
 
 
 @pytest.mark.parametrize("q", range(len(QUERIES)))
-@pytest.mark.parametrize("codec,mm,date_codec", [(1, 16, 255), (1, 4, 1), (0, 4, 0)])
+@pytest.mark.parametrize("codec,mm,date_codec", [(1, 16, 255), (1, 4, 1), (0, 4, 0), (2, 4, 2)])
 @pytest.mark.parametrize("mode", [0, 2])
 def test_star_matches_oracle(orc, golap, ctx, q, codec, mm, date_codec, mode):
     n = 2_000_000
