@@ -60,6 +60,12 @@ struct golap_batch {
     int windows[golap_impl::STAR_MAX_COLS] = {};
     int dslots[golap_impl::STAR_MAX_COLS] = {};  // scan: register slots of delta-bitpack columns
     uint32_t n_windows = 0;
+    // star, k_star schedules: windows only for the drivers' delta-bitpack columns and other coded
+    // pages, prefix-sum regions (1 + region, 0 = none) for the rest; the shared-filter schedule
+    // uses `windows` above
+    int pt_windows[golap_impl::STAR_MAX_COLS] = {};
+    int pt_anchors[golap_impl::STAR_MAX_COLS] = {};
+    uint32_t pt_n_windows = 0, pt_n_anchors = 0;
     // slices and work list
     uint32_t page_rows = 0, ppc = 0;
     uint64_t n_rows = 0, n_pages = 0;

@@ -118,7 +118,8 @@ bool lds_keys_fit(const golap_batch *b) { return bits_for(b->n_queries()) + grou
 size_t lds_bytes(const golap_batch *b, const golap_batch_config &c) {
     if (b->kind == BatchKind::STAR && c.shared_filter) return star_lds_bytes(0, b->page_rows, b->n_windows, 0);
     if (b->kind == BatchKind::STAR)
-        return star_lds_bytes(c.filter_lds ? b->max_filter_words : 0, b->page_rows, b->n_windows, c.lds_group_slots);
+        return star_lds_bytes(c.filter_lds ? b->max_filter_words : 0, b->page_rows, b->pt_n_windows, c.lds_group_slots,
+                              b->pt_n_anchors);
     return scan_lds_bytes(b->win_bytes, c.lds_group_slots, scan_page_waves(b->ctx, b->win_bytes, c.lds_group_slots));
 }
 
@@ -184,9 +185,11 @@ hipError_t launch_kernel(golap_batch *b, hipStream_t s, bool prepare_only) {
         L.n_cols = (uint32_t)b->cs.cols.size();
         for (uint32_t i = 0; i < L.n_cols; ++i) {
             L.col[i] = view(b->cs.cols[i]);
-            L.col_window[i] = b->windows[i];
+            L.col_window[i] = c.shared_filter ? b->windows[i] : b->pt_windows[i];
+            L.col_anchor[i] = c.shared_filter ? 0 : b->pt_anchors[i];
         }
-        L.n_windows = b->n_windows;
+        L.n_windows = c.shared_filter ? b->n_windows : b->pt_n_windows;
+        L.n_anchors = c.shared_filter ? 0 : b->pt_n_anchors;
         L.page_rows = b->page_rows;
         L.pages_per_chunk = b->ppc;
         L.n_rows = b->n_rows;
@@ -549,6 +552,9 @@ int batch_create_star(golap_ctx *ctx, const golap_star_query *q, uint32_t n, Bat
         pe = std::max(pe, b->qd[i].s.page_end);
     }
     b->n_windows = assign_windows(b->cs, std::min(pb, pe), pe, b->windows);
+    uint32_t drivers = 0;
+    for (uint32_t i = 0; i < n; ++i) drivers |= 1u << b->qd[i].fk_col[b->qd[i].driver];
+    b->pt_n_windows = assign_access(b->cs, std::min(pb, pe), pe, drivers, b->pt_windows, b->pt_anchors, &b->pt_n_anchors);
     if (n > 1)
         if (int r = build_shared_filter(ctx, b)) return bail(r);
     if (int r = finish_create(ctx, b, with_graph)) return bail(r);

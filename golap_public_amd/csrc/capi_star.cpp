@@ -234,12 +234,12 @@ uint32_t assign_windows(const ColumnSet &cs, uint64_t pb, uint64_t pe, int *win)
     return n;
 }
 
-// Windows and prefix-sum regions of a single plan's columns over pages [pb, pe). A column gets a
+// Windows and prefix-sum regions of a launch's columns over pages [pb, pe). A column gets a
 // window when some page there is neither read in place, point-readable nor delta-bitpacked, or
-// when it has delta-bitpack pages and is the driver's column (every row of it is tested, so a
-// full decode is cheaper); other delta-bitpack pages are point-read through per-tile prefix sums
-// (anchor[c] = 1 + region). Returns the windows.
-uint32_t assign_access(const ColumnSet &cs, uint64_t pb, uint64_t pe, int driver_col, int *win, int *anchor,
+// when it has delta-bitpack pages and is a driver's column (bit c of driver_cols: every row of it
+// is tested, so a full decode is cheaper); other delta-bitpack pages are point-read through
+// per-tile prefix sums (anchor[c] = 1 + region). Returns the windows.
+uint32_t assign_access(const ColumnSet &cs, uint64_t pb, uint64_t pe, uint32_t driver_cols, int *win, int *anchor,
                        uint32_t *n_anchors) {
     std::lock_guard<std::mutex> g(g_extra_mu);
     uint32_t n = 0, na = 0;
@@ -247,7 +247,7 @@ uint32_t assign_access(const ColumnSet &cs, uint64_t pb, uint64_t pe, int driver
         auto &ex = g_extra[cs.cols[c]];
         const uint32_t coded = ex.window_prefix[pe] - ex.window_prefix[pb];
         const uint32_t dbp = ex.dbp_prefix[pe] - ex.dbp_prefix[pb];
-        const bool needs_window = coded > dbp || (dbp && (int)c == driver_col);
+        const bool needs_window = coded > dbp || (dbp && ((driver_cols >> c) & 1u));
         win[c] = needs_window ? (int)n++ : -1;
         anchor[c] = !needs_window && dbp ? (int)++na : 0;
     }
@@ -366,8 +366,8 @@ int golap_star_run(golap_ctx *ctx, const golap_star_plan *plan, golap_star_resul
     StarLaunch L{};
     L.n_cols = (uint32_t)cs.cols.size();
     for (uint32_t c = 0; c < L.n_cols; ++c) L.col[c] = view(cs.cols[c]);
-    L.n_windows = assign_access(cs, qd.s.page_begin, qd.s.page_end, qd.fk_col[qd.driver], L.col_window, L.col_anchor,
-                                &L.n_anchors);
+    L.n_windows = assign_access(cs, qd.s.page_begin, qd.s.page_end, 1u << qd.fk_col[qd.driver], L.col_window,
+                                L.col_anchor, &L.n_anchors);
     L.page_rows = v->page_rows;
     L.pages_per_chunk = (uint32_t)(v->chunk_rows / v->page_rows);
     L.n_rows = v->n_rows;

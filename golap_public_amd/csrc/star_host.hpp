@@ -57,7 +57,7 @@ int emit_groups_host(golap_ctx *ctx, uint64_t n, const int64_t *hs, const uint64
 int prepare_query(golap_ctx *ctx, const golap_star_plan *plan, bool compact, ColumnSet &cs, StarQueryDev &qd,
                   QueryHost &qh, int host_dim = -1);
 uint32_t assign_windows(const ColumnSet &cs, uint64_t pb, uint64_t pe, int *win);
-uint32_t assign_access(const ColumnSet &cs, uint64_t pb, uint64_t pe, int driver_col, int *win, int *anchor,
+uint32_t assign_access(const ColumnSet &cs, uint64_t pb, uint64_t pe, uint32_t driver_cols, int *win, int *anchor,
                        uint32_t *n_anchors);
 int ensure_hfilter(golap_ctx *ctx, const golap_hashmap *dm);
 // Membership filter of a map's keys: its dense bitmap when that is small, else its Bloom filter.
