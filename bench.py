@@ -333,12 +333,14 @@ def run_batch(args):
         b.run()
         plain_all()
     # batched: graph replay + every query's groups back on the host
-    dev_ms, walls = [], []
+    dev_ms, walls, run_walls = [], [], []
     for _ in range(args.steps):
         t1 = time.perf_counter()
         info = b.run(use_graph=True)
+        t2 = time.perf_counter()
         res = b.results()
         walls.append(time.perf_counter() - t1)
+        run_walls.append(t2 - t1)
         dev_ms.append(info.ms)
     plain_walls, plain_dev = [], []
     for _ in range(max(1, args.steps // 4)):
@@ -367,6 +369,8 @@ def run_batch(args):
                       "plain_queries_per_s": len(qs) / pwall, "plain_ms": pwall * 1e3,
                       "plain_kernel_ms": statistics.median(plain_dev),
                       "speedup_vs_plain": pwall / wall, "map_build_ms_all_queries": build_ms,
+                      "run_ms": statistics.median(run_walls) * 1e3,
+                      "results_ms": (wall - statistics.median(run_walls)) * 1e3,
                       "tuned_config": best.as_dict(), "default_config": default_cfg, "tune_s": tune_s,
                       "shared_filter_ms": shared_ms},
            "roofline": {"bound": "hbm", "achieved": gbps, "peak": 8000.0, "unit": "GB/s", "frac": gbps / 8000.0,

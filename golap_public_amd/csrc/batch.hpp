@@ -82,6 +82,11 @@ struct golap_batch {
     int mcol = -1;
     uint64_t max_domain = 1;
     std::vector<uint64_t> res_off;
+    // compaction of the non-empty groups (golap_batch_results), allocated on first use: per-query
+    // {arena offset, domain} and first flattened index, the entries on the device and pinned
+    void *d_cmeta = nullptr;
+    uint8_t *d_centries = nullptr, *h_centries = nullptr;
+    uint64_t c_total = 0, c_cap = 0;
     uint64_t arena_bytes = 0;
     golap_batch_config cfg{};
     hipGraph_t graph = nullptr;

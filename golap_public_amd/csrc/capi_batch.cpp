@@ -652,21 +652,11 @@ int batch_finish(golap_batch *b, golap_batch_run_info *info) {
     return GOLAP_OK;
 }
 
-int batch_result(golap_batch *b, uint32_t q, golap_star_result *res, int32_t *keys, int64_t *sums, uint64_t *counts,
-                 uint64_t max_groups) {
-    if (!b->ran) return bfail(b, GOLAP_EINVAL, "batch: no completed run since creation or reconfiguration");
-    BatchSlot *sl = b->slot;
-    if (!b->h_valid) {  // every query's results in one copy
-        BATCH_HIP_CHECK(b, hipMemcpyAsync(sl->h_arena + ARENA_HEADER, sl->d_arena + ARENA_HEADER,
-                                          b->arena_bytes - ARENA_HEADER, hipMemcpyDeviceToHost, sl->stream));
-        BATCH_HIP_CHECK(b, hipStreamSynchronize(sl->stream));
-        b->h_valid = true;
-    }
+// Result fields of query q besides its groups; its group attributes' ranges.
+void result_meta(const golap_batch *b, uint32_t q, golap_star_result *res, const std::vector<int32_t> *&lo,
+                 const std::vector<uint32_t> *&card, const std::vector<uint64_t> *&stride) {
     memset(res, 0, sizeof(*res));
     res->kernel_ms = b->last_ms;
-    const std::vector<int32_t> *lo;
-    const std::vector<uint32_t> *card;
-    const std::vector<uint64_t> *stride;
     if (b->kind == BatchKind::STAR) {
         res->driver_dim = b->qd[q].driver;
         res->lds_group_slots = b->qd[q].use_lds ? b->cfg.lds_group_slots : 0;
@@ -681,6 +671,22 @@ int batch_result(golap_batch *b, uint32_t q, golap_star_result *res, int32_t *ke
         card = &b->sh[q].card;
         stride = &b->sh[q].stride;
     }
+}
+
+int batch_result(golap_batch *b, uint32_t q, golap_star_result *res, int32_t *keys, int64_t *sums, uint64_t *counts,
+                 uint64_t max_groups) {
+    if (!b->ran) return bfail(b, GOLAP_EINVAL, "batch: no completed run since creation or reconfiguration");
+    BatchSlot *sl = b->slot;
+    if (!b->h_valid) {  // every query's results in one copy
+        BATCH_HIP_CHECK(b, hipMemcpyAsync(sl->h_arena + ARENA_HEADER, sl->d_arena + ARENA_HEADER,
+                                          b->arena_bytes - ARENA_HEADER, hipMemcpyDeviceToHost, sl->stream));
+        BATCH_HIP_CHECK(b, hipStreamSynchronize(sl->stream));
+        b->h_valid = true;
+    }
+    const std::vector<int32_t> *lo;
+    const std::vector<uint32_t> *card;
+    const std::vector<uint64_t> *stride;
+    result_meta(b, q, res, lo, card, stride);
     const uint64_t dom = domain_of(b, q);
     const int64_t *hs = reinterpret_cast<const int64_t *>(sl->h_arena + b->res_off[q]);
     const uint64_t *hc = reinterpret_cast<const uint64_t *>(sl->h_arena + b->res_off[q] + dom * 8);
@@ -694,10 +700,123 @@ int batch_result(golap_batch *b, uint32_t q, golap_star_result *res, int32_t *ke
     return GOLAP_OK;
 }
 
+// Every query's non-empty groups compacted on the device and copied back alone, when they are
+// few (at most a quarter of the concatenated domains): *done = true and the results written as
+// golap_batch_results does. Otherwise *done = false (the dense copy serves).
+int batch_results_sparse(golap_batch *b, golap_star_result *res, uint64_t *group_off, int32_t *keys, int64_t *sums,
+                         uint64_t *counts, uint64_t max_total, bool *done) {
+    *done = false;
+    golap_ctx *ctx = b->ctx;
+    BatchSlot *sl = b->slot;
+    const uint32_t nq = b->n_queries();
+    if (!b->d_cmeta) {
+        uint64_t total = 0;
+        std::vector<uint64_t> meta(3ull * nq);  // {arena offset, domain} per query, then first indexes
+        for (uint32_t q = 0; q < nq; ++q) {
+            meta[2 * q] = b->res_off[q];
+            meta[2 * q + 1] = domain_of(b, q);
+            meta[2ull * nq + q] = total;
+            total += domain_of(b, q);
+        }
+        if (total < 4096) return GOLAP_OK;  // small enough to copy densely
+        void *m = nullptr, *e = nullptr;
+        if (int r = dev_alloc(ctx, meta.size() * 8, &m)) return r;
+        const uint64_t bytes = total * 24 + 8;
+        if (int r = dev_alloc(ctx, bytes, &e)) {
+            dev_release(ctx, m);
+            return r;
+        }
+        uint8_t *h = nullptr;
+        if (hipHostMalloc(reinterpret_cast<void **>(&h), bytes, hipHostMallocDefault) != hipSuccess) {
+            dev_release(ctx, m);
+            dev_release(ctx, e);
+            return fail(ctx, GOLAP_ENOMEM, "batch results: pinned buffer");
+        }
+        GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(m, meta.data(), meta.size() * 8, hipMemcpyHostToDevice, sl->stream));
+        GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(sl->stream));
+        b->d_cmeta = m;
+        b->d_centries = static_cast<uint8_t *>(e);
+        b->h_centries = h;
+        b->c_total = total;
+        b->c_cap = total;
+    }
+    const uint64_t cap = b->c_cap;
+    uint2 *d_ql = reinterpret_cast<uint2 *>(b->d_centries);
+    int64_t *d_sum = reinterpret_cast<int64_t *>(b->d_centries + cap * 8);
+    uint64_t *d_cnt = reinterpret_cast<uint64_t *>(b->d_centries + cap * 16);
+    unsigned long long *d_n = reinterpret_cast<unsigned long long *>(b->d_centries + cap * 24);
+    unsigned long long *h_n = reinterpret_cast<unsigned long long *>(b->h_centries + cap * 24);
+    GOLAP_HIP_CHECK(ctx, hipMemsetAsync(d_n, 0, 8, sl->stream));
+    GOLAP_HIP_CHECK(ctx, launch_compact_groups(sl->d_arena, b->d_cmeta, static_cast<const uint64_t *>(b->d_cmeta) + 2ull * nq,
+                                             nq, b->c_total, d_ql, d_sum, d_cnt, cap, d_n, (uint32_t)ctx->cus, sl->stream));
+    GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(h_n, d_n, 8, hipMemcpyDeviceToHost, sl->stream));
+    GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(sl->stream));
+    const uint64_t n = *h_n;
+    if (n > cap || n > b->c_total / 4) return GOLAP_OK;
+    uint2 *h_ql = reinterpret_cast<uint2 *>(b->h_centries);
+    int64_t *h_sum = reinterpret_cast<int64_t *>(b->h_centries + cap * 8);
+    uint64_t *h_cnt = reinterpret_cast<uint64_t *>(b->h_centries + cap * 16);
+    if (n) {
+        GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(h_ql, d_ql, n * 8, hipMemcpyDeviceToHost, sl->stream));
+        GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(h_sum, d_sum, n * 8, hipMemcpyDeviceToHost, sl->stream));
+        GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(h_cnt, d_cnt, n * 8, hipMemcpyDeviceToHost, sl->stream));
+        GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(sl->stream));
+    }
+    // entries by query, then by group index (the key tuple order)
+    std::vector<uint64_t> order(n);
+    for (uint64_t i = 0; i < n; ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](uint64_t x, uint64_t y) {
+        return h_ql[x].x != h_ql[y].x ? h_ql[x].x < h_ql[y].x : h_ql[x].y < h_ql[y].y;
+    });
+    uint64_t off = 0, koff = 0, e = 0;
+    group_off[0] = 0;
+    bool over = false;
+    for (uint32_t q = 0; q < nq; ++q) {
+        const std::vector<int32_t> *lo;
+        const std::vector<uint32_t> *card;
+        const std::vector<uint64_t> *stride;
+        result_meta(b, q, &res[q], lo, card, stride);
+        const size_t na = card->size();
+        uint64_t g = 0, rows = 0;
+        int64_t total = 0;
+        for (; e < n && h_ql[order[e]].x == q; ++e) {
+            const uint64_t k = order[e];
+            total += h_sum[k];
+            rows += h_cnt[k];
+            if (off + g < max_total && !over) {
+                if (keys) {
+                    uint64_t rem = h_ql[k].y;
+                    for (size_t a = na; a-- > 0;) {
+                        keys[koff + g * na + a] = (*lo)[a] + (int32_t)(rem % (*card)[a]);
+                        rem /= (*card)[a];
+                    }
+                }
+                if (sums) sums[off + g] = h_sum[k];
+                if (counts) counts[off + g] = h_cnt[k];
+            }
+            ++g;
+        }
+        (void)stride;
+        res[q].n_groups = g;
+        res[q].total_sum = total;
+        res[q].rows_matched = rows;
+        off += g;
+        koff += g * na;
+        group_off[q + 1] = off;
+        if (off > max_total) over = true;
+    }
+    *done = true;
+    if (off > max_total) return fail(ctx, GOLAP_ECAPACITY, "batch_results: more groups than max_total_groups");
+    return GOLAP_OK;
+}
+
 void batch_free(golap_batch *b) {
     if (!b) return;
     drop_graph(b);
     if (b->d_mfilt) dev_release(b->ctx, b->d_mfilt);
+    if (b->d_cmeta) dev_release(b->ctx, b->d_cmeta);
+    if (b->d_centries) dev_release(b->ctx, b->d_centries);
+    if (b->h_centries) (void)hipHostFree(b->h_centries);
     if (b->own_slot && b->slot) {
         b->slot->release();
         delete b->slot;
@@ -873,6 +992,12 @@ int golap_batch_results(golap_batch *b, golap_star_result *res, uint64_t *group_
     golap_ctx *ctx = b->ctx;
     Lock l(ctx->mu);
     if (int r = set_device(ctx)) return r;
+    if (!b->ran) return fail(ctx, GOLAP_EINVAL, "batch: no completed run since creation or reconfiguration");
+    if (!b->h_valid) {
+        bool done = false;
+        if (int r = batch_results_sparse(b, res, group_off, group_keys, sums, counts, max_total_groups, &done)) return r;
+        if (done) return GOLAP_OK;
+    }
     const uint32_t nq = b->n_queries();
     uint64_t off = 0, koff = 0;
     group_off[0] = 0;

@@ -260,6 +260,12 @@ hipError_t launch_batch_items(const void *qbase, uint32_t qstride, uint32_t n_qu
                               uint64_t total_chunks, uint32_t parts, uint2 *items, uint32_t *queue,
                               hipStream_t stream);
 hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t stream);
+// Non-empty groups of a batch's dense results (query q's sums at arena + qoff[q].x, its counts
+// qoff[q].y entries later; qfirst[q] = its first index of the concatenated domains, total in
+// all): {q, group} + sum + count appended in any order, *n_out counted (also beyond cap).
+hipError_t launch_compact_groups(const uint8_t *arena, const void *qoff, const uint64_t *qfirst, uint32_t nq,
+                                 uint64_t total, uint2 *out_ql, int64_t *out_sum, uint64_t *out_cnt, uint64_t cap,
+                                 unsigned long long *n_out, uint32_t cus, hipStream_t stream);
 // The keys map m holds, ascending, into keys (capacity cap); *n = their count (also when it
 // exceeds cap: then GOLAP_ECAPACITY). Synchronises the context stream.
 int collect_sorted_keys(golap_ctx *ctx, const golap_hashmap *m, int32_t *keys, uint64_t cap, uint64_t *n);

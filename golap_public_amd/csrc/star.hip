@@ -676,6 +676,55 @@ hipError_t launch_star(golap_ctx *ctx, const StarLaunch &l, hipStream_t stream, 
 #undef GOLAP_STAR_CASE
 }
 
+// Non-empty groups of a batch's dense results: entry i of the queries' concatenated domains
+// (qmeta[q] = {result offset in the arena, domain, first flattened index}) with a count is
+// appended as {query, group index} + sum + count, one atomic per wave. Appends stop at cap (the
+// count still grows, so the host sees it).
+__global__ __launch_bounds__(BLOCK) void k_compact_groups(const uint8_t *arena, const ulonglong2 *qoff,
+                                                          const uint64_t *qfirst, uint32_t nq, uint64_t total,
+                                                          uint2 *out_ql, int64_t *out_sum, uint64_t *out_cnt,
+                                                          uint64_t cap, unsigned long long *n_out) {
+    const uint32_t lane = threadIdx.x % WAVE;
+    const uint64_t stride = (uint64_t)gridDim.x * BLOCK;
+    for (uint64_t i0 = (uint64_t)blockIdx.x * BLOCK + (threadIdx.x - lane); i0 < total; i0 += stride) {
+        const uint64_t i = i0 + lane;
+        uint32_t q = 0;
+        uint64_t cnt = 0, g = 0;
+        if (i < total) {
+            uint32_t lo = 0, hi = nq - 1;  // last query whose first index is <= i
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) >> 1;
+                if (qfirst[mid] <= i) lo = mid;
+                else hi = mid - 1;
+            }
+            q = lo;
+            g = i - qfirst[q];
+            cnt = reinterpret_cast<const uint64_t *>(arena + qoff[q].x)[qoff[q].y + g];
+        }
+        const uint64_t m = __ballot(cnt != 0);
+        if (!m) continue;
+        unsigned long long base = 0;
+        if (lane == 0) base = atomicAdd(n_out, (unsigned long long)__popcll(m));
+        base = __shfl(base, 0, WAVE);
+        const uint64_t pos = base + __popcll(m & ((1ull << lane) - 1));
+        if (cnt && pos < cap) {
+            out_ql[pos] = make_uint2(q, (uint32_t)g);
+            out_sum[pos] = reinterpret_cast<const int64_t *>(arena + qoff[q].x)[g];
+            out_cnt[pos] = cnt;
+        }
+    }
+}
+
+hipError_t launch_compact_groups(const uint8_t *arena, const void *qoff, const uint64_t *qfirst, uint32_t nq,
+                                 uint64_t total, uint2 *out_ql, int64_t *out_sum, uint64_t *out_cnt, uint64_t cap,
+                                 unsigned long long *n_out, uint32_t cus, hipStream_t stream) {
+    if (!nq || !total) return hipSuccess;
+    const uint64_t blocks = std::min<uint64_t>((total + BLOCK - 1) / BLOCK, (uint64_t)cus * 8);
+    hipLaunchKernelGGL(k_compact_groups, dim3((uint32_t)blocks), dim3(BLOCK), 0, stream, arena,
+                       static_cast<const ulonglong2 *>(qoff), qfirst, nq, total, out_ql, out_sum, out_cnt, cap, n_out);
+    return hipGetLastError();
+}
+
 // Batch work list. Wave w handles the flattened chunk indexes [64w, 64w + 64) of the queries'
 // slices (chunk_off: prefix of slice lengths in chunks); survivors are compacted with a wave
 // ballot and one global atomic per wave, then write `parts` items each.

@@ -152,6 +152,11 @@ def test_scan_batch_matches_single_and_oracle(orc, golap, ctx, trip):
                 wk, ws, wc = orc.scan_groupby(list(host), p["preds"], p["keys"], p["value_col"], threads=4)
                 assert np.array_equal(got[1], wk) and np.array_equal(got[2], ws) and np.array_equal(got[3], wc)
         want = [b.result(i) for i in range(len(plans))]
+        # all queries' groups from one call (compacted on the device when sparse) equal them
+        for got, w in zip(b.results(), want):
+            assert all(np.array_equal(x, y) for x, y in zip(got[1:], w[1:]))
+            assert (got[0].n_groups, got[0].rows_matched, got[0].total_sum) == \
+                   (w[0].n_groups, w[0].rows_matched, w[0].total_sum)
         for cfg in [dict(lds_group_slots=0), dict(lds_group_slots=256, pages_per_item=1),
                     dict(lds_group_slots=4096, max_blocks_per_cu=1, pages_per_item=4)]:
             say("scan batch config", cfg)
@@ -186,6 +191,9 @@ def test_scan_batch_mixed_column_sets(orc, golap, ctx, trip):
             hosts = [host[dev.index(c)] for c in p["cols"]]
             wk, ws, wc = orc.scan_groupby(hosts, p["preds"], p["keys"], p["value_col"], threads=4)
             assert np.array_equal(got[1], wk) and np.array_equal(got[2], ws) and np.array_equal(got[3], wc)
+        # small domains: the bulk call copies the dense results
+        for i, got in enumerate(b.results()):
+            assert all(np.array_equal(x, y) for x, y in zip(got[1:], b.result(i)[1:])), i
 
 
 def test_scan_long_launch_four_row_form(orc, golap, ctx):
