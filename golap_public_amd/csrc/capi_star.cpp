@@ -58,17 +58,34 @@ int emit_groups_host(golap_ctx *ctx, uint64_t n, const int64_t *hs, const uint64
     uint64_t g = 0, nr = 0;
     int64_t t = 0;
     const size_t na = card.size();
+    // digits of index `at`: stepped like an odometer to the next group with rows, or recomputed
+    // by division when that group lies far ahead (sparse results skip the empty stretches)
+    std::vector<uint32_t> digit(na, 0);
+    uint64_t at = 0;
+    auto move_to = [&](uint64_t idx) {
+        if (idx - at > 64) {
+            uint64_t rem = idx;
+            for (size_t a = na; a-- > 0;) {
+                digit[a] = (uint32_t)(rem % card[a]);
+                rem /= card[a];
+            }
+            at = idx;
+            return;
+        }
+        for (; at < idx; ++at)
+            for (size_t a = na; a-- > 0;) {  // increment the last digit, carry left
+                if (++digit[a] < card[a]) break;
+                digit[a] = 0;
+            }
+    };
     for (uint64_t i = 0; i < n; ++i) {
-        if (!hc[i]) continue;  // most of a dense domain is empty: digits only for groups with rows
+        if (!hc[i]) continue;
         t += hs[i];
         nr += hc[i];
         if (g < max_groups) {
             if (keys) {
-                uint64_t rem = i;
-                for (size_t a = na; a-- > 0;) {
-                    keys[g * na + a] = lo[a] + (int32_t)(rem % card[a]);
-                    rem /= card[a];
-                }
+                move_to(i);
+                for (size_t a = 0; a < na; ++a) keys[g * na + a] = lo[a] + (int32_t)digit[a];
             }
             if (sums) sums[g] = hs[i];
             if (counts) counts[g] = hc[i];
