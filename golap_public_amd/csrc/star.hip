@@ -302,10 +302,12 @@ __device__ __forceinline__ void star_set_query(const StarArgs &a, StarShared &S,
 // part) items with one atomic each, which balances slices and pruning of very uneven size.
 // The LDS group table is keyed by (query, group); it is merged into every query's dense global
 // result once at the end.
-// Five workgroups per CU fit the LDS of the usual launches; the register bound that allows them
-// costs no spills except at 2,048-row pages (PER 8), which keep four.
+// Register bounds for more workgroups per CU (LDS allows six on the usual launches): six at
+// 4,096- and 8,192-row pages (80 registers, a 20-byte spill that costs less than the extra
+// workgroup gains), five at 1,024-row pages and four at 2,048-row pages, where tighter bounds
+// spill more.
 template <int PER, bool QUEUE, bool EMIT>
-__global__ __launch_bounds__(BLOCK, PER == 8 ? 4 : 5) void k_star(StarArgs a) {
+__global__ __launch_bounds__(BLOCK, PER == 8 ? 4 : (PER == 4 ? 5 : 6)) void k_star(StarArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     __shared__ StarShared S;
     const uint32_t P = a.page_rows;
