@@ -360,6 +360,10 @@ int golap_star_run_hostdim(golap_ctx *ctx, const golap_star_plan *plan, uint32_t
             if (e != hipSuccess) break;
             const uint64_t n = hcount[slot];
             if (n > cap) {  // more rows than the buffer holds: run the halves instead
+                if (cur.second - cur.first < 2) {  // one chunk always fits (cap >= its rows)
+                    e = hipErrorInvalidValue;
+                    break;
+                }
                 const uint64_t mid = cur.first + (cur.second - cur.first) / 2;
                 todo.emplace_front(mid, cur.second);
                 cur.second = mid;
