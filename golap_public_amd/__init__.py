@@ -156,6 +156,12 @@ class Q1FilePlan(C.Structure):
                 ("host_chunk_mask_len", u64), ("window_chunks", u32), ("zone_prune", u32)]
 
 
+class StarFilePlan(C.Structure):
+    _fields_ = [("fk_paths", C.c_char_p * 4), ("maps", vp * 4), ("group_lo", C.c_int32 * 4), ("group_card", u32 * 4),
+                ("n_dims", u32), ("value_path", C.c_char_p), ("host_chunk_mask", vp), ("host_chunk_mask_len", u64),
+                ("window_chunks", u32), ("agg_mode", u32)]
+
+
 class StreamStats(C.Structure):
     _fields_ = [("windows", u64), ("windows_skipped", u64), ("bytes_read", u64), ("bytes_copied", u64),
                 ("direct_io", u32), ("window_chunks", u32), ("read_ms", C.c_double), ("total_ms", C.c_double)]
@@ -213,6 +219,8 @@ _PROTOS = {
     "golap_minmax_check": (C.c_int, [vp, vp, i64, i64, vp]),
     "golap_combine_and": (C.c_int, [vp, vp, vp, u64]),
     "golap_q1_run": (C.c_int, [vp, C.POINTER(Q1Plan), C.POINTER(Q1Result)]),
+    "golap_star_run_files": (C.c_int, [vp, C.POINTER(StarFilePlan), C.POINTER(StarResult), C.POINTER(StreamStats),
+                                       vp, vp, vp, u64]),
     "golap_q1_run_files": (C.c_int, [vp, C.POINTER(Q1FilePlan), C.POINTER(Q1Result), C.POINTER(StreamStats)]),
     "golap_group_by": (C.c_int, [vp, C.POINTER(GroupKey), u32, vp, u32, u64, u32, C.POINTER(u64), vp, vp, vp,
                                  u64]),
@@ -663,6 +671,34 @@ class Context:
                                            max_groups), "star_run")
         n = r.n_groups
         return r, kout[:n * na].reshape(n, na), sums[:n], cnts[:n]
+
+    def star_run_files(self, dims, value_path, host_chunk_mask=None, window_chunks=0, agg_mode=AGG_AUTO,
+                       max_groups=None):
+        """A star plan streamed from column files: dims = [(fk file, map, group_lo, group_card)].
+        Returns (result, keys, sums, counts, StreamStats)."""
+        p = StarFilePlan()
+        paths = [str(d[0]).encode() for d in dims]
+        for i, (fk, m, glo, gcard) in enumerate(dims):
+            p.fk_paths[i], p.maps[i], p.group_lo[i], p.group_card[i] = paths[i], m, glo, gcard
+        p.n_dims = len(dims)
+        vpath = str(value_path).encode()
+        p.value_path = vpath
+        mask = None
+        if host_chunk_mask is not None:
+            mask = np.ascontiguousarray(host_chunk_mask, dtype=np.uint16)
+            p.host_chunk_mask, p.host_chunk_mask_len = mask.ctypes.data, mask.size
+        p.window_chunks, p.agg_mode = window_chunks, agg_mode
+        na = sum(1 for d in dims if d[3])
+        if max_groups is None:
+            max_groups = _domain(dims)
+        kout = np.zeros(max_groups * max(na, 1), np.int32)
+        sums = np.zeros(max_groups, np.int64)
+        cnts = np.zeros(max_groups, np.uint64)
+        r, st = StarResult(), StreamStats()
+        self.check(self.lib.golap_star_run_files(self.h, C.byref(p), C.byref(r), C.byref(st), _ptr(kout), _ptr(sums),
+                                                 _ptr(cnts), max_groups), "star_run_files")
+        n = r.n_groups
+        return r, kout[:n * na].reshape(n, na), sums[:n], cnts[:n], st
 
     def star_run_hostdim(self, dims, value, host_dim, hmap, threads=0, chunk_begin=0, chunk_end=None,
                          dev_chunk_mask=None, max_groups=None):

@@ -370,6 +370,27 @@ typedef struct {
 int golap_star_run(golap_ctx *ctx, const golap_star_plan *plan, golap_star_result *res,
                    int32_t *group_keys, int64_t *sums, uint64_t *counts, uint64_t max_groups);
 
+/* A star plan (golap_star_run) streamed from column files in the windows of golap_q1_run_files: the fact columns
+ * are files (fk_paths[d] per dimension, value_path; row-aligned), the dimension maps are device
+ * maps. Each window runs golap_star_run on the window's columns; the groups of all windows are
+ * added into one dense result and returned as golap_star_run returns them (kernel_ms sums the
+ * windows'). Chunks whose host_chunk_mask flag (optional) is 0 are never read. */
+typedef struct {
+    const char *fk_paths[GOLAP_STAR_MAX_DIMS];
+    const golap_hashmap *maps[GOLAP_STAR_MAX_DIMS];
+    int32_t group_lo[GOLAP_STAR_MAX_DIMS];
+    uint32_t group_card[GOLAP_STAR_MAX_DIMS];  /* 0: the dimension only filters */
+    uint32_t n_dims;
+    const char *value_path;
+    const uint16_t *host_chunk_mask;
+    uint64_t host_chunk_mask_len;
+    uint32_t window_chunks;
+    uint32_t agg_mode;
+} golap_star_file_plan;
+int golap_star_run_files(golap_ctx *ctx, const golap_star_file_plan *plan, golap_star_result *res,
+                         golap_stream_stats *stats, int32_t *group_keys, int64_t *sums, uint64_t *counts,
+                         uint64_t max_groups);
+
 /* ------------------------------------------------------------------ CPU/GPU co-execution
  * HostHashMap (core/host_structs.hpp:9-78): a host-side table over a caller-owned int32 key
  * column (which must stay valid until the build returns). build inserts the rows of

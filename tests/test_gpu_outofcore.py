@@ -143,3 +143,62 @@ def test_streamed_q1_throughput(orc, golap, ctx, tmp_path):
         ctx.free_hashmap(m)
         for b in bufs:
             ctx.free(b)
+
+
+@pytest.mark.parametrize("window", [1, 5, 0])
+def test_streamed_star_equals_resident(orc, golap, ctx, tmp_path, window):
+    """Star plans (query2/query3) streamed from column files in windows: the same groups as
+    golap_star_run over the resident columns and the oracle; a host mask prunes before reading."""
+    from test_gpu_star import build_map, star_data
+    D = star_data(golap, 1_500_000, seed=41)
+    enc = dict(page_rows=4096, chunk_rows=65536, lz4_min_match=4)
+    names = ("date", "cust", "supp", "part", "rev")
+    hosts = {n: golap.HostColumn.encode(D[n], codec=2 if n == "date" else 1, **enc) for n in names}
+    cols = {n: ctx.upload(h) for n, h in hosts.items()}
+    paths = {}
+    for n, h in hosts.items():
+        paths[n] = tmp_path / f"star_{n}.gcol"
+        h.save(paths[n])
+    maps, bufs = [], []
+    for key, n_dim, preds, pay in ((D["d_key"], 2556, [], D["d_year"]),
+                                   (D["c_key"], D["c_key"].size, [(D["c_nation"], 4, 9)], None),
+                                   (D["s_key"], D["s_key"].size, [(D["s_region"], 1, 1)], None),
+                                   (D["p_key"], D["p_key"].size, [(D["p_brand"], 1, 400)], D["p_brand"])):
+        m, b = build_map(ctx, golap, key, n_dim, preds, pay)
+        maps.append(m)
+        bufs += b
+    try:
+        groups = [(1992, 7), (0, 0), (0, 0), (1, 1000)]
+        dims = [(cols[n], maps[i], *groups[i]) for i, n in enumerate(names[:4])]
+        want = ctx.star_run(dims, cols["rev"])
+        fdims = [(paths[n], maps[i], *groups[i]) for i, n in enumerate(names[:4])]
+        got = ctx.star_run_files(fdims, paths["rev"], window_chunks=window)
+        say("streamed star window", window, got[4].as_dict())
+        for a, b in zip(got[1:4], want[1:4]):
+            assert np.array_equal(a, b)
+        assert (got[0].rows_matched, got[0].total_sum, got[0].chunks_run, got[0].comp_bytes) == \
+            (want[0].rows_matched, want[0].total_sum, want[0].chunks_run, want[0].comp_bytes)
+        wk, ws, wc = orc.star_groupby(
+            [(D["d_key"], [], D["d_year"]), (D["c_key"], [(D["c_nation"], 4, 9)], None),
+             (D["s_key"], [(D["s_region"], 1, 1)], None), (D["p_key"], [(D["p_brand"], 1, 400)], D["p_brand"])],
+            [D["date"], D["cust"], D["supp"], D["part"]], D["rev"], threads=4)
+        assert np.array_equal(got[1], wk) and np.array_equal(got[2], ws) and np.array_equal(got[3], wc)
+        # a host mask: the streamed plan equals the resident plan with the same device mask
+        nch = hosts["rev"].arrays()["desc"].n_chunks
+        hm = (np.arange(nch) % 3 == 0).astype(np.uint16)
+        dm = ctx.to_device(hm)
+        try:
+            wantm = ctx.star_run(dims, cols["rev"], dev_chunk_mask=dm)
+        finally:
+            ctx.free(dm)
+        gotm = ctx.star_run_files(fdims, paths["rev"], host_chunk_mask=hm, window_chunks=window)
+        for a, b in zip(gotm[1:4], wantm[1:4]):
+            assert np.array_equal(a, b)
+        assert gotm[4].bytes_read < got[4].bytes_read
+    finally:
+        for c in cols.values():
+            ctx.free_column(c)
+        for m in maps:
+            ctx.free_hashmap(m)
+        for b in bufs:
+            ctx.free(b)
