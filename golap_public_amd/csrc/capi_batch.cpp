@@ -1033,17 +1033,18 @@ int golap_batch_free(golap_batch *b) {
 
 namespace golap_impl {
 
-int solo_worklist(golap_ctx *ctx, const void *desc, uint32_t desc_size, uint32_t pages_per_chunk, const void **d_desc,
+int solo_worklist(golap_ctx *ctx, const void *desc, uint32_t desc_size, uint32_t parts, const void **d_desc,
                   uint32_t **queue, uint2 **items) {
     if (desc_size > 1024) return fail(ctx, GOLAP_EINVAL, "solo work list: descriptor too large");
     const SliceDev &s = *static_cast<const SliceDev *>(desc);
-    const uint64_t nch = s.chunk_end - s.chunk_begin;
-    if (nch > ctx->solo_items_cap) {
+    if (!parts || parts > 256) return fail(ctx, GOLAP_EINVAL, "solo work list: parts per chunk outside 1..256");
+    const uint64_t nch = s.chunk_end - s.chunk_begin, n_items = nch * parts;
+    if (n_items > ctx->solo_items_cap) {
         dev_release(ctx, ctx->d_solo_items);
         ctx->d_solo_items = nullptr;
         ctx->solo_items_cap = 0;
-        if (int r = dev_alloc(ctx, std::max<uint64_t>(nch, 1024) * sizeof(uint2), &ctx->d_solo_items)) return r;
-        ctx->solo_items_cap = std::max<uint64_t>(nch, 1024);
+        if (int r = dev_alloc(ctx, std::max<uint64_t>(n_items, 1024) * sizeof(uint2), &ctx->d_solo_items)) return r;
+        ctx->solo_items_cap = std::max<uint64_t>(n_items, 1024);
     }
     uint8_t *dd = ctx->d_scratch + 1024;
     uint64_t *off = reinterpret_cast<uint64_t *>(ctx->d_scratch + 2048);
@@ -1061,8 +1062,7 @@ int solo_worklist(golap_ctx *ctx, const void *desc, uint32_t desc_size, uint32_t
     GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(off, ctx->h_stage + 1024, sizeof(host_off), hipMemcpyHostToDevice, ctx->stream));
     GOLAP_HIP_CHECK(ctx, launch_fill_zero(q, 8, ctx->stream));
     uint2 *it = static_cast<uint2 *>(ctx->d_solo_items);
-    GOLAP_HIP_CHECK(ctx, launch_batch_items(dd, desc_size, 1, off, nch, 1, it, q, ctx->stream));
-    (void)pages_per_chunk;
+    GOLAP_HIP_CHECK(ctx, launch_batch_items(dd, desc_size, 1, off, nch, parts, it, q, ctx->stream));
     *d_desc = dd;
     *queue = q;
     *items = it;

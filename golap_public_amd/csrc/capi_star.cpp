@@ -421,11 +421,16 @@ int golap_star_run(golap_ctx *ctx, const golap_star_plan *plan, golap_star_resul
     // with a chunk mask, the surviving chunks may sit anywhere in the slice: a device-built work
     // list keeps every workgroup busy (the static page ranges would leave most of them idle)
     if (plan->dev_chunk_mask && L.n_items && dm->filled) {
+        // a slice of few chunks is cut into page ranges, so that its pages spread over about four
+        // workgroups per CU instead of queueing chunk after chunk on a few of them
+        const uint64_t nch = std::max<uint64_t>(1, qd.s.chunk_end - qd.s.chunk_begin);
+        const uint32_t parts = (uint32_t)std::min<uint64_t>(
+            {(uint64_t)L.pages_per_chunk, 256, std::max<uint64_t>(1, ((uint64_t)ctx->cus * 4 + nch - 1) / nch)});
         const void *dq;
-        if (int r = solo_worklist(ctx, &qd, sizeof(qd), L.pages_per_chunk, &dq, &L.queue, const_cast<uint2 **>(&L.items)))
+        if (int r = solo_worklist(ctx, &qd, sizeof(qd), parts, &dq, &L.queue, const_cast<uint2 **>(&L.items)))
             return r;
         L.queries = static_cast<const StarQueryDev *>(dq);
-        L.pages_per_item = L.pages_per_chunk;
+        L.pages_per_item = (L.pages_per_chunk + parts - 1) / parts;
     } else {
         L.q0 = qd;
     }

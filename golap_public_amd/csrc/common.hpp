@@ -277,9 +277,9 @@ hipError_t launch_read_probe(golap_ctx *ctx, const void *p, uint64_t bytes, uint
 
 // Work list of one pruned query (a batch of one): its descriptor (desc_size <= 1024 bytes,
 // beginning with its SliceDev) goes to the context scratch, the surviving chunks' items to a
-// context buffer. On return *d_desc, *queue and *items are the device pointers for the queue
-// schedule with pages_per_item = pages per chunk.
-int solo_worklist(golap_ctx *ctx, const void *desc, uint32_t desc_size, uint32_t pages_per_chunk, const void **d_desc,
+// context buffer, `parts` items per chunk (1..256; pages_per_item = pages per chunk / parts,
+// rounded up). On return *d_desc, *queue and *items are the device pointers for the queue schedule.
+int solo_worklist(golap_ctx *ctx, const void *desc, uint32_t desc_size, uint32_t parts, const void **d_desc,
                   uint32_t **queue, uint2 **items);
 size_t star_lds_bytes(uint32_t filter_words, uint32_t page_rows, uint32_t n_windows, uint32_t lds_gslots,
                       uint32_t n_anchors = 0);

@@ -71,6 +71,7 @@ struct Window {
     std::vector<Run> runs;    // maximal ranges of unpruned chunks
     uint64_t base[MAXC];      // 4 KiB-aligned start of the window's pages, relative to the data region
     uint64_t bytes[MAXC];     // end of the window's pages - base
+    bool all_kept;            // every chunk of the window is scanned: the scan gets no chunk mask
 };
 
 struct ColumnFile {
@@ -147,7 +148,8 @@ int open_files(golap_ctx *ctx, const char *const *paths, int n, FileSet &fs, con
 }
 
 // Called per window with the window's device columns (one per file, catalogs rebased to the
-// window), its chunk count and its device chunk mask; runs the plan's scan synchronously.
+// window), its chunk count and its device chunk mask (null when every chunk of the window is kept);
+// runs the plan's scan synchronously.
 using WindowScan = std::function<int(golap_column *const *cols, uint64_t n_chunks, const uint16_t *dmask)>;
 
 // Streams the chunks with keep[ch] set through `scan` in windows of W chunks (0: about 64 MiB of
@@ -178,6 +180,7 @@ int stream_windows(golap_ctx *ctx, FileSet &fs, const std::vector<uint8_t> &keep
             if (!w.runs.empty() && w.runs.back().c1 == ch) w.runs.back().c1 = ch + 1;
             else w.runs.push_back({ch, ch + 1});
         }
+        w.all_kept = w.runs.size() == 1 && w.runs[0].c0 == w.c0 && w.runs[0].c1 == w.c1;
         if (w.runs.empty()) {
             st.windows_skipped++;
             *skipped_chunks += w.c1 - w.c0;
@@ -320,7 +323,7 @@ int stream_windows(golap_ctx *ctx, FileSet &fs, const std::vector<uint8_t> &keep
     auto run_scan = [&](uint64_t i) -> int {
         const uint64_t s = i % SLOTS;
         GOLAP_HIP_CHECK(ctx, hipStreamWaitEvent(ctx->stream, copied[s], 0));
-        if (int e = scan(dcol[s], wins[i].c1 - wins[i].c0, dmask[s])) return e;
+        if (int e = scan(dcol[s], wins[i].c1 - wins[i].c0, wins[i].all_kept ? nullptr : dmask[s])) return e;
         st.windows++;
         return GOLAP_OK;
     };
