@@ -33,20 +33,40 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose=False):
-    """Compile every source for gfx950 and link the shared library (always recompiles)."""
-    out_tmp = LIB + ".tmp"
-    cmd = [hipcc()] + HIPCC_FLAGS + ["-shared", "-o", out_tmp, "-x", "hip"]
-    cmd += [os.path.join(CSRC, s) for s in HIP_SOURCES]
+def _compile(src, obj, verbose):
+    cmd = [hipcc()] + HIPCC_FLAGS + ["-c", "-o", obj, "-x", "hip", src]
     if verbose:
         print(" ".join(cmd), flush=True)
     proc = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     # the host compilation pass does not know the gfx950 feature flag and says so; harmless
     lines = [l for l in proc.stdout.splitlines() if "packed-fp32-ops" not in l]
-    if lines:
-        print("\n".join(lines), flush=True)
-    if proc.returncode != 0:
-        raise subprocess.CalledProcessError(proc.returncode, cmd)
+    return proc.returncode, cmd, "\n".join(lines)
+
+
+def build(verbose=False, jobs=None):
+    """Compile every source for gfx950 (one object per source, in parallel) and link the shared
+    library. Always recompiles."""
+    from concurrent.futures import ThreadPoolExecutor
+    objdir = os.path.join(os.path.dirname(HERE), "_build", "obj")
+    os.makedirs(objdir, exist_ok=True)
+    jobs = jobs or min(len(HIP_SOURCES), max(1, min(16, os.cpu_count() or 1)))
+    objs = [os.path.join(objdir, os.path.splitext(s)[0] + ".o") for s in HIP_SOURCES]
+    with ThreadPoolExecutor(jobs) as ex:
+        results = list(ex.map(lambda so: _compile(os.path.join(CSRC, so[0]), so[1], verbose),
+                              zip(HIP_SOURCES, objs)))
+    failed = None
+    for rc, cmd, out in results:
+        if out:
+            print(out, flush=True)
+        if rc != 0 and failed is None:
+            failed = (rc, cmd)
+    if failed:
+        raise subprocess.CalledProcessError(*failed)
+    out_tmp = LIB + ".tmp"
+    cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out_tmp] + objs
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
     os.replace(out_tmp, LIB)
     return LIB
 

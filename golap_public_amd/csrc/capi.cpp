@@ -14,9 +14,6 @@ using Lock = std::lock_guard<std::recursive_mutex>;
 
 namespace golap_impl {
 
-std::mutex g_extra_mu;
-std::unordered_map<const golap_column *, ColumnExtra> g_extra;
-
 bool page_direct_host(uint8_t codec) {
     return (codec & golap_fmt::PAGE_CODEC_MASK) == golap_fmt::RAW ||
            (codec & golap_fmt::PAGE_LITERAL_ONLY);
@@ -31,6 +28,7 @@ int dev_alloc(golap_ctx *ctx, uint64_t bytes, void **p) {
         (void)hipGetLastError();
         return fail(ctx, GOLAP_ENOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
     }
+    std::lock_guard<std::mutex> g(ctx->alloc_mu);
     ctx->allocated += bytes;
     ctx->allocs[*p] = bytes;
     return GOLAP_OK;
@@ -38,10 +36,13 @@ int dev_alloc(golap_ctx *ctx, uint64_t bytes, void **p) {
 
 void dev_release(golap_ctx *ctx, void *p) {
     if (!p) return;
-    auto it = ctx->allocs.find(p);
-    if (it != ctx->allocs.end()) {
-        ctx->allocated -= it->second;
-        ctx->allocs.erase(it);
+    {
+        std::lock_guard<std::mutex> g(ctx->alloc_mu);
+        auto it = ctx->allocs.find(p);
+        if (it != ctx->allocs.end()) {
+            ctx->allocated -= it->second;
+            ctx->allocs.erase(it);
+        }
     }
     (void)hipFree(p);
 }
@@ -136,7 +137,14 @@ int golap_ctx_destroy(golap_ctx *ctx) {
     return GOLAP_OK;
 }
 
-const char *golap_last_error(golap_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+// The message is copied into a per-thread buffer, so it stays valid while other threads of the
+// same context report errors.
+const char *golap_last_error(golap_ctx *ctx) {
+    if (!ctx) return "null context";
+    static thread_local std::string copy;
+    copy = last_error(ctx);
+    return copy.c_str();
+}
 
 int golap_sync(golap_ctx *ctx) {
     if (!ctx) return GOLAP_EINVAL;
@@ -167,7 +175,12 @@ int golap_dev_free(golap_ctx *ctx, void *dev_ptr) {
     if (!ctx) return GOLAP_EINVAL;
     Lock l(ctx->mu);
     if (!dev_ptr) return GOLAP_OK;
-    if (!ctx->allocs.count(dev_ptr)) return fail(ctx, GOLAP_EINVAL, "dev_free: unknown pointer");
+    bool known;
+    {
+        std::lock_guard<std::mutex> g(ctx->alloc_mu);
+        known = ctx->allocs.count(dev_ptr) != 0;
+    }
+    if (!known) return fail(ctx, GOLAP_EINVAL, "dev_free: unknown pointer");
     (void)hipStreamSynchronize(ctx->stream);
     dev_release(ctx, dev_ptr);
     return GOLAP_OK;
@@ -207,7 +220,11 @@ int golap_d2d(golap_ctx *ctx, void *dst, const void *src, uint64_t bytes) {
     GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     return GOLAP_OK;
 }
-uint64_t golap_device_allocated(golap_ctx *ctx) { return ctx ? ctx->allocated : 0; }
+uint64_t golap_device_allocated(golap_ctx *ctx) {
+    if (!ctx) return 0;
+    std::lock_guard<std::mutex> g(ctx->alloc_mu);
+    return ctx->allocated;
+}
 
 int golap_read_probe(golap_ctx *ctx, const void *dev_buf, uint64_t bytes, uint32_t blocks_per_cu, double *ms) {
     if (!ctx || !dev_buf || !ms || !bytes) return GOLAP_EINVAL;
@@ -380,7 +397,7 @@ int golap_impl::create_device_column(golap_ctx *ctx, const golap_host_column *h,
 }
 
 // The planning tables of a column from its host catalog (page kinds, payload per chunk).
-void golap_impl::register_extra(const golap_column *c) {
+void golap_impl::register_extra(golap_column *c) {
     ColumnExtra ex;
     ex.nondirect_prefix.resize(c->n_pages + 1, 0);
     ex.window_prefix.resize(c->n_pages + 1, 0);
@@ -400,8 +417,7 @@ void golap_impl::register_extra(const golap_column *c) {
         for (uint64_t i = ch * ppc; i < std::min<uint64_t>(c->n_pages, (ch + 1) * ppc); ++i) s += c->page_csize[i];
         ex.chunk_payload[ch + 1] = ex.chunk_payload[ch] + s;
     }
-    std::lock_guard<std::mutex> g(g_extra_mu);
-    g_extra[c] = std::move(ex);
+    c->extra = std::move(ex);
 }
 
 int golap_impl::set_column_window(golap_ctx *ctx, golap_column *c, const golap_host_column &w, uint64_t comp_bytes,
@@ -443,10 +459,6 @@ int golap_column_free(golap_ctx *ctx, golap_column *c) {
     dev_release(ctx, c->d_page_codec);
     dev_release(ctx, c->d_chunk_min);
     dev_release(ctx, c->d_chunk_max);
-    {
-        std::lock_guard<std::mutex> g(g_extra_mu);
-        g_extra.erase(c);
-    }
     delete c;
     return GOLAP_OK;
 }
@@ -771,9 +783,8 @@ int golap_q1_run(golap_ctx *ctx, const golap_q1_plan *plan, golap_q1_result *res
 
     bool any_general = false;
     {
-        std::lock_guard<std::mutex> g(g_extra_mu);
         for (auto *c : cols) {
-            auto &ex = g_extra[c];
+            auto &ex = c->extra;
             if (ex.nondirect_prefix[a.page_end] != ex.nondirect_prefix[a.page_begin]) any_general = true;
         }
     }
@@ -801,7 +812,6 @@ int golap_q1_run(golap_ctx *ctx, const golap_q1_plan *plan, golap_q1_result *res
     res->matches = host[2];
     res->kernel_ms = ms;
     {
-        std::lock_guard<std::mutex> g(g_extra_mu);
         for (uint64_t ch = plan->chunk_begin; ch < plan->chunk_end; ++ch) {
             if (!mask.empty() && !mask[ch]) {
                 res->chunks_pruned++;
@@ -809,7 +819,7 @@ int golap_q1_run(golap_ctx *ctx, const golap_q1_plan *plan, golap_q1_result *res
             }
             res->chunks_run++;
             for (auto *c : cols) {
-                auto &ex = g_extra[c];
+                auto &ex = c->extra;
                 res->comp_bytes += ex.chunk_payload[ch + 1] - ex.chunk_payload[ch];
             }
         }

@@ -386,7 +386,6 @@ int finish_create(golap_ctx *ctx, golap_batch *b, bool with_graph) {
         }
         b->union_pairs = 0;
         b->shared_bytes = 0;
-        std::lock_guard<std::mutex> g(g_extra_mu);
         for (uint64_t k = 0; k < un; ++k)
             for (uint32_t w = 0; w < nw; ++w) {
                 const uint32_t m = cmask[k * nw + w];
@@ -394,7 +393,7 @@ int finish_create(golap_ctx *ctx, golap_batch *b, bool with_graph) {
                 ++b->union_pairs;
                 for (size_t c = 0; c < b->cs.cols.size(); ++c)
                     if ((m >> c) & 1u) {
-                        const auto &ex = g_extra[b->cs.cols[c]];
+                        const auto &ex = b->cs.cols[c]->extra;
                         const uint64_t ch = b->union_begin + k;
                         b->shared_bytes += ex.chunk_payload[ch + 1] - ex.chunk_payload[ch];
                     }
@@ -471,7 +470,7 @@ int BatchSlot::reserve(uint64_t arena, uint64_t qbytes, uint64_t offs, uint64_t 
         cap = 0;
         void *q = nullptr;
         if (int r = dev_alloc(ctx, want * unit, &q)) {
-            err = ctx->err;
+            err = last_error(ctx);
             return r;
         }
         p = static_cast<std::remove_reference_t<decltype(p)>>(q);

@@ -104,11 +104,10 @@ std::vector<uint8_t> scan_survivors(const golap_scan_plan &p, const std::vector<
 }
 
 void scan_stats(const golap_scan_plan &p, uint32_t colmask, const std::vector<uint8_t> &run, golap_star_result *res) {
-    std::lock_guard<std::mutex> g(g_extra_mu);
     // per-chunk payload prefixes of the columns read, looked up once
     std::vector<const uint64_t *> pf;
     for (uint32_t c = 0; c < p.n_cols; ++c)
-        if ((colmask >> c) & 1u) pf.push_back(g_extra[p.cols[c]].chunk_payload.data());
+        if ((colmask >> c) & 1u) pf.push_back(p.cols[c]->extra.chunk_payload.data());
     for (uint64_t k = 0; k < run.size(); ++k) {
         const uint64_t ch = p.chunk_begin + k;
         if (!run[k]) {
@@ -126,13 +125,12 @@ void scan_stats(const golap_scan_plan &p, uint32_t colmask, const std::vector<ui
 // coded pages, or delta pages without a slot. Returns the window bytes.
 uint32_t scan_layout(const golap_column *const *cols, uint32_t n_cols, uint32_t colmask, uint64_t pb, uint64_t pe,
                      int32_t *woff, int32_t *dslot) {
-    std::lock_guard<std::mutex> g(g_extra_mu);
     uint32_t bytes = 0, slots = 0;
     for (uint32_t c = 0; c < n_cols; ++c) {
         woff[c] = -1;
         dslot[c] = -1;
         if (!((colmask >> c) & 1u)) continue;
-        auto &ex = g_extra[cols[c]];
+        auto &ex = cols[c]->extra;
         const uint32_t dbp = ex.dbp_prefix[pe] - ex.dbp_prefix[pb];
         uint32_t other = ex.window_prefix[pe] - ex.window_prefix[pb] - dbp;
         if (dbp) {

@@ -40,6 +40,7 @@ struct Request {
     std::vector<const golap_column *> cols;
     Clock::time_point t_submit, t_done;
     bool done = false;
+    bool claimed = false;  // a waiter owns it: a second wait on the ticket is refused
     int status = GOLAP_OK;
     std::string err;
     golap_star_result res{};
@@ -163,7 +164,7 @@ int launch(golap_server *s, Slot &slot, const std::vector<Request *> &reqs, gola
     golap_ctx *ctx = s->ctx;
     std::lock_guard<std::recursive_mutex> l(ctx->mu);
     if (int r = set_device(ctx)) {
-        err = ctx->err;
+        err = last_error(ctx);
         return r;
     }
     golap_batch *b = nullptr;
@@ -178,7 +179,7 @@ int launch(golap_server *s, Slot &slot, const std::vector<Request *> &reqs, gola
         r = batch_create_scan(ctx, q.data(), (uint32_t)q.size(), &slot.bs, false, &b);
     }
     if (r) {
-        err = ctx->err;
+        err = last_error(ctx);
         return r;
     }
     if ((r = batch_enqueue(b, false))) {
@@ -452,10 +453,12 @@ int golap_server_wait(golap_server *s, uint64_t ticket, golap_star_result *res, 
     {
         std::unique_lock<std::mutex> lk(s->m);
         auto it = s->all.find(ticket);
-        if (it == s->all.end()) return fail(s->ctx, GOLAP_EINVAL, "server: unknown ticket");
+        if (it == s->all.end() || it->second->claimed)
+            return fail(s->ctx, GOLAP_EINVAL, "server: unknown ticket or one already waited on");
         Request *x = it->second;
+        x->claimed = true;
         s->cv_done.wait(lk, [&] { return x->done; });
-        s->all.erase(it);
+        s->all.erase(ticket);  // looked up again: other tickets may have been erased meanwhile
         r.reset(x);
     }
     if (latency_ms) *latency_ms = std::chrono::duration<double, std::milli>(r->t_done - r->t_submit).count();

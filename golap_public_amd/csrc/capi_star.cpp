@@ -242,10 +242,9 @@ int prepare_query(golap_ctx *ctx, const golap_star_plan *plan, bool compact, Col
 // LDS windows: a column gets one when some page of [pb, pe) is neither read in place nor
 // point-readable.
 uint32_t assign_windows(const ColumnSet &cs, uint64_t pb, uint64_t pe, int *win) {
-    std::lock_guard<std::mutex> g(g_extra_mu);
     uint32_t n = 0;
     for (size_t c = 0; c < cs.cols.size(); ++c) {
-        auto &ex = g_extra[cs.cols[c]];
+        auto &ex = cs.cols[c]->extra;
         win[c] = ex.window_prefix[pe] != ex.window_prefix[pb] ? (int)n++ : -1;
     }
     return n;
@@ -258,10 +257,9 @@ uint32_t assign_windows(const ColumnSet &cs, uint64_t pb, uint64_t pe, int *win)
 // per-tile prefix sums (anchor[c] = 1 + region). Returns the windows.
 uint32_t assign_access(const ColumnSet &cs, uint64_t pb, uint64_t pe, uint32_t driver_cols, int *win, int *anchor,
                        uint32_t *n_anchors) {
-    std::lock_guard<std::mutex> g(g_extra_mu);
     uint32_t n = 0, na = 0;
     for (size_t c = 0; c < cs.cols.size(); ++c) {
-        auto &ex = g_extra[cs.cols[c]];
+        auto &ex = cs.cols[c]->extra;
         const uint32_t coded = ex.window_prefix[pe] - ex.window_prefix[pb];
         const uint32_t dbp = ex.dbp_prefix[pe] - ex.dbp_prefix[pb];
         const bool needs_window = coded > dbp || (dbp && ((driver_cols >> c) & 1u));
@@ -275,12 +273,11 @@ uint32_t assign_access(const ColumnSet &cs, uint64_t pb, uint64_t pe, uint32_t d
 // chunks_run / chunks_pruned / comp_bytes of a plan over its slice, given the chunk flags
 // (empty = all run).
 void slice_stats(const golap_star_plan &plan, const std::vector<uint8_t> &run, golap_star_result *res) {
-    std::lock_guard<std::mutex> g(g_extra_mu);
     // per-chunk payload prefixes of the plan's columns, looked up once
     const uint64_t *pf[STAR_MAX_DIMS + 1];
     uint32_t n = 0;
-    for (uint32_t d = 0; d < plan.n_dims; ++d) pf[n++] = g_extra[plan.dims[d].fk].chunk_payload.data();
-    pf[n++] = g_extra[plan.value].chunk_payload.data();
+    for (uint32_t d = 0; d < plan.n_dims; ++d) pf[n++] = plan.dims[d].fk->extra.chunk_payload.data();
+    pf[n++] = plan.value->extra.chunk_payload.data();
     for (uint64_t ch = plan.chunk_begin; ch < plan.chunk_end; ++ch) {
         if (!run.empty() && !run[ch - plan.chunk_begin]) {
             res->chunks_pruned++;

@@ -22,8 +22,13 @@ struct golap_ctx {
     char name[128] = {0};
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // Serialises the calls of one context (its stream and scratch). The allocation registry and
+    // the error string have locks of their own: the serving loop's completion threads free batch
+    // buffers and report errors while the scheduler holds mu to launch the next batch.
     std::recursive_mutex mu;
+    std::mutex err_mu;
     std::string err;
+    std::mutex alloc_mu;
     // Device scratch: word 0 error flags; [8,16) int64 sum; [16,24) u64 count; [64,80) hash
     // build; [1024,2048) a single query's descriptor, [2048,2064) its chunk offsets, [2304,2312)
     // its work-list counters (pruned single-query plans).
@@ -45,6 +50,15 @@ struct golap_ctx {
     uint8_t *h_stage = nullptr;  // pinned host staging for small uploads (4 KiB)
 };
 
+// Host planning tables of a device column, from its host catalog: launches are planned from
+// them without a device round trip.
+struct ColumnExtra {
+    std::vector<uint32_t> nondirect_prefix;  // pages that cannot be read in place, prefix count
+    std::vector<uint32_t> window_prefix;     // pages neither in place nor point-readable
+    std::vector<uint32_t> dbp_prefix;        // delta-bitpack pages that are not point-readable
+    std::vector<uint64_t> chunk_payload;     // payload bytes per chunk, prefix
+};
+
 struct golap_column {
     golap_ctx *ctx = nullptr;
     uint32_t type = 0, elem = 0, page_rows = 0, chunk_rows = 0;
@@ -60,6 +74,8 @@ struct golap_column {
     uint32_t *d_page_csize = nullptr;
     uint8_t *d_page_codec = nullptr;
     int64_t *d_chunk_min = nullptr, *d_chunk_max = nullptr;
+    // planning tables, rebuilt whenever the catalog changes (column creation, stream windows)
+    ColumnExtra extra;
 };
 
 // Open addressing over 8-byte slots: slot = (uint64(payload) << 32) | (uint32(key) ^ 0x80000000).
@@ -95,8 +111,16 @@ struct golap_hashmap {
 namespace golap_impl {
 
 inline int fail(golap_ctx *ctx, int code, const std::string &msg) {
-    if (ctx) ctx->err = msg;
+    if (ctx) {
+        std::lock_guard<std::mutex> g(ctx->err_mu);
+        ctx->err = msg;
+    }
     return code;
+}
+// A copy of the context's last error message, taken under its lock.
+inline std::string last_error(golap_ctx *ctx) {
+    std::lock_guard<std::mutex> g(ctx->err_mu);
+    return ctx->err;
 }
 
 #define GOLAP_HIP_CHECK(ctx, expr)                                                          \
@@ -108,14 +132,6 @@ inline int fail(golap_ctx *ctx, int code, const std::string &msg) {
     } while (0)
 
 // Runtime helpers shared by the C ABI translation units (capi.cpp).
-struct ColumnExtra {
-    std::vector<uint32_t> nondirect_prefix;  // pages that cannot be read in place, prefix count
-    std::vector<uint32_t> window_prefix;     // pages neither in place nor point-readable
-    std::vector<uint32_t> dbp_prefix;        // delta-bitpack pages that are not point-readable
-    std::vector<uint64_t> chunk_payload;     // payload bytes per chunk, prefix
-};
-extern std::mutex g_extra_mu;
-extern std::unordered_map<const golap_column *, ColumnExtra> g_extra;
 bool page_direct_host(uint8_t codec);
 int dev_alloc(golap_ctx *ctx, uint64_t bytes, void **p);
 // A device column from a host column's catalog and zone maps, with comp_bytes of page data
@@ -124,7 +140,7 @@ int dev_alloc(golap_ctx *ctx, uint64_t bytes, void **p);
 int create_device_column(golap_ctx *ctx, const golap_host_column *h, uint64_t comp_bytes, const uint8_t *comp,
                          golap_column **out);
 // The host planning tables of a column, from its host catalog.
-void register_extra(const golap_column *c);
+void register_extra(golap_column *c);
 // Points device column c (created with catalog capacity for at least w's pages and chunks) at a
 // window w of a larger column: catalog and zone maps copied (synchronously, on stream), planning
 // tables rebuilt; the caller fills d_comp with comp_bytes of page data.

@@ -124,3 +124,41 @@ def test_server_star_and_scan_mixed(golap, ctx, trip):
                 assert same(g, star_alone[i] if kind == "star" else scan_alone[i]), (kind, i)
     finally:
         fx.free()
+
+
+def test_server_star_batches_with_concurrent_clients(golap, ctx, trip):
+    """Two slots serve multi-query star batches (each batch frees its shared-column filter on the
+    completion thread) while four client threads keep submitting: the frees race with the
+    scheduler's next launch unless the context's allocation registry and error string are
+    locked. Every result equals its single-query run, and a second wait on a ticket is refused."""
+    fx = Fixture(golap, ctx, 1_000_000, seed=9)
+    try:
+        qs = make_queries(np.random.default_rng(36), 40, fx.nch)
+        flags = ctx.alloc(fx.nch * 2)
+        alone = [fx.single(q, flags) for q in qs]
+        ctx.free(flags)
+        entries = [fx.batch_entry(q) for q in qs]
+        with ctx.server(slots=2, max_batch=8, emit_threads=4) as srv:
+            got = [None] * len(qs)
+
+            def client(k):  # each client submits its share, then waits on it
+                mine = [(i, srv.submit_star(**entries[i])) for i in range(k, len(qs), 4)]
+                for i, t in mine:
+                    got[i] = srv.wait(t)
+
+            th = [threading.Thread(target=client, args=(k,)) for k in range(4)]
+            for t in th:
+                t.start()
+            for t in th:
+                t.join()
+            st = srv.stats()
+            t = srv.submit_star(**entries[0])
+            assert same(srv.wait(t), alone[0])
+            with pytest.raises(golap.GolapError):
+                srv.wait(t)
+        say("star server stats", st)
+        for i in range(len(qs)):
+            assert same(got[i], alone[i]), i
+        assert st["completed"] == st["submitted"] == len(qs) and st["mean_batch"] > 1
+    finally:
+        fx.free()
