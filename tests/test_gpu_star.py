@@ -89,6 +89,49 @@ def test_star_matches_oracle(orc, golap, ctx, q, codec, mm, date_codec, mode):
             ctx.free(b)
 
 
+@pytest.mark.parametrize("mm", [4, 16])
+def test_star_config3_dimension_sizes_match_oracle(orc, golap, ctx, mm):
+    """The star plan at its real dimension sizes: customer 2,000,000, supplier 200,000, part
+    1,400,000 rows (key ranges above the exact driver bitmap's 131,072, so the driver filter is the
+    blocked Bloom filter) and predicates at 1/5 (supplier region), 1/25 (customer nation) and
+    1/1000 (part brand), over 24M fact rows coded as the star workload codes them (date_key
+    bit-packed delta, keys and revenue LZ4). Groups, sums and counts equal the oracle's CPU plan."""
+    n = 24_000_000
+    D = star_data(golap, n, seed=23, n_cust=2_000_000, n_supp=200_000, n_part=1_400_000)
+    say("star config-3 dims, rows", n, "min match", mm)
+    enc = dict(page_rows=4096, chunk_rows=262144, lz4_min_match=mm)
+    cols = {nm: ctx.upload(golap.HostColumn.encode(D[nm], codec=2 if nm == "date" else 1, **enc))
+            for nm in ("date", "cust", "supp", "part", "rev")}
+    maps, bufs = [], []
+    specs = ((D["d_key"], 2556, [], D["d_year"]),
+             (D["c_key"], D["c_key"].size, [(D["c_nation"], 7, 7)], None),
+             (D["s_key"], D["s_key"].size, [(D["s_region"], 1, 1)], None),
+             (D["p_key"], D["p_key"].size, [(D["p_brand"], 42, 42)], D["p_brand"]))
+    for key, n_dim, preds, pay in specs:
+        m, b = build_map(ctx, golap, key, n_dim, preds, pay)
+        maps.append(m)
+        bufs += b
+    try:
+        dims = [(cols["date"], maps[0], 1992, 7), (cols["cust"], maps[1], 0, 0), (cols["supp"], maps[2], 0, 0),
+                (cols["part"], maps[3], 1, 1000)]
+        wk, ws, wc = orc.star_groupby([(k, p, pay) for k, _, p, pay in specs],
+                                      [D["date"], D["cust"], D["supp"], D["part"]], D["rev"], threads=8)
+        assert wk.shape[0] > 0
+        for mode in (0, 1, 2):
+            r, k, s, c = ctx.star_run(dims, cols["rev"], agg_mode=mode)
+            say("mode", mode, "groups", r.n_groups, "rows", r.rows_matched, "driver", r.driver_dim,
+                "kernel ms", round(r.kernel_ms, 3))
+            assert np.array_equal(k, wk) and np.array_equal(s, ws) and np.array_equal(c, wc), mode
+            assert r.driver_dim == 3  # part brand (1/1000) drives through the Bloom filter
+    finally:
+        for c_ in cols.values():
+            ctx.free_column(c_)
+        for m in maps:
+            ctx.free_hashmap(m)
+        for b in bufs:
+            ctx.free(b)
+
+
 def test_star_with_zone_map_pruning(orc, golap, ctx):
     """Date-sorted fact: pruning chunks by a date range gives the result of the date predicate."""
     n = 3_000_000
@@ -134,7 +177,7 @@ def ssb10(orc):
 
 
 @pytest.mark.parametrize("q", [0, 1, 2])
-def test_ssb_sf10_q2_reproduces_reference_log(golap, ctx, ssb10, q):
+def test_ssb_sf10_q2_reproduces_reference_log(orc, golap, ctx, ssb10, q):
     """Q2.1: p_category = MFGR#12, s_region = AMERICA; Q2.2: p_brand1 in MFGR#2221..MFGR#2228,
     s_region = ASIA; Q2.3: p_brand1 = MFGR#2239, s_region = EUROPE; group by (d_year, p_brand1)."""
     lo, d, p, s = ssb10
@@ -157,6 +200,12 @@ def test_ssb_sf10_q2_reproduces_reference_log(golap, ctx, ssb10, q):
                 (cols["suppkey"], maps[2], 0, 0)]
         r, k, sums, cnt = ctx.star_run(dims, cols["revenue"])
         assert (r.total_sum, r.n_groups) == SF10_Q2[q]
+        # group by group: the oracle's CPU plan over the same tables (pinned by the same log)
+        wk, ws, wc = orc.star_groupby([(d["key"], [], d["year"]),
+                                       (np.arange(n_parts, dtype=np.int32), part_pred, p["brand1"]),
+                                       (np.arange(n_supp, dtype=np.int32), [(s["region"], region, region)], None)],
+                                      [lo["orderdate"], lo["partkey"], lo["suppkey"]], lo["revenue"], threads=8)
+        assert np.array_equal(k, wk) and np.array_equal(sums, ws) and np.array_equal(cnt, wc)
     finally:
         for c_ in cols.values():
             ctx.free_column(c_)
