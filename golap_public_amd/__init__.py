@@ -944,6 +944,8 @@ class Server:
     def wait(self, ticket):
         """(result, keys, sums, counts, latency_ms)."""
         with self._lock:
+            if ticket not in self.shape:
+                raise GolapError(EINVAL, "server_wait: unknown ticket or one already waited on")
             na, dom = self.shape.pop(ticket)
         kout = np.zeros(dom * max(na, 1), np.int32)
         sums = np.zeros(dom, np.int64)

@@ -291,7 +291,7 @@ int golap_host_column_assemble(uint32_t type, uint64_t n_rows, uint32_t page_row
     std::vector<const uint8_t *> ptrs(c->n_pages);
     for (uint64_t p = 0; p < c->n_pages; ++p) {
         uint8_t cd = codec[p] & golap_fmt::PAGE_CODEC_MASK;
-        if (cd > golap_fmt::FOR) return GOLAP_EINVAL;
+        if (cd > golap_fmt::LZ4I) return GOLAP_EINVAL;
         if (csize[p] && !pages[p]) return GOLAP_EINVAL;
         uint64_t rows = std::min<uint64_t>(page_rows, n_rows - p * page_rows);
         if (cd == golap_fmt::LZ4 && golap_host::lz4_is_literal_only(pages[p], csize[p], rows * c->elem))
@@ -392,8 +392,18 @@ int golap_impl::create_device_column(golap_ctx *ctx, const golap_host_column *h,
     c->d_chunk_min = static_cast<int64_t *>(p[4]);
     c->d_chunk_max = static_cast<int64_t *>(p[5]);
     register_extra(c.get());
+    if (comp) c->extra.lz4i_max_nseq = lz4i_max_nseq(*h, comp);
     *out = c.release();
     return GOLAP_OK;
+}
+
+// Most sequences of an LZ4I page of a host column whose page data is comp.
+uint32_t golap_impl::lz4i_max_nseq(const golap_host_column &h, const uint8_t *comp) {
+    uint32_t m = 0;
+    for (uint64_t p = 0; p < h.n_pages; ++p)
+        if ((h.page_codec[p] & golap_fmt::PAGE_CODEC_MASK) == golap_fmt::LZ4I && h.page_csize[p] >= 4)
+            m = std::max<uint32_t>(m, (uint32_t)comp[h.page_off[p]] | ((uint32_t)comp[h.page_off[p] + 1] << 8));
+    return m;
 }
 
 // The planning tables of a column from its host catalog (page kinds, payload per chunk).
@@ -408,6 +418,7 @@ void golap_impl::register_extra(golap_column *c) {
         const bool point = (c->page_codec[i] & golap_fmt::PAGE_POINT) != 0;
         ex.window_prefix[i + 1] = ex.window_prefix[i] + (direct || point ? 0 : 1);
         const bool dbp = (c->page_codec[i] & golap_fmt::PAGE_CODEC_MASK) == golap_fmt::DBP && !point;
+        ex.has_lz4i = ex.has_lz4i || (c->page_codec[i] & golap_fmt::PAGE_CODEC_MASK) == golap_fmt::LZ4I;
         ex.dbp_prefix[i + 1] = ex.dbp_prefix[i] + (dbp ? 1 : 0);
     }
     ex.chunk_payload.resize(c->n_chunks + 1, 0);
@@ -782,12 +793,16 @@ int golap_q1_run(golap_ctx *ctx, const golap_q1_plan *plan, golap_q1_result *res
     GOLAP_HIP_CHECK(ctx, hipMemsetAsync(ctx->d_scratch + 8, 0, 16, ctx->stream));
 
     bool any_general = false;
-    {
-        for (auto *c : cols) {
-            auto &ex = c->extra;
-            if (ex.nondirect_prefix[a.page_end] != ex.nondirect_prefix[a.page_begin]) any_general = true;
-        }
+    uint32_t max_nseq = 0;
+    bool nseq_known = true;
+    for (auto *c : cols) {
+        auto &ex = c->extra;
+        if (ex.nondirect_prefix[a.page_end] != ex.nondirect_prefix[a.page_begin]) any_general = true;
+        max_nseq = std::max(max_nseq, ex.lz4i_max_nseq);
+        nseq_known = nseq_known && (ex.lz4i_max_nseq || !ex.has_lz4i);
     }
+    // one window serves every column of a tile: sized for the widest (4-byte) column
+    lz4i_window(d->page_rows * 4, nseq_known ? std::max(max_nseq, 1u) : 0, a.win_cap, a.tok_cap);
     GOLAP_HIP_CHECK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     if (any && a.page_end > a.page_begin) {  // an empty predicate range or map matches nothing
         GOLAP_HIP_CHECK(ctx, launch_q1(ctx, a, true));

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <mutex>
 #include <string>
@@ -57,6 +58,10 @@ struct ColumnExtra {
     std::vector<uint32_t> window_prefix;     // pages neither in place nor point-readable
     std::vector<uint32_t> dbp_prefix;        // delta-bitpack pages that are not point-readable
     std::vector<uint64_t> chunk_payload;     // payload bytes per chunk, prefix
+    // most sequences of any LZ4I page (sizes the token arrays of parallel decodes), 0 when the
+    // page data was not seen on the host
+    uint32_t lz4i_max_nseq = 0;
+    bool has_lz4i = false;
 };
 
 struct golap_column {
@@ -141,6 +146,7 @@ int create_device_column(golap_ctx *ctx, const golap_host_column *h, uint64_t co
                          golap_column **out);
 // The host planning tables of a column, from its host catalog.
 void register_extra(golap_column *c);
+uint32_t lz4i_max_nseq(const golap_host_column &h, const uint8_t *comp);
 // Points device column c (created with catalog capacity for at least w's pages and chunks) at a
 // window w of a larger column: catalog and zone maps copied (synchronously, on stream), planning
 // tables rebuilt; the caller fills d_comp with comp_bytes of page data.
@@ -441,7 +447,18 @@ struct Q1Launch {
     int64_t *sum;
     uint64_t *count;
     uint32_t *err;
+    // tiles with coded pages: LDS decode window (a multiple of 16 bytes, 0 = the four-wave
+    // kernel) and the LZ4I token offsets it holds
+    uint32_t win_cap = 0, tok_cap = 0;
 };
 hipError_t launch_q1(golap_ctx *ctx, const Q1Launch &a, bool direct);
+// Decode window and token capacity for LZ4I pages of at most page_bytes output bytes: the
+// window holds the output and, at its tail, the staged page (LZ4I pages shrink, so a slack of an
+// eighth of the page, at least 1 KiB, is enough for the in-place order); max_nseq 0 = unknown.
+inline void lz4i_window(uint32_t page_bytes, uint32_t max_nseq, uint32_t &win_cap, uint32_t &tok_cap) {
+    const uint32_t slack = std::max<uint32_t>(1024, page_bytes / 8);
+    win_cap = (page_bytes + slack + 15) & ~15u;
+    tok_cap = max_nseq ? max_nseq : std::min<uint32_t>(page_bytes / 4 + 1, std::max<uint32_t>(64, page_bytes / 8));
+}
 
 }  // namespace golap_impl

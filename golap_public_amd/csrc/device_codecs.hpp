@@ -20,14 +20,26 @@ constexpr int WAVE = 64;
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
 // Decode one LZ4 block src[0, csize) into dst[0, cap). Returns bytes written or -1.
-// All lanes of the wave must call it with the same arguments.
+// All lanes of the wave must call it with the same arguments. With idx (the u16 token offsets of
+// an LZ4I header, format.hpp), the block's own parse must agree with the index: nseq sequences,
+// sequence s << seg_log2 starting at idx[s - 1].
 __device__ inline int64_t lz4_decode_wave(const uint8_t *__restrict__ src, uint32_t csize,
-                                          uint8_t *__restrict__ dst, uint32_t cap) {
+                                          uint8_t *__restrict__ dst, uint32_t cap,
+                                          const uint8_t *idx = nullptr, uint32_t nseq = 0,
+                                          uint32_t seg_log2 = 0) {
     const uint32_t lane = lane_id();
-    uint32_t ip = 0, op = 0;
+    uint32_t ip = 0, op = 0, seq = 0;
     if (csize == 0) return cap == 0 ? 0 : -1;
     for (;;) {
         if (ip >= csize) return -1;
+        if (idx) {
+            if (seq >= nseq) return -1;
+            if (seq && (seq & ((1u << seg_log2) - 1)) == 0) {
+                const uint32_t s = seq >> seg_log2;
+                if (((uint32_t)idx[2 * (s - 1)] | ((uint32_t)idx[2 * (s - 1) + 1] << 8)) != ip) return -1;
+            }
+            ++seq;
+        }
         const uint32_t token = src[ip++];
         uint32_t lit = token >> 4;
         if (lit == 15) {
@@ -85,6 +97,7 @@ __device__ inline int64_t lz4_decode_wave(const uint8_t *__restrict__ src, uint3
         }
         op += ml;
     }
+    if (idx && seq != nseq) return -1;
     return op;
 }
 
@@ -96,6 +109,263 @@ __device__ __forceinline__ uint64_t wave_incl_scan_u64(uint64_t v) {
         if (lane >= (uint32_t)d) v += t;
     }
     return v;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int d = 1; d < WAVE; d <<= 1) {
+        const uint32_t t = __shfl_up(v, d, WAVE);
+        if (lane >= (uint32_t)d) v += t;
+    }
+    return v;
+}
+
+// ------------------------------------------------------------------ LZ4I: parallel parse
+// An LZ4I page (format.hpp) decoded by one wave with every lane busy:
+//   stage   the page is copied into the tail of the LDS window with 16-byte loads; the output
+//           grows from the window's start towards it.
+//   walk    lane s walks segment s of the sequence list from the offset the index gives,
+//           storing each sequence's token offset in tok[]; every segment must end where the next
+//           one starts (the last one at the end of the block), so the index is checked against
+//           the block's own parse.
+//   copy    sequences are taken 64 at a time in block order, one per lane: a wave prefix sum of
+//           (literals + match) gives every lane its output position; literals are copied lane by
+//           lane with unaligned 4-byte LDS accesses; a match is copied once every byte it reads
+//           is final: all bytes below the output position of the first lane whose match is not
+//           done yet are final, so each round copies at least that lane's match and usually all.
+// Lengths of 32 bytes or more are copied by the whole wave. The staged page is never overwritten
+// before it is read: each group's output must end below the group's first input byte, or the
+// decoder hands the page to the serial decoder.
+namespace lz4i {
+constexpr uint32_t LONG = 32;        // per-lane copies below, whole-wave copies from here
+constexpr uint32_t PAR_MIN = 8;      // fewer sequences: the serial decoder is cheaper
+
+struct Hdr {
+    uint32_t nseq, sl, nseg, hb;
+};
+
+// Header of an LZ4I page in global memory; false if malformed.
+__device__ __forceinline__ bool header(const uint8_t *src, uint32_t csize, Hdr &h) {
+    if (csize < 16) return false;
+    uint32_t w;
+    __builtin_memcpy(&w, src, 4);
+    h.nseq = w & 0xffffu;
+    h.sl = (w >> 16) & 0xffu;
+    if (h.nseq == 0 || h.sl > 15 || (w >> 24) != 0) return false;
+    h.nseg = golap_fmt::lz4i_segments(h.nseq, h.sl);
+    h.hb = (4 + 2 * (h.nseg - 1) + 15) & ~15u;
+    return h.hb <= csize && csize - h.hb <= golap_fmt::LZ4I_MAX_BLOCK;
+}
+
+// Header bytes between the index and the block must be zero (lanes check 64 at a time).
+__device__ __forceinline__ bool pad_zero(const uint8_t *src, const Hdr &h) {
+    bool ok = true;
+    for (uint32_t i = 4 + 2 * (h.nseg - 1) + lane_id(); i < h.hb; i += WAVE) ok &= src[i] == 0;
+    return __all(ok);
+}
+
+__device__ __forceinline__ uint32_t rd16(const uint8_t *p) { return (uint32_t)p[0] | ((uint32_t)p[1] << 8); }
+
+// n bytes from s to d, lane by lane; d below s or at least 4 bytes above it (s + j + 4 <= d + j
+// for a match, so every 4-byte read sees only final bytes or this lane's earlier writes).
+__device__ __forceinline__ void copy_lane(uint8_t *d, const uint8_t *s, uint32_t n) {
+    uint32_t j = 0;
+    for (; j + 4 <= n; j += 4) {
+        uint32_t v;
+        __builtin_memcpy(&v, s + j, 4);
+        __builtin_memcpy(d + j, &v, 4);
+    }
+    for (; j < n; ++j) d[j] = s[j];
+}
+
+// A match of n bytes at d whose source starts off bytes back, by one lane.
+__device__ __forceinline__ void match_lane(uint8_t *d, uint32_t off, uint32_t n) {
+    const uint8_t *s = d - off;  // off <= the output position: s stays in the window
+    if (off >= 4) {
+        copy_lane(d, s, n);
+    } else {
+        for (uint32_t j = 0; j < n; ++j) d[j] = s[j];  // byte by byte: replicates the pattern
+    }
+}
+
+// n bytes to d by the whole wave (arguments uniform). off == 0: a literal run from s, which does
+// not overlap d; otherwise a match with source d - off: rounds of 256 bytes when off >= 256,
+// else byte j of the match is byte j mod off of [d - off, d), final before the copy.
+__device__ __forceinline__ void copy_wave(uint8_t *d, const uint8_t *s, uint32_t n, uint32_t off) {
+    const uint32_t lane = lane_id();
+    if (off == 0 || off >= 4 * WAVE) {
+        const uint8_t *from = off ? d - off : s;
+        for (uint32_t b = 0; b < n; b += 4 * WAVE) {
+            const uint32_t j = b + 4 * lane;
+            if (j + 4 <= n) {
+                uint32_t v;
+                __builtin_memcpy(&v, from + j, 4);
+                __builtin_memcpy(d + j, &v, 4);
+            } else {
+                for (uint32_t x = j; x < n; ++x) d[x] = from[x];
+            }
+            __builtin_amdgcn_wave_barrier();
+        }
+    } else {
+        const uint8_t *pat = d - off;
+        for (uint32_t j = 4 * lane; j < n; j += 4 * WAVE) {
+            const uint32_t e = min(n - j, 4u);
+            uint32_t r = j % off;
+            for (uint32_t x = 0; x < e; ++x) {
+                d[j + x] = pat[r];
+                r = r + 1 == off ? 0 : r + 1;
+            }
+        }
+    }
+}
+
+// Walks one segment: sequences [seq, seq_end) starting at block offset ip; must end at end_ip.
+// Returns false on any disagreement or overrun.
+__device__ __forceinline__ bool walk(const uint8_t *blk, uint32_t bs, uint32_t ip, uint32_t seq, uint32_t seq_end,
+                                     uint32_t end_ip, uint32_t nseq, uint16_t *tok) {
+    for (; seq < seq_end; ++seq) {
+        if (ip >= bs) return false;
+        tok[seq] = (uint16_t)ip;
+        const uint32_t t = blk[ip++];
+        uint32_t lit = t >> 4;
+        if (lit == 15) {
+            uint32_t b;
+            do {
+                if (ip >= bs) return false;
+                b = blk[ip++];
+                lit += b;
+            } while (b == 255);
+        }
+        ip += lit;
+        if (ip > bs) return false;
+        if (ip == bs) {  // literals reach the end: the last sequence
+            if (seq + 1 != nseq) return false;
+            ++seq;
+            break;
+        }
+        if (seq + 1 == nseq) return false;  // the last sequence must end the block
+        ip += 2;
+        if (ip > bs) return false;
+        if ((t & 15) == 15) {
+            uint32_t b;
+            do {
+                if (ip >= bs) return false;
+                b = blk[ip++];
+            } while (b == 255);
+        }
+    }
+    return seq == seq_end && ip == end_ip;
+}
+
+// The parallel decode of a checked header h: the page src[0, csize) (16-byte aligned, readable
+// up to its next 16-byte boundary) into win[0, osize). win_cap: LDS window bytes (a multiple of
+// 16, at least csize rounded up to 16); tok: nseq entries. Returns osize, -1 if the page is
+// malformed, -2 if its groups cannot be decoded in place (the caller decodes it serially).
+__device__ inline int64_t decode_par(const uint8_t *__restrict__ src, uint32_t csize, const Hdr &h, uint8_t *win,
+                                     uint32_t win_cap, uint32_t osize, uint16_t *tok) {
+    const uint32_t lane = lane_id();
+    const uint32_t c16 = (csize + 15) & ~15u;
+    const uint32_t P = win_cap - c16;  // staged page
+    {
+        const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
+        uint4 *d4 = reinterpret_cast<uint4 *>(win + P);
+        for (uint32_t i = lane; i < c16 / 16; i += WAVE) d4[i] = s4[i];
+    }
+    __builtin_amdgcn_wave_barrier();
+    const uint8_t *hdr = win + P;
+    const uint8_t *blk = hdr + h.hb;
+    const uint32_t bs = csize - h.hb;
+    bool ok = true;
+    for (uint32_t sgm = lane; sgm < h.nseg; sgm += WAVE) {
+        const uint32_t ip = sgm ? rd16(hdr + 4 + 2 * (sgm - 1)) : 0u;
+        const uint32_t end_ip = sgm + 1 < h.nseg ? rd16(hdr + 4 + 2 * sgm) : bs;
+        const uint32_t s0 = sgm << h.sl, s1 = min(h.nseq, (sgm + 1) << h.sl);
+        ok &= walk(blk, bs, ip, s0, s1, end_ip, h.nseq, tok);
+    }
+    if (!__all(ok)) return -1;
+    __builtin_amdgcn_wave_barrier();
+    uint32_t base = 0;
+    for (uint32_t g0 = 0; g0 < h.nseq; g0 += WAVE) {
+        const uint32_t k = g0 + lane;
+        const bool valid = k < h.nseq;
+        uint32_t lit = 0, ml = 0, off = 0, ls = 0;
+        if (valid) {
+            uint32_t ip = tok[k];
+            const uint32_t t = blk[ip++];
+            lit = t >> 4;
+            if (lit == 15) {
+                uint32_t b;
+                do {
+                    b = blk[ip++];
+                    lit += b;
+                } while (b == 255);
+            }
+            ls = ip;
+            ip += lit;
+            if (k + 1 < h.nseq) {
+                off = rd16(blk + ip);
+                ip += 2;
+                ml = t & 15;
+                if (ml == 15) {
+                    uint32_t b;
+                    do {
+                        b = blk[ip++];
+                        ml += b;
+                    } while (b == 255);
+                }
+                ml += golap_fmt::LZ4_MINMATCH;
+            }
+        }
+        const uint32_t len = lit + ml;
+        const uint32_t incl = wave_incl_scan_u32(len);
+        const uint32_t dst = base + incl - len, mdst = dst + lit;
+        const uint32_t total = __shfl(incl, WAVE - 1, WAVE);
+        const uint32_t in0 = __shfl((uint32_t)(valid ? tok[k] : 0u), 0, WAVE);
+        // the walk bounds every length by the block size, so these sums do not wrap
+        const bool bad = valid && ((uint64_t)dst + len > osize || (ml && (off == 0 || off > mdst)));
+        if (__any(bad)) return -1;
+        if (base + total > P + h.hb + in0) return -2;  // would overwrite staged input still to be read
+        if (lit < LONG) copy_lane(win + dst, blk + ls, lit);
+        for (uint64_t lw = __ballot(lit >= LONG); lw; lw &= lw - 1) {
+            const int l = __builtin_ctzll(lw);
+            copy_wave(win + __shfl(dst, l, WAVE), blk + __shfl(ls, l, WAVE), __shfl(lit, l, WAVE), 0);
+        }
+        __builtin_amdgcn_wave_barrier();
+        bool pend = ml != 0;
+        for (uint64_t pm = __ballot(pend); pm; pm = __ballot(pend)) {
+            const uint32_t frontier = __shfl(mdst, __builtin_ctzll(pm), WAVE);
+            const uint32_t need = off >= ml ? mdst - off + ml : mdst;
+            const bool ready = pend && need <= frontier;
+            if (ready && ml < LONG) match_lane(win + mdst, off, ml);
+            for (uint64_t lw = __ballot(ready && ml >= LONG); lw; lw &= lw - 1) {
+                const int l = __builtin_ctzll(lw);
+                copy_wave(win + __shfl(mdst, l, WAVE), nullptr, __shfl(ml, l, WAVE), __shfl(off, l, WAVE));
+            }
+            pend = pend && !ready;
+            __builtin_amdgcn_wave_barrier();
+        }
+        base += total;
+    }
+    return base;
+}
+}  // namespace lz4i
+
+// Decodes an LZ4I page src[0, csize) into win[0, osize): in parallel when the page has enough
+// sequences, tok holds them (tok_cap entries) and the window (win_cap bytes, a multiple of 16)
+// holds the staged page; otherwise, and when its groups do not fit in place, serially from
+// global memory. Returns the bytes written or -1.
+__device__ inline int64_t lz4i_decode_wave(const uint8_t *__restrict__ src, uint32_t csize, uint8_t *win,
+                                           uint32_t win_cap, uint32_t osize, uint16_t *tok, uint32_t tok_cap) {
+    lz4i::Hdr h;
+    if (!lz4i::header(src, csize, h) || !lz4i::pad_zero(src, h)) return -1;
+    if (osize > win_cap) return -1;
+    if (h.nseq >= lz4i::PAR_MIN && h.nseq <= tok_cap && tok && ((csize + 15) & ~15u) <= win_cap &&
+        (reinterpret_cast<uintptr_t>(src) & 15) == 0 && (win_cap & 15) == 0) {
+        const int64_t r = lz4i::decode_par(src, csize, h, win, win_cap, osize, tok);
+        if (r != -2) return r;
+    }
+    return lz4_decode_wave(src + h.hb, csize - h.hb, win, osize, src + 4, h.nseq, h.sl);
 }
 
 __device__ __forceinline__ void store_elem(uint8_t *dst, uint64_t v, uint32_t elem) {
@@ -198,10 +468,13 @@ __device__ inline int64_t rle_decode_wave(const uint8_t *__restrict__ src, uint3
     return row == n ? (int64_t)n : -1;
 }
 
-// Decodes one page of any codec into dst[0, cap). Returns bytes written or -1.
+// Decodes one page of any codec into dst[0, cap). Returns bytes written or -1. LZ4I pages are
+// decoded in parallel when the caller gives a window of win_cap >= cap bytes (a multiple of 16)
+// and room for tok_cap token offsets, serially otherwise.
 __device__ inline int64_t page_decode_wave(uint8_t codec, const uint8_t *__restrict__ src,
                                            uint32_t csize, uint8_t *__restrict__ dst, uint32_t cap,
-                                           uint32_t elem) {
+                                           uint32_t elem, uint32_t win_cap = 0, uint16_t *tok = nullptr,
+                                           uint32_t tok_cap = 0) {
     switch (codec & golap_fmt::PAGE_CODEC_MASK) {
     case golap_fmt::RAW: {
         if (csize > cap) return -1;
@@ -209,6 +482,7 @@ __device__ inline int64_t page_decode_wave(uint8_t codec, const uint8_t *__restr
         return csize;
     }
     case golap_fmt::LZ4: return lz4_decode_wave(src, csize, dst, cap);
+    case golap_fmt::LZ4I: return lz4i_decode_wave(src, csize, dst, win_cap >= cap ? win_cap : cap, cap, tok, tok_cap);
     case golap_fmt::DBP: {
         int64_t r = dbp_decode_wave(src, csize, dst, cap);
         return r < 0 ? -1 : r * elem;

@@ -55,8 +55,9 @@ static inline uint8_t *put_len(uint8_t *op, uint64_t len) {  // length extension
 // rejected, so the table needs no reset between pages. Matches shorter than min_match are not
 // emitted: a short match saves at most a byte or two but costs the decoder a full sequence.
 uint64_t lz4_encode_block(const uint8_t *src, uint64_t n, uint8_t *dst, uint32_t min_match,
-                          int64_t *ht, int64_t base) {
+                          int64_t *ht, int64_t base, std::vector<uint32_t> *tokens) {
     uint8_t *op = dst;
+    if (tokens) tokens->clear();
     uint64_t anchor = 0, ip = 0;
     if (min_match < LZ4_MINMATCH) min_match = LZ4_MINMATCH;
     if (n >= LZ4_MFLIMIT + 1) {
@@ -75,6 +76,7 @@ uint64_t lz4_encode_block(const uint8_t *src, uint64_t n, uint8_t *dst, uint32_t
                 if (len >= min_match) {
                     uint64_t lit = ip - anchor, ml = len - LZ4_MINMATCH;
                     uint8_t *token = op++;
+                    if (tokens) tokens->push_back((uint32_t)(token - dst));
                     *token = (uint8_t)(((lit >= 15 ? 15 : lit) << 4) | (ml >= 15 ? 15 : ml));
                     if (lit >= 15) op = put_len(op, lit - 15);
                     memcpy(op, src + anchor, lit);
@@ -94,11 +96,30 @@ uint64_t lz4_encode_block(const uint8_t *src, uint64_t n, uint8_t *dst, uint32_t
         }
     }
     uint64_t lit = n - anchor;
+    if (tokens) tokens->push_back((uint32_t)(op - dst));
     *op++ = (uint8_t)((lit >= 15 ? 15 : lit) << 4);
     if (lit >= 15) op = put_len(op, lit - 15);
     memcpy(op, src + anchor, lit);
     op += lit;
     return (uint64_t)(op - dst);
+}
+
+uint64_t lz4i_make_page(const uint8_t *blk, uint64_t bsize, const std::vector<uint32_t> &tokens, uint8_t *out) {
+    const uint64_t nseq = tokens.size();
+    if (!nseq || bsize > LZ4I_MAX_BLOCK) return 0;
+    const uint32_t sl = lz4i_seg_log2((uint32_t)nseq);
+    const uint32_t nseg = lz4i_segments((uint32_t)nseq, sl), hb = lz4i_header_bytes((uint32_t)nseq, sl);
+    memmove(out + hb, blk, bsize);  // the block may already sit in out, behind the header
+    memset(out, 0, hb);
+    out[0] = (uint8_t)(nseq & 0xff);
+    out[1] = (uint8_t)(nseq >> 8);
+    out[2] = (uint8_t)sl;
+    for (uint32_t s = 1; s < nseg; ++s) {
+        const uint32_t t = tokens[(uint64_t)s << sl];
+        out[4 + 2 * (s - 1)] = (uint8_t)(t & 0xff);
+        out[5 + 2 * (s - 1)] = (uint8_t)(t >> 8);
+    }
+    return hb + bsize;
 }
 
 // ------------------------------------------------------------------------------------ DBP
@@ -243,14 +264,28 @@ void code_page(const uint8_t *src, uint64_t rows, uint32_t type, uint32_t elem, 
             keep.assign(cand.begin(), cand.begin() + sz);
         }
     };
-    if (codec == LZ4 || codec == GOLAP_CODEC_AUTO) {
-        cand.resize(lz4_bound(nbytes));
-        uint64_t sz = lz4_encode_block(src, nbytes, cand.data(), min_match, ht, base);
+    // LZ4: a block of one literal run stays a plain LZ4 page (read in place); any other block
+    // becomes an LZ4I page, the block behind its parse index, unless it is too large for one.
+    std::vector<uint32_t> toks;
+    auto lz4_candidate = [&]() {
+        cand.resize(160 + lz4_bound(nbytes) + 16);
+        uint8_t *blk = cand.data() + 160;  // behind the largest header (144 bytes)
+        const uint64_t sz = lz4_encode_block(src, nbytes, blk, min_match, ht, base, &toks);
         // a block with a match can have exactly the literal-only size: parse, do not compare sizes
-        uint8_t c = LZ4;
-        if (lz4_is_literal_only(cand.data(), sz, nbytes)) c |= PAGE_LITERAL_ONLY;
-        consider(sz, c);
-    }
+        if (lz4_is_literal_only(blk, sz, nbytes)) {
+            memmove(cand.data(), blk, sz);
+            consider(sz, (uint8_t)(LZ4 | PAGE_LITERAL_ONLY));
+            return;
+        }
+        uint64_t pz = lz4i_make_page(blk, sz, toks, cand.data());
+        if (pz) {
+            consider(pz, LZ4I);
+        } else {
+            memmove(cand.data(), blk, sz);
+            consider(sz, LZ4);
+        }
+    };
+    if (codec == LZ4 || codec == GOLAP_CODEC_AUTO) lz4_candidate();
     if (codec == DBP || codec == GOLAP_CODEC_AUTO) {
         cand.resize(dbp_bound(rows));
         const uint64_t sz = dbp_encode_page(src, rows, type, cand.data());
@@ -265,14 +300,7 @@ void code_page(const uint8_t *src, uint64_t rows, uint32_t type, uint32_t elem, 
         cand.resize(for_bound(rows));
         consider(for_encode_page(src, rows, type, cand.data()), (uint8_t)(golap_fmt::FOR | PAGE_POINT));
     }
-    if (best == UINT64_MAX) {  // DBP / FOR requested but the page's deltas or range are too wide
-        cand.resize(lz4_bound(nbytes));
-        uint64_t sz = lz4_encode_block(src, nbytes, cand.data(), min_match, ht, base);
-        // a block with a match can have exactly the literal-only size: parse, do not compare sizes
-        uint8_t c = LZ4;
-        if (lz4_is_literal_only(cand.data(), sz, nbytes)) c |= PAGE_LITERAL_ONLY;
-        consider(sz, c);
-    }
+    if (best == UINT64_MAX) lz4_candidate();  // DBP / FOR requested but the page's deltas or range are too wide
     emit(keep.data(), best, best_codec);
 }
 }  // namespace

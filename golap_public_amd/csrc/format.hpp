@@ -16,6 +16,14 @@
 //   RLE  (3)  run-length encoding (value, exclusive end row) pairs.
 //   FOR  (4)  frame of reference: v[r] = base + the r-th bw-bit field of a little-endian bit
 //             stream (unsorted low-range integers); every row is readable on its own.
+//   LZ4I (5)  one LZ4 block behind a parse index, so that the 64 lanes of a wave can walk the
+//             block's sequence list from 64 places at once: a 4-byte header {u16 nseq, the
+//             block's sequences (the last one literals only); u8 seg_log2; u8 0}, then
+//             u16 tok[nseg - 1] with nseg = ceil(nseq / 2^seg_log2), tok[s - 1] being the block
+//             offset of the token of sequence s * 2^seg_log2, zero bytes up to a multiple of 16,
+//             then the block (below 64 KiB). The block is standard LZ4 on its own; the index
+//             only says where the sequences start, and a reader rejects a page whose index
+//             disagrees with the block's own parse.
 // Flag bits above the codec: PAGE_POINT (0x40, see below) and PAGE_LITERAL_ONLY (0x80), which
 // marks an LZ4 page made of a single literal run; such a page's payload starts lz4_literal_header_bytes(n_bytes) bytes after the page start and the
 // encoder places it 16-byte aligned in the column buffer, so a scan reads it in place.
@@ -24,7 +32,7 @@
 
 namespace golap_fmt {
 
-enum Codec : uint8_t { RAW = 0, LZ4 = 1, DBP = 2, RLE = 3, FOR = 4 };
+enum Codec : uint8_t { RAW = 0, LZ4 = 1, DBP = 2, RLE = 3, FOR = 4, LZ4I = 5 };
 constexpr uint8_t PAGE_CODEC_MASK = 0x0f;
 constexpr uint8_t PAGE_LITERAL_ONLY = 0x80;
 // Bit 6 (PAGE_POINT): any row of the page can be read without decoding the page: every RLE
@@ -40,6 +48,25 @@ constexpr uint32_t LZ4_MAX_OFFSET = 65535;
 // Bytes of token + literal-length extension for a literal-only block of n bytes.
 __host__ __device__ inline uint32_t lz4_literal_header_bytes(uint32_t n) {
     return n < 15 ? 1u : 2u + (n - 15) / 255u;
+}
+
+// LZ4I page header bytes for nseq sequences indexed every 2^seg_log2 sequences.
+constexpr uint32_t LZ4I_MAX_BLOCK = 65535;  // token offsets fit 16 bits
+constexpr uint32_t LZ4I_MIN_SEG_LOG2 = 4;
+constexpr uint32_t LZ4I_MAX_SEGS = 64;      // the encoder's choice: one segment per lane
+__host__ __device__ inline uint32_t lz4i_segments(uint32_t nseq, uint32_t seg_log2) {
+    return (uint32_t)(((uint64_t)nseq + (1ull << seg_log2) - 1) >> seg_log2);
+}
+__host__ __device__ inline uint32_t lz4i_header_bytes(uint32_t nseq, uint32_t seg_log2) {
+    const uint32_t nseg = lz4i_segments(nseq, seg_log2);
+    return (4 + 2 * (nseg ? nseg - 1 : 0) + 15) & ~15u;
+}
+// The encoder's segment length: the shortest power of two, at least 16 sequences, that cuts
+// the block into at most LZ4I_MAX_SEGS segments.
+__host__ __device__ inline uint32_t lz4i_seg_log2(uint32_t nseq) {
+    uint32_t l = LZ4I_MIN_SEG_LOG2;
+    while (((uint64_t)LZ4I_MAX_SEGS << l) < nseq) ++l;
+    return l;
 }
 
 // DBP page header (32 bytes, little endian) followed by (n-1) packed deltas.

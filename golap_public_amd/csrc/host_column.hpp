@@ -27,8 +27,13 @@ int64_t load_value(const uint8_t *p, uint32_t type);
 uint64_t lz4_bound(uint64_t n);
 // hash_table: 1<<LZ4_HASH_LOG absolute positions; base: absolute position of src (must grow
 // from block to block when the table is reused, so stale entries fall below base).
+// tokens, when given, receives the block offset of every sequence's token, in order.
 uint64_t lz4_encode_block(const uint8_t *src, uint64_t n, uint8_t *dst, uint32_t min_match,
-                          int64_t *hash_table, int64_t base);
+                          int64_t *hash_table, int64_t base, std::vector<uint32_t> *tokens = nullptr);
+// An LZ4I page (format.hpp) from a block of bsize bytes and its token offsets; returns the
+// page size, 0 if the block does not fit the index (64 KiB or more). out must hold
+// lz4i_header_bytes(tokens.size(), lz4i_seg_log2(tokens.size())) + bsize bytes.
+uint64_t lz4i_make_page(const uint8_t *blk, uint64_t bsize, const std::vector<uint32_t> &tokens, uint8_t *out);
 constexpr uint32_t LZ4_HASH_LOG = 14;
 
 // Delta-bitpack page; returns 0 if the value range needs more than 32 bits per delta.

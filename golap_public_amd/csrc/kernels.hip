@@ -68,19 +68,21 @@ struct DecodeArgs {
     uint64_t n_rows;
     uint8_t *out;
     uint64_t out_row0;
-    uint32_t window;  // LDS bytes per wave
+    uint32_t window;   // LDS decode window (a multiple of 16 bytes, the page plus LZ4I slack)
+    uint32_t tok_cap;  // LZ4I token offsets after the window
     uint32_t *err;
 };
 
-// One wave per page. In-place pages are copied global->global with 16-byte accesses; coded
-// pages are decoded into the wave's LDS window, then written out with 16-byte stores.
-__global__ __launch_bounds__(BLOCK) void k_decode_pages(DecodeArgs a) {
+// One wave (workgroup) per page. In-place pages are copied global->global with 16-byte
+// accesses; coded pages are decoded into the wave's LDS window (LZ4I pages by the parallel
+// parse), then written out with 16-byte stores.
+__global__ __launch_bounds__(WAVE) void k_decode_pages(DecodeArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
-    const uint32_t wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    uint8_t *win = lds + (size_t)wave * a.window;
+    const uint32_t lane = threadIdx.x;
+    uint8_t *win = lds;
+    uint16_t *tok = reinterpret_cast<uint16_t *>(lds + a.window);
     const uint64_t n = a.page_end - a.page_begin;
-    for (uint64_t i = (uint64_t)blockIdx.x * WAVES_PER_BLOCK + wave; i < n;
-         i += (uint64_t)gridDim.x * WAVES_PER_BLOCK) {
+    for (uint64_t i = blockIdx.x; i < n; i += gridDim.x) {
         const uint64_t p = a.page_begin + i;
         const uint64_t r0 = p * a.page_rows;
         const uint32_t rows = (uint32_t)min<uint64_t>(a.page_rows, a.n_rows - r0);
@@ -102,7 +104,7 @@ __global__ __launch_bounds__(BLOCK) void k_decode_pages(DecodeArgs a) {
                 if (lane == 0) raise_err(a.err, DEVERR_CORRUPT);
                 continue;
             }
-            const int64_t got = page_decode_wave(codec, src, csize, win, nbytes, a.c.elem);
+            const int64_t got = page_decode_wave(codec, src, csize, win, nbytes, a.c.elem, a.window, tok, a.tok_cap);
             if (got != (int64_t)nbytes) {
                 if (lane == 0) raise_err(a.err, DEVERR_CORRUPT);
                 continue;
@@ -133,18 +135,23 @@ hipError_t launch_decode_pages(golap_ctx *ctx, const golap_column *col, uint64_t
     a.n_rows = col->n_rows;
     a.out = dev_out;
     a.out_row0 = out_row0;
-    a.window = (col->page_rows * col->elem + 15) & ~15u;
+    const uint32_t page_bytes = col->page_rows * col->elem;
+    if (col->extra.has_lz4i && page_bytes <= 65536) {
+        lz4i_window(page_bytes, col->extra.lz4i_max_nseq, a.window, a.tok_cap);
+    } else {
+        a.window = (page_bytes + 15) & ~15u;
+        a.tok_cap = 0;
+    }
     a.err = reinterpret_cast<uint32_t *>(ctx->d_scratch);
-    const size_t lds = (size_t)a.window * WAVES_PER_BLOCK;
+    const size_t lds = (size_t)a.window + ((2 * (size_t)a.tok_cap + 15) & ~(size_t)15);
     hipError_t e = prepare_lds(ctx, k_decode_pages, lds);
     if (e != hipSuccess) return e;
     int occ = 0;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_decode_pages, BLOCK, lds);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_decode_pages, WAVE, lds);
     if (e != hipSuccess) return e;
     occ = std::max(occ, 1);
-    const uint64_t need = (page_end - page_begin + WAVES_PER_BLOCK - 1) / WAVES_PER_BLOCK;
-    const uint32_t grid = (uint32_t)std::min<uint64_t>(need, (uint64_t)ctx->cus * occ);
-    hipLaunchKernelGGL(k_decode_pages, dim3(grid), dim3(BLOCK), lds, ctx->stream, a);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(page_end - page_begin, (uint64_t)ctx->cus * occ);
+    hipLaunchKernelGGL(k_decode_pages, dim3(grid), dim3(WAVE), lds, ctx->stream, a);
     return hipGetLastError();
 }
 
@@ -458,6 +465,8 @@ struct Q1Args {
     int64_t *sum;
     uint64_t *count;
     uint32_t *err;
+    // k_q1_wave: LDS bytes of the probe table, of the decode window and token offsets it holds
+    uint32_t tab_bytes, win_cap, tok_cap;
 };
 
 // How the date probe is answered: the open-addressing table in HBM/L2, the same table pinned
@@ -676,25 +685,245 @@ __global__ __launch_bounds__(BLOCK) void k_q1_general(Q1Args a) {
     q1_flush(a, sum, cnt);
 }
 
+// ------------------------------------------------------------------ one wave per tile
+// Tiles with coded pages (LZ4I, LZ4, DBP, RLE, FOR) at page sizes of 1024..8192 rows. The
+// workgroup is one wave with one LDS decode window and token array: the tile's columns are
+// decoded into the window one after another (LZ4I pages by the parallel parse), and each lane
+// keeps the state of its rows in registers between them: a pass bit per row and the discount
+// bytes. Lane l owns rows 256 i + 4 l + {0..3} (i < G4), so every column access is one
+// 16-byte (int32) or 4-byte (uint8) load per four rows, consecutive across the lanes. The date
+// probe goes first when it is the LDS bitmap (cheap, the most selective test); a hash probe goes
+// after the quantity and discount tests, for the rows that are still candidates. Once no row of
+// the tile is a candidate, the remaining columns are not decoded.
+template <int G4>
+struct Q1Lanes {
+    static constexpr int NW = (4 * G4 + 63) / 64;
+    uint64_t m[NW];     // bit 4 i + j: row 256 i + 4 lane + j is a candidate
+    uint32_t disc[G4];  // discount bytes of the four rows of group i
+    __device__ __forceinline__ bool bit(int b) const { return (m[b >> 6] >> (b & 63)) & 1ull; }
+    __device__ __forceinline__ void clear(int b) { m[b >> 6] &= ~(1ull << (b & 63)); }
+    __device__ __forceinline__ bool any() const {
+        uint64_t o = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) o |= m[w];
+        return o != 0;
+    }
+};
+
+// The rows of one column of tile p: in place in HBM (*lds = false) or decoded into the window.
+// Returns null for a corrupt page.
+__device__ __forceinline__ const uint8_t *q1_col(const Q1Args &a, const ColView &c, uint64_t p, uint32_t rows,
+                                                 uint8_t *win, uint16_t *tok, bool &in_lds) {
+    const uint8_t code = c.page_codec[p];
+    const uint8_t *src = c.comp + c.page_off[p];
+    const uint32_t csize = c.page_csize[p], nb = rows * c.elem;
+    if (page_direct(code)) {
+        const uint32_t hd = direct_header(code, nb);
+        in_lds = false;
+        return (csize == hd + nb && ((reinterpret_cast<uintptr_t>(src + hd) & 15) == 0)) ? src + hd : nullptr;
+    }
+    in_lds = true;
+    const int64_t got = page_decode_wave(code, src, csize, win, nb, c.elem, a.win_cap, tok, a.tok_cap);
+    return got == (int64_t)nb ? win : nullptr;
+}
+
+template <int G4, bool FULL>
+__device__ __forceinline__ void q1_date_pass(const Q1Args &a, const uint32_t *tab, const uint8_t *dp, uint32_t rows,
+                                             Q1Lanes<G4> &st, bool hash) {
+    const uint32_t lane = __lane_id();
+#pragma unroll
+    for (int i = 0; i < G4; ++i) {
+        const uint32_t r0 = 256 * i + 4 * lane;
+        int32_t v[4];
+        if (FULL || r0 + 4 <= rows) {
+            const int4 x = *reinterpret_cast<const int4 *>(dp + 4 * r0);
+            v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) v[j] = r0 + j < rows ? reinterpret_cast<const int32_t *>(dp)[r0 + j] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int b = 4 * i + j;
+            bool hit;
+            if (!hash) {
+                const uint32_t off = (uint32_t)v[j] - (uint32_t)a.kmin;
+                const bool in = off < a.krange;
+                const uint32_t w = tab[in ? (off >> 5) : 0u];
+                hit = in && ((w >> (off & 31)) & 1u);
+            } else {
+                uint32_t pay;
+                hit = st.bit(b) && ht_probe(reinterpret_cast<const uint64_t *>(tab), a.log2cap, v[j], pay);
+            }
+            if (!hit) st.clear(b);
+        }
+    }
+}
+
+// quantity (KEEP = false) or discount (KEEP = true: the bytes are kept for the sum)
+template <int G4, bool FULL, bool KEEP>
+__device__ __forceinline__ void q1_byte_pass(const uint8_t *cp, uint32_t rows, uint32_t lo, uint32_t span,
+                                             Q1Lanes<G4> &st) {
+    const uint32_t lane = __lane_id();
+#pragma unroll
+    for (int i = 0; i < G4; ++i) {
+        const uint32_t r0 = 256 * i + 4 * lane;
+        uint32_t x;
+        if (FULL || r0 + 4 <= rows) {
+            x = *reinterpret_cast<const uint32_t *>(cp + r0);
+        } else {
+            x = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) x |= r0 + j < rows ? (uint32_t)cp[r0 + j] << (8 * j) : 0u;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (((x >> (8 * j)) & 0xffu) - lo > span) st.clear(4 * i + j);
+        if (KEEP) st.disc[i] = x;
+    }
+}
+
+template <int G4, bool FULL>
+__device__ __forceinline__ void q1_price_pass(const Q1Args &a, const uint8_t *pp, uint32_t rows, const Q1Lanes<G4> &st,
+                                              int64_t &sum, uint32_t &cnt) {
+    const uint32_t lane = __lane_id();
+#pragma unroll
+    for (int i = 0; i < G4; ++i) {
+        const uint32_t r0 = 256 * i + 4 * lane;
+        int32_t v[4];
+        if (FULL || r0 + 4 <= rows) {
+            const int4 x = *reinterpret_cast<const int4 *>(pp + 4 * r0);
+            v[0] = x.x; v[1] = x.y; v[2] = x.z; v[3] = x.w;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 4; ++j) v[j] = r0 + j < rows ? reinterpret_cast<const int32_t *>(pp)[r0 + j] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const bool hit = st.bit(4 * i + j) && ((uint32_t)v[j] - a.price_lo <= a.price_span);
+            sum += (int64_t)v[j] * (int64_t)(hit ? (st.disc[i] >> (8 * j)) & 0xffu : 0u);
+            cnt += hit ? 1u : 0u;
+        }
+    }
+}
+
+// One pass over a column's rows, in LDS (the window) or in place in HBM: the two calls are
+// separate so that each compiles to its own memory instructions.
+#define Q1_PASS(in_lds, win, ptr, full, CALL)   \
+    do {                                        \
+        if (in_lds) {                           \
+            const uint8_t *q1p_ = (win);        \
+            if (full) CALL(true, q1p_); else CALL(false, q1p_); \
+        } else {                                \
+            const uint8_t *q1p_ = (ptr);        \
+            if (full) CALL(true, q1p_); else CALL(false, q1p_); \
+        }                                       \
+    } while (0)
+
+template <int PROBE, int G4>
+__global__ __launch_bounds__(WAVE) void k_q1_wave(Q1Args a) {
+    extern __shared__ __align__(16) uint8_t lds[];
+    constexpr bool HASH = PROBE != PROBE_BITMAP;
+    const uint32_t *tab;
+    if (PROBE == PROBE_HASH_GLOBAL) {
+        tab = reinterpret_cast<const uint32_t *>(a.slots);
+    } else {
+        uint32_t *t = reinterpret_cast<uint32_t *>(lds);
+        const uint32_t *src = PROBE == PROBE_BITMAP ? a.bitmap : reinterpret_cast<const uint32_t *>(a.slots);
+        const uint32_t words = PROBE == PROBE_BITMAP ? (a.krange + 31) / 32 : (2u << a.log2cap);
+        for (uint32_t i = threadIdx.x; i < words; i += WAVE) t[i] = src[i];
+        __syncthreads();
+        tab = t;
+    }
+    uint8_t *win = lds + a.tab_bytes;
+    uint16_t *tok = reinterpret_cast<uint16_t *>(win + a.win_cap);
+    const uint32_t P = a.page_rows;
+    int64_t sum = 0;
+    uint32_t cnt = 0;
+    const XcdRange xr = xcd_range(a.page_begin, a.page_end);
+    for (uint64_t p = xr.first; p < xr.end; p += xr.stride) {
+        if (a.chunk_mask && a.chunk_mask[p / a.pages_per_chunk] == 0) continue;
+        if (page_direct(a.date.page_codec[p]) && page_direct(a.disc.page_codec[p]) &&
+            page_direct(a.qty.page_codec[p]) && page_direct(a.price.page_codec[p]))
+            continue;  // k_q1_direct's tile
+        const uint32_t rows = (uint32_t)min<uint64_t>(P, a.n_rows - p * P);
+        const bool full = rows == (uint32_t)(256 * G4);
+        Q1Lanes<G4> st;
+#pragma unroll
+        for (int w = 0; w < Q1Lanes<G4>::NW; ++w) st.m[w] = ~0ull;
+        if (!full)
+            for (int b = 0; b < 4 * G4; ++b)
+                if (256 * (b / 4) + 4 * __lane_id() + (b % 4) >= rows) st.clear(b);
+        bool bad = false, in_lds = false;
+        const uint8_t *cp;
+#define Q1_DATE(F, ptr) q1_date_pass<G4, F>(a, tab, ptr, rows, st, HASH)
+#define Q1_QTY(F, ptr) q1_byte_pass<G4, F, false>(ptr, rows, a.qty_lo, a.qty_span, st)
+#define Q1_DISC(F, ptr) q1_byte_pass<G4, F, true>(ptr, rows, a.disc_lo, a.disc_span, st)
+#define Q1_PRICE(F, ptr) q1_price_pass<G4, F>(a, ptr, rows, st, sum, cnt)
+        if (!HASH) {
+            cp = q1_col(a, a.date, p, rows, win, tok, in_lds);
+            if (!(bad = cp == nullptr)) Q1_PASS(in_lds, win, cp, full, Q1_DATE);
+        }
+        if (!bad && __any(st.any())) {
+            cp = q1_col(a, a.qty, p, rows, win, tok, in_lds);
+            if (!(bad = cp == nullptr)) Q1_PASS(in_lds, win, cp, full, Q1_QTY);
+        }
+        // the discount bytes are kept in registers, so the window is free for the next column
+        if (!bad && __any(st.any())) {
+            cp = q1_col(a, a.disc, p, rows, win, tok, in_lds);
+            if (!(bad = cp == nullptr)) Q1_PASS(in_lds, win, cp, full, Q1_DISC);
+        }
+        if (HASH && !bad && __any(st.any())) {
+            cp = q1_col(a, a.date, p, rows, win, tok, in_lds);
+            if (!(bad = cp == nullptr)) Q1_PASS(in_lds, win, cp, full, Q1_DATE);
+        }
+        if (!bad && __any(st.any())) {
+            cp = q1_col(a, a.price, p, rows, win, tok, in_lds);
+            if (!(bad = cp == nullptr)) Q1_PASS(in_lds, win, cp, full, Q1_PRICE);
+        }
+#undef Q1_DATE
+#undef Q1_QTY
+#undef Q1_DISC
+#undef Q1_PRICE
+        if (bad && __lane_id() == 0) raise_err(a.err, DEVERR_CORRUPT);
+        __builtin_amdgcn_wave_barrier();  // the window is rewritten by the next tile
+    }
+    q1_flush(a, sum, cnt);
+}
+#undef Q1_PASS
+
 template <typename K>
-static hipError_t launch_persistent(golap_ctx *ctx, K kernel, const Q1Args &a, size_t lds) {
+static hipError_t launch_persistent(golap_ctx *ctx, K kernel, const Q1Args &a, size_t lds, int block = BLOCK) {
     hipError_t e = prepare_lds(ctx, kernel, lds);
     if (e != hipSuccess) return e;
     int occ = 0;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, BLOCK, lds);
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, block, lds);
     if (e != hipSuccess) return e;
     occ = std::max(occ, 1);
     const uint64_t tiles = a.page_end - a.page_begin;
     uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>(tiles, (uint64_t)ctx->cus * occ));
     if (grid >= XCDS) grid -= grid % XCDS;  // whole rounds over the XCDs (see xcd_range)
-    hipLaunchKernelGGL(kernel, dim3((uint32_t)grid), dim3(BLOCK), lds, ctx->stream, a);
+    hipLaunchKernelGGL(kernel, dim3((uint32_t)grid), dim3(block), lds, ctx->stream, a);
     return hipGetLastError();
 }
 
 template <int PROBE>
-static hipError_t launch_q1_mode(golap_ctx *ctx, const Q1Args &a, size_t tab, bool direct) {
-    if (!direct)
+static hipError_t launch_q1_mode(golap_ctx *ctx, Q1Args a, size_t tab, bool direct) {
+    if (!direct) {
+        // one wave per tile where the page size allows, else the four-wave general kernel
+        a.tab_bytes = (uint32_t)((tab + 15) & ~(size_t)15);
+        const size_t lds = (size_t)a.tab_bytes + a.win_cap + ((2 * (size_t)a.tok_cap + 15) & ~(size_t)15);
+        if (a.win_cap && lds <= (size_t)ctx->lds_max) {
+            switch (a.page_rows) {
+            case 1024: return launch_persistent(ctx, k_q1_wave<PROBE, 4>, a, lds, WAVE);
+            case 2048: return launch_persistent(ctx, k_q1_wave<PROBE, 8>, a, lds, WAVE);
+            case 4096: return launch_persistent(ctx, k_q1_wave<PROBE, 16>, a, lds, WAVE);
+            case 8192: return launch_persistent(ctx, k_q1_wave<PROBE, 32>, a, lds, WAVE);
+            default: break;
+            }
+        }
         return launch_persistent(ctx, k_q1_general<PROBE>, a, tab + (size_t)a.page_rows * 10);
+    }
     switch (a.page_rows) {
     case 1024: return launch_persistent(ctx, k_q1_direct<PROBE, 1>, a, tab);
     case 2048: return launch_persistent(ctx, k_q1_direct<PROBE, 2>, a, tab);
@@ -730,6 +959,8 @@ hipError_t launch_q1(golap_ctx *ctx, const Q1Launch &l, bool direct) {
     a.sum = l.sum;
     a.count = l.count;
     a.err = l.err;
+    a.win_cap = l.win_cap;
+    a.tok_cap = l.tok_cap;
     switch (l.probe) {
     case PROBE_BITMAP:
         return launch_q1_mode<PROBE_BITMAP>(ctx, a, (((size_t)l.krange + 31) / 32 * 4 + 15) & ~(size_t)15, direct);

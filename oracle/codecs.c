@@ -9,6 +9,7 @@
  * Every read and write is bounds checked, as in the library's "safe" decoder.
  */
 #include "oracle.h"
+#include <stdlib.h>
 #include <string.h>
 
 int64_t oracle_lz4_decode(const uint8_t *src, uint64_t csize, uint8_t *dst, uint64_t cap) {
@@ -50,6 +51,67 @@ int64_t oracle_lz4_decode(const uint8_t *src, uint64_t csize, uint8_t *dst, uint
         op += ml;
     }
     return (int64_t)op;
+}
+
+/* Token offsets of an LZ4 block's sequences by a serial parse (the parse of
+ * oracle_lz4_decode): returns the number of sequences, -1 if the block does not parse. */
+static int64_t lz4_token_offsets(const uint8_t *src, uint64_t csize, uint32_t *tok, uint64_t cap_tok) {
+    uint64_t ip = 0, n = 0;
+    if (csize == 0) return -1;
+    for (;;) {
+        if (ip >= csize) return -1;
+        if (n < cap_tok) tok[n] = (uint32_t)ip;
+        ++n;
+        uint32_t token = src[ip++];
+        uint64_t lit = token >> 4;
+        if (lit == 15) {
+            uint32_t b;
+            do {
+                if (ip >= csize) return -1;
+                b = src[ip++];
+                lit += b;
+            } while (b == 255);
+        }
+        if (ip + lit > csize) return -1;
+        ip += lit;
+        if (ip == csize) break;
+        if (ip + 2 > csize) return -1;
+        ip += 2;
+        if ((token & 15) == 15) {
+            uint32_t b;
+            do {
+                if (ip >= csize) return -1;
+                b = src[ip++];
+            } while (b == 255);
+        }
+    }
+    return (int64_t)n;
+}
+
+/* LZ4I page (this build's page format, golap_public_amd/csrc/format.hpp): header {u16 nseq,
+ * u8 seg_log2, u8 0}, u16 tok[nseg - 1] (block offset of the token of sequence s << seg_log2),
+ * zero bytes to a multiple of 16, then a standard LZ4 block below 64 KiB. The page is valid iff
+ * the block decodes (oracle_lz4_decode) and the header agrees with the block's serial parse:
+ * nseq sequences, every indexed offset at the token of its sequence. */
+int64_t oracle_lz4i_decode(const uint8_t *src, uint64_t csize, uint8_t *dst, uint64_t cap) {
+    if (csize < 4) return -1;
+    const uint32_t nseq = (uint32_t)src[0] | ((uint32_t)src[1] << 8), sl = src[2];
+    if (nseq == 0 || sl > 15 || src[3] != 0) return -1;
+    const uint32_t nseg = (uint32_t)(((uint64_t)nseq + (1ull << sl) - 1) >> sl);
+    const uint64_t hb = (4 + 2 * (uint64_t)(nseg - 1) + 15) & ~15ull;
+    if (hb > csize || csize - hb > 65535) return -1;
+    const uint8_t *blk = src + hb;
+    const uint64_t bs = csize - hb;
+    uint32_t *tok = (uint32_t *)malloc(((uint64_t)nseq + 1) * sizeof(uint32_t));
+    if (!tok) return -1;
+    int64_t got = lz4_token_offsets(blk, bs, tok, (uint64_t)nseq + 1);
+    int ok = got == (int64_t)nseq;
+    for (uint32_t s = 1; ok && s < nseg; ++s)
+        ok = ((uint32_t)src[4 + 2 * (s - 1)] | ((uint32_t)src[5 + 2 * (s - 1)] << 8)) == tok[(uint64_t)s << sl];
+    for (uint64_t i = 4 + 2 * (uint64_t)(nseg - 1); ok && i < hb; ++i) ok = src[i] == 0;
+    free(tok);
+    if (!ok) return -1;
+    return oracle_lz4_decode(blk, bs, dst, cap);
 }
 
 static inline uint64_t rd_le(const uint8_t *p, uint32_t n) {
@@ -131,7 +193,8 @@ int64_t oracle_rle_decode(const uint8_t *src, uint64_t csize, uint8_t *dst, uint
     return n;
 }
 
-/* codec: low 4 bits are the codec id; bit 7 (literal-only LZ4) is a layout hint only. */
+/* codec: low 4 bits are the codec id (0 RAW, 1 LZ4, 2 DBP, 3 RLE, 4 FOR, 5 LZ4I); bit 7
+ * (literal-only LZ4) and bit 6 (point-readable) are layout hints only. */
 int64_t oracle_page_decode(uint32_t codec, const uint8_t *src, uint64_t csize, uint8_t *dst,
                            uint64_t cap) {
     switch (codec & 0x0f) {
@@ -143,6 +206,7 @@ int64_t oracle_page_decode(uint32_t codec, const uint8_t *src, uint64_t csize, u
     case 2: return oracle_dbp_decode(src, csize, dst, cap);
     case 3: return oracle_rle_decode(src, csize, dst, cap);
     case 4: return oracle_for_decode(src, csize, dst, cap);
+    case 5: return oracle_lz4i_decode(src, csize, dst, cap);
     default: return -1;
     }
 }
@@ -160,7 +224,7 @@ int oracle_decode_column(const uint8_t *comp, const uint64_t *page_off, const ui
         int64_t got = oracle_page_decode(page_codec[p], comp + page_off[p], page_csize[p],
                                          dst + r0 * elem, rows * elem);
         uint32_t cd = page_codec[p] & 0x0f;
-        uint64_t want = cd == 0 || cd == 1 ? rows * elem : rows;
+        uint64_t want = cd == 0 || cd == 1 || cd == 5 ? rows * elem : rows;
         if (got < 0 || (uint64_t)got != want) return -1;
     }
     return 0;

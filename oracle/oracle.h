@@ -79,6 +79,8 @@ void oracle_gen_uniform_i32(uint64_t seed, uint32_t col, uint64_t n, int32_t lo,
  * LZ4_decompress_safe_partial(src, dst, csize, cap, cap) (core/comp_cpu.hpp:22-32).
  * Returns the number of bytes written, or -1 on a malformed/overrunning block. */
 int64_t oracle_lz4_decode(const uint8_t *src, uint64_t csize, uint8_t *dst, uint64_t cap);
+/* An LZ4I page (parse index + LZ4 block); -1 when the index disagrees with the block. */
+int64_t oracle_lz4i_decode(const uint8_t *src, uint64_t csize, uint8_t *dst, uint64_t cap);
 /* Bit-packed frame-of-reference delta page and RLE page decoders (format in oracle/README.md).
  * Return rows written or -1. */
 int64_t oracle_dbp_decode(const uint8_t *src, uint64_t csize, uint8_t *dst, uint64_t cap);

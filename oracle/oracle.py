@@ -65,6 +65,7 @@ def load():
             "oracle_scan_groupby": (i64, [u64, vp, vp, vp, u32, vp, u32, C.c_int32, u32, u64, vp, vp, vp]),
             "oracle_gen_uniform_i32": (None, [u64, u32, u64, C.c_int32, C.c_int32, vp]),
             "oracle_lz4_decode": (i64, [vp, u64, vp, u64]),
+            "oracle_lz4i_decode": (i64, [vp, u64, vp, u64]),
             "oracle_dbp_decode": (i64, [vp, u64, vp, u64]),
             "oracle_rle_decode": (i64, [vp, u64, vp, u64]),
             "oracle_page_decode": (i64, [u32, vp, u64, vp, u64]),

@@ -77,3 +77,21 @@ def lz4lib():
 
 def rng(seed=1234):
     return np.random.default_rng(seed)
+
+
+LZ4I = 5
+
+
+def lz4i_header_bytes(page):
+    """Header bytes of an LZ4I page: {u16 nseq, u8 seg_log2, u8 0}, u16 offsets of every
+    2^seg_log2-th sequence's token, zero padding to a multiple of 16."""
+    page = bytes(page)
+    nseq, sl = page[0] | (page[1] << 8), page[2]
+    nseg = -(-nseq // (1 << sl))
+    return (4 + 2 * (nseg - 1) + 15) // 16 * 16
+
+
+def lz4_block(page, codec):
+    """The standard LZ4 block of an LZ4 (codec 1) or LZ4I (codec 5) page."""
+    page = bytes(page)
+    return page[lz4i_header_bytes(page):] if codec & 0x0F == LZ4I else page

@@ -3,7 +3,8 @@ against the oracle's CPU decode of the same pages: bit-exact bytes, zero-padded 
 import numpy as np
 import pytest
 
-from conftest import rng
+from conftest import LZ4I, lz4_block, rng
+from test_lz4i_cpu import make_page
 from gpu_util import load_liblz4_fixture, say
 
 pytestmark = pytest.mark.gpu
@@ -96,5 +97,84 @@ def test_corrupt_page_reports_ecorrupt(orc, golap, ctx, damage):
         got, _ = ctx.decompress(c2)
         assert got.tobytes() == data.tobytes()
         ctx.free_column(c2)
+    finally:
+        ctx.free_column(col)
+
+
+def _lz4i_cases():
+    r = rng(61)
+    dates = (19920101 + r.integers(0, 2556, 4096 * 12 + 5)).astype(np.int32)
+    yield "dates_4096", dates, 4096
+    yield "dates_8192", dates, 8192
+    yield "lowcard_u8_4096", r.integers(0, 11, 4096 * 12).astype(np.uint8), 4096
+    # long literal runs and long overlapping matches
+    runs = np.repeat(r.integers(0, 3, 3000), r.integers(1, 200, 3000))[:4096 * 8].astype(np.int32)
+    yield "runs_i32_4096", runs, 4096
+    # few matches over a nearly incompressible page: the groups do not fit in place (serial)
+    planted = r.integers(0, 2**31, 4096 * 8).astype(np.int32)
+    for k in range(0, planted.size - 64, 200):  # ten or more sequences per page
+        planted[k + 40:k + 48] = planted[k:k + 8]
+    yield "planted_i32_4096", planted, 4096
+
+
+@pytest.mark.parametrize("name,values,page_rows", list(_lz4i_cases()), ids=[c[0] for c in _lz4i_cases()])
+def test_decode_lz4i_pages_match_oracle(orc, golap, ctx, name, values, page_rows):
+    """LZ4I pages through the decompress kernel's parallel parse (and its serial fallback),
+    against the oracle: as the encoder made them, and re-indexed with 16- and 64-sequence
+    segments (more than 64 segments per page: lanes walk several)."""
+    say("decode lz4i", name, "page_rows", page_rows)
+    hc = golap.HostColumn.encode(values, codec=1, page_rows=page_rows, chunk_rows=page_rows * 2, lz4_min_match=4)
+    a = hc.arrays()
+    d = a["desc"]
+    assert (a["page_codec"] == LZ4I).sum() >= d.n_pages - 1
+    variants = [hc]
+    for sl in (4, 6):
+        pages = []
+        for p in range(d.n_pages):
+            off, cs, code = int(a["page_off"][p]), int(a["page_csize"][p]), int(a["page_codec"][p])
+            pg = a["comp"][off:off + cs].tobytes()
+            pages.append(make_page(lz4_block(pg, code), sl) if code == LZ4I else pg)
+        codes = [LZ4I if c == LZ4I else int(c) & 0x0F for c in a["page_codec"]]
+        variants.append(golap.HostColumn.assemble(golap.TYPE_OF[values.dtype], values.size, page_rows,
+                                                  page_rows * 2, pages, codes))
+    for v in variants:
+        want = orc.decode_column(v.arrays())
+        assert want is not None and want[:values.nbytes].tobytes() == values.tobytes()
+        col = ctx.upload(v)
+        try:
+            got, _ = ctx.decompress(col)
+            assert np.array_equal(got, want)
+        finally:
+            ctx.free_column(col)
+
+
+@pytest.mark.parametrize("damage", ["entry", "count", "pad"])
+@pytest.mark.parametrize("rows", [4096, 64])
+def test_corrupt_lz4i_index_reports_ecorrupt(orc, golap, ctx, damage, rows):
+    """A damaged index is refused by the GPU decoders as by the oracle: by the parallel parse
+    (4,096-row page) and by the serial decoder (a 64-row page of few sequences)."""
+    r = rng(62)
+    v = (19920101 + r.integers(0, 2556, rows)).astype(np.int32)
+    a = golap.HostColumn.encode(v, codec=1, page_rows=rows, chunk_rows=rows, lz4_min_match=4).arrays()
+    assert a["page_codec"][0] == LZ4I
+    b = bytearray(a["comp"][int(a["page_off"][0]):int(a["page_off"][0]) + int(a["page_csize"][0])].tobytes())
+    nseq = b[0] | (b[1] << 8)
+    if damage == "entry":
+        if nseq <= 16:
+            b[2] = 1  # two-sequence segments: the header grows, the block no longer starts there
+        else:
+            b[4] ^= 1
+    elif damage == "count":
+        b[0], b[1] = (nseq - 1) & 0xFF, (nseq - 1) >> 8
+    else:
+        b[3] = 9
+    assert orc.page_decode(LZ4I, bytes(b), v.nbytes)[0] == -1
+    say("corrupt lz4i", damage, "rows", rows, "sequences", nseq)
+    hc = golap.HostColumn.assemble(golap.I32, rows, rows, rows, [bytes(b)], [LZ4I])
+    col = ctx.upload(hc)
+    try:
+        with pytest.raises(golap.GolapError) as e:
+            ctx.decompress(col)
+        assert e.value.status == golap.ECORRUPT
     finally:
         ctx.free_column(col)

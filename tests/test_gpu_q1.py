@@ -45,9 +45,10 @@ PLANS = [  # (year, discount range, quantity range, price range)
 
 
 @pytest.mark.parametrize("plan", range(len(PLANS)))
-@pytest.mark.parametrize("codec,mm,page_rows", [(1, 16, 4096), (1, 4, 1024), (0, 4, 2048), (255, 4, 4096)])
+@pytest.mark.parametrize("codec,mm,page_rows", [(1, 16, 4096), (1, 4, 1024), (0, 4, 2048), (255, 4, 4096),
+                                                 (1, 4, 4096), (1, 4, 8192)])
 def test_q1_matches_oracle(orc, golap, ctx, plan, codec, mm, page_rows):
-    n = 6_000_000 if (plan == 0 and codec == 1 and mm == 16) else 1_000_003
+    n = 6_000_000 if (plan == 0 and codec == 1 and page_rows == 4096) else 1_000_003
     ds = q1_dataset(orc, n)
     year, drng, qrng, prng = PLANS[plan]
     prng = prng or (-2**31, 2**31 - 1)
@@ -161,3 +162,30 @@ def test_ssb_sf10_q1_reproduces_reference_log(golap, ctx, ssb10, q):
         assert r.sum == SF10_GOLDEN[q]
     finally:
         release(ctx, cols, m, bufs)
+
+
+@pytest.mark.parametrize("years,page_rows", [((1995, 1995), 4096), ((1992, 1998), 4096), ((1993, 1994), 2048)])
+def test_q1_hash_probe_modes_match_oracle(orc, golap, ctx, years, page_rows):
+    """A date dimension with keys spread too wide for the membership bitmap: the date probe is
+    the open-addressing table, pinned in LDS (one year qualifies: 365 keys) or in HBM (all seven
+    years: 2,556 keys), on min-match-4 LZ4I pages (the one-wave-per-tile kernel, which probes the
+    date after the quantity and discount tests)."""
+    n = 1_500_017
+    r = np.random.default_rng(77)
+    key = (np.arange(2556, dtype=np.int64) * 997 + 13).astype(np.int32)  # range 2.5M > 2^18
+    year = (1992 + np.arange(2556) // 366).astype(np.int16)
+    ds = dict(d_key=key, d_year=year, date=key[r.integers(0, 2556, n)],
+              disc=r.integers(0, 11, n).astype(np.uint8), qty=r.integers(1, 51, n).astype(np.uint8),
+              price=r.integers(90_000, 10_500_001, n).astype(np.int32))
+    say("q1 hash probe", years, "page_rows", page_rows)
+    cols, hosts = upload_q1(ctx, golap, ds, codec=1, page_rows=page_rows, chunk_rows=262144, lz4_min_match=4)
+    dk, dy = ctx.to_device(key), ctx.to_device(year)
+    m = ctx.hashmap(key.size, dk, golap.I32)
+    ctx.hashmap_build(m, key.size, [(dy, golap.I16, years[0], years[1])])
+    try:
+        res = ctx.q1_run(*cols, m, (4, 6), (30, 50))
+        want, cnt = orc.q1_plan(key, [(year, years[0], years[1])], ds["date"], ds["disc"], ds["qty"], ds["price"],
+                                (4, 6), (30, 50), (-2**31, 2**31 - 1), threads=4)
+        assert (res.sum, res.matches) == (want, cnt)
+    finally:
+        release(ctx, cols, m, [dk, dy])

@@ -1,6 +1,7 @@
 """Serving loop: requests admitted by HBM budget and continuously batched into persistent fused
 launches on several streams. Every result must be the same alone (golap_scan_run /
 golap_star_run) and in the mix, bit-exact."""
+import ctypes
 import threading
 
 import numpy as np
@@ -156,6 +157,23 @@ def test_server_star_batches_with_concurrent_clients(golap, ctx, trip):
             assert same(srv.wait(t), alone[0])
             with pytest.raises(golap.GolapError):
                 srv.wait(t)
+            # two threads waiting on one ticket through the C ABI: exactly one gets the result
+            t = srv.submit_star(**entries[1])
+            dom = srv.shape.pop(t)[1]
+            rcs = []
+
+            def raw_wait():
+                r = golap.StarResult()
+                k, su, cn = np.zeros(dom * 2, np.int32), np.zeros(dom, np.int64), np.zeros(dom, np.uint64)
+                rcs.append(ctx.lib.golap_server_wait(srv.h, t, ctypes.byref(r), k.ctypes.data, su.ctypes.data,
+                                                     cn.ctypes.data, dom, None))
+
+            th = [threading.Thread(target=raw_wait) for _ in range(2)]
+            for x in th:
+                x.start()
+            for x in th:
+                x.join()
+            assert sorted(rcs) == [golap.EINVAL, 0], rcs
         say("star server stats", st)
         for i in range(len(qs)):
             assert same(got[i], alone[i]), i
