@@ -109,7 +109,8 @@ int golap_ctx_create(int device, golap_ctx **out) {
     int lds = 0;
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
         c->lds_max = lds;
-    strncpy(c->name, prop.name, sizeof(c->name) - 1);
+    // the marketing name can be empty on some drivers: then the architecture name
+    strncpy(c->name, prop.name[0] ? prop.name : prop.gcnArchName, sizeof(c->name) - 1);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return GOLAP_EHIP;
     if (hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) return GOLAP_EHIP;
     void *s = nullptr;
@@ -792,12 +793,14 @@ int golap_q1_run(golap_ctx *ctx, const golap_q1_plan *plan, golap_q1_result *res
     a.count = reinterpret_cast<uint64_t *>(ctx->d_scratch + 16);
     GOLAP_HIP_CHECK(ctx, hipMemsetAsync(ctx->d_scratch + 8, 0, 16, ctx->stream));
 
-    bool any_general = false;
+    bool any_general = false, any_direct = true;
     uint32_t max_nseq = 0;
     bool nseq_known = true;
     for (auto *c : cols) {
         auto &ex = c->extra;
-        if (ex.nondirect_prefix[a.page_end] != ex.nondirect_prefix[a.page_begin]) any_general = true;
+        const uint64_t nd = ex.nondirect_prefix[a.page_end] - ex.nondirect_prefix[a.page_begin];
+        if (nd) any_general = true;
+        if (nd == a.page_end - a.page_begin) any_direct = false;  // no tile has all four in place
         max_nseq = std::max(max_nseq, ex.lz4i_max_nseq);
         nseq_known = nseq_known && (ex.lz4i_max_nseq || !ex.has_lz4i);
     }
@@ -805,7 +808,7 @@ int golap_q1_run(golap_ctx *ctx, const golap_q1_plan *plan, golap_q1_result *res
     lz4i_window(d->page_rows * 4, nseq_known ? std::max(max_nseq, 1u) : 0, a.win_cap, a.tok_cap);
     GOLAP_HIP_CHECK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     if (any && a.page_end > a.page_begin) {  // an empty predicate range or map matches nothing
-        GOLAP_HIP_CHECK(ctx, launch_q1(ctx, a, true));
+        if (any_direct) GOLAP_HIP_CHECK(ctx, launch_q1(ctx, a, true));
         if (any_general) GOLAP_HIP_CHECK(ctx, launch_q1(ctx, a, false));
     }
     GOLAP_HIP_CHECK(ctx, hipEventRecord(ctx->ev1, ctx->stream));

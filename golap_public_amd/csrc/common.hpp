@@ -456,7 +456,7 @@ hipError_t launch_q1(golap_ctx *ctx, const Q1Launch &a, bool direct);
 // window holds the output and, at its tail, the staged page (LZ4I pages shrink, so a slack of an
 // eighth of the page, at least 1 KiB, is enough for the in-place order); max_nseq 0 = unknown.
 inline void lz4i_window(uint32_t page_bytes, uint32_t max_nseq, uint32_t &win_cap, uint32_t &tok_cap) {
-    const uint32_t slack = std::max<uint32_t>(1024, page_bytes / 8);
+    const uint32_t slack = std::max<uint32_t>(1024, page_bytes / 8);  // in-place room, see lz4i::decode_par
     win_cap = (page_bytes + slack + 15) & ~15u;
     tok_cap = max_nseq ? max_nseq : std::min<uint32_t>(page_bytes / 4 + 1, std::max<uint32_t>(64, page_bytes / 8));
 }

@@ -16,6 +16,7 @@
 namespace golap_dev {
 
 constexpr int WAVE = 64;
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));  // a native vector type: kept in registers
 
 __device__ __forceinline__ int lane_id() { return __lane_id(); }
 
@@ -23,7 +24,7 @@ __device__ __forceinline__ int lane_id() { return __lane_id(); }
 // All lanes of the wave must call it with the same arguments. With idx (the u16 token offsets of
 // an LZ4I header, format.hpp), the block's own parse must agree with the index: nseq sequences,
 // sequence s << seg_log2 starting at idx[s - 1].
-__device__ inline int64_t lz4_decode_wave(const uint8_t *__restrict__ src, uint32_t csize,
+__device__ __forceinline__ int64_t lz4_decode_wave(const uint8_t *__restrict__ src, uint32_t csize,
                                           uint8_t *__restrict__ dst, uint32_t cap,
                                           const uint8_t *idx = nullptr, uint32_t nseq = 0,
                                           uint32_t seg_log2 = 0) {
@@ -111,16 +112,6 @@ __device__ __forceinline__ uint64_t wave_incl_scan_u64(uint64_t v) {
     return v;
 }
 
-__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v) {
-    const uint32_t lane = lane_id();
-#pragma unroll
-    for (int d = 1; d < WAVE; d <<= 1) {
-        const uint32_t t = __shfl_up(v, d, WAVE);
-        if (lane >= (uint32_t)d) v += t;
-    }
-    return v;
-}
-
 // ------------------------------------------------------------------ LZ4I: parallel parse
 // An LZ4I page (format.hpp) decoded by one wave with every lane busy:
 //   stage   the page is copied into the tail of the LDS window with 16-byte loads; the output
@@ -138,14 +129,21 @@ __device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v) {
 // before it is read: each group's output must end below the group's first input byte, or the
 // decoder hands the page to the serial decoder.
 namespace lz4i {
-constexpr uint32_t LONG = 32;        // per-lane copies below, whole-wave copies from here
+constexpr uint32_t LONG = 64;        // per-lane copies below, whole-wave copies from here
 constexpr uint32_t PAR_MIN = 8;      // fewer sequences: the serial decoder is cheaper
+constexpr uint32_t SINK_BYTES = 4 * 64;  // per-lane 4-byte slots that take the stores of idle lanes
+// LDS after a decode window for pages of up to tok_cap sequences: the token offsets, the sink.
+__host__ __device__ inline uint32_t tok_bytes(uint32_t tok_cap) { return (2 * tok_cap + 15) & ~15u; }
+__host__ __device__ inline uint32_t scratch_bytes(uint32_t win_cap, uint32_t tok_cap) {
+    (void)win_cap;
+    return tok_bytes(tok_cap) + SINK_BYTES;
+}
 
 struct Hdr {
     uint32_t nseq, sl, nseg, hb;
 };
 
-// Header of an LZ4I page in global memory; false if malformed.
+// Header of an LZ4I page (in global memory or LDS); false if malformed.
 __device__ __forceinline__ bool header(const uint8_t *src, uint32_t csize, Hdr &h) {
     if (csize < 16) return false;
     uint32_t w;
@@ -166,6 +164,69 @@ __device__ __forceinline__ bool pad_zero(const uint8_t *src, const Hdr &h) {
 }
 
 __device__ __forceinline__ uint32_t rd16(const uint8_t *p) { return (uint32_t)p[0] | ((uint32_t)p[1] << 8); }
+__device__ __forceinline__ uint32_t ld32u(const uint8_t *p) {
+    uint32_t v;
+    __builtin_memcpy(&v, p, 4);
+    return v;
+}
+__device__ __forceinline__ void st32u(uint8_t *p, uint32_t v) { __builtin_memcpy(p, &v, 4); }
+__device__ __forceinline__ void st16u(uint8_t *p, uint32_t v) {
+    const uint16_t x = (uint16_t)v;
+    __builtin_memcpy(p, &x, 2);
+}
+
+// Inclusive prefix sum over the wave by DPP: row shifts by 1, 2, 4, 8 within each row of 16
+// lanes, then lane 15 of rows 0 and 2 into rows 1 and 3 and lane 31 into rows 2 and 3.
+__device__ __forceinline__ uint32_t wave_scan_dpp(uint32_t v) {
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false);
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false);
+    return v;
+}
+// Maximum and minimum over the wave by the same DPP steps (lane 63 ends with the whole wave's),
+// read back as a uniform value.
+__device__ __forceinline__ uint32_t wave_max_dpp(uint32_t v) {
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xf, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xa, 0xf, false));
+    v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xc, 0xf, false));
+    return __builtin_amdgcn_readlane(v, WAVE - 1);
+}
+__device__ __forceinline__ uint32_t wave_min_dpp(uint32_t v) {
+    return ~wave_max_dpp(~v);
+}
+
+// Exactly n <= 16 bytes from s to d when on; lanes that are off (and the unused stores of the
+// others) write their own 4-byte sink slot instead, so the copy has no branch. All 16 source
+// bytes are loaded before any store: s + 16 <= d, or only the first n bytes of s matter and
+// they lie below d, or s and d are different buffers. Reads up to 16 bytes past s + n.
+__device__ __forceinline__ void copy16(uint8_t *d, const uint8_t *s, uint32_t n, bool on, uint8_t *sink) {
+    const uint32_t v0 = ld32u(s), v1 = ld32u(s + 4), v2 = ld32u(s + 8), v3 = ld32u(s + 12);
+    const uint32_t k = n & ~3u, r = n & 3u;
+    const uint32_t vt = k == 0 ? v0 : k == 4 ? v1 : k == 8 ? v2 : v3;
+    st32u(on && n >= 4 ? d : sink, v0);
+    st32u(on && n >= 8 ? d + 4 : sink, v1);
+    st32u(on && n >= 12 ? d + 8 : sink, v2);
+    st32u(on && n >= 16 ? d + 12 : sink, v3);
+    st16u(on && (r & 2) ? d + k : sink, vt);
+    *(on && (r & 1) ? d + k + (r & 2) : sink) = (uint8_t)(vt >> (8 * (r & 2)));
+}
+
+// n < LONG bytes by chunks of 16 in order (each lane its own n; the loop runs while any lane
+// has bytes left). Valid for copies between different buffers and for matches whose offset is
+// at least 16 or at least n.
+__device__ __forceinline__ void copy_chunks(uint8_t *d, const uint8_t *s, uint32_t n, bool on, uint8_t *sink) {
+    for (uint32_t done = 0; __any(on && n > done); done += 16) {
+        const uint32_t c = n > done ? min(n - done, 16u) : 0u;
+        copy16(d + done, s + done, c, on && c > 0, sink);
+        __builtin_amdgcn_wave_barrier();
+    }
+}
 
 // n bytes from s to d, lane by lane; d below s or at least 4 bytes above it (s + j + 4 <= d + j
 // for a match, so every 4-byte read sees only final bytes or this lane's earlier writes).
@@ -221,7 +282,7 @@ __device__ __forceinline__ void copy_wave(uint8_t *d, const uint8_t *s, uint32_t
 }
 
 // Walks one segment: sequences [seq, seq_end) starting at block offset ip; must end at end_ip.
-// Returns false on any disagreement or overrun.
+// Stores every sequence's token offset. Returns false on any disagreement or overrun.
 __device__ __forceinline__ bool walk(const uint8_t *blk, uint32_t bs, uint32_t ip, uint32_t seq, uint32_t seq_end,
                                      uint32_t end_ip, uint32_t nseq, uint16_t *tok) {
     for (; seq < seq_end; ++seq) {
@@ -260,17 +321,32 @@ __device__ __forceinline__ bool walk(const uint8_t *blk, uint32_t bs, uint32_t i
 
 // The parallel decode of a checked header h: the page src[0, csize) (16-byte aligned, readable
 // up to its next 16-byte boundary) into win[0, osize). win_cap: LDS window bytes (a multiple of
-// 16, at least csize rounded up to 16); tok: nseq entries. Returns osize, -1 if the page is
-// malformed, -2 if its groups cannot be decoded in place (the caller decodes it serially).
-__device__ inline int64_t decode_par(const uint8_t *__restrict__ src, uint32_t csize, const Hdr &h, uint8_t *win,
-                                     uint32_t win_cap, uint32_t osize, uint16_t *tok) {
+// 16, at least csize rounded up to 16), followed in LDS by scratch_bytes(win_cap, tok_cap)
+// bytes (token offsets, sink) that tok points at; nseq <= tok_cap. Returns
+// osize, -1 if the page is malformed, -2 if its groups cannot be decoded in place (the caller
+// decodes it serially).
+__device__ __forceinline__ int64_t decode_par(const uint8_t *__restrict__ src, uint32_t csize, const Hdr &h, uint8_t *win,
+                                     uint32_t win_cap, uint32_t osize, uint16_t *tok, uint32_t tok_cap,
+                                     bool staged = false) {
     const uint32_t lane = lane_id();
+    uint8_t *sink = reinterpret_cast<uint8_t *>(tok) + tok_bytes(tok_cap);
     const uint32_t c16 = (csize + 15) & ~15u;
     const uint32_t P = win_cap - c16;  // staged page
-    {
-        const uint4 *s4 = reinterpret_cast<const uint4 *>(src);
-        uint4 *d4 = reinterpret_cast<uint4 *>(win + P);
-        for (uint32_t i = lane; i < c16 / 16; i += WAVE) d4[i] = s4[i];
+    if (!staged) {
+        // every load in flight before the first store: one memory latency per 16 KiB of page
+        const u32x4 *s4 = reinterpret_cast<const u32x4 *>(src);
+        u32x4 *d4 = reinterpret_cast<u32x4 *>(win + P);
+        const uint32_t n16 = c16 / 16;
+        for (uint32_t i0 = 0; i0 < n16; i0 += 16 * WAVE) {
+            u32x4 v[16];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) v[j] = s4[min(i0 + j * WAVE + lane, n16 - 1)];  // unconditional: in flight together
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                const uint32_t i = i0 + j * WAVE + lane;
+                if (i < n16) d4[i] = v[j];
+            }
+        }
     }
     __builtin_amdgcn_wave_barrier();
     const uint8_t *hdr = win + P;
@@ -286,61 +362,112 @@ __device__ inline int64_t decode_par(const uint8_t *__restrict__ src, uint32_t c
     if (!__all(ok)) return -1;
     __builtin_amdgcn_wave_barrier();
     uint32_t base = 0;
+    uint8_t *slot = sink + 4 * lane;
+    // the headers of group g + 1 are loaded before group g's copies: its input lies above
+    // everything group g writes (the in-place check), so the loads can run ahead
+    uint32_t nip = lane < h.nseq ? tok[lane] : 0u;
+    uint32_t nt = ld32u(blk + nip) & 0xffu;
+    uint32_t now = ld32u(blk + nip + 1 + (nt >> 4));  // the offset word unless the literal length extends
     for (uint32_t g0 = 0; g0 < h.nseq; g0 += WAVE) {
         const uint32_t k = g0 + lane;
         const bool valid = k < h.nseq;
-        uint32_t lit = 0, ml = 0, off = 0, ls = 0;
-        if (valid) {
-            uint32_t ip = tok[k];
-            const uint32_t t = blk[ip++];
-            lit = t >> 4;
+        const bool last = k + 1 == h.nseq;
+        // the sequence header: token, literal length, offset, match length (the walk has
+        // checked every byte of it lies in the block)
+        const uint32_t ip = nip, t = nt;
+        uint32_t lit = valid ? t >> 4 : 0u, ls = ip + 1, ow = now;
+        if (__any(lit == 15)) {
             if (lit == 15) {
                 uint32_t b;
                 do {
-                    b = blk[ip++];
+                    b = blk[ls++];
                     lit += b;
                 } while (b == 255);
-            }
-            ls = ip;
-            ip += lit;
-            if (k + 1 < h.nseq) {
-                off = rd16(blk + ip);
-                ip += 2;
-                ml = t & 15;
-                if (ml == 15) {
-                    uint32_t b;
-                    do {
-                        b = blk[ip++];
-                        ml += b;
-                    } while (b == 255);
-                }
-                ml += golap_fmt::LZ4_MINMATCH;
+                ow = ld32u(blk + ls + lit);
             }
         }
+        const bool has_match = valid && !last;
+        const uint32_t mp = ls + lit;  // offset bytes
+        uint32_t ml = has_match ? (t & 15) : 0u;
+        const uint32_t off = has_match ? (ow & 0xffffu) : 0u;
+        if (__any(ml == 15)) {
+            if (ml == 15) {
+                uint32_t q = mp + 2, b;
+                do {
+                    b = blk[q++];
+                    ml += b;
+                } while (b == 255);
+            }
+        }
+        if (has_match) ml += golap_fmt::LZ4_MINMATCH;
         const uint32_t len = lit + ml;
-        const uint32_t incl = wave_incl_scan_u32(len);
+        const uint32_t incl = wave_scan_dpp(len);
         const uint32_t dst = base + incl - len, mdst = dst + lit;
-        const uint32_t total = __shfl(incl, WAVE - 1, WAVE);
-        const uint32_t in0 = __shfl((uint32_t)(valid ? tok[k] : 0u), 0, WAVE);
+        const uint32_t total = __builtin_amdgcn_readlane(incl, WAVE - 1);
+        const uint32_t in0 = __builtin_amdgcn_readfirstlane(ip);
         // the walk bounds every length by the block size, so these sums do not wrap
         const bool bad = valid && ((uint64_t)dst + len > osize || (ml && (off == 0 || off > mdst)));
         if (__any(bad)) return -1;
-        if (base + total > P + h.hb + in0) return -2;  // would overwrite staged input still to be read
-        if (lit < LONG) copy_lane(win + dst, blk + ls, lit);
-        for (uint64_t lw = __ballot(lit >= LONG); lw; lw &= lw - 1) {
-            const int l = __builtin_ctzll(lw);
-            copy_wave(win + __shfl(dst, l, WAVE), blk + __shfl(ls, l, WAVE), __shfl(lit, l, WAVE), 0);
+        // In place, the group writes [base, base + total) while its input still lies in the
+        // staged page. When that range ends below the group's first input byte, literals are
+        // copied straight from the page. Otherwise short literals are first read into registers
+        // and long ones (LONG bytes or more) copied first, each by the whole wave from its start
+        // forwards: then the writes must only stay below the next group's input, and a long
+        // literal must not start above its own staged bytes.
+        const uint32_t kn = k + WAVE;
+        const bool direct = base + total <= P + h.hb + in0;
+        if (!direct) {
+            const uint32_t next_in = g0 + WAVE < h.nseq ? __builtin_amdgcn_readfirstlane(tok[g0 + WAVE]) : bs;
+            if (base + total > P + h.hb + next_in || __any(lit >= LONG && dst > P + h.hb + ls)) return -2;
+        }
+        nip = kn < h.nseq ? tok[kn] : 0u;
+        nt = ld32u(blk + nip) & 0xffu;
+        now = ld32u(blk + nip + 1 + (nt >> 4));
+        if (direct) {
+            copy_chunks(win + dst, blk + ls, lit, lit < LONG, slot);
+            for (uint64_t lw = __ballot(lit >= LONG); lw; lw &= lw - 1) {
+                const int l = __builtin_ctzll(lw);
+                copy_wave(win + __builtin_amdgcn_readlane(dst, l), blk + __builtin_amdgcn_readlane(ls, l),
+                          __builtin_amdgcn_readlane(lit, l), 0);
+            }
+        } else {
+            const uint32_t sl = lit < LONG ? lit : 0u;
+            const uint32_t nd = wave_max_dpp((sl + 3) >> 2);  // dwords, at most LONG / 4
+            uint32_t lv[LONG / 4];
+#pragma unroll
+            for (int j = 0; j < (int)(LONG / 4); ++j)
+                if ((uint32_t)j < nd) lv[j] = ld32u(blk + ls + 4 * j);
+            __builtin_amdgcn_wave_barrier();
+            for (uint64_t lw = __ballot(lit >= LONG); lw; lw &= lw - 1) {
+                const int l = __builtin_ctzll(lw);
+                copy_wave(win + __builtin_amdgcn_readlane(dst, l), blk + __builtin_amdgcn_readlane(ls, l),
+                          __builtin_amdgcn_readlane(lit, l), 0);
+            }
+#pragma unroll
+            for (int j = 0; j < (int)(LONG / 4); ++j) {
+                if ((uint32_t)j < nd) {
+                    const uint32_t b = 4 * j;
+                    uint8_t *d = win + dst + b;
+                    st32u(b + 4 <= sl ? d : slot, lv[j]);
+                    const uint32_t r = sl - b;  // 1..3 bytes of a last, partial dword
+                    st16u(b < sl && r < 4 && (r & 2) ? d : slot, lv[j]);
+                    *(b < sl && r < 4 && (r & 1) ? d + (r & 2) : slot) = (uint8_t)(lv[j] >> (8 * (r & 2)));
+                }
+            }
         }
         __builtin_amdgcn_wave_barrier();
         bool pend = ml != 0;
+        const uint32_t need = off >= ml ? mdst - off + ml : mdst;
+        const bool chunked = ml < LONG && (off >= 16 || off >= ml);
         for (uint64_t pm = __ballot(pend); pm; pm = __ballot(pend)) {
-            const uint32_t frontier = __shfl(mdst, __builtin_ctzll(pm), WAVE);
-            const uint32_t need = off >= ml ? mdst - off + ml : mdst;
+            const uint32_t frontier = __builtin_amdgcn_readlane(mdst, __builtin_ctzll(pm));
             const bool ready = pend && need <= frontier;
-            if (ready && ml < LONG) match_lane(win + mdst, off, ml);
+            copy_chunks(win + mdst, win + mdst - off, ml, ready && chunked, slot);
+            if (ready && !chunked && ml < LONG) match_lane(win + mdst, off, ml);
             for (uint64_t lw = __ballot(ready && ml >= LONG); lw; lw &= lw - 1) {
                 const int l = __builtin_ctzll(lw);
-                copy_wave(win + __shfl(mdst, l, WAVE), nullptr, __shfl(ml, l, WAVE), __shfl(off, l, WAVE));
+                copy_wave(win + __builtin_amdgcn_readlane(mdst, l), nullptr, __builtin_amdgcn_readlane(ml, l),
+                          __builtin_amdgcn_readlane(off, l));
             }
             pend = pend && !ready;
             __builtin_amdgcn_wave_barrier();
@@ -351,18 +478,41 @@ __device__ inline int64_t decode_par(const uint8_t *__restrict__ src, uint32_t c
 }
 }  // namespace lz4i
 
+// Where decode_par stages a page of csize bytes in a window of win_cap bytes.
+__device__ __forceinline__ uint32_t lz4i_stage_offset(uint32_t win_cap, uint32_t csize) {
+    return win_cap - ((csize + 15) & ~15u);
+}
+
+// An LZ4I page already staged by the caller at win + stage_offset(win_cap, csize) (its bytes in
+// LDS, from registers loaded ahead): header and padding read from LDS, then the parallel parse;
+// the serial fallback reads the page from global memory (src). Returns bytes written or -1.
+__device__ __forceinline__ int64_t lz4i_decode_staged(const uint8_t *__restrict__ src, uint32_t csize, uint8_t *win,
+                                                      uint32_t win_cap, uint32_t osize, uint16_t *tok,
+                                                      uint32_t tok_cap) {
+    const uint8_t *pg = win + lz4i_stage_offset(win_cap, csize);
+    lz4i::Hdr h;
+    if (!lz4i::header(pg, csize, h) || !lz4i::pad_zero(pg, h)) return -1;
+    if (osize > win_cap) return -1;
+    if (h.nseq >= lz4i::PAR_MIN && h.nseq <= tok_cap) {
+        const int64_t r = lz4i::decode_par(src, csize, h, win, win_cap, osize, tok, tok_cap, true);
+        if (r != -2) return r;
+    }
+    return lz4_decode_wave(src + h.hb, csize - h.hb, win, osize, src + 4, h.nseq, h.sl);
+}
+
 // Decodes an LZ4I page src[0, csize) into win[0, osize): in parallel when the page has enough
-// sequences, tok holds them (tok_cap entries) and the window (win_cap bytes, a multiple of 16)
-// holds the staged page; otherwise, and when its groups do not fit in place, serially from
-// global memory. Returns the bytes written or -1.
-__device__ inline int64_t lz4i_decode_wave(const uint8_t *__restrict__ src, uint32_t csize, uint8_t *win,
+// sequences, tok points at lz4i::scratch_bytes(win_cap, tok_cap) bytes of LDS for pages of up to
+// tok_cap sequences and the window (win_cap bytes, a multiple of 16) holds the staged page;
+// otherwise, and when its groups do not fit in place, serially from global memory. Returns the
+// bytes written or -1.
+__device__ __forceinline__ int64_t lz4i_decode_wave(const uint8_t *__restrict__ src, uint32_t csize, uint8_t *win,
                                            uint32_t win_cap, uint32_t osize, uint16_t *tok, uint32_t tok_cap) {
     lz4i::Hdr h;
     if (!lz4i::header(src, csize, h) || !lz4i::pad_zero(src, h)) return -1;
     if (osize > win_cap) return -1;
     if (h.nseq >= lz4i::PAR_MIN && h.nseq <= tok_cap && tok && ((csize + 15) & ~15u) <= win_cap &&
         (reinterpret_cast<uintptr_t>(src) & 15) == 0 && (win_cap & 15) == 0) {
-        const int64_t r = lz4i::decode_par(src, csize, h, win, win_cap, osize, tok);
+        const int64_t r = lz4i::decode_par(src, csize, h, win, win_cap, osize, tok, tok_cap);
         if (r != -2) return r;
     }
     return lz4_decode_wave(src + h.hb, csize - h.hb, win, osize, src + 4, h.nseq, h.sl);
@@ -471,7 +621,7 @@ __device__ inline int64_t rle_decode_wave(const uint8_t *__restrict__ src, uint3
 // Decodes one page of any codec into dst[0, cap). Returns bytes written or -1. LZ4I pages are
 // decoded in parallel when the caller gives a window of win_cap >= cap bytes (a multiple of 16)
 // and room for tok_cap token offsets, serially otherwise.
-__device__ inline int64_t page_decode_wave(uint8_t codec, const uint8_t *__restrict__ src,
+__device__ __forceinline__ int64_t page_decode_wave(uint8_t codec, const uint8_t *__restrict__ src,
                                            uint32_t csize, uint8_t *__restrict__ dst, uint32_t cap,
                                            uint32_t elem, uint32_t win_cap = 0, uint16_t *tok = nullptr,
                                            uint32_t tok_cap = 0) {

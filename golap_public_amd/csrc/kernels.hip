@@ -143,7 +143,7 @@ hipError_t launch_decode_pages(golap_ctx *ctx, const golap_column *col, uint64_t
         a.tok_cap = 0;
     }
     a.err = reinterpret_cast<uint32_t *>(ctx->d_scratch);
-    const size_t lds = (size_t)a.window + ((2 * (size_t)a.tok_cap + 15) & ~(size_t)15);
+    const size_t lds = (size_t)a.window + (a.tok_cap ? lz4i::scratch_bytes(a.window, a.tok_cap) : 0);
     hipError_t e = prepare_lds(ctx, k_decode_pages, lds);
     if (e != hipSuccess) return e;
     int occ = 0;
@@ -710,23 +710,6 @@ struct Q1Lanes {
     }
 };
 
-// The rows of one column of tile p: in place in HBM (*lds = false) or decoded into the window.
-// Returns null for a corrupt page.
-__device__ __forceinline__ const uint8_t *q1_col(const Q1Args &a, const ColView &c, uint64_t p, uint32_t rows,
-                                                 uint8_t *win, uint16_t *tok, bool &in_lds) {
-    const uint8_t code = c.page_codec[p];
-    const uint8_t *src = c.comp + c.page_off[p];
-    const uint32_t csize = c.page_csize[p], nb = rows * c.elem;
-    if (page_direct(code)) {
-        const uint32_t hd = direct_header(code, nb);
-        in_lds = false;
-        return (csize == hd + nb && ((reinterpret_cast<uintptr_t>(src + hd) & 15) == 0)) ? src + hd : nullptr;
-    }
-    in_lds = true;
-    const int64_t got = page_decode_wave(code, src, csize, win, nb, c.elem, a.win_cap, tok, a.tok_cap);
-    return got == (int64_t)nb ? win : nullptr;
-}
-
 template <int G4, bool FULL>
 __device__ __forceinline__ void q1_date_pass(const Q1Args &a, const uint32_t *tab, const uint8_t *dp, uint32_t rows,
                                              Q1Lanes<G4> &st, bool hash) {
@@ -807,21 +790,89 @@ __device__ __forceinline__ void q1_price_pass(const Q1Args &a, const uint8_t *pp
     }
 }
 
-// One pass over a column's rows, in LDS (the window) or in place in HBM: the two calls are
-// separate so that each compiles to its own memory instructions.
-#define Q1_PASS(in_lds, win, ptr, full, CALL)   \
-    do {                                        \
-        if (in_lds) {                           \
-            const uint8_t *q1p_ = (win);        \
-            if (full) CALL(true, q1p_); else CALL(false, q1p_); \
-        } else {                                \
-            const uint8_t *q1p_ = (ptr);        \
-            if (full) CALL(true, q1p_); else CALL(false, q1p_); \
-        }                                       \
-    } while (0)
+// Raw page bytes of one column in registers, loaded ahead of their use: slot j of lane l holds
+// bytes [16 (l + 64 j), +16) of the page (or of an in-place page's rows). n16 = 0 when the page
+// does not fit the slots (then it is read from global memory where it is used).
+template <int NU>
+struct RawPage {
+    u32x4 v[NU];
+    uint32_t n16;
+    // every slot is loaded (indexes past the page read its last 16 bytes again), so the loads
+    // carry no condition and all stay in flight
+    __device__ __forceinline__ void load(const uint8_t *src, uint32_t nbytes) {
+        const uint32_t n = (nbytes + 15) / 16;
+        n16 = n <= (uint32_t)(NU * WAVE) ? n : 0u;
+        const u32x4 *s4 = reinterpret_cast<const u32x4 *>(src);
+        const uint32_t last = n16 ? n16 - 1 : 0u;
+#pragma unroll
+        for (int j = 0; j < NU; ++j) v[j] = s4[min(__lane_id() + (uint32_t)j * WAVE, last)];
+    }
+    __device__ __forceinline__ void store(uint8_t *dst) const {
+        u32x4 *d4 = reinterpret_cast<u32x4 *>(dst);
+#pragma unroll
+        for (int j = 0; j < NU; ++j) {
+            const uint32_t i = __lane_id() + (uint32_t)j * WAVE;
+            if (i < n16) d4[i] = v[j];
+        }
+    }
+};
 
+// One column's page of the current tile (uniform values).
+struct Q1Page {
+    const uint8_t *src;
+    uint32_t csize;
+    uint8_t code;
+};
+__device__ __forceinline__ Q1Page q1_page(const ColView &c, uint64_t p) {
+    return Q1Page{c.comp + c.page_off[p], c.page_csize[p], c.page_codec[p]};
+}
+// The bytes to load ahead for a page: the rows of an in-place page, an LZ4I page whole; other
+// codecs are decoded from global memory (nothing loaded).
+__device__ __forceinline__ void q1_ahead(const Q1Page &pg, uint32_t nb, const uint8_t *&from, uint32_t &bytes) {
+    if (page_direct(pg.code)) {
+        from = pg.src + direct_header(pg.code, nb);
+        bytes = nb;
+    } else {
+        from = pg.src;
+        bytes = (pg.code & PAGE_CODEC_MASK) == golap_fmt::LZ4I ? pg.csize : 0u;
+    }
+}
+
+// The rows of one column of a tile into the window, in two steps so that other loads can be
+// issued between them. Stage: an in-place page's rows and an LZ4I page are copied from their
+// registers into the window (an LZ4I page at its tail). Decode: the staged LZ4I page by the
+// parallel parse; any other coded page from global memory. False for a corrupt page.
+enum : int { Q1_IN_PLACE = 0, Q1_STAGED = 1, Q1_GLOBAL = 2, Q1_BAD = 3 };
+template <int NU>
+__device__ __forceinline__ int q1_stage(const Q1Args &a, const Q1Page &pg, uint32_t nb, const RawPage<NU> &raw,
+                                        uint8_t *win) {
+    if (page_direct(pg.code)) {
+        const uint32_t hd = direct_header(pg.code, nb);
+        if (pg.csize != hd + nb || ((reinterpret_cast<uintptr_t>(pg.src + hd) & 15) != 0) || raw.n16 == 0) return Q1_BAD;
+        raw.store(win);
+        __builtin_amdgcn_wave_barrier();
+        return Q1_IN_PLACE;
+    }
+    if ((pg.code & PAGE_CODEC_MASK) == golap_fmt::LZ4I && raw.n16 && raw.n16 * 16 <= a.win_cap &&
+        (reinterpret_cast<uintptr_t>(pg.src) & 15) == 0) {
+        raw.store(win + lz4i_stage_offset(a.win_cap, pg.csize));
+        __builtin_amdgcn_wave_barrier();
+        return Q1_STAGED;
+    }
+    return Q1_GLOBAL;
+}
+__device__ __forceinline__ bool q1_decode(const Q1Args &a, int how, const Q1Page &pg, uint32_t nb, uint32_t elem,
+                                          uint8_t *win, uint16_t *tok) {
+    if (how == Q1_IN_PLACE) return true;
+    if (how == Q1_BAD) return false;
+    if (how == Q1_STAGED) return lz4i_decode_staged(pg.src, pg.csize, win, a.win_cap, nb, tok, a.tok_cap) == (int64_t)nb;
+    return page_decode_wave(pg.code, pg.src, pg.csize, win, nb, elem, a.win_cap, tok, a.tok_cap) == (int64_t)nb;
+}
+
+// LDS allows at most two of these waves per SIMD, so each may use up to 256 VGPRs (the pages
+// loaded ahead live in registers).
 template <int PROBE, int G4>
-__global__ __launch_bounds__(WAVE) void k_q1_wave(Q1Args a) {
+__global__ __launch_bounds__(WAVE) __attribute__((amdgpu_waves_per_eu(1, 2))) void k_q1_wave(Q1Args a) {
     extern __shared__ __align__(16) uint8_t lds[];
     constexpr bool HASH = PROBE != PROBE_BITMAP;
     const uint32_t *tab;
@@ -841,10 +892,26 @@ __global__ __launch_bounds__(WAVE) void k_q1_wave(Q1Args a) {
     int64_t sum = 0;
     uint32_t cnt = 0;
     const XcdRange xr = xcd_range(a.page_begin, a.page_end);
-    for (uint64_t p = xr.first; p < xr.end; p += xr.stride) {
+    uint64_t p = xr.first;
+    // the next tile's catalog is loaded while the current tile runs
+    Q1Page nd{}, nq{}, ns{}, np{};
+    if (p < xr.end) {
+        nd = q1_page(a.date, p);
+        nq = q1_page(a.qty, p);
+        ns = q1_page(a.disc, p);
+        np = q1_page(a.price, p);
+    }
+    for (; p < xr.end; p += xr.stride) {
+        const Q1Page pd = nd, pq = nq, ps = ns, pp = np;
+        const uint64_t pn = p + xr.stride;
+        if (pn < xr.end) {
+            nd = q1_page(a.date, pn);
+            nq = q1_page(a.qty, pn);
+            ns = q1_page(a.disc, pn);
+            np = q1_page(a.price, pn);
+        }
         if (a.chunk_mask && a.chunk_mask[p / a.pages_per_chunk] == 0) continue;
-        if (page_direct(a.date.page_codec[p]) && page_direct(a.disc.page_codec[p]) &&
-            page_direct(a.qty.page_codec[p]) && page_direct(a.price.page_codec[p]))
+        if (page_direct(pd.code) && page_direct(ps.code) && page_direct(pq.code) && page_direct(pp.code))
             continue;  // k_q1_direct's tile
         const uint32_t rows = (uint32_t)min<uint64_t>(P, a.n_rows - p * P);
         const bool full = rows == (uint32_t)(256 * G4);
@@ -854,43 +921,73 @@ __global__ __launch_bounds__(WAVE) void k_q1_wave(Q1Args a) {
         if (!full)
             for (int b = 0; b < 4 * G4; ++b)
                 if (256 * (b / 4) + 4 * __lane_id() + (b % 4) >= rows) st.clear(b);
-        bool bad = false, in_lds = false;
-        const uint8_t *cp;
-#define Q1_DATE(F, ptr) q1_date_pass<G4, F>(a, tab, ptr, rows, st, HASH)
-#define Q1_QTY(F, ptr) q1_byte_pass<G4, F, false>(ptr, rows, a.qty_lo, a.qty_span, st)
-#define Q1_DISC(F, ptr) q1_byte_pass<G4, F, true>(ptr, rows, a.disc_lo, a.disc_span, st)
-#define Q1_PRICE(F, ptr) q1_price_pass<G4, F>(a, ptr, rows, st, sum, cnt)
-        if (!HASH) {
-            cp = q1_col(a, a.date, p, rows, win, tok, in_lds);
-            if (!(bad = cp == nullptr)) Q1_PASS(in_lds, win, cp, full, Q1_DATE);
+        // every column's bytes are requested before the first is decoded; the price page
+        // reuses the date page's registers once those are in LDS
+        RawPage<G4 + 1> big;
+        RawPage<G4 / 4 + 1> rq, rs;
+        {
+            const uint8_t *f;
+            uint32_t n;
+            q1_ahead(pd, rows * 4, f, n);
+            big.load(f, n);
+            q1_ahead(pq, rows, f, n);
+            rq.load(f, n);
+            q1_ahead(ps, rows, f, n);
+            rs.load(f, n);
         }
+        bool bad = false;
+        // the date column: staged, the price page requested into the same registers (in flight
+        // while the date page decodes), decoded, tested
+#define Q1_DATE_STEP()                                                                           \
+    do {                                                                                         \
+        const int how = q1_stage(a, pd, rows * 4, big, win);                                     \
+        {                                                                                        \
+            const uint8_t *f;                                                                    \
+            uint32_t n;                                                                          \
+            q1_ahead(pp, rows * 4, f, n);                                                        \
+            big.load(f, n);                                                                      \
+        }                                                                                        \
+        bad = !q1_decode(a, how, pd, rows * 4, 4, win, tok);                                     \
+        if (!bad) {                                                                              \
+            if (full) q1_date_pass<G4, true>(a, tab, win, rows, st, HASH);                       \
+            else q1_date_pass<G4, false>(a, tab, win, rows, st, HASH);                           \
+        }                                                                                        \
+    } while (0)
+        if (!HASH) Q1_DATE_STEP();
         if (!bad && __any(st.any())) {
-            cp = q1_col(a, a.qty, p, rows, win, tok, in_lds);
-            if (!(bad = cp == nullptr)) Q1_PASS(in_lds, win, cp, full, Q1_QTY);
+            bad = !q1_decode(a, q1_stage(a, pq, rows, rq, win), pq, rows, 1, win, tok);
+            if (!bad) {
+                if (full) q1_byte_pass<G4, true, false>(win, rows, a.qty_lo, a.qty_span, st);
+                else q1_byte_pass<G4, false, false>(win, rows, a.qty_lo, a.qty_span, st);
+            }
         }
         // the discount bytes are kept in registers, so the window is free for the next column
         if (!bad && __any(st.any())) {
-            cp = q1_col(a, a.disc, p, rows, win, tok, in_lds);
-            if (!(bad = cp == nullptr)) Q1_PASS(in_lds, win, cp, full, Q1_DISC);
+            __builtin_amdgcn_wave_barrier();
+            bad = !q1_decode(a, q1_stage(a, ps, rows, rs, win), ps, rows, 1, win, tok);
+            if (!bad) {
+                if (full) q1_byte_pass<G4, true, true>(win, rows, a.disc_lo, a.disc_span, st);
+                else q1_byte_pass<G4, false, true>(win, rows, a.disc_lo, a.disc_span, st);
+            }
         }
         if (HASH && !bad && __any(st.any())) {
-            cp = q1_col(a, a.date, p, rows, win, tok, in_lds);
-            if (!(bad = cp == nullptr)) Q1_PASS(in_lds, win, cp, full, Q1_DATE);
+            __builtin_amdgcn_wave_barrier();
+            Q1_DATE_STEP();
         }
+#undef Q1_DATE_STEP
         if (!bad && __any(st.any())) {
-            cp = q1_col(a, a.price, p, rows, win, tok, in_lds);
-            if (!(bad = cp == nullptr)) Q1_PASS(in_lds, win, cp, full, Q1_PRICE);
+            __builtin_amdgcn_wave_barrier();
+            bad = !q1_decode(a, q1_stage(a, pp, rows * 4, big, win), pp, rows * 4, 4, win, tok);
+            if (!bad) {
+                if (full) q1_price_pass<G4, true>(a, win, rows, st, sum, cnt);
+                else q1_price_pass<G4, false>(a, win, rows, st, sum, cnt);
+            }
         }
-#undef Q1_DATE
-#undef Q1_QTY
-#undef Q1_DISC
-#undef Q1_PRICE
         if (bad && __lane_id() == 0) raise_err(a.err, DEVERR_CORRUPT);
         __builtin_amdgcn_wave_barrier();  // the window is rewritten by the next tile
     }
     q1_flush(a, sum, cnt);
 }
-#undef Q1_PASS
 
 template <typename K>
 static hipError_t launch_persistent(golap_ctx *ctx, K kernel, const Q1Args &a, size_t lds, int block = BLOCK) {
@@ -912,7 +1009,7 @@ static hipError_t launch_q1_mode(golap_ctx *ctx, Q1Args a, size_t tab, bool dire
     if (!direct) {
         // one wave per tile where the page size allows, else the four-wave general kernel
         a.tab_bytes = (uint32_t)((tab + 15) & ~(size_t)15);
-        const size_t lds = (size_t)a.tab_bytes + a.win_cap + ((2 * (size_t)a.tok_cap + 15) & ~(size_t)15);
+        const size_t lds = (size_t)a.tab_bytes + a.win_cap + lz4i::scratch_bytes(a.win_cap, a.tok_cap);
         if (a.win_cap && lds <= (size_t)ctx->lds_max) {
             switch (a.page_rows) {
             case 1024: return launch_persistent(ctx, k_q1_wave<PROBE, 4>, a, lds, WAVE);
