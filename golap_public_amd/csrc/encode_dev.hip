@@ -13,8 +13,10 @@
 //                    places runs by a wave prefix sum; LZ4 runs the host encoder's greedy
 //                    single-probe parse wave-uniformly (hash table of page positions in LDS,
 //                    matches extended 64 bytes per step by a ballot, literals copied by all
-//                    lanes). AUTO keeps the smallest of LZ4 / DBP / RLE / FOR, ties to the
-//                    earlier. Each page is coded into its own slot of a scratch buffer.
+//                    lanes) and, as the host does, puts a block with matches behind its parse
+//                    index (LZ4I: the token offset of every 16th sequence kept in LDS). AUTO
+//                    keeps the smallest of LZ4 / DBP / RLE / FOR, ties to the earlier. Each
+//                    page is coded into its own slot of a scratch buffer.
 //   k_chunk_minmax   one workgroup per chunk: the zone map.
 //   k_place_pages    after the host has laid the pages out (compute_page_offsets, the host
 //                    encoder's own layout), copies every slot to its place in the column.
@@ -40,6 +42,9 @@ constexpr uint32_t HT_LOG = golap_host::LZ4_HASH_LOG;
 constexpr uint32_t HT_SIZE = 1u << HT_LOG;
 constexpr uint32_t HT_EMPTY = 0xffffu;          // never accepted: a stored position is <= n - 12
 constexpr uint64_t LZ4_DEV_MAX_PAGE = 65536;    // page positions fit 16 bits
+// The LZ4 block is coded this far into its slot, room for the largest LZ4I header in front of
+// it (4 bytes + 63 index entries of 2 bytes, rounded up to 16).
+constexpr uint32_t LZ4I_HDR_MAX = 144;
 
 struct EncodeArgs {
     const uint8_t *src;
@@ -48,7 +53,9 @@ struct EncodeArgs {
     uint8_t *slots;
     uint64_t slot_bytes;
     uint32_t *csize;
+    uint32_t *shift;     // where the page's bytes start in its slot
     uint8_t *codecs;
+    uint32_t *max_nseq;  // most sequences of an LZ4I page (atomic maximum)
 };
 
 __device__ __forceinline__ int64_t load_typed(const uint8_t *p, uint32_t type, uint64_t i) {
@@ -134,12 +141,16 @@ __device__ __forceinline__ void copy_bytes(uint8_t *dst, const uint8_t *src, uin
 // pages. The workgroup is one wave, whose LDS accesses complete in issue order, so the probe
 // needs no barrier between reading and rewriting a table entry. A literal-only block is written
 // `shift` bytes into the slot so that its payload is 16-byte aligned there, as it will be in the
-// column. Returns the block size; *literal_only set.
+// column. The block offset of every 16th sequence's token goes to rec[seq / 16] (lane 0).
+// Returns the block size; literal_only, lit_shift (the literal-only block's offset in out) and
+// nseq (the block's sequences) set.
 __device__ __forceinline__ uint32_t lz4_page(const uint8_t *src, uint32_t n, uint8_t *out, uint32_t min_match,
-                                             uint16_t *ht, bool &literal_only) {
+                                             uint16_t *ht, uint16_t *rec, bool &literal_only, uint32_t &lit_shift,
+                                             uint32_t &nseq) {
     const uint32_t lane = threadIdx.x;
     if (min_match < LZ4_MINMATCH) min_match = LZ4_MINMATCH;
     uint32_t op = 0, anchor = 0, ip = 0;
+    nseq = 0;
     if (n >= LZ4_MFLIMIT + 1) {
         const uint32_t msl = n - LZ4_MFLIMIT, mel = n - LZ4_LASTLITERALS;
         uint32_t misses = 0;
@@ -162,6 +173,8 @@ __device__ __forceinline__ uint32_t lz4_page(const uint8_t *src, uint32_t n, uin
                 }
                 if (len >= min_match) {
                     const uint32_t lit = ip - anchor, ml = len - LZ4_MINMATCH;
+                    if (lane == 0 && (nseq & 15) == 0) rec[nseq >> 4] = (uint16_t)op;
+                    ++nseq;
                     op += put_token(out, op, lit, ml, true);
                     copy_bytes(out + op, src + anchor, lit);
                     op += lit;
@@ -190,10 +203,14 @@ __device__ __forceinline__ uint32_t lz4_page(const uint8_t *src, uint32_t n, uin
     }
     const uint32_t lit = n - anchor;
     literal_only = op == 0;
+    lit_shift = 0;
     if (literal_only) {
         const uint32_t hdr = 1 + len_bytes(lit);
-        out += (PAGE_ALIGN - hdr % PAGE_ALIGN) % PAGE_ALIGN;
+        lit_shift = (PAGE_ALIGN - hdr % PAGE_ALIGN) % PAGE_ALIGN;
+        out += lit_shift;
     }
+    if (lane == 0 && (nseq & 15) == 0) rec[nseq >> 4] = (uint16_t)op;
+    ++nseq;
     op += put_token(out, op, lit, 0, false);
     copy_bytes(out + op, src + anchor, lit);
     return op + lit;
@@ -355,9 +372,39 @@ __device__ void raw_write(const uint8_t *src, uint32_t n, uint8_t *out) {
     for (uint32_t i = n16 * 16 + threadIdx.x; i < n; i += PW) out[i] = src[i];
 }
 
-// Dynamic LDS of the LZ4 variant: the hash table, then the page being parsed.
+// Dynamic LDS of the LZ4 variant: the hash table, the page being parsed, then the token offset
+// of every 16th sequence. Every sequence but the last ends in a match of at least 4 bytes, so a
+// page of n bytes has at most n / 4 + 1 sequences.
 constexpr uint32_t HT_BYTES = HT_SIZE * 2;
-uint32_t encode_lds_bytes(uint64_t page_bytes) { return HT_BYTES + (uint32_t)((page_bytes + 15) / 16 * 16); }
+__host__ __device__ inline uint32_t page_lds_bytes(uint32_t page_bytes) { return (page_bytes + 15) / 16 * 16; }
+__host__ __device__ inline uint32_t rec_lds_bytes(uint32_t page_bytes) {
+    return ((page_bytes / 4 + 1) / 16 + 1) * 2;
+}
+uint32_t encode_lds_bytes(uint64_t page_bytes) {
+    return HT_BYTES + page_lds_bytes((uint32_t)page_bytes) + rec_lds_bytes((uint32_t)page_bytes);
+}
+
+// The LZ4I header (format.hpp) of a block of nseq sequences into hdr (lz4i_header_bytes bytes,
+// ending where the block starts), from the token offsets of every 16th sequence.
+__device__ __forceinline__ uint32_t lz4i_put_header(uint8_t *hdr_end, uint32_t nseq, const uint16_t *rec) {
+    const uint32_t sl = lz4i_seg_log2(nseq), nseg = lz4i_segments(nseq, sl), hb = lz4i_header_bytes(nseq, sl);
+    uint8_t *h = hdr_end - hb;
+    for (uint32_t x = threadIdx.x; x < hb; x += PW) {
+        uint32_t b = 0;
+        if (x == 0) b = nseq & 0xff;
+        else if (x == 1) b = nseq >> 8;
+        else if (x == 2) b = sl;
+        else if (x >= 4) {
+            const uint32_t s = (x - 4) / 2 + 1;  // entry s: the token of sequence s << sl
+            if (s < nseg) {
+                const uint32_t t = rec[s << (sl - LZ4I_MIN_SEG_LOG2)];
+                b = (x & 1) ? t >> 8 : t & 0xff;
+            }
+        }
+        h[x] = (uint8_t)b;
+    }
+    return hb;
+}
 
 // The page's bytes into LDS and the hash table emptied (the wave's loads and stores complete
 // before the barrier's wait).
@@ -378,6 +425,7 @@ __global__ __launch_bounds__(PW) void k_encode_pages(EncodeArgs a) {
     extern __shared__ __align__(16) uint8_t lds[];
     uint16_t *ht = reinterpret_cast<uint16_t *>(lds);
     uint8_t *page = lds + HT_BYTES;
+    uint16_t *rec = reinterpret_cast<uint16_t *>(page + page_lds_bytes(a.page_rows * a.elem));
     for (uint64_t p = blockIdx.x; p < a.n_pages; p += gridDim.x) {
         const uint64_t r0 = p * a.page_rows;
         const uint32_t rows = (uint32_t)min<uint64_t>(a.page_rows, a.n_rows - r0);
@@ -385,7 +433,7 @@ __global__ __launch_bounds__(PW) void k_encode_pages(EncodeArgs a) {
         const uint8_t *src = a.src + r0 * a.elem;
         uint8_t *out = a.slots + p * a.slot_bytes;
         const uint32_t c = a.codec;
-        uint32_t size = 0;
+        uint32_t size = 0, shift = 0, lz4i_nseq = 0;
         uint8_t code = RAW;
         if (c == RAW) {
             raw_write(src, nbytes, out);
@@ -396,11 +444,25 @@ __global__ __launch_bounds__(PW) void k_encode_pages(EncodeArgs a) {
             page_stats(src, a.type, a.elem, rows, auto_ || c == DBP, auto_ || c == FOR, auto_ || c == RLE, s);
             uint64_t best = UINT64_MAX;
             uint32_t pick = 0xff;  // codec whose bytes must be written (LZ4 is written as it is coded)
+            // LZ4 as the host encoder keeps it: one literal run stays a plain LZ4 page, any other
+            // block of up to LZ4I_MAX_BLOCK bytes goes behind its parse index (LZ4I)
             if (LZ && (c == LZ4 || auto_ || (c == DBP && !s.dbp_ok) || (c == FOR && !s.for_ok))) {
                 bool lit_only = false;
+                uint32_t lit_shift = 0, nseq = 0;
                 stage_page(src, nbytes, page, ht);
-                best = lz4_page(page, nbytes, out, a.min_match, ht, lit_only);
+                const uint32_t bsize =
+                    lz4_page(page, nbytes, out + LZ4I_HDR_MAX, a.min_match, ht, rec, lit_only, lit_shift, nseq);
+                best = bsize;
+                shift = LZ4I_HDR_MAX + lit_shift;
                 code = (uint8_t)(LZ4 | (lit_only ? PAGE_LITERAL_ONLY : 0));
+                if (!lit_only && bsize <= LZ4I_MAX_BLOCK) {
+                    __syncthreads();  // lane 0's token offsets
+                    const uint32_t hb = lz4i_put_header(out + LZ4I_HDR_MAX, nseq, rec);
+                    best = hb + bsize;
+                    shift = LZ4I_HDR_MAX - hb;
+                    code = LZ4I;
+                    lz4i_nseq = nseq;
+                }
                 pick = LZ4;
             }
             if ((auto_ || c == DBP) && s.dbp_ok && s.dbp_size < best) {
@@ -418,6 +480,7 @@ __global__ __launch_bounds__(PW) void k_encode_pages(EncodeArgs a) {
                 pick = FOR;
                 code = (uint8_t)(FOR | PAGE_POINT);
             }
+            if (pick != LZ4) shift = 0;
             if (pick == DBP) dbp_write(src, a.type, a.elem, rows, s, out);
             else if (pick == RLE) rle_write(src, a.type, a.elem, rows, s, out);
             else if (pick == FOR) for_write(src, a.type, a.elem, rows, s, out);
@@ -425,7 +488,9 @@ __global__ __launch_bounds__(PW) void k_encode_pages(EncodeArgs a) {
         }
         if (threadIdx.x == 0) {
             a.csize[p] = size;
+            a.shift[p] = shift;
             a.codecs[p] = code;
+            if (code == LZ4I) atomicMax(a.max_nseq, lz4i_nseq);
         }
         __syncthreads();
     }
@@ -462,23 +527,14 @@ __global__ __launch_bounds__(BLOCK) void k_chunk_minmax(const uint8_t *v, uint32
     }
 }
 
-// Offset of a page's bytes in its slot: literal-only LZ4 blocks sit so that their payload is
-// 16-byte aligned (the column places them the same way).
-__device__ __forceinline__ uint32_t slot_shift(uint8_t code, uint32_t nbytes) {
-    if (!(code & PAGE_LITERAL_ONLY)) return 0;
-    const uint32_t hdr = lz4_literal_header_bytes(nbytes);
-    return (PAGE_ALIGN - hdr % PAGE_ALIGN) % PAGE_ALIGN;
-}
-
-// Slot p -> comp + page_off[p]. Source and destination agree modulo 16 (pages start 16-byte
-// aligned, literal-only payloads are aligned in both), so the bulk moves in 16-byte units.
+// Slot p (its bytes from shift[p] on) -> comp + page_off[p]. Source and destination agree
+// modulo 16 (pages start 16-byte aligned, literal-only payloads are aligned in both), so the
+// bulk moves in 16-byte units.
 __global__ __launch_bounds__(BLOCK) void k_place_pages(const uint8_t *slots, uint64_t slot_bytes, const uint64_t *page_off,
-                                                       const uint32_t *csize, const uint8_t *codecs, uint64_t n_pages,
-                                                       uint64_t n_rows, uint32_t page_rows, uint32_t elem,
+                                                       const uint32_t *csize, const uint32_t *shift, uint64_t n_pages,
                                                        uint8_t *comp) {
     for (uint64_t p = blockIdx.x; p < n_pages; p += gridDim.x) {
-        const uint32_t rows = (uint32_t)min<uint64_t>(page_rows, n_rows - p * page_rows);
-        const uint8_t *s = slots + p * slot_bytes + slot_shift(codecs[p], rows * elem);
+        const uint8_t *s = slots + p * slot_bytes + shift[p];
         uint8_t *d = comp + page_off[p];
         const uint32_t n = csize[p];
         const uint32_t head = min<uint32_t>(n, (uint32_t)((PAGE_ALIGN - (reinterpret_cast<uintptr_t>(d) & 15)) & 15));
@@ -494,7 +550,7 @@ __global__ __launch_bounds__(BLOCK) void k_place_pages(const uint8_t *slots, uin
 uint64_t slot_bytes_for(uint32_t codec, uint32_t page_rows, uint32_t elem) {
     const uint64_t nb = (uint64_t)page_rows * elem;
     uint64_t b = nb;
-    if (codec != RAW) b = std::max<uint64_t>(b, golap_host::lz4_bound(nb) + PAGE_ALIGN);
+    if (codec != RAW) b = std::max<uint64_t>(b, LZ4I_HDR_MAX + golap_host::lz4_bound(nb) + PAGE_ALIGN);
     if (codec == DBP || codec == GOLAP_CODEC_AUTO) b = std::max<uint64_t>(b, golap_host::dbp_bound(page_rows));
     if (codec == RLE || codec == GOLAP_CODEC_AUTO) b = std::max<uint64_t>(b, golap_host::rle_bound(page_rows, elem));
     if (codec == FOR || codec == GOLAP_CODEC_AUTO)
@@ -547,10 +603,10 @@ extern "C" int golap_encode_device(golap_ctx *ctx, const void *dev_values, uint6
     h.chunk_min.resize(h.n_chunks);
     h.chunk_max.resize(h.n_chunks);
     const uint64_t slot = slot_bytes_for(o.codec, o.page_rows, elem);
-    // scratch: page slots | csize | codec | chunk min | chunk max
-    const uint64_t off_cs = h.n_pages * slot, off_cd = off_cs + h.n_pages * 4;
+    // scratch: page slots | csize | shift | codec | chunk min | chunk max | LZ4I sequence maximum
+    const uint64_t off_cs = h.n_pages * slot, off_sh = off_cs + h.n_pages * 4, off_cd = off_sh + h.n_pages * 4;
     const uint64_t off_mn = (off_cd + h.n_pages + 15) / 16 * 16, off_mx = off_mn + h.n_chunks * 8;
-    const uint64_t total = off_mx + h.n_chunks * 8;
+    const uint64_t off_nq = off_mx + h.n_chunks * 8, total = off_nq + 16;
     void *scratch = nullptr;
     if (total)
         if (int r = dev_alloc(ctx, total, &scratch)) return r;
@@ -560,6 +616,7 @@ extern "C" int golap_encode_device(golap_ctx *ctx, const void *dev_values, uint6
         if (scratch) dev_release(ctx, scratch);
     };
     float ms1 = 0, ms2 = 0;
+    uint32_t max_nseq = 0;
     hipError_t e = hipSuccess;
     if (h.n_pages) {
         EncodeArgs a{};
@@ -574,10 +631,13 @@ extern "C" int golap_encode_device(golap_ctx *ctx, const void *dev_values, uint6
         a.slots = sb;
         a.slot_bytes = slot;
         a.csize = reinterpret_cast<uint32_t *>(sb + off_cs);
+        a.shift = reinterpret_cast<uint32_t *>(sb + off_sh);
         a.codecs = sb + off_cd;
+        a.max_nseq = reinterpret_cast<uint32_t *>(sb + off_nq);
         const uint32_t grid = (uint32_t)std::min<uint64_t>(h.n_pages, (uint64_t)ctx->cus * 16);
         const uint32_t cgrid = (uint32_t)std::min<uint64_t>(h.n_chunks, (uint64_t)ctx->cus * 8);
-        e = hipEventRecord(ctx->ev0, ctx->stream);
+        e = hipMemsetAsync(a.max_nseq, 0, 4, ctx->stream);
+        if (e == hipSuccess) e = hipEventRecord(ctx->ev0, ctx->stream);
         if (e == hipSuccess) {
             const uint32_t lds = lz ? encode_lds_bytes(page_bytes) : 0;
             if (lds > 65536)
@@ -600,6 +660,7 @@ extern "C" int golap_encode_device(golap_ctx *ctx, const void *dev_values, uint6
         if (e == hipSuccess) e = hipMemcpyAsync(h.page_codec.data(), a.codecs, h.n_pages, hipMemcpyDeviceToHost, ctx->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(h.chunk_min.data(), sb + off_mn, h.n_chunks * 8, hipMemcpyDeviceToHost, ctx->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(h.chunk_max.data(), sb + off_mx, h.n_chunks * 8, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(&max_nseq, a.max_nseq, 4, hipMemcpyDeviceToHost, ctx->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
         if (e == hipSuccess) e = hipEventElapsedTime(&ms1, ctx->ev0, ctx->ev1);
     }
@@ -620,8 +681,8 @@ extern "C" int golap_encode_device(golap_ctx *ctx, const void *dev_values, uint6
         e = hipEventRecord(ctx->ev0, ctx->stream);
         if (e == hipSuccess) {
             hipLaunchKernelGGL(k_place_pages, dim3(grid), dim3(BLOCK), 0, ctx->stream, sb, slot, col->d_page_off,
-                               reinterpret_cast<const uint32_t *>(sb + off_cs), sb + off_cd, h.n_pages, n_rows,
-                               o.page_rows, elem, col->d_comp);
+                               reinterpret_cast<const uint32_t *>(sb + off_cs),
+                               reinterpret_cast<const uint32_t *>(sb + off_sh), h.n_pages, col->d_comp);
             e = hipGetLastError();
         }
         if (e == hipSuccess) e = hipEventRecord(ctx->ev1, ctx->stream);
@@ -634,6 +695,7 @@ extern "C" int golap_encode_device(golap_ctx *ctx, const void *dev_values, uint6
         golap_column_free(ctx, col);
         return fail(ctx, GOLAP_EHIP, std::string("encode_device: ") + hipGetErrorString(e));
     }
+    col->extra.lz4i_max_nseq = max_nseq;
     if (kernel_ms) *kernel_ms = (double)ms1 + (double)ms2;
     *out = col;
     return GOLAP_OK;
