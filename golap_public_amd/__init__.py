@@ -234,6 +234,7 @@ _PROTOS = {
     "golap_star_run_hostdim": (C.c_int, [vp, C.POINTER(StarPlan), u32, vp, u32, C.POINTER(StarResult),
                                          C.POINTER(CoexecStats), vp, vp, vp, u64]),
     "golap_scan_run": (C.c_int, [vp, C.POINTER(ScanPlan), C.POINTER(StarResult), vp, vp, vp, u64]),
+    "golap_scan_run_noalloc": (C.c_int, [vp, C.POINTER(ScanPlan), C.POINTER(StarResult), vp, vp, vp, u64]),
     "golap_batch_create_star": (C.c_int, [vp, C.POINTER(StarQuery), u32, C.POINTER(vp)]),
     "golap_batch_create_scan": (C.c_int, [vp, C.POINTER(ScanPlan), u32, C.POINTER(vp)]),
     "golap_batch_config_get": (C.c_int, [vp, C.POINTER(BatchConfig)]),
@@ -739,8 +740,9 @@ class Context:
             lds_group_slots
         return p
 
-    def scan_run(self, cols, preds=(), keys=(), value_col=-1, max_groups=None, **kw):
-        """Returns (result, keys[n, k], sums, counts); max_groups defaults to the group domain."""
+    def scan_run(self, cols, preds=(), keys=(), value_col=-1, max_groups=None, noalloc=False, **kw):
+        """Returns (result, keys[n, k], sums, counts); max_groups defaults to the group domain.
+        noalloc: through golap_scan_run_noalloc (no device allocation)."""
         p = self.scan_plan(cols, preds, keys, value_col, **kw)
         if max_groups is None:
             max_groups = 1
@@ -751,8 +753,8 @@ class Context:
         sums = np.zeros(max_groups, np.int64)
         cnts = np.zeros(max_groups, np.uint64)
         r = StarResult()
-        self.check(self.lib.golap_scan_run(self.h, C.byref(p), C.byref(r), _ptr(kout), _ptr(sums), _ptr(cnts),
-                                           max_groups), "scan_run")
+        fn = self.lib.golap_scan_run_noalloc if noalloc else self.lib.golap_scan_run
+        self.check(fn(self.h, C.byref(p), C.byref(r), _ptr(kout), _ptr(sums), _ptr(cnts), max_groups), "scan_run")
         n = r.n_groups
         return r, kout[:n * nk].reshape(n, nk), sums[:n], cnts[:n]
 

@@ -117,6 +117,7 @@ int golap_ctx_create(int device, golap_ctx **out) {
     if (dev_alloc(c.get(), golap_ctx::SCRATCH_BYTES, &s) != GOLAP_OK) return GOLAP_ENOMEM;
     c->d_scratch = static_cast<uint8_t *>(s);
     if (hipMemset(c->d_scratch, 0, golap_ctx::SCRATCH_BYTES) != hipSuccess) return GOLAP_EHIP;
+    fixed_scratch_init(c.get());
     *out = c.release();
     return GOLAP_OK;
 }

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cstdint>
 #include <cstring>
+#include <string>
 #include <vector>
 
 #include "common.hpp"
@@ -210,12 +211,15 @@ uint32_t scan_pages_per_item(golap_ctx *ctx, uint64_t surviving_chunks, uint32_t
 }
 
 }  // namespace starhost
-}  // namespace golap_impl
 
-extern "C" {
-
-int golap_scan_run(golap_ctx *ctx, const golap_scan_plan *plan, golap_star_result *res, int32_t *group_keys,
-                   int64_t *sums, uint64_t *counts, uint64_t max_groups) {
+// One scan call. With fixed == nullptr the device memory beyond inputs and outputs comes from
+// the context (its scratch word block, its work-item and group buffers, grown on demand, and a
+// pinned staging page); with fixed set it all comes from fixed->base (fixed->bytes bytes, 256-byte
+// aligned), laid out as: [0, 64) error word and queue counters, [256, 1280) the plan descriptor,
+// then the work items (16 bytes each) and the dense group result (16 bytes per group), and
+// nothing is allocated. Plans needing more than fixed->bytes fail with GOLAP_ECAPACITY.
+int scan_run_impl(golap_ctx *ctx, const golap_scan_plan *plan, golap_star_result *res, int32_t *group_keys,
+                  int64_t *sums, uint64_t *counts, uint64_t max_groups, const ScanFixedScratch *fixed) {
     if (!ctx || !plan || !res) return GOLAP_EINVAL;
     Lock l(ctx->mu);
     if (int r = set_device(ctx)) return r;
@@ -258,46 +262,74 @@ int golap_scan_run(golap_ctx *ctx, const golap_scan_plan *plan, golap_star_resul
     L.rows_per_lane = scan_rows_per_lane(ctx, surv * L.pages_per_chunk,
                                          scan_lds_bytes(L.win_bytes, L.agg_entries, L.page_waves));
     const uint32_t parts = (L.pages_per_chunk + L.pages_per_item - 1) / L.pages_per_item;
+    const uint64_t n_items = std::max<uint64_t>(nch * parts, 1);
     int64_t *d_sum;
     uint64_t *d_cnt;
-    if (int r = group_buffers(ctx, h.domain, &d_sum, &d_cnt)) return r;
-    qd.g_sum = d_sum;
-    qd.g_cnt = d_cnt;
-    L.err = reinterpret_cast<uint32_t *>(ctx->d_scratch);
-    // work list: the descriptor and the queue counters in the context scratch, the items in a
-    // context buffer
-    const uint64_t n_items = std::max<uint64_t>(nch * parts, 1);
-    if (n_items > ctx->scan_items_cap) {
-        dev_release(ctx, ctx->d_scan_items);
-        ctx->d_scan_items = nullptr;
-        ctx->scan_items_cap = 0;
-        const uint64_t cap = std::max<uint64_t>(n_items, 4096);
-        if (int r = dev_alloc(ctx, cap * sizeof(uint4), &ctx->d_scan_items)) return r;
-        ctx->scan_items_cap = cap;
+    uint32_t *err, *queue;
+    ScanQueryDev *dq;
+    uint4 *items;
+    if (fixed) {
+        const uint64_t need = SCAN_FIXED_ITEMS + n_items * sizeof(uint4) + h.domain * 16;
+        if (need > fixed->bytes)
+            return fail(ctx, GOLAP_ECAPACITY, "scan_run_noalloc: the plan needs " + std::to_string(need) +
+                                                  " bytes of scratch (" + std::to_string(n_items) + " work items, " +
+                                                  std::to_string(h.domain) + " groups), the maximum is " +
+                                                  std::to_string(fixed->bytes));
+        uint8_t *b = fixed->base;
+        err = reinterpret_cast<uint32_t *>(b);
+        queue = reinterpret_cast<uint32_t *>(b + 8);
+        dq = reinterpret_cast<ScanQueryDev *>(b + SCAN_FIXED_DESC);
+        items = reinterpret_cast<uint4 *>(b + SCAN_FIXED_ITEMS);
+        d_sum = reinterpret_cast<int64_t *>(b + SCAN_FIXED_ITEMS + n_items * sizeof(uint4));
+        d_cnt = reinterpret_cast<uint64_t *>(d_sum + h.domain);
+        // the array keeps the last call's contents: the error word, the counters and the groups
+        // start at zero; the descriptor goes in as a kernel argument (no staging memory)
+        GOLAP_HIP_CHECK(ctx, launch_fill_zero(b, 64, ctx->stream));
+        GOLAP_HIP_CHECK(ctx, launch_fill_zero(d_sum, h.domain * 16, ctx->stream));
+        qd.g_sum = d_sum;
+        qd.g_cnt = d_cnt;
+        GOLAP_HIP_CHECK(ctx, launch_put_words(dq, &qd, sizeof(qd), ctx->stream));
+    } else {
+        if (int r = group_buffers(ctx, h.domain, &d_sum, &d_cnt)) return r;
+        qd.g_sum = d_sum;
+        qd.g_cnt = d_cnt;
+        err = reinterpret_cast<uint32_t *>(ctx->d_scratch);
+        // work list: the descriptor and the queue counters in the context scratch, the items in a
+        // context buffer
+        if (n_items > ctx->scan_items_cap) {
+            dev_release(ctx, ctx->d_scan_items);
+            ctx->d_scan_items = nullptr;
+            ctx->scan_items_cap = 0;
+            const uint64_t cap = std::max<uint64_t>(n_items, 4096);
+            if (int r = dev_alloc(ctx, cap * sizeof(uint4), &ctx->d_scan_items)) return r;
+            ctx->scan_items_cap = cap;
+        }
+        if (!ctx->h_stage &&
+            hipHostMalloc(reinterpret_cast<void **>(&ctx->h_stage), 4096, hipHostMallocDefault) != hipSuccess) {
+            ctx->h_stage = nullptr;
+            return fail(ctx, GOLAP_ENOMEM, "scan: pinned staging");
+        }
+        dq = reinterpret_cast<ScanQueryDev *>(ctx->d_scratch + 1024);
+        queue = reinterpret_cast<uint32_t *>(ctx->d_scratch + 2304);
+        items = static_cast<uint4 *>(ctx->d_scan_items);
+        // the staging page is reused by the next call only after this call's final synchronisation
+        GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        memcpy(ctx->h_stage, &qd, sizeof(qd));
+        GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(dq, ctx->h_stage, sizeof(qd), hipMemcpyHostToDevice, ctx->stream));
+        GOLAP_HIP_CHECK(ctx, launch_fill_zero(queue, 8, ctx->stream));
     }
-    if (!ctx->h_stage && hipHostMalloc(reinterpret_cast<void **>(&ctx->h_stage), 4096, hipHostMallocDefault) != hipSuccess) {
-        ctx->h_stage = nullptr;
-        return fail(ctx, GOLAP_ENOMEM, "scan: pinned staging");
-    }
-    ScanQueryDev *dq = reinterpret_cast<ScanQueryDev *>(ctx->d_scratch + 1024);
-    uint32_t *queue = reinterpret_cast<uint32_t *>(ctx->d_scratch + 2304);
-    // the staging page is reused by the next call only after this call's final synchronisation
-    GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    memcpy(ctx->h_stage, &qd, sizeof(qd));
-    GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(dq, ctx->h_stage, sizeof(qd), hipMemcpyHostToDevice, ctx->stream));
-    GOLAP_HIP_CHECK(ctx, launch_fill_zero(queue, 8, ctx->stream));
+    L.err = err;
     L.queries = dq;
     L.queue = queue;
-    L.items = static_cast<const uint4 *>(ctx->d_scan_items);
+    L.items = items;
     GOLAP_HIP_CHECK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     if (nch) {
-        GOLAP_HIP_CHECK(ctx, launch_scan_items(dq, 1, plan->chunk_begin, nch, parts, static_cast<uint4 *>(ctx->d_scan_items),
-                                               queue, ctx->stream));
+        GOLAP_HIP_CHECK(ctx, launch_scan_items(dq, 1, plan->chunk_begin, nch, parts, items, queue, ctx->stream));
         GOLAP_HIP_CHECK(ctx, launch_scan(ctx, L, ctx->stream));
     }
     GOLAP_HIP_CHECK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     uint32_t flags = 0;
-    GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(&flags, ctx->d_scratch, 4, hipMemcpyDeviceToHost, ctx->stream));
+    GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(&flags, err, 4, hipMemcpyDeviceToHost, ctx->stream));
     GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     if (flags) return error_flags_status(ctx, flags, "scan");
     float ms = 0;
@@ -307,6 +339,15 @@ int golap_scan_run(golap_ctx *ctx, const golap_scan_plan *plan, golap_star_resul
     scan_stats(*plan, qd.colmask, run, res);
     return emit_groups(ctx, h.domain, d_sum, d_cnt, h.lo, h.card, h.stride, &res->n_groups, &res->total_sum,
                        group_keys, sums, counts, max_groups, &res->rows_matched);
+}
+
+}  // namespace golap_impl
+
+extern "C" {
+
+int golap_scan_run(golap_ctx *ctx, const golap_scan_plan *plan, golap_star_result *res, int32_t *group_keys,
+                   int64_t *sums, uint64_t *counts, uint64_t max_groups) {
+    return scan_run_impl(ctx, plan, res, group_keys, sums, counts, max_groups, nullptr);
 }
 
 }  // extern "C"

@@ -49,6 +49,11 @@ struct golap_ctx {
     void *d_scan_items = nullptr;
     uint64_t scan_items_cap = 0;
     uint8_t *h_stage = nullptr;  // pinned host staging for small uploads (4 KiB)
+    // the allocation-free scan's device array on this context's device (its address there),
+    // or the error met looking it up
+    uint8_t *d_fixed = nullptr;
+    uint64_t fixed_bytes = 0;
+    hipError_t fixed_err = hipSuccess;
 };
 
 // Host planning tables of a device column, from its host catalog: launches are planned from
@@ -282,6 +287,10 @@ hipError_t launch_batch_items(const void *qbase, uint32_t qstride, uint32_t n_qu
                               uint64_t total_chunks, uint32_t parts, uint2 *items, uint32_t *queue,
                               hipStream_t stream);
 hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t stream);
+// Copies bytes (a multiple of 4, at most PUT_WORDS_MAX) from host memory to dst (4-byte aligned)
+// as a kernel argument: no staging memory, a kernel node under capture.
+constexpr uint32_t PUT_WORDS_MAX = 2048;
+hipError_t launch_put_words(void *dst, const void *src, uint32_t bytes, hipStream_t stream);
 // Non-empty groups of a batch's dense results (query q's sums at arena + qoff[q].x, its counts
 // qoff[q].y entries later; qfirst[q] = its first index of the concatenated domains, total in
 // all): {q, group} + sum + count appended in any order, *n_out counted (also beyond cap).
@@ -460,5 +469,19 @@ inline void lz4i_window(uint32_t page_bytes, uint32_t max_nseq, uint32_t &win_ca
     win_cap = (page_bytes + slack + 15) & ~15u;
     tok_cap = max_nseq ? max_nseq : std::min<uint32_t>(page_bytes / 4 + 1, std::max<uint32_t>(64, page_bytes / 8));
 }
+
+// Device memory handed to one scan call in place of the context's buffers (the allocation-free
+// entry point): base is 256-byte aligned; the layout is fixed by scan_run_impl.
+struct ScanFixedScratch {
+    uint8_t *base;
+    uint64_t bytes;
+};
+constexpr uint64_t SCAN_FIXED_DESC = 256;                                  // the plan descriptor
+constexpr uint64_t SCAN_FIXED_ITEMS = SCAN_FIXED_DESC + 1024;              // then the work items
+int scan_run_impl(golap_ctx *ctx, const golap_scan_plan *plan, golap_star_result *res, int32_t *group_keys,
+                  int64_t *sums, uint64_t *counts, uint64_t max_groups, const ScanFixedScratch *fixed);
+// Looks up the allocation-free scan's device array on ctx's device (which must be current) and
+// keeps its address in ctx (ctx->fixed_err on failure). Called once by golap_ctx_create.
+void fixed_scratch_init(golap_ctx *ctx);
 
 }  // namespace golap_impl

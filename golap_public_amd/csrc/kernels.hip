@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstring>
 
 #include "common.hpp"
 #include "device_codecs.hpp"
@@ -204,6 +205,22 @@ hipError_t launch_fill_zero(void *p, uint64_t bytes, hipStream_t stream) {
     const uint64_t n = bytes / 8;
     const uint32_t grid = (uint32_t)std::min<uint64_t>((n + BLOCK - 1) / BLOCK, 4096);
     hipLaunchKernelGGL(k_fill_zero, dim3(std::max(grid, 1u)), dim3(BLOCK), 0, stream, static_cast<uint64_t *>(p), n);
+    return hipGetLastError();
+}
+
+struct PutWords {
+    uint32_t w[PUT_WORDS_MAX / 4];
+};
+__global__ void k_put_words(uint32_t *dst, PutWords v, uint32_t n) {
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) dst[i] = v.w[i];
+}
+
+hipError_t launch_put_words(void *dst, const void *src, uint32_t bytes, hipStream_t stream) {
+    if (bytes % 4 || bytes > PUT_WORDS_MAX || reinterpret_cast<uintptr_t>(dst) % 4) return hipErrorInvalidValue;
+    if (!bytes) return hipSuccess;
+    PutWords v;
+    memcpy(v.w, src, bytes);
+    hipLaunchKernelGGL(k_put_words, dim3(1), dim3(BLOCK), 0, stream, static_cast<uint32_t *>(dst), v, bytes / 4);
     return hipGetLastError();
 }
 

@@ -477,6 +477,20 @@ typedef struct {
 int golap_scan_run(golap_ctx *ctx, const golap_scan_plan *plan, golap_star_result *res, int32_t *group_keys,
                    int64_t *sums, uint64_t *counts, uint64_t max_groups);
 
+/* golap_scan_run without device allocation. No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Arguments, results and errors are golap_scan_run's. The
+ * device memory the call needs beyond its inputs and outputs (error word, work-list counters,
+ * plan descriptor, work items, dense group result) is one library array of
+ * GOLAP_NOALLOC_SCRATCH_BYTES per device, held until the process ends; a plan needing more
+ * (1,280 bytes + 16 per work item + 16 per group of the key domain) fails with GOLAP_ECAPACITY
+ * and runs nothing. The array keeps the last call's contents and is cleared where needed by
+ * each call. All calls on one GPU share it: calls that overlap on one GPU would race, so the
+ * library serialises them within a process (a call returns with its work finished). */
+#define GOLAP_NOALLOC_SCRATCH_BYTES (8u << 20)
+int golap_scan_run_noalloc(golap_ctx *ctx, const golap_scan_plan *plan, golap_star_result *res,
+                           int32_t *group_keys, int64_t *sums, uint64_t *counts, uint64_t max_groups);
+
 /* ------------------------------------------------------------------ batched plans
  * Many plans of one kind over one fact table in one persistent fused launch. The reference
  * runs one plan per call, with a worker loop over chunks (query2/query3, 02.hpp:54-307,
