@@ -2,13 +2,17 @@
 
 Workload (one "step" = one query): 600,000,000 fact rows (date_key int32 over 2,556 days,
 discount uint8 U[0,10], quantity uint8 U[1,50], price_cents int32 U[90000,10500000]) stored as
-LZ4 pages in 262,144-row chunks, resident in HBM. Each step builds the date hash table
+LZ4 pages (minimum match 4, as LZ4_compress_default codes them; pages with matches carry a parse
+index) in 262,144-row chunks, resident in HBM. Each step builds the date hash table
 (year == 1995) on the device and runs the fused kernel for
 discount in [4,6] and quantity >= 30, SUM(price*discount) as int64.
 
-Prints one JSON line (see README of the driver contract): rows/s over the whole job, the
-roofline of the fused kernel (algorithmic bytes / kernel time, against 8 TB/s), and the CPU
-path (oracle decode + plan on the host cores) timed on the same data.
+Prints one JSON line: rows/s over the whole job, the roofline of the fused kernel (algorithmic
+bytes / kernel time, against 8 TB/s), the CPU path (oracle decode + plan on the host cores, and
+decode alone by the oracle and by liblz4) timed on the same pages, and as secondary fields the
+same plan over minimum-match-16 pages and a serving sweep of the trip-table workload (served
+and plain queries/s and p50/p99 at 0.5x and 0.9x the plain capacity and 0.9x the served one).
+Other workloads: --workload star | batch | serve | q1_files | encode.
 """
 import argparse
 import json
@@ -42,16 +46,19 @@ def parse():
     ap.add_argument("--queries", type=int, default=64, help="batch workload: queries per batch")
     ap.add_argument("--zone-rows", type=int, default=65536, help="batch workload: rows per zone-map chunk")
     ap.add_argument("--trip-rows", type=int, default=1_500_000_000, help="serve workload: trip-table rows")
-    ap.add_argument("--requests", type=int, default=1000, help="serve workload: arrivals per run")
-    ap.add_argument("--qps", type=float, default=0.0,
-                    help="serve workload: arrival rate; 0 = 4 x the measured plain one-at-a-time rate")
+    ap.add_argument("--requests", type=int, default=400, help="serve workload: arrivals per rate")
+    ap.add_argument("--sweep-requests", type=int, default=300,
+                    help="q1 workload: requests per rate of the serving sweep reported beside it")
+    ap.add_argument("--no-serve-sweep", action="store_true", help="q1 workload: skip the serving sweep")
+    ap.add_argument("--no-mm16", action="store_true", help="q1 workload: skip the min-match 16 run")
     ap.add_argument("--budget-mb", type=int, default=4096, help="serve workload: HBM admission budget")
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--rows", type=int, default=600_000_000)
     ap.add_argument("--page-rows", type=int, default=4096)
     ap.add_argument("--chunk-rows", type=int, default=262144)
-    ap.add_argument("--min-match", type=int, default=16)
+    ap.add_argument("--min-match", type=int, default=4,
+                    help="LZ4 minimum match (4: what LZ4_compress_default emits)")
     ap.add_argument("--seed", type=int, default=20240601)
     ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -97,10 +104,32 @@ def cpu_baseline(args, hosts, key, year, gpu_sum):
     except Exception:
         pass
     log(f"cpu baseline: {threads} threads, median {med*1e3:.1f} ms, sum {res} (gpu {gpu_sum})")
+    # decode alone, by the oracle's decoders and by liblz4 (the reference's CPU codec), on the
+    # same pages: the checksums of the decoded bytes must agree
+    addrs = [(h.desc().comp, h.desc().page_off, h.desc().page_csize, h.desc().page_codec) for h in hosts]
+    elems = [4, 1, 1, 4]
+    raw = d0.n_rows * sum(elems)
+    decode = {}
+    for name, use_lib in (("oracle", False), ("liblz4", True)):
+        runs = []
+        for _ in range(3):
+            st, cs, sec = oracle.decode_timing(addrs, elems, d0.n_rows, d0.page_rows, use_lib, threads)
+            if st != 0:
+                break
+            runs.append((sec, cs))
+        if runs:
+            sec = statistics.median(r[0] for r in runs)
+            decode[name] = {"rows_per_s": d0.n_rows / sec, "decoded_GBps": raw / sec / 1e9, "ms": sec * 1e3,
+                            "checksum": runs[-1][1]}
+            log(f"cpu decode ({name}): {sec*1e3:.1f} ms, {raw/sec/1e9:.2f} GB/s decoded")
+        else:
+            decode[name] = {"status": st}
+    if "checksum" in decode.get("oracle", {}) and "checksum" in decode.get("liblz4", {}):
+        decode["checksums_equal"] = decode["oracle"]["checksum"] == decode["liblz4"]["checksum"]
     return {"value": d0.n_rows / med, "unit": "rows/s", "cores": threads, "kind": "port",
             "sample": f"full workload ({d0.n_rows} rows, same compressed pages), median of "
                       f"{args.cpu_runs} runs, CPU: {model}",
-            "bit_exact_vs_gpu": bool(res == gpu_sum)}
+            "bit_exact_vs_gpu": bool(res == gpu_sum), "decode_only_median_of_3": decode}
 
 
 def run_star(args):
@@ -440,129 +469,174 @@ def trip_request(rng, t0):
     return trip_template(t, h0, span, t0)
 
 
-def run_serve(args):
-    """Serving form on the trip table: Poisson arrivals at a fixed rate drawn from four request
-    templates, served by the request loop (HBM-budget admission, continuous batching into
-    persistent fused launches on two streams), against one-query-at-a-time plain launches
-    (golap_scan_run) at the same arrival times. Latency = arrival to results on the host."""
-    import ctypes as C
-    import threading
-    import golap_public_amd as g
-    n = args.trip_rows
-    log(f"generating {n} trip rows")
-    t_setup = time.perf_counter()
-    cols = g.gen_trip(args.seed, n)
-    log(f"encoding ({time.perf_counter()-t_setup:.1f} s)")
-    enc = dict(page_rows=args.page_rows, chunk_rows=args.zone_rows, lz4_min_match=args.min_match)
-    codecs = [g.CODEC_DBP, g.CODEC_LZ4, g.CODEC_FOR, g.CODEC_FOR, g.CODEC_FOR]
-    hosts = []
-    for c, k in zip(cols, codecs):
-        hosts.append(g.HostColumn.encode(c, codec=k, **enc))
-    cpu_rows = 0 if args.no_cpu_baseline else min(n, 150_000_000)
-    cpu_cols = [c[:cpu_rows].copy() for c in cols]  # the CPU baseline's bounded sample
-    del cols
-    ctx = g.Context(0)
-    dev = [ctx.upload(h) for h in hosts]
-    comp = sum(h.desc().payload_bytes for h in hosts)
-    del hosts
-    log(f"compressed {comp/1e9:.3f} GB for {n*19/1e9:.3f} GB raw; setup {time.perf_counter()-t_setup:.1f} s")
-    rng = np.random.default_rng(args.seed)
-    reqs = [dict(cols=dev, **trip_request(rng, g.TRIP_T0)) for _ in range(args.requests)]
-    plans = [ctx.scan_plan(**r) for r in reqs]
-    doms = [int(np.prod([k[3] for k in r["keys"]])) for r in reqs]
-    maxd = max(doms)
-    lib = ctx.lib
+class ServeRig:
+    """The serving workload: the trip table resident in HBM (generated on the host, coded on the
+    device by golap_encode_device, byte-identical to the host encoder), seeded requests drawn
+    from the four templates, and the two ways of running them: the request loop (HBM-budget
+    admission, continuous batching into persistent fused launches on two streams) and
+    one-query-at-a-time plain launches (golap_scan_run). Latency = arrival to results on the
+    host."""
 
-    def bufs():
-        return (np.empty(maxd * 2, np.int32), np.empty(maxd, np.int64), np.empty(maxd, np.uint64))
+    OPTS = dict(slots=2, max_batch=64, emit_threads=8)
 
-    pb = bufs()
+    def __init__(self, args, ctx, n_requests, keep_cpu_rows=0):
+        import golap_public_amd as g
+        self.g, self.ctx, self.args = g, ctx, args
+        n = args.trip_rows
+        t_setup = time.perf_counter()
+        log(f"serve: generating {n} trip rows")
+        cols = g.gen_trip(args.seed, n)
+        enc = dict(page_rows=args.page_rows, chunk_rows=args.zone_rows, lz4_min_match=args.min_match)
+        codecs = [g.CODEC_DBP, g.CODEC_LZ4, g.CODEC_FOR, g.CODEC_FOR, g.CODEC_FOR]
+        self.cpu_cols = [c[:keep_cpu_rows].copy() for c in cols] if keep_cpu_rows else None
+        self.dev, self.comp = [], 0
+        for c, k in zip(cols, codecs):
+            dv = ctx.to_device(c)
+            col, _ = ctx.encode_device(dv, c.size, g.TYPE_OF[c.dtype], codec=k, **enc)
+            ctx.free(dv)
+            self.dev.append(col)
+            self.comp += ctx.column_desc(col).payload_bytes
+        del cols
+        self.n = n
+        log(f"serve: compressed {self.comp/1e9:.3f} GB for {n*19/1e9:.3f} GB raw; setup "
+            f"{time.perf_counter()-t_setup:.1f} s")
+        rng = np.random.default_rng(args.seed)
+        self.reqs = [dict(cols=self.dev, **trip_request(rng, g.TRIP_T0)) for _ in range(n_requests)]
+        self.plans = [ctx.scan_plan(**r) for r in self.reqs]
+        self.doms = [int(np.prod([k[3] for k in r["keys"]])) for r in self.reqs]
+        self.maxd = max(self.doms)
+        self.pb = self.bufs()
+        for i in range(min(8, n_requests)):  # warm-up
+            self.plain(i)
 
-    def plain(i):
+    def bufs(self):
+        return (np.empty(self.maxd * 2, np.int32), np.empty(self.maxd, np.int64), np.empty(self.maxd, np.uint64))
+
+    def plain(self, i):
+        import ctypes as C
+        g, ctx = self.g, self.ctx
         r = g.StarResult()
-        ctx.check(lib.golap_scan_run(ctx.h, C.byref(plans[i]), C.byref(r), g._ptr(pb[0]), g._ptr(pb[1]),
-                                     g._ptr(pb[2]), doms[i]), "scan_run")
+        ctx.check(ctx.lib.golap_scan_run(ctx.h, C.byref(self.plans[i]), C.byref(r), g._ptr(self.pb[0]),
+                                         g._ptr(self.pb[1]), g._ptr(self.pb[2]), self.doms[i]), "scan_run")
         return r
 
-    for i in range(min(8, len(reqs))):
-        plain(i)
-    # plain one-at-a-time capacity, back to back
-    k = min(len(reqs), 200)
-    t0 = time.perf_counter()
-    for i in range(k):
-        plain(i)
-    plain_rate = k / (time.perf_counter() - t0)
-    qps = args.qps or 4.0 * plain_rate
-    arrivals = np.cumsum(np.random.default_rng(args.seed + 1).exponential(1.0 / qps, len(reqs)))
-    log(f"plain capacity {plain_rate:.1f} q/s; offered load {qps:.1f} q/s, {len(reqs)} requests")
+    def plain_capacity(self, k=200):
+        """One-at-a-time plain launches back to back (queries/s)."""
+        k = min(k, len(self.reqs))
+        t0 = time.perf_counter()
+        for i in range(k):
+            self.plain(i)
+        return k / (time.perf_counter() - t0)
 
-    # served
-    opts = dict(hbm_budget=args.budget_mb << 20, slots=2, max_batch=64, emit_threads=8)
-    lat_srv = [0.0] * len(reqs)
-    res_srv = [None] * len(reqs)
-    with ctx.server(**opts) as srv:
-        tickets = [None] * len(reqs)
+    def served(self, arrivals):
+        """Requests 0..len(arrivals)-1 submitted at their arrival times (seconds from the start;
+        all at 0 for a closed burst). Returns (latencies ms, results, wall s, server stats)."""
+        import ctypes as C
+        import threading
+        g, ctx = self.g, self.ctx
+        m = len(arrivals)
+        lat, res, chunks = [0.0] * m, [None] * m, [0] * m
+        tickets, sub_delay = [None] * m, [0.0] * m
+        with ctx.server(hbm_budget=self.args.budget_mb << 20, **self.OPTS) as srv:
+            def waiter(w, nw):
+                b = self.bufs()
+                for i in range(w, m, nw):
+                    while tickets[i] is None:
+                        time.sleep(0.0001)
+                    r, lt = g.StarResult(), C.c_double()
+                    ctx.check(ctx.lib.golap_server_wait(srv.h, tickets[i], C.byref(r), g._ptr(b[0]), g._ptr(b[1]),
+                                                        g._ptr(b[2]), self.doms[i], C.byref(lt)), "server_wait")
+                    lat[i] = lt.value + sub_delay[i]
+                    res[i] = (r.n_groups, r.total_sum, r.rows_matched)
+                    chunks[i] = r.chunks_run
 
-        def waiter(w, nw):
-            b = bufs()
-            for i in range(w, len(reqs), nw):
-                while tickets[i] is None:
-                    time.sleep(0.0001)
-                r, lat = g.StarResult(), C.c_double()
-                ctx.check(lib.golap_server_wait(srv.h, tickets[i], C.byref(r), g._ptr(b[0]), g._ptr(b[1]),
-                                                g._ptr(b[2]), doms[i], C.byref(lat)), "server_wait")
-                lat_srv[i] = lat.value + sub_delay[i]
-                res_srv[i] = (r.n_groups, r.total_sum, r.rows_matched)
-                chunks_srv[i] = r.chunks_run
+            nw = 8
+            ws = [threading.Thread(target=waiter, args=(w, nw)) for w in range(nw)]
+            for w in ws:
+                w.start()
+            start = time.perf_counter()
+            for i in range(m):
+                now = time.perf_counter() - start
+                if arrivals[i] > now:
+                    time.sleep(arrivals[i] - now)
+                t = C.c_uint64()
+                ctx.check(ctx.lib.golap_server_submit_scan(srv.h, C.byref(self.plans[i]), C.byref(t)), "server_submit")
+                sub_delay[i] = max(0.0, (time.perf_counter() - start) - arrivals[i]) * 1e3  # late submits count
+                tickets[i] = t.value
+            for w in ws:
+                w.join()
+            wall = time.perf_counter() - start
+            st = srv.stats()
+        self.chunks = chunks
+        return lat, res, wall, st
 
-        sub_delay = [0.0] * len(reqs)
-        chunks_srv = [0] * len(reqs)
-        nw = 8
-        ws = [threading.Thread(target=waiter, args=(w, nw)) for w in range(nw)]
-        for w in ws:
-            w.start()
+    def plain_at(self, arrivals, want):
+        """Plain launches at the same arrivals (one worker, arrival order), each result checked
+        against the served one. Returns (latencies ms, wall s)."""
+        lat = [0.0] * len(arrivals)
         start = time.perf_counter()
-        for i in range(len(reqs)):
+        for i in range(len(arrivals)):
             now = time.perf_counter() - start
             if arrivals[i] > now:
                 time.sleep(arrivals[i] - now)
-            t = C.c_uint64()
-            ctx.check(lib.golap_server_submit_scan(srv.h, C.byref(plans[i]), C.byref(t)), "server_submit")
-            sub_delay[i] = max(0.0, (time.perf_counter() - start) - arrivals[i]) * 1e3  # late submits count
-            tickets[i] = t.value
-        for w in ws:
-            w.join()
-        srv_wall = time.perf_counter() - start
-        st = srv.stats()
-    # plain at the same arrivals: one worker, arrival order
-    lat_pl = [0.0] * len(reqs)
-    start = time.perf_counter()
-    for i in range(len(reqs)):
-        now = time.perf_counter() - start
-        if arrivals[i] > now:
-            time.sleep(arrivals[i] - now)
-        r = plain(i)
-        lat_pl[i] = (time.perf_counter() - start - arrivals[i]) * 1e3
-        assert (r.n_groups, r.total_sum, r.rows_matched) == res_srv[i], f"request {i} differs"
-    pl_wall = time.perf_counter() - start
-    srv_qps, pl_qps = len(reqs) / srv_wall, len(reqs) / pl_wall
-    p = lambda a, q: float(np.percentile(np.array(a), q))
-    log(f"served {srv_qps:.1f} q/s p50 {p(lat_srv, 50):.2f} ms p99 {p(lat_srv, 99):.2f} ms; plain {pl_qps:.1f} q/s "
-        f"p50 {p(lat_pl, 50):.2f} ms p99 {p(lat_pl, 99):.2f} ms; stats {st}")
+            r = self.plain(i)
+            lat[i] = (time.perf_counter() - start - arrivals[i]) * 1e3
+            assert (r.n_groups, r.total_sum, r.rows_matched) == want[i], f"request {i} differs"
+        return lat, time.perf_counter() - start
+
+    def sweep(self, n_req):
+        """Poisson arrivals of n_req requests at 0.5x and 0.9x the plain capacity and at 0.9x the
+        served capacity (a closed burst of n_req requests), both forms at each rate."""
+        n_req = min(n_req, len(self.reqs))
+        plain_cap = self.plain_capacity()
+        _, _, burst_wall, _ = self.served([0.0] * n_req)
+        served_cap = n_req / burst_wall
+        log(f"serve: plain capacity {plain_cap:.1f} q/s, served capacity {served_cap:.1f} q/s")
+        pct = lambda a, q: float(np.percentile(np.array(a), q))
+        rows, last_st = [], None
+        for name, rate in (("0.5x plain capacity", 0.5 * plain_cap), ("0.9x plain capacity", 0.9 * plain_cap),
+                           ("0.9x served capacity", 0.9 * served_cap)):
+            arr = np.cumsum(np.random.default_rng(self.args.seed + 1).exponential(1.0 / rate, n_req))
+            ls, res, ws, st = self.served(arr)
+            lp, wp = self.plain_at(arr, res)
+            row = {"rate": name, "offered_qps": rate, "requests": n_req,
+                   "served_qps": n_req / ws, "served_p50_ms": pct(ls, 50), "served_p99_ms": pct(ls, 99),
+                   "plain_qps": n_req / wp, "plain_p50_ms": pct(lp, 50), "plain_p99_ms": pct(lp, 99)}
+            row["served_over_plain_qps"] = row["served_qps"] / row["plain_qps"]
+            row["meets_3x_and_lower_p99"] = bool(row["served_over_plain_qps"] >= 3.0 and
+                                                 row["served_p99_ms"] < row["plain_p99_ms"])
+            log(f"serve @ {name} ({rate:.1f} q/s): served {row['served_qps']:.1f} q/s p50 "
+                f"{row['served_p50_ms']:.2f} p99 {row['served_p99_ms']:.2f} ms; plain {row['plain_qps']:.1f} q/s "
+                f"p50 {row['plain_p50_ms']:.2f} p99 {row['plain_p99_ms']:.2f} ms")
+            rows.append(row)
+            last_st = st
+        return {"plain_capacity_qps": plain_cap, "served_capacity_qps": served_cap, "rates": rows}, last_st
+
+    def free(self):
+        for d in self.dev:
+            self.ctx.free_column(d)
+
+
+def run_serve(args):
+    """Serving form on the trip table: the sweep of ServeRig.sweep; the line's value is the served
+    queries/s at 0.9x the served capacity (the rate where plain launches fall behind)."""
+    import golap_public_amd as g
+    ctx = g.Context(0)
+    cpu_rows = 0 if args.no_cpu_baseline else min(args.trip_rows, 150_000_000)
+    rig = ServeRig(args, ctx, args.requests, keep_cpu_rows=cpu_rows)
+    sw, st = rig.sweep(args.requests)
+    top = sw["rates"][-1]
     # algorithmic bytes: every admitted request's non-pruned compressed bytes; the shared launches
     # read each surviving chunk's columns once for all requests of a batch (shared_comp_bytes)
     gbps = st["comp_bytes"] / (st["device_ms"] * 1e-3) / 1e9 if st["device_ms"] else 0.0
     out = {"metric": "queries/s + p99 latency (served trip-table requests, Poisson arrivals, 1 GPU)",
-           "value": srv_qps, "unit": "queries/s", "n_gpus": 1, "steps": len(reqs), "warmup": 8,
-           "ms_per_step": srv_wall * 1e3 / len(reqs), "higher_is_better": True, "scaling": "weak",
+           "value": top["served_qps"], "unit": "queries/s", "n_gpus": 1, "steps": top["requests"], "warmup": 8,
+           "ms_per_step": 1e3 / top["served_qps"], "higher_is_better": True, "scaling": "weak",
            "vs_baseline": None, "dtype": "int64", "data": "synthetic: SplitMix64-seeded trip table, seeded requests",
-           "config": {"workload": "serve_trip_1.5B", "rows": n, "zone_rows": args.zone_rows, "page_rows": args.page_rows,
-                      "codecs": "ts DBP, passenger LZ4, zone/fare/distance FOR", "comp_bytes_table": comp,
-                      "offered_qps": qps, "requests": len(reqs), "served_qps": srv_qps,
-                      "served_p50_ms": p(lat_srv, 50), "served_p99_ms": p(lat_srv, 99),
-                      "plain_qps": pl_qps, "plain_p50_ms": p(lat_pl, 50), "plain_p99_ms": p(lat_pl, 99),
-                      "plain_capacity_qps": plain_rate, "speedup_vs_plain": srv_qps / pl_qps,
-                      "server": st, "server_opts": opts},
+           "config": {"workload": "serve_trip_1.5B", "rows": rig.n, "zone_rows": args.zone_rows,
+                      "page_rows": args.page_rows, "lz4_min_match": args.min_match,
+                      "codecs": "ts DBP, passenger LZ4, zone/fare/distance FOR", "comp_bytes_table": rig.comp,
+                      "sweep": sw, "server": st, "server_opts": dict(hbm_budget=args.budget_mb << 20, **rig.OPTS)},
            "roofline": {"bound": "hbm", "achieved": gbps, "peak": 8000.0, "unit": "GB/s", "frac": gbps / 8000.0,
                         "traffic": traffic_of("serve_trip_1.5B")}}
     calibrate(ctx, args)
@@ -575,25 +649,22 @@ def run_serve(args):
         per_t = []
         for t in range(4):
             q = trip_template(t, 0, TRIP_HOURS, g.TRIP_T0)
-            preds, keys = q["preds"], q["keys"]
             ts = []
             for _ in range(3):
                 t0 = time.perf_counter()
-                w = oracle.scan_groupby(cpu_cols, preds, keys, q["value_col"], threads=threads)
+                w = oracle.scan_groupby(rig.cpu_cols, q["preds"], q["keys"], q["value_col"], threads=threads)
                 ts.append(time.perf_counter() - t0)
                 assert w is not None
             per_t.append(statistics.median(ts))
         rows_s = 4 * cpu_rows / sum(per_t)
-        rows_per_req = statistics.mean(chunks_srv) * args.zone_rows
+        rows_per_req = statistics.mean(rig.chunks) * args.zone_rows
         out["cpu_baseline"] = {"value": rows_s / rows_per_req, "unit": "queries/s", "cores": threads,
                                "kind": "port",
                                "sample": f"the four request templates over the first {cpu_rows} decoded rows "
-                                         f"({rows_s:.3g} rows/s, median of 3), scaled to this run's mean "
+                                         f"({rows_s:.3g} rows/s, median of 3), scaled to the last rate's mean "
                                          f"{rows_per_req:.3g} scanned rows per request"}
-    del cpu_cols
     print(json.dumps(out), flush=True)
-    for d in dev:
-        ctx.free_column(d)
+    rig.free()
     ctx.close()
 
 
@@ -757,22 +828,30 @@ def main():
     log(f"generating {args.rows} rows")
     cols = g.gen_q1_fact(args.seed, args.rows, key)
     log("encoding")
-    hosts = [g.HostColumn.encode(c, codec=g.CODEC_LZ4, page_rows=args.page_rows, chunk_rows=args.chunk_rows,
-                                 lz4_min_match=args.min_match) for c in cols]
-    del cols
+    enc = dict(codec=g.CODEC_LZ4, page_rows=args.page_rows, chunk_rows=args.chunk_rows)
+    hosts = [g.HostColumn.encode(c, lz4_min_match=args.min_match, **enc) for c in cols]
     raw_bytes = args.rows * 10
     comp_payload = sum(h.desc().payload_bytes for h in hosts)
     log(f"compressed payload {comp_payload/1e9:.3f} GB for {raw_bytes/1e9:.3f} GB raw")
     ctx = g.Context(0)
     cus, name = ctx.info()
     dcols = [ctx.upload(h) for h in hosts]
+    # the same columns coded with minimum match 16 (on the device; byte-identical to the host
+    # encoder): a secondary line where most pages are one literal run, read in place
+    mm16 = []
+    if not args.no_mm16 and args.min_match != 16:
+        for c in cols:
+            dv = ctx.to_device(c)
+            mm16.append(ctx.encode_device(dv, c.size, g.TYPE_OF[c.dtype], lz4_min_match=16, **enc)[0])
+            ctx.free(dv)
+    del cols
     dk, dy = ctx.to_device(key), ctx.to_device(year)
     m = ctx.hashmap(key.size, dk, g.I32)
     log(f"setup {time.perf_counter()-t_setup:.1f} s on {name} ({cus} CUs)")
 
-    def step():
+    def step(cols=dcols):
         ctx.hashmap_build(m, key.size, [(dy, g.I16, 1995, 1995)])
-        return ctx.q1_run(*dcols, m, (4, 6), (30, 50))
+        return ctx.q1_run(*cols, m, (4, 6), (30, 50))
 
     for _ in range(args.warmup):
         r = step()
@@ -823,9 +902,35 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
     }
+    if mm16:
+        for _ in range(args.warmup):
+            r16 = step(mm16)
+        k16 = []
+        for _ in range(args.steps):
+            r16 = step(mm16)
+            k16.append(r16.kernel_ms)
+        ms16 = statistics.mean(k16)
+        a16 = (r16.comp_bytes + key.size * (4 + 2) + 24) / (ms16 * 1e-3) / 1e9
+        assert (r16.sum, r16.matches) == (r.sum, r.matches)
+        out["config"]["lz4_min_match_16"] = {
+            "comp_bytes": r16.comp_bytes, "kernel_ms": ms16, "achieved_GBps": a16, "frac": a16 / HBM_PEAK_GBS,
+            "rows_per_s_kernel": args.rows / (ms16 * 1e-3), "same_result": True}
+        log(f"min-match 16 pages: kernel {ms16:.3f} ms, {a16:.0f} GB/s")
+        for c in mm16:
+            ctx.free_column(c)
     calibrate(ctx, args)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args, hosts, key, year, r.sum)
+    del hosts
+    if not args.no_serve_sweep:
+        rig = ServeRig(args, ctx, args.sweep_requests)
+        sw, _ = rig.sweep(args.sweep_requests)
+        top = sw["rates"][-1]
+        out["config"]["serving"] = {
+            "workload": f"serve_trip_1.5B sweep ({rig.n} trip rows, {args.sweep_requests} requests per rate)",
+            "served_over_plain_qps": top["served_over_plain_qps"], "served_p99_ms": top["served_p99_ms"],
+            "plain_p99_ms": top["plain_p99_ms"], "at_rate": top["rate"], "sweep": sw}
+        rig.free()
     print(json.dumps(out), flush=True)
     ctx.close()
 

@@ -59,6 +59,9 @@ struct golap_batch {
     golap_impl::starhost::ColumnSet cs;
     int windows[golap_impl::STAR_MAX_COLS] = {};
     int dslots[golap_impl::STAR_MAX_COLS] = {};  // scan: register slots of delta-bitpack columns
+    uint32_t wcaps[golap_impl::STAR_MAX_COLS] = {};  // scan: window bytes of parallel LZ4I decodes
+    int32_t tok_off = -1;                        // scan: LZ4I token offsets in the wave's region
+    uint32_t tok_cap = 0;
     uint32_t n_windows = 0;
     // star, k_star schedules: windows only for the drivers' delta-bitpack columns and other coded
     // pages, prefix-sum regions (1 + region, 0 = none) for the rest; the shared-filter schedule

@@ -214,7 +214,9 @@ hipError_t launch_kernel(golap_batch *b, hipStream_t s, bool prepare_only) {
     ScanLaunch L{};
     L.n_cols = (uint32_t)b->cs.cols.size();
     for (uint32_t i = 0; i < L.n_cols; ++i)
-        L.col[i] = ScanColView{view(b->cs.cols[i]), b->cs.cols[i]->type, b->windows[i], b->dslots[i], 0};
+        L.col[i] = ScanColView{view(b->cs.cols[i]), b->cs.cols[i]->type, b->windows[i], b->dslots[i], b->wcaps[i]};
+    L.tok_off = b->tok_off;
+    L.tok_cap = b->tok_cap;
     L.page_rows = b->page_rows;
     L.pages_per_chunk = b->ppc;
     L.n_rows = b->n_rows;
@@ -596,11 +598,13 @@ int batch_create_scan(golap_ctx *ctx, const golap_scan_plan *q, uint32_t n, Batc
     // register slots and LDS windows of the used columns over the union of the slices
     {
         int32_t woff[SCAN_MAX_COLS], dslot[SCAN_MAX_COLS];
+        uint32_t wcap[SCAN_MAX_COLS];
         b->win_bytes = scan_layout(b->cs.cols.data(), (uint32_t)b->cs.cols.size(), used, std::min(pb, pe), pe, woff,
-                                   dslot);
+                                   dslot, wcap, &b->tok_off, &b->tok_cap);
         for (size_t c = 0; c < b->cs.cols.size(); ++c) {
             b->windows[c] = woff[c];
             b->dslots[c] = dslot[c];
+            b->wcaps[c] = wcap[c];
             b->n_windows += woff[c] >= 0;
         }
     }

@@ -125,11 +125,12 @@ void scan_stats(const golap_scan_plan &p, uint32_t colmask, const std::vector<ui
 // registers; a column gets a window (16-byte aligned, page_rows * elem bytes) when it has other
 // coded pages, or delta pages without a slot. Returns the window bytes.
 uint32_t scan_layout(const golap_column *const *cols, uint32_t n_cols, uint32_t colmask, uint64_t pb, uint64_t pe,
-                     int32_t *woff, int32_t *dslot) {
-    uint32_t bytes = 0, slots = 0;
+                     int32_t *woff, int32_t *dslot, uint32_t *wcap, int32_t *tok_off, uint32_t *tok_cap) {
+    uint32_t bytes = 0, slots = 0, tcap = 0;
     for (uint32_t c = 0; c < n_cols; ++c) {
         woff[c] = -1;
         dslot[c] = -1;
+        wcap[c] = 0;
         if (!((colmask >> c) & 1u)) continue;
         auto &ex = cols[c]->extra;
         const uint32_t dbp = ex.dbp_prefix[pe] - ex.dbp_prefix[pb];
@@ -140,9 +141,19 @@ uint32_t scan_layout(const golap_column *const *cols, uint32_t n_cols, uint32_t 
         }
         if (!other) continue;
         woff[c] = (int32_t)bytes;
-        bytes += (cols[c]->page_rows * cols[c]->elem + 15) & ~15u;
+        const uint32_t pbytes = cols[c]->page_rows * cols[c]->elem;
+        if (ex.has_lz4i && pbytes <= 65536) {  // room to stage the page and decode it in parallel
+            uint32_t tc = 0;
+            lz4i_window(pbytes, ex.lz4i_max_nseq, wcap[c], tc);
+            tcap = std::max(tcap, tc);
+            bytes += wcap[c];
+        } else {
+            bytes += (pbytes + 15) & ~15u;
+        }
     }
-    return bytes;
+    *tok_off = tcap ? (int32_t)bytes : -1;
+    *tok_cap = tcap;
+    return bytes + (tcap ? lz4i_scratch_host(tcap) : 0);
 }
 
 // Waves per workgroup that take pages (each needs its own windows): all of them when their
@@ -244,9 +255,11 @@ int scan_run_impl(golap_ctx *ctx, const golap_scan_plan *plan, golap_star_result
     ScanLaunch L{};
     L.n_cols = plan->n_cols;
     int32_t woff[SCAN_MAX_COLS], dslot[SCAN_MAX_COLS];
-    L.win_bytes = scan_layout(plan->cols, plan->n_cols, qd.colmask, qd.s.page_begin, qd.s.page_end, woff, dslot);
+    uint32_t wcap[SCAN_MAX_COLS];
+    L.win_bytes = scan_layout(plan->cols, plan->n_cols, qd.colmask, qd.s.page_begin, qd.s.page_end, woff, dslot, wcap,
+                              &L.tok_off, &L.tok_cap);
     for (uint32_t c = 0; c < plan->n_cols; ++c)
-        L.col[c] = ScanColView{view(plan->cols[c]), plan->cols[c]->type, woff[c], dslot[c], 0};
+        L.col[c] = ScanColView{view(plan->cols[c]), plan->cols[c]->type, woff[c], dslot[c], wcap[c]};
     L.page_rows = f->page_rows;
     L.pages_per_chunk = (uint32_t)(f->chunk_rows / f->page_rows);
     L.n_rows = f->n_rows;

@@ -332,7 +332,7 @@ struct ScanColView {
     uint32_t type;
     int32_t woff;   // byte offset of the column's LDS window, -1 = none
     int32_t dslot;  // register slot for its delta-bitpack pages, -1 = none
-    uint32_t pad;
+    uint32_t wcap;  // window bytes when it decodes LZ4I pages in parallel (output + staging), else 0
 };
 struct ScanPredDev {
     uint32_t col, pad;
@@ -382,6 +382,10 @@ struct ScanLaunch {
     uint32_t max_blocks_per_cu;
     uint32_t rows_per_lane;  // 8: short launches (fewer steps per page); 4: long ones (more workgroups per CU)
     uint32_t *err;
+    // LZ4I token offsets and sink of the wave's parallel decodes: byte offset in the wave's LDS
+    // region (-1 = none) and the most sequences they hold
+    int32_t tok_off = -1;
+    uint32_t tok_cap = 0;
 };
 constexpr uint32_t SCAN_AGG_MAX = 8192;  // 96 KiB of LDS
 hipError_t launch_scan(golap_ctx *ctx, const ScanLaunch &l, hipStream_t stream, bool prepare_only = false);
@@ -469,6 +473,10 @@ inline void lz4i_window(uint32_t page_bytes, uint32_t max_nseq, uint32_t &win_ca
     win_cap = (page_bytes + slack + 15) & ~15u;
     tok_cap = max_nseq ? max_nseq : std::min<uint32_t>(page_bytes / 4 + 1, std::max<uint32_t>(64, page_bytes / 8));
 }
+// LDS beside the windows for LZ4I pages of up to tok_cap sequences: 2-byte token offsets
+// (rounded up to 16 bytes), then a 256-byte sink for the copies of idle lanes
+// (lz4i::scratch_bytes on the device side).
+inline uint32_t lz4i_scratch_host(uint32_t tok_cap) { return ((2 * tok_cap + 15) & ~15u) + 256; }
 
 // Device memory handed to one scan call in place of the context's buffers (the allocation-free
 // entry point): base is 256-byte aligned; the layout is fixed by scan_run_impl.

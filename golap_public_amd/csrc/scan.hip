@@ -54,6 +54,8 @@ struct ScanArgs {
     uint32_t *queue;
     uint32_t pages_per_item;
     uint32_t *err;
+    int32_t tok_off;   // LZ4I token offsets + sink in the wave's region, -1 = none
+    uint32_t tok_cap;
 };
 
 // How one typed column is read in the current page. off: byte offset of the data in the
@@ -569,8 +571,11 @@ __device__ void wave_page(const ScanArgs &a, ScanShared &S, uint64_t p, uint32_t
         if (!((used >> c) & 1u) || tp[c].mode != TA_WINDOW) continue;
         const ColView &cv = S.cols[c].c;
         const uint32_t want = rows * cv.elem;
+        // LZ4I pages of a column with a staging window decode in parallel (token offsets and
+        // sink behind the windows), others serially
+        uint16_t *tok = a.tok_off >= 0 ? reinterpret_cast<uint16_t *>(wlds + a.tok_off) : nullptr;
         const int64_t got = page_decode_wave(cv.page_codec[p], cv.comp + cv.page_off[p], cv.page_csize[p],
-                                             wlds + tp[c].off, want, cv.elem);
+                                             wlds + tp[c].off, want, cv.elem, S.cols[c].wcap, tok, a.tok_cap);
         if (got != (int64_t)want) {  // wave-uniform
             if (lane == 0) raise_err(a.err, DEVERR_CORRUPT);
             return;
@@ -994,6 +999,14 @@ hipError_t launch_scan(golap_ctx *ctx, const ScanLaunch &l, hipStream_t stream, 
     a.queue = l.queue;
     a.pages_per_item = l.pages_per_item;
     a.err = l.err;
+    a.tok_off = l.tok_off;
+    a.tok_cap = l.tok_cap;
+    if (l.tok_off >= 0 && ((uint32_t)l.tok_off % 16 || (uint32_t)l.tok_off + lz4i::scratch_bytes(0, l.tok_cap) > l.win_bytes))
+        return hipErrorInvalidValue;
+    for (uint32_t c = 0; c < l.n_cols; ++c)
+        if (l.col[c].wcap && (l.col[c].woff < 0 || l.col[c].wcap % 16 || (uint32_t)l.col[c].woff + l.col[c].wcap > l.win_bytes ||
+                              l.col[c].wcap < l.page_rows * l.col[c].c.elem))
+            return hipErrorInvalidValue;
     const size_t lds = scan_lds_bytes(l.win_bytes, l.agg_entries, l.page_waves);
     if (lds > scan_lds_budget(ctx)) return hipErrorInvalidConfiguration;
     return l.rows_per_lane == 8 ? launch_scan_t(ctx, k_scan<8>, a, lds, stream, l.max_blocks_per_cu, prepare_only)

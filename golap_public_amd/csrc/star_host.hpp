@@ -69,8 +69,12 @@ bool is_int_type(uint32_t t);
 int prepare_scan(golap_ctx *ctx, const golap_scan_plan *p, ScanQueryDev &qd, ScanHost &h);
 std::vector<uint8_t> scan_survivors(const golap_scan_plan &p, const std::vector<uint16_t> &mask);
 void scan_stats(const golap_scan_plan &p, uint32_t colmask, const std::vector<uint8_t> &run, golap_star_result *res);
+// LDS windows of one wave over columns colmask and pages [pb, pe): woff (-1 = none), delta-bitpack
+// register slots dslot (-1 = none), window bytes wcap of columns whose LZ4I pages decode in
+// parallel (0 = the page bytes), and the offset (-1 = none) and capacity of the LZ4I token
+// offsets behind the windows. Returns the bytes of one wave's region.
 uint32_t scan_layout(const golap_column *const *cols, uint32_t n_cols, uint32_t colmask, uint64_t pb, uint64_t pe,
-                     int32_t *woff, int32_t *dslot);
+                     int32_t *woff, int32_t *dslot, uint32_t *wcap, int32_t *tok_off, uint32_t *tok_cap);
 uint32_t scan_page_waves(golap_ctx *ctx, uint32_t win_bytes, uint32_t agg_entries);
 uint32_t scan_agg_entries(golap_ctx *ctx, uint32_t agg_mode, uint32_t requested, uint32_t win_bytes, uint32_t page_waves,
                           uint32_t dflt);

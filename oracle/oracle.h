@@ -152,6 +152,16 @@ int64_t oracle_q1_compressed(const int32_t *dim_key, uint64_t n_dim, const oracl
                              uint64_t chunk_begin, uint64_t chunk_end,
                              const oracle_q1_consts *c, uint32_t threads, uint64_t *matches);
 
+/* Decode timing (the CPU baseline's decode lines): every page of n_cols row-aligned paged
+ * columns (elem[c] bytes per value) decoded on `threads` threads into per-thread buffers, by the
+ * oracle's decoders, or with use_liblz4 by liblz4's LZ4_decompress_safe_partial for LZ4 and
+ * LZ4I pages (core/comp_cpu.hpp:22-32; liblz4.so.1 loaded at run time). *checksum: a
+ * position-weighted sum of the decoded words, equal for both decoders on the same pages;
+ * *seconds: wall time of the decode. Returns 0, -1 if a page fails, -2 without liblz4. */
+int oracle_decode_timing(const oracle_colref *cols, const uint32_t *elem, uint32_t n_cols, uint64_t n_rows,
+                         uint32_t page_rows, uint32_t use_liblz4, uint32_t threads, uint64_t *checksum,
+                         double *seconds);
+
 /* Star join + group-by, the plan of 02inmem.hpp:58-206 (query2 CPU) / pipeline_q2 (02.cuh:102-193):
  * n_dims dimensions, each with a key column, predicate list and a group attribute column
  * (int32, or NULL for none); fact FK columns fk[d]; value column val (int32 or uint32 as given).

@@ -74,6 +74,8 @@ def load():
                                      C.POINTER(u64)]),
             "oracle_q1_compressed": (i64, [vp, u64, vp, u32, vp, u64, u32, u32, u64, u64,
                                            C.POINTER(Q1Consts), u32, C.POINTER(u64)]),
+            "oracle_decode_timing": (C.c_int, [vp, vp, u32, u64, u32, u32, u32, C.POINTER(u64),
+                                               C.POINTER(C.c_double)]),
             "oracle_star_groupby": (i64, [vp, u32, vp, vp, u64, u32, u64, vp, vp, vp]),
             "oracle_groupby": (i64, [vp, u32, vp, u64, u64, vp, vp, vp]),
             "oracle_minmax_check_i64": (None, [vp, vp, u64, i64, i64, vp]),
@@ -236,6 +238,40 @@ def q1_compressed(dim_key, dim_preds, col_arrays, n_fact, page_rows, chunk_rows,
                                     C.cast(refs, vp), n_fact, page_rows, chunk_rows, chunk_begin, chunk_end,
                                     C.byref(c), threads, C.byref(m))
     return s, m.value
+
+
+def decode_timing(col_arrays, elems, n_rows, page_rows, use_liblz4=False, threads=1):
+    """Decodes every page of the row-aligned columns (oracle decoders, or liblz4 for LZ4 and
+    LZ4I pages); returns (status, checksum, seconds). status -2: liblz4 is not installed.
+    col_arrays: HostColumn.arrays() dicts, or (comp, page_off, page_csize, page_codec) addresses."""
+    refs = (ColRef * len(col_arrays))()
+    for i, a in enumerate(col_arrays):
+        if isinstance(a, dict):
+            refs[i] = ColRef(a["comp"].ctypes.data, a["page_off"].ctypes.data, a["page_csize"].ctypes.data,
+                             a["page_codec"].ctypes.data)
+        else:  # (comp, page_off, page_csize, page_codec) addresses
+            refs[i] = ColRef(*a)
+    el = np.ascontiguousarray(elems, np.uint32)
+    cs, sec = u64(), C.c_double()
+    r = load().oracle_decode_timing(C.cast(refs, vp), _p(el), len(col_arrays), n_rows, page_rows,
+                                    int(bool(use_liblz4)), threads, C.byref(cs), C.byref(sec))
+    return r, cs.value, sec.value
+
+
+def decode_checksum(columns):
+    """The checksum decode_timing reports, computed from the raw values: column c's bytes, zero
+    padded to 8, as 64-bit words w_g summed with weights 2 * (g + c * 2^40) + 1 (mod 2^64)."""
+    total = np.uint64(0)
+    with np.errstate(over="ignore"):
+        for c, v in enumerate(columns):
+            b = np.ascontiguousarray(v).view(np.uint8)
+            pad = (-b.size) % 8
+            if pad:
+                b = np.concatenate([b, np.zeros(pad, np.uint8)])
+            w = b.view(np.uint64)
+            g = np.arange(w.size, dtype=np.uint64) + np.uint64(c << 40)
+            total += np.sum(w * (np.uint64(2) * g + np.uint64(1)), dtype=np.uint64)
+    return int(total)
 
 
 def star_groupby(dims, fks, val, threads=1, max_groups=1 << 22):

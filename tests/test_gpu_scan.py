@@ -112,6 +112,31 @@ def test_scan_codecs(orc, golap, ctx, codec):
             ctx.free_column(d)
 
 
+@pytest.mark.parametrize("page_rows", [1024, 4096])
+def test_scan_lz4i_pages_decode_in_parallel(orc, golap, ctx, page_rows):
+    """Every column LZ4-coded at minimum match 4 (LZ4I pages: timestamps with thousands of
+    sequences a page, passenger counts with tens): the scan's windows stage and decode them in
+    parallel, single plans and a batch of them, against the oracle."""
+    host, dev = trip_cols(golap, ctx, 700_000, seed=19, codec=1, page_rows=page_rows, chunk_rows=65536)
+    try:
+        codes = [ctx.to_host(ctx.column_desc(d).page_codec, ctx.column_desc(d).n_pages, np.uint8) for d in dev]
+        assert all((c == 5).mean() > 0.9 for c in codes[:2])  # ts and passenger pages carry an index
+        plans = []
+        for t in range(4):
+            preds, keys, vc = template(t, 500, 7000)
+            say("scan lz4i template", t, "page_rows", page_rows)
+            check(orc, ctx, host, dev, preds, keys, vc)
+            plans.append(dict(cols=dev, preds=preds, keys=keys, value_col=vc))
+        with ctx.scan_batch(plans) as b:
+            b.run()
+            for i, p in enumerate(plans):
+                got, ref = b.result(i), ctx.scan_run(**p)
+                assert all(np.array_equal(x, y) for x, y in zip(got[1:], ref[1:])), i
+    finally:
+        for d in dev:
+            ctx.free_column(d)
+
+
 def test_scan_key_out_of_range_is_an_error(golap, ctx, trip):
     host, dev = trip
     preds, keys, vc = template(0, 1000, 1700)

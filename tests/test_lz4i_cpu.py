@@ -164,3 +164,27 @@ def test_oracle_refuses_damaged_indices(orc, golap):
         assert dec(pg) == len(want)
     # a block that no longer parses is refused whatever the index says
     assert dec(make_page(blk, 5)[:-1]) == -1
+
+
+@pytest.mark.parametrize("use_liblz4", [False, True])
+def test_decode_timing_checksum_matches_values(orc, golap, use_liblz4):
+    """The CPU baseline's decode lines (oracle decoders, liblz4) decode every page of a Q1-shaped
+    table to its values: their checksum equals one computed from the raw columns."""
+    r = rng(43)
+    n = 4096 * 5 + 77
+    cols = [(19920101 + r.integers(0, 2556, n)).astype(np.int32), r.integers(0, 11, n).astype(np.uint8),
+            r.integers(1, 51, n).astype(np.uint8), r.integers(90000, 10500001, n).astype(np.int32)]
+    arrays = [golap.HostColumn.encode(c, codec=1, page_rows=4096, chunk_rows=8192, lz4_min_match=4).arrays()
+              for c in cols]
+    assert (arrays[0]["page_codec"] == LZ4I).sum() >= 5
+    st, cs, sec = orc.decode_timing(arrays, [c.itemsize for c in cols], n, 4096, use_liblz4, threads=3)
+    if st == -2:
+        pytest.skip("liblz4 missing")
+    assert st == 0 and sec >= 0
+    assert cs == orc.decode_checksum(cols)
+    bad = [dict(a) for a in arrays]
+    bad[0]["comp"] = bad[0]["comp"].copy()
+    mid = int(bad[0]["page_off"][1]) + int(bad[0]["page_csize"][1]) // 2
+    bad[0]["comp"][mid] ^= 0x5A  # a byte in the middle of page 1's block (behind its index)
+    st2, cs2, _ = orc.decode_timing(bad, [c.itemsize for c in cols], n, 4096, use_liblz4, threads=3)
+    assert st2 == -1 or cs2 != cs
