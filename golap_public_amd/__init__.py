@@ -31,7 +31,7 @@ i64 = C.c_int64
 
 class EncodeOpts(C.Structure):
     _fields_ = [("codec", u32), ("page_rows", u32), ("chunk_rows", u32),
-                ("lz4_min_match", u32), ("threads", u32)]
+                ("lz4_min_match", u32), ("threads", u32), ("lz4_unit", u32)]
 
 
 class ColumnDesc(C.Structure):
@@ -291,11 +291,11 @@ def _ptr(a):
     return C.c_void_p(a.ctypes.data) if a is not None else None
 
 
-def encode_opts(codec=CODEC_LZ4, page_rows=4096, chunk_rows=262144, lz4_min_match=4, threads=0):
+def encode_opts(codec=CODEC_LZ4, page_rows=4096, chunk_rows=262144, lz4_min_match=4, threads=0, lz4_unit=1):
     o = EncodeOpts()
     load().golap_encode_opts_default(C.byref(o))
-    o.codec, o.page_rows, o.chunk_rows, o.lz4_min_match, o.threads = (
-        codec, page_rows, chunk_rows, lz4_min_match, threads)
+    o.codec, o.page_rows, o.chunk_rows, o.lz4_min_match, o.threads, o.lz4_unit = (
+        codec, page_rows, chunk_rows, lz4_min_match, threads, lz4_unit)
     return o
 
 

@@ -253,6 +253,7 @@ void golap_encode_opts_default(golap_encode_opts *o) {
     o->chunk_rows = 262144;
     o->lz4_min_match = 4;
     o->threads = 0;
+    o->lz4_unit = 1;
 }
 
 int golap_encode(const void *host_values, uint64_t n_rows, uint32_t type,
@@ -420,7 +421,10 @@ void golap_impl::register_extra(golap_column *c) {
         const bool point = (c->page_codec[i] & golap_fmt::PAGE_POINT) != 0;
         ex.window_prefix[i + 1] = ex.window_prefix[i] + (direct || point ? 0 : 1);
         const bool dbp = (c->page_codec[i] & golap_fmt::PAGE_CODEC_MASK) == golap_fmt::DBP && !point;
-        ex.has_lz4i = ex.has_lz4i || (c->page_codec[i] & golap_fmt::PAGE_CODEC_MASK) == golap_fmt::LZ4I;
+        if ((c->page_codec[i] & golap_fmt::PAGE_CODEC_MASK) == golap_fmt::LZ4I) {
+            ex.has_lz4i = true;
+            ex.lz4i_max_csize = std::max(ex.lz4i_max_csize, c->page_csize[i]);
+        }
         ex.dbp_prefix[i + 1] = ex.dbp_prefix[i] + (dbp ? 1 : 0);
     }
     ex.chunk_payload.resize(c->n_chunks + 1, 0);

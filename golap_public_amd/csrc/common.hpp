@@ -66,6 +66,7 @@ struct ColumnExtra {
     // most sequences of any LZ4I page (sizes the token arrays of parallel decodes), 0 when the
     // page data was not seen on the host
     uint32_t lz4i_max_nseq = 0;
+    uint32_t lz4i_max_csize = 0;  // largest LZ4I page (bytes), from the catalog
     bool has_lz4i = false;
 };
 

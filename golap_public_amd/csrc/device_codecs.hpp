@@ -27,7 +27,8 @@ __device__ __forceinline__ int lane_id() { return __lane_id(); }
 __device__ __forceinline__ int64_t lz4_decode_wave(const uint8_t *__restrict__ src, uint32_t csize,
                                           uint8_t *__restrict__ dst, uint32_t cap,
                                           const uint8_t *idx = nullptr, uint32_t nseq = 0,
-                                          uint32_t seg_log2 = 0) {
+                                          uint32_t seg_log2 = 0, uint32_t unit_log2 = 0) {
+    const uint32_t umask = (1u << unit_log2) - 1;  // an LZ4I page's lengths and offsets in whole units
     const uint32_t lane = lane_id();
     uint32_t ip = 0, op = 0, seq = 0;
     if (csize == 0) return cap == 0 ? 0 : -1;
@@ -51,7 +52,7 @@ __device__ __forceinline__ int64_t lz4_decode_wave(const uint8_t *__restrict__ s
                 lit += b;
             } while (b == 255 && lit < (1u << 30));
         }
-        if (lit > csize - ip || lit > cap - op) return -1;
+        if (lit > csize - ip || lit > cap - op || (lit & umask)) return -1;
         // literals: global -> window
         uint32_t k = lane;
         for (; k + 3 * WAVE < lit; k += 4 * WAVE) {
@@ -80,7 +81,7 @@ __device__ __forceinline__ int64_t lz4_decode_wave(const uint8_t *__restrict__ s
             } while (b == 255 && ml < (1u << 30));
         }
         ml += golap_fmt::LZ4_MINMATCH;
-        if (ml > cap - op) return -1;
+        if (ml > cap - op || ((off | ml) & umask)) return -1;
         // match: byte j of the match equals byte (j mod off) of dst[op-off, op), which is final
         // before the copy starts, so every lane may copy independently.
         const uint32_t from = op - off;
@@ -140,7 +141,7 @@ __host__ __device__ inline uint32_t scratch_bytes(uint32_t win_cap, uint32_t tok
 }
 
 struct Hdr {
-    uint32_t nseq, sl, nseg, hb;
+    uint32_t nseq, sl, nseg, hb, ul;  // ul: log2 of the unit the header claims (0..2)
 };
 
 // Header of an LZ4I page (in global memory or LDS); false if malformed.
@@ -150,7 +151,8 @@ __device__ __forceinline__ bool header(const uint8_t *src, uint32_t csize, Hdr &
     __builtin_memcpy(&w, src, 4);
     h.nseq = w & 0xffffu;
     h.sl = (w >> 16) & 0xffu;
-    if (h.nseq == 0 || h.sl > 15 || (w >> 24) != 0) return false;
+    h.ul = w >> 24;
+    if (h.nseq == 0 || h.sl > 15 || h.ul > 2) return false;
     h.nseg = golap_fmt::lz4i_segments(h.nseq, h.sl);
     h.hb = (4 + 2 * (h.nseg - 1) + 15) & ~15u;
     return h.hb <= csize && csize - h.hb <= golap_fmt::LZ4I_MAX_BLOCK;
@@ -406,7 +408,9 @@ __device__ __forceinline__ int64_t decode_par(const uint8_t *__restrict__ src, u
         const uint32_t total = __builtin_amdgcn_readlane(incl, WAVE - 1);
         const uint32_t in0 = __builtin_amdgcn_readfirstlane(ip);
         // the walk bounds every length by the block size, so these sums do not wrap
-        const bool bad = valid && ((uint64_t)dst + len > osize || (ml && (off == 0 || off > mdst)));
+        const uint32_t umask = (1u << h.ul) - 1;  // lengths and offsets in whole units
+        const bool bad = valid && ((uint64_t)dst + len > osize || (ml && (off == 0 || off > mdst)) ||
+                                   ((lit | off | ml) & umask) != 0);
         if (__any(bad)) return -1;
         // In place, the group writes [base, base + total) while its input still lies in the
         // staged page. When that range ends below the group's first input byte, literals are
@@ -497,7 +501,7 @@ __device__ __forceinline__ int64_t lz4i_decode_staged(const uint8_t *__restrict_
         const int64_t r = lz4i::decode_par(src, csize, h, win, win_cap, osize, tok, tok_cap, true);
         if (r != -2) return r;
     }
-    return lz4_decode_wave(src + h.hb, csize - h.hb, win, osize, src + 4, h.nseq, h.sl);
+    return lz4_decode_wave(src + h.hb, csize - h.hb, win, osize, src + 4, h.nseq, h.sl, h.ul);
 }
 
 // Decodes an LZ4I page src[0, csize) into win[0, osize): in parallel when the page has enough
@@ -515,7 +519,7 @@ __device__ __forceinline__ int64_t lz4i_decode_wave(const uint8_t *__restrict__ 
         const int64_t r = lz4i::decode_par(src, csize, h, win, win_cap, osize, tok, tok_cap);
         if (r != -2) return r;
     }
-    return lz4_decode_wave(src + h.hb, csize - h.hb, win, osize, src + 4, h.nseq, h.sl);
+    return lz4_decode_wave(src + h.hb, csize - h.hb, win, osize, src + 4, h.nseq, h.sl, h.ul);
 }
 
 __device__ __forceinline__ void store_elem(uint8_t *dst, uint64_t v, uint32_t elem) {

@@ -54,12 +54,16 @@ static inline uint8_t *put_len(uint8_t *op, uint64_t len) {  // length extension
 // absolute positions (base + offset into src); entries older than this block are < base and
 // rejected, so the table needs no reset between pages. Matches shorter than min_match are not
 // emitted: a short match saves at most a byte or two but costs the decoder a full sequence.
+// unit (1, 2 or 4): every sequence starts at a multiple of unit bytes and every literal run,
+// match and offset is a multiple of unit bytes (the parse visits unit-aligned positions only and
+// rounds match lengths down), so a decoder copies whole units; the block is standard LZ4.
 uint64_t lz4_encode_block(const uint8_t *src, uint64_t n, uint8_t *dst, uint32_t min_match,
-                          int64_t *ht, int64_t base, std::vector<uint32_t> *tokens) {
+                          int64_t *ht, int64_t base, std::vector<uint32_t> *tokens, uint32_t unit) {
     uint8_t *op = dst;
     if (tokens) tokens->clear();
     uint64_t anchor = 0, ip = 0;
     if (min_match < LZ4_MINMATCH) min_match = LZ4_MINMATCH;
+    if (unit != 2 && unit != 4) unit = 1;
     if (n >= LZ4_MFLIMIT + 1) {
         const uint64_t match_start_limit = n - LZ4_MFLIMIT;  // a match starts at ip <= n-12
         const uint64_t match_end_limit = n - LZ4_LASTLITERALS;
@@ -73,6 +77,7 @@ uint64_t lz4_encode_block(const uint8_t *src, uint64_t n, uint8_t *dst, uint32_t
                 rd32(src + ref) == seq) {
                 uint64_t len = 4;
                 while (ip + len < match_end_limit && src[ref + len] == src[ip + len]) ++len;
+                len -= len % unit;
                 if (len >= min_match) {
                     uint64_t lit = ip - anchor, ml = len - LZ4_MINMATCH;
                     uint8_t *token = op++;
@@ -92,7 +97,7 @@ uint64_t lz4_encode_block(const uint8_t *src, uint64_t n, uint8_t *dst, uint32_t
                 }
             }
             // liblz4-style acceleration: step grows with consecutive misses.
-            ip += 1 + (misses++ >> 6);
+            ip += unit * (1 + (misses++ >> 6));
         }
     }
     uint64_t lit = n - anchor;
@@ -104,7 +109,8 @@ uint64_t lz4_encode_block(const uint8_t *src, uint64_t n, uint8_t *dst, uint32_t
     return (uint64_t)(op - dst);
 }
 
-uint64_t lz4i_make_page(const uint8_t *blk, uint64_t bsize, const std::vector<uint32_t> &tokens, uint8_t *out) {
+uint64_t lz4i_make_page(const uint8_t *blk, uint64_t bsize, const std::vector<uint32_t> &tokens, uint8_t *out,
+                        uint32_t unit) {
     const uint64_t nseq = tokens.size();
     if (!nseq || bsize > LZ4I_MAX_BLOCK) return 0;
     const uint32_t sl = lz4i_seg_log2((uint32_t)nseq);
@@ -114,6 +120,7 @@ uint64_t lz4i_make_page(const uint8_t *blk, uint64_t bsize, const std::vector<ui
     out[0] = (uint8_t)(nseq & 0xff);
     out[1] = (uint8_t)(nseq >> 8);
     out[2] = (uint8_t)sl;
+    out[3] = (uint8_t)(unit == 4 ? 2 : unit == 2 ? 1 : 0);
     for (uint32_t s = 1; s < nseg; ++s) {
         const uint32_t t = tokens[(uint64_t)s << sl];
         out[4 + 2 * (s - 1)] = (uint8_t)(t & 0xff);
@@ -234,6 +241,10 @@ uint64_t rle_encode_page(const uint8_t *src, uint64_t rows, uint32_t elem, uint8
 }
 
 // ------------------------------------------------------------------------------------ column
+uint32_t lz4_unit_of(const golap_encode_opts &o, uint32_t elem) {
+    if (o.lz4_unit == 1 || o.lz4_unit == 2 || o.lz4_unit == 4) return std::min(o.lz4_unit, std::max(elem, 1u));
+    return std::min(elem, 4u);  // the element width; 8-byte elements in 4-byte units
+}
 namespace {
 struct PageOut {
     uint64_t pos;    // offset in the worker's buffer
@@ -243,7 +254,7 @@ struct PageOut {
 
 // Codes one page; for AUTO the smallest of LZ4 / DBP / RLE / FOR wins (ties keep the earlier).
 void code_page(const uint8_t *src, uint64_t rows, uint32_t type, uint32_t elem, uint32_t codec,
-               uint32_t min_match, int64_t *ht, int64_t base, std::vector<uint8_t> &tmp,
+               uint32_t min_match, uint32_t unit, int64_t *ht, int64_t base, std::vector<uint8_t> &tmp,
                std::vector<uint8_t> &buf, PageOut &po) {
     const uint64_t nbytes = rows * elem;
     auto emit = [&](const uint8_t *p, uint64_t sz, uint8_t c) {
@@ -270,14 +281,14 @@ void code_page(const uint8_t *src, uint64_t rows, uint32_t type, uint32_t elem, 
     auto lz4_candidate = [&]() {
         cand.resize(160 + lz4_bound(nbytes) + 16);
         uint8_t *blk = cand.data() + 160;  // behind the largest header (144 bytes)
-        const uint64_t sz = lz4_encode_block(src, nbytes, blk, min_match, ht, base, &toks);
+        const uint64_t sz = lz4_encode_block(src, nbytes, blk, min_match, ht, base, &toks, unit);
         // a block with a match can have exactly the literal-only size: parse, do not compare sizes
         if (lz4_is_literal_only(blk, sz, nbytes)) {
             memmove(cand.data(), blk, sz);
             consider(sz, (uint8_t)(LZ4 | PAGE_LITERAL_ONLY));
             return;
         }
-        uint64_t pz = lz4i_make_page(blk, sz, toks, cand.data());
+        uint64_t pz = lz4i_make_page(blk, sz, toks, cand.data(), unit);
         if (pz) {
             consider(pz, LZ4I);
         } else {
@@ -349,7 +360,7 @@ int encode_column(const void *values, uint64_t n_rows, uint32_t type, const gola
             uint64_t r0 = p * o.page_rows;
             uint64_t rows = std::min<uint64_t>(o.page_rows, n_rows - r0);
             PageOut po;
-            code_page(src + r0 * elem, rows, type, elem, o.codec, o.lz4_min_match, ht.data(),
+            code_page(src + r0 * elem, rows, type, elem, o.codec, o.lz4_min_match, lz4_unit_of(o, elem), ht.data(),
                       (int64_t)(r0 * elem), tmp, buf, po);
             pos[t].push_back(po);
         }

@@ -48,7 +48,7 @@ constexpr uint32_t LZ4I_HDR_MAX = 144;
 
 struct EncodeArgs {
     const uint8_t *src;
-    uint32_t type, elem, page_rows, codec, min_match;
+    uint32_t type, elem, page_rows, codec, min_match, unit;
     uint64_t n_rows, n_pages;
     uint8_t *slots;
     uint64_t slot_bytes;
@@ -134,8 +134,8 @@ __device__ __forceinline__ void copy_bytes(uint8_t *dst, const uint8_t *src, uin
     for (uint32_t x = threadIdx.x; x < n; x += PW) dst[x] = src[x];
 }
 
-// The host encoder's greedy parse (one hash probe per position, acceleration after 64 misses,
-// matches shorter than min_match not emitted), run by the whole wave with identical control
+// The host encoder's greedy parse (one hash probe per unit-aligned position, acceleration after 64
+// misses, match lengths rounded down to whole units, matches shorter than min_match not emitted), run by the whole wave with identical control
 // flow over the page staged in LDS (src). The hash table holds page positions; HT_EMPTY
 // entries and entries not below ip are rejected exactly as the host rejects entries of earlier
 // pages. The workgroup is one wave, whose LDS accesses complete in issue order, so the probe
@@ -145,10 +145,11 @@ __device__ __forceinline__ void copy_bytes(uint8_t *dst, const uint8_t *src, uin
 // Returns the block size; literal_only, lit_shift (the literal-only block's offset in out) and
 // nseq (the block's sequences) set.
 __device__ __forceinline__ uint32_t lz4_page(const uint8_t *src, uint32_t n, uint8_t *out, uint32_t min_match,
-                                             uint16_t *ht, uint16_t *rec, bool &literal_only, uint32_t &lit_shift,
-                                             uint32_t &nseq) {
+                                             uint32_t unit, uint16_t *ht, uint16_t *rec, bool &literal_only,
+                                             uint32_t &lit_shift, uint32_t &nseq) {
     const uint32_t lane = threadIdx.x;
     if (min_match < LZ4_MINMATCH) min_match = LZ4_MINMATCH;
+    if (unit != 2 && unit != 4) unit = 1;
     uint32_t op = 0, anchor = 0, ip = 0;
     nseq = 0;
     if (n >= LZ4_MFLIMIT + 1) {
@@ -171,6 +172,7 @@ __device__ __forceinline__ uint32_t lz4_page(const uint8_t *src, uint32_t n, uin
                     }
                     len += PW;
                 }
+                len -= len % unit;
                 if (len >= min_match) {
                     const uint32_t lit = ip - anchor, ml = len - LZ4_MINMATCH;
                     if (lane == 0 && (nseq & 15) == 0) rec[nseq >> 4] = (uint16_t)op;
@@ -198,7 +200,7 @@ __device__ __forceinline__ uint32_t lz4_page(const uint8_t *src, uint32_t n, uin
                     continue;
                 }
             }
-            ip += 1 + (misses++ >> 6);
+            ip += unit * (1 + (misses++ >> 6));
         }
     }
     const uint32_t lit = n - anchor;
@@ -386,7 +388,7 @@ uint32_t encode_lds_bytes(uint64_t page_bytes) {
 
 // The LZ4I header (format.hpp) of a block of nseq sequences into hdr (lz4i_header_bytes bytes,
 // ending where the block starts), from the token offsets of every 16th sequence.
-__device__ __forceinline__ uint32_t lz4i_put_header(uint8_t *hdr_end, uint32_t nseq, const uint16_t *rec) {
+__device__ __forceinline__ uint32_t lz4i_put_header(uint8_t *hdr_end, uint32_t nseq, const uint16_t *rec, uint32_t unit) {
     const uint32_t sl = lz4i_seg_log2(nseq), nseg = lz4i_segments(nseq, sl), hb = lz4i_header_bytes(nseq, sl);
     uint8_t *h = hdr_end - hb;
     for (uint32_t x = threadIdx.x; x < hb; x += PW) {
@@ -394,6 +396,7 @@ __device__ __forceinline__ uint32_t lz4i_put_header(uint8_t *hdr_end, uint32_t n
         if (x == 0) b = nseq & 0xff;
         else if (x == 1) b = nseq >> 8;
         else if (x == 2) b = sl;
+        else if (x == 3) b = unit == 4 ? 2u : unit == 2 ? 1u : 0u;
         else if (x >= 4) {
             const uint32_t s = (x - 4) / 2 + 1;  // entry s: the token of sequence s << sl
             if (s < nseg) {
@@ -451,13 +454,13 @@ __global__ __launch_bounds__(PW) void k_encode_pages(EncodeArgs a) {
                 uint32_t lit_shift = 0, nseq = 0;
                 stage_page(src, nbytes, page, ht);
                 const uint32_t bsize =
-                    lz4_page(page, nbytes, out + LZ4I_HDR_MAX, a.min_match, ht, rec, lit_only, lit_shift, nseq);
+                    lz4_page(page, nbytes, out + LZ4I_HDR_MAX, a.min_match, a.unit, ht, rec, lit_only, lit_shift, nseq);
                 best = bsize;
                 shift = LZ4I_HDR_MAX + lit_shift;
                 code = (uint8_t)(LZ4 | (lit_only ? PAGE_LITERAL_ONLY : 0));
                 if (!lit_only && bsize <= LZ4I_MAX_BLOCK) {
                     __syncthreads();  // lane 0's token offsets
-                    const uint32_t hb = lz4i_put_header(out + LZ4I_HDR_MAX, nseq, rec);
+                    const uint32_t hb = lz4i_put_header(out + LZ4I_HDR_MAX, nseq, rec, a.unit);
                     best = hb + bsize;
                     shift = LZ4I_HDR_MAX - hb;
                     code = LZ4I;
@@ -626,6 +629,7 @@ extern "C" int golap_encode_device(golap_ctx *ctx, const void *dev_values, uint6
         a.page_rows = o.page_rows;
         a.codec = o.codec;
         a.min_match = o.lz4_min_match;
+        a.unit = golap_host::lz4_unit_of(o, elem);
         a.n_rows = n_rows;
         a.n_pages = h.n_pages;
         a.slots = sb;

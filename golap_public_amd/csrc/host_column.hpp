@@ -28,12 +28,19 @@ uint64_t lz4_bound(uint64_t n);
 // hash_table: 1<<LZ4_HASH_LOG absolute positions; base: absolute position of src (must grow
 // from block to block when the table is reused, so stale entries fall below base).
 // tokens, when given, receives the block offset of every sequence's token, in order.
+// unit (1, 2 or 4): sequences start, and literal runs, matches and offsets measure, in whole units.
 uint64_t lz4_encode_block(const uint8_t *src, uint64_t n, uint8_t *dst, uint32_t min_match,
-                          int64_t *hash_table, int64_t base, std::vector<uint32_t> *tokens = nullptr);
+                          int64_t *hash_table, int64_t base, std::vector<uint32_t> *tokens = nullptr,
+                          uint32_t unit = 1);
+// The LZ4 unit of a column of elem-byte values under opts (lz4_unit 0: the element width, 8-byte
+// elements in 4-byte units).
+uint32_t lz4_unit_of(const golap_encode_opts &opts, uint32_t elem);
 // An LZ4I page (format.hpp) from a block of bsize bytes and its token offsets; returns the
 // page size, 0 if the block does not fit the index (64 KiB or more). out must hold
 // lz4i_header_bytes(tokens.size(), lz4i_seg_log2(tokens.size())) + bsize bytes.
-uint64_t lz4i_make_page(const uint8_t *blk, uint64_t bsize, const std::vector<uint32_t> &tokens, uint8_t *out);
+// unit: the block's LZ4 unit, recorded in the header.
+uint64_t lz4i_make_page(const uint8_t *blk, uint64_t bsize, const std::vector<uint32_t> &tokens, uint8_t *out,
+                        uint32_t unit = 1);
 constexpr uint32_t LZ4_HASH_LOG = 14;
 
 // Delta-bitpack page; returns 0 if the value range needs more than 32 bits per delta.

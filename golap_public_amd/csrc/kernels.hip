@@ -61,101 +61,6 @@ __device__ __forceinline__ uint32_t direct_header(uint8_t codec, uint32_t nbytes
     return (codec & PAGE_LITERAL_ONLY) ? golap_fmt::lz4_literal_header_bytes(nbytes) : 0u;
 }
 
-// ------------------------------------------------------------------------ page decompression
-struct DecodeArgs {
-    ColView c;
-    uint64_t page_begin, page_end;
-    uint32_t page_rows;
-    uint64_t n_rows;
-    uint8_t *out;
-    uint64_t out_row0;
-    uint32_t window;   // LDS decode window (a multiple of 16 bytes, the page plus LZ4I slack)
-    uint32_t tok_cap;  // LZ4I token offsets after the window
-    uint32_t *err;
-};
-
-// One wave (workgroup) per page. In-place pages are copied global->global with 16-byte
-// accesses; coded pages are decoded into the wave's LDS window (LZ4I pages by the parallel
-// parse), then written out with 16-byte stores.
-__global__ __launch_bounds__(WAVE) void k_decode_pages(DecodeArgs a) {
-    extern __shared__ __align__(16) uint8_t lds[];
-    const uint32_t lane = threadIdx.x;
-    uint8_t *win = lds;
-    uint16_t *tok = reinterpret_cast<uint16_t *>(lds + a.window);
-    const uint64_t n = a.page_end - a.page_begin;
-    for (uint64_t i = blockIdx.x; i < n; i += gridDim.x) {
-        const uint64_t p = a.page_begin + i;
-        const uint64_t r0 = p * a.page_rows;
-        const uint32_t rows = (uint32_t)min<uint64_t>(a.page_rows, a.n_rows - r0);
-        const uint32_t nbytes = rows * a.c.elem;
-        const uint8_t codec = a.c.page_codec[p];
-        const uint8_t *src = a.c.comp + a.c.page_off[p];
-        const uint32_t csize = a.c.page_csize[p];
-        uint8_t *dst = a.out + (r0 - a.out_row0) * a.c.elem;
-        const uint8_t *from;
-        if (page_direct(codec)) {
-            const uint32_t hdr = direct_header(codec, nbytes);
-            if ((uint64_t)hdr + nbytes != csize) {
-                if (lane == 0) raise_err(a.err, DEVERR_CORRUPT);
-                continue;
-            }
-            from = src + hdr;
-        } else {
-            if (nbytes > a.window) {
-                if (lane == 0) raise_err(a.err, DEVERR_CORRUPT);
-                continue;
-            }
-            const int64_t got = page_decode_wave(codec, src, csize, win, nbytes, a.c.elem, a.window, tok, a.tok_cap);
-            if (got != (int64_t)nbytes) {
-                if (lane == 0) raise_err(a.err, DEVERR_CORRUPT);
-                continue;
-            }
-            from = win;
-        }
-        const bool aligned = ((reinterpret_cast<uintptr_t>(from) | reinterpret_cast<uintptr_t>(dst)) & 15) == 0;
-        uint32_t done = 0;
-        if (aligned) {
-            const uint32_t nv = nbytes / 16;
-            const uint4 *s4 = reinterpret_cast<const uint4 *>(from);
-            uint4 *d4 = reinterpret_cast<uint4 *>(dst);
-            for (uint32_t k = lane; k < nv; k += WAVE) d4[k] = s4[k];
-            done = nv * 16;
-        }
-        for (uint32_t k = done + lane; k < nbytes; k += WAVE) dst[k] = from[k];
-    }
-}
-
-hipError_t launch_decode_pages(golap_ctx *ctx, const golap_column *col, uint64_t page_begin,
-                               uint64_t page_end, uint8_t *dev_out, uint64_t out_row0) {
-    if (page_end <= page_begin) return hipSuccess;
-    DecodeArgs a;
-    a.c = view(col);
-    a.page_begin = page_begin;
-    a.page_end = page_end;
-    a.page_rows = col->page_rows;
-    a.n_rows = col->n_rows;
-    a.out = dev_out;
-    a.out_row0 = out_row0;
-    const uint32_t page_bytes = col->page_rows * col->elem;
-    if (col->extra.has_lz4i && page_bytes <= 65536) {
-        lz4i_window(page_bytes, col->extra.lz4i_max_nseq, a.window, a.tok_cap);
-    } else {
-        a.window = (page_bytes + 15) & ~15u;
-        a.tok_cap = 0;
-    }
-    a.err = reinterpret_cast<uint32_t *>(ctx->d_scratch);
-    const size_t lds = (size_t)a.window + (a.tok_cap ? lz4i::scratch_bytes(a.window, a.tok_cap) : 0);
-    hipError_t e = prepare_lds(ctx, k_decode_pages, lds);
-    if (e != hipSuccess) return e;
-    int occ = 0;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_decode_pages, WAVE, lds);
-    if (e != hipSuccess) return e;
-    occ = std::max(occ, 1);
-    const uint32_t grid = (uint32_t)std::min<uint64_t>(page_end - page_begin, (uint64_t)ctx->cus * occ);
-    hipLaunchKernelGGL(k_decode_pages, dim3(grid), dim3(WAVE), lds, ctx->stream, a);
-    return hipGetLastError();
-}
-
 // ------------------------------------------------------------------------ hash table
 __device__ __host__ __forceinline__ uint32_t ht_hash(int32_t key, uint32_t log2cap) {
     return ((uint32_t)key * 0x9E3779B1u) >> (32 - log2cap);

@@ -130,6 +130,7 @@ struct CompInfo {
     std::string comp_algo = "LZ4";
     uint64_t page_rows = 4096;  // rows per independently coded page (nvcomp's nvchunk role)
     uint32_t lz4_min_match = 4;
+    uint32_t lz4_unit = 1;      // LZ4 sequence unit (1: byte-granular; 0: the element width, typed)
     uint32_t value_size = 0;    // bytes per value, set by prepare_compressed_device
     std::vector<BlockInfo> blocks;
     std::shared_ptr<golap_column> col;  // device column
@@ -236,6 +237,7 @@ golap_encode_opts encode_opts_of(const CompInfo &ci) {
     o.page_rows = (uint32_t)ci.page_rows;
     o.chunk_rows = (uint32_t)(ci.chunk_bytes / sizeof(T));
     o.lz4_min_match = ci.lz4_min_match;
+    o.lz4_unit = ci.lz4_unit;
     return o;
 }
 }  // namespace detail

@@ -91,6 +91,11 @@ typedef struct {
     uint32_t chunk_rows;     /* rows per chunk; a multiple of page_rows */
     uint32_t lz4_min_match;  /* shortest match the LZ4 encoder emits (>= 4) */
     uint32_t threads;        /* host encoder threads, 0 = all hardware threads */
+    uint32_t lz4_unit;       /* LZ4 sequences in whole units of this many bytes: 1 (the default,
+                              * byte-granular as LZ4_compress_default, core/comp_cpu.hpp:30), 2, 4;
+                              * 0 = the element width, 8-byte elements in 4-byte units, like the
+                              * reference's typed nvcomp LZ4 (core/apps.cuh:262-270,
+                              * core/comp.cuh:230-236) */
 } golap_encode_opts;
 
 /* Default options: LZ4, 4096-row pages, 262,144-row chunks (1 MiB of an int32 column, the

@@ -88,15 +88,47 @@ static int64_t lz4_token_offsets(const uint8_t *src, uint64_t csize, uint32_t *t
     return (int64_t)n;
 }
 
+/* Does every sequence of an LZ4 block (parsed as above) measure its literal run, its offset and
+ * its match in whole units of `unit` bytes? */
+static int lz4_in_units(const uint8_t *src, uint64_t csize, uint32_t unit) {
+    uint64_t ip = 0;
+    for (;;) {
+        uint32_t token = src[ip++];
+        uint64_t lit = token >> 4;
+        if (lit == 15) {
+            uint32_t b;
+            do {
+                b = src[ip++];
+                lit += b;
+            } while (b == 255);
+        }
+        if (lit % unit) return 0;
+        ip += lit;
+        if (ip == csize) return 1;
+        const uint32_t off = (uint32_t)src[ip] | ((uint32_t)src[ip + 1] << 8);
+        ip += 2;
+        uint64_t ml = token & 15;
+        if (ml == 15) {
+            uint32_t b;
+            do {
+                b = src[ip++];
+                ml += b;
+            } while (b == 255);
+        }
+        if (off % unit || (ml + 4) % unit) return 0;
+    }
+}
+
 /* LZ4I page (this build's page format, golap_public_amd/csrc/format.hpp): header {u16 nseq,
- * u8 seg_log2, u8 0}, u16 tok[nseg - 1] (block offset of the token of sequence s << seg_log2),
- * zero bytes to a multiple of 16, then a standard LZ4 block below 64 KiB. The page is valid iff
- * the block decodes (oracle_lz4_decode) and the header agrees with the block's serial parse:
- * nseq sequences, every indexed offset at the token of its sequence. */
+ * u8 seg_log2, u8 unit_log2}, u16 tok[nseg - 1] (block offset of the token of sequence
+ * s << seg_log2), zero bytes to a multiple of 16, then a standard LZ4 block below 64 KiB. The page
+ * is valid iff the block decodes (oracle_lz4_decode) and the header agrees with the block's
+ * serial parse: nseq sequences, every indexed offset at the token of its sequence, and with
+ * unit_log2 = u in 0..2 every literal run, offset and match a multiple of 2^u bytes. */
 int64_t oracle_lz4i_decode(const uint8_t *src, uint64_t csize, uint8_t *dst, uint64_t cap) {
     if (csize < 4) return -1;
     const uint32_t nseq = (uint32_t)src[0] | ((uint32_t)src[1] << 8), sl = src[2];
-    if (nseq == 0 || sl > 15 || src[3] != 0) return -1;
+    if (nseq == 0 || sl > 15 || src[3] > 2) return -1;
     const uint32_t nseg = (uint32_t)(((uint64_t)nseq + (1ull << sl) - 1) >> sl);
     const uint64_t hb = (4 + 2 * (uint64_t)(nseg - 1) + 15) & ~15ull;
     if (hb > csize || csize - hb > 65535) return -1;
@@ -110,7 +142,7 @@ int64_t oracle_lz4i_decode(const uint8_t *src, uint64_t csize, uint8_t *dst, uin
         ok = ((uint32_t)src[4 + 2 * (s - 1)] | ((uint32_t)src[5 + 2 * (s - 1)] << 8)) == tok[(uint64_t)s << sl];
     for (uint64_t i = 4 + 2 * (uint64_t)(nseg - 1); ok && i < hb; ++i) ok = src[i] == 0;
     free(tok);
-    if (!ok) return -1;
+    if (!ok || (src[3] && !lz4_in_units(blk, bs, 1u << src[3]))) return -1;
     return oracle_lz4_decode(blk, bs, dst, cap);
 }
 

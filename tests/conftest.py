@@ -83,7 +83,7 @@ LZ4I = 5
 
 
 def lz4i_header_bytes(page):
-    """Header bytes of an LZ4I page: {u16 nseq, u8 seg_log2, u8 0}, u16 offsets of every
+    """Header bytes of an LZ4I page: {u16 nseq, u8 seg_log2, u8 unit_log2}, u16 offsets of every
     2^seg_log2-th sequence's token, zero padding to a multiple of 16."""
     page = bytes(page)
     nseq, sl = page[0] | (page[1] << 8), page[2]

@@ -178,3 +178,51 @@ def test_corrupt_lz4i_index_reports_ecorrupt(orc, golap, ctx, damage, rows):
         assert e.value.status == golap.ECORRUPT
     finally:
         ctx.free_column(col)
+
+
+@pytest.mark.parametrize("dtype", [np.int16, np.int32, np.int64])
+@pytest.mark.parametrize("page_rows", [1024, 4096])
+def test_decode_typed_pages_match_values(orc, golap, ctx, dtype, page_rows):
+    """Typed LZ4 pages (sequences in whole units of the element width, 4 bytes for int64, as the
+    reference's typed nvcomp LZ4 codes them): the cooperative decoder copies whole words; the
+    result equals the values, the oracle and the byte-granular pages of the same values."""
+    r = rng(71)
+    n = page_rows * 5 + 77
+    v = (np.repeat(r.integers(0, 1000, n), r.integers(1, 6, n))[:n] * 7 + 12345).astype(dtype)  # runs: many matches
+    say("decode typed", np.dtype(dtype).name, "page_rows", page_rows)
+    for unit in (0, 1):
+        hc = golap.HostColumn.encode(v, codec=1, page_rows=page_rows, chunk_rows=page_rows * 2, lz4_min_match=4,
+                                     lz4_unit=unit)
+        a = hc.arrays()
+        lz4i = [p for p in range(a["desc"].n_pages) if a["page_codec"][p] == LZ4I]
+        assert len(lz4i) >= 5
+        first = a["comp"][int(a["page_off"][lz4i[0]]) + 3]
+        assert first == ((min(v.itemsize, 4).bit_length() - 1) if unit == 0 else 0)
+        want = orc.decode_column(a)
+        assert want[:v.nbytes].tobytes() == v.tobytes()
+        col = ctx.upload(hc)
+        try:
+            got, _ = ctx.decompress(col)
+            assert np.array_equal(got, want)
+        finally:
+            ctx.free_column(col)
+
+
+def test_false_unit_claim_reports_ecorrupt(orc, golap, ctx):
+    """A byte-granular page that claims 4-byte units is refused by the GPU decoders as by the
+    oracle (the cooperative decoder checks every sequence's lengths and offset)."""
+    r = rng(72)
+    v = (19920101 + r.integers(0, 2556, 4096)).astype(np.int32)
+    a = golap.HostColumn.encode(v, codec=1, page_rows=4096, chunk_rows=4096, lz4_min_match=4, lz4_unit=1).arrays()
+    b = bytearray(a["comp"][int(a["page_off"][0]):int(a["page_off"][0]) + int(a["page_csize"][0])].tobytes())
+    assert b[3] == 0 and orc.page_decode(LZ4I, bytes(b), v.nbytes)[0] == v.nbytes
+    b[3] = 2
+    assert orc.page_decode(LZ4I, bytes(b), v.nbytes)[0] == -1
+    hc = golap.HostColumn.assemble(golap.I32, 4096, 4096, 4096, [bytes(b)], [LZ4I])
+    col = ctx.upload(hc)
+    try:
+        with pytest.raises(golap.GolapError) as e:
+            ctx.decompress(col)
+        assert e.value.status == golap.ECORRUPT
+    finally:
+        ctx.free_column(col)

@@ -148,3 +148,16 @@ def test_encode_device_decodes_and_runs_q1(orc, golap, ctx):
         assert (r.sum, r.matches) == (want, cnt)
     finally:
         release(ctx, cols, m, bufs)
+
+
+@pytest.mark.parametrize("dtype", [np.int16, np.int32, np.int64])
+def test_encode_device_typed_matches_host(golap, ctx, dtype):
+    """Typed LZ4 (sequences in whole units of the element width): the device encoder's pages,
+    catalog and zone maps equal the host encoder's byte for byte."""
+    type_ = golap.TYPE_OF[np.dtype(dtype)]
+    for i, kind in enumerate(["runs", "lowrange", "text", "random"]):
+        v = pattern(kind, 90_001, dtype, seed=40 + i)
+        say("encode_device typed", np.dtype(dtype).name, kind)
+        for codec in ("lz4", "auto"):
+            compare(ctx, golap, v, type_, codec=CODECS[codec], page_rows=4096, chunk_rows=65536, lz4_min_match=4,
+                    lz4_unit=0)

@@ -46,15 +46,18 @@ PLANS = [  # (year, discount range, quantity range, price range)
 
 @pytest.mark.parametrize("plan", range(len(PLANS)))
 @pytest.mark.parametrize("codec,mm,page_rows", [(1, 16, 4096), (1, 4, 1024), (0, 4, 2048), (255, 4, 4096),
-                                                 (1, 4, 4096), (1, 4, 8192)])
+                                                 (1, 4, 4096), (1, 4, 8192), ("typed", 4, 4096)])
 def test_q1_matches_oracle(orc, golap, ctx, plan, codec, mm, page_rows):
     n = 6_000_000 if (plan == 0 and codec == 1 and page_rows == 4096) else 1_000_003
     ds = q1_dataset(orc, n)
     year, drng, qrng, prng = PLANS[plan]
     prng = prng or (-2**31, 2**31 - 1)
     say("q1 plan", plan, "codec", codec, "min_match", mm, "page_rows", page_rows, "rows", n)
+    unit = 1
+    if codec == "typed":  # LZ4 in whole units of the element width (date and price: 4 bytes)
+        codec, unit = 1, 0
     cols, hosts = upload_q1(ctx, golap, ds, codec=codec, page_rows=page_rows, chunk_rows=262144,
-                            lz4_min_match=mm)
+                            lz4_min_match=mm, lz4_unit=unit)
     m, bufs = date_map(ctx, golap, ds, year)
     try:
         r = ctx.q1_run(*cols, m, drng, qrng, prng)

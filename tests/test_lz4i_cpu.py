@@ -8,7 +8,10 @@ What pins them:
 - the oracle (oracle_lz4i_decode) accepts exactly the pages whose index agrees with that parse:
   damaged entries, counts, padding or segment lengths are refused, and a page re-indexed here
   with another segment length is still accepted (the format does not depend on the encoder's
-  choice)."""
+  choice);
+- typed pages (2- and 4-byte columns, the reference's typed nvcomp LZ4, core/apps.cuh:262-270)
+  record their unit in header byte 3, and the Python parse here checks that every literal run,
+  offset and match is a whole number of units; the oracle refuses a unit the block does not keep."""
 import numpy as np
 import pytest
 
@@ -52,14 +55,47 @@ def parse_tokens(blk):
                     break
 
 
-def make_page(blk, sl):
-    """An LZ4I page of block blk with segments of 2^sl sequences."""
+def parse_sequences(blk):
+    """(literal length, offset, match length) of every sequence of a well-formed LZ4 block; the
+    last sequence has offset and match length 0."""
+    ip, out = 0, []
+    while True:
+        t = blk[ip]
+        ip += 1
+        lit = t >> 4
+        if lit == 15:
+            while True:
+                b = blk[ip]
+                ip += 1
+                lit += b
+                if b != 255:
+                    break
+        ip += lit
+        if ip == len(blk):
+            out.append((lit, 0, 0))
+            return out
+        off = blk[ip] | (blk[ip + 1] << 8)
+        ip += 2
+        ml = t & 15
+        if ml == 15:
+            while True:
+                b = blk[ip]
+                ip += 1
+                ml += b
+                if b != 255:
+                    break
+        out.append((lit, off, ml + 4))
+
+
+def make_page(blk, sl, unit_log2=0):
+    """An LZ4I page of block blk with segments of 2^sl sequences, claiming units of 2^unit_log2
+    bytes."""
     toks = parse_tokens(blk)
     n = len(toks)
     nseg = -(-n // (1 << sl))
     hb = (4 + 2 * (nseg - 1) + 15) // 16 * 16
     h = bytearray(hb)
-    h[0], h[1], h[2] = n & 0xFF, n >> 8, sl
+    h[0], h[1], h[2], h[3] = n & 0xFF, n >> 8, sl, unit_log2
     for s in range(1, nseg):
         t = toks[s << sl]
         h[4 + 2 * (s - 1)], h[5 + 2 * (s - 1)] = t & 0xFF, t >> 8
@@ -86,8 +122,10 @@ def columns():
 
 @pytest.mark.parametrize("name,values", list(columns()), ids=[c[0] for c in columns()])
 @pytest.mark.parametrize("page_rows", [1024, 4096])
-def test_encoder_lz4i_pages_agree_with_parse_and_liblz4(orc, golap, lz4lib, name, values, page_rows):
-    hc = golap.HostColumn.encode(values, codec=1, page_rows=page_rows, chunk_rows=page_rows * 4, lz4_min_match=4)
+@pytest.mark.parametrize("typed", [False, True])
+def test_encoder_lz4i_pages_agree_with_parse_and_liblz4(orc, golap, lz4lib, name, values, page_rows, typed):
+    hc = golap.HostColumn.encode(values, codec=1, page_rows=page_rows, chunk_rows=page_rows * 4, lz4_min_match=4,
+                                 lz4_unit=0 if typed else 1)
     a = hc.arrays()
     d = a["desc"]
     raw = values.view(np.uint8)
@@ -107,7 +145,9 @@ def test_encoder_lz4i_pages_agree_with_parse_and_liblz4(orc, golap, lz4lib, name
         toks = parse_tokens(blk)
         assert toks is not None
         nseq, sl = page[0] | (page[1] << 8), page[2]
-        assert nseq == len(toks) and sl == encoder_sl(nseq) and page[3] == 0
+        unit = min(values.itemsize, 4) if typed else 1
+        assert nseq == len(toks) and sl == encoder_sl(nseq) and page[3] == unit.bit_length() - 1
+        assert all(lit % unit == 0 and off % unit == 0 and ml % unit == 0 for lit, off, ml in parse_sequences(blk))
         hb = lz4i_header_bytes(page)
         nseg = -(-nseq // (1 << sl))
         assert nseg <= 64 and hb % 16 == 0
@@ -124,7 +164,7 @@ def test_encoder_lz4i_pages_agree_with_parse_and_liblz4(orc, golap, lz4lib, name
 def test_oracle_refuses_damaged_indices(orc, golap):
     r = rng(42)
     v = (19920101 + r.integers(0, 2556, 4096)).astype(np.int32)
-    a = golap.HostColumn.encode(v, codec=1, page_rows=4096, chunk_rows=4096, lz4_min_match=4).arrays()
+    a = golap.HostColumn.encode(v, codec=1, page_rows=4096, chunk_rows=4096, lz4_min_match=4, lz4_unit=0).arrays()
     page = a["comp"][int(a["page_off"][0]):int(a["page_off"][0]) + int(a["page_csize"][0])].tobytes()
     assert a["page_codec"][0] == LZ4I
     want = v.tobytes()
@@ -149,8 +189,20 @@ def test_oracle_refuses_damaged_indices(orc, golap):
     b = bytearray(page)
     b[2] += 1  # longer segments: the entries point at the wrong sequences
     assert dec(bytes(b)) == -1
+    assert page[3] == 2  # 4-byte units: the column's element width
     b = bytearray(page)
-    b[3] = 1  # reserved byte
+    b[3] = 3  # no such unit
+    assert dec(bytes(b)) == -1
+    b = bytearray(page)
+    b[3] = 0  # a unit-aligned block may claim nothing
+    assert dec(bytes(b)) == len(want)
+    # a byte-granular block of the same values does not keep 4-byte units: claiming them is refused
+    a1 = golap.HostColumn.encode(v, codec=1, page_rows=4096, chunk_rows=4096, lz4_min_match=4, lz4_unit=1).arrays()
+    p1 = a1["comp"][int(a1["page_off"][0]):int(a1["page_off"][0]) + int(a1["page_csize"][0])].tobytes()
+    assert p1[3] == 0 and orc.page_decode(LZ4I, p1, len(want))[0] == len(want)
+    assert any(lit % 4 or off % 4 or ml % 4 for lit, off, ml in parse_sequences(lz4_block(p1, LZ4I)))
+    b = bytearray(p1)
+    b[3] = 2
     assert dec(bytes(b)) == -1
     hb = lz4i_header_bytes(page)
     nseg = -(-nseq // (1 << page[2]))
@@ -160,7 +212,7 @@ def test_oracle_refuses_damaged_indices(orc, golap):
         assert dec(bytes(b)) == -1
     # the same block indexed with other segment lengths is a valid page
     for sl in (4, 6, 9):
-        pg = make_page(blk, sl)
+        pg = make_page(blk, sl, 2)
         assert dec(pg) == len(want)
     # a block that no longer parses is refused whatever the index says
     assert dec(make_page(blk, 5)[:-1]) == -1
