@@ -279,8 +279,9 @@ void code_page(const uint8_t *src, uint64_t rows, uint32_t type, uint32_t elem, 
     // becomes an LZ4I page, the block behind its parse index, unless it is too large for one.
     std::vector<uint32_t> toks;
     auto lz4_candidate = [&]() {
-        cand.resize(160 + lz4_bound(nbytes) + 16);
-        uint8_t *blk = cand.data() + 160;  // behind the largest header (144 bytes)
+        constexpr uint64_t HDR_ROOM = (4 + 2 * (LZ4I_MAX_SEGS - 1) + 15) & ~15u;  // the largest LZ4I header
+        cand.resize(HDR_ROOM + lz4_bound(nbytes) + 16);
+        uint8_t *blk = cand.data() + HDR_ROOM;
         const uint64_t sz = lz4_encode_block(src, nbytes, blk, min_match, ht, base, &toks, unit);
         // a block with a match can have exactly the literal-only size: parse, do not compare sizes
         if (lz4_is_literal_only(blk, sz, nbytes)) {

@@ -43,8 +43,9 @@ constexpr uint32_t HT_SIZE = 1u << HT_LOG;
 constexpr uint32_t HT_EMPTY = 0xffffu;          // never accepted: a stored position is <= n - 12
 constexpr uint64_t LZ4_DEV_MAX_PAGE = 65536;    // page positions fit 16 bits
 // The LZ4 block is coded this far into its slot, room for the largest LZ4I header in front of
-// it (4 bytes + 63 index entries of 2 bytes, rounded up to 16).
-constexpr uint32_t LZ4I_HDR_MAX = 144;
+// it (4 bytes + 255 index entries of 2 bytes, rounded up to 16).
+constexpr uint32_t LZ4I_HDR_MAX = 528;
+static_assert(LZ4I_HDR_MAX >= ((4 + 2 * (golap_fmt::LZ4I_MAX_SEGS - 1) + 15) & ~15u), "LZ4I header room");
 
 struct EncodeArgs {
     const uint8_t *src;
@@ -141,7 +142,7 @@ __device__ __forceinline__ void copy_bytes(uint8_t *dst, const uint8_t *src, uin
 // pages. The workgroup is one wave, whose LDS accesses complete in issue order, so the probe
 // needs no barrier between reading and rewriting a table entry. A literal-only block is written
 // `shift` bytes into the slot so that its payload is 16-byte aligned there, as it will be in the
-// column. The block offset of every 16th sequence's token goes to rec[seq / 16] (lane 0).
+// column. The block offset of every 8th sequence's token goes to rec[seq / 8] (lane 0).
 // Returns the block size; literal_only, lit_shift (the literal-only block's offset in out) and
 // nseq (the block's sequences) set.
 __device__ __forceinline__ uint32_t lz4_page(const uint8_t *src, uint32_t n, uint8_t *out, uint32_t min_match,
@@ -175,7 +176,7 @@ __device__ __forceinline__ uint32_t lz4_page(const uint8_t *src, uint32_t n, uin
                 len -= len % unit;
                 if (len >= min_match) {
                     const uint32_t lit = ip - anchor, ml = len - LZ4_MINMATCH;
-                    if (lane == 0 && (nseq & 15) == 0) rec[nseq >> 4] = (uint16_t)op;
+                    if (lane == 0 && (nseq & 7) == 0) rec[nseq >> 3] = (uint16_t)op;
                     ++nseq;
                     op += put_token(out, op, lit, ml, true);
                     copy_bytes(out + op, src + anchor, lit);
@@ -211,7 +212,7 @@ __device__ __forceinline__ uint32_t lz4_page(const uint8_t *src, uint32_t n, uin
         lit_shift = (PAGE_ALIGN - hdr % PAGE_ALIGN) % PAGE_ALIGN;
         out += lit_shift;
     }
-    if (lane == 0 && (nseq & 15) == 0) rec[nseq >> 4] = (uint16_t)op;
+    if (lane == 0 && (nseq & 7) == 0) rec[nseq >> 3] = (uint16_t)op;
     ++nseq;
     op += put_token(out, op, lit, 0, false);
     copy_bytes(out + op, src + anchor, lit);
@@ -375,19 +376,19 @@ __device__ void raw_write(const uint8_t *src, uint32_t n, uint8_t *out) {
 }
 
 // Dynamic LDS of the LZ4 variant: the hash table, the page being parsed, then the token offset
-// of every 16th sequence. Every sequence but the last ends in a match of at least 4 bytes, so a
+// of every 8th sequence. Every sequence but the last ends in a match of at least 4 bytes, so a
 // page of n bytes has at most n / 4 + 1 sequences.
 constexpr uint32_t HT_BYTES = HT_SIZE * 2;
 __host__ __device__ inline uint32_t page_lds_bytes(uint32_t page_bytes) { return (page_bytes + 15) / 16 * 16; }
 __host__ __device__ inline uint32_t rec_lds_bytes(uint32_t page_bytes) {
-    return ((page_bytes / 4 + 1) / 16 + 1) * 2;
+    return ((page_bytes / 4 + 1) / 8 + 1) * 2;
 }
 uint32_t encode_lds_bytes(uint64_t page_bytes) {
     return HT_BYTES + page_lds_bytes((uint32_t)page_bytes) + rec_lds_bytes((uint32_t)page_bytes);
 }
 
 // The LZ4I header (format.hpp) of a block of nseq sequences into hdr (lz4i_header_bytes bytes,
-// ending where the block starts), from the token offsets of every 16th sequence.
+// ending where the block starts), from the token offsets of every 8th sequence.
 __device__ __forceinline__ uint32_t lz4i_put_header(uint8_t *hdr_end, uint32_t nseq, const uint16_t *rec, uint32_t unit) {
     const uint32_t sl = lz4i_seg_log2(nseq), nseg = lz4i_segments(nseq, sl), hb = lz4i_header_bytes(nseq, sl);
     uint8_t *h = hdr_end - hb;

@@ -52,8 +52,8 @@ __host__ __device__ inline uint32_t lz4_literal_header_bytes(uint32_t n) {
 
 // LZ4I page header bytes for nseq sequences indexed every 2^seg_log2 sequences.
 constexpr uint32_t LZ4I_MAX_BLOCK = 65535;  // token offsets fit 16 bits
-constexpr uint32_t LZ4I_MIN_SEG_LOG2 = 4;
-constexpr uint32_t LZ4I_MAX_SEGS = 64;      // the encoder's choice: one segment per lane
+constexpr uint32_t LZ4I_MIN_SEG_LOG2 = 3;
+constexpr uint32_t LZ4I_MAX_SEGS = 256;     // the encoder's choice: one segment per thread of a workgroup
 __host__ __device__ inline uint32_t lz4i_segments(uint32_t nseq, uint32_t seg_log2) {
     return (uint32_t)(((uint64_t)nseq + (1ull << seg_log2) - 1) >> seg_log2);
 }
@@ -61,7 +61,7 @@ __host__ __device__ inline uint32_t lz4i_header_bytes(uint32_t nseq, uint32_t se
     const uint32_t nseg = lz4i_segments(nseq, seg_log2);
     return (4 + 2 * (nseg ? nseg - 1 : 0) + 15) & ~15u;
 }
-// The encoder's segment length: the shortest power of two, at least 16 sequences, that cuts
+// The encoder's segment length: the shortest power of two, at least 8 sequences, that cuts
 // the block into at most LZ4I_MAX_SEGS segments.
 __host__ __device__ inline uint32_t lz4i_seg_log2(uint32_t nseq) {
     uint32_t l = LZ4I_MIN_SEG_LOG2;

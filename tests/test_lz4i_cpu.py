@@ -103,9 +103,9 @@ def make_page(blk, sl, unit_log2=0):
 
 
 def encoder_sl(n):
-    """The encoder's segment length: the smallest power of two >= 16 giving <= 64 segments."""
-    sl = 4
-    while (64 << sl) < n:
+    """The encoder's segment length: the smallest power of two >= 8 giving <= 256 segments."""
+    sl = 3
+    while (256 << sl) < n:
         sl += 1
     return sl
 
@@ -150,7 +150,7 @@ def test_encoder_lz4i_pages_agree_with_parse_and_liblz4(orc, golap, lz4lib, name
         assert all(lit % unit == 0 and off % unit == 0 and ml % unit == 0 for lit, off, ml in parse_sequences(blk))
         hb = lz4i_header_bytes(page)
         nseg = -(-nseq // (1 << sl))
-        assert nseg <= 64 and hb % 16 == 0
+        assert nseg <= 256 and hb % 16 == 0
         for s in range(1, nseg):
             assert page[4 + 2 * (s - 1)] | (page[5 + 2 * (s - 1)] << 8) == toks[s << sl]
         assert not any(page[4 + 2 * (nseg - 1):hb])
