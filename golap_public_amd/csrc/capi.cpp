@@ -395,18 +395,32 @@ int golap_impl::create_device_column(golap_ctx *ctx, const golap_host_column *h,
     c->d_chunk_min = static_cast<int64_t *>(p[4]);
     c->d_chunk_max = static_cast<int64_t *>(p[5]);
     register_extra(c.get());
-    if (comp) c->extra.lz4i_max_nseq = lz4i_max_nseq(*h, comp);
+    if (comp) {
+        c->extra.lz4i_max_nseq = lz4i_max_nseq(*h, comp, &c->extra.lz4i_coop_csize);
+        c->extra.lz4i_coop_known = true;
+    }
     *out = c.release();
     return GOLAP_OK;
 }
 
 // Most sequences of an LZ4I page of a host column whose page data is comp.
-uint32_t golap_impl::lz4i_max_nseq(const golap_host_column &h, const uint8_t *comp) {
-    uint32_t m = 0;
+// coop_csize (optional): the largest of those pages with more sequences than the cooperative
+// decode hands to the in-order one.
+uint32_t golap_impl::lz4i_max_nseq(const golap_host_column &h, const uint8_t *comp, uint32_t *coop_csize) {
+    uint32_t m = 0, cs = 0;
     for (uint64_t p = 0; p < h.n_pages; ++p)
-        if ((h.page_codec[p] & golap_fmt::PAGE_CODEC_MASK) == golap_fmt::LZ4I && h.page_csize[p] >= 4)
-            m = std::max<uint32_t>(m, (uint32_t)comp[h.page_off[p]] | ((uint32_t)comp[h.page_off[p] + 1] << 8));
+        if ((h.page_codec[p] & golap_fmt::PAGE_CODEC_MASK) == golap_fmt::LZ4I && h.page_csize[p] >= 4) {
+            const uint32_t n = (uint32_t)comp[h.page_off[p]] | ((uint32_t)comp[h.page_off[p] + 1] << 8);
+            m = std::max(m, n);
+            if (n > golap_fmt::LZ4I_SERIAL_MAX) cs = std::max(cs, h.page_csize[p]);
+        }
+    if (coop_csize) *coop_csize = cs;
     return m;
+}
+
+uint32_t golap_impl::lz4i_stage_bytes(const ColumnExtra &ex) {
+    if (!ex.has_lz4i) return 0;
+    return ((ex.lz4i_coop_known ? ex.lz4i_coop_csize : ex.lz4i_max_csize) + 15) & ~15u;
 }
 
 // The planning tables of a column from its host catalog (page kinds, payload per chunk).
@@ -808,13 +822,21 @@ int golap_q1_run(golap_ctx *ctx, const golap_q1_plan *plan, golap_q1_result *res
         if (nd == a.page_end - a.page_begin) any_direct = false;  // no tile has all four in place
         max_nseq = std::max(max_nseq, ex.lz4i_max_nseq);
         nseq_known = nseq_known && (ex.lz4i_max_nseq || !ex.has_lz4i);
+        a.stage_cap = std::max(a.stage_cap, lz4i_stage_bytes(ex));
     }
     // one window serves every column of a tile: sized for the widest (4-byte) column
     lz4i_window(d->page_rows * 4, nseq_known ? std::max(max_nseq, 1u) : 0, a.win_cap, a.tok_cap);
+    a.max_nseq = nseq_known ? max_nseq : 0;
     GOLAP_HIP_CHECK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     if (any && a.page_end > a.page_begin) {  // an empty predicate range or map matches nothing
         if (any_direct) GOLAP_HIP_CHECK(ctx, launch_q1(ctx, a, true));
-        if (any_general) GOLAP_HIP_CHECK(ctx, launch_q1(ctx, a, false));
+        if (any_general) {
+            // LZ4I pages: one workgroup per tile, decoding them cooperatively; other coded pages
+            // (and page shapes that kernel does not take): one wave per tile
+            hipError_t e = a.stage_cap ? launch_q1_wg(ctx, a) : hipErrorNotSupported;
+            if (e == hipErrorNotSupported) e = launch_q1(ctx, a, false);
+            GOLAP_HIP_CHECK(ctx, e);
+        }
     }
     GOLAP_HIP_CHECK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     uint64_t host[3] = {0, 0, 0};

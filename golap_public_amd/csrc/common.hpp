@@ -67,6 +67,10 @@ struct ColumnExtra {
     // page data was not seen on the host
     uint32_t lz4i_max_nseq = 0;
     uint32_t lz4i_max_csize = 0;  // largest LZ4I page (bytes), from the catalog
+    // largest LZ4I page of more than LZ4I_SERIAL_MAX sequences (the pages a cooperative decode
+    // stages in LDS); known when the page headers were read on the host
+    uint32_t lz4i_coop_csize = 0;
+    bool lz4i_coop_known = false;
     bool has_lz4i = false;
 };
 
@@ -152,7 +156,9 @@ int create_device_column(golap_ctx *ctx, const golap_host_column *h, uint64_t co
                          golap_column **out);
 // The host planning tables of a column, from its host catalog.
 void register_extra(golap_column *c);
-uint32_t lz4i_max_nseq(const golap_host_column &h, const uint8_t *comp);
+uint32_t lz4i_max_nseq(const golap_host_column &h, const uint8_t *comp, uint32_t *coop_csize = nullptr);
+// Bytes an LZ4I stage region needs for a column's pages (a multiple of 16).
+uint32_t lz4i_stage_bytes(const ColumnExtra &ex);
 // Points device column c (created with catalog capacity for at least w's pages and chunks) at a
 // window w of a larger column: catalog and zone maps copied (synchronously, on stream), planning
 // tables rebuilt; the caller fills d_comp with comp_bytes of page data.
@@ -464,8 +470,14 @@ struct Q1Launch {
     // tiles with coded pages: LDS decode window (a multiple of 16 bytes, 0 = the four-wave
     // kernel) and the LZ4I token offsets it holds
     uint32_t win_cap = 0, tok_cap = 0;
+    // the workgroup kernel: the largest LZ4I page of the four columns (bytes, a multiple of 16)
+    // and the most sequences of one (0: unknown, bounded by the page size)
+    uint32_t stage_cap = 0, max_nseq = 0;
 };
 hipError_t launch_q1(golap_ctx *ctx, const Q1Launch &a, bool direct);
+// Tiles with coded pages by one workgroup each, LZ4I pages decoded by the whole workgroup
+// (q1wg.hip); hipErrorNotSupported when the page shape or LDS does not allow it.
+hipError_t launch_q1_wg(golap_ctx *ctx, const Q1Launch &a);
 // Decode window and token capacity for LZ4I pages of at most page_bytes output bytes: the
 // window holds the output and, at its tail, the staged page (LZ4I pages shrink, so a slack of an
 // eighth of the page, at least 1 KiB, is enough for the in-place order); max_nseq 0 = unknown.

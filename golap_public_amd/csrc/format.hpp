@@ -53,7 +53,10 @@ __host__ __device__ inline uint32_t lz4_literal_header_bytes(uint32_t n) {
 // LZ4I page header bytes for nseq sequences indexed every 2^seg_log2 sequences.
 constexpr uint32_t LZ4I_MAX_BLOCK = 65535;  // token offsets fit 16 bits
 constexpr uint32_t LZ4I_MIN_SEG_LOG2 = 3;
-constexpr uint32_t LZ4I_MAX_SEGS = 256;     // the encoder's choice: one segment per thread of a workgroup
+constexpr uint32_t LZ4I_MAX_SEGS = 256;
+// Pages of at most this many sequences are decoded in order by one wave (few, long sequences:
+// cheapest that way); more go to the workgroup-cooperative decode.
+constexpr uint32_t LZ4I_SERIAL_MAX = 32;     // the encoder's choice: one segment per thread of a workgroup
 __host__ __device__ inline uint32_t lz4i_segments(uint32_t nseq, uint32_t seg_log2) {
     return (uint32_t)(((uint64_t)nseq + (1ull << seg_log2) - 1) >> seg_log2);
 }

@@ -33,7 +33,7 @@ namespace golap_dev {
 namespace lz4wg {
 
 constexpr uint32_t MAXJ = 8;         // sequences per thread (match descriptors kept in registers)
-constexpr uint32_t SERIAL_MAX = 32;  // pages with at most this many sequences are decoded in order by one wave
+constexpr uint32_t SERIAL_MAX = golap_fmt::LZ4I_SERIAL_MAX;  // pages with at most this many sequences: in order
 constexpr uint32_t SHORT = 16;       // literals and matches up to this long are copied in the main passes
 constexpr uint32_t LIST_MIN = 256;   // entries the lists hold at least
 
@@ -278,14 +278,24 @@ __device__ __forceinline__ bool walk(const uint8_t *blk, uint32_t bs, uint32_t i
     return ok && ip == end_ip;
 }
 
-// Decodes the LZ4I page staged at lds + l.stage (csize bytes, staged before a __syncthreads())
-// into lds + l.out[0, osize). All NT threads call it. Returns osize, -1 if the page is malformed,
-// -2 if this path does not take it: pages of at most SERIAL_MAX sequences (long literal runs,
-// cheapest in order), more sequences than MAXJ per thread or than l.tok_cap, more segments than
-// threads, a decoded size of 64 KiB or more, or more long literals or pending matches than a
-// list holds. The caller then decodes the page in order. The result is the same in every thread.
-template <int NT>
-__device__ __forceinline__ int64_t decode(uint8_t *lds, const Layout &l, uint32_t csize, uint32_t osize) {
+// What the parse leaves for the later phases: per thread, sequences k = tid + NT j (j < J):
+// lt[j] = literal start in the block | literal length << 16, mo[j] = match offset | match
+// length << 16, pos[j] = output position.
+struct Parsed {
+    uint32_t lt[MAXJ], mo[MAXJ], pos[MAXJ];
+    uint32_t J, nseq, ub, bs;
+    bool words;
+    const uint8_t *blk;
+};
+
+// Header, walk, parse and output positions of the LZ4I page staged at lds + l.stage (csize bytes,
+// staged before a __syncthreads()): 0, -1 (malformed) or -2 (not taken: pages of at most
+// SERIAL_MAX sequences, which are cheapest in order, more sequences than MAXJ per thread or than
+// l.tok_cap, more segments than threads, a decoded size of 64 KiB or more). FULL: the output
+// window is zeroed and the bitmap cleared for decode(). The result is the same in every thread.
+template <int NT, bool FULL>
+__device__ __forceinline__ int parse(uint8_t *lds, const Layout &l, uint32_t csize, uint32_t osize, BlockAny &any,
+                                     Parsed &ps) {
     constexpr uint32_t NW = NT / WAVE;
     const uint32_t tid = threadIdx.x, lane = tid % WAVE, wave = tid / WAVE;
     const uint8_t *hdr = lds + l.stage;
@@ -293,9 +303,6 @@ __device__ __forceinline__ int64_t decode(uint8_t *lds, const Layout &l, uint32_
     uint32_t *bits = reinterpret_cast<uint32_t *>(lds + l.bits);
     uint16_t *tok = reinterpret_cast<uint16_t *>(lds + l.tok);
     uint32_t *wtot = reinterpret_cast<uint32_t *>(lds + l.wtot);
-    uint32_t *misc = reinterpret_cast<uint32_t *>(lds + l.misc);
-    BlockAny any{misc + 4, 0};  // zeroed by stage()
-
     // header (uniform): {u16 nseq, u8 seg_log2, u8 unit_log2}, nseg - 1 token offsets, zero padding
     if (csize < 16 || csize > l.stage_cap || osize > l.out_cap) return -1;
     const uint32_t w = ld32(hdr);
@@ -313,9 +320,15 @@ __device__ __forceinline__ int64_t decode(uint8_t *lds, const Layout &l, uint32_
     const uint32_t umask = (1u << ul) - 1;  // lengths and offsets are multiples of the unit
     const bool words = ul == 2 && (osize & 3) == 0;  // whole aligned words
     const uint32_t ub = words ? 2u : 0u;             // bitmap unit: a word or a byte
+    ps.J = J;
+    ps.nseq = nseq;
+    ps.words = words;
+    ps.ub = ub;
+    ps.blk = blk;
+    ps.bs = bs;
 
     // walk; the output window zeroed (bytes are ORed in) and the bitmap cleared
-    {
+    if (FULL) {
         const uint32_t nz = words ? 0u : (osize + 15) / 16;
         u32x4 *o16 = reinterpret_cast<u32x4 *>(out);
         for (uint32_t i = tid; i < nz; i += NT) o16[i] = u32x4{0, 0, 0, 0};
@@ -333,7 +346,8 @@ __device__ __forceinline__ int64_t decode(uint8_t *lds, const Layout &l, uint32_
     // past a short last sequence stay inside the staged page's spare bytes). All token offsets
     // are read together, then the two 16-byte blocks at every token, then every offset.
     // lt[j] = literal start | literal length << 16, mo[j] = match offset | match length << 16.
-    uint32_t lt[MAXJ], mo[MAXJ], pos[MAXJ], rd[MAXJ], tk[MAXJ];
+    uint32_t *lt = ps.lt, *mo = ps.mo, *pos = ps.pos;
+    uint32_t rd[MAXJ], tk[MAXJ];
 #pragma unroll
     for (uint32_t j = 0; j < MAXJ; ++j) {
         const uint32_t k = tid + NT * j;
@@ -424,6 +438,28 @@ __device__ __forceinline__ int64_t decode(uint8_t *lds, const Layout &l, uint32_
         }
         if (carry != osize) return -1;  // every thread computes the same total
     }
+    return 0;
+}
+
+// Decodes the LZ4I page staged at lds + l.stage (csize bytes, staged before a __syncthreads())
+// into lds + l.out[0, osize). All NT threads call it. Returns osize, -1 if the page is malformed,
+// -2 if this path does not take it (parse's cases, or more long literals or pending matches than
+// a list holds). The caller then decodes the page in order. The result is the same in every thread.
+template <int NT>
+__device__ __forceinline__ int64_t decode(uint8_t *lds, const Layout &l, uint32_t csize, uint32_t osize) {
+    const uint32_t tid = threadIdx.x;
+    uint8_t *out = lds + l.out;
+    uint32_t *bits = reinterpret_cast<uint32_t *>(lds + l.bits);
+    uint32_t *misc = reinterpret_cast<uint32_t *>(lds + l.misc);
+    BlockAny any{misc + 4, 0};  // zeroed by stage()
+    Parsed ps;
+    if (const int r = parse<NT, true>(lds, l, csize, osize, any, ps)) return r;
+    const uint32_t J = ps.J, ub = ps.ub;
+    const bool words = ps.words;
+    const uint8_t *blk = ps.blk;
+    uint32_t *lt = ps.lt, *mo = ps.mo, *pos = ps.pos;
+    bool bad = false;
+
 
     // literals up to SHORT bytes; mo[j] becomes the match destination | offset << 16, ml[] its length
     uint32_t ml[MAXJ], pend = 0, longs = 0;
