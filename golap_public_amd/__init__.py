@@ -62,7 +62,8 @@ class StarDim(C.Structure):
 
 class StarPlan(C.Structure):
     _fields_ = [("dims", StarDim * 4), ("n_dims", u32), ("value", vp), ("chunk_begin", u64),
-                ("chunk_end", u64), ("dev_chunk_mask", vp), ("agg_mode", u32), ("lds_group_slots", u32)]
+                ("chunk_end", u64), ("dev_chunk_mask", vp), ("agg_mode", u32), ("lds_group_slots", u32),
+                ("value_minus", vp)]
 
 
 class StarResult(C.Structure):
@@ -645,7 +646,7 @@ class Context:
         return kout[:n * len(keys)].reshape(n, len(keys)), sums[:n], cnts[:n]
 
     def star_plan(self, dims, value, chunk_begin=0, chunk_end=None, dev_chunk_mask=None, agg_mode=AGG_AUTO,
-                  lds_group_slots=0):
+                  lds_group_slots=0, value_minus=None):
         if chunk_end is None:
             chunk_end = self.column_desc(value).n_chunks
         p = StarPlan()
@@ -654,13 +655,16 @@ class Context:
         p.n_dims = len(dims)
         p.value, p.chunk_begin, p.chunk_end = value, chunk_begin, chunk_end
         p.dev_chunk_mask, p.agg_mode, p.lds_group_slots = dev_chunk_mask, agg_mode, lds_group_slots
+        p.value_minus = value_minus
         return p
 
     def star_run(self, dims, value, chunk_begin=0, chunk_end=None, dev_chunk_mask=None, agg_mode=AGG_AUTO,
-                 lds_group_slots=0, max_groups=None):
+                 lds_group_slots=0, max_groups=None, value_minus=None):
         """dims: list of (fk column, map, group_lo, group_card). Returns (result, keys, sums, counts).
-        max_groups defaults to the group domain (the product of the group cards)."""
-        p = self.star_plan(dims, value, chunk_begin, chunk_end, dev_chunk_mask, agg_mode, lds_group_slots)
+        max_groups defaults to the group domain (the product of the group cards). value_minus: sum
+        value - value_minus."""
+        p = self.star_plan(dims, value, chunk_begin, chunk_end, dev_chunk_mask, agg_mode, lds_group_slots,
+                           value_minus)
         na = sum(1 for d in dims if d[3])
         if max_groups is None:
             max_groups = _domain(dims)

@@ -537,6 +537,7 @@ int batch_create_star(golap_ctx *ctx, const golap_star_query *q, uint32_t n, Bat
     b->qh.resize(n);
     b->qd.resize(n);
     for (uint32_t i = 0; i < n; ++i) {
+        if (b->sq[i].plan.value_minus) return bail(fail(ctx, GOLAP_EINVAL, "batch: value_minus is a golap_star_run option"));
         if (int r = prepare_query(ctx, &b->sq[i].plan, true, b->cs, b->qd[i], b->qh[i])) return bail(r);
         const golap_column *zc = b->sq[i].zone_col;
         if (zc) {

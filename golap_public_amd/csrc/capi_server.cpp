@@ -342,6 +342,7 @@ int size_request(golap_ctx *ctx, Request *r) {
         ColumnSet cs;
         StarQueryDev qd;
         QueryHost qh;
+        if (r->star.plan.value_minus) return fail(ctx, GOLAP_EINVAL, "server: value_minus is a golap_star_run option");
         if (int e = prepare_query(ctx, &r->star.plan, true, cs, qd, qh)) return e;
         const golap_column *zc = r->star.zone_col;
         if (zc && (zc->ctx != ctx || zc->n_rows != r->star.plan.value->n_rows ||

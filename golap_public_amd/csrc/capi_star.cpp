@@ -227,6 +227,14 @@ int prepare_query(golap_ctx *ctx, const golap_star_plan *plan, bool compact, Col
     if (int r = cs.index(ctx, v, &qd.val_col)) return r;
     qd.n_dims = (uint8_t)nd;
     qd.val_signed = v->type == GOLAP_I32;
+    if (const golap_column *vm = plan->value_minus) {
+        if (vm->ctx != ctx || !is32(vm)) return fail(ctx, GOLAP_EINVAL, "star: value_minus column must be 32-bit");
+        if (vm->n_rows != v->n_rows || vm->page_rows != v->page_rows || vm->chunk_rows != v->chunk_rows)
+            return fail(ctx, GOLAP_EINVAL, "star: columns are not row-aligned");
+        if (int r = cs.index(ctx, vm, &qd.val2_col)) return r;
+        qd.has_val2 = 1;
+        qd.val2_signed = vm->type == GOLAP_I32;
+    }
     // driver filter: dense exact bitmap, or the blocked Bloom filter of the map's keys
     if (int r = filter_view(ctx, plan->dims[qd.driver].map, &qd.filter, &qd.filter_words)) return r;
     const uint64_t ppc = v->chunk_rows / v->page_rows;
@@ -274,10 +282,11 @@ uint32_t assign_access(const ColumnSet &cs, uint64_t pb, uint64_t pe, uint32_t d
 // (empty = all run).
 void slice_stats(const golap_star_plan &plan, const std::vector<uint8_t> &run, golap_star_result *res) {
     // per-chunk payload prefixes of the plan's columns, looked up once
-    const uint64_t *pf[STAR_MAX_DIMS + 1];
+    const uint64_t *pf[STAR_MAX_DIMS + 2];
     uint32_t n = 0;
     for (uint32_t d = 0; d < plan.n_dims; ++d) pf[n++] = plan.dims[d].fk->extra.chunk_payload.data();
     pf[n++] = plan.value->extra.chunk_payload.data();
+    if (plan.value_minus) pf[n++] = plan.value_minus->extra.chunk_payload.data();
     for (uint64_t ch = plan.chunk_begin; ch < plan.chunk_end; ++ch) {
         if (!run.empty() && !run[ch - plan.chunk_begin]) {
             res->chunks_pruned++;

@@ -235,7 +235,10 @@ struct StarQueryDev {
     uint32_t driver;                // dimension whose filter is tested first
     DriverFilterView filter;
     uint32_t filter_words;
-    uint32_t pad0;
+    uint8_t has_val2;               // SUM(value - value2): the second value column val2_col
+    uint8_t val2_col;
+    uint8_t val2_signed;
+    uint8_t pad0;
     uint64_t n_groups;
     int64_t *g_sum;
     uint64_t *g_cnt;

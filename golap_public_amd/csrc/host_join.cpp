@@ -178,6 +178,7 @@ int golap_star_run_hostdim(golap_ctx *ctx, const golap_star_plan *plan, uint32_t
     ColumnSet cs;
     StarQueryDev qd;
     QueryHost qh;
+    if (plan && plan->value_minus) return fail(ctx, GOLAP_EINVAL, "co-execution: value_minus is a golap_star_run option");
     if (int r = prepare_query(ctx, plan, true, cs, qd, qh, (int)host_dim)) return r;
     const golap_column *v = plan->value;
     StarLaunch L{};

@@ -65,10 +65,27 @@ static_assert(sizeof(StarQueryDev) % 8 == 0, "query descriptor copied as words")
 // LDS of one workgroup besides the dynamic arrays.
 struct StarShared {
     StarQueryDev q;                        // current query
-    ColAccess acc[STAR_MAX_DIMS + 1];      // per tile: foreign keys of q in dimension order, then value
+    ColAccess acc[STAR_MAX_DIMS + 2];      // per tile: foreign keys of q in dimension order, value, value2
     uint32_t cand_n, tile_bad;
     uint32_t item;
 };
+
+// Column c of the tile's access list: foreign keys of the current query in dimension order, then
+// the value column, then the second value column.
+__device__ __forceinline__ uint32_t tile_col(const StarQueryDev &q, uint32_t c) {
+    return c < q.n_dims ? q.fk_col[c] : c == q.n_dims ? q.val_col : q.val2_col;
+}
+// The row's value: value, or value - value2 (each widened by its own signedness).
+__device__ __forceinline__ int64_t row_value(const StarShared &S, uint32_t r) {
+    const uint32_t nd = S.q.n_dims;
+    const int32_t raw = col_value(S.acc[nd], r);
+    int64_t v = S.q.val_signed ? (int64_t)raw : (int64_t)(uint32_t)raw;
+    if (S.q.has_val2) {
+        const int32_t raw2 = col_value(S.acc[nd + 1], r);
+        v -= S.q.val2_signed ? (int64_t)raw2 : (int64_t)(uint32_t)raw2;
+    }
+    return v;
+}
 
 // Emit mode: candidates probe every dimension but emit_dim exactly; the rows that pass all of
 // them are appended as {emit_dim's key, group index over the probed dimensions, value lo, hi}
@@ -108,8 +125,7 @@ __device__ __forceinline__ void star_emit(const StarArgs &a, StarShared &S, cons
                     gidx += (uint64_t)gv * pd.group_stride;
                 }
             }
-            const int32_t raw = col_value(S.acc[nd], r);
-            v = S.q.val_signed ? (int64_t)raw : (int64_t)(uint32_t)raw;
+            v = row_value(S, r);
         }
         const uint64_t m = __ballot(ok);
         if (!m) continue;
@@ -133,14 +149,14 @@ __device__ __forceinline__ void star_tile(const StarArgs &a, StarShared &S, uint
     constexpr int G = PER / 4;
     const uint32_t P = a.page_rows;
     const uint32_t wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    const uint32_t nd = S.q.n_dims, ncols = nd + 1;
+    const uint32_t nd = S.q.n_dims, ncols = nd + 1 + S.q.has_val2;
     const uint32_t rows = (uint32_t)min<uint64_t>(P, a.n_rows - p * P);
     if (threadIdx.x == 0) {
         S.cand_n = 0;
         S.tile_bad = 0;
     }
     if (threadIdx.x < ncols) {
-        const StarCol &sc = a.col[threadIdx.x < nd ? S.q.fk_col[threadIdx.x] : S.q.val_col];
+        const StarCol &sc = a.col[tile_col(S.q, threadIdx.x)];
         S.acc[threadIdx.x] = col_access(sc, p, rows, sc.window >= 0 ? win + (size_t)sc.window * P * 4 : nullptr,
                                         sc.anchors >= 0 ? ancs + (size_t)sc.anchors * (P / 32) : nullptr);
     }
@@ -222,7 +238,7 @@ __device__ __forceinline__ void star_tile(const StarArgs &a, StarShared &S, uint
             continue;
         }
         if (c == drv || S.acc[c].mode != ACC_WINDOW) continue;
-        const ColView &cv = a.col[c < nd ? S.q.fk_col[c] : S.q.val_col].c;
+        const ColView &cv = a.col[tile_col(S.q, c)].c;
         const int64_t got = page_decode_wave(cv.page_codec[p], cv.comp + cv.page_off[p], cv.page_csize[p],
                                              const_cast<uint8_t *>(S.acc[c].base), rows * 4, 4);
         if (got != (int64_t)rows * 4 && lane == 0) {
@@ -251,7 +267,7 @@ __device__ __forceinline__ void star_tile(const StarArgs &a, StarShared &S, uint
             // memory round trips for the row instead of two per dimension
             int32_t key[STAR_MAX_DIMS];
             for (uint32_t d = 0; d < nd; ++d) key[d] = col_value(S.acc[d], r);
-            const int32_t raw = col_value(S.acc[nd], r);
+            const int64_t v = row_value(S, r);
             uint32_t h[STAR_MAX_DIMS];
             uint64_t first[STAR_MAX_DIMS];
             for (uint32_t d = 0; d < nd; ++d) {
@@ -273,7 +289,6 @@ __device__ __forceinline__ void star_tile(const StarArgs &a, StarShared &S, uint
                 }
             }
             if (!ok) continue;
-            const int64_t v = S.q.val_signed ? (int64_t)raw : (int64_t)(uint32_t)raw;
             if (!use_lds || !group_add_lds(T, qkey | (uint32_t)gidx, v)) group_add_global(S.q.g_sum, S.q.g_cnt, gidx, v);
         }
     }

@@ -361,6 +361,10 @@ typedef struct {
     const uint16_t *dev_chunk_mask;
     uint32_t agg_mode;         /* golap_agg_mode */
     uint32_t lds_group_slots;  /* LDS partial-table slots (power of two); 0 = library choice */
+    /* optional: SUM(value - value_minus), the two row-aligned 32-bit columns each widened by
+     * its own signedness (pipeline_q4's lo_revenue - lo_supplycost, 04.cuh:99-153); NULL = SUM(value).
+     * golap_star_run only (batches, co-execution and streamed plans refuse it). */
+    const golap_column *value_minus;
 } golap_star_plan;
 typedef struct {
     uint64_t n_groups;

@@ -26,9 +26,11 @@ typedef struct {
     uint64_t q1[3];          /* Q1.1, Q1.2, Q1.3: SUM(lo_extendedprice*lo_discount) */
     uint64_t q2_sum[3];      /* Q2.1..Q2.3: sum over groups of SUM(lo_revenue) */
     uint64_t q2_groups[3];   /* Q2.1..Q2.3: number of (d_year, p_brand1) groups */
+    uint64_t q34_sum[7];     /* Q3.1..Q3.4 SUM(lo_revenue), Q4.1..Q4.3 SUM(lo_revenue - lo_supplycost) */
+    uint64_t q34_groups[7];  /* their numbers of groups */
 } oracle_ssb_golden;
 
-/* Streams the whole SSB database at scale factor sf and evaluates Q1.x / Q2.x on the fly. */
+/* Streams the whole SSB database at scale factor sf and evaluates Q1.x..Q4.x on the fly. */
 int oracle_ssb_golden_run(uint32_t sf, oracle_ssb_golden *out);
 
 /* Table sizes of SSB_Tables_col (SSB_def.hpp:144-153); lineorder rows are only known after
@@ -36,18 +38,27 @@ int oracle_ssb_golden_run(uint32_t sf, oracle_ssb_golden *out);
 void oracle_ssb_sizes(uint32_t sf, uint64_t *orders, uint64_t *parts, uint64_t *suppliers,
                       uint64_t *customers, uint64_t *dates);
 
-/* Materialises the columns the Q1/Q2 plans read. Any output pointer may be NULL.
- * Fact arrays must hold orders*7 rows (upper bound). Returns the number of lineorder rows.
- * Dimension outputs: d_key/d_year/d_yearmonthnum/d_weeknuminyear [2555],
- * p_category (num1*10+num2) / p_brand1 (num1*1000+num2*100+num3) [parts],
- * s_region [suppliers] (index into AFRICA, AMERICA, ASIA, EUROPE, MIDDLE EAST). */
-uint64_t oracle_ssb_generate(uint32_t sf,
-                             int32_t *lo_orderdate, uint8_t *lo_discount, uint8_t *lo_quantity,
-                             uint32_t *lo_extendedprice, uint32_t *lo_partkey,
-                             uint32_t *lo_suppkey, uint32_t *lo_revenue,
-                             int32_t *d_key, int16_t *d_year, int32_t *d_yearmonthnum,
-                             int16_t *d_weeknuminyear,
-                             int16_t *p_category, int16_t *p_brand1, uint8_t *s_region);
+/* Columns of the generated database; any pointer may be NULL. Fact arrays hold orders*7 rows
+ * (an upper bound; the row count is returned). Dimension arrays: d_* [2555]; p_* [parts]:
+ * p_category = num1*10+num2, p_brand1 = num1*1000+num2*100+num3, p_mfgr = num1; s_* [suppliers]
+ * and c_* [customers]: region = index into AFRICA, AMERICA, ASIA, EUROPE, MIDDLE EAST, nation =
+ * index into the sorted nation names, city = nation*10 + city code. */
+typedef struct {
+    int32_t *lo_orderdate;
+    uint8_t *lo_discount, *lo_quantity;
+    uint32_t *lo_extendedprice, *lo_partkey, *lo_suppkey, *lo_custkey, *lo_revenue, *lo_supplycost;
+    int32_t *d_key;
+    int16_t *d_year;
+    int32_t *d_yearmonthnum;
+    int16_t *d_weeknuminyear;
+    int16_t *p_category, *p_brand1;
+    uint8_t *p_mfgr;
+    uint8_t *s_region, *s_nation;
+    int16_t *s_city;
+    uint8_t *c_region, *c_nation;
+    int16_t *c_city;
+} oracle_ssb_cols;
+uint64_t oracle_ssb_generate(uint32_t sf, const oracle_ssb_cols *out);
 
 /* ------------------------------------------------------------------ synthetic data
  * Counter-based generator (SplitMix64 of (seed, column, row)) for the star-schema workloads.

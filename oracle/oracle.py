@@ -16,7 +16,22 @@ vp, u64, u32, i64 = C.c_void_p, C.c_uint64, C.c_uint32, C.c_int64
 
 
 class SsbGolden(C.Structure):
-    _fields_ = [("lineorder_rows", u64), ("q1", u64 * 3), ("q2_sum", u64 * 3), ("q2_groups", u64 * 3)]
+    _fields_ = [("lineorder_rows", u64), ("q1", u64 * 3), ("q2_sum", u64 * 3), ("q2_groups", u64 * 3),
+                ("q34_sum", u64 * 7), ("q34_groups", u64 * 7)]
+
+
+SSB_COLS = [("lo_orderdate", "lo", "orderdate"), ("lo_discount", "lo", "discount"), ("lo_quantity", "lo", "quantity"),
+            ("lo_extendedprice", "lo", "extendedprice"), ("lo_partkey", "lo", "partkey"),
+            ("lo_suppkey", "lo", "suppkey"), ("lo_custkey", "lo", "custkey"), ("lo_revenue", "lo", "revenue"),
+            ("lo_supplycost", "lo", "supplycost"), ("d_key", "d", "key"), ("d_year", "d", "year"),
+            ("d_yearmonthnum", "d", "yearmonthnum"), ("d_weeknuminyear", "d", "weeknuminyear"),
+            ("p_category", "p", "category"), ("p_brand1", "p", "brand1"), ("p_mfgr", "p", "mfgr"),
+            ("s_region", "s", "region"), ("s_nation", "s", "nation"), ("s_city", "s", "city"),
+            ("c_region", "c", "region"), ("c_nation", "c", "nation"), ("c_city", "c", "city")]
+
+
+class SsbCols(C.Structure):
+    _fields_ = [(name, vp) for name, _, _ in SSB_COLS]
 
 
 class Pred(C.Structure):
@@ -55,7 +70,7 @@ def load():
         protos = {
             "oracle_ssb_golden_run": (C.c_int, [u32, C.POINTER(SsbGolden)]),
             "oracle_ssb_sizes": (None, [u32] + [C.POINTER(u64)] * 5),
-            "oracle_ssb_generate": (u64, [u32] + [vp] * 14),
+            "oracle_ssb_generate": (u64, [u32, C.POINTER(SsbCols)]),
             "oracle_splitmix64": (u64, [u64]),
             "oracle_uniform": (u32, [u64, u32, u64, u32, u32]),
             "oracle_gen_q1_fact": (None, [u64, u64, u64, vp, u32, vp, vp, vp, vp]),
@@ -101,7 +116,8 @@ def _p(a):
 def ssb_golden(sf):
     g = SsbGolden()
     assert load().oracle_ssb_golden_run(sf, C.byref(g)) == 0
-    return dict(rows=g.lineorder_rows, q1=list(g.q1), q2_sum=list(g.q2_sum), q2_groups=list(g.q2_groups))
+    return dict(rows=g.lineorder_rows, q1=list(g.q1), q2_sum=list(g.q2_sum), q2_groups=list(g.q2_groups),
+                q34_sum=list(g.q34_sum), q34_groups=list(g.q34_groups))
 
 
 def ssb_sizes(sf):
@@ -112,25 +128,27 @@ def ssb_sizes(sf):
 
 
 def ssb_generate(sf):
-    """Columns of the SSB generator needed by the Q1/Q2 plans (lineorder + dims)."""
-    s = ssb_sizes(sf)
-    cap = s["orders"] * 7
+    """Columns of the SSB generator (lineorder + the four dimensions): (lo, d, p, s, c) dicts."""
+    z = ssb_sizes(sf)
+    cap = z["orders"] * 7
+    u32c = lambda: np.empty(cap, np.uint32)
     lo = dict(orderdate=np.empty(cap, np.int32), discount=np.empty(cap, np.uint8),
-              quantity=np.empty(cap, np.uint8), extendedprice=np.empty(cap, np.uint32),
-              partkey=np.empty(cap, np.uint32), suppkey=np.empty(cap, np.uint32),
-              revenue=np.empty(cap, np.uint32))
+              quantity=np.empty(cap, np.uint8), extendedprice=u32c(), partkey=u32c(), suppkey=u32c(),
+              custkey=u32c(), revenue=u32c(), supplycost=u32c())
     d = dict(key=np.empty(2555, np.int32), year=np.empty(2555, np.int16),
              yearmonthnum=np.empty(2555, np.int32), weeknuminyear=np.empty(2555, np.int16))
-    p = dict(category=np.empty(s["parts"], np.int16), brand1=np.empty(s["parts"], np.int16))
-    sup = dict(region=np.empty(s["suppliers"], np.uint8))
-    n = load().oracle_ssb_generate(sf, *[_p(lo[k]) for k in ("orderdate", "discount", "quantity",
-                                                              "extendedprice", "partkey", "suppkey",
-                                                              "revenue")],
-                                   _p(d["key"]), _p(d["year"]), _p(d["yearmonthnum"]),
-                                   _p(d["weeknuminyear"]), _p(p["category"]), _p(p["brand1"]),
-                                   _p(sup["region"]))
+    p = dict(category=np.empty(z["parts"], np.int16), brand1=np.empty(z["parts"], np.int16),
+             mfgr=np.empty(z["parts"], np.uint8))
+    sup = dict(region=np.empty(z["suppliers"], np.uint8), nation=np.empty(z["suppliers"], np.uint8),
+               city=np.empty(z["suppliers"], np.int16))
+    cus = dict(region=np.empty(z["customers"], np.uint8), nation=np.empty(z["customers"], np.uint8),
+               city=np.empty(z["customers"], np.int16))
+    tabs = dict(lo=lo, d=d, p=p, s=sup, c=cus)
+    cols = SsbCols(*[_p(tabs[t][k]) for _, t, k in SSB_COLS])
+    n = load().oracle_ssb_generate(sf, C.byref(cols))
+    assert n > 0
     lo = {k: v[:n].copy() for k, v in lo.items()}
-    return lo, d, p, sup
+    return lo, d, p, sup, cus
 
 
 def uniform(seed, col, row, lo, hi):

@@ -76,24 +76,34 @@ static void gen_dates(ssb_dates *d) {
     }
 }
 
+/* Nation, city and region as integers: nation = index into the sorted nation names, city =
+ * nation * 10 + city_code (the city string is the nation name's first nine characters followed by
+ * the code, SSB_def.hpp:335,362, and those prefixes differ between nations, so the mapping is
+ * one to one), region = index into AFRICA, AMERICA, ASIA, EUROPE, MIDDLE EAST. */
 typedef struct {
     uint64_t seed;
     uint64_t orders, parts, suppliers, customers;
     ssb_dates dates;
     int16_t *p_category, *p_brand1;
-    uint8_t *s_region;
+    uint8_t *p_mfgr;
+    uint8_t *s_region, *s_nation;
+    int16_t *s_city;
+    uint8_t *c_region, *c_nation;
+    int16_t *c_city;
 } ssb_state;
 
-/* populate_customer (SSB_def.hpp:344-370): 6 draws per row; only the LCG state matters for
- * the Q1/Q2 plans. */
+/* populate_customer (SSB_def.hpp:344-370): 6 draws per row. */
 static void gen_customers(ssb_state *s) {
     for (uint64_t i = 0; i < s->customers; ++i) {
-        lcg_uniform(&s->seed, 0, 25);     /* nation_idx */
-        lcg_uniform(&s->seed, 0, 10);     /* city_code */
+        uint32_t nation = lcg_uniform(&s->seed, 0, 25);
+        uint32_t city = lcg_uniform(&s->seed, 0, 10);
         lcg_uniform(&s->seed, 100, 1000); /* acode */
         lcg_uniform(&s->seed, 100, 1000); /* exchg */
         lcg_uniform(&s->seed, 1000, 10000); /* pnumber */
         lcg_uniform(&s->seed, 0, N_SEGMENT); /* mktsegment */
+        s->c_nation[i] = (uint8_t)nation;
+        s->c_city[i] = (int16_t)(nation * 10 + city);
+        s->c_region[i] = (uint8_t)NATION_REGION_SORTED[nation];
     }
 }
 
@@ -101,11 +111,13 @@ static void gen_customers(ssb_state *s) {
 static void gen_suppliers(ssb_state *s) {
     for (uint64_t i = 0; i < s->suppliers; ++i) {
         uint32_t nation = lcg_uniform(&s->seed, 0, 25);
-        lcg_uniform(&s->seed, 0, 10);
+        uint32_t city = lcg_uniform(&s->seed, 0, 10);
         lcg_uniform(&s->seed, 100, 1000);
         lcg_uniform(&s->seed, 100, 1000);
         lcg_uniform(&s->seed, 1000, 10000);
         s->s_region[i] = (uint8_t)NATION_REGION_SORTED[nation];
+        s->s_nation[i] = (uint8_t)nation;
+        s->s_city[i] = (int16_t)(nation * 10 + city);
     }
 }
 
@@ -128,6 +140,7 @@ static void gen_parts(ssb_state *s) {
         lcg_uniform(&s->seed, 0, N_COLOR); /* name2 */
         lcg_uniform(&s->seed, 0, N_COLOR); /* col1 */
         s->p_category[i] = (int16_t)(num1 * 10 + num2);
+        s->p_mfgr[i] = (uint8_t)num1; /* "MFGR#<num1>", SSB_def.hpp:302 */
         s->p_brand1[i] = (int16_t)(num1 * 1000 + num2 * 100 + num3);
     }
 }
@@ -140,8 +153,16 @@ static int ssb_init(ssb_state *s, uint32_t sf) {
     gen_dates(&s->dates);
     s->p_category = (int16_t *)malloc(s->parts * sizeof(int16_t));
     s->p_brand1 = (int16_t *)malloc(s->parts * sizeof(int16_t));
+    s->p_mfgr = (uint8_t *)malloc(s->parts);
     s->s_region = (uint8_t *)malloc(s->suppliers);
-    if (!s->p_category || !s->p_brand1 || !s->s_region) return -1;
+    s->s_nation = (uint8_t *)malloc(s->suppliers);
+    s->s_city = (int16_t *)malloc(s->suppliers * sizeof(int16_t));
+    s->c_region = (uint8_t *)malloc(s->customers);
+    s->c_nation = (uint8_t *)malloc(s->customers);
+    s->c_city = (int16_t *)malloc(s->customers * sizeof(int16_t));
+    if (!s->p_category || !s->p_brand1 || !s->p_mfgr || !s->s_region || !s->s_nation || !s->s_city ||
+        !s->c_region || !s->c_nation || !s->c_city)
+        return -1;
     gen_customers(s);
     gen_suppliers(s);
     gen_parts(s);
@@ -151,15 +172,21 @@ static int ssb_init(ssb_state *s, uint32_t sf) {
 static void ssb_free(ssb_state *s) {
     free(s->p_category);
     free(s->p_brand1);
+    free(s->p_mfgr);
     free(s->s_region);
+    free(s->s_nation);
+    free(s->s_city);
+    free(s->c_region);
+    free(s->c_nation);
+    free(s->c_city);
 }
 
 typedef struct {
     int32_t orderdate;
     uint32_t date_idx;
     uint8_t quantity, discount;
-    uint32_t extendedprice, revenue;
-    uint32_t partkey, suppkey;
+    uint32_t extendedprice, revenue, supplycost;
+    uint32_t partkey, suppkey, custkey;
 } ssb_line;
 
 /* populate_lineorder (SSB_def.hpp:201-279). Calls fn for each generated line in order. */
@@ -186,6 +213,8 @@ static uint64_t gen_lineorder(ssb_state *s, line_fn fn, void *ctx) {
             /* lo_revenue = cur_extendedprice * (100.0-cur_discount)/100 stored as uint32
              * (SSB_def.hpp:251): double arithmetic, truncated on assignment. */
             x.revenue = (uint32_t)((double)x.extendedprice * (100.0 - (double)x.discount) / 100.0);
+            x.supplycost = 6 * partkey_to_price(x.partkey) / 10; /* SSB_def.hpp:252 */
+            x.custkey = cust;
             lcg_uniform(&s->seed, 30, 91);          /* commitdate offset */
             lcg_uniform(&s->seed, 0, N_SHIPMODE);   /* shipmode */
             if (fn) fn(ctx, total + l, &x);
@@ -198,12 +227,26 @@ static uint64_t gen_lineorder(ssb_state *s, line_fn fn, void *ctx) {
 /* ---- golden evaluation: Q1.x (01inmem.hpp:51-141), Q2.x (02inmem.hpp:58-206) ---- */
 #define Q2_YEARS 7
 #define Q2_BRANDS 6000
+/* Q3.x / Q4.x group tables (dense, indexed by the integer group attributes) */
+#define Q3N (25u * 25u * Q2_YEARS)   /* Q3.1: (c_nation, s_nation, d_year) */
+#define Q3C (250u * 250u * Q2_YEARS) /* Q3.2..Q3.4: (c_city, s_city, d_year) */
+#define Q41 (Q2_YEARS * 25u)         /* (d_year, c_nation) */
+#define Q42 (Q2_YEARS * 25u * 56u)   /* (d_year, s_nation, p_category) */
+#define Q43 (Q2_YEARS * 250u * 41u)  /* (d_year, s_city, p_brand1 - 1400) for category MFGR#14 */
+static const uint32_t Q34_SLOTS[7] = {Q3N, Q3C, Q3C, Q3C, Q41, Q42, Q43};
 typedef struct {
     ssb_state *s;
     uint64_t q1[3];
     uint64_t *q2sum[3];
     uint64_t *q2cnt[3];
+    uint64_t *qsum[7]; /* Q3.1..Q3.4, Q4.1..Q4.3 */
+    uint64_t *qcnt[7];
 } golden_ctx;
+
+static inline void q34_add(golden_ctx *g, int q, uint32_t slot, uint64_t v) {
+    g->qsum[q][slot] += v;
+    g->qcnt[q][slot] += 1;
+}
 
 static void golden_line(void *vctx, uint64_t row, const ssb_line *x) {
     (void)row;
@@ -235,6 +278,33 @@ static void golden_line(void *vctx, uint64_t row, const ssb_line *x) {
         g->q2sum[q][slot] += x->revenue;
         g->q2cnt[q][slot] += 1;
     }
+
+    /* Q3.x (03.hpp:611-677, pipeline_q3 03.cuh:118-170): customer, supplier and date
+     * predicates, SUM(lo_revenue) by (customer attribute, supplier attribute, d_year) */
+    const ssb_state *s = g->s;
+    const uint32_t c = x->custkey, sp = x->suppkey, y = (uint32_t)(d->year[di] - 1992);
+    const int y9297 = d->year[di] >= 1992 && d->year[di] <= 1997;
+    const int cki = s->c_city[c] == 221 || s->c_city[c] == 225; /* UNITED KI1, UNITED KI5 */
+    const int ski = s->s_city[sp] == 221 || s->s_city[sp] == 225;
+    if (s->c_region[c] == 2 && s->s_region[sp] == 2 && y9297) /* ASIA, ASIA */
+        q34_add(g, 0, ((uint32_t)s->c_nation[c] * 25u + s->s_nation[sp]) * Q2_YEARS + y, x->revenue);
+    const uint32_t cs = ((uint32_t)s->c_city[c] * 250u + (uint32_t)s->s_city[sp]) * Q2_YEARS + y;
+    if (s->c_nation[c] == 23 && s->s_nation[sp] == 23 && y9297) /* UNITED STATES */
+        q34_add(g, 1, cs, x->revenue);
+    if (cki && ski && y9297) q34_add(g, 2, cs, x->revenue);
+    if (cki && ski && d->yearmonthnum[di] == 199712) q34_add(g, 3, cs, x->revenue); /* Dec1997 */
+
+    /* Q4.x (04.hpp:316-381, pipeline_q4 04.cuh:99-153): SUM(lo_revenue - lo_supplycost), the
+     * difference taken in 32-bit unsigned arithmetic as the reference's uint32 columns give it */
+    const uint64_t profit = (uint32_t)(x->revenue - x->supplycost);
+    const uint32_t mf = s->p_mfgr[x->partkey];
+    const int y9798 = d->year[di] == 1997 || d->year[di] == 1998;
+    if (s->c_region[c] == 1 && s->s_region[sp] == 1 && (mf == 1 || mf == 2)) { /* AMERICA, MFGR#1/2 */
+        q34_add(g, 4, y * 25u + s->c_nation[c], profit);
+        if (y9798) q34_add(g, 5, (y * 25u + s->s_nation[sp]) * 56u + (uint32_t)cat, profit);
+    }
+    if (s->c_region[c] == 1 && s->s_nation[sp] == 23 && y9798 && cat == 14) /* MFGR#14 */
+        q34_add(g, 6, (y * 250u + (uint32_t)s->s_city[sp]) * 41u + (uint32_t)(brand - 1400), profit);
 }
 
 int oracle_ssb_golden_run(uint32_t sf, oracle_ssb_golden *out) {
@@ -246,6 +316,10 @@ int oracle_ssb_golden_run(uint32_t sf, oracle_ssb_golden *out) {
     for (int q = 0; q < 3; ++q) {
         g.q2sum[q] = (uint64_t *)calloc(Q2_YEARS * Q2_BRANDS, sizeof(uint64_t));
         g.q2cnt[q] = (uint64_t *)calloc(Q2_YEARS * Q2_BRANDS, sizeof(uint64_t));
+    }
+    for (int q = 0; q < 7; ++q) {
+        g.qsum[q] = (uint64_t *)calloc(Q34_SLOTS[q], sizeof(uint64_t));
+        g.qcnt[q] = (uint64_t *)calloc(Q34_SLOTS[q], sizeof(uint64_t));
     }
     out->lineorder_rows = gen_lineorder(&s, golden_line, &g);
     for (int q = 0; q < 3; ++q) {
@@ -262,46 +336,57 @@ int oracle_ssb_golden_run(uint32_t sf, oracle_ssb_golden *out) {
         free(g.q2sum[q]);
         free(g.q2cnt[q]);
     }
+    for (int q = 0; q < 7; ++q) {
+        uint64_t sum = 0, groups = 0;
+        for (uint32_t i = 0; i < Q34_SLOTS[q]; ++i) {
+            if (g.qcnt[q][i]) {
+                groups++;
+                sum += g.qsum[q][i];
+            }
+        }
+        out->q34_sum[q] = sum;
+        out->q34_groups[q] = groups;
+        free(g.qsum[q]);
+        free(g.qcnt[q]);
+    }
     ssb_free(&s);
     return 0;
 }
 
-typedef struct {
-    int32_t *orderdate;
-    uint8_t *discount, *quantity;
-    uint32_t *extendedprice, *partkey, *suppkey, *revenue;
-} mat_ctx;
-
 static void mat_line(void *vctx, uint64_t row, const ssb_line *x) {
-    mat_ctx *m = (mat_ctx *)vctx;
-    if (m->orderdate) m->orderdate[row] = x->orderdate;
-    if (m->discount) m->discount[row] = x->discount;
-    if (m->quantity) m->quantity[row] = x->quantity;
-    if (m->extendedprice) m->extendedprice[row] = x->extendedprice;
-    if (m->partkey) m->partkey[row] = x->partkey;
-    if (m->suppkey) m->suppkey[row] = x->suppkey;
-    if (m->revenue) m->revenue[row] = x->revenue;
+    const oracle_ssb_cols *m = (const oracle_ssb_cols *)vctx;
+    if (m->lo_orderdate) m->lo_orderdate[row] = x->orderdate;
+    if (m->lo_discount) m->lo_discount[row] = x->discount;
+    if (m->lo_quantity) m->lo_quantity[row] = x->quantity;
+    if (m->lo_extendedprice) m->lo_extendedprice[row] = x->extendedprice;
+    if (m->lo_partkey) m->lo_partkey[row] = x->partkey;
+    if (m->lo_suppkey) m->lo_suppkey[row] = x->suppkey;
+    if (m->lo_custkey) m->lo_custkey[row] = x->custkey;
+    if (m->lo_revenue) m->lo_revenue[row] = x->revenue;
+    if (m->lo_supplycost) m->lo_supplycost[row] = x->supplycost;
 }
 
-uint64_t oracle_ssb_generate(uint32_t sf, int32_t *lo_orderdate, uint8_t *lo_discount,
-                             uint8_t *lo_quantity, uint32_t *lo_extendedprice,
-                             uint32_t *lo_partkey, uint32_t *lo_suppkey, uint32_t *lo_revenue,
-                             int32_t *d_key, int16_t *d_year, int32_t *d_yearmonthnum,
-                             int16_t *d_weeknuminyear, int16_t *p_category, int16_t *p_brand1,
-                             uint8_t *s_region) {
+#define COPY_OUT(dst, src, n) \
+    do {                      \
+        if (dst) memcpy(dst, src, n); \
+    } while (0)
+uint64_t oracle_ssb_generate(uint32_t sf, const oracle_ssb_cols *o) {
     ssb_state s;
     if (ssb_init(&s, sf)) return 0;
-    mat_ctx m = {lo_orderdate, lo_discount, lo_quantity, lo_extendedprice,
-                 lo_partkey,   lo_suppkey,  lo_revenue};
-    uint64_t rows = gen_lineorder(&s, mat_line, &m);
-    if (d_key) memcpy(d_key, s.dates.key, sizeof(s.dates.key));
-    if (d_year) memcpy(d_year, s.dates.year, sizeof(s.dates.year));
-    if (d_yearmonthnum) memcpy(d_yearmonthnum, s.dates.yearmonthnum, sizeof(s.dates.yearmonthnum));
-    if (d_weeknuminyear)
-        memcpy(d_weeknuminyear, s.dates.weeknuminyear, sizeof(s.dates.weeknuminyear));
-    if (p_category) memcpy(p_category, s.p_category, s.parts * sizeof(int16_t));
-    if (p_brand1) memcpy(p_brand1, s.p_brand1, s.parts * sizeof(int16_t));
-    if (s_region) memcpy(s_region, s.s_region, s.suppliers);
+    uint64_t rows = gen_lineorder(&s, mat_line, (void *)o);
+    COPY_OUT(o->d_key, s.dates.key, sizeof(s.dates.key));
+    COPY_OUT(o->d_year, s.dates.year, sizeof(s.dates.year));
+    COPY_OUT(o->d_yearmonthnum, s.dates.yearmonthnum, sizeof(s.dates.yearmonthnum));
+    COPY_OUT(o->d_weeknuminyear, s.dates.weeknuminyear, sizeof(s.dates.weeknuminyear));
+    COPY_OUT(o->p_category, s.p_category, s.parts * sizeof(int16_t));
+    COPY_OUT(o->p_brand1, s.p_brand1, s.parts * sizeof(int16_t));
+    COPY_OUT(o->p_mfgr, s.p_mfgr, s.parts);
+    COPY_OUT(o->s_region, s.s_region, s.suppliers);
+    COPY_OUT(o->s_nation, s.s_nation, s.suppliers);
+    COPY_OUT(o->s_city, s.s_city, s.suppliers * sizeof(int16_t));
+    COPY_OUT(o->c_region, s.c_region, s.customers);
+    COPY_OUT(o->c_nation, s.c_nation, s.customers);
+    COPY_OUT(o->c_city, s.c_city, s.customers * sizeof(int16_t));
     ssb_free(&s);
     return rows;
 }

@@ -140,7 +140,7 @@ def ssb10(orc):
 
 @pytest.mark.parametrize("q", [0, 1, 2])
 def test_ssb_sf10_q1_reproduces_reference_log(golap, ctx, ssb10, q):
-    lo, d, _, _ = ssb10
+    lo, d, _, _, _ = ssb10
     say("ssb sf10 q1." + str(q + 1), "rows", lo["orderdate"].size)
     price = lo["extendedprice"]
     assert price.max() < 2**31

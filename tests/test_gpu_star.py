@@ -180,7 +180,7 @@ def ssb10(orc):
 def test_ssb_sf10_q2_reproduces_reference_log(orc, golap, ctx, ssb10, q):
     """Q2.1: p_category = MFGR#12, s_region = AMERICA; Q2.2: p_brand1 in MFGR#2221..MFGR#2228,
     s_region = ASIA; Q2.3: p_brand1 = MFGR#2239, s_region = EUROPE; group by (d_year, p_brand1)."""
-    lo, d, p, s = ssb10
+    lo, d, p, s, _ = ssb10
     say("ssb sf10 q2." + str(q + 1), "rows", lo["orderdate"].size)
     n_parts, n_supp = p["category"].size, s["region"].size
     part_pred = [(p["category"], 12, 12)] if q == 0 else [(p["brand1"], 2221, 2228)] if q == 1 else \
@@ -211,5 +211,104 @@ def test_ssb_sf10_q2_reproduces_reference_log(orc, golap, ctx, ssb10, q):
             ctx.free_column(c_)
         for m in maps:
             ctx.free_hashmap(m)
+        for b in bufs:
+            ctx.free(b)
+
+
+def _ki(city):
+    """CityKIPred (query_common.cuh:73-82): city 'UNITED KI1' or 'UNITED KI5' (221, 225)."""
+    return ((city == 221) | (city == 225)).astype(np.uint8)
+
+
+# SSB Q3.x / Q4.x (03.hpp:611-677, 04.hpp:316-381): per dimension (predicates, group attribute,
+# group range); dimensions in group-key order. The "IN (KI1, KI5)" city predicate is a derived
+# 0/1 column of the dimension (CityKIPred evaluated over the dimension rows).
+def _q34(q, d, p, s, c):
+    date = ("date", d["key"])
+    cust = ("cust", np.arange(c["region"].size, dtype=np.int32))
+    supp = ("supp", np.arange(s["region"].size, dtype=np.int32))
+    part = ("part", np.arange(p["category"].size, dtype=np.int32))
+    y9297 = [(d["year"], 1992, 1997)]
+    y9798 = [(d["year"], 1997, 1998)]
+    year = (d["year"], 1992, 7)
+    if q == 0:
+        return [(cust, [(c["region"], 2, 2)], (c["nation"], 0, 25)), (supp, [(s["region"], 2, 2)], (s["nation"], 0, 25)),
+                (date, y9297, year)], False
+    if q == 1:
+        return [(cust, [(c["nation"], 23, 23)], (c["city"], 0, 250)), (supp, [(s["nation"], 23, 23)], (s["city"], 0, 250)),
+                (date, y9297, year)], False
+    if q in (2, 3):
+        dp = y9297 if q == 2 else [(d["yearmonthnum"], 199712, 199712)]
+        return [(cust, [(_ki(c["city"]), 1, 1)], (c["city"], 0, 250)), (supp, [(_ki(s["city"]), 1, 1)], (s["city"], 0, 250)),
+                (date, dp, year)], False
+    if q == 4:
+        return [(date, [], year), (cust, [(c["region"], 1, 1)], (c["nation"], 0, 25)),
+                (supp, [(s["region"], 1, 1)], None), (part, [(p["mfgr"], 1, 2)], None)], True
+    if q == 5:
+        return [(date, y9798, year), (cust, [(c["region"], 1, 1)], None), (supp, [(s["region"], 1, 1)], (s["nation"], 0, 25)),
+                (part, [(p["mfgr"], 1, 2)], (p["category"], 11, 45))], True
+    return [(date, y9798, year), (cust, [(c["region"], 1, 1)], None), (supp, [(s["nation"], 23, 23)], (s["city"], 0, 250)),
+            (part, [(p["category"], 14, 14)], (p["brand1"], 1401, 40))], True
+
+
+SF10_Q34 = [(7511687354847, 150), (301286847125, 600), (12099576386, 24), (140730476, 4),  # query_sf10_export.csv
+            (3440778955099, 35), (981566989074, 100), (19815949200, 798)]
+
+
+@pytest.mark.parametrize("q", range(7), ids=["q3.1", "q3.2", "q3.3", "q3.4", "q4.1", "q4.2", "q4.3"])
+def test_ssb_sf10_q3_q4_reproduce_reference_log(orc, golap, ctx, ssb10, q):
+    """pipeline_q3 (03.cuh:118-170): SUM(lo_revenue) over customer, supplier and date; pipeline_q4
+    (04.cuh:99-153): SUM(lo_revenue - lo_supplycost) over date, customer, supplier and part, the
+    difference taken on the device (the plan's value_minus column). The logged sums and group
+    counts, and every group against the oracle's CPU plan over the same tables."""
+    lo, d, p, s, c = ssb10
+    dims, q4 = _q34(q, d, p, s, c)
+    say("ssb sf10", ["q3.1", "q3.2", "q3.3", "q3.4", "q4.1", "q4.2", "q4.3"][q], "rows", lo["orderdate"].size)
+    fk_of = dict(date="orderdate", cust="custkey", supp="suppkey", part="partkey")
+    enc = dict(codec=1, page_rows=4096, chunk_rows=262144, lz4_min_match=4)
+    names = [fk_of[t] for (t, _), _, _ in dims] + ["revenue"] + (["supplycost"] if q4 else [])
+    cols = {k: ctx.upload(golap.HostColumn.encode(lo[k], **enc)) for k in names}
+    maps, bufs = [], []
+    try:
+        plan = []
+        for (t, key), preds, grp in dims:
+            m, b = build_map(ctx, golap, key, key.size, preds, None if grp is None else grp[0])
+            maps.append(m)
+            bufs += b
+            plan.append((cols[fk_of[t]], m, 0 if grp is None else grp[1], 0 if grp is None else grp[2]))
+        r, k, sums, cnt = ctx.star_run(plan, cols["revenue"], value_minus=cols["supplycost"] if q4 else None)
+        assert (r.total_sum, r.n_groups) == SF10_Q34[q]
+        val = lo["revenue"].astype(np.int64) - (lo["supplycost"].astype(np.int64) if q4 else 0)
+        assert val.min() >= 0 and val.max() < 2**31
+        wk, ws, wc = orc.star_groupby([(key, preds, None if grp is None else grp[0]) for (t, key), preds, grp in dims],
+                                      [lo[fk_of[t]] for (t, _), _, _ in dims], val.astype(np.int32), threads=8)
+        assert np.array_equal(k, wk) and np.array_equal(sums, ws) and np.array_equal(cnt, wc)
+    finally:
+        for c_ in cols.values():
+            ctx.free_column(c_)
+        for m in maps:
+            ctx.free_hashmap(m)
+        for b in bufs:
+            ctx.free(b)
+
+
+def test_value_minus_refused_outside_single_plans(golap, ctx):
+    """value_minus is a golap_star_run option: a batch carrying it is refused, not run without it."""
+    rng = np.random.default_rng(5)
+    n = 100_000
+    fk = rng.integers(0, 100, n).astype(np.int32)
+    v = rng.integers(0, 1000, n).astype(np.int32)
+    enc = dict(codec=0, page_rows=4096, chunk_rows=65536)
+    cf, cv = ctx.upload(golap.HostColumn.encode(fk, **enc)), ctx.upload(golap.HostColumn.encode(v, **enc))
+    m, bufs = build_map(ctx, golap, np.arange(100, dtype=np.int32), 100, [], None)
+    try:
+        r, _, sums, _ = ctx.star_run([(cf, m, 0, 0)], cv, value_minus=cv)
+        assert r.total_sum == 0 and r.rows_matched == n
+        with pytest.raises(golap.GolapError):
+            ctx.star_batch([dict(dims=[(cf, m, 0, 0)], value=cv, value_minus=cv)])
+    finally:
+        ctx.free_column(cf)
+        ctx.free_column(cv)
+        ctx.free_hashmap(m)
         for b in bufs:
             ctx.free(b)
