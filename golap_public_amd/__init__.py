@@ -254,6 +254,21 @@ _PROTOS = {
     "golap_gen_uniform_i32": (C.c_int, [u64, u32, u64, C.c_int32, C.c_int32, vp, u32]),
     "golap_gen_date_dim": (C.c_int, [u32, vp, vp, vp]),
     "golap_gen_q1_fact": (C.c_int, [u64, u64, u64, vp, u32, vp, vp, vp, vp, u32]),
+    "golap_stream_create": (C.c_int, [vp, C.POINTER(vp)]),
+    "golap_stream_destroy": (C.c_int, [vp]),
+    "golap_stream_sync": (C.c_int, [vp]),
+    "golap_event_create": (C.c_int, [vp, C.POINTER(vp)]),
+    "golap_event_destroy": (C.c_int, [vp]),
+    "golap_event_record": (C.c_int, [vp, vp]),
+    "golap_stream_wait_event": (C.c_int, [vp, vp, vp]),
+    "golap_event_sync": (C.c_int, [vp]),
+    "golap_stream_copy": (C.c_int, [vp, vp, vp, vp, u64]),
+    "golap_decompress_chunks_async": (C.c_int, [vp, vp, u64, u64, vp, vp]),
+    "golap_hashagg_create": (C.c_int, [vp, C.POINTER(GroupKey), u32, u32, C.POINTER(vp)]),
+    "golap_hashagg_add": (C.c_int, [vp, vp, vp, u32, u64, vp]),
+    "golap_hashagg_results": (C.c_int, [vp, C.POINTER(u64), vp, vp, vp, u64]),
+    "golap_hashagg_clear": (C.c_int, [vp]),
+    "golap_hashagg_destroy": (C.c_int, [vp]),
 }
 
 _lib = None
@@ -558,6 +573,65 @@ class Context:
         finally:
             self.free(buf)
         return out, ms.value
+
+    # streams, events, asynchronous decode, device-resident group-by
+    def stream(self):
+        """A stream of its own (golap_stream_create); release with free_stream."""
+        h = vp()
+        self.check(self.lib.golap_stream_create(self.h, C.byref(h)), "stream_create")
+        return h.value
+
+    def free_stream(self, s):
+        self.lib.golap_stream_destroy(s)
+
+    def stream_sync(self, s):
+        self.check(self.lib.golap_stream_sync(s), "stream_sync")
+
+    def event(self):
+        h = vp()
+        self.check(self.lib.golap_event_create(self.h, C.byref(h)), "event_create")
+        return h.value
+
+    def free_event(self, e):
+        self.lib.golap_event_destroy(e)
+
+    def event_record(self, e, stream=None):
+        self.check(self.lib.golap_event_record(e, stream), "event_record")
+
+    def stream_wait_event(self, stream, e):
+        self.check(self.lib.golap_stream_wait_event(self.h, stream, e), "stream_wait_event")
+
+    def decompress_async(self, col, dev_out, chunk_begin=0, chunk_end=None, stream=None):
+        if chunk_end is None:
+            chunk_end = self.column_desc(col).n_chunks
+        self.check(self.lib.golap_decompress_chunks_async(self.h, col, chunk_begin, chunk_end, dev_out, stream),
+                   "decompress_async")
+
+    def hashagg(self, keys, agg_mode=AGG_AUTO):
+        """keys: list of (type, lo, card). Returns a golap_hashagg handle."""
+        arr = (GroupKey * len(keys))()
+        for i, (t, lo, card) in enumerate(keys):
+            arr[i] = GroupKey(None, t, lo, card)
+        h = vp()
+        self.check(self.lib.golap_hashagg_create(self.h, arr, len(keys), agg_mode, C.byref(h)), "hashagg_create")
+        return h.value
+
+    def hashagg_add(self, h, dev_keys, dev_val, val_type, n, stream=None):
+        ks = (vp * len(dev_keys))(*dev_keys)
+        self.check(self.lib.golap_hashagg_add(h, ks, dev_val, val_type, n, stream), "hashagg_add")
+
+    def hashagg_results(self, h, n_keys, max_groups):
+        ng = u64()
+        k = np.zeros(max_groups * n_keys, np.int32)
+        sums = np.zeros(max_groups, np.int64)
+        cnts = np.zeros(max_groups, np.uint64)
+        self.check(self.lib.golap_hashagg_results(h, C.byref(ng), _ptr(k), _ptr(sums), _ptr(cnts), max_groups),
+                   "hashagg_results")
+        n = ng.value
+        return k[:n * n_keys].reshape(n, n_keys), sums[:n], cnts[:n]
+
+    def free_hashagg(self, h):
+        self.lib.golap_hashagg_destroy(h)
 
     # hash join
     def hashmap(self, ht_size, dev_keys, key_type):

@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libgolap_hip.so")
 
-HIP_SOURCES = ["kernels.hip", "decode.hip", "q1wg.hip", "star.hip", "scan.hip", "keys.hip", "chunkmeta.hip", "capi_scan.cpp", "capi.cpp", "capi_star.cpp", "capi_batch.cpp", "capi_server.cpp", "encode.cpp", "synth.cpp", "storage.cpp", "outofcore.cpp", "encode_dev.hip", "host_join.cpp", "capi_noalloc.hip"]
+HIP_SOURCES = ["kernels.hip", "decode.hip", "q1wg.hip", "star.hip", "scan.hip", "keys.hip", "chunkmeta.hip", "capi_scan.cpp", "capi.cpp", "capi_star.cpp", "capi_batch.cpp", "capi_server.cpp", "encode.cpp", "synth.cpp", "storage.cpp", "outofcore.cpp", "encode_dev.hip", "host_join.cpp", "capi_noalloc.hip", "capi_stream.cpp"]
 
 # -packed-fp32-ops: no v_pk_*_f32 generation (nothing on this path is floating point, the flag
 # keeps it that way if code is added).

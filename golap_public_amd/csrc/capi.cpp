@@ -148,12 +148,13 @@ const char *golap_last_error(golap_ctx *ctx) {
     return copy.c_str();
 }
 
+// Also reports device errors raised by work enqueued on the context's stream without a wait
+// (golap_decompress_chunks_async / golap_hashagg_add with a null stream).
 int golap_sync(golap_ctx *ctx) {
     if (!ctx) return GOLAP_EINVAL;
     Lock l(ctx->mu);
     if (int r = set_device(ctx)) return r;
-    GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    return GOLAP_OK;
+    return check_device_errors(ctx, "sync");
 }
 
 int golap_device_info(golap_ctx *ctx, int *cus, char *name, int name_len) {

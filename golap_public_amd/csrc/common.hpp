@@ -56,6 +56,29 @@ struct golap_ctx {
     hipError_t fixed_err = hipSuccess;
 };
 
+// A caller's stream (golap_stream_create) with its own device error word, so work enqueued on it
+// reports its errors when the stream is synchronised, not through the context's scratch.
+struct golap_stream {
+    golap_ctx *ctx = nullptr;
+    hipStream_t stream = nullptr;
+    uint32_t *d_err = nullptr;
+};
+struct golap_event {
+    golap_ctx *ctx = nullptr;
+    hipEvent_t event = nullptr;
+};
+// A group-by accumulator kept on the device across calls (golap_hashagg_create).
+struct golap_hashagg {
+    golap_ctx *ctx = nullptr;
+    uint32_t n_keys = 0, agg_mode = 0;
+    std::vector<int32_t> lo;
+    std::vector<uint32_t> card, type;
+    std::vector<uint64_t> stride;
+    uint64_t domain = 0;
+    int64_t *d_sum = nullptr;  // [domain] then counts [domain]
+    uint64_t *d_cnt = nullptr;
+};
+
 // Host planning tables of a device column, from its host catalog: launches are planned from
 // them without a device round trip.
 struct ColumnExtra {
@@ -428,6 +451,7 @@ struct GroupByLaunch {
     int64_t *g_sum;
     uint64_t *g_cnt;
     uint32_t *err;
+    hipStream_t stream = nullptr;  // nullptr: the context's stream
 };
 hipError_t launch_group_by(golap_ctx *ctx, const GroupByLaunch &l);
 
@@ -439,8 +463,10 @@ struct PredView {
 constexpr int MAX_PREDS = 4;
 
 // kernel launchers (kernels.hip)
+// stream / err: where to launch and raise device errors (nullptr: the context's stream and scratch)
 hipError_t launch_decode_pages(golap_ctx *ctx, const golap_column *col, uint64_t page_begin,
-                               uint64_t page_end, uint8_t *dev_out, uint64_t out_row0);
+                               uint64_t page_end, uint8_t *dev_out, uint64_t out_row0,
+                               hipStream_t stream = nullptr, uint32_t *err = nullptr);
 hipError_t launch_hash_clear(golap_ctx *ctx, uint64_t *slots, uint64_t n);
 hipError_t launch_hash_build(golap_ctx *ctx, golap_hashmap *m, uint64_t n_rows,
                              const PredView *preds, uint32_t n_preds, const int32_t *payload,

@@ -186,8 +186,11 @@ __global__ __launch_bounds__(BLOCK) void k_decode_wg(DecodeArgs a, lz4wg::Layout
 }  // namespace
 
 hipError_t launch_decode_pages(golap_ctx *ctx, const golap_column *col, uint64_t page_begin,
-                               uint64_t page_end, uint8_t *dev_out, uint64_t out_row0) {
+                               uint64_t page_end, uint8_t *dev_out, uint64_t out_row0, hipStream_t stream,
+                               uint32_t *err) {
     if (page_end <= page_begin) return hipSuccess;
+    if (!stream) stream = ctx->stream;
+    if (!err) err = reinterpret_cast<uint32_t *>(ctx->d_scratch);
     DecodeArgs a{};
     a.c = view(col);
     a.page_begin = page_begin;
@@ -197,7 +200,7 @@ hipError_t launch_decode_pages(golap_ctx *ctx, const golap_column *col, uint64_t
     a.out = dev_out;
     a.out_row0 = out_row0;
     const uint32_t page_bytes = col->page_rows * col->elem;
-    a.err = reinterpret_cast<uint32_t *>(ctx->d_scratch);
+    a.err = err;
     if (col->extra.has_lz4i && page_bytes < 65536) {
         const uint32_t tok_cap = col->extra.lz4i_max_nseq ? col->extra.lz4i_max_nseq : lz4wg::max_sequences(page_bytes);
         const lz4wg::Layout l = lz4wg::layout(lz4i_stage_bytes(col->extra), page_bytes, tok_cap, BLOCK);
@@ -206,7 +209,7 @@ hipError_t launch_decode_pages(golap_ctx *ctx, const golap_column *col, uint64_t
             if (e != hipSuccess) return e;
             const uint32_t grid = (uint32_t)std::min<uint64_t>(page_end - page_begin,
                                                                (uint64_t)ctx->cus * wg_per_cu(ctx, l.total));
-            hipLaunchKernelGGL(k_decode_wg, dim3(grid), dim3(BLOCK), l.total, ctx->stream, a, l);
+            hipLaunchKernelGGL(k_decode_wg, dim3(grid), dim3(BLOCK), l.total, stream, a, l);
             return hipGetLastError();
         }
     }
@@ -216,7 +219,6 @@ hipError_t launch_decode_pages(golap_ctx *ctx, const golap_column *col, uint64_t
         a.window = (page_bytes + 15) & ~15u;
         a.tok_cap = 0;
     }
-    a.err = reinterpret_cast<uint32_t *>(ctx->d_scratch);
     const size_t lds = (size_t)a.window + (a.tok_cap ? lz4i::scratch_bytes(a.window, a.tok_cap) : 0);
     hipError_t e = prepare_lds(ctx, k_decode_pages, lds);
     if (e != hipSuccess) return e;
@@ -225,7 +227,7 @@ hipError_t launch_decode_pages(golap_ctx *ctx, const golap_column *col, uint64_t
     if (e != hipSuccess) return e;
     occ = std::max(occ, 1);
     const uint32_t grid = (uint32_t)std::min<uint64_t>(page_end - page_begin, (uint64_t)ctx->cus * occ);
-    hipLaunchKernelGGL(k_decode_pages, dim3(grid), dim3(WAVE), lds, ctx->stream, a);
+    hipLaunchKernelGGL(k_decode_pages, dim3(grid), dim3(WAVE), lds, stream, a);
     return hipGetLastError();
 }
 

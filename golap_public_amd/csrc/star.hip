@@ -884,7 +884,7 @@ hipError_t launch_group_by(golap_ctx *ctx, const GroupByLaunch &l) {
     if (e != hipSuccess) return e;
     const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>((l.n + BLOCK - 1) / BLOCK,
                                                                    (uint64_t)ctx->cus * std::max(occ, 1)));
-    hipLaunchKernelGGL(k_group_by, dim3((uint32_t)grid), dim3(BLOCK), lds, ctx->stream, g);
+    hipLaunchKernelGGL(k_group_by, dim3((uint32_t)grid), dim3(BLOCK), lds, l.stream ? l.stream : ctx->stream, g);
     return hipGetLastError();
 }
 

@@ -9,14 +9,18 @@
 //    memory (query1_files); no cuFile;
 //  * the fused plans (query1, star) take whole chunk ranges in one launch instead of one
 //    kernel per chunk driven from host worker threads; the per-chunk Op protocol remains for
-//    callers that step chunks themselves (ChunkDecompressor, Collector, TableLoader).
+//    callers that step chunks themselves (ChunkDecompressor, Collector, TableLoader, load),
+//    with the reference's streams and events (CStream, CEvent, Op::step(stream, parent_event)).
 #pragma once
 #include <algorithm>
 #include <array>
+#include <atomic>
+#include <chrono>
 #include <cstdint>
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <type_traits>
 #include <utility>
 #include <vector>
@@ -286,90 +290,294 @@ double prepare_compressed_device(Device &dev, const DeviceMem &dev_col, uint64_t
     return ms;
 }
 
+// Groups of a group-by or star plan in key order: keys[i] holds one value per group attribute.
+struct GroupResult {
+    std::vector<std::vector<int32_t>> keys;
+    std::vector<int64_t> sums;
+    std::vector<uint64_t> counts;
+};
+
+// ---------------------------------------------------------------- streams and events (core/core.hpp:192-235)
+// CStream: a stream of its own (golap_stream_create); .stream is what Op::step takes. A null
+// golap_stream* anywhere means the context's own stream, where the fused plans run.
+struct CStream {
+    explicit CStream(Device &dev) : dev_(&dev) {
+        detail::check(golap_stream_create(dev.get(), &stream), dev.get(), "stream_create");
+    }
+    CStream(const CStream &) = delete;
+    CStream &operator=(const CStream &) = delete;
+    ~CStream() {
+        if (stream) golap_stream_destroy(stream);
+    }
+    // waits for the stream; throws on a corrupt page or key out of range raised by its work
+    void sync() { detail::check(golap_stream_sync(stream), dev_->get(), "stream_sync"); }
+    golap_stream *stream = nullptr;
+
+private:
+    Device *dev_;
+};
+struct CEvent {
+    explicit CEvent(Device &dev) : dev_(&dev) {
+        detail::check(golap_event_create(dev.get(), &event), dev.get(), "event_create");
+    }
+    CEvent(const CEvent &) = delete;
+    CEvent &operator=(const CEvent &) = delete;
+    ~CEvent() {
+        if (event) golap_event_destroy(event);
+    }
+    void record(golap_stream *s) { detail::check(golap_event_record(event, s), dev_->get(), "event_record"); }
+    void sync() { detail::check(golap_event_sync(event), dev_->get(), "event_sync"); }
+    golap_event *event = nullptr;
+
+private:
+    Device *dev_;
+};
+
 // ---------------------------------------------------------------- Op protocol (core/core.hpp:94-124)
+// step(stream, parent_event): enqueue the next chunk's work on stream once parent_event (may be
+// null) has fired, then record this op's event on stream; false when exhausted. skip_step
+// advances without work (a pruned chunk, 01.hpp:368-378) and records the event too, so waiters
+// never see a stale one. The argument-less forms run one step on the context's stream and wait.
 class Op {
 public:
+    explicit Op(Device &dev) : dev_(dev), ev_(dev), event(ev_.event) {}
+    Op(const Op &) = delete;
+    Op &operator=(const Op &) = delete;
     virtual ~Op() = default;
-    // produce the next chunk; false when exhausted (core.hpp:109)
-    virtual bool step() = 0;
-    // advance without work (pruned chunk, 01.hpp:368-378)
-    virtual bool skip_step() = 0;
-    virtual void finish_step() {}
+    virtual bool step(golap_stream *stream, golap_event *parent_event) = 0;
+    virtual bool skip_step(golap_stream *stream, golap_event *parent_event) = 0;
+    virtual void finish_step(golap_stream *stream) {
+        detail::check(stream ? golap_stream_sync(stream) : golap_sync(dev_.get()), dev_.get(), "finish_step");
+    }
+    bool step() {
+        const bool r = step(nullptr, nullptr);
+        finish_step(nullptr);
+        return r;
+    }
+    bool skip_step() { return skip_step(nullptr, nullptr); }
+
+protected:
+    void wait_for(golap_stream *stream, golap_event *parent_event) {
+        if (parent_event)
+            detail::check(golap_stream_wait_event(dev_.get(), stream, parent_event), dev_.get(), "stream_wait_event");
+    }
+    void done(golap_stream *stream) { ev_.record(stream); }
+    Device &dev_;
+
+private:
+    CEvent ev_;
+
+public:
+    golap_event *event;          // recorded at the end of every step
     uint64_t last_produced = 0;  // bytes produced by the last step
 };
 
 // ChunkDecompressor (core/comp.cuh:151-216): decodes the next chunk of its block list into its
-// own device buffer (decomp_buf).
+// own device buffer (decomp_buf) on the stream it is stepped on.
 class ChunkDecompressor : public Op {
 public:
     ChunkDecompressor(Device &dev, const CompInfo &ci, std::vector<uint64_t> block_idxs)
-        : dev_(dev), ci_(ci), idxs_(std::move(block_idxs)),
-          decomp_buf(dev, Tag<uint8_t>{}, ci.chunk_bytes) {}
-    bool step() override {
+        : Op(dev), ci_(ci), idxs_(std::move(block_idxs)), decomp_buf(dev, Tag<uint8_t>{}, ci.chunk_bytes) {}
+    using Op::skip_step;
+    using Op::step;
+    bool step(golap_stream *stream, golap_event *parent_event) override {
         if (cur_ >= idxs_.size()) return false;
         const uint64_t b = idxs_[cur_++];
-        detail::check(golap_decompress_chunks(dev_.get(), ci_.col.get(), b, b + 1, decomp_buf.ptr<void>(), nullptr),
+        wait_for(stream, parent_event);
+        detail::check(golap_decompress_chunks_async(dev_.get(), ci_.col.get(), b, b + 1, decomp_buf.ptr<void>(), stream),
                       dev_.get(), "decompress");
         last_produced = ci_.blocks[b].tuples * ci_.value_size;
+        done(stream);
         return true;
     }
-    bool skip_step() override {
+    bool skip_step(golap_stream *stream, golap_event *parent_event) override {
         if (cur_ >= idxs_.size()) return false;
         ++cur_;
         last_produced = 0;
+        wait_for(stream, parent_event);
+        done(stream);
         return true;
     }
     uint64_t current_block() const { return idxs_[cur_ - 1]; }
-    Device &dev_;
     const CompInfo &ci_;
     std::vector<uint64_t> idxs_;
     uint64_t cur_ = 0;
     DeviceMem decomp_buf;
 };
 
-// Collector (core/core.hpp:252-307): steps every child column chain for the same chunk.
+// Collector (core/core.hpp:252-307): steps every child for the same chunk, each on its own
+// stream behind the parent event; the collector's stream then waits for every child's event.
 class Collector : public Op {
 public:
-    void add(Op *child) { children.push_back(child); }
-    bool step() override {
-        bool any = false;
-        for (Op *c : children) any = c->step() || any;
-        return any;
+    explicit Collector(Device &dev) : Op(dev) {}
+    using Op::skip_step;
+    using Op::step;
+    // opstream null: the child runs on the stream the collector is stepped on
+    void add_child(Op *op, golap_stream *opstream) { children.emplace_back(op, opstream); }
+    void add(Op *child) { add_child(child, nullptr); }
+    bool step(golap_stream *stream, golap_event *parent_event) override {
+        bool res = false;
+        for (auto &c : children) {
+            golap_stream *cs = c.second ? c.second : stream;
+            wait_for(cs, parent_event);
+            res = c.first->step(cs, parent_event) || res;
+            if (cs != stream) wait_for(stream, c.first->event);
+        }
+        done(stream);
+        return res;
     }
-    bool skip_step() override {
-        bool any = false;
-        for (Op *c : children) any = c->skip_step() || any;
-        return any;
+    bool skip_step(golap_stream *stream, golap_event *parent_event) override {
+        bool res = false;
+        for (auto &c : children) {
+            golap_stream *cs = c.second ? c.second : stream;
+            res = c.first->skip_step(cs, parent_event) || res;
+            if (cs != stream) wait_for(stream, c.first->event);
+        }
+        done(stream);
+        return res;
     }
-    std::vector<Op *> children;
+    std::vector<std::pair<Op *, golap_stream *>> children;
 };
 
-// DecompressEnv (core/apps.cuh:341-396) for one column of one worker's block slice.
+// DecompressEnv (core/apps.cuh:341-396) for one column of one worker's block slice: its blocks,
+// its stream and event, and the decompressor that is its root op.
 struct DecompressEnv {
     DecompressEnv(Device &dev, const std::vector<uint64_t> &all_block_idxs, uint64_t start, uint64_t past_end,
                   const CompInfo &ci)
         : myblock_idxs(all_block_idxs.begin() + start, all_block_idxs.begin() + past_end),
-          myblocks(past_end - start), root(dev, ci, myblock_idxs) {}
+          gstream(dev), gevent(dev), root(dev, ci, myblock_idxs) {
+        for (uint64_t b : myblock_idxs) myblocks.push_back(ci.blocks[b]);
+    }
     std::vector<uint64_t> myblock_idxs;
-    uint64_t myblocks;
+    std::vector<BlockInfo> myblocks;
+    CStream gstream;
+    CEvent gevent;
     ChunkDecompressor root;
     DeviceMem &decomp_buf() { return root.decomp_buf; }
 };
 
-// TableLoader<BLOCKENV> (core/apps.cuh:611-630): one env per column, stepped together.
+// TableLoader<BLOCKENV> (core/apps.cuh:611-630): one env per column, each column stepped on its
+// env's stream by one Collector (rootop); rootstream/rootevent are the caller's.
 template <typename BLOCKENV>
 struct TableLoader {
-    explicit TableLoader(Device &dev, uint32_t reserve = 4) : dev_(dev) { blockenvs.reserve(reserve); }
+    explicit TableLoader(Device &dev, uint32_t reserve = 4) : dev_(dev), rootop(dev), rootstream(dev), rootevent(dev) {
+        blockenvs.reserve(reserve);
+    }
     void add(const std::string &name, const std::vector<uint64_t> &all_block_idxs, uint64_t start,
              uint64_t past_end, const CompInfo &ci) {
         names.push_back(name);
         blockenvs.emplace_back(new BLOCKENV(dev_, all_block_idxs, start, past_end, ci));
-        rootop.add(&blockenvs.back()->root);
+        rootop.add_child(&blockenvs.back()->root, blockenvs.back()->gstream.stream);
+    }
+    BLOCKENV &at(const std::string &name) {
+        for (size_t i = 0; i < names.size(); ++i)
+            if (names[i] == name) return *blockenvs[i];
+        throw Error(GOLAP_EINVAL, "TableLoader: no column " + name);
     }
     Device &dev_;
     std::vector<std::string> names;
     std::vector<std::unique_ptr<BLOCKENV>> blockenvs;
     Collector rootop;
+    CStream rootstream;
+    CEvent rootevent;
+};
+
+// load<BLOCKENV, T> (core/apps.cuh:682-793): worker_num threads, each with its env over a
+// contiguous slice of the column's chunks, step their decompressor on their own stream and copy
+// every decoded chunk to host_ptr + chunk * chunk_bytes (the SSD2GPU2CPU dataflow; chunks stay
+// in HBM here). Chunks whose prune_check flag is 0 are skipped (skip_step) and not written.
+// Returns the wall time in ms; pruned_chunks (may be null) gets the skipped count.
+template <typename BLOCKENV, typename T>
+double load(Device &dev, uint64_t worker_num, T *host_ptr, const CompInfo &ci, uint64_t &device_allocated,
+            const uint16_t *prune_check = nullptr, uint64_t *pruned_chunks = nullptr) {
+    const uint64_t nb = ci.blocks.size();
+    worker_num = std::max<uint64_t>(1, std::min<uint64_t>(worker_num, std::max<uint64_t>(nb, 1)));
+    std::vector<uint64_t> all(nb);
+    for (uint64_t i = 0; i < nb; ++i) all[i] = i;
+    std::vector<std::unique_ptr<BLOCKENV>> envs;
+    for (uint64_t w = 0; w < worker_num; ++w)  // util::SliceSeq: contiguous, sizes differ by at most one
+        envs.emplace_back(new BLOCKENV(dev, all, nb * w / worker_num, nb * (w + 1) / worker_num, ci));
+    std::vector<std::thread> threads;
+    std::vector<std::string> errors(worker_num);
+    std::atomic<uint64_t> pruned{0};
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint64_t w = 0; w < worker_num; ++w) {
+        threads.emplace_back([&, w] {
+            try {
+                BLOCKENV &env = *envs[w];
+                for (size_t round = 0; round < env.myblocks.size(); ++round) {
+                    const uint64_t b = env.myblock_idxs[round];
+                    if (prune_check && prune_check[b] == 0) {
+                        env.root.skip_step(env.gstream.stream, env.gevent.event);
+                        pruned.fetch_add(1, std::memory_order_relaxed);
+                        continue;
+                    }
+                    env.root.step(env.gstream.stream, env.gevent.event);
+                    detail::check(golap_stream_copy(dev.get(), env.gstream.stream,
+                                                    reinterpret_cast<uint8_t *>(host_ptr) + b * ci.chunk_bytes,
+                                                    env.decomp_buf().template ptr<void>(),
+                                                    env.myblocks[round].tuples * ci.value_size),
+                                  dev.get(), "load copy");
+                    env.gevent.record(env.gstream.stream);
+                }
+                env.gstream.sync();
+            } catch (const std::exception &e) {
+                errors[w] = e.what();
+            }
+        });
+    }
+    for (auto &t : threads) t.join();
+    const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    for (auto &e : errors)
+        if (!e.empty()) throw Error(GOLAP_ECORRUPT, "load: " + e);
+    device_allocated = dev.allocated();
+    if (pruned_chunks) *pruned_chunks = pruned.load();
+    return ms;
+}
+
+// HashAggregate (core/dev_structs.cuh:157-253): a group-by accumulator that stays on the device
+// across add() calls (each one HashAggregate::add per row, with an atomic SUM and a count).
+// The reference takes the number of groups and its own group/result arrays; here the group
+// domain is given by each key's lo and card (golap_group_key.dev_col is not used).
+class HashAggregate {
+public:
+    HashAggregate(Device &dev, const std::vector<golap_group_key> &keys, uint32_t agg_mode = GOLAP_AGG_AUTO)
+        : dev_(dev), n_keys_((uint32_t)keys.size()) {
+        detail::check(golap_hashagg_create(dev.get(), keys.data(), n_keys_, agg_mode, &h_), dev.get(), "hashagg");
+    }
+    HashAggregate(const HashAggregate &) = delete;
+    HashAggregate &operator=(const HashAggregate &) = delete;
+    ~HashAggregate() {
+        if (h_) golap_hashagg_destroy(h_);
+    }
+    // n rows of the key columns (device, one per key, the types given at construction) and a
+    // value column, on stream (null: the context's stream)
+    template <typename V>
+    void add(const std::vector<const void *> &dev_keys, const V *dev_val, uint64_t n, golap_stream *stream = nullptr) {
+        detail::check(golap_hashagg_add(h_, dev_keys.data(), dev_val, detail::type_of<V>(), n, stream), dev_.get(),
+                      "hashagg_add");
+    }
+    // the groups with at least one row, in key order (waits for the context's stream: make it
+    // wait for the adds on other streams first, e.g. through a Collector's event)
+    GroupResult results(uint64_t max_groups) {
+        std::vector<int32_t> k(max_groups * n_keys_);
+        GroupResult g;
+        g.sums.resize(max_groups);
+        g.counts.resize(max_groups);
+        uint64_t n = 0;
+        detail::check(golap_hashagg_results(h_, &n, k.data(), g.sums.data(), g.counts.data(), max_groups), dev_.get(),
+                      "hashagg_results");
+        g.sums.resize(n);
+        g.counts.resize(n);
+        for (uint64_t i = 0; i < n; ++i) g.keys.emplace_back(k.begin() + i * n_keys_, k.begin() + (i + 1) * n_keys_);
+        return g;
+    }
+    void clear() { detail::check(golap_hashagg_clear(h_), dev_.get(), "hashagg_clear"); }
+
+private:
+    Device &dev_;
+    uint32_t n_keys_;
+    golap_hashagg *h_ = nullptr;
 };
 
 // ---------------------------------------------------------------- joins (dev_structs.cuh, join.cuh)
@@ -586,12 +794,6 @@ inline Q1Result query1(Device &dev, const CompInfo &orderdate, const CompInfo &d
 }
 
 // HashAggregate result of a grouping plan (dev_structs.cuh:157-253), canonical order.
-struct GroupResult {
-    std::vector<std::vector<int32_t>> keys;
-    std::vector<int64_t> sums;
-    std::vector<uint64_t> counts;
-};
-
 // query2 / query3 (02.hpp, 03.hpp) fused star join + group-by.
 struct StarDim {
     const CompInfo *fk;

@@ -51,6 +51,9 @@ typedef struct golap_ctx golap_ctx;
 typedef struct golap_host_column golap_host_column;
 typedef struct golap_column golap_column;
 typedef struct golap_hashmap golap_hashmap;
+typedef struct golap_stream golap_stream;
+typedef struct golap_event golap_event;
+typedef struct golap_hashagg golap_hashagg;
 
 /* ------------------------------------------------------------------ library / context */
 int golap_api_version(void);
@@ -61,9 +64,33 @@ int golap_ctx_create(int device, golap_ctx **out);
 int golap_ctx_destroy(golap_ctx *ctx);
 /* Replaces util::Log::error_fmt (core/util.hpp:67-144): the last error of this context. */
 const char *golap_last_error(golap_ctx *ctx);
+/* cudaStreamSynchronize of the context's stream (core/core.hpp:110 finish_step); also reports the
+ * device errors of work enqueued there without a wait (GOLAP_ECORRUPT / GOLAP_EINVAL). */
 int golap_sync(golap_ctx *ctx);
 /* Number of compute units and the device name (for reports). */
 int golap_device_info(golap_ctx *ctx, int *cus, char *name, int name_len);
+
+/* ------------------------------------------------------------------ streams and events
+ * CStream / CEvent (core/core.hpp:192-235): a worker's stream and the events the Op protocol
+ * passes between streams (Op::step(stream, parent_event), Op::event, core/core.hpp:94-124;
+ * Collector::step's cudaStreamWaitEvent fan-in, core.hpp:252-307). A stream has its own device
+ * error word: work enqueued on it (golap_decompress_chunks_async, golap_hashagg_add) reports
+ * corrupt pages or out-of-range keys at golap_stream_sync. In every call taking a stream, NULL
+ * means the context's own stream (where the plan entry points run), so plans can wait on
+ * worker streams through events. Streams have the default priority. */
+int golap_stream_create(golap_ctx *ctx, golap_stream **out);
+int golap_stream_destroy(golap_stream *s);
+/* cudaStreamSynchronize; then GOLAP_ECORRUPT / GOLAP_EINVAL for errors raised on the stream
+ * (the error word is cleared). */
+int golap_stream_sync(golap_stream *s);
+int golap_event_create(golap_ctx *ctx, golap_event **out);
+int golap_event_destroy(golap_event *e);
+/* cudaEventRecord(e, s) / cudaStreamWaitEvent(s, e) / cudaEventSynchronize(e). */
+int golap_event_record(golap_event *e, golap_stream *s);
+int golap_stream_wait_event(golap_ctx *ctx, golap_stream *s, golap_event *e);
+int golap_event_sync(golap_event *e);
+/* cudaMemcpyAsync on a stream (any direction; pinned host memory keeps it asynchronous). */
+int golap_stream_copy(golap_ctx *ctx, golap_stream *s, void *dst, const void *src, uint64_t bytes);
 
 /* ------------------------------------------------------------------ device memory
  * DeviceMem / MirrorMem RAII (core/mem.hpp:136-371) and the global DEVICE_ALLOCATED counter
@@ -179,6 +206,12 @@ int golap_encode_device(golap_ctx *ctx, const void *dev_values, uint64_t n_rows,
  * its bounds checks. *kernel_ms (may be NULL) gets the device time. */
 int golap_decompress_chunks(golap_ctx *ctx, const golap_column *col, uint64_t chunk_begin,
                             uint64_t chunk_end, void *dev_out, double *kernel_ms);
+/* The same decode enqueued on stream s without waiting for it (ChunkDecompressor::step on a
+ * worker's stream, core/comp.cuh:171-195, core/apps.cuh:347): corrupt pages are reported by
+ * golap_stream_sync(s). Concurrent calls on different streams of one context may overlap on the
+ * device. */
+int golap_decompress_chunks_async(golap_ctx *ctx, const golap_column *col, uint64_t chunk_begin,
+                                  uint64_t chunk_end, void *dev_out, golap_stream *s);
 
 /* ------------------------------------------------------------------ predicates
  * PredInfo / ConstPred (core/dev_structs.cuh:21-33) generalised to an inclusive range:
@@ -338,6 +371,24 @@ typedef struct {
 int golap_group_by(golap_ctx *ctx, const golap_group_key *keys, uint32_t n_keys, const void *dev_val,
                    uint32_t val_type, uint64_t n_rows, uint32_t agg_mode, uint64_t *n_groups,
                    int32_t *group_keys, int64_t *sums, uint64_t *counts, uint64_t max_groups);
+
+/* HashAggregate (core/dev_structs.cuh:157-253): a group-by accumulator that stays on the device
+ * across calls, as the reference's HashAggregate object outlives the per-chunk pipeline kernels.
+ * keys[k] give each key's type, lo and card (dev_col is ignored here); the groups are the dense
+ * mixed-radix domain of the cards (product below 2^31). golap_hashagg_add adds n_rows rows of
+ * key columns dev_keys[0..n_keys) and a value column (HashAggregate::add with an atomic SUM
+ * functor and a count, dev_structs.cuh:209-244, group_by_agg_*col core/group.cuh:14-36) on
+ * stream s (NULL: the context's stream); a key outside its range is reported by the stream's
+ * sync (golap_sync for NULL). golap_hashagg_results waits for the context's stream and copies
+ * the groups with at least one row out as golap_group_by does; golap_hashagg_clear zeroes them. */
+int golap_hashagg_create(golap_ctx *ctx, const golap_group_key *keys, uint32_t n_keys, uint32_t agg_mode,
+                         golap_hashagg **out);
+int golap_hashagg_add(golap_hashagg *h, const void *const *dev_keys, const void *dev_val, uint32_t val_type,
+                      uint64_t n_rows, golap_stream *s);
+int golap_hashagg_results(golap_hashagg *h, uint64_t *n_groups, int32_t *group_keys, int64_t *sums,
+                          uint64_t *counts, uint64_t max_groups);
+int golap_hashagg_clear(golap_hashagg *h);
+int golap_hashagg_destroy(golap_hashagg *h);
 
 /* ------------------------------------------------------------------ star join + group-by
  * query2 / query3 (usecases/ssb/queries/02.hpp:54-307, 03.hpp:79-677) with pipeline_q2 /

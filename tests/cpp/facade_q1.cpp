@@ -2,7 +2,8 @@
 // (usecases/ssb/queries/01.hpp:53-449): prepare_compressed_device per column with MinMaxMeta,
 // HashMap + hash_map_build_pred_other on the date dimension, MetaChecker pruning, the fused
 // query1, and a TableLoader<DecompressEnv> stepping every chunk (the select path,
-// core/hl/select.cuh:11-83); also a column coded on the device, a star plan with one dimension
+// core/hl/select.cuh:11-83), the same loader stepped on worker streams with events into a
+// HashAggregate, load<DecompressEnv> with and without pruned chunks; also a column coded on the device, a star plan with one dimension
 // joined through a HostHashMap, and query1 streamed from column files. Results are checked
 // against the oracle (oracle/liboracle.so).
 // Prints "OK" and exits 0 on success.
@@ -213,6 +214,70 @@ int main() {
         ++c;
     }
     REQUIRE(c == chunk_num);
+
+    // the stream protocol (core/core.hpp:94-124, 252-307): a Collector steps the discount,
+    // quantity and price columns on their envs' streams behind the caller's event, the caller's
+    // stream waits for the collector's event, and a HashAggregate adds every chunk's rows
+    // (group by discount, quantity; SUM(price)) on that stream; checked against the oracle
+    {
+        golap::TableLoader<golap::DecompressEnv> tl(dev);
+        tl.add("lo_discount", all, 0, chunk_num, ci_disc);
+        tl.add("lo_quantity", all, 0, chunk_num, ci_qty);
+        tl.add("lo_price", all, 0, chunk_num, ci_price);
+        std::vector<golap_group_key> gk(2);
+        gk[0].type = GOLAP_U8, gk[0].lo = 0, gk[0].card = 11;
+        gk[1].type = GOLAP_U8, gk[1].lo = 1, gk[1].card = 50;
+        golap::HashAggregate agg(dev, gk);
+        uint64_t steps = 0;
+        tl.rootevent.record(tl.rootstream.stream);
+        while (tl.rootop.step(tl.rootstream.stream, tl.rootevent.event)) {
+            const uint64_t rows = ci_disc.blocks[steps].tuples;
+            agg.add<int32_t>({tl.at("lo_discount").decomp_buf().ptr<void>(), tl.at("lo_quantity").decomp_buf().ptr<void>()},
+                             tl.at("lo_price").decomp_buf().ptr<int32_t>(), rows, tl.rootstream.stream);
+            // the next step's decode into the same buffers waits for this chunk's adds
+            tl.rootevent.record(tl.rootstream.stream);
+            ++steps;
+        }
+        REQUIRE(steps == chunk_num);
+        tl.rootstream.sync();
+        auto got = agg.results(11 * 50);
+        std::vector<int32_t> k0(disc.begin(), disc.end()), k1(qty.begin(), qty.end());
+        std::vector<int64_t> v64(price.begin(), price.end());
+        const int32_t *keys[2] = {k0.data(), k1.data()};
+        std::vector<int32_t> wk(550 * 2);
+        std::vector<int64_t> ws(550);
+        std::vector<uint64_t> wc(550);
+        const int64_t ng = oracle_groupby(keys, 2, v64.data(), n, 550, wk.data(), ws.data(), wc.data());
+        REQUIRE(ng > 0 && (uint64_t)ng == got.sums.size());
+        for (int64_t g = 0; g < ng; ++g) {
+            REQUIRE(got.keys[g][0] == wk[2 * g] && got.keys[g][1] == wk[2 * g + 1]);
+            REQUIRE(got.sums[g] == ws[g] && got.counts[g] == wc[g]);
+        }
+        // a second pass accumulates on top; clear() starts over
+        agg.clear();
+        REQUIRE(agg.results(550).sums.empty());
+    }
+
+    // load<BLOCKENV, T> (core/apps.cuh:682-793): three workers decode the date column into host
+    // memory, once whole and once with the MetaChecker's pruned chunks skipped
+    {
+        std::vector<int32_t> host(chunk_num * tuples_per_chunk, -1);
+        uint64_t alloc = 0, skipped = 0;
+        const double ms = golap::load<golap::DecompressEnv>(dev, 3, host.data(), ci_date, alloc);
+        REQUIRE(ms >= 0 && alloc > 0 && !std::memcmp(host.data(), date.data(), n * 4));
+        std::vector<int32_t> host2(chunk_num * tuples_per_chunk, -1);
+        golap::load<golap::DecompressEnv>(dev, 3, host2.data(), ci_date, alloc, mask.data(), &skipped);
+        REQUIRE(skipped == chunk_num - keep);
+        for (uint64_t c2 = 0; c2 < chunk_num; ++c2) {
+            const uint64_t rows = ci_date.blocks[c2].tuples;
+            const int32_t *got = host2.data() + c2 * tuples_per_chunk;
+            if (mask[c2]) {
+                REQUIRE(!std::memcmp(got, date.data() + c2 * tuples_per_chunk, rows * 4));
+            } else {
+                REQUIRE(std::all_of(got, got + rows, [](int32_t v) { return v == -1; }));
+            }
+        }
+    }
     std::printf("OK sum=%lld matches=%llu chunks=%llu pruned=%llu\n", (long long)full.sum,
                 (unsigned long long)full.matches, (unsigned long long)chunk_num,
                 (unsigned long long)pruned.chunks_pruned);
