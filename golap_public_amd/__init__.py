@@ -269,6 +269,8 @@ _PROTOS = {
     "golap_hashagg_results": (C.c_int, [vp, C.POINTER(u64), vp, vp, vp, u64]),
     "golap_hashagg_clear": (C.c_int, [vp]),
     "golap_hashagg_destroy": (C.c_int, [vp]),
+    "golap_prejoin_minmax": (C.c_int, [vp, vp, vp, C.POINTER(vp)]),
+    "golap_chunkmeta_minmax_check": (C.c_int, [vp, vp, i64, i64, vp]),
 }
 
 _lib = None
@@ -692,6 +694,15 @@ class Context:
         d = ChunkMetaDesc()
         self.check(self.lib.golap_chunkmeta_info(h, C.byref(d)), "chunkmeta_info")
         return d
+
+    def prejoin_minmax(self, fk_col, dim_map):
+        """Per-chunk min/max of the dimension attribute (the map's payload) fk_col's rows join to."""
+        h = vp()
+        self.check(self.lib.golap_prejoin_minmax(self.h, fk_col, dim_map, C.byref(h)), "prejoin_minmax")
+        return h.value
+
+    def chunkmeta_minmax_check(self, h, lo, hi, dev_flags):
+        self.check(self.lib.golap_chunkmeta_minmax_check(self.h, h, lo, hi, dev_flags), "chunkmeta_minmax_check")
 
     def chunkmeta_free(self, h):
         self.check(self.lib.golap_chunkmeta_free(self.h, h), "chunkmeta_free")

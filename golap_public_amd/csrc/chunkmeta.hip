@@ -17,6 +17,7 @@
 #include <string>
 
 #include "common.hpp"
+#include "star_dev.hpp"
 
 namespace golap_impl {
 namespace {
@@ -25,7 +26,7 @@ constexpr int BLOCK = 256;
 constexpr uint32_t HIST_MAX_BUCKETS = 8192;   // u32 LDS counters: 32 KiB
 constexpr uint64_t BLOOM_MAX_BYTES = 65536;  // one chunk's filter in LDS
 
-enum : uint32_t { META_HIST = 1, META_BLOOM = 2 };
+enum : uint32_t { META_HIST = 1, META_BLOOM = 2, META_MINMAX = 3 };
 
 // Value of row r of a typed column, sign- or zero-extended.
 __device__ __forceinline__ int64_t load_typed(const void *p, uint32_t type, uint64_t r) {
@@ -166,6 +167,50 @@ __global__ __launch_bounds__(BLOCK) void k_bloom_check(const uint32_t *state, ui
 }
 
 bool int_type(uint32_t t) { return t >= GOLAP_I8 && t <= GOLAP_I64; }
+
+// SortHelper::prejoin_tables (usecases/ssb/helper.hpp:115-152) + MinMaxMeta of the prejoined
+// column: workgroup per chunk; each row's foreign key (decoded) is probed into the dimension map,
+// whose payload is the attribute; min and max over the rows that join (none: min > max).
+__global__ __launch_bounds__(BLOCK) void k_prejoin_minmax(const int32_t *fk, uint64_t n, uint64_t chunk_rows,
+                                                          uint64_t n_chunks, const uint64_t *slots, uint32_t log2cap,
+                                                          int64_t *mins, int64_t *maxs) {
+    __shared__ int64_t red[2 * BLOCK];
+    for (uint64_t c = blockIdx.x; c < n_chunks; c += gridDim.x) {
+        const uint64_t r0 = c * chunk_rows, r1 = min(n, r0 + chunk_rows);
+        int64_t a = INT64_MAX, b = INT64_MIN;
+        for (uint64_t r = r0 + threadIdx.x; r < r1; r += BLOCK) {
+            const int32_t key = fk[r];
+            const uint32_t h = stardev::hash32((uint32_t)key) >> (32 - log2cap);
+            uint32_t pay;
+            if (stardev::probe_exact(slots, log2cap, key, slots[h], h, pay)) {
+                a = min(a, (int64_t)(int32_t)pay);
+                b = max(b, (int64_t)(int32_t)pay);
+            }
+        }
+        red[threadIdx.x] = a;
+        red[BLOCK + threadIdx.x] = b;
+        __syncthreads();
+        for (int s = BLOCK / 2; s > 0; s >>= 1) {
+            if ((int)threadIdx.x < s) {
+                red[threadIdx.x] = min(red[threadIdx.x], red[threadIdx.x + s]);
+                red[BLOCK + threadIdx.x] = max(red[BLOCK + threadIdx.x], red[BLOCK + threadIdx.x + s]);
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            mins[c] = red[0];
+            maxs[c] = red[BLOCK];
+        }
+        __syncthreads();
+    }
+}
+
+// check_mmmeta (core/metadata.cuh:539-547) over a min/max meta: flag = lo <= max && hi >= min.
+__global__ __launch_bounds__(BLOCK) void k_minmax_meta_check(const int64_t *mins, const int64_t *maxs, uint64_t n,
+                                                             int64_t lo, int64_t hi, uint16_t *flags) {
+    for (uint64_t c = (uint64_t)blockIdx.x * BLOCK + threadIdx.x; c < n; c += (uint64_t)gridDim.x * BLOCK)
+        flags[c] = (lo <= maxs[c] && hi >= mins[c]) ? 1 : 0;
+}
 
 }  // namespace
 }  // namespace golap_impl
@@ -310,6 +355,65 @@ int golap_bloommeta_check(golap_ctx *ctx, const golap_chunkmeta *h, int64_t valu
     }
     hipLaunchKernelGGL(k_bloom_check, dim3(grid_for(ctx, (h->n_chunks + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0,
                        ctx->stream, h->d_state, h->n_chunks, h->m, (uint32_t)h->k, x, dev_flags);
+    GOLAP_HIP_CHECK(ctx, hipGetLastError());
+    return GOLAP_OK;
+}
+
+int golap_prejoin_minmax(golap_ctx *ctx, const golap_column *fk, const golap_hashmap *map, golap_chunkmeta **out) {
+    if (!ctx || !fk || !map || !out || fk->ctx != ctx || map->ctx != ctx) return GOLAP_EINVAL;
+    if (fk->type != GOLAP_I32 && fk->type != GOLAP_U32) return fail(ctx, GOLAP_EINVAL, "prejoin: 32-bit keys only");
+    if (!map->built) return fail(ctx, GOLAP_EINVAL, "prejoin: map not built");
+    golap_chunkmeta *h = nullptr;
+    if (int r = meta_new(ctx, fk, GOLAP_I32, fk->n_rows, fk->chunk_rows, out, &h)) return r;
+    Lock l(ctx->mu);
+    if (int r = set_device(ctx)) return meta_release(h), r;
+    h->kind = META_MINMAX;
+    const uint64_t nc = std::max<uint64_t>(h->n_chunks, 1);
+    void *a = nullptr, *b = nullptr, *keys = nullptr;
+    if (int r = dev_alloc(ctx, nc * 8, &a)) return meta_release(h), r;
+    h->d_mins = static_cast<int64_t *>(a);
+    if (int r = dev_alloc(ctx, nc * 8, &b)) return meta_release(h), r;
+    h->d_maxs = static_cast<int64_t *>(b);
+    if (h->n_chunks) {
+        // the keys decoded chunk range by chunk range (at most 64 chunks at a time)
+        const uint64_t span = std::min<uint64_t>(h->n_chunks, 64);
+        if (int r = dev_alloc(ctx, span * fk->chunk_rows * 4, &keys)) return meta_release(h), r;
+        const uint64_t ppc = fk->chunk_rows / fk->page_rows;
+        const uint64_t *slots = map->small_log2 ? map->d_small : map->d_slots;
+        const uint32_t log2cap = map->small_log2 ? map->small_log2 : map->log2cap;
+        hipError_t e = hipSuccess;
+        for (uint64_t c0 = 0; c0 < h->n_chunks && e == hipSuccess; c0 += span) {
+            const uint64_t c1 = std::min(h->n_chunks, c0 + span);
+            const uint64_t rows = std::min(fk->n_rows, c1 * fk->chunk_rows) - c0 * fk->chunk_rows;
+            e = launch_decode_pages(ctx, fk, c0 * ppc, std::min<uint64_t>(fk->n_pages, c1 * ppc),
+                                    static_cast<uint8_t *>(keys), c0 * fk->chunk_rows);
+            if (e == hipSuccess) {
+                hipLaunchKernelGGL(k_prejoin_minmax, dim3(grid_for(ctx, c1 - c0)), dim3(BLOCK), 0, ctx->stream,
+                                   static_cast<const int32_t *>(keys), rows, fk->chunk_rows, c1 - c0, slots, log2cap,
+                                   h->d_mins + c0, h->d_maxs + c0);
+                e = hipGetLastError();
+            }
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        dev_release(ctx, keys);
+        if (e != hipSuccess) {
+            meta_release(h);
+            return fail(ctx, GOLAP_EHIP, std::string("prejoin_minmax: ") + hipGetErrorString(e));
+        }
+        if (int r = check_device_errors(ctx, "prejoin_minmax")) return meta_release(h), r;
+    }
+    *out = h;
+    return GOLAP_OK;
+}
+
+int golap_chunkmeta_minmax_check(golap_ctx *ctx, const golap_chunkmeta *h, int64_t lo, int64_t hi,
+                                 uint16_t *dev_flags) {
+    if (!ctx || !h || h->ctx != ctx || !h->d_mins || (h->n_chunks && !dev_flags)) return GOLAP_EINVAL;
+    Lock l(ctx->mu);
+    if (int r = set_device(ctx)) return r;
+    if (!h->n_chunks) return GOLAP_OK;
+    hipLaunchKernelGGL(k_minmax_meta_check, dim3(grid_for(ctx, (h->n_chunks + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0,
+                       ctx->stream, h->d_mins, h->d_maxs, h->n_chunks, lo, hi, dev_flags);
     GOLAP_HIP_CHECK(ctx, hipGetLastError());
     return GOLAP_OK;
 }

@@ -279,6 +279,17 @@ int golap_eqhist_check(golap_ctx *ctx, const golap_chunkmeta *h, int64_t lo, int
 int golap_bloommeta_build(golap_ctx *ctx, const void *dev_values, uint32_t type, uint64_t n_rows,
                           uint64_t chunk_rows, double p, uint64_t m_bytes, golap_chunkmeta **out);
 int golap_bloommeta_check(golap_ctx *ctx, const golap_chunkmeta *b, int64_t value, uint16_t *dev_flags);
+/* Prejoined metadata: SortHelper::prejoin_tables (usecases/ssb/helper.hpp:115-152) with
+ * MinMaxMeta over the prejoined column (MetaFlags::META, usecases/ssb/queries/01.hpp:70-78,146-150).
+ * Per chunk of the 32-bit foreign-key column fk, the min and max of the dimension attribute its
+ * rows join to: the payload of map (built with the attribute as dev_payload) for every row whose
+ * key the map holds. A chunk with no joining row has min > max and never qualifies. The reference
+ * materialises the prejoined column on the host; only its per-chunk range is kept here.
+ * golap_chunkmeta_minmax_check: dev_flags[c] = (lo <= max[c] && hi >= min[c]) (check_mmmeta,
+ * core/metadata.cuh:539-547) for a min/max meta (prejoined or histogram). */
+int golap_prejoin_minmax(golap_ctx *ctx, const golap_column *fk, const golap_hashmap *map, golap_chunkmeta **out);
+int golap_chunkmeta_minmax_check(golap_ctx *ctx, const golap_chunkmeta *m, int64_t lo, int64_t hi,
+                                 uint16_t *dev_flags);
 int golap_chunkmeta_info(const golap_chunkmeta *m, golap_chunkmeta_desc *out);
 int golap_chunkmeta_free(golap_ctx *ctx, golap_chunkmeta *m);
 
