@@ -468,12 +468,14 @@ int golap_impl::set_column_window(golap_ctx *ctx, golap_column *c, const golap_h
     c->page_codec = w.page_codec;
     c->chunk_min = w.chunk_min;
     c->chunk_max = w.chunk_max;
-    hipError_t e = hipMemcpyAsync(c->d_page_off, w.page_off.data(), w.n_pages * 8, hipMemcpyHostToDevice, stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(c->d_page_csize, w.page_csize.data(), w.n_pages * 4, hipMemcpyHostToDevice, stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(c->d_page_codec, w.page_codec.data(), w.n_pages, hipMemcpyHostToDevice, stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(c->d_chunk_min, w.chunk_min.data(), w.n_chunks * 8, hipMemcpyHostToDevice, stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(c->d_chunk_max, w.chunk_max.data(), w.n_chunks * 8, hipMemcpyHostToDevice, stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(stream);
+    // Copied from the column's own host catalog, which stays until the column's next window: no
+    // wait here. The caller orders every reader after these copies (an event on the stream) and
+    // sets the next window only once the scans of this one have returned.
+    hipError_t e = hipMemcpyAsync(c->d_page_off, c->page_off.data(), w.n_pages * 8, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->d_page_csize, c->page_csize.data(), w.n_pages * 4, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->d_page_codec, c->page_codec.data(), w.n_pages, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->d_chunk_min, c->chunk_min.data(), w.n_chunks * 8, hipMemcpyHostToDevice, stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->d_chunk_max, c->chunk_max.data(), w.n_chunks * 8, hipMemcpyHostToDevice, stream);
     if (e != hipSuccess) return fail(ctx, GOLAP_EHIP, std::string("column window: ") + hipGetErrorString(e));
     register_extra(c);
     return GOLAP_OK;

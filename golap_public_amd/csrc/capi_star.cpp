@@ -423,15 +423,25 @@ int golap_star_run(golap_ctx *ctx, const golap_star_plan *plan, golap_star_resul
     qd.g_cnt = d_cnt;
     L.err = reinterpret_cast<uint32_t *>(ctx->d_scratch);
     const golap_hashmap *dm = plan->dims[qd.driver].map;
+    // the chunk flags (for the slice statistics, and to size the work items by the chunks that
+    // survive)
+    std::vector<uint16_t> mask;
+    uint64_t surviving = 0;
+    if (plan->dev_chunk_mask && plan->chunk_end > plan->chunk_begin) {
+        mask.resize(plan->chunk_end - plan->chunk_begin);
+        GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(mask.data(), plan->dev_chunk_mask + plan->chunk_begin, mask.size() * 2,
+                                            hipMemcpyDeviceToHost, ctx->stream));
+        GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        for (uint16_t f : mask) surviving += f != 0;
+    }
     GOLAP_HIP_CHECK(ctx, hipEventRecord(ctx->ev0, ctx->stream));
     // with a chunk mask, the surviving chunks may sit anywhere in the slice: a device-built work
     // list keeps every workgroup busy (the static page ranges would leave most of them idle)
-    if (plan->dev_chunk_mask && L.n_items && dm->filled) {
-        // a slice of few chunks is cut into page ranges, so that its pages spread over about four
-        // workgroups per CU instead of queueing chunk after chunk on a few of them
-        const uint64_t nch = std::max<uint64_t>(1, qd.s.chunk_end - qd.s.chunk_begin);
+    if (plan->dev_chunk_mask && L.n_items && dm->filled && surviving) {
+        // few surviving chunks are cut into page ranges, so that their pages spread over about
+        // four workgroups per CU instead of queueing chunk after chunk on a few of them
         const uint32_t parts = (uint32_t)std::min<uint64_t>(
-            {(uint64_t)L.pages_per_chunk, 256, std::max<uint64_t>(1, ((uint64_t)ctx->cus * 4 + nch - 1) / nch)});
+            {(uint64_t)L.pages_per_chunk, 256, std::max<uint64_t>(1, ((uint64_t)ctx->cus * 4 + surviving - 1) / surviving)});
         const void *dq;
         if (int r = solo_worklist(ctx, &qd, sizeof(qd), parts, &dq, &L.queue, const_cast<uint2 **>(&L.items)))
             return r;
@@ -440,16 +450,11 @@ int golap_star_run(golap_ctx *ctx, const golap_star_plan *plan, golap_star_resul
     } else {
         L.q0 = qd;
     }
-    if (L.n_items && dm->filled) GOLAP_HIP_CHECK(ctx, launch_star(ctx, L, ctx->stream));
+    if (L.n_items && dm->filled && (!plan->dev_chunk_mask || surviving))
+        GOLAP_HIP_CHECK(ctx, launch_star(ctx, L, ctx->stream));
     GOLAP_HIP_CHECK(ctx, hipEventRecord(ctx->ev1, ctx->stream));
     uint32_t flags = 0;
     GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(&flags, ctx->d_scratch, 4, hipMemcpyDeviceToHost, ctx->stream));
-    std::vector<uint16_t> mask;
-    if (plan->dev_chunk_mask && plan->chunk_end > plan->chunk_begin) {
-        mask.resize(plan->chunk_end - plan->chunk_begin);
-        GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(mask.data(), plan->dev_chunk_mask + plan->chunk_begin, mask.size() * 2,
-                                            hipMemcpyDeviceToHost, ctx->stream));
-    }
     GOLAP_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     if (flags) return error_flags_status(ctx, flags, "star");
     float ms = 0;

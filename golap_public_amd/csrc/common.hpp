@@ -183,8 +183,9 @@ uint32_t lz4i_max_nseq(const golap_host_column &h, const uint8_t *comp, uint32_t
 // Bytes an LZ4I stage region needs for a column's pages (a multiple of 16).
 uint32_t lz4i_stage_bytes(const ColumnExtra &ex);
 // Points device column c (created with catalog capacity for at least w's pages and chunks) at a
-// window w of a larger column: catalog and zone maps copied (synchronously, on stream), planning
-// tables rebuilt; the caller fills d_comp with comp_bytes of page data.
+// window w of a larger column: catalog and zone maps copied on stream without a wait (from the
+// column's own host copy, kept until its next window), planning tables rebuilt; the caller fills
+// d_comp with comp_bytes of page data and orders every reader of the column after the copies.
 int set_column_window(golap_ctx *ctx, golap_column *c, const golap_host_column &w, uint64_t comp_bytes,
                       hipStream_t stream);
 void dev_release(golap_ctx *ctx, void *p);

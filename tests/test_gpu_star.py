@@ -168,6 +168,43 @@ def test_star_with_zone_map_pruning(orc, golap, ctx):
             ctx.free(b)
 
 
+def test_star_masked_page_ranges_match_oracle(orc, golap, ctx):
+    """A mask keeping 45 of 62 chunks of 64 pages (1,024-row pages, 65,536-row chunks): the
+    surviving chunks are cut into page ranges sized by their count (23 ranges of 3 pages on 256
+    CUs: the last range of a chunk is clamped, ranges past its end are empty); the groups equal the
+    oracle's over the surviving chunks' rows."""
+    n = 62 * 65536 - 1000
+    D = star_data(golap, n, seed=31)
+    enc = dict(page_rows=1024, chunk_rows=65536, lz4_min_match=4)
+    cols = {nm: ctx.upload(golap.HostColumn.encode(D[nm], codec=1, **enc)) for nm in ("date", "supp", "rev")}
+    maps, bufs = [], []
+    for key, n_dim, preds, pay in ((D["d_key"], 2556, [], D["d_year"]),
+                                   (D["s_key"], D["s_key"].size, [(D["s_region"], 2, 2)], None)):
+        m, b = build_map(ctx, golap, key, n_dim, preds, pay)
+        maps.append(m)
+        bufs += b
+    nch = ctx.column_desc(cols["date"]).n_chunks
+    keep = np.ones(nch, np.uint16)
+    keep[np.random.default_rng(32).choice(nch, nch - 45, replace=False)] = 0
+    flags = ctx.to_device(keep)
+    say("star masked page ranges: chunks", nch, "kept", int(keep.sum()))
+    try:
+        dims = [(cols["date"], maps[0], 1992, 7), (cols["supp"], maps[1], 0, 0)]
+        r, k, s_, c = ctx.star_run(dims, cols["rev"], dev_chunk_mask=flags)
+        assert r.chunks_pruned == nch - 45 and r.chunks_run == 45
+        rows = np.repeat(keep.astype(bool), 65536)[:n]
+        wk, ws, wc = orc.star_groupby([(D["d_key"], [], D["d_year"]), (D["s_key"], [(D["s_region"], 2, 2)], None)],
+                                      [D["date"][rows], D["supp"][rows]], D["rev"][rows], threads=4)
+        assert np.array_equal(k, wk) and np.array_equal(s_, ws) and np.array_equal(c, wc)
+    finally:
+        for c_ in cols.values():
+            ctx.free_column(c_)
+        for m in maps:
+            ctx.free_hashmap(m)
+        for b in bufs + [flags]:
+            ctx.free(b)
+
+
 SF10_Q2 = [(1737351738436, 280), (347570089279, 56), (43762644335, 7)]  # query_sf10_export.csv
 
 
