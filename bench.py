@@ -59,6 +59,10 @@ def parse():
     ap.add_argument("--chunk-rows", type=int, default=262144)
     ap.add_argument("--min-match", type=int, default=4,
                     help="LZ4 minimum match (4: what LZ4_compress_default emits)")
+    ap.add_argument("--lz4-unit", type=int, default=1,
+                    help="q1 workload: LZ4 sequence unit; 1: bytes (LZ4_compress_default), 0: whole "
+                         "elements of the column's width (nvcomp's LZ4 manager built with the column's "
+                         "data type, core/comp.cuh:230-236)")
     ap.add_argument("--seed", type=int, default=20240601)
     ap.add_argument("--cpu-runs", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -829,7 +833,7 @@ def main():
     cols = g.gen_q1_fact(args.seed, args.rows, key)
     log("encoding")
     enc = dict(codec=g.CODEC_LZ4, page_rows=args.page_rows, chunk_rows=args.chunk_rows)
-    hosts = [g.HostColumn.encode(c, lz4_min_match=args.min_match, **enc) for c in cols]
+    hosts = [g.HostColumn.encode(c, lz4_min_match=args.min_match, lz4_unit=args.lz4_unit, **enc) for c in cols]
     raw_bytes = args.rows * 10
     comp_payload = sum(h.desc().payload_bytes for h in hosts)
     log(f"compressed payload {comp_payload/1e9:.3f} GB for {raw_bytes/1e9:.3f} GB raw")
@@ -894,7 +898,7 @@ def main():
         "data": "synthetic: SplitMix64-seeded columns with the workload's distributions, LZ4 pages",
         "config": {"workload": "q1_fused_600M_lz4", "rows": args.rows, "chunk_rows": args.chunk_rows,
                    "page_rows": args.page_rows, "codec": "lz4", "lz4_min_match": args.min_match,
-                   "comp_bytes": r.comp_bytes, "raw_bytes": raw_bytes,
+                   "lz4_unit": "bytes" if args.lz4_unit == 1 else "element", "comp_bytes": r.comp_bytes, "raw_bytes": raw_bytes,
                    "plan": "year=1995, discount 4..6, quantity>=30, SUM(price*discount)",
                    "result_sum": r.sum, "result_matches": r.matches, "kernel_ms": kernel_ms,
                    "device": name, "read_probe_GBps": probe_gbps,

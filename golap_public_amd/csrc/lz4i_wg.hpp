@@ -62,35 +62,15 @@ __host__ __device__ inline Layout layout(uint32_t stage_cap, uint32_t out_cap, u
 __host__ __device__ inline uint32_t max_sequences(uint32_t osize) { return osize / 4 + 1; }
 
 // ---- aligned LDS access
-// The two aligned 16-byte blocks holding bytes [p, p + 16) of base (16-byte aligned).
-struct Blk32 {
-    u32x4 a, b;
-};
-__device__ __forceinline__ Blk32 ld_blk32(const uint8_t *base, uint32_t p) {
-    const u32x4 *q = reinterpret_cast<const u32x4 *>(base + (p & ~15u));
-    return Blk32{q[0], q[1]};
+// Bytes [p, p + 16) of base (4-byte aligned, readable 20 bytes past p & ~3): the five aligned
+// words from p & ~3, funnel-shifted by p % 4 bytes.
+__device__ __forceinline__ u32x4 ld_any16(const uint8_t *base, uint32_t p) {
+    const uint32_t *w = reinterpret_cast<const uint32_t *>(base + (p & ~3u));
+    const uint32_t d0 = w[0], d1 = w[1], d2 = w[2], d3 = w[3], d4 = w[4];
+    const uint32_t sh = 8 * (p & 3);
+    return u32x4{__builtin_amdgcn_alignbit(d1, d0, sh), __builtin_amdgcn_alignbit(d2, d1, sh),
+                 __builtin_amdgcn_alignbit(d3, d2, sh), __builtin_amdgcn_alignbit(d4, d3, sh)};
 }
-// Bytes [r, r + 16) of the 32 bytes of w, r <= 16 (bytes past 31 read as zero): the words are
-// shifted by 4, 2 and 1 places under the bits of r / 4 (plain per-lane selects, no indexed
-// register access), then by r % 4 bytes.
-__device__ __forceinline__ u32x4 bytes16(const Blk32 &w, uint32_t r) {
-    const uint32_t q = r >> 2, sh = 8 * (r & 3);
-    const bool s4 = q & 4, s2 = q & 2, s1 = q & 1;
-    const uint32_t w0 = w.a.x, w1 = w.a.y, w2 = w.a.z, w3 = w.a.w, w4 = w.b.x, w5 = w.b.y, w6 = w.b.z, w7 = w.b.w;
-    const uint32_t t0 = s4 ? w4 : w0, t1 = s4 ? w5 : w1, t2 = s4 ? w6 : w2, t3 = s4 ? w7 : w3;
-    const uint32_t t4 = s4 ? 0u : w4, t5 = s4 ? 0u : w5, t6 = s4 ? 0u : w6, t7 = s4 ? 0u : w7;
-    const uint32_t u0 = s2 ? t2 : t0, u1 = s2 ? t3 : t1, u2 = s2 ? t4 : t2, u3 = s2 ? t5 : t3, u4 = s2 ? t6 : t4;
-    const uint32_t u5 = s2 ? t7 : t5;
-    const uint32_t d0 = s1 ? u1 : u0, d1 = s1 ? u2 : u1, d2 = s1 ? u3 : u2, d3 = s1 ? u4 : u3, d4 = s1 ? u5 : u4;
-    u32x4 r4;
-    r4.x = (uint32_t)((((uint64_t)d1 << 32) | d0) >> sh);
-    r4.y = (uint32_t)((((uint64_t)d2 << 32) | d1) >> sh);
-    r4.z = (uint32_t)((((uint64_t)d3 << 32) | d2) >> sh);
-    r4.w = (uint32_t)((((uint64_t)d4 << 32) | d3) >> sh);
-    return r4;
-}
-// Bytes [p, p + 16) of base (16-byte aligned, readable 32 bytes past p & ~15).
-__device__ __forceinline__ u32x4 ld_any16(const uint8_t *base, uint32_t p) { return bytes16(ld_blk32(base, p), p & 15); }
 // Bytes [p, p + 4) of base (readable 8 bytes past p & ~3): two aligned words.
 __device__ __forceinline__ uint32_t ld_any4(const uint8_t *base, uint32_t p) {
     const uint32_t *w = reinterpret_cast<const uint32_t *>(base + (p & ~3u));
@@ -98,19 +78,22 @@ __device__ __forceinline__ uint32_t ld_any4(const uint8_t *base, uint32_t p) {
 }
 __device__ __forceinline__ uint32_t ld32(const uint8_t *p) { return *reinterpret_cast<const uint32_t *>(p); }
 __device__ __forceinline__ uint32_t ld16b(const uint8_t *p) { return *reinterpret_cast<const uint16_t *>(p); }
-// Mask of bytes [lo, hi) of a word, 0 <= lo, hi <= 4.
-__device__ __forceinline__ uint32_t byte_mask(int lo, int hi) {
-    return hi > lo ? (0xffffffffu >> (32 - 8 * (hi - lo))) << (8 * lo) : 0u;
+// The byte mask of a word from its four byte enables (bit i: byte i): the enables spread to bits
+// 0, 8, 16 and 24 by one 24-bit multiply, then each widened to its byte.
+__device__ __forceinline__ uint32_t byte_mask(uint32_t e) {
+    const uint32_t x = (e * 0x204081u) & 0x01010101u;  // e < 16: no carries between the copies
+    return (x << 8) - x;
 }
 // Bytes 0..n-1 of v (n <= 16) ORed into out[op, op + n): the output bytes there are zero.
 __device__ __forceinline__ void or_bytes(uint8_t *out, uint32_t op, const u32x4 &v, uint32_t n) {
     uint32_t *o32 = reinterpret_cast<uint32_t *>(out) + (op >> 2);
     const int o = (int)(op & 3), sh = 32 - 8 * o;
     const uint32_t L[6] = {0u, v.x, v.y, v.z, v.w, 0u};
+    const uint32_t en = ((1u << n) - 1u) << o;  // byte enables of the five words, four bits each
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
         const uint32_t w = (uint32_t)((((uint64_t)L[k + 1] << 32) | L[k]) >> sh);
-        const uint32_t m = byte_mask(max(o - 4 * k, 0), min(max((int)n + o - 4 * k, 0), 4));
+        const uint32_t m = byte_mask((en >> (4 * k)) & 15u);
         if (m) atomicOr(o32 + k, w & m);
     }
 }
@@ -132,12 +115,12 @@ __device__ __forceinline__ void put16(uint8_t *out, uint32_t op, const u32x4 &v,
 __device__ __forceinline__ uint32_t bmask(uint32_t lo, uint32_t hi) {
     return (0xffffffffu >> (31u - hi)) & (0xffffffffu << lo);
 }
-// Masks of units [a, a + n) in words a >> 5 and (a >> 5) + 1, for n <= 33 (both zero for n == 0).
+// Masks of units [a, a + n) in words a >> 5 and (a >> 5) + 1, for n <= 33 (both zero for n == 0):
+// the halves of one 64-bit mask.
 __device__ __forceinline__ void masks2(uint32_t a, uint32_t n, uint32_t &m0, uint32_t &m1) {
-    const uint32_t e = a + (n ? n - 1 : 0u);
-    const bool one = (e >> 5) == (a >> 5);
-    m0 = n ? bmask(a & 31, one ? (e & 31) : 31u) : 0u;
-    m1 = n && !one ? bmask(0, e & 31) : 0u;
+    const uint64_t m = ((1ull << n) - 1) << (a & 31);
+    m0 = (uint32_t)m;
+    m1 = (uint32_t)(m >> 32);
 }
 // Are units [a, a + n) final (1 <= n <= 33)? The bitmap has a spare word past its end.
 __device__ __forceinline__ bool bits_all(const uint32_t *bits, uint32_t a, uint32_t n) {
