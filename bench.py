@@ -46,8 +46,8 @@ def parse():
     ap.add_argument("--queries", type=int, default=64, help="batch workload: queries per batch")
     ap.add_argument("--zone-rows", type=int, default=65536, help="batch workload: rows per zone-map chunk")
     ap.add_argument("--trip-rows", type=int, default=1_500_000_000, help="serve workload: trip-table rows")
-    ap.add_argument("--requests", type=int, default=400, help="serve workload: arrivals per rate")
-    ap.add_argument("--sweep-requests", type=int, default=300,
+    ap.add_argument("--requests", type=int, default=1500, help="serve workload: arrivals per rate")
+    ap.add_argument("--sweep-requests", type=int, default=1500,
                     help="q1 workload: requests per rate of the serving sweep reported beside it")
     ap.add_argument("--no-serve-sweep", action="store_true", help="q1 workload: skip the serving sweep")
     ap.add_argument("--no-mm16", action="store_true", help="q1 workload: skip the min-match 16 run")
@@ -589,8 +589,11 @@ class ServeRig:
         return lat, time.perf_counter() - start
 
     def sweep(self, n_req):
-        """Poisson arrivals of n_req requests at 0.5x and 0.9x the plain capacity and at 0.9x the
-        served capacity (a closed burst of n_req requests), both forms at each rate."""
+        """Poisson arrivals of n_req requests at 0.5x and 0.9x the plain capacity, at 0.9x the
+        served capacity (a closed burst of n_req requests) and at 3.2x the plain capacity, both
+        forms at each rate. A form keeps up with a rate when the median latency of the last
+        quarter of the requests is less than twice that of the first quarter (a growing queue
+        shows as growth whatever the run length)."""
         n_req = min(n_req, len(self.reqs))
         plain_cap = self.plain_capacity()
         _, _, burst_wall, _ = self.served([0.0] * n_req)
@@ -598,8 +601,10 @@ class ServeRig:
         log(f"serve: plain capacity {plain_cap:.1f} q/s, served capacity {served_cap:.1f} q/s")
         pct = lambda a, q: float(np.percentile(np.array(a), q))
         rows, last_st = [], None
+        q = max(1, n_req // 4)
+        growth = lambda a: float(np.median(np.array(a[-q:])) / max(np.median(np.array(a[:q])), 1e-3))
         for name, rate in (("0.5x plain capacity", 0.5 * plain_cap), ("0.9x plain capacity", 0.9 * plain_cap),
-                           ("0.9x served capacity", 0.9 * served_cap)):
+                           ("0.9x served capacity", 0.9 * served_cap), ("3.2x plain capacity", 3.2 * plain_cap)):
             arr = np.cumsum(np.random.default_rng(self.args.seed + 1).exponential(1.0 / rate, n_req))
             ls, res, ws, st = self.served(arr)
             lp, wp = self.plain_at(arr, res)
@@ -607,11 +612,15 @@ class ServeRig:
                    "served_qps": n_req / ws, "served_p50_ms": pct(ls, 50), "served_p99_ms": pct(ls, 99),
                    "plain_qps": n_req / wp, "plain_p50_ms": pct(lp, 50), "plain_p99_ms": pct(lp, 99)}
             row["served_over_plain_qps"] = row["served_qps"] / row["plain_qps"]
+            row["served_latency_growth"], row["plain_latency_growth"] = growth(ls), growth(lp)
+            row["served_keeps_up"] = row["served_latency_growth"] < 2.0
+            row["plain_keeps_up"] = row["plain_latency_growth"] < 2.0
             row["meets_3x_and_lower_p99"] = bool(row["served_over_plain_qps"] >= 3.0 and
                                                  row["served_p99_ms"] < row["plain_p99_ms"])
             log(f"serve @ {name} ({rate:.1f} q/s): served {row['served_qps']:.1f} q/s p50 "
                 f"{row['served_p50_ms']:.2f} p99 {row['served_p99_ms']:.2f} ms; plain {row['plain_qps']:.1f} q/s "
-                f"p50 {row['plain_p50_ms']:.2f} p99 {row['plain_p99_ms']:.2f} ms")
+                f"p50 {row['plain_p50_ms']:.2f} p99 {row['plain_p99_ms']:.2f} ms; latency growth served "
+                f"{row['served_latency_growth']:.2f} plain {row['plain_latency_growth']:.2f}")
             rows.append(row)
             last_st = st
         return {"plain_capacity_qps": plain_cap, "served_capacity_qps": served_cap, "rates": rows}, last_st
@@ -623,7 +632,7 @@ class ServeRig:
 
 def run_serve(args):
     """Serving form on the trip table: the sweep of ServeRig.sweep; the line's value is the served
-    queries/s at 0.9x the served capacity (the rate where plain launches fall behind)."""
+    queries/s at 3.2x the plain capacity (a rate plain launches cannot keep up with)."""
     import golap_public_amd as g
     ctx = g.Context(0)
     cpu_rows = 0 if args.no_cpu_baseline else min(args.trip_rows, 150_000_000)
@@ -933,7 +942,8 @@ def main():
         out["config"]["serving"] = {
             "workload": f"serve_trip_1.5B sweep ({rig.n} trip rows, {args.sweep_requests} requests per rate)",
             "served_over_plain_qps": top["served_over_plain_qps"], "served_p99_ms": top["served_p99_ms"],
-            "plain_p99_ms": top["plain_p99_ms"], "at_rate": top["rate"], "sweep": sw}
+            "plain_p99_ms": top["plain_p99_ms"], "at_rate": top["rate"], "served_keeps_up": top["served_keeps_up"],
+            "plain_keeps_up": top["plain_keeps_up"], "sweep": sw}
         rig.free()
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -203,7 +203,7 @@ hipError_t launch_decode_pages(golap_ctx *ctx, const golap_column *col, uint64_t
     a.err = err;
     if (col->extra.has_lz4i && page_bytes < 65536) {
         const uint32_t tok_cap = col->extra.lz4i_max_nseq ? col->extra.lz4i_max_nseq : lz4wg::max_sequences(page_bytes);
-        const lz4wg::Layout l = lz4wg::layout(lz4i_stage_bytes(col->extra), page_bytes, tok_cap, BLOCK);
+        const lz4wg::Layout l = lz4wg::layout(lz4i_stage_bytes(col->extra), page_bytes, tok_cap, BLOCK, true);
         if (l.total <= (uint32_t)ctx->lds_max) {
             hipError_t e = prepare_lds(ctx, k_decode_wg, l.total);
             if (e != hipSuccess) return e;

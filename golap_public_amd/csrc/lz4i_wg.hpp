@@ -38,12 +38,16 @@ constexpr uint32_t SHORT = 16;       // literals and matches up to this long are
 constexpr uint32_t LIST_MIN = 256;   // entries the lists hold at least
 
 // Byte offsets of the decode's LDS regions from the caller's base, for pages of up to stage_cap
-// coded bytes and tok_cap sequences decoding to at most out_cap bytes, with nt threads.
+// coded bytes and tok_cap sequences decoding to at most out_cap bytes, with nt threads. The
+// token array is reused for the lists of long literals and pending matches (8 bytes an entry):
+// at least LIST_MIN entries, or one per sequence with full_list (a page whose pending matches
+// overflow the list is handed back for the in-order decode).
 struct Layout {
     uint32_t stage, out, bits, tok, wtot, misc, total;
-    uint32_t stage_cap, out_cap, tok_cap;
+    uint32_t stage_cap, out_cap, tok_cap, list_cap;
 };
-__host__ __device__ inline Layout layout(uint32_t stage_cap, uint32_t out_cap, uint32_t tok_cap, uint32_t nt) {
+__host__ __device__ inline Layout layout(uint32_t stage_cap, uint32_t out_cap, uint32_t tok_cap, uint32_t nt,
+                                         bool full_list = false) {
     Layout l;
     l.stage_cap = stage_cap;
     l.out_cap = out_cap;
@@ -52,7 +56,9 @@ __host__ __device__ inline Layout layout(uint32_t stage_cap, uint32_t out_cap, u
     l.out = (stage_cap + 32 + 15) & ~15u;           // two 16-byte blocks may be read past the page end
     l.bits = l.out + ((out_cap + 32 + 15) & ~15u);  // and past the output end
     l.tok = l.bits + ((((out_cap + 31) / 32 + 2) * 4 + 15) & ~15u);
-    l.wtot = l.tok + ((max(2 * tok_cap, 8u * LIST_MIN) + 15) & ~15u);  // the token array, later lists of 8-byte entries
+    const uint32_t list_bytes = max(2 * tok_cap, 8u * max(LIST_MIN, full_list ? tok_cap : 0u));
+    l.list_cap = list_bytes / 8;
+    l.wtot = l.tok + ((list_bytes + 15) & ~15u);  // the token array, later the lists
     l.misc = l.wtot + MAXJ * (nt / 64) * 4;
     l.total = l.misc + 32;
     return l;
@@ -464,7 +470,7 @@ __device__ __forceinline__ int64_t decode(uint8_t *lds, const Layout &l, uint32_
     // longer literals: gathered into a list (in the token array's place, no longer read) and
     // copied one per thread
     uint32_t *list = reinterpret_cast<uint32_t *>(lds + l.tok);
-    const uint32_t list_cap = max(l.tok_cap / 4, LIST_MIN);
+    const uint32_t list_cap = l.list_cap;
     if (tid == 0) misc[0] = 0;
     if (any(bad)) return -1;
     if (any(longs != 0)) {
