@@ -119,7 +119,10 @@ __global__ __launch_bounds__(WAVE) void k_decode_pages(DecodeArgs a) {
 // One workgroup (256 threads) per page, for columns with LZ4I pages: an LZ4I page is staged in
 // LDS and decoded by the whole workgroup (lz4wg::decode); in-place pages are copied
 // global->global; any other coded page (and an LZ4I page the cooperative decode does not take)
-// is decoded by the first wave. The decoded page is written out with 16-byte stores.
+// is decoded by the first wave. The decoded page is written out with 16-byte stores. JUMP: the
+// layout has a source map, and matches left pending by the ordered pass are resolved by pointer
+// jumping (pages whose matches copy from matches, in long chains).
+template <bool JUMP>
 __global__ __launch_bounds__(BLOCK) void k_decode_wg(DecodeArgs a, lz4wg::Layout l) {
     extern __shared__ __align__(16) uint8_t lds[];
     const uint32_t tid = threadIdx.x;
@@ -150,7 +153,7 @@ __global__ __launch_bounds__(BLOCK) void k_decode_wg(DecodeArgs a, lz4wg::Layout
                 (reinterpret_cast<uintptr_t>(src) & 15) == 0) {
                 lz4wg::stage<BLOCK>(lds, l, src, csize);
                 __syncthreads();
-                got = lz4wg::decode<BLOCK>(lds, l, csize, nbytes);
+                got = lz4wg::decode<BLOCK, JUMP>(lds, l, csize, nbytes);
                 in_order = lds + l.stage;
             }
             if (got == -2) {
@@ -203,13 +206,18 @@ hipError_t launch_decode_pages(golap_ctx *ctx, const golap_column *col, uint64_t
     a.err = err;
     if (col->extra.has_lz4i && page_bytes < 65536) {
         const uint32_t tok_cap = col->extra.lz4i_max_nseq ? col->extra.lz4i_max_nseq : lz4wg::max_sequences(page_bytes);
+        // the source map for pointer jumping when it leaves room for two workgroups per CU
+        const lz4wg::Layout lj = lz4wg::layout(lz4i_stage_bytes(col->extra), page_bytes, tok_cap, BLOCK, true, true);
         const lz4wg::Layout l = lz4wg::layout(lz4i_stage_bytes(col->extra), page_bytes, tok_cap, BLOCK, true);
+        const bool jump = 2 * (uint64_t)lj.total <= (uint64_t)ctx->lds_max;
         if (l.total <= (uint32_t)ctx->lds_max) {
-            hipError_t e = prepare_lds(ctx, k_decode_wg, l.total);
+            const lz4wg::Layout &u = jump ? lj : l;
+            auto kernel = jump ? k_decode_wg<true> : k_decode_wg<false>;
+            hipError_t e = prepare_lds(ctx, kernel, u.total);
             if (e != hipSuccess) return e;
             const uint32_t grid = (uint32_t)std::min<uint64_t>(page_end - page_begin,
-                                                               (uint64_t)ctx->cus * wg_per_cu(ctx, l.total));
-            hipLaunchKernelGGL(k_decode_wg, dim3(grid), dim3(BLOCK), l.total, stream, a, l);
+                                                               (uint64_t)ctx->cus * wg_per_cu(ctx, u.total));
+            hipLaunchKernelGGL(kernel, dim3(grid), dim3(BLOCK), u.total, stream, a, u);
             return hipGetLastError();
         }
     }

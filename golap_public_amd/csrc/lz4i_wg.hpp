@@ -42,12 +42,13 @@ constexpr uint32_t LIST_MIN = 256;   // entries the lists hold at least
 // token array is reused for the lists of long literals and pending matches (8 bytes an entry):
 // at least LIST_MIN entries, or one per sequence with full_list (a page whose pending matches
 // overflow the list is handed back for the in-order decode).
+// With jump, a source map of two bytes per output byte follows for decode<NT, true>.
 struct Layout {
-    uint32_t stage, out, bits, tok, wtot, misc, total;
+    uint32_t stage, out, bits, tok, wtot, misc, jump, total;
     uint32_t stage_cap, out_cap, tok_cap, list_cap;
 };
 __host__ __device__ inline Layout layout(uint32_t stage_cap, uint32_t out_cap, uint32_t tok_cap, uint32_t nt,
-                                         bool full_list = false) {
+                                         bool full_list = false, bool jump = false) {
     Layout l;
     l.stage_cap = stage_cap;
     l.out_cap = out_cap;
@@ -60,7 +61,8 @@ __host__ __device__ inline Layout layout(uint32_t stage_cap, uint32_t out_cap, u
     l.list_cap = list_bytes / 8;
     l.wtot = l.tok + ((list_bytes + 15) & ~15u);  // the token array, later the lists
     l.misc = l.wtot + MAXJ * (nt / 64) * 4;
-    l.total = l.misc + 32;
+    l.jump = jump ? l.misc + 32 : 0u;
+    l.total = jump ? l.jump + ((2 * out_cap + 15) & ~15u) : l.misc + 32;
     return l;
 }
 // The most sequences a page decoding to osize bytes can have: every sequence but the last
@@ -430,11 +432,76 @@ __device__ __forceinline__ int parse(uint8_t *lds, const Layout &l, uint32_t csi
     return 0;
 }
 
+// Resolves the pending matches of the threads (pend, mo = destination | offset << 16, ml =
+// length) by pointer jumping over a source map of the output units: a pending unit maps to the
+// unit it copies, every other unit (final: literal or copied match) to itself. Each round
+// replaces a unit's source by its source's source, halving every chain (a source always lies
+// before its unit, so chains end at a final unit); then every pending unit takes its final
+// source's value. Rounds: about log2 of the longest chain, against one list pass per link.
+template <int NT>
+__device__ __forceinline__ int64_t resolve_by_jumping(uint8_t *lds, const Layout &l, uint32_t osize, uint32_t ub,
+                                                      uint32_t pend, const uint32_t (&mo)[MAXJ],
+                                                      const uint32_t (&ml)[MAXJ], BlockAny &any) {
+    const uint32_t tid = threadIdx.x;
+    uint8_t *out = lds + l.out;
+    uint16_t *map = reinterpret_cast<uint16_t *>(lds + l.jump);
+    const uint32_t nu = osize >> ub;
+    for (uint32_t i = tid; i < nu; i += NT) map[i] = (uint16_t)i;
+    __syncthreads();
+#pragma unroll
+    for (uint32_t j = 0; j < MAXJ; ++j) {
+        if (!((pend >> j) & 1u)) continue;
+        const uint32_t d = (mo[j] & 0xffffu) >> ub, s = ((mo[j] & 0xffffu) - (mo[j] >> 16)) >> ub, n = ml[j] >> ub;
+        for (uint32_t k = 0; k < n; ++k) map[d + k] = (uint16_t)(s + k);
+    }
+    __syncthreads();
+    for (uint32_t round = 0;; ++round) {
+        bool changed = false;
+        for (uint32_t i = tid; i < nu; i += NT) {
+            const uint32_t m = map[i];
+            if (m != i) {
+                const uint32_t mm = map[m];
+                if (mm != m) {
+                    map[i] = (uint16_t)mm;
+                    changed = true;
+                }
+            }
+        }
+        if (!any(changed)) break;
+        if (round > 20) return -1;  // chains of at most 2^16 units end within 17 rounds
+    }
+    // every source is final now and no final unit is written below
+    uint32_t *o32 = reinterpret_cast<uint32_t *>(out);
+    if (ub == 2) {
+        for (uint32_t i = tid; i < nu; i += NT) {
+            const uint32_t m = map[i];
+            if (m != i) o32[i] = o32[m];
+        }
+    } else {
+        for (uint32_t w = tid; w < (osize + 3) / 4; w += NT) {
+            uint32_t add = 0;
+#pragma unroll
+            for (uint32_t b = 0; b < 4; ++b) {
+                const uint32_t i = 4 * w + b;
+                if (i < osize) {
+                    const uint32_t m = map[i];
+                    if (m != i) add |= (uint32_t)out[m] << (8 * b);
+                }
+            }
+            if (add) o32[w] |= add;  // the word's other bytes are final; this thread owns it here
+        }
+    }
+    __syncthreads();
+    return osize;
+}
+
 // Decodes the LZ4I page staged at lds + l.stage (csize bytes, staged before a __syncthreads())
 // into lds + l.out[0, osize). All NT threads call it. Returns osize, -1 if the page is malformed,
 // -2 if this path does not take it (parse's cases, or more long literals or pending matches than
 // a list holds). The caller then decodes the page in order. The result is the same in every thread.
-template <int NT>
+// JUMP (l.jump set): matches the ordered pass leaves pending are resolved by pointer jumping
+// instead of list passes.
+template <int NT, bool JUMP = false>
 __device__ __forceinline__ int64_t decode(uint8_t *lds, const Layout &l, uint32_t csize, uint32_t osize) {
     const uint32_t tid = threadIdx.x;
     uint8_t *out = lds + l.out;
@@ -534,6 +601,7 @@ __device__ __forceinline__ int64_t decode(uint8_t *lds, const Layout &l, uint32_
     __syncthreads();
     const uint32_t total = misc[0];
     if (total == 0) return osize;
+    if constexpr (JUMP) return resolve_by_jumping<NT>(lds, l, osize, ub, pend, ps.mo, ml, any);
     if (total > list_cap) return -2;
     {
         uint32_t i = at;

@@ -69,6 +69,28 @@ def test_large_column_decode(orc, golap, ctx):
         ctx.free_column(col)
 
 
+@pytest.mark.parametrize("dtype,page_rows", [(np.uint8, 4096), (np.uint8, 16384), (np.int32, 4096)])
+def test_decode_long_match_chains(orc, golap, ctx, dtype, page_rows):
+    """Pages whose matches copy from other matches in chains tens of links deep (small Zipf
+    values: the trip table's passenger count), where most matches are left pending by the ordered
+    pass and resolved by pointer jumping; the typed case resolves whole words."""
+    r = rng(17)
+    n = 40 * page_rows + 123
+    v = np.minimum(r.zipf(1.2, n), 6).astype(dtype)
+    say("decode zipf chains", np.dtype(dtype).name, "page_rows", page_rows, "rows", n)
+    for unit in (1, 0):
+        hc = golap.HostColumn.encode(v, codec=1, page_rows=page_rows, chunk_rows=4 * page_rows, lz4_min_match=4,
+                                     lz4_unit=unit)
+        want = orc.decode_column(hc.arrays())
+        assert want[:v.nbytes].tobytes() == v.tobytes()
+        col = ctx.upload(hc)
+        try:
+            got, _ = ctx.decompress(col)
+            assert np.array_equal(got, want)
+        finally:
+            ctx.free_column(col)
+
+
 @pytest.mark.parametrize("damage", ["truncate", "bad_offset"])
 def test_corrupt_page_reports_ecorrupt(orc, golap, ctx, damage):
     data = np.frombuffer((b"abcdefgh" * 1024), np.uint8)
