@@ -28,12 +28,17 @@ struct BatchSlot {
     uint2 *d_items = nullptr;   // star work list
     uint4 *d_items4 = nullptr;  // scan work list
     uint64_t arena_cap = 0, q_cap = 0, off_cap = 0, items_cap = 0, items4_cap = 0;
+    uint8_t *d_dec = nullptr;        // scan batches: columns decoded ahead of the launch
+    uint32_t *d_dec_err = nullptr;   // their decode's error word
+    uint64_t dec_cap = 0;
     std::string err;
 
     // own_stream: a new stream (serving slots); otherwise the context stream.
     int init(golap_ctx *c, bool own);
     // Grows the buffers to at least these sizes (bytes, bytes, entries, entries, entries).
     int reserve(uint64_t arena, uint64_t qbytes, uint64_t offs, uint64_t items, uint64_t items4);
+    // Grows the decode buffer to at least bytes (and allocates its error word).
+    int reserve_dec(uint64_t bytes);
     void release();
 };
 
@@ -79,6 +84,10 @@ struct golap_batch {
     // scan batches: union of the slices, surviving (chunk, plan word) pairs, bytes read once
     uint64_t union_begin = 0, union_end = 0, union_pairs = 0, shared_bytes = 0;
     uint64_t max_local = 0;  // largest local key domain of a plan on a surviving chunk
+    // scan batches of several plans: LZ4I columns decoded ahead of the launch (the scan reads them
+    // in place; its passes over an item do not decode them again), over these chunk runs
+    uint32_t dec_mask = 0;
+    std::vector<std::pair<uint64_t, uint64_t>> dec_runs;
     // star batches: fact column mcol that every query probes, and each query's membership filter
     // of its dimension on that column (64 entries per word of queries), for the shared filter
     golap_impl::DriverFilterView *d_mfilt = nullptr;

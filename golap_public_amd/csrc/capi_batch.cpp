@@ -215,6 +215,9 @@ hipError_t launch_kernel(golap_batch *b, hipStream_t s, bool prepare_only) {
     L.n_cols = (uint32_t)b->cs.cols.size();
     for (uint32_t i = 0; i < L.n_cols; ++i)
         L.col[i] = ScanColView{view(b->cs.cols[i]), b->cs.cols[i]->type, b->windows[i], b->dslots[i], b->wcaps[i]};
+    if (b->dec_mask)
+        scan_decode_views(b->cs.cols.data(), L.n_cols, b->dec_mask, b->union_begin, b->union_end - b->union_begin,
+                          sl->d_dec, L.col);
     L.tok_off = b->tok_off;
     L.tok_cap = b->tok_cap;
     L.page_rows = b->page_rows;
@@ -386,6 +389,22 @@ int finish_create(golap_ctx *ctx, golap_batch *b, bool with_graph) {
             for (uint64_t k = 0; k < d.chunk_end - d.chunk_begin; ++k)
                 if (b->survive[i][k]) cmask[(d.chunk_begin + k - b->union_begin) * nw + i / 64] |= colmask_of(b, i) | (1u << 31);
         }
+        b->dec_runs.clear();
+        for (uint64_t k = 0; b->dec_mask && k < un;) {  // chunks where a kept plan reads a decoded column
+            auto needs = [&](uint64_t j) {
+                for (uint32_t w = 0; w < nw; ++w)
+                    if (cmask[j * nw + w] & b->dec_mask) return true;
+                return false;
+            };
+            if (!needs(k)) {
+                ++k;
+                continue;
+            }
+            uint64_t e = k + 1;
+            while (e < un && needs(e)) ++e;
+            b->dec_runs.emplace_back(b->union_begin + k, b->union_begin + e);
+            k = e;
+        }
         b->union_pairs = 0;
         b->shared_bytes = 0;
         for (uint64_t k = 0; k < un; ++k)
@@ -415,6 +434,11 @@ int finish_create(golap_ctx *ctx, golap_batch *b, bool with_graph) {
     if (int r = b->slot->reserve(off, (uint64_t)nq * desc_size(b), b->chunk_off.size(), b->kind == BatchKind::STAR ? items : 1,
                                  items4))
         return fail(ctx, r, b->slot->err);
+    if (b->dec_mask) {
+        const uint64_t need = scan_decode_bytes(b->cs.cols.data(), (uint32_t)b->cs.cols.size(), b->dec_mask,
+                                                b->union_end - b->union_begin);
+        if (int r = b->slot->reserve_dec(need)) return fail(ctx, r, b->slot->err);
+    }
     GOLAP_HIP_CHECK(ctx, hipMemcpyAsync(b->slot->d_chunk_off, b->chunk_off.data(), b->chunk_off.size() * 8,
                                         hipMemcpyHostToDevice, b->slot->stream));
     b->shape = shape_of(b);
@@ -498,7 +522,35 @@ int BatchSlot::reserve(uint64_t arena, uint64_t qbytes, uint64_t offs, uint64_t 
     return GOLAP_OK;
 }
 
+int BatchSlot::reserve_dec(uint64_t bytes) {
+    if (!d_dec_err) {
+        void *q = nullptr;
+        if (int r = dev_alloc(ctx, 256, &q)) {
+            err = last_error(ctx);
+            return r;
+        }
+        d_dec_err = static_cast<uint32_t *>(q);
+    }
+    if (bytes <= dec_cap) return GOLAP_OK;
+    dev_release(ctx, d_dec);
+    d_dec = nullptr;
+    dec_cap = 0;
+    void *q = nullptr;
+    if (int r = dev_alloc(ctx, bytes, &q)) {
+        err = last_error(ctx);
+        return r;
+    }
+    d_dec = static_cast<uint8_t *>(q);
+    dec_cap = bytes;
+    return GOLAP_OK;
+}
+
 void BatchSlot::release() {
+    dev_release(ctx, d_dec);
+    dev_release(ctx, d_dec_err);
+    d_dec = nullptr;
+    d_dec_err = nullptr;
+    dec_cap = 0;
     dev_release(ctx, d_arena);
     dev_release(ctx, d_q);
     dev_release(ctx, d_chunk_off);
@@ -600,8 +652,10 @@ int batch_create_scan(golap_ctx *ctx, const golap_scan_plan *q, uint32_t n, Batc
     {
         int32_t woff[SCAN_MAX_COLS], dslot[SCAN_MAX_COLS];
         uint32_t wcap[SCAN_MAX_COLS];
-        b->win_bytes = scan_layout(b->cs.cols.data(), (uint32_t)b->cs.cols.size(), used, std::min(pb, pe), pe, woff,
-                                   dslot, wcap, &b->tok_off, &b->tok_cap);
+        b->dec_mask = n >= 2 ? scan_decode_mask(b->cs.cols.data(), (uint32_t)b->cs.cols.size(), used, std::min(pb, pe), pe)
+                             : 0u;
+        b->win_bytes = scan_layout(b->cs.cols.data(), (uint32_t)b->cs.cols.size(), used & ~b->dec_mask, std::min(pb, pe),
+                                   pe, woff, dslot, wcap, &b->tok_off, &b->tok_cap);
         for (size_t c = 0; c < b->cs.cols.size(); ++c) {
             b->windows[c] = woff[c];
             b->dslots[c] = dslot[c];
@@ -622,6 +676,13 @@ int batch_enqueue(golap_batch *b, bool use_graph) {
         BATCH_HIP_CHECK(b, hipStreamWaitEvent(sl->stream, sl->dep, 0));
     }
     BATCH_HIP_CHECK(b, hipEventRecord(sl->ev0, sl->stream));
+    if (b->dec_mask) {  // outside the graph: its fill kernel clears the arena's error word
+        BATCH_HIP_CHECK(b, hipMemsetAsync(sl->d_dec_err, 0, 4, sl->stream));
+        for (const auto &r : b->dec_runs)
+            BATCH_HIP_CHECK(b, scan_decode_enqueue(b->ctx, b->cs.cols.data(), (uint32_t)b->cs.cols.size(), b->dec_mask,
+                                                   b->union_begin, b->union_end - b->union_begin, sl->d_dec, r.first,
+                                                   r.second, sl->stream, sl->d_dec_err));
+    }
     if (use_graph && b->exec) {
         BATCH_HIP_CHECK(b, hipGraphLaunch(b->exec, sl->stream));
     } else if (int r = enqueue_kernels(b, sl->stream)) {
@@ -637,7 +698,9 @@ int batch_finish(golap_batch *b, golap_batch_run_info *info) {
     BatchSlot *sl = b->slot;
     uint32_t *hdr = reinterpret_cast<uint32_t *>(sl->h_arena);
     BATCH_HIP_CHECK(b, hipMemcpyAsync(hdr, sl->d_arena, 16, hipMemcpyDeviceToHost, sl->stream));
+    if (b->dec_mask) BATCH_HIP_CHECK(b, hipMemcpyAsync(hdr + 3, sl->d_dec_err, 4, hipMemcpyDeviceToHost, sl->stream));
     BATCH_HIP_CHECK(b, hipStreamSynchronize(sl->stream));
+    if (b->dec_mask && hdr[3]) return bfail(b, GOLAP_ECORRUPT, "batch: corrupt page in a decoded column");
     float ms = 0;
     BATCH_HIP_CHECK(b, hipEventElapsedTime(&ms, sl->ev0, sl->ev1));
     b->last_ms = ms;

@@ -156,6 +156,63 @@ uint32_t scan_layout(const golap_column *const *cols, uint32_t n_cols, uint32_t 
     return bytes + (tcap ? lz4i_scratch_host(tcap) : 0);
 }
 
+uint32_t scan_decode_mask(const golap_column *const *cols, uint32_t n_cols, uint32_t colmask, uint64_t pb, uint64_t pe) {
+    uint32_t m = 0;
+    for (uint32_t c = 0; c < n_cols; ++c) {
+        if (!((colmask >> c) & 1u)) continue;
+        const auto &ex = cols[c]->extra;
+        const uint64_t pbytes = (uint64_t)cols[c]->page_rows * cols[c]->elem;
+        const uint32_t coded = ex.window_prefix[pe] - ex.window_prefix[pb] - (ex.dbp_prefix[pe] - ex.dbp_prefix[pb]);
+        if (ex.has_lz4i && pbytes < 65536 && coded) m |= 1u << c;
+    }
+    return m;
+}
+
+namespace {
+uint64_t region_bytes(const golap_column *c, uint64_t un) { return (un * c->chunk_rows * c->elem + 255) & ~255ull; }
+}  // namespace
+
+uint64_t scan_decode_bytes(const golap_column *const *cols, uint32_t n_cols, uint32_t mask, uint64_t un) {
+    uint64_t b = 0;
+    for (uint32_t c = 0; c < n_cols; ++c)
+        if ((mask >> c) & 1u) b += region_bytes(cols[c], un);
+    return b;
+}
+
+void scan_decode_views(const golap_column *const *cols, uint32_t n_cols, uint32_t mask, uint64_t c0, uint64_t un,
+                       uint8_t *buf, ScanColView *views) {
+    uint64_t off = 0;
+    for (uint32_t c = 0; c < n_cols; ++c) {
+        if (!((mask >> c) & 1u)) continue;
+        views[c].decoded = 1;
+        views[c].c.comp = buf + off;
+        views[c].dec_row0 = c0 * cols[c]->chunk_rows;
+        views[c].woff = -1;
+        views[c].dslot = -1;
+        views[c].wcap = 0;
+        off += region_bytes(cols[c], un);
+    }
+}
+
+hipError_t scan_decode_enqueue(golap_ctx *ctx, const golap_column *const *cols, uint32_t n_cols, uint32_t mask,
+                               uint64_t c0, uint64_t un, uint8_t *buf, uint64_t cb, uint64_t ce, hipStream_t st,
+                               uint32_t *err) {
+    if (cb < c0 || ce > c0 + un || cb >= ce) return cb == ce ? hipSuccess : hipErrorInvalidValue;
+    uint64_t off = 0;
+    for (uint32_t c = 0; c < n_cols; ++c) {
+        if (!((mask >> c) & 1u)) continue;
+        const golap_column *col = cols[c];
+        const uint64_t ppc = col->chunk_rows / col->page_rows;
+        const uint64_t pe = std::min<uint64_t>(col->n_pages, ce * ppc);
+        if (cb * ppc < pe) {
+            const hipError_t e = launch_decode_pages(ctx, col, cb * ppc, pe, buf + off, c0 * col->chunk_rows, st, err);
+            if (e != hipSuccess) return e;
+        }
+        off += region_bytes(col, un);
+    }
+    return hipSuccess;
+}
+
 // Waves per workgroup that take pages (each needs its own windows): all of them when their
 // windows fit beside agg_entries dense partial entries, else fewer.
 uint32_t scan_page_waves(golap_ctx *ctx, uint32_t win_bytes, uint32_t agg_entries) {
@@ -256,6 +313,8 @@ int scan_run_impl(golap_ctx *ctx, const golap_scan_plan *plan, golap_star_result
     L.n_cols = plan->n_cols;
     int32_t woff[SCAN_MAX_COLS], dslot[SCAN_MAX_COLS];
     uint32_t wcap[SCAN_MAX_COLS];
+    // one plan: LZ4I pages decode inside the scan, each wave into its window, overlapped with
+    // the other waves' work (decoding ahead measured slower here; batches decode ahead)
     L.win_bytes = scan_layout(plan->cols, plan->n_cols, qd.colmask, qd.s.page_begin, qd.s.page_end, woff, dslot, wcap,
                               &L.tok_off, &L.tok_cap);
     for (uint32_t c = 0; c < plan->n_cols; ++c)

@@ -367,6 +367,9 @@ struct ScanColView {
     int32_t woff;   // byte offset of the column's LDS window, -1 = none
     int32_t dslot;  // register slot for its delta-bitpack pages, -1 = none
     uint32_t wcap;  // window bytes when it decodes LZ4I pages in parallel (output + staging), else 0
+    uint32_t decoded;   // 1: decoded before the launch; row r is read in place at c.comp + (r - dec_row0) * elem
+    uint32_t pad;
+    uint64_t dec_row0;
 };
 struct ScanPredDev {
     uint32_t col, pad;

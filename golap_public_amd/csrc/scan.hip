@@ -557,8 +557,16 @@ __device__ void wave_page(const ScanArgs &a, ScanShared &S, uint64_t p, uint32_t
     TPlan *tp = S.tp[wave];
     if (lane < a.n_cols && ((used >> lane) & 1u)) {
         const ScanColView &v = S.cols[lane];
-        const uint64_t off = v.c.page_off[p];
-        tp[lane] = plan_page(v, v.c.comp + off, off, v.c.page_codec[p], v.c.page_csize[p], rows);
+        if (v.decoded) {  // decoded ahead of the launch: read in place
+            TPlan x{};
+            x.off = (p * a.page_rows - v.dec_row0) * v.c.elem;
+            x.mode = TA_DIRECT;
+            x.aux = !((reinterpret_cast<uintptr_t>(v.c.comp) + x.off) % 16);
+            tp[lane] = x;
+        } else {
+            const uint64_t off = v.c.page_off[p];
+            tp[lane] = plan_page(v, v.c.comp + off, off, v.c.page_codec[p], v.c.page_csize[p], rows);
+        }
     }
     // the wave's own LDS writes are visible to its later LDS reads (in order within a wave)
     bool bad = false;

@@ -75,6 +75,21 @@ void scan_stats(const golap_scan_plan &p, uint32_t colmask, const std::vector<ui
 // offsets behind the windows. Returns the bytes of one wave's region.
 uint32_t scan_layout(const golap_column *const *cols, uint32_t n_cols, uint32_t colmask, uint64_t pb, uint64_t pe,
                      int32_t *woff, int32_t *dslot, uint32_t *wcap, int32_t *tok_off, uint32_t *tok_cap);
+// Columns of colmask decoded ahead of a scan over pages [pb, pe): those with LZ4I pages there that
+// a wave would otherwise decode into its LDS window page by page. The chunk decompressor (a
+// workgroup per page, pointer jumping for long copy chains) writes them to an HBM buffer and the
+// scan reads them in place.
+uint32_t scan_decode_mask(const golap_column *const *cols, uint32_t n_cols, uint32_t colmask, uint64_t pb, uint64_t pe);
+// Buffer bytes for the decoded columns of mask over un chunks (each column's region 256-byte aligned).
+uint64_t scan_decode_bytes(const golap_column *const *cols, uint32_t n_cols, uint32_t mask, uint64_t un);
+// Points the views of the decoded columns at their regions of buf (chunks from chunk c0 on).
+void scan_decode_views(const golap_column *const *cols, uint32_t n_cols, uint32_t mask, uint64_t c0, uint64_t un,
+                       uint8_t *buf, ScanColView *views);
+// Enqueues the decodes of the chunks [cb, ce) of the decoded columns into their regions (chunks
+// from c0 on); corrupt pages set err.
+hipError_t scan_decode_enqueue(golap_ctx *ctx, const golap_column *const *cols, uint32_t n_cols, uint32_t mask,
+                               uint64_t c0, uint64_t un, uint8_t *buf, uint64_t cb, uint64_t ce, hipStream_t st,
+                               uint32_t *err);
 uint32_t scan_page_waves(golap_ctx *ctx, uint32_t win_bytes, uint32_t agg_entries);
 uint32_t scan_agg_entries(golap_ctx *ctx, uint32_t agg_mode, uint32_t requested, uint32_t win_bytes, uint32_t page_waves,
                           uint32_t dflt);

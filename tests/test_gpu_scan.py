@@ -137,6 +137,49 @@ def test_scan_lz4i_pages_decode_in_parallel(orc, golap, ctx, page_rows):
             ctx.free_column(d)
 
 
+def test_scan_batch_reports_corrupt_page_of_a_decoded_column(orc, golap, ctx):
+    """A scan batch of several plans decodes its LZ4I columns ahead of the launch; a truncated
+    passenger page there is reported as ECORRUPT by the run, and the batch over intact columns
+    still equals the single-plan runs afterwards."""
+    host = golap.gen_trip(23, 300_000)
+    good, bad = [], []
+    try:
+        for i, c in enumerate(host):
+            hc = golap.HostColumn.encode(c, codec=1 if i == 1 else 255, page_rows=4096, chunk_rows=65536)
+            good.append(ctx.upload(hc))
+            if i == 1:
+                a = hc.arrays()
+                n = a["desc"].n_pages
+                pages = [bytes(a["comp"][int(a["page_off"][p]):int(a["page_off"][p]) + int(a["page_csize"][p])])
+                         for p in range(n)]
+                assert int(a["page_codec"][5]) == 5  # an LZ4I page
+                pages[5] = pages[5][:len(pages[5]) // 2]
+                hc = golap.HostColumn.assemble(golap.U8, c.size, 4096, 65536, pages, a["page_codec"].tolist())
+                bad.append(ctx.upload(hc))
+            else:
+                bad.append(good[-1])
+        say("scan batch corrupt decoded page")
+        for cols, corrupt in ((bad, True), (good, False)):
+            plans = []
+            for t in (1, 3):
+                preds, keys, vc = template(t, 0, 8783, full_hours=True)
+                plans.append(dict(cols=cols, preds=preds, keys=keys, value_col=vc))
+            with ctx.scan_batch(plans) as b:
+                if corrupt:
+                    with pytest.raises(golap.GolapError) as e:
+                        b.run()
+                    assert e.value.status == golap.ECORRUPT
+                else:
+                    b.run()
+                    for i, p in enumerate(plans):
+                        got, ref = b.result(i), ctx.scan_run(**p)
+                        assert all(np.array_equal(x, y) for x, y in zip(got[1:], ref[1:])), i
+    finally:
+        ctx.free_column(bad[1])
+        for d in good:
+            ctx.free_column(d)
+
+
 def test_scan_key_out_of_range_is_an_error(golap, ctx, trip):
     host, dev = trip
     preds, keys, vc = template(0, 1000, 1700)
