@@ -121,9 +121,10 @@ __global__ __launch_bounds__(WAVE) void k_decode_pages(DecodeArgs a) {
 // global->global; any other coded page (and an LZ4I page the cooperative decode does not take)
 // is decoded by the first wave. The decoded page is written out with 16-byte stores. JUMP: the
 // layout has a source map, and matches left pending by the ordered pass are resolved by pointer
-// jumping (pages whose matches copy from matches, in long chains).
-template <bool JUMP>
-__global__ __launch_bounds__(BLOCK) void k_decode_wg(DecodeArgs a, lz4wg::Layout l) {
+// jumping (pages whose matches copy from matches, in long chains). NT threads take up to 8 NT
+// sequences a page: 256, or 1,024 for columns whose pages have more than 2,048.
+template <bool JUMP, int NT>
+__global__ __launch_bounds__(NT) void k_decode_wg(DecodeArgs a, lz4wg::Layout l) {
     extern __shared__ __align__(16) uint8_t lds[];
     const uint32_t tid = threadIdx.x;
     uint8_t *win = lds + l.out;
@@ -151,9 +152,9 @@ __global__ __launch_bounds__(BLOCK) void k_decode_wg(DecodeArgs a, lz4wg::Layout
             const uint8_t *in_order = src;
             if ((codec & PAGE_CODEC_MASK) == golap_fmt::LZ4I && csize <= l.stage_cap && nbytes <= l.out_cap &&
                 (reinterpret_cast<uintptr_t>(src) & 15) == 0) {
-                lz4wg::stage<BLOCK>(lds, l, src, csize);
+                lz4wg::stage<NT>(lds, l, src, csize);
                 __syncthreads();
-                got = lz4wg::decode<BLOCK, JUMP>(lds, l, csize, nbytes);
+                got = lz4wg::decode<NT, JUMP>(lds, l, csize, nbytes);
                 in_order = lds + l.stage;
             }
             if (got == -2) {
@@ -178,10 +179,10 @@ __global__ __launch_bounds__(BLOCK) void k_decode_wg(DecodeArgs a, lz4wg::Layout
             const uint32_t nv = nbytes / 16;
             const uint4 *s4 = reinterpret_cast<const uint4 *>(from);
             uint4 *d4 = reinterpret_cast<uint4 *>(dst);
-            for (uint32_t k = tid; k < nv; k += BLOCK) d4[k] = s4[k];
+            for (uint32_t k = tid; k < nv; k += NT) d4[k] = s4[k];
             done = nv * 16;
         }
-        for (uint32_t k = done + tid; k < nbytes; k += BLOCK) dst[k] = from[k];
+        for (uint32_t k = done + tid; k < nbytes; k += NT) dst[k] = from[k];
         __syncthreads();
     }
 }
@@ -206,18 +207,26 @@ hipError_t launch_decode_pages(golap_ctx *ctx, const golap_column *col, uint64_t
     a.err = err;
     if (col->extra.has_lz4i && page_bytes < 65536) {
         const uint32_t tok_cap = col->extra.lz4i_max_nseq ? col->extra.lz4i_max_nseq : lz4wg::max_sequences(page_bytes);
-        // the source map for pointer jumping when it leaves room for two workgroups per CU
-        const lz4wg::Layout lj = lz4wg::layout(lz4i_stage_bytes(col->extra), page_bytes, tok_cap, BLOCK, true, true);
-        const lz4wg::Layout l = lz4wg::layout(lz4i_stage_bytes(col->extra), page_bytes, tok_cap, BLOCK, true);
-        const bool jump = 2 * (uint64_t)lj.total <= (uint64_t)ctx->lds_max;
+        const uint32_t nt = tok_cap > lz4wg::MAXJ * BLOCK ? 4 * BLOCK : BLOCK;
+        const lz4wg::Layout lj = lz4wg::layout(lz4i_stage_bytes(col->extra), page_bytes, tok_cap, nt, true, true);
+        const lz4wg::Layout l = lz4wg::layout(lz4i_stage_bytes(col->extra), page_bytes, tok_cap, nt, true);
+        // the source map when it keeps at least three quarters of the workgroups per CU
+        auto per_cu = [&](uint32_t lds) {
+            return std::min<uint64_t>((uint64_t)ctx->lds_max / std::max<uint32_t>(lds, 1), 2048 / nt);
+        };
+        const bool jump = lj.total <= (uint32_t)ctx->lds_max && 4 * per_cu(lj.total) >= 3 * per_cu(l.total);
         if (l.total <= (uint32_t)ctx->lds_max) {
             const lz4wg::Layout &u = jump ? lj : l;
-            auto kernel = jump ? k_decode_wg<true> : k_decode_wg<false>;
+            auto kernel = nt == BLOCK ? (jump ? k_decode_wg<true, BLOCK> : k_decode_wg<false, BLOCK>)
+                                      : (jump ? k_decode_wg<true, 4 * BLOCK> : k_decode_wg<false, 4 * BLOCK>);
             hipError_t e = prepare_lds(ctx, kernel, u.total);
             if (e != hipSuccess) return e;
-            const uint32_t grid = (uint32_t)std::min<uint64_t>(page_end - page_begin,
-                                                               (uint64_t)ctx->cus * wg_per_cu(ctx, u.total));
-            hipLaunchKernelGGL(kernel, dim3(grid), dim3(BLOCK), u.total, stream, a, u);
+            int occ = 0;
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, (int)nt, u.total);
+            if (e != hipSuccess) return e;
+            const uint32_t wgs = std::max<uint32_t>(1, std::min<uint32_t>((uint32_t)occ, wg_per_cu(ctx, u.total)));
+            const uint32_t grid = (uint32_t)std::min<uint64_t>(page_end - page_begin, (uint64_t)ctx->cus * wgs);
+            hipLaunchKernelGGL(kernel, dim3(grid), dim3(nt), u.total, stream, a, u);
             return hipGetLastError();
         }
     }

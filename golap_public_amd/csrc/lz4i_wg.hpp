@@ -500,7 +500,7 @@ __device__ __forceinline__ int64_t resolve_by_jumping(uint8_t *lds, const Layout
 // -2 if this path does not take it (parse's cases, or more long literals or pending matches than
 // a list holds). The caller then decodes the page in order. The result is the same in every thread.
 // JUMP (l.jump set): matches the ordered pass leaves pending are resolved by pointer jumping
-// instead of list passes.
+// instead of list passes when they are at least 1/16 of the output units.
 template <int NT, bool JUMP = false>
 __device__ __forceinline__ int64_t decode(uint8_t *lds, const Layout &l, uint32_t csize, uint32_t osize) {
     const uint32_t tid = threadIdx.x;
@@ -601,7 +601,10 @@ __device__ __forceinline__ int64_t decode(uint8_t *lds, const Layout &l, uint32_
     __syncthreads();
     const uint32_t total = misc[0];
     if (total == 0) return osize;
-    if constexpr (JUMP) return resolve_by_jumping<NT>(lds, l, osize, ub, pend, ps.mo, ml, any);
+    // pointer jumping scans every output unit a round: worth it when pending matches are dense
+    // (long copy chains), the list passes otherwise
+    if constexpr (JUMP)
+        if (16 * total > (osize >> ub)) return resolve_by_jumping<NT>(lds, l, osize, ub, pend, ps.mo, ml, any);
     if (total > list_cap) return -2;
     {
         uint32_t i = at;
