@@ -36,6 +36,7 @@ constexpr uint32_t MAXJ = 8;         // sequences per thread (match descriptors 
 constexpr uint32_t SERIAL_MAX = golap_fmt::LZ4I_SERIAL_MAX;  // pages with at most this many sequences: in order
 constexpr uint32_t SHORT = 16;       // literals and matches up to this long are copied in the main passes
 constexpr uint32_t LIST_MIN = 256;   // entries the lists hold at least
+constexpr uint32_t SPIN = 32;        // barrier-free tries of a thread's list entries in the first pass
 
 // Byte offsets of the decode's LDS regions from the caller's base, for pages of up to stage_cap
 // coded bytes and tok_cap sequences decoding to at most out_cap bytes, with nt threads. The
@@ -617,22 +618,30 @@ __device__ __forceinline__ int64_t decode(uint8_t *lds, const Layout &l, uint32_
             }
     }
     __syncthreads();
+    // In the first pass a thread retries its entries without a barrier, up to SPIN times, while
+    // other threads and waves complete the sources they wait for (the earliest pending match
+    // always has a final source); every later pass tries each entry once and ends in a barrier.
     for (uint32_t pass = 0;; ++pass) {
         bool left = false;
-        for (uint32_t e = tid; e < total; e += NT) {
-            const uint32_t n = list[2 * e + 1];
-            if (!n) continue;
-            const uint32_t dst = list[2 * e] & 0xffffu, off = list[2 * e] >> 16;
-            const bool ready = bits_all_any(bits, (dst - off) >> ub, min(n, off) >> ub);
-            order();
-            if (ready) {
-                match_thread(out, dst, off, n, words);
+        for (uint32_t it = 0; it < (pass ? 1u : SPIN); ++it) {
+            left = false;
+            for (uint32_t e = tid; e < total; e += NT) {
+                const uint32_t n = list[2 * e + 1];
+                if (!n) continue;
+                const uint32_t dst = list[2 * e] & 0xffffu, off = list[2 * e] >> 16;
+                const bool ready = bits_all_any(bits, (dst - off) >> ub, min(n, off) >> ub);
                 order();
-                bits_set_any(bits, dst >> ub, n >> ub);
-                list[2 * e + 1] = 0;
-            } else {
-                left = true;
+                if (ready) {
+                    match_thread(out, dst, off, n, words);
+                    order();
+                    bits_set_any(bits, dst >> ub, n >> ub);
+                    list[2 * e + 1] = 0;
+                } else {
+                    left = true;
+                }
             }
+            if (!left) break;
+            order();  // the next try reads the bitmap again
         }
         if (!any(left)) return osize;
         if (pass > total) return -1;  // a pass that copies nothing: a broken invariant
