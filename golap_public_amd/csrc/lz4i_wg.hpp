@@ -37,6 +37,7 @@ constexpr uint32_t SERIAL_MAX = golap_fmt::LZ4I_SERIAL_MAX;  // pages with at mo
 constexpr uint32_t SHORT = 16;       // literals and matches up to this long are copied in the main passes
 constexpr uint32_t LIST_MIN = 256;   // entries the lists hold at least
 constexpr uint32_t SPIN = 32;        // barrier-free tries of a thread's list entries in the first pass
+constexpr uint32_t STEP_TRIES = 4;   // tries of a short match in its step of the ordered pass
 
 // Byte offsets of the decode's LDS regions from the caller's base, for pages of up to stage_cap
 // coded bytes and tok_cap sequences decoding to at most out_cap bytes, with nt threads. The
@@ -578,7 +579,9 @@ __device__ __forceinline__ int64_t decode(uint8_t *lds, const Layout &l, uint32_
     }
 
     // matches: one pass over the registers in sequence order (a barrier per step), short matches
-    // whose source lies at least their length (or 16 bytes) back
+    // whose source lies at least their length (or 16 bytes) back; a match whose source another
+    // thread of the same step is still copying is tried again a few times before it goes to the
+    // list
 #pragma unroll
     for (uint32_t j = 0; j < MAXJ; ++j) {
         if (j >= J) continue;
@@ -587,13 +590,17 @@ __device__ __forceinline__ int64_t decode(uint8_t *lds, const Layout &l, uint32_
         const uint32_t dst = mo[j] & 0xffffu, off = mo[j] >> 16, n = ml[j];
         const bool fast = ((pend >> j) & 1u) && n <= SHORT && (off >= 16 || off >= n);
         const uint32_t src = fast ? dst - off : 0u;
-        const bool ready = fast && bits_all(bits, src >> ub, n >> ub);
-        order();
-        const u32x4 v = ld_any16(out, src);
-        if (ready) put16(out, dst, v, n, words);
-        order();
-        if (ready) bits_set(bits, dst >> ub, n >> ub);
-        pend &= ready ? ~(1u << j) : ~0u;
+        bool todo = fast;
+        for (uint32_t it = 0; it < STEP_TRIES && __any(todo); ++it) {
+            const bool ready = todo && bits_all(bits, src >> ub, n >> ub);
+            order();
+            const u32x4 v = ld_any16(out, src);
+            if (ready) put16(out, dst, v, n, words);
+            order();
+            if (ready) bits_set(bits, dst >> ub, n >> ub);
+            pend &= ready ? ~(1u << j) : ~0u;
+            todo = todo && !ready;
+        }
     }
     // the rest, gathered into the list, worked through by the whole workgroup
     __syncthreads();
