@@ -821,8 +821,8 @@ def run_encode(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the sharded form is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: this library runs on one GPU; --gpus {args.gpus} is not supported "
+              "(each GPU would scan its own share of the chunks)", file=sys.stderr)
         sys.exit(2)
     if args.workload == "star":
         return run_star(args)
