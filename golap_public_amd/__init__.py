@@ -123,7 +123,8 @@ class ChunkMetaDesc(C.Structure):
 
 class BatchConfig(C.Structure):
     _fields_ = [("pages_per_item", u32), ("lds_group_slots", u32), ("lds_cutover", u32), ("filter_lds", u32),
-                ("compact_tables", u32), ("max_blocks_per_cu", u32), ("shared_filter", u32)]
+                ("compact_tables", u32), ("max_blocks_per_cu", u32), ("shared_filter", u32),
+                ("rows_per_lane", u32)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -134,6 +134,8 @@ int check_config(golap_batch *b, const golap_batch_config &c) {
     if (c.filter_lds > 1 || c.compact_tables > 1 || c.shared_filter > 1) return bfail(b, GOLAP_EINVAL, "batch: bad flag");
     if (c.shared_filter && (b->kind != BatchKind::STAR || !b->d_mfilt))
         return bfail(b, GOLAP_EINVAL, "batch: no shared filter for this batch");
+    if (c.rows_per_lane && (b->kind != BatchKind::SCAN || (c.rows_per_lane != 4 && c.rows_per_lane != 8)))
+        return bfail(b, GOLAP_EINVAL, "batch: rows_per_lane must be 0, 4 or 8 (scan batches)");
     const size_t budget = b->kind == BatchKind::SCAN ? scan_lds_budget(b->ctx) : (size_t)b->ctx->lds_max;
     if (lds_bytes(b, c) > budget) return bfail(b, GOLAP_EINVAL, "batch: LDS budget exceeded");
     return GOLAP_OK;
@@ -233,8 +235,9 @@ hipError_t launch_kernel(golap_batch *b, hipStream_t s, bool prepare_only) {
     L.queue = reinterpret_cast<uint32_t *>(sl->d_arena);
     L.pages_per_item = c.pages_per_item;
     L.max_blocks_per_cu = c.max_blocks_per_cu;
-    L.rows_per_lane = scan_rows_per_lane(b->ctx, b->union_pairs * b->ppc,
-                                         scan_lds_bytes(L.win_bytes, L.agg_entries, L.page_waves));
+    L.rows_per_lane = c.rows_per_lane ? c.rows_per_lane
+                                      : scan_rows_per_lane(b->ctx, b->union_pairs * b->ppc,
+                                                           scan_lds_bytes(L.win_bytes, L.agg_entries, L.page_waves));
     L.err = reinterpret_cast<uint32_t *>(sl->d_arena + 8);
     return launch_scan(b->ctx, L, s, prepare_only);
 }
@@ -964,7 +967,7 @@ int golap_batch_tune(golap_batch *b, uint32_t reps, golap_batch_config *best, do
     if (int r = time_cfg(cur, &cur_ms)) return fail(ctx, r, b->err);
     const bool star = b->kind == BatchKind::STAR;
     // coordinate descent over the knobs, in order of expected effect
-    enum Axis { SHARED, LDS, GRAIN, FILTER, TABLES, OCC };
+    enum Axis { SHARED, LDS, GRAIN, FILTER, TABLES, OCC, ROWS };
     std::vector<std::pair<Axis, std::vector<golap_batch_config>>> axes;
     {
         std::vector<golap_batch_config> ax;
@@ -996,6 +999,15 @@ int golap_batch_tune(golap_batch *b, uint32_t reps, golap_batch_config *best, do
         on.compact_tables = 1;
         axes.emplace_back(TABLES, std::vector<golap_batch_config>{on, off});
     }
+    if (!star) {  // rows per wave of the scan's sub-tiles
+        std::vector<golap_batch_config> ax;
+        for (uint32_t rpl : {4u, 8u}) {
+            golap_batch_config c{};
+            c.rows_per_lane = rpl;
+            ax.push_back(c);
+        }
+        axes.emplace_back(ROWS, ax);
+    }
     {
         std::vector<golap_batch_config> ax;
         for (uint32_t mb : {0u, 1u, 2u, 3u}) {
@@ -1025,6 +1037,7 @@ int golap_batch_tune(golap_batch *b, uint32_t reps, golap_batch_config *best, do
             case FILTER: c.filter_lds = v.filter_lds; break;
             case TABLES: c.compact_tables = v.compact_tables; break;
             case OCC: c.max_blocks_per_cu = v.max_blocks_per_cu; break;
+            case ROWS: c.rows_per_lane = v.rows_per_lane; break;
             }
             if (!memcmp(&c, &cur, sizeof(c))) continue;
             double ms = 0;

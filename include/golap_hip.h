@@ -606,6 +606,8 @@ typedef struct {
                                     (available when every query has that dimension and its
                                     keys span at most 2^27 values); candidates then probe
                                     every dimension per query, aggregating in global memory */
+    uint32_t rows_per_lane;      /* scan: rows of a lane per sub-tile of a page (rows per wave = 64 x
+                                    this): 4 or 8, 0 = chosen from the launch size */
 } golap_batch_config;
 typedef struct {
     double ms;                   /* device time of the run (graph or plain launches) */

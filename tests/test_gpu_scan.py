@@ -226,7 +226,8 @@ def test_scan_batch_matches_single_and_oracle(orc, golap, ctx, trip):
             assert (got[0].n_groups, got[0].rows_matched, got[0].total_sum) == \
                    (w[0].n_groups, w[0].rows_matched, w[0].total_sum)
         for cfg in [dict(lds_group_slots=0), dict(lds_group_slots=256, pages_per_item=1),
-                    dict(lds_group_slots=4096, max_blocks_per_cu=1, pages_per_item=4)]:
+                    dict(lds_group_slots=4096, max_blocks_per_cu=1, pages_per_item=4),
+                    dict(lds_group_slots=4096, rows_per_lane=8), dict(rows_per_lane=4)]:
             say("scan batch config", cfg)
             b.set_config(**cfg)
             for rep in range(2):
@@ -234,8 +235,11 @@ def test_scan_batch_matches_single_and_oracle(orc, golap, ctx, trip):
                 for i in range(len(plans)):
                     got = b.result(i)
                     assert all(np.array_equal(x, y) for x, y in zip(got[1:], want[i][1:])), (cfg, rep, i)
+        with pytest.raises(golap.GolapError):
+            b.set_config(rows_per_lane=5)  # rows per lane are 4 or 8
         c, ms = b.tune(reps=2)
         say("scan batch tuned", c.as_dict(), ms)
+        assert c.rows_per_lane in (0, 4, 8)  # the tuner's rows-per-wave axis (0: the launch-size rule kept)
         b.run()
         for i in range(len(plans)):
             got = b.result(i)
